@@ -1,0 +1,120 @@
+// Internal definitions shared by the gfx950 kernel sources of libfe_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/fe_hip.h"
+
+namespace fe {
+
+// thread-local error message behind fe_last_error()
+void set_error(const std::string& msg);
+
+// Growable device scratch owned by a context (split-K partials, reductions,
+// eigensolver work). Grown with hipMalloc only when a call needs more; never
+// shrunk until the context dies.
+struct Scratch {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace fe
+
+struct fe_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cus = 0;
+  size_t hbm = 0;
+  char name[128] = {0};
+  fe::Scratch ws[4];           // independent scratch slots
+  int* info_dev = nullptr;     // small device status words
+  hipEvent_t events[16] = {};
+};
+
+struct fe_csr_s {
+  fe_ctx ctx = nullptr;
+  int n = 0;
+  int64_t nnz = 0;
+  int* rowptr = nullptr;
+  int* colind = nullptr;
+  double* vals64 = nullptr;
+  float* vals32 = nullptr;
+  // SELL-C-sigma copy (optional)
+  int sell_C = 0, sell_sigma = 0;
+  int64_t sell_slices = 0;
+  int64_t sell_padded = 0;     // stored entries including padding
+  int64_t* sell_ptr = nullptr; // slice start offsets (slices + 1)
+  int* sell_len = nullptr;     // slice widths (max row length in the slice)
+  int* sell_perm = nullptr;    // sorted position -> original row
+  int* sell_col = nullptr;     // column-major within a slice: [slice][w][C]
+  double* sell_v64 = nullptr;
+  float* sell_v32 = nullptr;
+  int use_sell = 0;
+};
+
+namespace fe {
+
+inline int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+#define FE_HIP_TRY(expr)                                                        \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess)                                                       \
+      return ::fe::fail(-3, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define FE_CHECK_LAUNCH()                                                        \
+  do {                                                                           \
+    hipError_t _e = hipGetLastError();                                           \
+    if (_e != hipSuccess)                                                        \
+      return ::fe::fail(-3, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+  } while (0)
+
+#define FE_ARG(cond, msg)                              \
+  do {                                                 \
+    if (!(cond)) return ::fe::fail(-2, msg);           \
+  } while (0)
+
+// Ensure scratch slot `slot` of ctx holds at least `bytes`.
+int scratch(fe_ctx ctx, int slot, size_t bytes, void** out);
+
+// Logical tile index for an XCD-aware launch: the hardware hands workgroups
+// to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), so tile
+// (b % 8) * per_xcd + b / 8 gives each XCD one contiguous range of tiles and
+// keeps that range's neighbours in the XCD's own L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t per_xcd) {
+  return (b & 7) * per_xcd + (b >> 3);
+}
+
+template <typename T>
+struct prec_of;
+template <>
+struct prec_of<double> {
+  static constexpr int value = FE_F64;
+};
+template <>
+struct prec_of<float> {
+  static constexpr int value = FE_F32;
+};
+
+inline size_t prec_bytes(int prec) { return prec == FE_F64 ? 8 : 4; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+inline unsigned blocks_for(int64_t work, int per_block) {
+  return static_cast<unsigned>((work + per_block - 1) / per_block);
+}
+
+}  // namespace fe

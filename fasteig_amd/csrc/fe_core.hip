@@ -1,0 +1,563 @@
+// Context, memory, CSR upload and the column-wise block kernels of libfe_hip.so.
+#include "fe_common.hpp"
+
+#include <cstring>
+
+namespace fe {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+
+int scratch(fe_ctx ctx, int slot, size_t bytes, void** out) {
+  Scratch& s = ctx->ws[slot];
+  if (s.bytes < bytes) {
+    if (s.ptr) {
+      FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+      FE_HIP_TRY(hipFree(s.ptr));
+      s.ptr = nullptr;
+      s.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4 + 4096;
+    FE_HIP_TRY(hipMalloc(&s.ptr, want));
+    s.bytes = want;
+  }
+  *out = s.ptr;
+  return 0;
+}
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void k_zero2d(char* p, int64_t rows, int64_t rowbytes, int64_t ldbytes) {
+  const int64_t total = rows * rowbytes;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / rowbytes, c = t - r * rowbytes;
+    p[r * ldbytes + c] = 0;
+  }
+}
+
+template <typename T>
+__global__ void k_zero_t(T* p, int64_t rows, int64_t cols, int64_t ld) {
+  const int64_t total = rows * cols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / cols, c = t - r * cols;
+    p[r * ld + c] = T(0);
+  }
+}
+
+__global__ void k_to_f32(const double* a, float* b, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    b[t] = static_cast<float>(a[t]);
+}
+
+__global__ void k_scale_sym(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                            double* v64, float* v32, const double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double di = d[i];
+  for (int p = rp[i]; p < rp[i + 1]; ++p) {
+    const double nv = di * v64[p] * d[ci[p]];
+    v64[p] = nv;
+    v32[p] = static_cast<float>(nv);
+  }
+}
+
+// Y = a X + b Y + s_j X  (row-major, column-scaled)
+template <typename T>
+__global__ void k_axpby_cols(int64_t n, int k, const T* __restrict__ X, int64_t ldx, T* Y,
+                             int64_t ldy, T a, T b, const double* __restrict__ s) {
+  const int64_t total = n * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / k;
+    const int j = static_cast<int>(t - i * k);
+    const T x = X[i * ldx + j];
+    T coef = a;
+    if (s) coef += static_cast<T>(s[j]);
+    T& y = Y[i * ldy + j];
+    y = (b == T(0) ? T(0) : b * y) + coef * x;
+  }
+}
+
+// Column sums of squares over row chunks: part[chunk][j]. One thread per column
+// within a chunk of `rows_per` rows; threads of a row are adjacent columns, so
+// the row-major block is read coalesced. The chunk partials are reduced in a
+// fixed order by k_reduce_cols.
+template <typename T, bool RES>
+__global__ void k_colsq(int64_t n, int k, const T* __restrict__ Y, int64_t ldy,
+                        const T* __restrict__ AY, int64_t lday, const double* __restrict__ theta,
+                        T* R, int64_t ldr, double* part_r, double* part_y, int64_t rows_per) {
+  const int j = blockIdx.y * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  const int64_t r0 = blockIdx.x * rows_per;
+  const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+  double sy = 0.0, sr = 0.0;
+  const double th = RES ? theta[j] : 0.0;
+  for (int64_t i = r0; i < r1; ++i) {
+    const double y = static_cast<double>(Y[i * ldy + j]);
+    sy += y * y;
+    if (RES) {
+      const T rv = static_cast<T>(static_cast<double>(AY[i * lday + j]) - th * y);
+      if (R) R[i * ldr + j] = rv;
+      const double rd = static_cast<double>(rv);
+      sr += rd * rd;
+    }
+  }
+  part_y[blockIdx.x * (int64_t)k + j] = sy;
+  if (RES) part_r[blockIdx.x * (int64_t)k + j] = sr;
+}
+
+__global__ void k_reduce_cols(int chunks, int k, const double* __restrict__ part, double* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  double s = 0.0;
+  for (int c = 0; c < chunks; ++c) s += part[(int64_t)c * k + j];
+  out[j] = sqrt(s);
+}
+
+template <typename S, typename D>
+__global__ void k_copy2d(int64_t rows, int cols, const S* __restrict__ src, int64_t lds, D* dst,
+                         int64_t ldd) {
+  const int64_t total = rows * cols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / cols;
+    const int c = static_cast<int>(t - r * cols);
+    dst[r * ldd + c] = static_cast<D>(src[r * lds + c]);
+  }
+}
+
+// column-major FP64 (n x k, ld n) <-> row-major block
+template <typename D>
+__global__ void k_colmajor_to_rows(int64_t n, int k, const double* __restrict__ src, D* dst,
+                                   int64_t ldd) {
+  const int64_t total = n * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / k;
+    const int c = static_cast<int>(t - r * k);
+    dst[r * ldd + c] = static_cast<D>(src[(int64_t)c * n + r]);
+  }
+}
+
+template <typename S>
+__global__ void k_rows_to_colmajor(int64_t n, int k, const S* __restrict__ src, int64_t lds,
+                                   double* dst) {
+  const int64_t total = n * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int c = static_cast<int>(t / n);
+    const int64_t r = t - (int64_t)c * n;
+    dst[t] = static_cast<double>(src[r * lds + c]);
+  }
+}
+
+__global__ void k_sym_small(int m, double* A, int64_t lda) {
+  const int64_t total = (int64_t)m * m;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = static_cast<int>(t / m), i = static_cast<int>(t - (int64_t)j * m);
+    if (i > j) {
+      const double v = 0.5 * (A[(int64_t)j * lda + i] + A[(int64_t)i * lda + j]);
+      A[(int64_t)j * lda + i] = v;
+      A[(int64_t)i * lda + j] = v;
+    }
+  }
+}
+
+__global__ void k_gather_cols(int m, int p, const double* __restrict__ V, int64_t ldv,
+                              const int* __restrict__ idx, double* C, int64_t ldc) {
+  const int64_t total = (int64_t)m * p;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = static_cast<int>(t / m), i = static_cast<int>(t - (int64_t)j * m);
+    C[(int64_t)j * ldc + i] = V[(int64_t)idx[j] * ldv + i];
+  }
+}
+
+__global__ void k_zero_rows_small(int r, int p, double* C, int64_t ldc) {
+  const int64_t total = (int64_t)r * p;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = static_cast<int>(t / r), i = static_cast<int>(t - (int64_t)j * r);
+    C[(int64_t)j * ldc + i] = 0.0;
+  }
+}
+
+inline unsigned grid_for(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return static_cast<unsigned>(b);
+}
+
+}  // namespace fe
+
+using namespace fe;
+
+extern "C" {
+
+const char* fe_last_error(void) { return g_error.c_str(); }
+const char* fe_version(void) { return "fasteig-amd 0.1 (gfx950)"; }
+
+int fe_ctx_create(int device, fe_ctx* out) {
+  FE_ARG(out, "fe_ctx_create: null output");
+  int count = 0;
+  FE_HIP_TRY(hipGetDeviceCount(&count));
+  FE_ARG(device >= 0 && device < count, "fe_ctx_create: device index out of range");
+  FE_HIP_TRY(hipSetDevice(device));
+  auto* c = new fe_ctx_s();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return fail(-3, "fe_ctx_create: hipGetDeviceProperties failed");
+  }
+  c->cus = prop.multiProcessorCount;
+  c->hbm = prop.totalGlobalMem;
+  std::snprintf(c->name, sizeof(c->name), "%s", prop.gcnArchName);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(-3, "fe_ctx_create: hipStreamCreate failed");
+  }
+  if (hipMalloc(&c->info_dev, 64 * sizeof(int)) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return fail(-3, "fe_ctx_create: hipMalloc(info) failed");
+  }
+  for (auto& e : c->events) hipEventCreate(&e);
+  *out = c;
+  return 0;
+}
+
+int fe_ctx_destroy(fe_ctx ctx) {
+  if (!ctx) return 0;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& s : ctx->ws)
+    if (s.ptr) hipFree(s.ptr);
+  if (ctx->info_dev) hipFree(ctx->info_dev);
+  for (auto& e : ctx->events)
+    if (e) hipEventDestroy(e);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+void* fe_ctx_stream(fe_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int fe_ctx_sync(fe_ctx ctx) {
+  FE_ARG(ctx, "fe_ctx_sync: null context");
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fe_ctx_info(fe_ctx ctx, int* cus, size_t* hbm, char* name, int name_len) {
+  FE_ARG(ctx, "fe_ctx_info: null context");
+  if (cus) *cus = ctx->cus;
+  if (hbm) *hbm = ctx->hbm;
+  if (name && name_len > 0) std::snprintf(name, name_len, "%s", ctx->name);
+  return 0;
+}
+
+int fe_alloc(fe_ctx ctx, size_t bytes, void** ptr) {
+  FE_ARG(ctx && ptr, "fe_alloc: null argument");
+  *ptr = nullptr;
+  if (bytes == 0) return 0;
+  FE_HIP_TRY(hipMalloc(ptr, bytes));
+  return 0;
+}
+
+int fe_free(fe_ctx ctx, void* ptr) {
+  FE_ARG(ctx, "fe_free: null context");
+  if (!ptr) return 0;
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  FE_HIP_TRY(hipFree(ptr));
+  return 0;
+}
+
+int fe_memcpy_h2d(fe_ctx ctx, void* dst, const void* src, size_t bytes) {
+  FE_ARG(ctx, "fe_memcpy_h2d: null context");
+  if (!bytes) return 0;
+  FE_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fe_memcpy_d2h(fe_ctx ctx, void* dst, const void* src, size_t bytes) {
+  FE_ARG(ctx, "fe_memcpy_d2h: null context");
+  if (!bytes) return 0;
+  FE_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fe_memcpy_d2d(fe_ctx ctx, void* dst, const void* src, size_t bytes) {
+  FE_ARG(ctx, "fe_memcpy_d2d: null context");
+  if (!bytes) return 0;
+  FE_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return 0;
+}
+
+int fe_zero2d(fe_ctx ctx, void* ptr, int64_t rows, int64_t cols, int64_t ld, int elem_bytes) {
+  FE_ARG(ctx && (ptr || rows * cols == 0), "fe_zero2d: bad argument");
+  if (rows * cols == 0) return 0;
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL(k_zero_t<double>, dim3(grid_for(rows * cols)), dim3(256), 0, ctx->stream,
+                       (double*)ptr, rows, cols, ld);
+  else if (elem_bytes == 4)
+    hipLaunchKernelGGL(k_zero_t<float>, dim3(grid_for(rows * cols)), dim3(256), 0, ctx->stream,
+                       (float*)ptr, rows, cols, ld);
+  else
+    hipLaunchKernelGGL(k_zero2d, dim3(grid_for(rows * cols * elem_bytes)), dim3(256), 0, ctx->stream,
+                       (char*)ptr, rows, cols * elem_bytes, ld * elem_bytes);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_csr_create(fe_ctx ctx, int n, int64_t nnz, const int* rowptr, const int* colind,
+                  const double* vals, fe_csr* out) {
+  FE_ARG(ctx && out && rowptr && (nnz == 0 || (colind && vals)), "fe_csr_create: null argument");
+  FE_ARG(n > 0 && nnz >= 0, "fe_csr_create: invalid dimensions");
+  FE_ARG(rowptr[0] == 0 && rowptr[n] == nnz, "fe_csr_create: rowptr does not span nnz");
+  for (int i = 0; i < n; ++i)
+    FE_ARG(rowptr[i] <= rowptr[i + 1], "fe_csr_create: rowptr is not monotone");
+  for (int64_t p = 0; p < nnz; ++p)
+    FE_ARG(colind[p] >= 0 && colind[p] < n, "fe_csr_create: column index out of range");
+  auto* A = new fe_csr_s();
+  A->ctx = ctx;
+  A->n = n;
+  A->nnz = nnz;
+  auto cleanup = [&](int code, const char* what) {
+    fe_csr_destroy(A);
+    return fail(code, what);
+  };
+  if (hipMalloc(&A->rowptr, sizeof(int) * (n + 1)) != hipSuccess ||
+      hipMalloc(&A->colind, sizeof(int) * (nnz ? nnz : 1)) != hipSuccess ||
+      hipMalloc(&A->vals64, sizeof(double) * (nnz ? nnz : 1)) != hipSuccess ||
+      hipMalloc(&A->vals32, sizeof(float) * (nnz ? nnz : 1)) != hipSuccess)
+    return cleanup(-3, "fe_csr_create: device allocation failed");
+  if (hipMemcpyAsync(A->rowptr, rowptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice, ctx->stream) !=
+          hipSuccess ||
+      (nnz && hipMemcpyAsync(A->colind, colind, sizeof(int) * nnz, hipMemcpyHostToDevice,
+                             ctx->stream) != hipSuccess) ||
+      (nnz && hipMemcpyAsync(A->vals64, vals, sizeof(double) * nnz, hipMemcpyHostToDevice,
+                             ctx->stream) != hipSuccess))
+    return cleanup(-3, "fe_csr_create: upload failed");
+  if (nnz) hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nnz)), dim3(256), 0, ctx->stream, A->vals64, A->vals32, nnz);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return cleanup(-3, "fe_csr_create: sync failed");
+  *out = A;
+  return 0;
+}
+
+int fe_csr_destroy(fe_csr A) {
+  if (!A) return 0;
+  if (A->ctx) hipStreamSynchronize(A->ctx->stream);
+  void* ptrs[] = {A->rowptr, A->colind, A->vals64, A->vals32, A->sell_ptr, A->sell_len,
+                  A->sell_perm, A->sell_col, A->sell_v64, A->sell_v32};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  delete A;
+  return 0;
+}
+
+int fe_csr_info(fe_csr A, int* n, int64_t* nnz) {
+  FE_ARG(A, "fe_csr_info: null matrix");
+  if (n) *n = A->n;
+  if (nnz) *nnz = A->nnz;
+  return 0;
+}
+
+int fe_csr_scale_sym(fe_ctx ctx, fe_csr A, const double* d) {
+  FE_ARG(ctx && A && d, "fe_csr_scale_sym: null argument");
+  FE_ARG(!A->sell_col, "fe_csr_scale_sym: scale before building SELL");
+  hipLaunchKernelGGL(k_scale_sym, dim3(blocks_for(A->n, 256)), dim3(256), 0, ctx->stream, A->n,
+                     A->rowptr, A->colind, A->vals64, A->vals32, d);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_csr_use(fe_csr A, int format) {
+  FE_ARG(A, "fe_csr_use: null matrix");
+  FE_ARG(format == FE_FMT_CSR || (format == FE_FMT_SELL && A->sell_col),
+         "fe_csr_use: SELL copy not built");
+  A->use_sell = format == FE_FMT_SELL;
+  return 0;
+}
+
+int fe_axpby_cols(fe_ctx ctx, int prec, int64_t n, int k, const void* X, int64_t ldx, void* Y,
+                  int64_t ldy, double a, double b, const double* s) {
+  FE_ARG(ctx && X && Y, "fe_axpby_cols: null argument");
+  if (n * k == 0) return 0;
+  if (prec == FE_F64)
+    hipLaunchKernelGGL(k_axpby_cols<double>, dim3(grid_for(n * k)), dim3(256), 0, ctx->stream, n, k,
+                       (const double*)X, ldx, (double*)Y, ldy, a, b, s);
+  else
+    hipLaunchKernelGGL(k_axpby_cols<float>, dim3(grid_for(n * k)), dim3(256), 0, ctx->stream, n, k,
+                       (const float*)X, ldx, (float*)Y, ldy, (float)a, (float)b, s);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+static int colsq_launch(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, int64_t ldy,
+                        const void* AY, int64_t lday, const double* theta, void* R, int64_t ldr,
+                        double* rnorm, double* ynorm) {
+  if (k == 0) return 0;
+  // chunk rows so that ~2048 x ceil(k/64) workgroups exist
+  int64_t chunks = 2048 / ((k + 63) / 64);
+  if (chunks < 64) chunks = 64;
+  if (chunks > n) chunks = n > 0 ? n : 1;
+  const int64_t rows_per = (n + chunks - 1) / chunks;
+  chunks = (n + rows_per - 1) / rows_per;
+  if (chunks < 1) chunks = 1;
+  void* ws = nullptr;
+  int rc = scratch(ctx, 3, sizeof(double) * 2 * chunks * k, &ws);
+  if (rc) return rc;
+  double* part_y = (double*)ws;
+  double* part_r = part_y + chunks * k;
+  dim3 grid((unsigned)chunks, (unsigned)((k + 63) / 64));
+  const bool res = AY != nullptr;
+  if (prec == FE_F64) {
+    if (res)
+      hipLaunchKernelGGL((k_colsq<double, true>), grid, dim3(64), 0, ctx->stream, n, k, (const double*)Y,
+                         ldy, (const double*)AY, lday, theta, (double*)R, ldr, part_r, part_y, rows_per);
+    else
+      hipLaunchKernelGGL((k_colsq<double, false>), grid, dim3(64), 0, ctx->stream, n, k, (const double*)Y,
+                         ldy, nullptr, 0, nullptr, nullptr, 0, nullptr, part_y, rows_per);
+  } else {
+    if (res)
+      hipLaunchKernelGGL((k_colsq<float, true>), grid, dim3(64), 0, ctx->stream, n, k, (const float*)Y, ldy,
+                         (const float*)AY, lday, theta, (float*)R, ldr, part_r, part_y, rows_per);
+    else
+      hipLaunchKernelGGL((k_colsq<float, false>), grid, dim3(64), 0, ctx->stream, n, k, (const float*)Y, ldy,
+                         nullptr, 0, nullptr, nullptr, 0, nullptr, part_y, rows_per);
+  }
+  FE_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 256)), dim3(256), 0, ctx->stream, (int)chunks, k,
+                     part_y, ynorm);
+  if (res)
+    hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 256)), dim3(256), 0, ctx->stream, (int)chunks, k,
+                       part_r, rnorm);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_colnorms(fe_ctx ctx, int prec, int64_t n, int k, const void* X, int64_t ldx, double* out) {
+  FE_ARG(ctx && X && out, "fe_colnorms: null argument");
+  return colsq_launch(ctx, prec, n, k, X, ldx, nullptr, 0, nullptr, nullptr, 0, nullptr, out);
+}
+
+int fe_ritz_residuals(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, int64_t ldy,
+                      const void* AY, int64_t lday, const double* theta, void* R, int64_t ldr,
+                      double* rnorm, double* ynorm) {
+  FE_ARG(ctx && Y && AY && theta && rnorm && ynorm, "fe_ritz_residuals: null argument");
+  return colsq_launch(ctx, prec, n, k, Y, ldy, AY, lday, theta, R, ldr, rnorm, ynorm);
+}
+
+int fe_copy2d(fe_ctx ctx, int64_t rows, int cols, const void* src, int sp, int64_t lds, void* dst,
+              int dp, int64_t ldd) {
+  FE_ARG(ctx && src && dst, "fe_copy2d: null argument");
+  if (rows * cols == 0) return 0;
+  const dim3 g(grid_for(rows * cols)), b(256);
+  if (sp == FE_F64 && dp == FE_F64)
+    hipLaunchKernelGGL((k_copy2d<double, double>), g, b, 0, ctx->stream, rows, cols, (const double*)src, lds,
+                       (double*)dst, ldd);
+  else if (sp == FE_F64 && dp == FE_F32)
+    hipLaunchKernelGGL((k_copy2d<double, float>), g, b, 0, ctx->stream, rows, cols, (const double*)src, lds,
+                       (float*)dst, ldd);
+  else if (sp == FE_F32 && dp == FE_F64)
+    hipLaunchKernelGGL((k_copy2d<float, double>), g, b, 0, ctx->stream, rows, cols, (const float*)src, lds,
+                       (double*)dst, ldd);
+  else
+    hipLaunchKernelGGL((k_copy2d<float, float>), g, b, 0, ctx->stream, rows, cols, (const float*)src, lds,
+                       (float*)dst, ldd);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_upload_colmajor(fe_ctx ctx, int64_t n, int k, const double* host, void* dst, int prec,
+                       int64_t ldd) {
+  FE_ARG(ctx && host && dst, "fe_upload_colmajor: null argument");
+  if (n * k == 0) return 0;
+  void* ws = nullptr;
+  int rc = scratch(ctx, 2, sizeof(double) * n * k, &ws);
+  if (rc) return rc;
+  FE_HIP_TRY(hipMemcpyAsync(ws, host, sizeof(double) * n * k, hipMemcpyHostToDevice, ctx->stream));
+  if (prec == FE_F64)
+    hipLaunchKernelGGL(k_colmajor_to_rows<double>, dim3(grid_for(n * k)), dim3(256), 0, ctx->stream, n, k,
+                       (const double*)ws, (double*)dst, ldd);
+  else
+    hipLaunchKernelGGL(k_colmajor_to_rows<float>, dim3(grid_for(n * k)), dim3(256), 0, ctx->stream, n, k,
+                       (const double*)ws, (float*)dst, ldd);
+  FE_CHECK_LAUNCH();
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fe_download_colmajor(fe_ctx ctx, int64_t n, int k, const void* src, int prec, int64_t lds,
+                         double* host) {
+  FE_ARG(ctx && host && src, "fe_download_colmajor: null argument");
+  if (n * k == 0) return 0;
+  void* ws = nullptr;
+  int rc = scratch(ctx, 2, sizeof(double) * n * k, &ws);
+  if (rc) return rc;
+  if (prec == FE_F64)
+    hipLaunchKernelGGL(k_rows_to_colmajor<double>, dim3(grid_for(n * k)), dim3(256), 0, ctx->stream, n, k,
+                       (const double*)src, lds, (double*)ws);
+  else
+    hipLaunchKernelGGL(k_rows_to_colmajor<float>, dim3(grid_for(n * k)), dim3(256), 0, ctx->stream, n, k,
+                       (const float*)src, lds, (double*)ws);
+  FE_CHECK_LAUNCH();
+  FE_HIP_TRY(hipMemcpyAsync(host, ws, sizeof(double) * n * k, hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fe_sym_small(fe_ctx ctx, int m, double* A, int64_t lda) {
+  FE_ARG(ctx && A && lda >= m, "fe_sym_small: bad argument");
+  hipLaunchKernelGGL(k_sym_small, dim3(grid_for((int64_t)m * m)), dim3(256), 0, ctx->stream, m, A, lda);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_gather_cols(fe_ctx ctx, int m, int p, const double* V, int64_t ldv, const int* idx_host,
+                   double* C, int64_t ldc) {
+  FE_ARG(ctx && V && C && idx_host, "fe_gather_cols: null argument");
+  if ((int64_t)m * p == 0) return 0;
+  for (int j = 0; j < p; ++j) FE_ARG(idx_host[j] >= 0 && idx_host[j] < m, "fe_gather_cols: index out of range");
+  void* ws = nullptr;
+  int rc = scratch(ctx, 2, sizeof(int) * p, &ws);
+  if (rc) return rc;
+  FE_HIP_TRY(hipMemcpyAsync(ws, idx_host, sizeof(int) * p, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_gather_cols, dim3(grid_for((int64_t)m * p)), dim3(256), 0, ctx->stream, m, p, V, ldv,
+                     (const int*)ws, C, ldc);
+  FE_CHECK_LAUNCH();
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fe_zero_rows_small(fe_ctx ctx, int r, int p, double* C, int64_t ldc) {
+  FE_ARG(ctx && C, "fe_zero_rows_small: null argument");
+  if ((int64_t)r * p == 0) return 0;
+  hipLaunchKernelGGL(k_zero_rows_small, dim3(grid_for((int64_t)r * p)), dim3(256), 0, ctx->stream, r, p, C,
+                     ldc);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_event_record(fe_ctx ctx, int slot) {
+  FE_ARG(ctx && slot >= 0 && slot < 16, "fe_event_record: bad slot");
+  FE_HIP_TRY(hipEventRecord(ctx->events[slot], ctx->stream));
+  return 0;
+}
+
+int fe_event_elapsed_ms(fe_ctx ctx, int s0, int s1, float* ms) {
+  FE_ARG(ctx && ms && s0 >= 0 && s0 < 16 && s1 >= 0 && s1 < 16, "fe_event_elapsed_ms: bad slot");
+  FE_HIP_TRY(hipEventSynchronize(ctx->events[s1]));
+  FE_HIP_TRY(hipEventElapsedTime(ms, ctx->events[s0], ctx->events[s1]));
+  return 0;
+}
+
+}  // extern "C"

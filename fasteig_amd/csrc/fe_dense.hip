@@ -1,0 +1,384 @@
+// MFMA kernels for the tall-skinny dense products of the eigensolver:
+//   Gram   G = X^T diag(w) Y   (split-K over row chunks, fixed-order reduction)
+//   GEMM   Y = alpha X C + beta Y (C small, FP64 on the device)
+//
+// Both use 16x16x4 MFMA tiles (FP64: v_mfma_f64_16x16x4_f64, FP32:
+// v_mfma_f32_16x16x4_f32), one operand element per lane:
+//   A operand: lane l holds A[l % 16][l / 16]      (16 x 4)
+//   B operand: lane l holds B[l / 16][l % 16]      (4 x 16)
+//   D result : lane l, item i holds D[4 (l / 16) + i][l % 16]
+// The BF16x3 Gram splits each FP32 operand into bf16 hi + lo parts and runs
+// v_mfma_f32_16x16x32_bf16 (8 bf16 per lane: lane l holds A[l % 16][8 (l/16) + 0..7]).
+// A workgroup is 4 wavefronts computing a 64 x 64 output tile (32 x 32 each).
+#include "fe_common.hpp"
+
+namespace fe {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+template <typename T>
+struct mfma;
+template <>
+struct mfma<double> {
+  using acc = d4_t;
+  __device__ static __forceinline__ acc run(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct mfma<float> {
+  using acc = f4_t;
+  __device__ static __forceinline__ acc run(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int TILE = 64;   // output tile edge per workgroup
+constexpr int KCG = 32;    // rows per LDS stage in the Gram kernel
+constexpr int KCM = 16;    // inner-dimension chunk in the GEMM kernel
+constexpr int PAD = 1;     // LDS row padding (elements)
+
+// ---------------------------------------------------------------- Gram (native)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
+                                              const T* __restrict__ Y, int64_t ldy,
+                                              const double* __restrict__ w, int tiles_b, int ntiles,
+                                              const int* __restrict__ tile_list, int64_t rows_per,
+                                              double* __restrict__ part) {
+  __shared__ T Xs[KCG][TILE + PAD];
+  __shared__ T Ys[KCG][TILE + PAD];
+  const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
+  const int ta = tile / tiles_b, tb = tile % tiles_b;
+  const int a0 = ta * TILE, b0 = tb * TILE;
+  const int64_t chunk = blockIdx.y;
+  const int64_t r0 = chunk * rows_per;
+  const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  using acc_t = typename mfma<T>::acc;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{};
+  // each thread stages 8 elements of X and 8 of Y per stage: row = tid / 8, cols (tid % 8) * 8 ..
+  const int lr = tid >> 3, lc = (tid & 7) * 8;
+  for (int64_t rb = r0; rb < r1; rb += KCG) {
+    {
+      const int64_t r = rb + lr;
+      const bool rok = r < r1;
+      const double wr = (w && rok) ? w[r] : 1.0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int ca = a0 + lc + e, cb = b0 + lc + e;
+        Xs[lr][lc + e] = (rok && ca < m) ? X[r * ldx + ca] : T(0);
+        T yv = (rok && cb < p) ? Y[r * ldy + cb] : T(0);
+        if (w) yv = static_cast<T>(static_cast<double>(yv) * wr);
+        Ys[lr][lc + e] = yv;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KCG; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      T af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = Xs[kr][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Ys[kr][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  // partial tile, column-major 64 x 64: part[((chunk * ntiles) + blockIdx.x) * 4096 + col * 64 + row]
+  double* P = part + ((int64_t)chunk * ntiles + blockIdx.x) * (TILE * TILE);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = wm * 32 + i * 16 + 4 * (lane >> 4) + e;
+        const int col = wn * 32 + j * 16 + (lane & 15);
+        P[col * TILE + row] = static_cast<double>(acc[i][j][e]);
+      }
+}
+
+// ---------------------------------------------------------------- Gram (BF16x3)
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+__global__ __launch_bounds__(256) void k_gram_bf16x3(int64_t n, int m, int p, const float* __restrict__ X,
+                                                     int64_t ldx, const float* __restrict__ Y, int64_t ldy,
+                                                     const double* __restrict__ w, int tiles_b, int ntiles,
+                                                     const int* __restrict__ tile_list, int64_t rows_per,
+                                                     double* __restrict__ part) {
+  // stage KCG rows; store as [col][row] so each lane's 8 consecutive k values are contiguous
+  __shared__ __bf16 Xh[TILE][KCG + 8], Xl[TILE][KCG + 8], Yh[TILE][KCG + 8], Yl[TILE][KCG + 8];
+  const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
+  const int ta = tile / tiles_b, tb = tile % tiles_b;
+  const int a0 = ta * TILE, b0 = tb * TILE;
+  const int64_t chunk = blockIdx.y;
+  const int64_t r0 = chunk * rows_per;
+  const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  f4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f4_t{};
+  const int lr = tid >> 3, lc = (tid & 7) * 8;
+  for (int64_t rb = r0; rb < r1; rb += KCG) {
+    {
+      const int64_t r = rb + lr;
+      const bool rok = r < r1;
+      const float wr = (w && rok) ? static_cast<float>(w[r]) : 1.0f;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int ca = a0 + lc + e, cb = b0 + lc + e;
+        const float xv = (rok && ca < m) ? X[r * ldx + ca] : 0.0f;
+        float yv = (rok && cb < p) ? Y[r * ldy + cb] : 0.0f;
+        if (w) yv *= wr;
+        __bf16 h, l;
+        split_bf16(xv, h, l);
+        Xh[lc + e][lr] = h;
+        Xl[lc + e][lr] = l;
+        split_bf16(yv, h, l);
+        Yh[lc + e][lr] = h;
+        Yl[lc + e][lr] = l;
+      }
+    }
+    __syncthreads();
+    const int kb = 8 * (lane >> 4);
+    bf8_t ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int c = wm * 32 + i * 16 + (lane & 15);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        ah[i][e] = Xh[c][kb + e];
+        al[i][e] = Xl[c][kb + e];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        bh[j][e] = Yh[c][kb + e];
+        bl[j][e] = Yl[c][kb + e];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  double* P = part + ((int64_t)chunk * ntiles + blockIdx.x) * (TILE * TILE);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = wm * 32 + i * 16 + 4 * (lane >> 4) + e;
+        const int col = wn * 32 + j * 16 + (lane & 15);
+        P[col * TILE + row] = static_cast<double>(acc[i][j][e]);
+      }
+}
+
+// Fixed-order sum over chunks; mirrors the lower triangle for symmetric Grams.
+__global__ void k_gram_reduce(int m, int p, int tiles_b, int ntiles, const int* __restrict__ tile_list,
+                              int chunks, const double* __restrict__ part, double* G, int64_t ldg, int sym) {
+  const int t = blockIdx.x;  // computed tile slot
+  const int tile = tile_list ? tile_list[t] : t;
+  const int ta = tile / tiles_b, tb = tile % tiles_b;
+  for (int e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
+    const int col = e / TILE, row = e % TILE;
+    const int a = ta * TILE + row, b = tb * TILE + col;
+    if (a >= m || b >= p) continue;
+    double s = 0.0;
+    for (int c = 0; c < chunks; ++c) s += part[((int64_t)c * ntiles + t) * (TILE * TILE) + e];
+    if (sym) {
+      if (ta == tb) {
+        if (a <= b) {
+          G[(int64_t)b * ldg + a] = s;
+          G[(int64_t)a * ldg + b] = s;
+        }
+      } else {
+        G[(int64_t)b * ldg + a] = s;
+        G[(int64_t)a * ldg + b] = s;
+      }
+    } else {
+      G[(int64_t)b * ldg + a] = s;
+    }
+  }
+}
+
+// For symmetric diagonal tiles the MFMA result is symmetric up to rounding of
+// the same products in a different order; using the upper entry for both
+// halves makes G exactly symmetric.
+
+// ---------------------------------------------------------------- GEMM NN
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
+                                                 const double* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
+                                                 T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles) {
+  __shared__ T Xs[TILE][KCM + PAD];
+  __shared__ T Cs[KCM][TILE + PAD];
+  const int64_t t = xcd_tile(blockIdx.x, per_xcd);
+  if (t >= ntiles) return;
+  const int64_t rt = t / tiles_c;
+  const int ct = static_cast<int>(t % tiles_c);
+  const int64_t r0 = rt * TILE;
+  const int c0 = ct * TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  using acc_t = typename mfma<T>::acc;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{};
+  // X stage: 64 rows x 16 k: thread -> row tid/4, k (tid%4)*4 .. +3
+  const int xr = tid >> 2, xk = (tid & 3) * 4;
+  // C stage: 16 k x 64 cols: thread -> col tid/4, k (tid%4)*4 .. +3 (C is column-major)
+  const int cc = tid >> 2, ck = (tid & 3) * 4;
+  for (int k0 = 0; k0 < m; k0 += KCM) {
+    {
+      const int64_t r = r0 + xr;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int kx = k0 + xk + e;
+        Xs[xr][xk + e] = (r < n && kx < m) ? X[r * ldx + kx] : T(0);
+      }
+      const int col = c0 + cc;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int kx = k0 + ck + e;
+        Cs[ck + e][cc] = (col < p && kx < m) ? static_cast<T>(C[(int64_t)col * ldc + kx]) : T(0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KCM; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      T af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = Xs[wm * 32 + i * 16 + (lane & 15)][kr];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Cs[kr][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = r0 + wm * 32 + i * 16 + 4 * (lane >> 4) + e;
+        const int col = c0 + wn * 32 + j * 16 + (lane & 15);
+        if (row < n && col < p) {
+          T& y = Y[row * ldy + col];
+          const T v = alpha * static_cast<T>(acc[i][j][e]);
+          y = beta == T(0) ? v : v + beta * y;
+        }
+      }
+}
+
+}  // namespace fe
+
+using namespace fe;
+
+extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                       const void* Y, int64_t ldy, const double* w, double* G, int64_t ldg) {
+  FE_ARG(ctx && X && Y && G, "fe_gram: null argument");
+  FE_ARG(m >= 0 && p >= 0 && n >= 0 && ldx >= m && ldy >= p && ldg >= m, "fe_gram: bad dimensions");
+  FE_ARG(prec == FE_F64 || prec == FE_F32, "fe_gram: bad precision");
+  FE_ARG(mode == FE_GRAM_NATIVE || (mode == FE_GRAM_BF16X3 && prec == FE_F32), "fe_gram: bad mode");
+  if (m == 0 || p == 0) return 0;
+  const int tiles_a = (m + TILE - 1) / TILE, tiles_b = (p + TILE - 1) / TILE;
+  const bool sym = (X == Y) && (m == p) && (ldx == ldy) && !w;
+  // tile list (upper triangle only when symmetric)
+  std::vector<int> tl;
+  for (int a = 0; a < tiles_a; ++a)
+    for (int b = 0; b < tiles_b; ++b)
+      if (!sym || a <= b) tl.push_back(a * tiles_b + b);
+  const int ntiles = static_cast<int>(tl.size());
+  if (n == 0) return fe_zero2d(ctx, G, p, m, ldg, 8);
+  // split rows so that about 4 workgroups per CU are live
+  const int target = 4 * (ctx->cus > 0 ? ctx->cus : 256);
+  int64_t chunks = (target + ntiles - 1) / ntiles;
+  int64_t rows_per = (n + chunks - 1) / chunks;
+  if (rows_per < KCG) rows_per = KCG;
+  rows_per = (rows_per + KCG - 1) / KCG * KCG;
+  chunks = (n + rows_per - 1) / rows_per;
+  void* ws = nullptr;
+  const size_t part_bytes = sizeof(double) * (size_t)chunks * ntiles * TILE * TILE;
+  const size_t list_bytes = sizeof(int) * ntiles;
+  int rc = scratch(ctx, 0, part_bytes + list_bytes + 256, &ws);
+  if (rc) return rc;
+  double* part = (double*)ws;
+  int* dlist = (int*)((char*)ws + part_bytes);
+  FE_HIP_TRY(hipMemcpyAsync(dlist, tl.data(), list_bytes, hipMemcpyHostToDevice, ctx->stream));
+  const dim3 grid((unsigned)ntiles, (unsigned)chunks);
+  if (prec == FE_F64)
+    hipLaunchKernelGGL(k_gram<double>, grid, dim3(256), 0, ctx->stream, n, m, p, (const double*)X, ldx,
+                       (const double*)Y, ldy, w, tiles_b, ntiles, dlist, rows_per, part);
+  else if (mode == FE_GRAM_NATIVE)
+    hipLaunchKernelGGL(k_gram<float>, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                       (const float*)Y, ldy, w, tiles_b, ntiles, dlist, rows_per, part);
+  else
+    hipLaunchKernelGGL(k_gram_bf16x3, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                       (const float*)Y, ldy, w, tiles_b, ntiles, dlist, rows_per, part);
+  FE_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles), dim3(256), 0, ctx->stream, m, p, tiles_b, ntiles,
+                     dlist, (int)chunks, part, G, ldg, sym ? 1 : 0);
+  FE_CHECK_LAUNCH();
+  // the host tile list must outlive the async copy: wait for it
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                          const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta) {
+  FE_ARG(ctx && X && C && Y, "fe_gemm_nn: null argument");
+  FE_ARG(X != Y, "fe_gemm_nn: Y must not alias X");
+  FE_ARG(n >= 0 && m >= 0 && p >= 0 && ldx >= m && ldy >= p && ldc >= m, "fe_gemm_nn: bad dimensions");
+  if (n == 0 || p == 0) return 0;
+  if (m == 0) {
+    if (beta == 0.0) return fe_zero2d(ctx, Y, n, p, ldy, (int)prec_bytes(prec));
+    return fe_axpby_cols(ctx, prec, n, p, Y, ldy, Y, ldy, 0.0, beta, nullptr);
+  }
+  const int tiles_c = (p + TILE - 1) / TILE;
+  const int64_t tiles_r = (n + TILE - 1) / TILE;
+  const int64_t ntiles = tiles_r * tiles_c;
+  const int64_t nb8 = (ntiles + 7) / 8 * 8;
+  if (prec == FE_F64)
+    hipLaunchKernelGGL(k_gemm_nn<double>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p,
+                       (const double*)X, ldx, C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles);
+  else
+    hipLaunchKernelGGL(k_gemm_nn<float>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p,
+                       (const float*)X, ldx, C, ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8,
+                       ntiles);
+  FE_CHECK_LAUNCH();
+  return 0;
+}

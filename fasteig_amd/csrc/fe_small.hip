@@ -1,0 +1,403 @@
+// Small dense FP64 kernels for the Rayleigh-Ritz and Cholesky-QR steps.
+// All matrices are column-major on the device.
+//
+//   k_potrf       Cholesky G = L L^T, one workgroup (right-looking, columns)
+//   k_chol_rinv   (L^T)^{-1}, one thread per column of L^{-1}
+//   k_syev_tri    one workgroup: Householder tridiagonalization (Golub & Van
+//                 Loan, Alg. 8.3.1), Sturm-count bisection for every eigenvalue
+//                 (GvL 8.4.1), inverse iteration for the eigenvectors of the
+//                 tridiagonal with modified Gram-Schmidt inside clusters of close
+//                 eigenvalues (the strategy of LAPACK dstein; Wilkinson's
+//                 inverse iteration)
+//   k_syev_back   multi-workgroup back-transformation V = H_0 ... H_{m-3} Z with
+//                 the eigenvector slice of each workgroup held in LDS
+#include "fe_common.hpp"
+
+#include <cfloat>
+
+namespace fe {
+
+constexpr int SYEV_MAX = 1024;  // largest projected matrix handled on the device
+
+__global__ __launch_bounds__(1024) void k_potrf(int k, double* G, int64_t ldg, int* info) {
+  __shared__ int fail;
+  __shared__ double djj;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) {
+    fail = 0;
+    info[0] = 0;
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    if (tid == 0) {
+      const double d = G[(int64_t)j * ldg + j];
+      if (!(d > 0.0) || !isfinite(d)) {
+        fail = j + 1;
+        info[0] = j + 1;
+      } else {
+        djj = sqrt(d);
+        G[(int64_t)j * ldg + j] = djj;
+      }
+    }
+    __syncthreads();
+    if (fail) return;
+    const double inv = 1.0 / djj;
+    for (int i = j + 1 + tid; i < k; i += nt) G[(int64_t)j * ldg + i] *= inv;
+    __syncthreads();
+    const int r = k - j - 1;
+    const int64_t tot = (int64_t)r * r;
+    for (int64_t t = tid; t < tot; t += nt) {
+      const int c = j + 1 + static_cast<int>(t / r);
+      const int i = j + 1 + static_cast<int>(t % r);
+      if (i >= c) G[(int64_t)c * ldg + i] -= G[(int64_t)j * ldg + i] * G[(int64_t)j * ldg + c];
+    }
+    __syncthreads();
+  }
+}
+
+// Rinv = (L^T)^{-1}: thread j solves L x = e_j (x_i = 0, i < j) and writes x
+// as row j of Rinv (Rinv(j, i) = x_i), which is upper triangular.
+__global__ void k_chol_rinv(int k, const double* __restrict__ L, int64_t ldl, double* Rinv, int64_t ldr) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  for (int i = 0; i < j; ++i) Rinv[(int64_t)i * ldr + j] = 0.0;
+  Rinv[(int64_t)j * ldr + j] = 1.0 / L[(int64_t)j * ldl + j];
+  for (int i = j + 1; i < k; ++i) {
+    double s = 0.0;
+    for (int p = j; p < i; ++p) s += L[(int64_t)p * ldl + i] * Rinv[(int64_t)p * ldr + j];
+    Rinv[(int64_t)i * ldr + j] = -s / L[(int64_t)i * ldl + i];
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nw; ++w) s += red[w];  // fixed order: every thread gets the same value
+  return s;
+}
+
+__device__ __forceinline__ int sturm_count(int m, const double* d, const double* e2, double x, double pivmin) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0.0) ++cnt;
+  for (int i = 1; i < m; ++i) {
+    q = d[i] - x - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+__device__ __forceinline__ double hash_unit(uint32_t a, uint32_t b) {
+  uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return (static_cast<double>(h) / 4294967295.0) * 2.0 - 1.0;
+}
+
+// work layout (doubles): u1, u2, u3, ml, piv, Z  -- each m*m, interleaved [i * m + t]
+__global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda, double* W, double* tau,
+                                                   double* work, int* info) {
+  __shared__ double vs[SYEV_MAX], ps[SYEV_MAX], wsv[SYEV_MAX];
+  __shared__ double d[SYEV_MAX], e2[SYEV_MAX], lam[SYEV_MAX];
+  __shared__ double red[16];
+  __shared__ double s_tau, s_scale, s_gl, s_gu, s_pivmin, s_onenrm;
+  __shared__ int s_skip;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+  if (tid == 0) info[0] = 0;
+
+  // symmetrize from the lower triangle
+  for (int64_t t = tid; t < (int64_t)m * m; t += nt) {
+    const int j = static_cast<int>(t / m), i = static_cast<int>(t % m);
+    if (i < j) A[(int64_t)j * lda + i] = A[(int64_t)i * lda + j];
+  }
+  __syncthreads();
+
+  // ---- 1. Householder tridiagonalization
+  for (int k = 0; k + 2 < m; ++k) {
+    const int len = m - k - 1;
+    double* x = A + (int64_t)k * lda + (k + 1);
+    double part = 0.0;
+    for (int i = 1 + tid; i < len; i += nt) part += x[i] * x[i];
+    const double xn2 = block_sum(part, red);
+    if (tid == 0) {
+      const double alpha = x[0];
+      if (xn2 == 0.0) {
+        s_skip = 1;
+        tau[k] = 0.0;
+        e2[k] = alpha;  // store e temporarily (squared later)
+      } else {
+        s_skip = 0;
+        const double beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        s_tau = (beta - alpha) / beta;
+        s_scale = 1.0 / (alpha - beta);
+        tau[k] = s_tau;
+        e2[k] = beta;
+      }
+    }
+    __syncthreads();
+    if (s_skip) continue;
+    const double t_k = s_tau, sc = s_scale;
+    for (int i = tid; i < len; i += nt) {
+      const double v = i == 0 ? 1.0 : x[i] * sc;
+      vs[i] = v;
+    }
+    __syncthreads();
+    for (int i = tid; i < len; i += nt) x[i] = vs[i];  // keep v for the back-transformation
+    // p = tau * A22 v : one wavefront per column of A22
+    for (int j = wave; j < len; j += nw) {
+      const double* col = A + (int64_t)(k + 1 + j) * lda + (k + 1);
+      double s = 0.0;
+      for (int i = lane; i < len; i += 64) s += col[i] * vs[i];
+      s = wave_sum(s);
+      if (lane == 0) ps[j] = t_k * s;
+    }
+    __syncthreads();
+    double pv = 0.0;
+    for (int i = tid; i < len; i += nt) pv += ps[i] * vs[i];
+    const double K = 0.5 * t_k * block_sum(pv, red);
+    for (int i = tid; i < len; i += nt) wsv[i] = ps[i] - K * vs[i];
+    __syncthreads();
+    for (int j = wave; j < len; j += nw) {
+      double* col = A + (int64_t)(k + 1 + j) * lda + (k + 1);
+      const double vj = vs[j], wj = wsv[j];
+      for (int i = lane; i < len; i += 64) col[i] -= vs[i] * wj + wsv[i] * vj;
+    }
+    __syncthreads();
+  }
+  // diagonal and off-diagonal of T
+  for (int i = tid; i < m; i += nt) d[i] = A[(int64_t)i * lda + i];
+  if (tid == 0 && m >= 2) e2[m - 2] = A[(int64_t)(m - 2) * lda + (m - 1)];
+  __syncthreads();
+  // e2 currently holds e; Gershgorin bounds and squares
+  if (tid == 0) {
+    double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double el = i > 0 ? fabs(e2[i - 1]) : 0.0;
+      const double er = i + 1 < m ? fabs(e2[i]) : 0.0;
+      gl = fmin(gl, d[i] - el - er);
+      gu = fmax(gu, d[i] + el + er);
+      if (i + 1 < m) emax2 = fmax(emax2, e2[i] * e2[i]);
+    }
+    const double onenrm = fmax(fabs(gl), fabs(gu));
+    s_onenrm = onenrm > 0.0 ? onenrm : 1.0;
+    s_pivmin = DBL_MIN * fmax(1.0, emax2);
+    // widen the interval slightly
+    const double pad = 2.0 * DBL_EPSILON * s_onenrm * m + 2.0 * s_pivmin;
+    s_gl = gl - pad;
+    s_gu = gu + pad;
+  }
+  __syncthreads();
+  // keep e (unsquared) in ps for the inverse iteration, squares in e2
+  for (int i = tid; i + 1 < m; i += nt) {
+    ps[i] = e2[i];
+    e2[i] = e2[i] * e2[i];
+  }
+  __syncthreads();
+
+  // ---- 2. bisection: thread t finds the t-th smallest eigenvalue
+  const double pivmin = s_pivmin, onenrm = s_onenrm;
+  for (int t = tid; t < m; t += nt) {
+    double lo = s_gl, hi = s_gu;
+    for (int it = 0; it < 200; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin || mid == lo || mid == hi) break;
+      if (sturm_count(m, d, e2, mid, pivmin) > t) hi = mid;
+      else lo = mid;
+    }
+    lam[t] = 0.5 * (lo + hi);
+    W[t] = lam[t];
+  }
+  __syncthreads();
+
+  // ---- 3. inverse iteration (cluster leaders process their cluster in order)
+  // Cluster threshold for reorthogonalization. LAPACK dstein uses 1e-3*||T||;
+  // 1e-5*||T|| keeps chains of merely close Ritz values short while pairs that
+  // are not reorthogonalized stay orthogonal to about eps*||T||/gap <= 1e-11.
+  const double ortol = 1.0e-5 * onenrm;
+  const double pertol = 10.0 * DBL_EPSILON * onenrm;
+  const int64_t MM = (int64_t)m * m;
+  double* u1 = work;
+  double* u2 = work + MM;
+  double* u3 = work + 2 * MM;
+  double* ml = work + 3 * MM;
+  double* pv_ = work + 4 * MM;
+  double* Z = work + 5 * MM;
+  for (int t = tid; t < m; t += nt) {
+    if (t > 0 && lam[t] - lam[t - 1] <= ortol) continue;  // not a leader
+    int last = t;
+    while (last + 1 < m && lam[last + 1] - lam[last] <= ortol) ++last;
+    double prev_shift = 0.0;
+    for (int j = t; j <= last; ++j) {
+      double xj = lam[j];
+      if (j > t && xj - prev_shift < pertol) xj = prev_shift + pertol;
+      prev_shift = xj;
+      // LU with partial pivoting of T - xj I into slot t (interleaved)
+      const double tiny = fmax(DBL_EPSILON * onenrm, pivmin);
+      u1[0 * m + t] = d[0] - xj;
+      if (m > 1) u2[0 * m + t] = ps[0];
+      for (int i = 0; i + 1 < m; ++i) {
+        const double ui = u1[(int64_t)i * m + t];
+        const double bi = ps[i];
+        const double a1 = d[i + 1] - xj;
+        const double c1 = i + 2 < m ? ps[i + 1] : 0.0;
+        const double u2i = u2[(int64_t)i * m + t];
+        if (fabs(ui) >= fabs(bi)) {
+          const double piv = ui != 0.0 ? ui : tiny;
+          u1[(int64_t)i * m + t] = piv;
+          const double l = bi / piv;
+          ml[(int64_t)i * m + t] = l;
+          pv_[(int64_t)i * m + t] = 0.0;
+          u3[(int64_t)i * m + t] = 0.0;
+          u1[(int64_t)(i + 1) * m + t] = a1 - l * u2i;
+          if (i + 2 < m) u2[(int64_t)(i + 1) * m + t] = c1;
+        } else {
+          const double l = ui / bi;
+          ml[(int64_t)i * m + t] = l;
+          pv_[(int64_t)i * m + t] = 1.0;
+          u1[(int64_t)i * m + t] = bi;
+          u2[(int64_t)i * m + t] = a1;
+          u3[(int64_t)i * m + t] = c1;
+          u1[(int64_t)(i + 1) * m + t] = u2i - l * a1;
+          if (i + 2 < m) u2[(int64_t)(i + 1) * m + t] = -l * c1;
+        }
+      }
+      if (u1[(int64_t)(m - 1) * m + t] == 0.0) u1[(int64_t)(m - 1) * m + t] = tiny;
+      // start vector
+      for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] = hash_unit(j + 1, i + 1);
+      for (int iter = 0; iter < 3; ++iter) {
+        // forward elimination with the recorded interchanges
+        for (int i = 0; i + 1 < m; ++i) {
+          double yi = Z[(int64_t)i * m + j], yi1 = Z[(int64_t)(i + 1) * m + j];
+          if (pv_[(int64_t)i * m + t] != 0.0) {
+            const double tmp = yi;
+            yi = yi1;
+            yi1 = tmp;
+          }
+          yi1 -= ml[(int64_t)i * m + t] * yi;
+          Z[(int64_t)i * m + j] = yi;
+          Z[(int64_t)(i + 1) * m + j] = yi1;
+        }
+        // back substitution
+        double nrm = 0.0;
+        for (int i = m - 1; i >= 0; --i) {
+          double s = Z[(int64_t)i * m + j];
+          if (i + 1 < m) s -= u2[(int64_t)i * m + t] * Z[(int64_t)(i + 1) * m + j];
+          if (i + 2 < m) s -= u3[(int64_t)i * m + t] * Z[(int64_t)(i + 2) * m + j];
+          const double piv = u1[(int64_t)i * m + t];
+          double yi = s / piv;
+          if (!isfinite(yi)) yi = s / tiny;
+          Z[(int64_t)i * m + j] = yi;
+        }
+        // scale down to avoid overflow, reorthogonalize inside the cluster, normalize
+        double mx = 0.0;
+        for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(Z[(int64_t)i * m + j]));
+        if (mx > 0.0)
+          for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] /= mx;
+        for (int q = t; q < j; ++q) {
+          double dot = 0.0;
+          for (int i = 0; i < m; ++i) dot += Z[(int64_t)i * m + q] * Z[(int64_t)i * m + j];
+          for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] -= dot * Z[(int64_t)i * m + q];
+        }
+        for (int i = 0; i < m; ++i) nrm += Z[(int64_t)i * m + j] * Z[(int64_t)i * m + j];
+        nrm = sqrt(nrm);
+        if (!(nrm > 0.0)) {
+          info[0] = 2;
+          nrm = 1.0;
+        }
+        for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] /= nrm;
+      }
+    }
+  }
+}
+
+// V(:, cols) = H_0 ... H_{m-3} Z(:, cols); Householder vectors in A's lower part.
+// Each workgroup owns CB columns of Z, held in LDS as [row][CB].
+__global__ __launch_bounds__(256) void k_syev_back(int m, const double* __restrict__ A, int64_t lda,
+                                                   const double* __restrict__ tau, const double* __restrict__ Z,
+                                                   double* V, int64_t ldv, int CB) {
+  extern __shared__ double zs[];  // m * CB
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * CB;
+  const int ncol = min(CB, m - c0);
+  for (int t = tid; t < m * CB; t += blockDim.x) {
+    const int i = t / CB, c = t % CB;
+    zs[t] = c < ncol ? Z[(int64_t)i * m + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int TPC = 256 / CB;         // threads per column
+  const int c = tid / TPC, sub = tid % TPC;
+  for (int k = m - 3; k >= 0; --k) {
+    const double tk = tau[k];
+    if (tk == 0.0) continue;
+    const int len = m - k - 1;
+    const double* v = A + (int64_t)k * lda + (k + 1);
+    double s = 0.0;
+    for (int i = sub; i < len; i += TPC) s += v[i] * zs[(k + 1 + i) * CB + c];
+    for (int o = TPC / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    s *= tk;
+    __syncthreads();
+    for (int i = sub; i < len; i += TPC) zs[(k + 1 + i) * CB + c] -= s * v[i];
+    __syncthreads();
+  }
+  for (int t = tid; t < m * CB; t += blockDim.x) {
+    const int cc = t / m, i = t % m;
+    if (cc < ncol) V[(int64_t)(c0 + cc) * ldv + i] = zs[i * CB + cc];
+  }
+}
+
+}  // namespace fe
+
+using namespace fe;
+
+extern "C" int fe_potrf(fe_ctx ctx, int k, double* G, int64_t ldg) {
+  FE_ARG(ctx && G && k >= 0 && ldg >= k, "fe_potrf: bad argument");
+  if (k == 0) return 0;
+  hipLaunchKernelGGL(k_potrf, dim3(1), dim3(1024), 0, ctx->stream, k, G, ldg, ctx->info_dev);
+  FE_CHECK_LAUNCH();
+  int info = 0;
+  FE_HIP_TRY(hipMemcpyAsync(&info, ctx->info_dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return info;
+}
+
+extern "C" int fe_chol_rinv(fe_ctx ctx, int k, const double* L, int64_t ldl, double* Rinv, int64_t ldr) {
+  FE_ARG(ctx && L && Rinv && k >= 0 && ldl >= k && ldr >= k, "fe_chol_rinv: bad argument");
+  if (k == 0) return 0;
+  hipLaunchKernelGGL(k_chol_rinv, dim3(blocks_for(k, 64)), dim3(64), 0, ctx->stream, k, L, ldl, Rinv, ldr);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, double* V, int64_t ldv) {
+  FE_ARG(ctx && A && W && V && m >= 0 && lda >= m && ldv >= m, "fe_syev: bad argument");
+  FE_ARG(m <= SYEV_MAX, "fe_syev: matrix larger than 1024 is not supported");
+  if (m == 0) return 0;
+  const int64_t MM = (int64_t)m * m;
+  void* ws = nullptr;
+  int rc = scratch(ctx, 1, sizeof(double) * (6 * MM + m + 64), &ws);
+  if (rc) return rc;
+  double* tau = (double*)ws;
+  double* work = tau + m + 64;
+  hipLaunchKernelGGL(k_syev_tri, dim3(1), dim3(1024), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev);
+  FE_CHECK_LAUNCH();
+  int CB = 32;  // columns per workgroup, LDS slice m*CB doubles within 64 KiB
+  while (CB > 1 && (size_t)m * CB * sizeof(double) > 65536) CB >>= 1;
+  const size_t lds = sizeof(double) * (size_t)m * CB;
+  hipLaunchKernelGGL(k_syev_back, dim3(blocks_for(m, CB)), dim3(256), lds, ctx->stream, m, A, lda, tau,
+                     work + 5 * MM, V, ldv, CB);
+  FE_CHECK_LAUNCH();
+  int info = 0;
+  FE_HIP_TRY(hipMemcpyAsync(&info, ctx->info_dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return info;
+}

@@ -1,0 +1,437 @@
+// Fused block SpMM for gfx950: Y = alpha * A X + beta * X + gamma * Z.
+//
+// Blocks are row-major, so the k values that one nonzero a_ij multiplies are
+// the contiguous row j of X. A group of LPR lanes (power of two, <= 64) owns
+// one row of A and VEC consecutive columns per lane (16-byte vector loads);
+// a 64-lane wavefront therefore holds 64/LPR rows. The group first loads up to
+// LPR (column, value) pairs of its row with one coalesced load, then walks
+// them, broadcasting each pair with a lane shuffle (a scalar readlane when the
+// whole wavefront is one row) and gathering the 16*LPR-byte X row segment.
+// A is read once per column tile; X rows are re-read from L2 by the rows that
+// share them; Y is written once. Workgroups are mapped XCD-contiguously.
+#include "fe_common.hpp"
+
+namespace fe {
+
+template <typename T, int VEC>
+struct vec_t;
+template <>
+struct vec_t<float, 4> {
+  using type = float4;
+};
+template <>
+struct vec_t<float, 2> {
+  using type = float2;
+};
+template <>
+struct vec_t<float, 1> {
+  using type = float;
+};
+template <>
+struct vec_t<double, 2> {
+  using type = double2;
+};
+template <>
+struct vec_t<double, 1> {
+  using type = double;
+};
+
+template <typename T, int VEC>
+__device__ __forceinline__ void load_vec(const T* p, T (&v)[VEC]) {
+  if constexpr (VEC == 1) {
+    v[0] = *p;
+  } else {
+    using V = typename vec_t<T, VEC>::type;
+    const V x = *reinterpret_cast<const V*>(p);
+    if constexpr (VEC == 2) {
+      v[0] = x.x;
+      v[1] = x.y;
+    } else {
+      v[0] = x.x;
+      v[1] = x.y;
+      v[2] = x.z;
+      v[3] = x.w;
+    }
+  }
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void store_vec(T* p, const T (&v)[VEC]) {
+  if constexpr (VEC == 1) {
+    *p = v[0];
+  } else {
+    using V = typename vec_t<T, VEC>::type;
+    V x;
+    if constexpr (VEC == 2) {
+      x.x = v[0];
+      x.y = v[1];
+    } else {
+      x.x = v[0];
+      x.y = v[1];
+      x.z = v[2];
+      x.w = v[3];
+    }
+    *reinterpret_cast<V*>(p) = x;
+  }
+}
+
+template <int LPR>
+__device__ __forceinline__ int bcast_int(int v, int src_lane) {
+  if constexpr (LPR == 64) return __builtin_amdgcn_readlane(v, src_lane);
+  else return __shfl(v, src_lane, 64);
+}
+
+template <int LPR, typename T>
+__device__ __forceinline__ T bcast_val(T v, int src_lane) {
+  if constexpr (LPR == 64) {
+    if constexpr (sizeof(T) == 8) {
+      const long long b = __double_as_longlong(v);
+      const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), src_lane);
+      const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), src_lane);
+      return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    } else {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src_lane));
+    }
+  } else {
+    return __shfl(v, src_lane, 64);
+  }
+}
+
+// VEC elements per lane, LPR lanes per row, RS vectors per lane per row
+// (RS > 1 lets one lane carry 2 vectors so a wave covers 2*64*VEC columns).
+template <typename T, int VEC, int LPR, int RS>
+__global__ __launch_bounds__(256) void k_spmm_csr(
+    int n, const int* __restrict__ rowptr, const int* __restrict__ colind, const T* __restrict__ vals,
+    int k, const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
+    const T* Z, int64_t ldz, T gamma, int64_t per_xcd, int64_t ntiles) {
+  constexpr int RPW = 64 / LPR;          // rows per wavefront
+  constexpr int RPB = 4 * RPW;           // rows per workgroup (4 waves)
+  constexpr int COLS = LPR * VEC * RS;   // columns per column tile
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t row = tile * RPB + wave * RPW + lane / LPR;
+  const int sub = lane % LPR;
+  const int gbase = lane - sub;          // first lane of this row group
+  const int c0 = blockIdx.y * COLS + sub * VEC;
+  const bool rowok = row < n;
+  const int beg = rowok ? rowptr[row] : 0;
+  const int end = rowok ? rowptr[row + 1] : 0;
+
+  T acc[RS][VEC];
+#pragma unroll
+  for (int r = 0; r < RS; ++r)
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[r][v] = T(0);
+  bool colok[RS];
+#pragma unroll
+  for (int r = 0; r < RS; ++r) colok[r] = c0 + r * LPR * VEC < k;
+
+  for (int p = beg; p < end; p += LPR) {
+    const int q = p + sub;
+    const int mycol = q < end ? colind[q] : 0;
+    const T myval = q < end ? vals[q] : T(0);
+    const int cnt = end - p < LPR ? end - p : LPR;
+    int t = 0;
+    for (; t + 4 <= cnt; t += 4) {
+      int cc[4];
+      T vv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cc[u] = bcast_int<LPR>(mycol, gbase + t + u);
+        vv[u] = bcast_val<LPR>(myval, gbase + t + u);
+      }
+      T xv[4][RS][VEC];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < RS; ++r)
+          if (colok[r]) load_vec<T, VEC>(X + (int64_t)cc[u] * ldx + c0 + r * LPR * VEC, xv[u][r]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < RS; ++r)
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) acc[r][v] += vv[u] * xv[u][r][v];
+    }
+    for (; t < cnt; ++t) {
+      const int cc = bcast_int<LPR>(mycol, gbase + t);
+      const T vv = bcast_val<LPR>(myval, gbase + t);
+#pragma unroll
+      for (int r = 0; r < RS; ++r)
+        if (colok[r]) {
+          T xv[VEC];
+          load_vec<T, VEC>(X + (int64_t)cc * ldx + c0 + r * LPR * VEC, xv);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) acc[r][v] += vv * xv[v];
+        }
+    }
+  }
+  if (!rowok) return;
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    if (!colok[r]) continue;
+    const int c = c0 + r * LPR * VEC;
+    T out[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) out[v] = alpha * acc[r][v];
+    if (beta != T(0)) {
+      T xv[VEC];
+      load_vec<T, VEC>(X + row * ldx + c, xv);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) out[v] += beta * xv[v];
+    }
+    if (Z && gamma != T(0)) {
+      T zv[VEC];
+      load_vec<T, VEC>(Z + row * ldz + c, zv);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) out[v] += gamma * zv[v];
+    }
+    store_vec<T, VEC>(Y + row * ldy + c, out);
+  }
+}
+
+// SELL-C-sigma variant: one wavefront per slice row group. Slices hold C rows;
+// within a slice the entries are stored column by column ([w][C]), so lane
+// groups of consecutive sorted rows read consecutive column/value entries.
+template <typename T, int VEC, int LPR>
+__global__ __launch_bounds__(256) void k_spmm_sell(
+    int n, int C, const int64_t* __restrict__ sptr, const int* __restrict__ slen,
+    const int* __restrict__ perm, const int* __restrict__ scol, const T* __restrict__ sval, int k,
+    const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta, const T* Z, int64_t ldz,
+    T gamma, int64_t per_xcd, int64_t ntiles) {
+  constexpr int RPW = 64 / LPR;
+  constexpr int RPB = 4 * RPW;
+  constexpr int COLS = LPR * VEC;
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t srow = tile * RPB + wave * RPW + lane / LPR;  // sorted row position
+  const int sub = lane % LPR;
+  const int c = blockIdx.y * COLS + sub * VEC;
+  if (srow >= n || c >= k) return;
+  const int64_t slice = srow / C;
+  const int within = static_cast<int>(srow - slice * C);
+  const int64_t base = sptr[slice];
+  const int w = slen[slice];
+  T acc[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) acc[v] = T(0);
+  int t = 0;
+  for (; t + 2 <= w; t += 2) {
+    const int64_t e0 = base + (int64_t)t * C + within;
+    const int64_t e1 = e0 + C;
+    const int c0i = scol[e0], c1i = scol[e1];
+    const T v0 = sval[e0], v1 = sval[e1];
+    T x0[VEC], x1[VEC];
+    load_vec<T, VEC>(X + (int64_t)c0i * ldx + c, x0);
+    load_vec<T, VEC>(X + (int64_t)c1i * ldx + c, x1);
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[v] += v0 * x0[v] + v1 * x1[v];
+  }
+  if (t < w) {
+    const int64_t e0 = base + (int64_t)t * C + within;
+    const int c0i = scol[e0];
+    const T v0 = sval[e0];
+    T x0[VEC];
+    load_vec<T, VEC>(X + (int64_t)c0i * ldx + c, x0);
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[v] += v0 * x0[v];
+  }
+  const int64_t row = perm[srow];
+  T out[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) out[v] = alpha * acc[v];
+  if (beta != T(0)) {
+    T xv[VEC];
+    load_vec<T, VEC>(X + row * ldx + c, xv);
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) out[v] += beta * xv[v];
+  }
+  if (Z && gamma != T(0)) {
+    T zv[VEC];
+    load_vec<T, VEC>(Z + row * ldz + c, zv);
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) out[v] += gamma * zv[v];
+  }
+  store_vec<T, VEC>(Y + row * ldy + c, out);
+}
+
+template <typename T, int VEC, int LPR, int RS>
+static int launch_csr(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha,
+                      T beta, const T* Z, int64_t ldz, T gamma, const T* vals) {
+  constexpr int RPB = 4 * (64 / LPR);
+  constexpr int COLS = LPR * VEC * RS;
+  const int64_t ntiles = (A->n + RPB - 1) / RPB;
+  const int64_t nb8 = (ntiles + 7) / 8 * 8;
+  const unsigned ycols = static_cast<unsigned>((k + COLS - 1) / COLS);
+  hipLaunchKernelGGL((k_spmm_csr<T, VEC, LPR, RS>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream,
+                     A->n, A->rowptr, A->colind, vals, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8,
+                     ntiles);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+template <typename T, int VEC, int LPR>
+static int launch_sell(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha,
+                       T beta, const T* Z, int64_t ldz, T gamma, const T* vals) {
+  constexpr int RPB = 4 * (64 / LPR);
+  constexpr int COLS = LPR * VEC;
+  const int64_t ntiles = (A->n + RPB - 1) / RPB;
+  const int64_t nb8 = (ntiles + 7) / 8 * 8;
+  const unsigned ycols = static_cast<unsigned>((k + COLS - 1) / COLS);
+  hipLaunchKernelGGL((k_spmm_sell<T, VEC, LPR>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream, A->n,
+                     A->sell_C, A->sell_ptr, A->sell_len, A->sell_perm, A->sell_col, vals, k, X, ldx, Y, ldy,
+                     alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+// Choose lanes-per-row so that one row group spans the k columns with
+// 16-byte vectors when the layout allows it.
+template <typename T>
+static int dispatch(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, double a,
+                    double b, const T* Z, int64_t ldz, double g) {
+  constexpr int V16 = 16 / sizeof(T);  // elements per 16-byte vector
+  auto aligned = [&](const void* p, int64_t ld) {
+    return ((reinterpret_cast<uintptr_t>(p) % 16) == 0) && (ld % V16 == 0);
+  };
+  const bool vec = (k % V16 == 0) && aligned(X, ldx) && aligned(Y, ldy) && (!Z || aligned(Z, ldz));
+  const T alpha = static_cast<T>(a), beta = static_cast<T>(b), gamma = static_cast<T>(g);
+  const T* v = A->use_sell ? nullptr : (sizeof(T) == 8 ? (const T*)A->vals64 : (const T*)A->vals32);
+  if (A->use_sell) {
+    const T* sv = sizeof(T) == 8 ? (const T*)A->sell_v64 : (const T*)A->sell_v32;
+    const int groups = vec ? k / V16 : k;
+    if (vec) {
+      if (groups <= 4) return launch_sell<T, V16, 4>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
+      if (groups <= 8) return launch_sell<T, V16, 8>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
+      if (groups <= 16) return launch_sell<T, V16, 16>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
+      if (groups <= 32) return launch_sell<T, V16, 32>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
+      return launch_sell<T, V16, 64>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
+    }
+    if (groups <= 8) return launch_sell<T, 1, 8>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
+    if (groups <= 32) return launch_sell<T, 1, 32>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
+    return launch_sell<T, 1, 64>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
+  }
+  if (vec) {
+    const int groups = k / V16;
+    if (groups <= 1) return launch_csr<T, V16, 1, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+    if (groups <= 2) return launch_csr<T, V16, 2, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+    if (groups <= 4) return launch_csr<T, V16, 4, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+    if (groups <= 8) return launch_csr<T, V16, 8, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+    if (groups <= 16) return launch_csr<T, V16, 16, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+    if (groups <= 32) return launch_csr<T, V16, 32, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+    if (groups <= 64) return launch_csr<T, V16, 64, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+    return launch_csr<T, V16, 64, 2>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  }
+  if (k <= 1) return launch_csr<T, 1, 1, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  if (k <= 2) return launch_csr<T, 1, 2, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  if (k <= 4) return launch_csr<T, 1, 4, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  if (k <= 8) return launch_csr<T, 1, 8, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  if (k <= 16) return launch_csr<T, 1, 16, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  if (k <= 32) return launch_csr<T, 1, 32, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  return launch_csr<T, 1, 64, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+}
+
+// ------------------------------------------------------------ SELL-C-sigma build
+
+// Host-side construction (a one-time format conversion): rows are sorted by
+// length, descending, inside windows of sigma rows; slices of C sorted rows
+// are padded to their longest row. Stored column-major within a slice.
+static int build_sell_host(fe_ctx ctx, fe_csr A, int C, int sigma) {
+  const int n = A->n;
+  std::vector<int> rp(n + 1), ci(A->nnz);
+  std::vector<double> v(A->nnz);
+  FE_HIP_TRY(hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+  if (A->nnz) {
+    FE_HIP_TRY(hipMemcpyAsync(ci.data(), A->colind, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    FE_HIP_TRY(hipMemcpyAsync(v.data(), A->vals64, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int w0 = 0; w0 < n; w0 += sigma) {
+    const int w1 = w0 + sigma < n ? w0 + sigma : n;
+    std::stable_sort(perm.begin() + w0, perm.begin() + w1,
+                     [&](int a, int b) { return (rp[a + 1] - rp[a]) > (rp[b + 1] - rp[b]); });
+  }
+  const int64_t slices = (n + C - 1) / C;
+  std::vector<int64_t> sptr(slices + 1, 0);
+  std::vector<int> slen(slices, 0);
+  for (int64_t s = 0; s < slices; ++s) {
+    int w = 0;
+    for (int r = 0; r < C; ++r) {
+      const int64_t sr = s * C + r;
+      if (sr < n) w = std::max(w, rp[perm[sr] + 1] - rp[perm[sr]]);
+    }
+    slen[s] = w;
+    sptr[s + 1] = sptr[s] + (int64_t)w * C;
+  }
+  const int64_t total = sptr[slices];
+  std::vector<int> scol(total > 0 ? total : 1, 0);
+  std::vector<double> sv(total > 0 ? total : 1, 0.0);
+  for (int64_t s = 0; s < slices; ++s)
+    for (int r = 0; r < C; ++r) {
+      const int64_t sr = s * C + r;
+      if (sr >= n) continue;
+      const int row = perm[sr];
+      const int len = rp[row + 1] - rp[row];
+      for (int t = 0; t < slen[s]; ++t) {
+        const int64_t e = sptr[s] + (int64_t)t * C + r;
+        if (t < len) {
+          scol[e] = ci[rp[row] + t];
+          sv[e] = v[rp[row] + t];
+        } else {
+          scol[e] = row;  // padding reads the row's own X entry with value 0
+          sv[e] = 0.0;
+        }
+      }
+    }
+  std::vector<float> sv32(sv.size());
+  for (size_t t = 0; t < sv.size(); ++t) sv32[t] = static_cast<float>(sv[t]);
+  FE_HIP_TRY(hipMalloc(&A->sell_ptr, sizeof(int64_t) * (slices + 1)));
+  FE_HIP_TRY(hipMalloc(&A->sell_len, sizeof(int) * slices));
+  FE_HIP_TRY(hipMalloc(&A->sell_perm, sizeof(int) * n));
+  FE_HIP_TRY(hipMalloc(&A->sell_col, sizeof(int) * scol.size()));
+  FE_HIP_TRY(hipMalloc(&A->sell_v64, sizeof(double) * sv.size()));
+  FE_HIP_TRY(hipMalloc(&A->sell_v32, sizeof(float) * sv.size()));
+  FE_HIP_TRY(hipMemcpyAsync(A->sell_ptr, sptr.data(), sizeof(int64_t) * (slices + 1), hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipMemcpyAsync(A->sell_len, slen.data(), sizeof(int) * slices, hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipMemcpyAsync(A->sell_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipMemcpyAsync(A->sell_col, scol.data(), sizeof(int) * scol.size(), hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipMemcpyAsync(A->sell_v64, sv.data(), sizeof(double) * sv.size(), hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipMemcpyAsync(A->sell_v32, sv32.data(), sizeof(float) * sv.size(), hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  A->sell_C = C;
+  A->sell_sigma = sigma;
+  A->sell_slices = slices;
+  A->sell_padded = total;
+  return 0;
+}
+
+}  // namespace fe
+
+using namespace fe;
+
+extern "C" int fe_spmm(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y,
+                       int64_t ldy, double alpha, double beta, const void* Z, int64_t ldz, double gamma) {
+  FE_ARG(ctx && A && X && Y, "fe_spmm: null argument");
+  FE_ARG(k >= 0 && ldx >= k && ldy >= k && (!Z || ldz >= k), "fe_spmm: bad leading dimension");
+  FE_ARG(X != Y, "fe_spmm: Y must not alias X");
+  FE_ARG(prec == FE_F64 || prec == FE_F32, "fe_spmm: bad precision");
+  if (k == 0 || A->n == 0) return 0;
+  if (prec == FE_F64)
+    return dispatch<double>(ctx, A, k, (const double*)X, ldx, (double*)Y, ldy, alpha, beta, (const double*)Z,
+                            ldz, gamma);
+  return dispatch<float>(ctx, A, k, (const float*)X, ldx, (float*)Y, ldy, alpha, beta, (const float*)Z, ldz,
+                         gamma);
+}
+
+extern "C" int fe_csr_build_sell(fe_ctx ctx, fe_csr A, int C, int sigma) {
+  FE_ARG(ctx && A, "fe_csr_build_sell: null argument");
+  FE_ARG(C > 0 && C <= 1024 && sigma >= 1, "fe_csr_build_sell: bad C or sigma");
+  FE_ARG(!A->sell_col, "fe_csr_build_sell: already built");
+  return build_sell_host(ctx, A, C, sigma);
+}

@@ -1,0 +1,185 @@
+/*
+ * fe_hip.h -- C ABI of the MI355X (gfx950) kernel library for the fasteig
+ * block-eigensolver hot path (libfe_hip.so).
+ *
+ * This is the drop-in boundary: the C++ host layer (fasteig::hip, in
+ * libfasteig_hip.so) and any FFI caller reach the device only through these
+ * entry points. No C++ or framework types cross it: plain pointers, 64-bit
+ * sizes, int status codes.
+ *
+ * Conventions
+ *  - Return value: 0 = OK; > 0 = numerical information (e.g. the failing
+ *    column + 1 of a Cholesky factorization); < 0 = bad argument (-2) or
+ *    device/runtime error (-3). fe_last_error() gives the message of the last
+ *    failure on the calling thread.
+ *  - Every call is ordered on the context's single HIP stream. Calls that
+ *    return host values synchronize that stream; the others are asynchronous.
+ *    A context is not re-entrant across host threads.
+ *  - Tall blocks (n rows, k columns) are ROW-MAJOR on the device: element
+ *    (i, j) lives at ptr[i * ld + j], ld >= k. Each nonzero of the sparse
+ *    product therefore gathers one contiguous k-vector.
+ *  - Small matrices (Gram, Cholesky factor, Ritz coefficients; at most a few
+ *    thousand on a side) are FP64, COLUMN-MAJOR on the device, like the
+ *    reference's Eigen MatrixXd.
+ *  - prec: FE_F64 blocks hold double, FE_F32 blocks hold float.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *  - fasteig::core::Executor           include/core/Executor.hpp:22-47
+ *  - fasteig::core::MultiVector        include/core/MultiVector.hpp:12-57
+ *  - fasteig::core::LinearOperator     include/core/LinearOperator.hpp:13-38
+ *  - fasteig::core::CsrMatrix          include/core/CsrMatrix.hpp:15-44
+ *  - CUDA backend roles                src/gpu/CudaContext.cpp:9-17,
+ *    src/gpu/CudaCsrMatrix.cpp:31-224, src/gpu/CudaMultiVector.cpp:310-475,
+ *    src/core/Orthogonalization.cpp:185-246
+ */
+#ifndef FE_HIP_H
+#define FE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fe_ctx_s* fe_ctx;
+typedef struct fe_csr_s* fe_csr;
+
+enum fe_prec { FE_F64 = 0, FE_F32 = 1 };
+enum fe_gram_mode { FE_GRAM_NATIVE = 0, FE_GRAM_BF16X3 = 1 };
+enum fe_format { FE_FMT_CSR = 0, FE_FMT_SELL = 1 };
+
+/* ---- errors / version ------------------------------------------------- */
+const char* fe_last_error(void);
+const char* fe_version(void);
+
+/* ---- context (replaces CudaContext + CudaExecutor, src/gpu/CudaContext.cpp:9-17,
+ *      src/gpu/CudaExecutor.cpp:18-43) ------------------------------------ */
+int fe_ctx_create(int device, fe_ctx* out);
+int fe_ctx_destroy(fe_ctx ctx);
+/* The hipStream_t the context orders its work on. */
+void* fe_ctx_stream(fe_ctx ctx);
+/* Executor::synchronize, include/core/Executor.hpp:43 */
+int fe_ctx_sync(fe_ctx ctx);
+/* Device properties: compute units and HBM bytes. */
+int fe_ctx_info(fe_ctx ctx, int* cus, size_t* hbm_bytes, char* name, int name_len);
+
+/* ---- memory (64-bit sizes; replaces per-object cudaMalloc of the CUDA backend) */
+int fe_alloc(fe_ctx ctx, size_t bytes, void** ptr);
+int fe_free(fe_ctx ctx, void* ptr);
+int fe_memcpy_h2d(fe_ctx ctx, void* dst, const void* src, size_t bytes);
+int fe_memcpy_d2h(fe_ctx ctx, void* dst, const void* src, size_t bytes);
+int fe_memcpy_d2d(fe_ctx ctx, void* dst, const void* src, size_t bytes);
+/* Zero a rows x cols region (elem_bytes each) with leading dimension ld. */
+int fe_zero2d(fe_ctx ctx, void* ptr, int64_t rows, int64_t cols, int64_t ld, int elem_bytes);
+
+/* ---- sparse matrices (replaces CudaCsrMatrix upload, src/gpu/CudaCsrMatrix.cpp:31-42;
+ *      core::CsrMatrixData, include/core/CsrMatrix.hpp:15-25) ----------------
+ * Uploads a full (both triangles) CSR matrix with int32 indices from host
+ * memory and keeps FP64 and FP32 copies of the values. */
+int fe_csr_create(fe_ctx ctx, int n, int64_t nnz, const int* rowptr, const int* colind,
+                  const double* vals, fe_csr* out);
+int fe_csr_destroy(fe_csr A);
+int fe_csr_info(fe_csr A, int* n, int64_t* nnz);
+/* A <- D A D with D = diag(d) (d on the device, FP64, n entries). Used for the
+ * congruence M^{-1/2} K M^{-1/2} of a generalized problem with diagonal M. */
+int fe_csr_scale_sym(fe_ctx ctx, fe_csr A, const double* d);
+/* Builds the SELL-C-sigma copy of A (slices of C rows, rows sorted by length
+ * within windows of sigma rows); later products may use it (fe_csr_use). */
+int fe_csr_build_sell(fe_ctx ctx, fe_csr A, int C, int sigma);
+int fe_csr_use(fe_csr A, int format);
+
+/* ---- fused block SpMM (replaces LinearOperator::apply_block,
+ *      include/core/LinearOperator.hpp:32-37; CUDA: k x cusparseSpMV,
+ *      src/gpu/CudaCsrMatrix.cpp:196-224; Chebyshev step
+ *      src/ChebyshevSolver.cpp:243-256) ------------------------------------
+ *   Y = alpha * (A X) + beta * X + gamma * Z
+ * X, Y, Z: n x k row-major blocks of precision prec. Z may alias Y (then it
+ * is read before Y is written); Y must not alias X. Z may be NULL (gamma
+ * ignored). The Chebyshev step Y_j = 2c A Y_{j-1} + 2e Y_{j-1} - Y_{j-2} is
+ * alpha = 2c, beta = 2e, gamma = -1, Z = Y_{j-2}. */
+int fe_spmm(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y,
+            int64_t ldy, double alpha, double beta, const void* Z, int64_t ldz, double gamma);
+
+/* ---- block Gram products on MFMA (replaces MultiVector::transpose_multiply,
+ *      include/core/MultiVector.hpp:51-53; CUDA cublasDgemm TN,
+ *      src/gpu/CudaMultiVector.cpp:361-407) ---------------------------------
+ *   G[m x p] = X[:, 0:m]^T * diag(w) * Y[:, 0:p]     (w may be NULL)
+ * G is FP64 column-major with leading dimension ldg, on the device. mode
+ * FE_GRAM_NATIVE uses FP64 MFMA for FE_F64 and FP32 MFMA (FP32 accumulate per
+ * row chunk, FP64 across chunks) for FE_F32; FE_GRAM_BF16X3 (FE_F32 only)
+ * splits each FP32 operand into a high and a low bfloat16 part and sums the
+ * three products hi*hi + hi*lo + lo*hi on BF16 MFMA with FP32 accumulation
+ * (about 16 significant bits per product; a throughput option for the
+ * intermediate filter cycles). If X == Y and m == p only the upper triangle of tiles is computed
+ * and mirrored. The chunk reduction order is fixed, so results are bitwise
+ * reproducible. */
+int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+            const void* Y, int64_t ldy, const double* w, double* G, int64_t ldg);
+
+/* ---- tall-skinny times small (replaces right_multiply_active_columns,
+ *      include/core/MultiVector.hpp:47-50; CUDA cublasDgemm NN,
+ *      src/gpu/CudaMultiVector.cpp:310-359) ---------------------------------
+ *   Y[:, 0:p] = alpha * X[:, 0:m] * C + beta * Y[:, 0:p]
+ * C is FP64 column-major m x p (ldc) on the device; it is rounded to the
+ * block precision inside the kernel. Y must not alias X. */
+int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+               const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta);
+
+/* ---- small dense FP64 on the device -----------------------------------------
+ * Cholesky G = L L^T of the leading k x k block (lower triangle written, upper
+ * left untouched); returns 0 or j+1 for the first non-positive pivot (the
+ * reference's LLT failure, src/core/Orthogonalization.cpp:49-57,216-225).
+ * Synchronizes to read the status. */
+int fe_potrf(fe_ctx ctx, int k, double* G, int64_t ldg);
+/* Rinv = (L^T)^{-1} (upper triangular k x k, column-major, ld k) from the lower
+ * Cholesky factor L; the triangular solve X R^{-1} then runs as fe_gemm_nn. */
+int fe_chol_rinv(fe_ctx ctx, int k, const double* L, int64_t ldl, double* Rinv, int64_t ldr);
+/* Symmetric eigensolver for the projected Rayleigh-Ritz matrix (replaces
+ * Eigen::SelfAdjointEigenSolver, src/core/RitzExtraction.cpp:103-110, and the
+ * CUDA dsyevd, src/gpu/CudaLanczosWorkspace.cpp:155):
+ *   A (m x m, lda, only the lower triangle read after symmetrization (A+A^T)/2)
+ *   -> W (m eigenvalues, ascending), V (m x m column-major, ldv, orthonormal).
+ * Householder tridiagonalization, bisection, inverse iteration with
+ * reorthogonalization inside eigenvalue clusters, back-transformation; all on
+ * the device. A is overwritten. Returns 0 or a positive code on failure. */
+int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, double* V, int64_t ldv);
+
+/* ---- column-wise block operations (replaces add_scaled_columns, column_norms,
+ *      copies; include/core/MultiVector.hpp:23-56, src/gpu/CudaMultiVector.cpp:409-475) */
+/* Y[:, j] = a * X[:, j] + b * Y[:, j] + s[j] * X[:, j] (s may be NULL; device FP64). */
+int fe_axpby_cols(fe_ctx ctx, int prec, int64_t n, int k, const void* X, int64_t ldx, void* Y,
+                  int64_t ldy, double a, double b, const double* s);
+/* out[j] = ||X[:, j]||_2 (FP64 accumulation, device output). */
+int fe_colnorms(fe_ctx ctx, int prec, int64_t n, int k, const void* X, int64_t ldx, double* out);
+/* Ritz residuals, fused (replaces src/core/RitzExtraction.cpp:125-139):
+ *   R[:, j] = AY[:, j] - theta[j] * Y[:, j]   (R may be NULL: norms only)
+ *   rnorm[j] = ||R[:, j]||, ynorm[j] = ||Y[:, j]||. theta, rnorm, ynorm on the device. */
+int fe_ritz_residuals(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, int64_t ldy,
+                      const void* AY, int64_t lday, const double* theta, void* R, int64_t ldr,
+                      double* rnorm, double* ynorm);
+/* Copy a rows x cols block between (possibly different) precisions. */
+int fe_copy2d(fe_ctx ctx, int64_t rows, int cols, const void* src, int src_prec, int64_t lds,
+              void* dst, int dst_prec, int64_t ldd);
+/* Host column-major FP64 (n x k, ld n) <-> device row-major block. */
+int fe_upload_colmajor(fe_ctx ctx, int64_t n, int k, const double* host, void* dst, int prec,
+                       int64_t ldd);
+int fe_download_colmajor(fe_ctx ctx, int64_t n, int k, const void* src, int prec, int64_t lds,
+                         double* host);
+/* Small FP64 helpers on the device (column-major):
+ *   fe_sym_small: A <- (A + A^T)/2 (m x m)
+ *   fe_gather_cols: C[:, j] = V[:, idx[j]] (idx host int array of length p)
+ *   fe_zero_rows_small: C[0:r, :] = 0 */
+int fe_sym_small(fe_ctx ctx, int m, double* A, int64_t lda);
+int fe_gather_cols(fe_ctx ctx, int m, int p, const double* V, int64_t ldv, const int* idx_host,
+                   double* C, int64_t ldc);
+int fe_zero_rows_small(fe_ctx ctx, int r, int p, double* C, int64_t ldc);
+
+/* ---- timing helpers for benchmarks (HIP events on the context stream) ---- */
+int fe_event_record(fe_ctx ctx, int slot);
+int fe_event_elapsed_ms(fe_ctx ctx, int slot0, int slot1, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FE_HIP_H */

@@ -1,0 +1,200 @@
+"""ctypes front-end of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the fasteig_amd package. Dense blocks are
+numpy float64 arrays of shape (n, k); they are passed column-major (Fortran
+order) because the oracle stores blocks like Eigen's MatrixXd.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_SO):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_random_block.argtypes = [C.c_int, C.c_int, C.c_uint, _dp]
+        L.orc_spmm.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, C.c_double, C.c_double]
+        L.orc_syev.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_cholqr.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_householder_qr.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_jacobi_svd.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_orthonormalize_block.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.orc_lanczos_bounds.argtypes = [C.c_int, _ip, _ip, _dp, C.c_double, C.c_uint,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_cheb_scalars.argtypes = [C.c_int, C.c_double, C.c_double,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_cheb_filter.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_double]
+        solver_tail = [C.c_double, C.c_char_p, C.c_uint, _dp, C.c_void_p, _dp, C.POINTER(C.c_long)]
+        L.orc_chebyshev.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int] + solver_tail
+        L.orc_lobpcg.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
+        L.orc_davidson.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
+        L.orc_residual_norms.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _F(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _csr(A):
+    """A is any object with indptr/indices/data (scipy CSR or fasteig_amd.generators.Csr)."""
+    return (np.ascontiguousarray(A.indptr, dtype=np.int32),
+            np.ascontiguousarray(A.indices, dtype=np.int32),
+            _F(A.data))
+
+
+def _check(rc):
+    if rc < 0:
+        msg = lib().orc_last_error().decode()
+        if rc == -2:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+    return rc
+
+
+def set_threads(t: int) -> None:
+    lib().orc_set_threads(int(t))
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())
+
+
+def random_block(n, k, seed=1):
+    out = np.zeros((n, k), order="F")
+    lib().orc_random_block(n, k, seed, out)
+    return out
+
+
+def spmm(A, X, alpha=1.0, beta=0.0, Y=None):
+    rp, ci, v = _csr(A)
+    X = _F(X)
+    n, k = X.shape
+    Y = np.zeros((n, k), order="F") if Y is None else _F(Y).copy(order="F")
+    lib().orc_spmm(n, rp, ci, v, k, X, Y, alpha, beta)
+    return Y
+
+
+def syev(A):
+    A = _F(A)
+    m = A.shape[0]
+    w = np.zeros(m, order="F")
+    V = np.zeros((m, m), order="F")
+    if lib().orc_syev(m, A, w, V) != 0:
+        raise RuntimeError("orc_syev failed")
+    return w, V
+
+
+def cholqr(X):
+    X = _F(X).copy(order="F")
+    n, k = X.shape
+    R = np.zeros((k, k), order="F")
+    info = lib().orc_cholqr(n, k, X, R)
+    return info, X, R
+
+
+def householder_qr(X):
+    X = _F(X).copy(order="F")
+    n, k = X.shape
+    R = np.zeros((k, k), order="F")
+    lib().orc_householder_qr(n, k, X, R)
+    return X, R
+
+
+def jacobi_svd(R):
+    R = _F(R)
+    r, c = R.shape
+    s = np.zeros(c, order="F")
+    U = np.zeros((r, c), order="F")
+    lib().orc_jacobi_svd(r, c, R, s, U)
+    return s, U
+
+
+def orthonormalize_block(basis, active, src):
+    basis = _F(basis)
+    src = _F(src)
+    n, b = src.shape
+    out = np.zeros((n, b), order="F")
+    rank = _check(lib().orc_orthonormalize_block(n, active, basis, b, src, out))
+    return out[:, :rank]
+
+
+def lanczos_bounds(A, sign=1.0, seed=1):
+    rp, ci, v = _csr(A)
+    lo, hi = C.c_double(), C.c_double()
+    _check(lib().orc_lanczos_bounds(len(rp) - 1, rp, ci, v, sign, seed, C.byref(lo), C.byref(hi)))
+    return lo.value, hi.value
+
+
+def cheb_scalars(degree, lower, upper):
+    c, d = C.c_double(), C.c_double()
+    lib().orc_cheb_scalars(degree, lower, upper, C.byref(c), C.byref(d))
+    return c.value, d.value
+
+
+def cheb_filter(A, X, degree, lower, upper):
+    rp, ci, v = _csr(A)
+    X = _F(X).copy(order="F")
+    n, k = X.shape
+    _check(lib().orc_cheb_filter(n, rp, ci, v, k, X, degree, lower, upper))
+    return X
+
+
+def _solve(fn, A, nev, extra, maxiter, tol, which, seed, want_vectors):
+    rp, ci, v = _csr(A)
+    n = len(rp) - 1
+    ev = np.zeros(nev, order="F")
+    res = np.zeros(nev, order="F")
+    vec = np.zeros((n, nev), order="F") if want_vectors else None
+    it = C.c_long()
+    vec_ptr = vec.ctypes.data_as(C.c_void_p) if want_vectors else None
+    status = _check(fn(n, rp, ci, v, nev, *extra, maxiter, tol, which.lower().encode(), seed,
+                       ev, vec_ptr, res, C.byref(it)))
+    return {"eigenvalues": ev, "residuals": res, "eigenvectors": vec, "iterations": it.value,
+            "status": status}
+
+
+def chebyshev(A, nev, block=0, degree=10, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True):
+    return _solve(lib().orc_chebyshev, A, nev, (block, degree), maxiter, tol, which, seed, want_vectors)
+
+
+def lobpcg(A, nev, block=0, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True):
+    return _solve(lib().orc_lobpcg, A, nev, (block,), maxiter, tol, which, seed, want_vectors)
+
+
+def davidson(A, nev, maxblks=10, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True):
+    return _solve(lib().orc_davidson, A, nev, (maxblks,), maxiter, tol, which, seed, want_vectors)
+
+
+def residual_norms(A, evals, X):
+    rp, ci, v = _csr(A)
+    X = _F(X)
+    n, k = X.shape
+    out = np.zeros(k, order="F")
+    lib().orc_residual_norms(n, rp, ci, v, k, _F(evals), X, out)
+    return out

@@ -1,0 +1,285 @@
+"""Kernel-level parity of the HIP library (C ABI) against the CPU oracle.
+
+Every test calls libfe_hip.so through its C ABI (fasteig_amd._fe) on seeded
+inputs and compares with the oracle (oracle/oracle.cpp) or with numpy closed
+forms. Tolerances: FP64 kernels 1e-13 relative (different summation order
+only); FP32 kernels 1e-5 relative to the operand scale; BF16x3 Gram 2e-4.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from fasteig_amd import _fe
+from fasteig_amd import generators as G
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+L = _fe.lib
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def mats():
+    rng = np.random.default_rng(7)
+    out = {
+        "lap2d": G.laplacian_2d(37, 29),
+        "cond3d": G.conductivity_3d(11, seed=2),
+        "h27": G.hamiltonian_27pt(9, seed=4, wells=5),
+        "powlaw": G.power_law(3000, seed=5, dmax=300),
+    }
+    # ragged: empty rows and one dense row
+    R = sp.random(500, 500, density=0.01, random_state=3, format="csr")
+    R = R + R.T
+    R = sp.lil_matrix(R)
+    R[17, :] = 0.0
+    R[:, 17] = 0.0
+    R[3, :] = rng.standard_normal(500)
+    R[:, 3] = R[3, :].T.toarray()
+    out["ragged"] = G._finish(sp.csr_matrix(R))
+    return out
+
+
+@pytest.mark.parametrize("name", ["lap2d", "cond3d", "h27", "powlaw", "ragged"])
+@pytest.mark.parametrize("k", [1, 3, 8, 12, 16, 48, 96, 256, 512])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_spmm_fused(dev, mats, name, k, prec):
+    A = mats[name]
+    n = A.shape[0]
+    rng = np.random.default_rng(k + 13 * prec)
+    X = rng.uniform(-1, 1, (n, k))
+    Z = rng.uniform(-1, 1, (n, k))
+    alpha, beta, gamma = 1.7, -0.3, -1.0
+    ref = orc.spmm(A, X, alpha, 0.0) + beta * X + gamma * Z
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    Xd = dev.put(X, dt)
+    Yd = dev.put(Z, dt)  # Z aliases Y: read before written
+    C = dev.csr(A)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Yd.ptr, k, alpha, beta, Yd.ptr, k, gamma))
+    got = Yd.download().astype(np.float64)
+    scale = np.max(np.abs(A).sum(axis=1)) + 2.0
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    assert np.max(np.abs(got - ref)) <= tol * scale
+
+
+@pytest.mark.parametrize("k", [4, 16, 48, 64])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_spmm_sell_matches_csr(dev, mats, k, prec):
+    A = mats["powlaw"]
+    n = A.shape[0]
+    X = np.random.default_rng(1).uniform(-1, 1, (n, k))
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    C = dev.csr(A)
+    Xd = dev.put(X, dt)
+    Y1 = dev.buffer((n, k), dt)
+    Y2 = dev.buffer((n, k), dt)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y1.ptr, k, 1.0, 0.0, None, 0, 0.0))
+    C.build_sell(32, 256)
+    C.use(_fe.FMT_SELL)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y2.ptr, k, 1.0, 0.0, None, 0, 0.0))
+    ref = orc.spmm(A, X)
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    scale = np.max(np.abs(A).sum(axis=1))
+    assert np.max(np.abs(Y1.download() - ref)) <= tol * scale
+    assert np.max(np.abs(Y2.download() - ref)) <= tol * scale
+
+
+def test_spmm_strided_subblock(dev, mats):
+    """Column sub-ranges with ld > k (unaligned) take the scalar path."""
+    A = mats["lap2d"]
+    n = A.shape[0]
+    X = np.random.default_rng(2).uniform(-1, 1, (n, 21))
+    Xd = dev.put(X)
+    Yd = dev.put(np.zeros((n, 21)))
+    C = dev.csr(A)
+    off = 8 * 3  # bytes: start at column 3
+    _fe.check(L().fe_spmm(dev.ctx, C.h, _fe.F64, 7, Xd.ptr + off, 21, Yd.ptr + off, 21, 1.0, 0.0, None, 0, 0.0))
+    got = Yd.download()
+    ref = orc.spmm(A, X[:, 3:10])
+    assert np.max(np.abs(got[:, 3:10] - ref)) < 1e-13
+    assert np.all(got[:, :3] == 0) and np.all(got[:, 10:] == 0)
+
+
+@pytest.mark.parametrize("n,m,p", [(1000, 16, 16), (5000, 48, 40), (20000, 96, 96), (3000, 200, 70),
+                                   (777, 5, 3), (100000, 256, 256), (40000, 512, 512)])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_gram(dev, n, m, p, prec):
+    rng = np.random.default_rng(n + m)
+    X = rng.uniform(-1, 1, (n, m))
+    Y = X[:, :p] * 0.5 + rng.uniform(-1, 1, (n, p)) * 0.5
+    w = rng.uniform(0.5, 1.5, n)
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    Xd, Yd, wd = dev.put(X, dt), dev.put(Y, dt), dev.put(w)
+    Gd = dev.buffer((m, p), np.float64, order="F")
+    _fe.check(L().fe_gram(dev.ctx, 0, prec, n, m, p, Xd.ptr, m, Yd.ptr, p, wd.ptr, Gd.ptr, m))
+    Xr = X.astype(dt).astype(np.float64)
+    Yr = Y.astype(dt).astype(np.float64)
+    ref = Xr.T @ (w[:, None] * Yr)
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    assert rel(Gd.download(), ref) <= tol * np.sqrt(n) / 10 + tol
+    # symmetric path (X == Y, no weight): exactly symmetric output
+    Sd = dev.buffer((m, m), np.float64, order="F")
+    _fe.check(L().fe_gram(dev.ctx, 0, prec, n, m, m, Xd.ptr, m, Xd.ptr, m, None, Sd.ptr, m))
+    S = Sd.download()
+    assert np.array_equal(S, S.T)
+    assert rel(S, Xr.T @ Xr) <= tol * np.sqrt(n) / 10 + tol
+
+
+def test_gram_bf16x3(dev):
+    rng = np.random.default_rng(3)
+    n, m = 50000, 128
+    X = rng.uniform(-1, 1, (n, m)).astype(np.float32)
+    Xd = dev.put(X, np.float32)
+    Gd = dev.buffer((m, m), np.float64, order="F")
+    _fe.check(L().fe_gram(dev.ctx, _fe.GRAM_BF16X3, _fe.F32, n, m, m, Xd.ptr, m, Xd.ptr, m, None, Gd.ptr, m))
+    ref = X.astype(np.float64).T @ X.astype(np.float64)
+    # hi*hi + hi*lo + lo*hi keeps ~16 significant bits per product
+    assert rel(Gd.download(), ref) < 2e-4
+
+
+def test_gram_reproducible(dev):
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((30000, 64))
+    Xd = dev.put(X)
+    outs = []
+    for _ in range(2):
+        Gd = dev.buffer((64, 64), np.float64, order="F")
+        _fe.check(L().fe_gram(dev.ctx, 0, _fe.F64, 30000, 64, 64, Xd.ptr, 64, Xd.ptr, 64, None, Gd.ptr, 64))
+        outs.append(Gd.download())
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("n,m,p", [(1000, 16, 16), (5000, 48, 10), (20000, 96, 96), (3333, 200, 70),
+                                   (100000, 256, 256), (7, 3, 2)])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_gemm_nn(dev, n, m, p, prec):
+    rng = np.random.default_rng(n + p)
+    X = rng.uniform(-1, 1, (n, m))
+    Cm = rng.uniform(-1, 1, (m, p))
+    Y0 = rng.uniform(-1, 1, (n, p))
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    Xd, Yd = dev.put(X, dt), dev.put(Y0, dt)
+    Cd = dev.put(Cm, np.float64, order="F")
+    _fe.check(L().fe_gemm_nn(dev.ctx, prec, n, m, p, Xd.ptr, m, Cd.ptr, m, Yd.ptr, p, 0.7, -0.5))
+    Xr = X.astype(dt).astype(np.float64)
+    ref = 0.7 * Xr @ Cm.astype(dt).astype(np.float64) - 0.5 * Y0.astype(dt).astype(np.float64)
+    tol = 1e-13 if prec == _fe.F64 else 3e-6
+    assert np.max(np.abs(Yd.download() - ref)) <= tol * m
+
+
+@pytest.mark.parametrize("k", [1, 7, 64, 130, 512])
+def test_potrf_rinv_cholqr(dev, k):
+    rng = np.random.default_rng(k)
+    n = max(4 * k, 50)
+    X = rng.uniform(-1, 1, (n, k))
+    Xd = dev.put(X)
+    Gd = dev.buffer((k, k), np.float64, order="F")
+    _fe.check(L().fe_gram(dev.ctx, 0, _fe.F64, n, k, k, Xd.ptr, k, Xd.ptr, k, None, Gd.ptr, k))
+    assert _fe.check(L().fe_potrf(dev.ctx, k, Gd.ptr, k)) == 0
+    Rd = dev.buffer((k, k), np.float64, order="F")
+    _fe.check(L().fe_chol_rinv(dev.ctx, k, Gd.ptr, k, Rd.ptr, k))
+    Qd = dev.buffer((n, k))
+    _fe.check(L().fe_gemm_nn(dev.ctx, _fe.F64, n, k, k, Xd.ptr, k, Rd.ptr, k, Qd.ptr, k, 1.0, 0.0))
+    Q = Qd.download()
+    Rinv = Rd.download()
+    assert np.allclose(np.triu(Rinv), Rinv)
+    assert np.max(np.abs(Q.T @ Q - np.eye(k))) < 1e-10
+    # Q R = X with R = inv(Rinv); oracle CholQR gives the same factors
+    info, Qo, Ro = orc.cholqr(X)
+    assert info == 0
+    assert np.max(np.abs(Q - Qo)) < 1e-9
+
+
+def test_potrf_reports_failure(dev):
+    k = 6
+    Gm = np.eye(k)
+    Gm[3, 3] = -1.0
+    Gd = dev.put(Gm, np.float64, order="F")
+    assert L().fe_potrf(dev.ctx, k, Gd.ptr, k) == 4
+
+
+def _syev(dev, A):
+    m = A.shape[0]
+    Ad = dev.put(A, np.float64, order="F")
+    Wd = dev.buffer((m,), np.float64)
+    Vd = dev.buffer((m, m), np.float64, order="F")
+    info = _fe.check(L().fe_syev(dev.ctx, m, Ad.ptr, m, Wd.ptr, Vd.ptr, m))
+    return info, Wd.download(), Vd.download()
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 17, 64, 100, 257, 512])
+def test_syev_random(dev, m):
+    rng = np.random.default_rng(m)
+    A = rng.standard_normal((m, m))
+    A = A + A.T
+    info, w, V = _syev(dev, A)
+    assert info == 0
+    wr = np.linalg.eigvalsh(A)
+    scale = max(np.max(np.abs(wr)), 1.0)
+    assert np.max(np.abs(w - wr)) <= 1e-13 * scale * max(1, np.sqrt(m))
+    assert np.max(np.abs(V.T @ V - np.eye(m))) <= 1e-11
+    assert np.max(np.abs(A @ V - V * w)) <= 1e-12 * scale * max(1, np.sqrt(m))
+    wo, _ = orc.syev(A)
+    assert np.max(np.abs(w - wo)) <= 1e-13 * scale * max(1, np.sqrt(m))
+
+
+def test_syev_degenerate_and_diagonal(dev):
+    # exact pairs (2D Laplacian symmetry), a diagonal matrix and a Wilkinson matrix
+    L2 = G.laplacian_2d(6).toarray()
+    for A in (L2, np.diag(np.arange(10.0)[::-1]), np.diag(np.abs(np.arange(-10, 11.0))) +
+              np.diag(np.ones(20), 1) + np.diag(np.ones(20), -1), np.zeros((4, 4)), np.eye(9) * 3):
+        info, w, V = _syev(dev, A)
+        assert info == 0
+        m = A.shape[0]
+        scale = max(np.max(np.abs(A)), 1.0)
+        assert np.max(np.abs(w - np.linalg.eigvalsh(A))) <= 1e-13 * scale * 4
+        assert np.max(np.abs(V.T @ V - np.eye(m))) <= 1e-11
+        assert np.max(np.abs(A @ V - V * w)) <= 1e-12 * scale * 4
+
+
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_colnorms_and_ritz_residuals(dev, prec):
+    rng = np.random.default_rng(11)
+    n, k = 12345, 37
+    Y = rng.uniform(-1, 1, (n, k))
+    AY = rng.uniform(-1, 1, (n, k))
+    th = rng.uniform(-2, 2, k)
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    Yd, AYd, thd = dev.put(Y, dt), dev.put(AY, dt), dev.put(th)
+    Rd = dev.buffer((n, k), dt)
+    rn, yn = dev.buffer((k,)), dev.buffer((k,))
+    _fe.check(L().fe_ritz_residuals(dev.ctx, prec, n, k, Yd.ptr, k, AYd.ptr, k, thd.ptr, Rd.ptr, k, rn.ptr, yn.ptr))
+    Yr, AYr = Y.astype(dt).astype(np.float64), AY.astype(dt).astype(np.float64)
+    R = AYr - Yr * th
+    tol = 1e-13 if prec == _fe.F64 else 1e-6
+    assert rel(rn.download(), np.linalg.norm(R, axis=0)) < tol
+    assert rel(yn.download(), np.linalg.norm(Yr, axis=0)) < tol
+    assert rel(Rd.download().astype(np.float64), R) < (1e-15 if prec == _fe.F64 else 1e-6)
+    cn = dev.buffer((k,))
+    _fe.check(L().fe_colnorms(dev.ctx, prec, n, k, Yd.ptr, k, cn.ptr))
+    assert rel(cn.download(), np.linalg.norm(Yr, axis=0)) < tol
+
+
+def test_axpby_copy_upload(dev):
+    rng = np.random.default_rng(12)
+    n, k = 1000, 9
+    X = rng.uniform(-1, 1, (n, k))
+    Y = rng.uniform(-1, 1, (n, k))
+    s = rng.uniform(-1, 1, k)
+    Xd, Yd, sd = dev.put(X), dev.put(Y), dev.put(s)
+    _fe.check(L().fe_axpby_cols(dev.ctx, _fe.F64, n, k, Xd.ptr, k, Yd.ptr, k, 0.5, 2.0, sd.ptr))
+    assert np.max(np.abs(Yd.download() - (2.0 * Y + (0.5 + s) * X))) < 1e-14
+    # column-major host <-> row-major device in both precisions
+    F = np.asfortranarray(X)
+    for prec, dt in ((_fe.F64, np.float64), (_fe.F32, np.float32)):
+        B = dev.buffer((n, k), dt)
+        _fe.check(L().fe_upload_colmajor(dev.ctx, n, k, F.ctypes.data, B.ptr, prec, k))
+        assert np.array_equal(B.download(), X.astype(dt))
+        back = np.zeros((n, k), order="F")
+        _fe.check(L().fe_download_colmajor(dev.ctx, n, k, B.ptr, prec, k, back.ctypes.data))
+        assert np.array_equal(back, X.astype(dt).astype(np.float64))
+    B32 = dev.buffer((n, k), np.float32)
+    _fe.check(L().fe_copy2d(dev.ctx, n, k, Xd.ptr, _fe.F64, k, B32.ptr, _fe.F32, k))
+    assert np.array_equal(B32.download(), X.astype(np.float32))
