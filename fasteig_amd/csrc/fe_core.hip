@@ -117,6 +117,17 @@ __global__ void k_reduce_cols(int chunks, int k, const double* __restrict__ part
   out[j] = sqrt(s);
 }
 
+template <typename T>
+__global__ void k_scale_rows(int64_t n, int k, T* X, int64_t ldx, const double* __restrict__ d) {
+  const int64_t total = n * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / k;
+    const int j = static_cast<int>(t - i * k);
+    X[i * ldx + j] = static_cast<T>(static_cast<double>(X[i * ldx + j]) * d[i]);
+  }
+}
+
 template <typename S, typename D>
 __global__ void k_copy2d(int64_t rows, int cols, const S* __restrict__ src, int64_t lds, D* dst,
                          int64_t ldd) {
@@ -454,6 +465,17 @@ int fe_ritz_residuals(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, int
                       double* rnorm, double* ynorm) {
   FE_ARG(ctx && Y && AY && theta && rnorm && ynorm, "fe_ritz_residuals: null argument");
   return colsq_launch(ctx, prec, n, k, Y, ldy, AY, lday, theta, R, ldr, rnorm, ynorm);
+}
+
+int fe_scale_rows(fe_ctx ctx, int prec, int64_t n, int k, void* X, int64_t ldx, const double* d) {
+  FE_ARG(ctx && X && d, "fe_scale_rows: null argument");
+  if (n * k == 0) return 0;
+  if (prec == FE_F64)
+    hipLaunchKernelGGL(k_scale_rows<double>, dim3(grid_for(n * k)), dim3(256), 0, ctx->stream, n, k, (double*)X, ldx, d);
+  else
+    hipLaunchKernelGGL(k_scale_rows<float>, dim3(grid_for(n * k)), dim3(256), 0, ctx->stream, n, k, (float*)X, ldx, d);
+  FE_CHECK_LAUNCH();
+  return 0;
 }
 
 int fe_copy2d(fe_ctx ctx, int64_t rows, int cols, const void* src, int sp, int64_t lds, void* dst,

@@ -1,0 +1,291 @@
+// Device-resident orthogonalization, Rayleigh-Ritz extraction and the Lanczos
+// bound estimate (see core_ops.hpp for the reference anchors).
+#include "core_ops.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+
+namespace fasteig {
+namespace hip {
+
+Workspace::Workspace(HipExecutor& e, int max_m, int gram) : ex(e), gram_mode(gram) { ensure(std::max(max_m, 1)); }
+
+void Workspace::ensure(int m) {
+  if (m <= cap) return;
+  cap = std::max(m, cap + cap / 2);
+  G = Small(ex, cap, cap);
+  R = Small(ex, cap, cap);
+  V = Small(ex, cap, cap);
+  C = Small(ex, cap, cap);
+  W = Small(ex, cap, 1);
+  theta = Small(ex, cap, 1);
+  rn = Small(ex, cap, 1);
+  yn = Small(ex, cap, 1);
+}
+
+static void scale_columns(HipExecutor& ex, const BlockRef& X, const double* s_dev) {
+  check(fe_axpby_cols(ex.ctx(), X.prec, X.rows, X.cols, X.ptr, X.ld, X.ptr, X.ld, 0.0, 0.0, s_dev),
+        "fe_axpby_cols(scale)");
+}
+
+int cholesky_qr(Workspace& ws, Block& X, Block& tmp) {
+  HipExecutor& ex = ws.ex;
+  const int k = X.ncols();
+  if (k <= 0 || X.rows() < k) throw std::invalid_argument("thin_qr_in_place: row count must be >= column count.");
+  ws.ensure(k);
+  gram(ex, ws.gram_mode, X.ref(), X.ref(), ws.G.data(), ws.cap);
+  const int info = fe_potrf(ex.ctx(), k, ws.G.data(), ws.cap);
+  check(info, "fe_potrf");
+  if (info != 0) return -1;
+  check(fe_chol_rinv(ex.ctx(), k, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
+  if (tmp.ncols() == k && tmp.rows() == X.rows() && tmp.prec() == X.prec()) {
+    gemm(ex, X.ref(), ws.R.data(), ws.cap, tmp.ref(), 1.0, 0.0);
+    X.swap(tmp);
+  } else {
+    const BlockRef t = tmp.cols(0, k);
+    if (tmp.ncols() < k || tmp.rows() != X.rows() || tmp.prec() != X.prec())
+      throw std::invalid_argument("cholesky_qr: scratch block too small");
+    gemm(ex, X.ref(), ws.R.data(), ws.cap, t, 1.0, 0.0);
+    copy(ex, t, X.ref());
+  }
+  return 0;
+}
+
+void orthonormal_qr(Workspace& ws, Block& X, Block& tmp) {
+  HipExecutor& ex = ws.ex;
+  const int k = X.ncols();
+  if (cholesky_qr(ws, X, tmp) != 0) {
+    // shifted Cholesky QR: G + s I = R^T R with s = 11 (n k + k (k+1)) u ||X||_2^2
+    ws.ensure(k);
+    gram(ex, ws.gram_mode, X.ref(), X.ref(), ws.G.data(), ws.cap);
+    MatrixXd Gh = ws.G.to_host();
+    double tr = 0.0;
+    for (int i = 0; i < k; ++i) tr += Gh(i, i);
+    const double u = X.prec() == FE_F64 ? std::numeric_limits<double>::epsilon() * 0.5
+                                        : std::numeric_limits<float>::epsilon() * 0.5;
+    const double s = 11.0 * (static_cast<double>(X.rows()) * k + static_cast<double>(k) * (k + 1)) * u * tr;
+    for (int i = 0; i < k; ++i) Gh(i, i) += s;
+    ws.G.from_host(Gh);
+    const int info = fe_potrf(ex.ctx(), k, ws.G.data(), ws.cap);
+    check(info, "fe_potrf(shifted)");
+    if (info != 0) throw std::runtime_error("orthonormal_qr: shifted Cholesky QR failed (non-finite block?)");
+    check(fe_chol_rinv(ex.ctx(), k, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
+    const BlockRef t = tmp.cols(0, k);
+    gemm(ex, X.ref(), ws.R.data(), ws.cap, t, 1.0, 0.0);
+    copy(ex, t, X.ref());
+    if (cholesky_qr(ws, X, tmp) != 0) throw std::runtime_error("orthonormal_qr: Cholesky QR failed after shift");
+  }
+  if (cholesky_qr(ws, X, tmp) != 0) throw std::runtime_error("orthonormal_qr: second Cholesky QR pass failed");
+}
+
+void block_gram_schmidt(Workspace& ws, const BlockRef& basis, const BlockRef& S, const BlockRef& out) {
+  HipExecutor& ex = ws.ex;
+  if (out.ptr != S.ptr) copy(ex, S, out);
+  const int a = basis.cols, b = S.cols;
+  if (a == 0 || b == 0) return;
+  ws.ensure(std::max(a, b));
+  gram(ex, ws.gram_mode, basis, out, ws.G.data(), ws.cap);
+  gemm(ex, basis, ws.G.data(), ws.cap, out, -1.0, 1.0);
+  gram(ex, ws.gram_mode, basis, out, ws.R.data(), ws.cap);
+  gemm(ex, basis, ws.R.data(), ws.cap, out, -1.0, 1.0);
+}
+
+Block orthonormalize_block(Workspace& ws, const BlockRef& basis, const BlockRef& src, double rank_tol) {
+  HipExecutor& ex = ws.ex;
+  if (src.cols <= 0 || src.rows < src.cols) throw std::invalid_argument("orthonormalize_block: invalid source block.");
+  if (!std::isfinite(rank_tol) || rank_tol <= 0.0 || rank_tol >= 1.0)
+    throw std::invalid_argument("orthonormalize_block: rank tolerance must be in (0,1).");
+  if (basis.cols == basis.rows) return Block();
+  Block cur(ex, src.rows, src.cols, src.prec);
+  copy(ex, src, cur.ref());
+  const double eig_floor = src.prec == FE_F64 ? 1e-14 : 1e-7;  // resolvable Gram eigenvalue ratio
+  for (int pass = 0; pass < 2; ++pass) {
+    const std::vector<double> n0 = column_norms(ex, cur.ref());
+    double scale = 0.0;
+    for (double v : n0) {
+      if (!std::isfinite(v)) throw std::runtime_error("orthonormalize_block: non-finite column norms.");
+      scale = std::max(scale, v);
+    }
+    if (scale == 0.0) return Block();
+    block_gram_schmidt(ws, basis, cur.ref(), cur.ref());
+    const std::vector<double> n1 = column_norms(ex, cur.ref());
+    std::vector<int> keep;
+    for (int j = 0; j < cur.ncols(); ++j)
+      if (n1[j] > rank_tol * scale) keep.push_back(j);
+    const int capacity = static_cast<int>(std::min<int64_t>(cur.ncols(), src.rows - basis.cols));
+    if (keep.empty() || capacity <= 0) return Block();
+    // compact the kept columns and scale them to unit norm
+    Block kept(ex, src.rows, static_cast<int>(keep.size()), src.prec);
+    for (size_t j = 0; j < keep.size(); ++j) copy(ex, cur.cols(keep[j], 1), kept.cols(static_cast<int>(j), 1));
+    const int c = kept.ncols();
+    ws.ensure(c);
+    {
+      std::vector<double> inv(c);
+      for (int j = 0; j < c; ++j) inv[j] = 1.0 / n1[keep[j]];
+      check(fe_memcpy_h2d(ex.ctx(), ws.theta.data(), inv.data(), sizeof(double) * c), "h2d");
+      scale_columns(ex, kept.ref(), ws.theta.data());
+    }
+    // Gram eigen-decomposition of the unit-column block (SVQB, Stathopoulos & Wu 2002)
+    gram(ex, ws.gram_mode, kept.ref(), kept.ref(), ws.G.data(), ws.cap);
+    check(fe_syev(ex.ctx(), c, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap), "fe_syev");
+    std::vector<double> w(c);
+    check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * c), "d2h");
+    MatrixXd Vh = ws.V.to_host();
+    const double wmax = w.back();
+    std::vector<int> good;
+    for (int i = c - 1; i >= 0; --i)  // descending singular values, like the SVD ordering
+      if (w[i] > eig_floor * wmax && w[i] > 0.0 && static_cast<int>(good.size()) < capacity) good.push_back(i);
+    if (good.empty()) return Block();
+    const int r = static_cast<int>(good.size());
+    MatrixXd Cm(ws.cap, r);
+    for (int j = 0; j < r; ++j) {
+      const double f = 1.0 / std::sqrt(w[good[j]]);
+      for (int i = 0; i < c; ++i) Cm(i, j) = Vh(i, good[j]) * f;
+    }
+    check(fe_memcpy_h2d(ex.ctx(), ws.C.data(), Cm.data(), sizeof(double) * Cm.size()), "h2d");
+    Block next(ex, src.rows, r, src.prec);
+    gemm(ex, kept.ref(), ws.C.data(), ws.cap, next.ref(), 1.0, 0.0);
+    cur = std::move(next);
+  }
+  return cur;
+}
+
+std::vector<int> select_indices(const std::vector<double>& v, const std::string& which) {
+  const int m = static_cast<int>(v.size());
+  std::vector<int> idx(m);
+  std::iota(idx.begin(), idx.end(), 0);
+  if (which == "sr") std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return v[a] < v[b]; });
+  else if (which == "lr") std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return v[a] > v[b]; });
+  else if (which == "sm")
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(v[a]) < std::fabs(v[b]); });
+  else if (which == "lm")
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return std::fabs(v[a]) > std::fabs(v[b]); });
+  else if (which == "be") {
+    int pos = 0;
+    for (int i = 0; i < m / 2; ++i) {
+      idx[pos++] = m - 1 - i;
+      idx[pos++] = i;
+    }
+    if (m % 2 == 1) idx[pos] = m / 2;
+  } else {
+    throw std::invalid_argument("invalid which option (use SR/LR/SM/LM/BE).");
+  }
+  return idx;
+}
+
+RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
+                       const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R) {
+  HipExecutor& ex = ws.ex;
+  if (m <= 0 || requested <= 0 || requested > m)
+    throw std::invalid_argument("extract_projected_ritz_pairs: invalid active/requested Ritz dimensions.");
+  if (Q.cols < m || AQ.cols < m || Y.cols != requested || AY.cols != requested)
+    throw std::invalid_argument("extract_projected_ritz_pairs: multivector dimension mismatch.");
+  ws.ensure(m);
+  check(fe_sym_small(ex.ctx(), m, ws.G.data(), ws.cap), "fe_sym_small");
+  const int info = fe_syev(ex.ctx(), m, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap);
+  check(info, "fe_syev");
+  if (info != 0) throw std::runtime_error("extract_projected_ritz_pairs: projected eigensolve failed.");
+  std::vector<double> w(m);
+  check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * m), "d2h(ritz values)");
+  std::vector<int> idx = select_indices(w, which);
+  idx.resize(requested);
+  check(fe_gather_cols(ex.ctx(), m, requested, ws.V.data(), ws.cap, idx.data(), ws.C.data(), ws.cap),
+        "fe_gather_cols");
+  gemm(ex, Q.range(0, m), ws.C.data(), ws.cap, Y, 1.0, 0.0);
+  gemm(ex, AQ.range(0, m), ws.C.data(), ws.cap, AY, 1.0, 0.0);
+  RitzPairs out;
+  out.selected = idx;
+  out.values.resize(requested);
+  for (int j = 0; j < requested; ++j) out.values[j] = w[idx[j]];
+  check(fe_memcpy_h2d(ex.ctx(), ws.theta.data(), out.values.data(), sizeof(double) * requested), "h2d(theta)");
+  check(fe_ritz_residuals(ex.ctx(), Y.prec, Y.rows, requested, Y.ptr, Y.ld, AY.ptr, AY.ld, ws.theta.data(),
+                          R ? R->ptr : nullptr, R ? R->ld : 0, ws.rn.data(), ws.yn.data()),
+        "fe_ritz_residuals");
+  std::vector<double> rn(requested), yn(requested);
+  check(fe_memcpy_d2h(ex.ctx(), rn.data(), ws.rn.data(), sizeof(double) * requested), "d2h");
+  check(fe_memcpy_d2h(ex.ctx(), yn.data(), ws.yn.data(), sizeof(double) * requested), "d2h");
+  out.relres.resize(requested);
+  for (int j = 0; j < requested; ++j) {
+    const double den = (1.0 + std::fabs(out.values[j])) * yn[j];
+    out.relres[j] = den > 0.0 ? rn[j] / den : rn[j];
+  }
+  return out;
+}
+
+void lanczos_bounds(HipExecutor& ex, const HipCsrMatrix& A, double sign, unsigned seed, double& lower,
+                    double& upper) {
+  const int64_t n = A.rows();
+  const int steps = static_cast<int>(std::min<int64_t>(20, n));
+  Workspace ws(ex, std::max(steps, 2), FE_GRAM_NATIVE);
+  Block r(ex, n, 1, FE_F64), av(ex, n, 1, FE_F64), basis(ex, n, steps, FE_F64);
+  copy(ex, ex.random_block(n, 1, seed).ref(), r.ref());
+  MatrixXd T(steps, steps);
+  const double eps = std::numeric_limits<double>::epsilon();
+  double beta_next = 0.0;
+  // two-pass classical Gram-Schmidt of src against basis[:, :active] into res;
+  // returns the coefficient of the last basis column (the Lanczos alpha)
+  auto two_pass = [&](int active, const BlockRef& src, const BlockRef& res) -> double {
+    if (src.ptr != res.ptr) copy(ex, src, res);
+    if (active == 0) return 0.0;
+    const BlockRef Vb = basis.cols(0, active);
+    gram(ex, FE_GRAM_NATIVE, Vb, res, ws.G.data(), ws.cap);
+    gemm(ex, Vb, ws.G.data(), ws.cap, res, -1.0, 1.0);
+    gram(ex, FE_GRAM_NATIVE, Vb, res, ws.R.data(), ws.cap);
+    gemm(ex, Vb, ws.R.data(), ws.cap, res, -1.0, 1.0);
+    double h = 0.0, s = 0.0;
+    check(fe_memcpy_d2h(ex.ctx(), &h, ws.G.data() + (active - 1), sizeof(double)), "d2h");
+    check(fe_memcpy_d2h(ex.ctx(), &s, ws.R.data() + (active - 1), sizeof(double)), "d2h");
+    return h + s;
+  };
+  for (int it = 0; it < steps; ++it) {
+    double beta = column_norms(ex, r.ref())[0];
+    if (!std::isfinite(beta)) throw std::runtime_error("core::KrylovSubspace: non-finite residual norm encountered.");
+    bool restarted = false;
+    if (!(beta > eps)) {
+      const Block& cand = ex.random_block(n, 1, seed + static_cast<unsigned>(it + 1));
+      two_pass(it, cand.ref(), r.ref());
+      restarted = true;
+      beta = column_norms(ex, r.ref())[0];
+      if (beta <= eps) throw std::runtime_error("core::KrylovSubspace: breakdown, unable to generate new basis vector.");
+    }
+    check(fe_axpby_cols(ex.ctx(), FE_F64, n, 1, r.data(), r.ld(), r.data(), r.ld(), 1.0 / beta, 0.0, nullptr),
+          "scale");
+    copy(ex, r.ref(), basis.cols(it, 1));
+    spmm(ex, A, basis.cols(it, 1), av.ref(), sign);
+    const double alpha = two_pass(it + 1, av.ref(), r.ref());
+    beta_next = column_norms(ex, r.ref())[0];
+    if (!std::isfinite(beta_next))
+      throw std::runtime_error("core::KrylovSubspace: non-finite next residual norm encountered.");
+    if (it != 0) {
+      const double b = restarted ? 0.0 : beta;
+      T(it, it - 1) = b;
+      T(it - 1, it) = b;
+    }
+    T(it, it) = alpha;
+  }
+  MatrixXd Tp(ws.cap, ws.cap);
+  for (int j = 0; j < steps; ++j)
+    for (int i = 0; i < steps; ++i) Tp(i, j) = T(i, j);
+  ws.G.from_host(Tp);
+  const int info = fe_syev(ex.ctx(), steps, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap);
+  check(info, "fe_syev(lanczos)");
+  if (info != 0) throw std::runtime_error("ChebyshevSolver: Lanczos snapshot eigensolve failed.");
+  std::vector<double> w(steps);
+  check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * steps), "d2h");
+  MatrixXd Vh = ws.V.to_host();
+  lower = std::numeric_limits<double>::infinity();
+  upper = -std::numeric_limits<double>::infinity();
+  for (int j = 0; j < steps; ++j) {
+    const double rad = std::fabs(beta_next * Vh(steps - 1, j));
+    lower = std::min(lower, w[j] - rad);
+    upper = std::max(upper, w[j] + rad);
+  }
+  if (!std::isfinite(lower) || !std::isfinite(upper) || lower > upper)
+    throw std::runtime_error("ChebyshevSolver: invalid spectral bound estimates.");
+}
+
+}  // namespace hip
+}  // namespace fasteig

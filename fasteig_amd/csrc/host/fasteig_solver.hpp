@@ -1,0 +1,68 @@
+// Solver base class and factory (reference: include/fasteig_solver.hpp:84-169,
+// src/fasteig_solver.cpp:131-168). The block solvers named by the hot path run
+// on the HIP backend; the Krylov, SPPC and RMM-DIIS families of the reference
+// are not part of this library and the factory rejects them with a message.
+#pragma once
+
+#include <memory>
+#include <optional>
+#include <string>
+
+#include "fasteig.hpp"
+#include "fasteig_options.hpp"
+
+class fasteig_problem;
+
+namespace fasteig {
+
+class fasteig_solver {
+ public:
+  virtual ~fasteig_solver() = default;
+  virtual FASTEIG_INT solve() = 0;
+  FASTEIG_INT status() const noexcept { return status_; }
+  std::optional<long long> iterations() const noexcept {
+    return iterations_ >= 0 ? std::optional<long long>(iterations_) : std::nullopt;
+  }
+
+ protected:
+  fasteig_solver(::fasteig_problem& problem, const fasteig_options& options);
+  void set_iterations(long long it) noexcept { iterations_ = it; }
+
+  ::fasteig_problem& problem_;
+  VectorXd& eigvals_;
+  VectorXd& resnrms_;
+  fasteig_options options_;
+  FASTEIG_INT status_ = 0;
+  long long iterations_ = -1;
+};
+
+// Chebyshev-filtered subspace iteration (src/ChebyshevSolver.cpp:67-258).
+// Extensions: blksize > nev sets the subspace capacity; precision=mixed runs
+// the filter, CholQR Gram and Rayleigh-Ritz blocks in FP32 until the FP32
+// residuals stop improving, then finishes with FP64 sweeps.
+class ChebyshevSolver final : public fasteig_solver {
+ public:
+  ChebyshevSolver(::fasteig_problem& problem, const fasteig_options& options);
+  FASTEIG_INT solve() override;
+};
+
+// LOBPCG without preconditioner (src/LOBPCGSolver.cpp:25-173). Extension:
+// blksize > nev widens the block; convergence and output use the first nev.
+class LOBPCGSolver final : public fasteig_solver {
+ public:
+  LOBPCGSolver(::fasteig_problem& problem, const fasteig_options& options);
+  FASTEIG_INT solve() override;
+};
+
+// Block Davidson with thick restart (src/DavidsonSolver.cpp:27-264).
+class DavidsonSolver final : public fasteig_solver {
+ public:
+  DavidsonSolver(::fasteig_problem& problem, const fasteig_options& options);
+  FASTEIG_INT solve() override;
+};
+
+// Prefix dispatch like the reference (src/fasteig_solver.cpp:131-168).
+std::unique_ptr<fasteig_solver> make_fasteig_solver(const std::string& solver, ::fasteig_problem& problem,
+                                                    const fasteig_options& options);
+
+}  // namespace fasteig

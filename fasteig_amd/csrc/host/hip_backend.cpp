@@ -1,0 +1,199 @@
+// fasteig::hip host classes over the libfe_hip.so C ABI.
+#include "hip_backend.hpp"
+
+#include <random>
+#include <stdexcept>
+
+namespace fasteig {
+namespace hip {
+
+void check(int rc, const char* what) {
+  if (rc >= 0) return;
+  const std::string msg = std::string(what) + ": " + fe_last_error();
+  if (rc == -2) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+static int64_t padded_ld(int cols) { return cols <= 0 ? 4 : (static_cast<int64_t>(cols) + 3) / 4 * 4; }
+
+// ---------------------------------------------------------------- Block
+Block::Block(HipExecutor& ex, int64_t rows, int cols, int prec)
+    : ex_(&ex), rows_(rows), cols_(cols), ld_(padded_ld(cols)), prec_(prec) {
+  if (rows < 0 || cols < 0) throw std::invalid_argument("Block: negative dimensions");
+  const size_t bytes = static_cast<size_t>(rows) * static_cast<size_t>(ld_) * (prec == FE_F64 ? 8 : 4);
+  check(fe_alloc(ex.ctx(), bytes, &ptr_), "fe_alloc(block)");
+}
+
+Block::~Block() {
+  if (ptr_ && ex_) fe_free(ex_->ctx(), ptr_);
+}
+
+Block& Block::operator=(Block&& o) noexcept {
+  if (this != &o) {
+    if (ptr_ && ex_) fe_free(ex_->ctx(), ptr_);
+    ex_ = o.ex_;
+    ptr_ = o.ptr_;
+    rows_ = o.rows_;
+    cols_ = o.cols_;
+    ld_ = o.ld_;
+    prec_ = o.prec_;
+    o.ptr_ = nullptr;
+    o.ex_ = nullptr;
+  }
+  return *this;
+}
+
+void Block::swap(Block& o) noexcept {
+  std::swap(ex_, o.ex_);
+  std::swap(ptr_, o.ptr_);
+  std::swap(rows_, o.rows_);
+  std::swap(cols_, o.cols_);
+  std::swap(ld_, o.ld_);
+  std::swap(prec_, o.prec_);
+}
+
+// ---------------------------------------------------------------- Small
+Small::Small(HipExecutor& ex, int rows, int cols) : ex_(&ex), r_(rows), c_(cols) {
+  void* p = nullptr;
+  check(fe_alloc(ex.ctx(), sizeof(double) * static_cast<size_t>(std::max(rows, 1)) * std::max(cols, 1), &p),
+        "fe_alloc(small)");
+  ptr_ = static_cast<double*>(p);
+}
+
+Small::~Small() {
+  if (ptr_ && ex_) fe_free(ex_->ctx(), ptr_);
+}
+
+Small& Small::operator=(Small&& o) noexcept {
+  if (this != &o) {
+    if (ptr_ && ex_) fe_free(ex_->ctx(), ptr_);
+    ex_ = o.ex_;
+    ptr_ = o.ptr_;
+    r_ = o.r_;
+    c_ = o.c_;
+    o.ptr_ = nullptr;
+    o.ex_ = nullptr;
+  }
+  return *this;
+}
+
+MatrixXd Small::to_host() const {
+  MatrixXd M(r_, c_);
+  check(fe_memcpy_d2h(ex_->ctx(), M.data(), ptr_, sizeof(double) * M.size()), "Small::to_host");
+  return M;
+}
+
+void Small::from_host(const MatrixXd& M) {
+  if (M.rows() != r_ || M.cols() != c_) throw std::invalid_argument("Small::from_host: shape mismatch");
+  check(fe_memcpy_h2d(ex_->ctx(), ptr_, M.data(), sizeof(double) * M.size()), "Small::from_host");
+}
+
+// ---------------------------------------------------------------- CSR
+HipCsrMatrix::HipCsrMatrix(HipExecutor& ex, int n, const std::vector<int>& rowptr, const std::vector<int>& colind,
+                           const std::vector<double>& vals)
+    : ex_(ex), n_(n), nnz_(static_cast<int64_t>(vals.size())) {
+  if (static_cast<int>(rowptr.size()) != n + 1 || colind.size() != vals.size())
+    throw std::invalid_argument("HipCsrMatrix: inconsistent CSR arrays");
+  check(fe_csr_create(ex.ctx(), n, nnz_, rowptr.data(), colind.data(), vals.data(), &h_), "fe_csr_create");
+}
+
+HipCsrMatrix::~HipCsrMatrix() {
+  if (h_) fe_csr_destroy(h_);
+}
+
+void HipCsrMatrix::use_sell(int C, int sigma) {
+  if (!sell_built_) {
+    check(fe_csr_build_sell(ex_.ctx(), h_, C, sigma), "fe_csr_build_sell");
+    sell_built_ = true;
+  }
+  check(fe_csr_use(h_, FE_FMT_SELL), "fe_csr_use");
+  sell_ = true;
+}
+
+void HipCsrMatrix::use_csr() {
+  check(fe_csr_use(h_, FE_FMT_CSR), "fe_csr_use");
+  sell_ = false;
+}
+
+// ---------------------------------------------------------------- executor
+HipExecutor::HipExecutor(int device) : device_(device) { check(fe_ctx_create(device, &ctx_), "fe_ctx_create"); }
+
+HipExecutor::~HipExecutor() {
+  random_cache_.clear();
+  if (ctx_) fe_ctx_destroy(ctx_);
+}
+
+void HipExecutor::synchronize() const { check(fe_ctx_sync(ctx_), "fe_ctx_sync"); }
+
+const Block& HipExecutor::random_block(int64_t rows, int cols, unsigned seed) {
+  auto key = std::make_tuple(rows, cols, seed);
+  auto it = random_cache_.find(key);
+  if (it != random_cache_.end()) return it->second;
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<double> dist(-1.0, 1.0);
+  MatrixXd host(static_cast<int>(rows), cols);
+  for (int j = 0; j < cols; ++j)
+    for (int64_t i = 0; i < rows; ++i) host(static_cast<int>(i), j) = dist(rng);
+  Block b(*this, rows, cols, FE_F64);
+  upload(*this, host, b.ref());
+  return random_cache_.emplace(key, std::move(b)).first->second;
+}
+
+void HipExecutor::clear_cache() { random_cache_.clear(); }
+
+// ---------------------------------------------------------------- primitives
+void spmm(HipExecutor& ex, const HipCsrMatrix& A, const BlockRef& X, const BlockRef& Y, double alpha, double beta,
+          const BlockRef* Z, double gamma) {
+  if (X.prec != Y.prec || (Z && Z->prec != X.prec)) throw std::invalid_argument("spmm: precision mismatch");
+  if (X.cols != Y.cols || X.rows != A.rows() || Y.rows != A.rows())
+    throw std::invalid_argument("spmm: dimension mismatch");
+  check(fe_spmm(ex.ctx(), A.handle(), X.prec, X.cols, X.ptr, X.ld, Y.ptr, Y.ld, alpha, beta, Z ? Z->ptr : nullptr,
+                Z ? Z->ld : 0, gamma),
+        "fe_spmm");
+}
+
+void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, double* G, int64_t ldg, const double* w) {
+  if (X.prec != Y.prec || X.rows != Y.rows) throw std::invalid_argument("gram: operand mismatch");
+  check(fe_gram(ex.ctx(), X.prec == FE_F64 ? FE_GRAM_NATIVE : mode, X.prec, X.rows, X.cols, Y.cols, X.ptr, X.ld,
+                Y.ptr, Y.ld, w, G, ldg),
+        "fe_gram");
+}
+
+void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, double alpha,
+          double beta) {
+  if (X.prec != Y.prec || X.rows != Y.rows) throw std::invalid_argument("gemm: operand mismatch");
+  check(fe_gemm_nn(ex.ctx(), X.prec, X.rows, X.cols, Y.cols, X.ptr, X.ld, C, ldc, Y.ptr, Y.ld, alpha, beta),
+        "fe_gemm_nn");
+}
+
+void copy(HipExecutor& ex, const BlockRef& src, const BlockRef& dst) {
+  if (src.rows != dst.rows || src.cols != dst.cols) throw std::invalid_argument("copy: shape mismatch");
+  check(fe_copy2d(ex.ctx(), src.rows, src.cols, src.ptr, src.prec, src.ld, dst.ptr, dst.prec, dst.ld), "fe_copy2d");
+}
+
+void zero(HipExecutor& ex, const BlockRef& X) {
+  check(fe_zero2d(ex.ctx(), X.ptr, X.rows, X.cols, X.ld, static_cast<int>(X.esize())), "fe_zero2d");
+}
+
+std::vector<double> column_norms(HipExecutor& ex, const BlockRef& X) {
+  std::vector<double> out(X.cols);
+  if (X.cols == 0) return out;
+  Small d(ex, X.cols, 1);
+  check(fe_colnorms(ex.ctx(), X.prec, X.rows, X.cols, X.ptr, X.ld, d.data()), "fe_colnorms");
+  check(fe_memcpy_d2h(ex.ctx(), out.data(), d.data(), sizeof(double) * X.cols), "column_norms");
+  return out;
+}
+
+void upload(HipExecutor& ex, const MatrixXd& host, const BlockRef& dst) {
+  if (host.rows() != dst.rows || host.cols() != dst.cols) throw std::invalid_argument("upload: shape mismatch");
+  check(fe_upload_colmajor(ex.ctx(), dst.rows, dst.cols, host.data(), dst.ptr, dst.prec, dst.ld), "fe_upload");
+}
+
+MatrixXd download(HipExecutor& ex, const BlockRef& src) {
+  MatrixXd host(static_cast<int>(src.rows), src.cols);
+  check(fe_download_colmajor(ex.ctx(), src.rows, src.cols, src.ptr, src.prec, src.ld, host.data()), "fe_download");
+  return host;
+}
+
+}  // namespace hip
+}  // namespace fasteig

@@ -1,0 +1,166 @@
+// fasteig::hip -- the MI355X backend namespace that replaces the reference's
+// CUDA backend (include/gpu/*, src/gpu/*). It plays the roles of
+//   core::Executor      (include/core/Executor.hpp:22-47)      -> HipExecutor
+//   core::MultiVector   (include/core/MultiVector.hpp:12-57)   -> Block / BlockRef
+//   core::CsrMatrix     (include/core/CsrMatrix.hpp:15-44)     -> HipCsrMatrix
+// and calls the device only through the C ABI of libfe_hip.so (fe_hip.h).
+//
+// Device layout: blocks are row-major n x cols (ld >= cols, padded to a
+// multiple of 4 elements); small matrices are FP64 column-major. Work is
+// ordered on the executor's single stream; calls that return host values
+// synchronize it. Failures of the C ABI become std::invalid_argument (-2) or
+// std::runtime_error (otherwise), like the reference's CUDA checks
+// (include/gpu/CudaCheck.hpp:18-23).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "fasteig.hpp"
+#include "fe_hip.h"
+
+namespace fasteig {
+namespace hip {
+
+void check(int rc, const char* what);
+
+class HipExecutor;
+
+// Non-owning view of a row-major device block (possibly a column range).
+struct BlockRef {
+  void* ptr = nullptr;
+  int64_t rows = 0;
+  int cols = 0;
+  int64_t ld = 0;
+  int prec = FE_F64;
+  size_t esize() const { return prec == FE_F64 ? 8 : 4; }
+  BlockRef range(int first, int count) const {
+    BlockRef r = *this;
+    r.ptr = static_cast<char*>(ptr) + static_cast<size_t>(first) * esize();
+    r.cols = count;
+    return r;
+  }
+};
+
+// Owning row-major device block (replaces CudaMultiVector storage).
+class Block {
+ public:
+  Block() = default;
+  Block(HipExecutor& ex, int64_t rows, int cols, int prec);
+  ~Block();
+  Block(Block&& o) noexcept { *this = std::move(o); }
+  Block& operator=(Block&& o) noexcept;
+  Block(const Block&) = delete;
+  Block& operator=(const Block&) = delete;
+
+  BlockRef ref() const { return BlockRef{ptr_, rows_, cols_, ld_, prec_}; }
+  BlockRef cols(int first, int count) const { return ref().range(first, count); }
+  void* data() const { return ptr_; }
+  int64_t rows() const { return rows_; }
+  int ncols() const { return cols_; }
+  int64_t ld() const { return ld_; }
+  int prec() const { return prec_; }
+  bool empty() const { return ptr_ == nullptr; }
+  void swap(Block& o) noexcept;
+
+ private:
+  HipExecutor* ex_ = nullptr;
+  void* ptr_ = nullptr;
+  int64_t rows_ = 0;
+  int cols_ = 0;
+  int64_t ld_ = 0;
+  int prec_ = FE_F64;
+};
+
+// Device FP64 column-major small matrix (Gram, Cholesky factor, Ritz coefficients).
+class Small {
+ public:
+  Small() = default;
+  Small(HipExecutor& ex, int rows, int cols);
+  ~Small();
+  Small(Small&& o) noexcept { *this = std::move(o); }
+  Small& operator=(Small&& o) noexcept;
+  Small(const Small&) = delete;
+  Small& operator=(const Small&) = delete;
+  double* data() const { return ptr_; }
+  int rows() const { return r_; }
+  int cols() const { return c_; }
+  int64_t ld() const { return r_; }
+  MatrixXd to_host() const;
+  void from_host(const MatrixXd& M);
+
+ private:
+  HipExecutor* ex_ = nullptr;
+  double* ptr_ = nullptr;
+  int r_ = 0, c_ = 0;
+};
+
+// Device copy of a full CSR matrix (replaces CudaCsrMatrix, src/gpu/CudaCsrMatrix.cpp:31-42).
+class HipCsrMatrix {
+ public:
+  HipCsrMatrix(HipExecutor& ex, int n, const std::vector<int>& rowptr, const std::vector<int>& colind,
+               const std::vector<double>& vals);
+  ~HipCsrMatrix();
+  HipCsrMatrix(const HipCsrMatrix&) = delete;
+  HipCsrMatrix& operator=(const HipCsrMatrix&) = delete;
+  fe_csr handle() const { return h_; }
+  int rows() const { return n_; }
+  int64_t nonzeros() const { return nnz_; }
+  void use_sell(int C = 32, int sigma = 256);
+  void use_csr();
+  bool sell() const { return sell_; }
+
+ private:
+  HipExecutor& ex_;
+  fe_csr h_ = nullptr;
+  int n_ = 0;
+  int64_t nnz_ = 0;
+  bool sell_built_ = false, sell_ = false;
+};
+
+class HipExecutor {
+ public:
+  explicit HipExecutor(int device = 0);
+  ~HipExecutor();
+  HipExecutor(const HipExecutor&) = delete;
+  HipExecutor& operator=(const HipExecutor&) = delete;
+  fe_ctx ctx() const { return ctx_; }
+  const char* name() const noexcept { return "HIP"; }
+  bool is_gpu() const noexcept { return true; }
+  void synchronize() const;
+  int device() const { return device_; }
+
+  // Seeded start blocks (reference: mt19937 + uniform(-1,1), column-major,
+  // src/core/Executor.cpp:20-62), generated on the host once per
+  // (rows, cols, seed) and kept resident; later solves copy the cached block.
+  const Block& random_block(int64_t rows, int cols, unsigned seed);
+  // Drop cached start blocks (frees HBM).
+  void clear_cache();
+
+ private:
+  fe_ctx ctx_ = nullptr;
+  int device_ = 0;
+  std::map<std::tuple<int64_t, int, unsigned>, Block> random_cache_;
+};
+
+// ---------------------------------------------------------------- primitives
+// Y = alpha A X + beta X + gamma Z (Z may alias Y)
+void spmm(HipExecutor& ex, const HipCsrMatrix& A, const BlockRef& X, const BlockRef& Y, double alpha,
+          double beta = 0.0, const BlockRef* Z = nullptr, double gamma = 0.0);
+// G (m x p, ldg) = X^T Y
+void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, double* G, int64_t ldg,
+          const double* w = nullptr);
+// Y = alpha X C + beta Y, C FP64 column-major (m x p)
+void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, double alpha,
+          double beta);
+void copy(HipExecutor& ex, const BlockRef& src, const BlockRef& dst);
+void zero(HipExecutor& ex, const BlockRef& X);
+std::vector<double> column_norms(HipExecutor& ex, const BlockRef& X);
+void upload(HipExecutor& ex, const MatrixXd& host, const BlockRef& dst);
+MatrixXd download(HipExecutor& ex, const BlockRef& src);
+
+}  // namespace hip
+}  // namespace fasteig

@@ -1,0 +1,407 @@
+// Chebyshev, LOBPCG and Davidson on the HIP backend. The control flow of each
+// solver follows the reference source line by line (cited per step); every
+// n-sized operation runs on the device through libfe_hip.so.
+#include <algorithm>
+#include <cmath>
+#include <iostream>
+#include <limits>
+#include <stdexcept>
+
+#include "core_ops.hpp"
+#include "fasteig_problem.hpp"
+#include "fasteig_solver.hpp"
+
+namespace fasteig {
+
+using hip::Block;
+using hip::BlockRef;
+using hip::HipCsrMatrix;
+using hip::HipExecutor;
+using hip::Workspace;
+
+namespace {
+std::string lower_copy(std::string s) {
+  std::transform(s.begin(), s.end(), s.begin(), [](unsigned char c) { return static_cast<char>(std::tolower(c)); });
+  return s;
+}
+// s starts with the first n characters of prefix (src/fasteig_solver.cpp:21-23)
+bool has_prefix(const std::string& s, const std::string& prefix, size_t n) {
+  return s.size() >= n && s.compare(0, n, prefix, 0, n) == 0;
+}
+double max_head(const std::vector<double>& v, int count) {
+  double m = 0.0;
+  for (int j = 0; j < count && j < static_cast<int>(v.size()); ++j) m = std::max(m, v[j]);
+  return m;
+}
+bool all_finite(const std::vector<double>& v) {
+  for (double x : v)
+    if (!std::isfinite(x)) return false;
+  return true;
+}
+void progress(const fasteig_options& o, long long iter, const std::vector<double>& lam, const std::vector<double>& rr,
+              int count, double sign) {
+  if (o.print_every() <= 0 || o.verbosity() < 2 || iter % o.print_every() != 0) return;
+  std::cout << "  iter " << iter << ":";
+  for (int j = 0; j < count && j < static_cast<int>(lam.size()); ++j)
+    std::cout << " " << sign * lam[j] << "(" << rr[j] << ")";
+  std::cout << "\n";
+}
+// Start block of `cols` columns in precision prec: the cached seeded block,
+// with the first columns of options.v0 when provided (reference:
+// src/ChebyshevSolver.cpp:95-103, src/LOBPCGSolver.cpp:48-56).
+void start_block(HipExecutor& ex, const fasteig_options& o, int64_t n, int cols, int v0_max, unsigned seed,
+                 const Block& dst, const char* who) {
+  hip::copy(ex, ex.random_block(n, cols, seed).ref(), dst.ref());
+  if (o.v0().size() > 0) {
+    if (o.v0().rows() != n) throw std::invalid_argument(std::string(who) + ": v0 row count does not match the problem.");
+    const int pc = std::min(v0_max, o.v0().cols());
+    if (pc > 0) {
+      MatrixXd head(static_cast<int>(n), pc);
+      std::copy(o.v0().data(), o.v0().data() + n * pc, head.data());
+      Block tmp(ex, n, pc, FE_F64);
+      hip::upload(ex, head, tmp.ref());
+      hip::copy(ex, tmp.ref(), dst.cols(0, pc));
+    }
+  }
+}
+}  // namespace
+
+fasteig_solver::fasteig_solver(::fasteig_problem& problem, const fasteig_options& options)
+    : problem_(problem), eigvals_(problem.eigenvalues()), resnrms_(problem.residual_norms()), options_(options) {}
+
+ChebyshevSolver::ChebyshevSolver(::fasteig_problem& p, const fasteig_options& o) : fasteig_solver(p, o) {}
+LOBPCGSolver::LOBPCGSolver(::fasteig_problem& p, const fasteig_options& o) : fasteig_solver(p, o) {}
+DavidsonSolver::DavidsonSolver(::fasteig_problem& p, const fasteig_options& o) : fasteig_solver(p, o) {}
+
+// ============================================================== Chebyshev
+// Reference: src/ChebyshevSolver.cpp:67-221 (solve) and :223-258 (filter).
+FASTEIG_INT ChebyshevSolver::solve() {
+  options_.validate();
+  const int64_t n = problem_.n();
+  const int nev = options_.nev();
+  const int maxiter = options_.maxiter();
+  const int degree = options_.degree();
+  const double tol = options_.tol();
+  const std::string which = lower_copy(options_.which());
+  if (n <= 0 || nev <= 0 || nev > n) throw std::invalid_argument("ChebyshevSolver: nev must be in [1,n].");
+  if (which != "sr" && which != "lr") throw std::invalid_argument("ChebyshevSolver: only SR and LR are supported.");
+
+  auto exp = problem_.executor(options_);
+  HipExecutor& ex = *exp;
+  HipCsrMatrix& A = problem_.device_matrix(options_);
+  const double sign = which == "lr" ? -1.0 : 1.0;
+  const unsigned seed = options_.seed().value_or(1u);
+  const int capacity = options_.blksize() > nev ? static_cast<int>(std::min<int64_t>(options_.blksize(), n))
+                                                : nev + static_cast<int>(std::min<int64_t>(nev, n - nev));
+  const bool mixed = options_.mixed();
+  int prec = mixed ? FE_F32 : FE_F64;
+  const int gram_mode = options_.gram() == "bf16x3" ? FE_GRAM_BF16X3 : FE_GRAM_NATIVE;
+  Workspace ws(ex, capacity, mixed ? gram_mode : FE_GRAM_NATIVE);
+
+  // persistent blocks: V (subspace), F/P (filter recurrence), Q/AQ (basis), AY
+  Block V(ex, n, capacity, prec), F(ex, n, capacity, prec), P(ex, n, capacity, prec);
+  Block Q(ex, n, capacity, prec), AQ(ex, n, capacity, prec), AY(ex, n, capacity, prec);
+  start_block(ex, options_, n, capacity, nev, seed, V, "ChebyshevSolver");
+  if (hip::cholesky_qr(ws, V, Q) != 0) throw std::runtime_error("ChebyshevSolver: initial Cholesky QR failed.");
+
+  double lower = 0.0, upper = 0.0;
+  hip::lanczos_bounds(ex, A, sign, seed, lower, upper);
+
+  int locked = 0;
+  int subspace_cols = nev;
+  hip::RitzPairs ritz;
+  long long it_done = 0;
+  double prev_max = std::numeric_limits<double>::infinity();
+  bool skip_filter = false;  // the first FP64 cycle after the switch re-runs Rayleigh-Ritz only
+
+  for (int iter = 0; iter < maxiter; ++iter) {
+    const int active = subspace_cols - locked;
+    // 1. filter the active columns: Y = T_degree(c*op + d*I) X (ChebyshevSolver.cpp:151-155, 223-258)
+    if (!skip_filter && active > 0 && upper != lower) {
+      const double xlb = std::cosh(std::acosh(1.0e3) / degree);
+      const double cutoff = (2.0 * lower + upper * (xlb - 1.0)) / (1.0 + xlb);
+      const double c = 2.0 / (upper - cutoff);
+      const double d = -(upper + cutoff) / (upper - cutoff);
+      if (!std::isfinite(c) || !std::isfinite(d)) throw std::runtime_error("ChebyshevSolver: invalid filter scaling.");
+      BlockRef prev = F.cols(0, active), cur = P.cols(0, active);
+      hip::copy(ex, V.cols(locked, active), prev);
+      hip::spmm(ex, A, prev, cur, sign * c, d);                       // Y1 = (cA + dI) Y0
+      for (int k = 2; k <= degree; ++k) {                            // Y_k = 2(cA+dI)Y_{k-1} - Y_{k-2}
+        hip::spmm(ex, A, cur, prev, 2.0 * sign * c, 2.0 * d, &prev, -1.0);
+        std::swap(prev, cur);
+      }
+      // 2. deflate against the locked Ritz vectors (ChebyshevSolver.cpp:159-166)
+      if (locked > 0) hip::block_gram_schmidt(ws, V.cols(0, locked), cur, cur);
+      hip::copy(ex, cur, V.cols(locked, active));
+    }
+    skip_filter = false;
+    // CholQR of V[:, :subspace_cols] into Q (ChebyshevSolver.cpp:167-171)
+    {
+      ws.ensure(subspace_cols);
+      const BlockRef Vb = V.cols(0, subspace_cols);
+      hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap);
+      const int info = fe_potrf(ex.ctx(), subspace_cols, ws.G.data(), ws.cap);
+      hip::check(info, "fe_potrf");
+      if (info != 0) throw std::runtime_error("ChebyshevSolver: Cholesky QR failed during filtering.");
+      hip::check(fe_chol_rinv(ex.ctx(), subspace_cols, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
+      hip::gemm(ex, Vb, ws.R.data(), ws.cap, Q.cols(0, subspace_cols), 1.0, 0.0);
+    }
+    // 3. Rayleigh-Ritz (ChebyshevSolver.cpp:175-183); Ritz vectors go straight into V
+    hip::spmm(ex, A, Q.cols(0, subspace_cols), AQ.cols(0, subspace_cols), sign);
+    hip::gram(ex, ws.gram_mode, Q.cols(0, subspace_cols), AQ.cols(0, subspace_cols), ws.G.data(), ws.cap);
+    ritz = hip::extract_ritz(ws, subspace_cols, Q.ref(), AQ.ref(), subspace_cols, "sr", V.cols(0, subspace_cols),
+                             AY.cols(0, subspace_cols), nullptr);
+    if (!all_finite(ritz.values) || !all_finite(ritz.relres))
+      throw std::runtime_error("ChebyshevSolver: non-finite Ritz values or residuals.");
+    it_done = iter + 1;
+    const double mx = max_head(ritz.relres, nev);
+    progress(options_, it_done, ritz.values, ritz.relres, nev, sign);
+
+    if (prec == FE_F32) {
+      // Mixed precision: FP32 cycles until the FP32 residuals stall or reach the
+      // FP32 floor, then convert the subspace to FP64 and re-run Rayleigh-Ritz
+      // in FP64 before any FP64 filter cycle.
+      const bool floor_reached = mx <= std::max(tol, 2e-5);
+      const bool stalled = iter >= 2 && mx > 0.5 * prev_max;
+      prev_max = std::min(prev_max, mx);
+      if (floor_reached || stalled || iter + 1 == maxiter) {
+        prec = FE_F64;
+        Block V64(ex, n, capacity, FE_F64);
+        hip::copy(ex, V.ref(), V64.ref());
+        V = std::move(V64);
+        F = Block(ex, n, capacity, FE_F64);
+        P = Block(ex, n, capacity, FE_F64);
+        Q = Block(ex, n, capacity, FE_F64);
+        AQ = Block(ex, n, capacity, FE_F64);
+        AY = Block(ex, n, capacity, FE_F64);
+        ws.gram_mode = FE_GRAM_NATIVE;
+        locked = 0;
+        skip_filter = true;
+        if (iter + 1 == maxiter) --iter;  // always finish with the FP64 sweep
+        continue;
+      }
+      continue;  // no locking on FP32 residuals
+    }
+
+    // 4. ordered soft locking (ChebyshevSolver.cpp:188-192)
+    locked = 0;
+    while (locked < nev && ritz.relres[locked] <= tol) ++locked;
+    subspace_cols = std::max(subspace_cols, nev + std::min(locked, capacity - nev));
+    if (mx <= tol) break;  // (ChebyshevSolver.cpp:197-207)
+  }
+
+  eigvals_.resize(nev);
+  resnrms_.resize(nev);
+  for (int j = 0; j < nev; ++j) {
+    eigvals_(j) = sign * ritz.values[j];
+    resnrms_(j) = ritz.relres[j];
+  }
+  Block X(ex, n, nev, FE_F64);
+  hip::copy(ex, V.cols(0, nev), X.ref());
+  problem_.set_device_eigenvectors(std::move(X));
+  set_iterations(it_done);
+  return status_;
+}
+
+// ============================================================== LOBPCG
+// Reference: src/LOBPCGSolver.cpp:25-173.
+FASTEIG_INT LOBPCGSolver::solve() {
+  options_.validate();
+  const int64_t n = problem_.n();
+  const int nev = options_.nev();
+  const int maxiter = options_.maxiter();
+  const double tol = options_.tol();
+  const std::string which = lower_copy(options_.which());
+  if (n <= 0 || nev <= 0 || nev > n || maxiter <= 0)
+    throw std::invalid_argument("LOBPCGSolver: require 0 < nev <= n and maxiter > 0.");
+  if (which != "sr" && which != "lr") throw std::invalid_argument("LOBPCGSolver: only SR and LR are supported.");
+  auto exp = problem_.executor(options_);
+  HipExecutor& ex = *exp;
+  HipCsrMatrix& A = problem_.device_matrix(options_);
+  const double sign = which == "lr" ? -1.0 : 1.0;
+  const unsigned seed = options_.seed().value_or(1u);
+  const int bw = static_cast<int>(std::min<int64_t>(std::max(options_.blksize(), nev), n));
+  const int capacity = static_cast<int>(std::min<int64_t>(n, 3LL * bw));
+  const int prec = FE_F64;
+  Workspace ws(ex, capacity, FE_GRAM_NATIVE);
+
+  Block X(ex, n, bw, prec), AX(ex, n, bw, prec), Res(ex, n, bw, prec), tmp(ex, n, bw, prec);
+  Block basis(ex, n, capacity, prec), abasis(ex, n, capacity, prec), P(ex, n, bw, prec);
+  start_block(ex, options_, n, bw, nev, seed, X, "LOBPCGSolver");
+  hip::orthonormal_qr(ws, X, tmp);
+  // initial Rayleigh-Ritz (LOBPCGSolver.cpp:72-78)
+  hip::copy(ex, X.ref(), basis.cols(0, bw));
+  hip::spmm(ex, A, basis.cols(0, bw), abasis.cols(0, bw), sign);
+  hip::gram(ex, FE_GRAM_NATIVE, basis.cols(0, bw), abasis.cols(0, bw), ws.G.data(), ws.cap);
+  const BlockRef res0 = Res.ref();
+  hip::RitzPairs ritz = hip::extract_ritz(ws, bw, basis.ref(), abasis.ref(), bw, "sr", X.ref(), AX.ref(), &res0);
+  bool haveP = false;
+  long long it_done = 0;
+  for (int iter = 0; iter < maxiter; ++iter) {
+    if (!all_finite(ritz.values) || !all_finite(ritz.relres))
+      throw std::runtime_error("LOBPCGSolver: non-finite Ritz values or residuals.");
+    if (max_head(ritz.relres, nev) <= tol) break;
+    // soft locking: residual directions of unconverged pairs (LOBPCGSolver.cpp:95-103)
+    std::vector<int> act;
+    for (int j = 0; j < bw; ++j)
+      if (ritz.relres[j] > tol) act.push_back(j);
+    const int na = static_cast<int>(act.size());
+    Block Wsrc(ex, n, na, prec);
+    for (int j = 0; j < na; ++j) hip::copy(ex, Res.cols(act[j], 1), Wsrc.cols(j, 1));
+    // basis [X, orth(W), orth(P)] (LOBPCGSolver.cpp:105-129)
+    hip::orthonormal_qr(ws, X, tmp);
+    hip::copy(ex, X.ref(), basis.cols(0, bw));
+    int sc = bw;
+    Block W = hip::orthonormalize_block(ws, basis.cols(0, sc), Wsrc.ref());
+    if (!W.empty()) {
+      const int add = std::min(W.ncols(), capacity - sc);
+      hip::copy(ex, W.cols(0, add), basis.cols(sc, add));
+      sc += add;
+    }
+    if (haveP && sc < capacity) {
+      Block Psrc(ex, n, na, prec);
+      for (int j = 0; j < na; ++j) hip::copy(ex, P.cols(act[j], 1), Psrc.cols(j, 1));
+      Block Po = hip::orthonormalize_block(ws, basis.cols(0, sc), Psrc.ref());
+      if (!Po.empty()) {
+        const int add = std::min(Po.ncols(), capacity - sc);
+        hip::copy(ex, Po.cols(0, add), basis.cols(sc, add));
+        sc += add;
+      }
+    }
+    // Rayleigh-Ritz in S (LOBPCGSolver.cpp:131-141)
+    hip::spmm(ex, A, basis.cols(0, sc), abasis.cols(0, sc), sign);
+    ws.ensure(sc);
+    hip::gram(ex, FE_GRAM_NATIVE, basis.cols(0, sc), abasis.cols(0, sc), ws.G.data(), ws.cap);
+    const BlockRef rr = Res.ref();
+    ritz = hip::extract_ritz(ws, sc, basis.ref(), abasis.ref(), bw, "sr", X.ref(), AX.ref(), &rr);
+    if (!all_finite(ritz.values) || !all_finite(ritz.relres))
+      throw std::runtime_error("LOBPCGSolver: non-finite Ritz values or residuals.");
+    // P = S C with the X rows of C zeroed (LOBPCGSolver.cpp:143-150)
+    hip::check(fe_zero_rows_small(ex.ctx(), bw, bw, ws.C.data(), ws.cap), "fe_zero_rows_small");
+    hip::gemm(ex, basis.cols(0, sc), ws.C.data(), ws.cap, P.ref(), 1.0, 0.0);
+    haveP = true;
+    it_done = iter + 1;
+    progress(options_, it_done, ritz.values, ritz.relres, nev, sign);
+    const bool converged = max_head(ritz.relres, nev) <= tol;
+    const bool exhausted = sc == bw;
+    if (converged || exhausted) break;
+  }
+  eigvals_.resize(nev);
+  resnrms_.resize(nev);
+  for (int j = 0; j < nev; ++j) {
+    eigvals_(j) = sign * ritz.values[j];
+    resnrms_(j) = ritz.relres[j];
+  }
+  Block Xo(ex, n, nev, FE_F64);
+  hip::copy(ex, X.cols(0, nev), Xo.ref());
+  problem_.set_device_eigenvectors(std::move(Xo));
+  set_iterations(it_done);
+  return status_;
+}
+
+// ============================================================== Davidson
+// Reference: src/DavidsonSolver.cpp:27-264.
+FASTEIG_INT DavidsonSolver::solve() {
+  const int64_t n = problem_.n();
+  const int nev = options_.nev();
+  const int maxiter = options_.maxiter();
+  const int maxblks = options_.maxblks();
+  const double tol = options_.tol();
+  const std::string which = lower_copy(options_.which());
+  if (n <= 0) throw std::invalid_argument("DavidsonSolver: n must be > 0.");
+  if (nev <= 0 || nev > n) throw std::invalid_argument("DavidsonSolver: nev must be in [1,n].");
+  if (maxiter <= 0) throw std::invalid_argument("DavidsonSolver: maxiter must be > 0.");
+  if (maxblks < 1) throw std::invalid_argument("DavidsonSolver: maxblks must be at least 1.");
+  if (which == "sai") throw std::invalid_argument("DavidsonSolver: SAI is not supported yet; use SR/LR/SM/LM/BE.");
+  if (which != "sr" && which != "lr" && which != "sm" && which != "lm" && which != "be")
+    throw std::invalid_argument("DavidsonSolver: invalid which option (use SR/LR/SM/LM/BE).");
+  auto exp = problem_.executor(options_);
+  HipExecutor& ex = *exp;
+  HipCsrMatrix& A = problem_.device_matrix(options_);
+  const unsigned seed = options_.seed().value_or(1u);
+  const int prec = FE_F64;
+  const int maxcols = static_cast<int>(std::min<int64_t>(n, static_cast<int64_t>(maxblks) * nev));
+  Workspace ws(ex, maxcols, FE_GRAM_NATIVE);
+  hip::Small proj(ex, maxcols, maxcols);  // bordered projected matrix, column-major, ld maxcols
+
+  Block V0(ex, n, nev, prec), tmp(ex, n, nev, prec);
+  start_block(ex, options_, n, nev, nev, seed, V0, "DavidsonSolver");
+  if (hip::cholesky_qr(ws, V0, tmp) != 0) throw std::runtime_error("DavidsonSolver: initial Cholesky QR failed.");
+  Block basis(ex, n, maxcols, prec), abasis(ex, n, maxcols, prec);
+  Block Y(ex, n, nev, prec), AY(ex, n, nev, prec), Res(ex, n, nev, prec);
+  const BlockRef resref = Res.ref();
+
+  auto load_projected = [&](int m) {  // proj[:m, :m] -> ws.G (ld cap)
+    ws.ensure(m);
+    hip::check(fe_copy2d(ex.ctx(), m, m, proj.data(), FE_F64, maxcols, ws.G.data(), FE_F64, ws.cap), "copy(proj)");
+  };
+  // initial projection (DavidsonSolver.cpp:91-102)
+  hip::copy(ex, V0.ref(), basis.cols(0, nev));
+  hip::spmm(ex, A, basis.cols(0, nev), abasis.cols(0, nev), 1.0);
+  hip::gram(ex, FE_GRAM_NATIVE, basis.cols(0, nev), abasis.cols(0, nev), proj.data(), maxcols);
+  load_projected(nev);
+  hip::RitzPairs ritz = hip::extract_ritz(ws, nev, basis.ref(), abasis.ref(), nev, which, Y.ref(), AY.ref(), &resref);
+  const double locktol = std::min(1.0e-5, tol);
+  long long it_done = 0;
+  for (int iter = 0; iter < maxiter; ++iter) {
+    // thick restart (DavidsonSolver.cpp:131-155)
+    hip::orthonormal_qr(ws, Y, tmp);
+    hip::spmm(ex, A, Y.ref(), AY.ref(), 1.0);
+    hip::copy(ex, Y.ref(), basis.cols(0, nev));
+    hip::copy(ex, AY.ref(), abasis.cols(0, nev));
+    hip::gram(ex, FE_GRAM_NATIVE, basis.cols(0, nev), abasis.cols(0, nev), proj.data(), maxcols);
+    int act = nev;
+    load_projected(nev);
+    ritz = hip::extract_ritz(ws, nev, basis.ref(), abasis.ref(), nev, which, Y.ref(), AY.ref(), &resref);
+    for (int blk = 1; blk < maxblks; ++blk) {
+      std::vector<int> ai;
+      for (int j = 0; j < nev; ++j)
+        if (ritz.relres[j] > locktol) ai.push_back(j);
+      const int cc = static_cast<int>(ai.size());
+      if (cc == 0 || act + cc > maxcols) break;
+      // correction block from raw residuals (DavidsonSolver.cpp:172-198)
+      Block corr(ex, n, cc, prec), ctmp(ex, n, cc, prec), acorr(ex, n, cc, prec);
+      for (int j = 0; j < cc; ++j) hip::copy(ex, Res.cols(ai[j], 1), corr.cols(j, 1));
+      hip::block_gram_schmidt(ws, basis.cols(0, act), corr.ref(), corr.ref());
+      hip::orthonormal_qr(ws, corr, ctmp);
+      hip::spmm(ex, A, corr.ref(), acorr.ref(), 1.0);
+      // bordered projection (DavidsonSolver.cpp:204-220): [V, corr]^T A corr and its transpose
+      double* col_act = proj.data() + static_cast<int64_t>(act) * maxcols;
+      hip::copy(ex, corr.ref(), basis.cols(act, cc));
+      hip::copy(ex, acorr.ref(), abasis.cols(act, cc));
+      hip::gram(ex, FE_GRAM_NATIVE, basis.cols(0, act + cc), acorr.ref(), col_act, maxcols);
+      hip::gram(ex, FE_GRAM_NATIVE, acorr.ref(), basis.cols(0, act), proj.data() + act, maxcols);
+      act += cc;
+      load_projected(act);
+      ritz = hip::extract_ritz(ws, act, basis.ref(), abasis.ref(), nev, which, Y.ref(), AY.ref(), &resref);
+    }
+    it_done = iter + 1;
+    progress(options_, it_done, ritz.values, ritz.relres, nev, 1.0);
+    double mx = 0.0;
+    for (double r : ritz.relres) mx = std::max(mx, r);
+    if (!ritz.relres.empty() && mx < tol) break;  // strict (DavidsonSolver.cpp:236-238)
+  }
+  eigvals_.resize(nev);
+  resnrms_.resize(nev);
+  for (int j = 0; j < nev; ++j) {
+    eigvals_(j) = ritz.values[j];
+    resnrms_(j) = ritz.relres[j];
+  }
+  Block Xo(ex, n, nev, FE_F64);
+  hip::copy(ex, Y.ref(), Xo.ref());
+  problem_.set_device_eigenvectors(std::move(Xo));
+  set_iterations(it_done);
+  return status_;
+}
+
+std::unique_ptr<fasteig_solver> make_fasteig_solver(const std::string& solver, ::fasteig_problem& problem,
+                                                    const fasteig_options& options) {
+  if (has_prefix(solver, "Davidson", 3)) return std::make_unique<DavidsonSolver>(problem, options);
+  if (has_prefix(solver, "LOBPCG", 3)) return std::make_unique<LOBPCGSolver>(problem, options);
+  if (has_prefix(solver, "Chebyshev", 3)) return std::make_unique<ChebyshevSolver>(problem, options);
+  if (has_prefix(solver, "auto", 3)) return nullptr;
+  throw std::invalid_argument("solver '" + solver +
+                              "' is not available on the HIP backend (Chebyshev, LOBPCG, Davidson)");
+}
+
+}  // namespace fasteig

@@ -158,6 +158,8 @@ int fe_colnorms(fe_ctx ctx, int prec, int64_t n, int k, const void* X, int64_t l
 int fe_ritz_residuals(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, int64_t ldy,
                       const void* AY, int64_t lday, const double* theta, void* R, int64_t ldr,
                       double* rnorm, double* ynorm);
+/* X[i, :] *= d[i] (d device FP64, n entries): diagonal scaling of a block. */
+int fe_scale_rows(fe_ctx ctx, int prec, int64_t n, int k, void* X, int64_t ldx, const double* d);
 /* Copy a rows x cols block between (possibly different) precisions. */
 int fe_copy2d(fe_ctx ctx, int64_t rows, int cols, const void* src, int src_prec, int64_t lds,
               void* dst, int dst_prec, int64_t ldd);
