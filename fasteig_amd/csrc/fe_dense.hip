@@ -6,7 +6,8 @@
 // v_mfma_f32_16x16x4_f32), one operand element per lane:
 //   A operand: lane l holds A[l % 16][l / 16]      (16 x 4)
 //   B operand: lane l holds B[l / 16][l % 16]      (4 x 16)
-//   D result : lane l, item i holds D[4 (l / 16) + i][l % 16]
+//   D result : lane l, item i holds D[4 (l / 16) + i][l % 16] (FP32)
+//              and D[l / 16 + 4 i][l % 16] (FP64: rows interleaved)
 // The BF16x3 Gram splits each FP32 operand into bf16 hi + lo parts and runs
 // v_mfma_f32_16x16x32_bf16 (8 bf16 per lane: lane l holds A[l % 16][8 (l/16) + 0..7]).
 // A workgroup is 4 wavefronts computing a 64 x 64 output tile (32 x 32 each).
@@ -26,6 +27,8 @@ struct mfma<double> {
   __device__ static __forceinline__ acc run(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // output row of accumulator item e in lane `lane`
+  __device__ static __forceinline__ int row(int lane, int e) { return (lane >> 4) + 4 * e; }
 };
 template <>
 struct mfma<float> {
@@ -33,6 +36,7 @@ struct mfma<float> {
   __device__ static __forceinline__ acc run(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 
 constexpr int TILE = 64;   // output tile edge per workgroup
@@ -40,17 +44,34 @@ constexpr int KCG = 32;    // rows per LDS stage in the Gram kernel
 constexpr int KCM = 16;    // inner-dimension chunk in the GEMM kernel
 constexpr int PAD = 1;     // LDS row padding (elements)
 
+// Tile t of the output: row-major over (ta, tb) tiles, or, for a symmetric
+// Gram (sym != 0), over the upper triangle ta <= tb only.
+__device__ __forceinline__ void gram_tile(int t, int tiles_b, int sym, int& ta, int& tb) {
+  if (!sym) {
+    ta = t / tiles_b;
+    tb = t % tiles_b;
+    return;
+  }
+  int a = 0;
+  while (t >= tiles_b - a) {
+    t -= tiles_b - a;
+    ++a;
+  }
+  ta = a;
+  tb = a + t;
+}
+
 // ---------------------------------------------------------------- Gram (native)
 template <typename T>
 __global__ __launch_bounds__(256) void k_gram(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
                                               const T* __restrict__ Y, int64_t ldy,
                                               const double* __restrict__ w, int tiles_b, int ntiles,
-                                              const int* __restrict__ tile_list, int64_t rows_per,
+                                              int sym, int64_t rows_per,
                                               double* __restrict__ part) {
   __shared__ T Xs[KCG][TILE + PAD];
   __shared__ T Ys[KCG][TILE + PAD];
-  const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
-  const int ta = tile / tiles_b, tb = tile % tiles_b;
+  int ta, tb;
+  gram_tile(blockIdx.x, tiles_b, sym, ta, tb);
   const int a0 = ta * TILE, b0 = tb * TILE;
   const int64_t chunk = blockIdx.y;
   const int64_t r0 = chunk * rows_per;
@@ -103,7 +124,7 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, int m, int p, const T* 
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = wm * 32 + i * 16 + 4 * (lane >> 4) + e;
+        const int row = wm * 32 + i * 16 + mfma<T>::row(lane, e);
         const int col = wn * 32 + j * 16 + (lane & 15);
         P[col * TILE + row] = static_cast<double>(acc[i][j][e]);
       }
@@ -118,12 +139,12 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 __global__ __launch_bounds__(256) void k_gram_bf16x3(int64_t n, int m, int p, const float* __restrict__ X,
                                                      int64_t ldx, const float* __restrict__ Y, int64_t ldy,
                                                      const double* __restrict__ w, int tiles_b, int ntiles,
-                                                     const int* __restrict__ tile_list, int64_t rows_per,
+                                                     int sym, int64_t rows_per,
                                                      double* __restrict__ part) {
   // stage KCG rows; store as [col][row] so each lane's 8 consecutive k values are contiguous
   __shared__ __bf16 Xh[TILE][KCG + 8], Xl[TILE][KCG + 8], Yh[TILE][KCG + 8], Yl[TILE][KCG + 8];
-  const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
-  const int ta = tile / tiles_b, tb = tile % tiles_b;
+  int ta, tb;
+  gram_tile(blockIdx.x, tiles_b, sym, ta, tb);
   const int a0 = ta * TILE, b0 = tb * TILE;
   const int64_t chunk = blockIdx.y;
   const int64_t r0 = chunk * rows_per;
@@ -201,11 +222,11 @@ __global__ __launch_bounds__(256) void k_gram_bf16x3(int64_t n, int m, int p, co
 }
 
 // Fixed-order sum over chunks; mirrors the lower triangle for symmetric Grams.
-__global__ void k_gram_reduce(int m, int p, int tiles_b, int ntiles, const int* __restrict__ tile_list,
-                              int chunks, const double* __restrict__ part, double* G, int64_t ldg, int sym) {
+__global__ void k_gram_reduce(int m, int p, int tiles_b, int ntiles, int chunks, const double* __restrict__ part,
+                              double* G, int64_t ldg, int sym) {
   const int t = blockIdx.x;  // computed tile slot
-  const int tile = tile_list ? tile_list[t] : t;
-  const int ta = tile / tiles_b, tb = tile % tiles_b;
+  int ta, tb;
+  gram_tile(t, tiles_b, sym, ta, tb);
   for (int e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
     const int col = e / TILE, row = e % TILE;
     const int a = ta * TILE + row, b = tb * TILE + col;
@@ -294,7 +315,7 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t n, int m, int p, const 
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int64_t row = r0 + wm * 32 + i * 16 + 4 * (lane >> 4) + e;
+        const int64_t row = r0 + wm * 32 + i * 16 + mfma<T>::row(lane, e);
         const int col = c0 + wn * 32 + j * 16 + (lane & 15);
         if (row < n && col < p) {
           T& y = Y[row * ldy + col];
@@ -317,12 +338,7 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
   if (m == 0 || p == 0) return 0;
   const int tiles_a = (m + TILE - 1) / TILE, tiles_b = (p + TILE - 1) / TILE;
   const bool sym = (X == Y) && (m == p) && (ldx == ldy) && !w;
-  // tile list (upper triangle only when symmetric)
-  std::vector<int> tl;
-  for (int a = 0; a < tiles_a; ++a)
-    for (int b = 0; b < tiles_b; ++b)
-      if (!sym || a <= b) tl.push_back(a * tiles_b + b);
-  const int ntiles = static_cast<int>(tl.size());
+  const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
   if (n == 0) return fe_zero2d(ctx, G, p, m, ldg, 8);
   // split rows so that about 4 workgroups per CU are live
   const int target = 4 * (ctx->cus > 0 ? ctx->cus : 256);
@@ -333,28 +349,24 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
   chunks = (n + rows_per - 1) / rows_per;
   void* ws = nullptr;
   const size_t part_bytes = sizeof(double) * (size_t)chunks * ntiles * TILE * TILE;
-  const size_t list_bytes = sizeof(int) * ntiles;
-  int rc = scratch(ctx, 0, part_bytes + list_bytes + 256, &ws);
+  int rc = scratch(ctx, 0, part_bytes, &ws);
   if (rc) return rc;
   double* part = (double*)ws;
-  int* dlist = (int*)((char*)ws + part_bytes);
-  FE_HIP_TRY(hipMemcpyAsync(dlist, tl.data(), list_bytes, hipMemcpyHostToDevice, ctx->stream));
+  const int symi = sym ? 1 : 0;
   const dim3 grid((unsigned)ntiles, (unsigned)chunks);
   if (prec == FE_F64)
     hipLaunchKernelGGL(k_gram<double>, grid, dim3(256), 0, ctx->stream, n, m, p, (const double*)X, ldx,
-                       (const double*)Y, ldy, w, tiles_b, ntiles, dlist, rows_per, part);
+                       (const double*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
   else if (mode == FE_GRAM_NATIVE)
     hipLaunchKernelGGL(k_gram<float>, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
-                       (const float*)Y, ldy, w, tiles_b, ntiles, dlist, rows_per, part);
+                       (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
   else
     hipLaunchKernelGGL(k_gram_bf16x3, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
-                       (const float*)Y, ldy, w, tiles_b, ntiles, dlist, rows_per, part);
+                       (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
   FE_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles), dim3(256), 0, ctx->stream, m, p, tiles_b, ntiles,
-                     dlist, (int)chunks, part, G, ldg, sym ? 1 : 0);
+                     (int)chunks, part, G, ldg, symi);
   FE_CHECK_LAUNCH();
-  // the host tile list must outlive the async copy: wait for it
-  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 

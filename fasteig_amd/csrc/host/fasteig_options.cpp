@@ -91,6 +91,10 @@ bool fasteig_options::set(std::string_view key, std::string_view value) {
     const std::string v = lower(value);
     if (v != "csr" && v != "sell") throw std::invalid_argument("format must be 'csr' or 'sell'");
     format_ = v;
+  } else if (k == "cheb_bounds") {
+    const std::string v = lower(value);
+    if (v != "amplification" && v != "ritz") throw std::invalid_argument("cheb_bounds must be 'amplification' or 'ritz'");
+    cheb_bounds_ = v;
   } else if (k == "device") device_ = static_cast<int>(parse_int(key, value));
   else return false;
   return true;
@@ -119,6 +123,7 @@ std::string fasteig_options::get(std::string_view key) const {
   if (k == "precision") return precision_;
   if (k == "gram") return gram_;
   if (k == "format") return format_;
+  if (k == "cheb_bounds") return cheb_bounds_;
   if (k == "device") return std::to_string(device_);
   throw std::out_of_range("Unknown option key: " + std::string(key));
 }

@@ -6,6 +6,9 @@
 //   gram       "native" (FP64/FP32 MFMA) | "bf16x3" (split-BF16 Gram, FP32 blocks)
 //   format     "csr" | "sell" (SELL-32-sigma sparse product)
 //   device     HIP device index
+//   cheb_bounds "amplification" (the reference's fixed cutoff) | "ritz" (the
+//              damped interval starts at the largest Ritz value of the current
+//              full-capacity subspace; scaled recurrence)
 #pragma once
 
 #include <optional>
@@ -51,6 +54,7 @@ class fasteig_options {
   const std::string& precision() const noexcept { return precision_; }
   const std::string& gram() const noexcept { return gram_; }
   const std::string& format() const noexcept { return format_; }
+  const std::string& cheb_bounds() const noexcept { return cheb_bounds_; }
   int device() const noexcept { return device_; }
   bool mixed() const noexcept { return precision_ == "mixed"; }
 
@@ -86,6 +90,7 @@ class fasteig_options {
   std::string precision_ = "fp64";
   std::string gram_ = "native";
   std::string format_ = "csr";
+  std::string cheb_bounds_ = "amplification";
   int device_ = 0;
   MatrixXd v0_;
 

@@ -107,38 +107,68 @@ FASTEIG_INT ChebyshevSolver::solve() {
   double lower = 0.0, upper = 0.0;
   hip::lanczos_bounds(ex, A, sign, seed, lower, upper);
 
+  const bool ritz_bounds = options_.cheb_bounds() == "ritz";
   int locked = 0;
-  int subspace_cols = nev;
+  int subspace_cols = ritz_bounds ? capacity : nev;
   hip::RitzPairs ritz;
   long long it_done = 0;
   double prev_max = std::numeric_limits<double>::infinity();
-  bool skip_filter = false;  // the first FP64 cycle after the switch re-runs Rayleigh-Ritz only
+  double cutoff = std::numeric_limits<double>::quiet_NaN();  // adaptive bound (ritz_bounds)
+  hip::Small inv_norms(ex, capacity, 1);
 
   for (int iter = 0; iter < maxiter; ++iter) {
     const int active = subspace_cols - locked;
-    // 1. filter the active columns: Y = T_degree(c*op + d*I) X (ChebyshevSolver.cpp:151-155, 223-258)
-    if (!skip_filter && active > 0 && upper != lower) {
-      const double xlb = std::cosh(std::acosh(1.0e3) / degree);
-      const double cutoff = (2.0 * lower + upper * (xlb - 1.0)) / (1.0 + xlb);
-      const double c = 2.0 / (upper - cutoff);
-      const double d = -(upper + cutoff) / (upper - cutoff);
-      if (!std::isfinite(c) || !std::isfinite(d)) throw std::runtime_error("ChebyshevSolver: invalid filter scaling.");
+    // 1. filter the active columns (ChebyshevSolver.cpp:151-155, 223-258)
+    if (active > 0 && upper != lower) {
       BlockRef prev = F.cols(0, active), cur = P.cols(0, active);
       hip::copy(ex, V.cols(locked, active), prev);
-      hip::spmm(ex, A, prev, cur, sign * c, d);                       // Y1 = (cA + dI) Y0
-      for (int k = 2; k <= degree; ++k) {                            // Y_k = 2(cA+dI)Y_{k-1} - Y_{k-2}
-        hip::spmm(ex, A, cur, prev, 2.0 * sign * c, 2.0 * d, &prev, -1.0);
-        std::swap(prev, cur);
+      if (ritz_bounds && std::isfinite(cutoff) && cutoff > lower && cutoff < upper) {
+        // scaled filter on [cutoff, upper] normalised at the lower bound
+        // (Zhou & Saad 2007, Alg. 4.1): sigma_1 = e/(a0 - c),
+        // Y_1 = (sigma_1/e)(A - cI) X, Y_{k+1} = (2 s_{k+1}/e)(A - cI) Y_k - s_k s_{k+1} Y_{k-1}
+        const double e = 0.5 * (upper - cutoff), c = 0.5 * (upper + cutoff);
+        double sigma = e / (lower - c);
+        const double s1 = sigma;
+        hip::spmm(ex, A, prev, cur, sign * s1 / e, -c * s1 / e);
+        for (int k = 2; k <= degree; ++k) {
+          const double sn = 1.0 / (2.0 / s1 - sigma);
+          hip::spmm(ex, A, cur, prev, sign * 2.0 * sn / e, -c * 2.0 * sn / e, &prev, -sigma * sn);
+          std::swap(prev, cur);
+          sigma = sn;
+        }
+      } else {
+        const double xlb = std::cosh(std::acosh(1.0e3) / degree);
+        const double cut = (2.0 * lower + upper * (xlb - 1.0)) / (1.0 + xlb);
+        const double c = 2.0 / (upper - cut);
+        const double d = -(upper + cut) / (upper - cut);
+        if (!std::isfinite(c) || !std::isfinite(d)) throw std::runtime_error("ChebyshevSolver: invalid filter scaling.");
+        hip::spmm(ex, A, prev, cur, sign * c, d);                       // Y1 = (cA + dI) Y0
+        for (int k = 2; k <= degree; ++k) {                            // Y_k = 2(cA+dI)Y_{k-1} - Y_{k-2}
+          hip::spmm(ex, A, cur, prev, 2.0 * sign * c, 2.0 * d, &prev, -1.0);
+          std::swap(prev, cur);
+        }
       }
       // 2. deflate against the locked Ritz vectors (ChebyshevSolver.cpp:159-166)
       if (locked > 0) hip::block_gram_schmidt(ws, V.cols(0, locked), cur, cur);
+      if (prec == FE_F32) {
+        // FP32 blocks: equilibrate the columns so the FP32 Gram of the filtered
+        // block is not squared-condition-number limited by their norm spread
+        const std::vector<double> nr = hip::column_norms(ex, cur);
+        std::vector<double> inv(active);
+        for (int j = 0; j < active; ++j) inv[j] = nr[j] > 0.0 ? 1.0 / nr[j] : 1.0;
+        hip::check(fe_memcpy_h2d(ex.ctx(), inv_norms.data(), inv.data(), sizeof(double) * active), "h2d");
+        hip::check(fe_axpby_cols(ex.ctx(), FE_F32, n, active, cur.ptr, cur.ld, cur.ptr, cur.ld, 0.0, 0.0,
+                                 inv_norms.data()),
+                   "scale");
+      }
       hip::copy(ex, cur, V.cols(locked, active));
     }
-    skip_filter = false;
-    // CholQR of V[:, :subspace_cols] into Q (ChebyshevSolver.cpp:167-171)
-    {
+    // CholQR of V[:, :subspace_cols] into Q (ChebyshevSolver.cpp:167-171); the
+    // FP32 phase runs it twice (CholQR2) for orthogonality at FP32 accuracy
+    for (int pass = 0; pass < (prec == FE_F32 ? 2 : 1); ++pass) {
       ws.ensure(subspace_cols);
-      const BlockRef Vb = V.cols(0, subspace_cols);
+      const BlockRef Vb = pass == 0 ? V.cols(0, subspace_cols) : AY.cols(0, subspace_cols);
+      if (pass == 1) hip::copy(ex, Q.cols(0, subspace_cols), Vb);
       hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap);
       const int info = fe_potrf(ex.ctx(), subspace_cols, ws.G.data(), ws.cap);
       hip::check(info, "fe_potrf");
@@ -154,13 +184,14 @@ FASTEIG_INT ChebyshevSolver::solve() {
     if (!all_finite(ritz.values) || !all_finite(ritz.relres))
       throw std::runtime_error("ChebyshevSolver: non-finite Ritz values or residuals.");
     it_done = iter + 1;
+    cutoff = ritz.values[subspace_cols - 1];
     const double mx = max_head(ritz.relres, nev);
     progress(options_, it_done, ritz.values, ritz.relres, nev, sign);
 
     if (prec == FE_F32) {
       // Mixed precision: FP32 cycles until the FP32 residuals stall or reach the
-      // FP32 floor, then convert the subspace to FP64 and re-run Rayleigh-Ritz
-      // in FP64 before any FP64 filter cycle.
+      // FP32 floor; then the subspace is promoted to FP64 and the remaining
+      // cycles (filter, CholQR, Rayleigh-Ritz) run in FP64.
       const bool floor_reached = mx <= std::max(tol, 2e-5);
       const bool stalled = iter >= 2 && mx > 0.5 * prev_max;
       prev_max = std::min(prev_max, mx);
@@ -176,9 +207,7 @@ FASTEIG_INT ChebyshevSolver::solve() {
         AY = Block(ex, n, capacity, FE_F64);
         ws.gram_mode = FE_GRAM_NATIVE;
         locked = 0;
-        skip_filter = true;
-        if (iter + 1 == maxiter) --iter;  // always finish with the FP64 sweep
-        continue;
+        if (iter + 1 == maxiter) --iter;  // always finish with an FP64 cycle
       }
       continue;  // no locking on FP32 residuals
     }

@@ -654,6 +654,32 @@ void cheb_filter(const Csr& A, double sign, Mat& Xb, int degree, double lower, d
   Xb = std::move(cur);
 }
 
+// Scaled Chebyshev filter on [cutoff, upper] normalised at a0 (Zhou & Saad,
+// "A Chebyshev-Davidson algorithm for large symmetric eigenproblems", SIAM J.
+// Matrix Anal. Appl. 29 (2007), Algorithm 4.1). The adaptive-bound extension
+// of the reference's filter: the damped interval starts at the largest Ritz
+// value of the current subspace instead of the fixed-amplification cutoff of
+// src/ChebyshevSolver.cpp:232-236. Operates on sign*A.
+void cheb_filter_scaled(const Csr& A, double sign, Mat& Xb, int degree, double a0, double cutoff, double upper) {
+  const double e = 0.5 * (upper - cutoff), c = 0.5 * (upper + cutoff);
+  double sigma = e / (a0 - c);
+  const double s1 = sigma;
+  const int k = Xb.c;
+  Mat X = Xb, Y(Xb.r, k), Yn(Xb.r, k);
+  spmm(A, X, 0, k, Y, sign * s1 / e, 0.0);
+  for (size_t t = 0; t < Y.a.size(); ++t) Y.a[t] -= (s1 / e) * c * X.a[t];
+  for (int deg = 2; deg <= degree; ++deg) {
+    const double sn = 1.0 / (2.0 / s1 - sigma);
+    Yn = X;
+    spmm(A, Y, 0, k, Yn, sign * 2.0 * sn / e, -sigma * sn);
+    for (size_t t = 0; t < Yn.a.size(); ++t) Yn.a[t] -= (2.0 * sn / e) * c * Y.a[t];
+    std::swap(X, Y);
+    std::swap(Y, Yn);
+    sigma = sn;
+  }
+  Xb = std::move(Y);
+}
+
 // ---------------------------------------------------------------- solvers
 
 struct Result {
@@ -666,7 +692,7 @@ struct Result {
 // Reference: src/ChebyshevSolver.cpp:67-221. Extension: block > nev sets the
 // subspace capacity (the reference uses nev + min(nev, n - nev)).
 Result chebyshev(const Csr& A, int nev, int block, int degree, int maxiter, double tol,
-                 const std::string& which, unsigned seed) {
+                 const std::string& which, unsigned seed, bool ritz_bounds = false) {
   const int n = A.n;
   if (n <= 0 || nev <= 0 || nev > n) throw std::invalid_argument("ChebyshevSolver: nev must be in [1,n].");
   if (which != "sr" && which != "lr") throw std::invalid_argument("ChebyshevSolver: only SR and LR are supported.");
@@ -675,9 +701,10 @@ Result chebyshev(const Csr& A, int nev, int block, int degree, int maxiter, doub
   Mat vectors = random_block(n, capacity, seed);
   Mat Rq;
   if (cholesky_qr(vectors, Rq) != 0) throw std::runtime_error("ChebyshevSolver: initial Cholesky QR failed.");
-  int locked = 0, subspace_cols = nev;
+  int locked = 0, subspace_cols = ritz_bounds ? capacity : nev;
   double lower, upper;
   lanczos_bounds(A, sign, seed, lower, upper);
+  double cutoff = std::numeric_limits<double>::quiet_NaN();  // adaptive bound (ritz_bounds)
   Ritz ritz;
   Mat ritz_vectors;
   Result res;
@@ -685,7 +712,10 @@ Result chebyshev(const Csr& A, int nev, int block, int degree, int maxiter, doub
     const int active = subspace_cols - locked;
     Mat filtered(n, active);
     copy_cols(filtered, 0, vectors, locked, active);
-    cheb_filter(A, sign, filtered, degree, lower, upper);
+    if (ritz_bounds && std::isfinite(cutoff) && cutoff > lower && cutoff < upper)
+      cheb_filter_scaled(A, sign, filtered, degree, lower, cutoff, upper);
+    else
+      cheb_filter(A, sign, filtered, degree, lower, upper);
     if (locked > 0) {
       Mat out, H;
       block_gram_schmidt(vectors, locked, filtered, out, H);
@@ -703,6 +733,7 @@ Result chebyshev(const Csr& A, int nev, int block, int degree, int maxiter, doub
       if (!std::isfinite(ritz.values[j]) || !std::isfinite(ritz.relres[j]))
         throw std::runtime_error("ChebyshevSolver: non-finite Ritz values or residuals.");
     copy_cols(vectors, 0, ritz_vectors, 0, subspace_cols);
+    cutoff = ritz.values[subspace_cols - 1];
     locked = 0;
     while (locked < nev && ritz.relres[locked] <= tol) ++locked;
     subspace_cols = std::max(subspace_cols, nev + std::min(locked, capacity - nev));
@@ -1051,10 +1082,10 @@ int orc_cheb_filter(int n, const int* rp, const int* ci, const double* v, int k,
 }
 
 int orc_chebyshev(int n, const int* rp, const int* ci, const double* v, int nev, int block, int degree,
-                  int maxiter, double tol, const char* which, unsigned seed, double* ev, double* evec,
-                  double* res, long* iters) {
+                  int ritz_bounds, int maxiter, double tol, const char* which, unsigned seed, double* ev,
+                  double* evec, double* res, long* iters) {
   return guarded([&] {
-    auto r = orc::chebyshev(csr(n, rp, ci, v), nev, block, degree, maxiter, tol, which, seed);
+    auto r = orc::chebyshev(csr(n, rp, ci, v), nev, block, degree, maxiter, tol, which, seed, ritz_bounds != 0);
     return finish(r, nev, n, ev, evec, res, iters);
   });
 }

@@ -47,7 +47,7 @@ def lib():
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_cheb_filter.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_double]
         solver_tail = [C.c_double, C.c_char_p, C.c_uint, _dp, C.c_void_p, _dp, C.POINTER(C.c_long)]
-        L.orc_chebyshev.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int] + solver_tail
+        L.orc_chebyshev.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_lobpcg.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_davidson.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_residual_norms.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
@@ -179,8 +179,12 @@ def _solve(fn, A, nev, extra, maxiter, tol, which, seed, want_vectors):
             "status": status}
 
 
-def chebyshev(A, nev, block=0, degree=10, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True):
-    return _solve(lib().orc_chebyshev, A, nev, (block, degree), maxiter, tol, which, seed, want_vectors)
+def chebyshev(A, nev, block=0, degree=10, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True,
+              bounds="amplification"):
+    """bounds: "amplification" (the reference's fixed cutoff) or "ritz" (adaptive
+    cutoff at the largest Ritz value, scaled recurrence, full-capacity subspace)."""
+    return _solve(lib().orc_chebyshev, A, nev, (block, degree, 1 if bounds == "ritz" else 0), maxiter, tol,
+                  which, seed, want_vectors)
 
 
 def lobpcg(A, nev, block=0, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True):

@@ -126,17 +126,49 @@ def test_generalized_diagonal_mass_matches_dense_pencil():
 
 
 def test_mixed_precision_reaches_fp64_accuracy():
-    """FP32 filter/Gram cycles + FP64 closing sweeps give eigenvalues within
-    1e-10 of the plain FP64 path and FP64 residuals <= 1e-8."""
+    """FP32 filter/Gram cycles + FP64 closing cycles give eigenvalues within
+    1e-10 of the plain FP64 path and FP64 residuals <= 1e-8. Uses the adaptive
+    filter bounds: on this spectrum (one deep well state far below the rest)
+    the reference's fixed-amplification cutoff leaves every wanted eigenvalue
+    but the first in the damped interval and does not converge (the oracle
+    shows the same after 1000 cycles)."""
     A = G.hamiltonian_27pt(20, seed=4, wells=20)
     nev = 24
+    exact = np.linalg.eigvalsh(A.toarray())[:nev]
     p = Problem(A)
-    r64 = p.solve(solver="Chebyshev", nev=nev, degree=20, tol=1e-8, maxiter=500, which="SR")
+    base = dict(solver="Chebyshev", nev=nev, degree=20, tol=1e-8, maxiter=200, which="SR", cheb_bounds="ritz")
+    r64 = p.solve(**base)
+    assert r64.residual_norms.max() <= 1e-8
+    assert np.all(np.abs(r64.eigenvalues - exact) <= 1e-10 * np.abs(exact))
     for gram in ("native", "bf16x3"):
-        rm = p.solve(solver="Chebyshev", nev=nev, degree=20, tol=1e-8, maxiter=500, which="SR", precision="mixed",
-                     gram=gram)
+        rm = p.solve(precision="mixed", gram=gram, **base)
         assert rm.residual_norms.max() <= 1e-8
         assert np.all(np.abs(rm.eigenvalues - r64.eigenvalues) <= 1e-10 * np.abs(r64.eigenvalues))
+
+
+@pytest.mark.parametrize("which", ["SR", "LR"])
+def test_chebyshev_ritz_bounds_matches_oracle(which):
+    A = G.hamiltonian_27pt(16, seed=4, wells=10)
+    nev = 12
+    o = orc.chebyshev(A, nev, degree=16, maxiter=300, tol=1e-11, which=which.lower(), seed=2, bounds="ritz")
+    p = Problem(A)
+    r = p.solve(solver="Chebyshev", nev=nev, degree=16, tol=1e-11, maxiter=300, which=which, seed=2,
+                cheb_bounds="ritz")
+    assert np.all(np.abs(r.eigenvalues - o["eigenvalues"]) <= 1e-10 * np.abs(o["eigenvalues"]))
+    assert r.residual_norms.max() <= 2e-11
+    gap = 1e-8 * np.abs(r.eigenvalues).max()
+    assert subspace_angle_max(p.eigenvectors(), o["eigenvectors"], r.eigenvalues, gap) <= 1e-8
+
+
+def test_reference_bounds_follow_oracle_when_unconverged():
+    """With the reference's fixed cutoff on a spectrum it cannot resolve, the
+    HIP path stops at maxiter like the oracle and reports the same converged
+    first pair (parity of the failure mode, not just of successes)."""
+    A = G.hamiltonian_27pt(12, seed=4, wells=6)
+    o = orc.chebyshev(A, 6, degree=20, maxiter=40, tol=1e-8, which="sr", seed=1, want_vectors=False)
+    r = Problem(A).solve(solver="Chebyshev", nev=6, degree=20, tol=1e-8, maxiter=40, which="SR", seed=1)
+    assert r.iterations == o["iterations"] == 40
+    assert abs(r.eigenvalues[0] - o["eigenvalues"][0]) <= 1e-10 * abs(o["eigenvalues"][0])
 
 
 def test_sell_format_same_result():
