@@ -48,13 +48,19 @@ def build_matrix(cfg):
     return G.hamiltonian_27pt(32, seed=4, wells=30), dict(nev=32, degree=20, tol=1e-8, blksize=0)
 
 
-def spmm_roofline(A, k, prec, reps=20):
+def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
     """Average time of the fused Chebyshev step Y = 2c A X + 2e X - Z on the
-    library stream (HIP events), and its algorithmic-byte bandwidth."""
+    library stream (HIP events), and its algorithmic-byte bandwidth. fmt
+    "blocked" times the row-block gather copy (LDS-staged X rows)."""
     from fasteig_amd import _fe
     L = _fe.lib()
     dev = _fe.Device(0)
     C = dev.csr(A)
+    if fmt == "blocked":
+        built, reuse = C.build_blocked(32)
+        if not built:
+            raise RuntimeError(f"row-block copy not built (reuse {reuse:.2f})")
+        C.use(_fe.FMT_BLOCKED)
     n = A.shape[0]
     dt = np.float64 if prec == _fe.F64 else np.float32
     s = np.dtype(dt).itemsize
@@ -81,6 +87,28 @@ def spmm_roofline(A, k, prec, reps=20):
     return per, algo
 
 
+def copy_reference(nbytes=1 << 30, reps=5):
+    """A plain device copy of nbytes (fe_copy2d, FP32) used to calibrate the
+    FETCH_SIZE / WRITE_SIZE counters: it reads and writes exactly nbytes."""
+    import ctypes
+    from fasteig_amd import _fe
+    L = _fe.lib()
+    dev = _fe.Device(0)
+    rows = nbytes // (4 * 256)
+    a = dev.buffer((rows, 256), np.float32)
+    b = dev.buffer((rows, 256), np.float32)
+    _fe.check(L.fe_zero2d(dev.ctx, a.ptr, rows, 256, 256, 4))
+    _fe.check(L.fe_event_record(dev.ctx, 2))
+    for _ in range(reps):
+        _fe.check(L.fe_copy2d(dev.ctx, rows, 256, a.ptr, _fe.F32, 256, b.ptr, _fe.F32, 256))
+    _fe.check(L.fe_event_record(dev.ctx, 3))
+    ms = ctypes.c_float()
+    _fe.check(L.fe_event_elapsed_ms(dev.ctx, 2, 3, ctypes.byref(ms)))
+    del a, b
+    dev.close()
+    return ms.value / reps, rows * 256 * 4
+
+
 def cpu_sample(A, k, target_s=15.0):
     """CPU oracle FP64 fused Chebyshev step (Y = 2c A X + 2e X - Z) on the host
     cores, on k columns; returns seconds per step at that k."""
@@ -105,8 +133,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c3", "c2", "small"])
     ap.add_argument("--gram", default="native", choices=["native", "bf16x3"])
+    ap.add_argument("--format", default="csr", choices=["auto", "csr", "blocked", "sell"],
+                    help="sparse format of the solve (auto: row-block gather when the matrix allows)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--only-kernels", action="store_true",
+                    help="time only the fused Chebyshev step and a reference copy (for counter runs)")
+    ap.add_argument("--kernel-formats", default="csr,blocked",
+                    help="sparse formats timed by --only-kernels (csr, blocked)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -117,6 +151,20 @@ def main():
     from fasteig_amd.api import Problem
 
     t0 = time.time()
+    if args.only_kernels:
+        M, cfg = build_matrix(args.config)
+        A = M[0] if isinstance(M, tuple) else M
+        out = {}
+        for fmt in args.kernel_formats.split(","):
+            for pname, prec in (("fp32", _fe.F32), ("fp64", _fe.F64)):
+                per, algo = spmm_roofline(A, cfg["nev"], prec, fmt=fmt)
+                out[f"{fmt}_{pname}_ms"] = per
+                out[f"{fmt}_{pname}_gbs"] = algo / (per * 1e-3) / 1e9
+                log(f"fused step {fmt} {pname} k={cfg['nev']}: {per:.3f} ms {out[f'{fmt}_{pname}_gbs']:.0f} GB/s")
+        copy_ms, copy_bytes = copy_reference()
+        out.update(copy_ms=copy_ms, copy_bytes_each_way=copy_bytes)
+        print(json.dumps(out), flush=True)
+        return
     M, cfg = build_matrix(args.config)
     mass = None
     if isinstance(M, tuple):
@@ -126,7 +174,7 @@ def main():
     log(f"matrix {args.config}: n={n} nnz={A.nnz} built in {time.time() - t0:.1f}s")
     p = Problem(A, mass=mass)
     base = dict(solver="Chebyshev", nev=cfg["nev"], degree=cfg["degree"], tol=cfg["tol"], which="SR",
-                maxiter=300, seed=1, cheb_bounds="ritz")
+                maxiter=300, seed=1, cheb_bounds="ritz", format=args.format)
     if cfg["blksize"]:
         base["blksize"] = cfg["blksize"]
     modes = {"mixed": dict(base, precision="mixed", gram=args.gram)}
@@ -203,7 +251,7 @@ def main():
         "config": {"workload": {"c3": "H=-0.5*L27+V 96^3 (n=884736), nev 256, Chebyshev degree 20, tol 1e-8",
                                 "c2": "K u = l M u 7-pt 64^3, nev 64, block 96, degree 16",
                                 "small": "H=-0.5*L27+V 32^3, nev 32, degree 20"}[args.config],
-                   "n": n, "nnz": int(A.nnz), "gram": args.gram},
+                   "n": n, "nnz": int(A.nnz), "gram": args.gram, "format": args.format},
         "latency_s": {m: {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99))}
                       for m, v in lat.items()},
         "accuracy": out_acc,

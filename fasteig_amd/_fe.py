@@ -8,6 +8,7 @@ is missing: there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -18,7 +19,7 @@ FE_SO = os.path.join(LIBDIR, "libfe_hip.so")
 
 F64, F32 = 0, 1
 GRAM_NATIVE, GRAM_BF16X3 = 0, 1
-FMT_CSR, FMT_SELL = 0, 1
+FMT_CSR, FMT_SELL, FMT_BLOCKED = 0, 1, 2
 
 _lib = None
 
@@ -54,6 +55,7 @@ def lib():
         L.fe_csr_scale_sym.argtypes = [vp, vp, vp]
         L.fe_csr_build_sell.argtypes = [vp, vp, i32, i32]
         L.fe_csr_use.argtypes = [vp, i32]
+        L.fe_csr_build_blocked.argtypes = [vp, vp, i32, C.POINTER(dbl)]
         L.fe_spmm.argtypes = [vp, vp, i32, i32, vp, i64, vp, i64, dbl, dbl, vp, i64, dbl]
         L.fe_gram.argtypes = [vp, i32, i32, i64, i32, i32, vp, i64, vp, i64, vp, vp, i64]
         L.fe_gemm_nn.argtypes = [vp, i32, i64, i32, i32, vp, i64, vp, i64, vp, i64, dbl, dbl]
@@ -112,7 +114,7 @@ class Buffer:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.dev.ctx:
             lib().fe_free(self.dev.ctx, self.ptr)
             self.ptr = None
 
@@ -135,18 +137,29 @@ class Csr:
         check(lib().fe_csr_create(dev.ctx, self.n, self.nnz, rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
                                   C.byref(h)))
         self.h = h.value
+        dev._matrices.add(self)
 
     def build_sell(self, C_=32, sigma=256):
         check(lib().fe_csr_build_sell(self.dev.ctx, self.h, C_, sigma))
 
+    def build_blocked(self, rb=32):
+        """Build the row-block gather copy; returns (built, reuse)."""
+        r = C.c_double(0.0)
+        rc = check(lib().fe_csr_build_blocked(self.dev.ctx, self.h, rb, C.byref(r)))
+        return rc == 0, r.value
+
     def use(self, fmt):
         check(lib().fe_csr_use(self.h, fmt))
 
+    def destroy(self):
+        """Release the device copy (must happen before its context dies)."""
+        if self.h:
+            lib().fe_csr_destroy(self.h)
+            self.h = None
+
     def __del__(self):
         try:
-            if self.h:
-                lib().fe_csr_destroy(self.h)
-                self.h = None
+            self.destroy()
         except Exception:
             pass
 
@@ -156,6 +169,9 @@ class Device:
         h = C.c_void_p()
         check(lib().fe_ctx_create(device, C.byref(h)))
         self.ctx = h.value
+        # matrices are synchronised on their context's stream when destroyed,
+        # so close() releases the live ones first
+        self._matrices = weakref.WeakSet()
 
     def sync(self):
         check(lib().fe_ctx_sync(self.ctx))
@@ -180,5 +196,7 @@ class Device:
 
     def close(self):
         if self.ctx:
+            for m in list(self._matrices):
+                m.destroy()
             lib().fe_ctx_destroy(self.ctx)
             self.ctx = None

@@ -55,6 +55,17 @@ struct fe_csr_s {
   double* sell_v64 = nullptr;
   float* sell_v32 = nullptr;
   int use_sell = 0;
+  // row-block gather copy (optional): blocks of bg_rb rows, the sorted union of
+  // their column indices (bg_ucol, offsets bg_bptr) and per-nonzero positions
+  // in that union (bg_lcol, 16-bit)
+  int bg_rb = 0;
+  int64_t bg_nblk = 0;
+  int bg_maxu = 0;
+  int64_t* bg_bptr = nullptr;
+  int* bg_ucol = nullptr;
+  uint16_t* bg_lcol = nullptr;
+  double bg_reuse = 0.0;      // nonzeros / union rows (gathers saved per staged row)
+  int use_bg = 0;
 };
 
 namespace fe {

@@ -1,8 +1,6 @@
 // Small dense FP64 kernels for the Rayleigh-Ritz and Cholesky-QR steps.
 // All matrices are column-major on the device.
 //
-//   k_potrf       Cholesky G = L L^T, one workgroup (right-looking, columns)
-//   k_chol_rinv   (L^T)^{-1}, one thread per column of L^{-1}
 //   k_syev_tri    one workgroup: Householder tridiagonalization (Golub & Van
 //                 Loan, Alg. 8.3.1), Sturm-count bisection for every eigenvalue
 //                 (GvL 8.4.1), inverse iteration for the eigenvectors of the
@@ -18,56 +16,6 @@
 namespace fe {
 
 constexpr int SYEV_MAX = 1024;  // largest projected matrix handled on the device
-
-__global__ __launch_bounds__(1024) void k_potrf(int k, double* G, int64_t ldg, int* info) {
-  __shared__ int fail;
-  __shared__ double djj;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  if (tid == 0) {
-    fail = 0;
-    info[0] = 0;
-  }
-  __syncthreads();
-  for (int j = 0; j < k; ++j) {
-    if (tid == 0) {
-      const double d = G[(int64_t)j * ldg + j];
-      if (!(d > 0.0) || !isfinite(d)) {
-        fail = j + 1;
-        info[0] = j + 1;
-      } else {
-        djj = sqrt(d);
-        G[(int64_t)j * ldg + j] = djj;
-      }
-    }
-    __syncthreads();
-    if (fail) return;
-    const double inv = 1.0 / djj;
-    for (int i = j + 1 + tid; i < k; i += nt) G[(int64_t)j * ldg + i] *= inv;
-    __syncthreads();
-    const int r = k - j - 1;
-    const int64_t tot = (int64_t)r * r;
-    for (int64_t t = tid; t < tot; t += nt) {
-      const int c = j + 1 + static_cast<int>(t / r);
-      const int i = j + 1 + static_cast<int>(t % r);
-      if (i >= c) G[(int64_t)c * ldg + i] -= G[(int64_t)j * ldg + i] * G[(int64_t)j * ldg + c];
-    }
-    __syncthreads();
-  }
-}
-
-// Rinv = (L^T)^{-1}: thread j solves L x = e_j (x_i = 0, i < j) and writes x
-// as row j of Rinv (Rinv(j, i) = x_i), which is upper triangular.
-__global__ void k_chol_rinv(int k, const double* __restrict__ L, int64_t ldl, double* Rinv, int64_t ldr) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= k) return;
-  for (int i = 0; i < j; ++i) Rinv[(int64_t)i * ldr + j] = 0.0;
-  Rinv[(int64_t)j * ldr + j] = 1.0 / L[(int64_t)j * ldl + j];
-  for (int i = j + 1; i < k; ++i) {
-    double s = 0.0;
-    for (int p = j; p < i; ++p) s += L[(int64_t)p * ldl + i] * Rinv[(int64_t)p * ldr + j];
-    Rinv[(int64_t)i * ldr + j] = -s / L[(int64_t)i * ldl + i];
-  }
-}
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
@@ -105,7 +53,11 @@ __device__ __forceinline__ double hash_unit(uint32_t a, uint32_t b) {
 
 // work layout (doubles): u1, u2, u3, ml, piv, Z  -- each m*m, interleaved [i * m + t]
 __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda, double* W, double* tau,
-                                                   double* work, int* info) {
+                                                   double* work, int* info, const double* __restrict__ dg,
+                                                   const double* __restrict__ eg, int phase) {
+  // phase 0: everything in one workgroup; with a precomputed tridiagonal (dg, eg)
+  // phase 1 runs the bisection and phase 2 the inverse iteration, each spread
+  // over the launch's workgroups (phase 2 reads the eigenvalues back from W).
   __shared__ double vs[SYEV_MAX], ps[SYEV_MAX], wsv[SYEV_MAX];
   __shared__ double d[SYEV_MAX], e2[SYEV_MAX], lam[SYEV_MAX];
   __shared__ double red[16];
@@ -113,71 +65,80 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
   __shared__ int s_skip;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-  if (tid == 0) info[0] = 0;
+  if (tid == 0 && blockIdx.x == 0 && phase != 2) info[0] = 0;
 
-  // symmetrize from the lower triangle
-  for (int64_t t = tid; t < (int64_t)m * m; t += nt) {
-    const int j = static_cast<int>(t / m), i = static_cast<int>(t % m);
-    if (i < j) A[(int64_t)j * lda + i] = A[(int64_t)i * lda + j];
-  }
-  __syncthreads();
+  if (dg) {
+    // tridiagonal already formed by the multi-workgroup reduction
+    for (int i = tid; i < m; i += nt) {
+      d[i] = dg[i];
+      if (i + 1 < m) e2[i] = eg[i];
+    }
+    __syncthreads();
+  } else {
+    // symmetrize from the lower triangle
+    for (int64_t t = tid; t < (int64_t)m * m; t += nt) {
+      const int j = static_cast<int>(t / m), i = static_cast<int>(t % m);
+      if (i < j) A[(int64_t)j * lda + i] = A[(int64_t)i * lda + j];
+    }
+    __syncthreads();
 
-  // ---- 1. Householder tridiagonalization
-  for (int k = 0; k + 2 < m; ++k) {
-    const int len = m - k - 1;
-    double* x = A + (int64_t)k * lda + (k + 1);
-    double part = 0.0;
-    for (int i = 1 + tid; i < len; i += nt) part += x[i] * x[i];
-    const double xn2 = block_sum(part, red);
-    if (tid == 0) {
-      const double alpha = x[0];
-      if (xn2 == 0.0) {
-        s_skip = 1;
-        tau[k] = 0.0;
-        e2[k] = alpha;  // store e temporarily (squared later)
-      } else {
-        s_skip = 0;
-        const double beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-        s_tau = (beta - alpha) / beta;
-        s_scale = 1.0 / (alpha - beta);
-        tau[k] = s_tau;
-        e2[k] = beta;
+    // ---- 1. Householder tridiagonalization
+    for (int k = 0; k + 2 < m; ++k) {
+      const int len = m - k - 1;
+      double* x = A + (int64_t)k * lda + (k + 1);
+      double part = 0.0;
+      for (int i = 1 + tid; i < len; i += nt) part += x[i] * x[i];
+      const double xn2 = block_sum(part, red);
+      if (tid == 0) {
+        const double alpha = x[0];
+        if (xn2 == 0.0) {
+          s_skip = 1;
+          tau[k] = 0.0;
+          e2[k] = alpha;  // store e temporarily (squared later)
+        } else {
+          s_skip = 0;
+          const double beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+          s_tau = (beta - alpha) / beta;
+          s_scale = 1.0 / (alpha - beta);
+          tau[k] = s_tau;
+          e2[k] = beta;
+        }
       }
+      __syncthreads();
+      if (s_skip) continue;
+      const double t_k = s_tau, sc = s_scale;
+      for (int i = tid; i < len; i += nt) {
+        const double v = i == 0 ? 1.0 : x[i] * sc;
+        vs[i] = v;
+      }
+      __syncthreads();
+      for (int i = tid; i < len; i += nt) x[i] = vs[i];  // keep v for the back-transformation
+      // p = tau * A22 v : one wavefront per column of A22
+      for (int j = wave; j < len; j += nw) {
+        const double* col = A + (int64_t)(k + 1 + j) * lda + (k + 1);
+        double s = 0.0;
+        for (int i = lane; i < len; i += 64) s += col[i] * vs[i];
+        s = wave_sum(s);
+        if (lane == 0) ps[j] = t_k * s;
+      }
+      __syncthreads();
+      double pv = 0.0;
+      for (int i = tid; i < len; i += nt) pv += ps[i] * vs[i];
+      const double K = 0.5 * t_k * block_sum(pv, red);
+      for (int i = tid; i < len; i += nt) wsv[i] = ps[i] - K * vs[i];
+      __syncthreads();
+      for (int j = wave; j < len; j += nw) {
+        double* col = A + (int64_t)(k + 1 + j) * lda + (k + 1);
+        const double vj = vs[j], wj = wsv[j];
+        for (int i = lane; i < len; i += 64) col[i] -= vs[i] * wj + wsv[i] * vj;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (s_skip) continue;
-    const double t_k = s_tau, sc = s_scale;
-    for (int i = tid; i < len; i += nt) {
-      const double v = i == 0 ? 1.0 : x[i] * sc;
-      vs[i] = v;
-    }
-    __syncthreads();
-    for (int i = tid; i < len; i += nt) x[i] = vs[i];  // keep v for the back-transformation
-    // p = tau * A22 v : one wavefront per column of A22
-    for (int j = wave; j < len; j += nw) {
-      const double* col = A + (int64_t)(k + 1 + j) * lda + (k + 1);
-      double s = 0.0;
-      for (int i = lane; i < len; i += 64) s += col[i] * vs[i];
-      s = wave_sum(s);
-      if (lane == 0) ps[j] = t_k * s;
-    }
-    __syncthreads();
-    double pv = 0.0;
-    for (int i = tid; i < len; i += nt) pv += ps[i] * vs[i];
-    const double K = 0.5 * t_k * block_sum(pv, red);
-    for (int i = tid; i < len; i += nt) wsv[i] = ps[i] - K * vs[i];
-    __syncthreads();
-    for (int j = wave; j < len; j += nw) {
-      double* col = A + (int64_t)(k + 1 + j) * lda + (k + 1);
-      const double vj = vs[j], wj = wsv[j];
-      for (int i = lane; i < len; i += 64) col[i] -= vs[i] * wj + wsv[i] * vj;
-    }
+    // diagonal and off-diagonal of T
+    for (int i = tid; i < m; i += nt) d[i] = A[(int64_t)i * lda + i];
+    if (tid == 0 && m >= 2) e2[m - 2] = A[(int64_t)(m - 2) * lda + (m - 1)];
     __syncthreads();
   }
-  // diagonal and off-diagonal of T
-  for (int i = tid; i < m; i += nt) d[i] = A[(int64_t)i * lda + i];
-  if (tid == 0 && m >= 2) e2[m - 2] = A[(int64_t)(m - 2) * lda + (m - 1)];
-  __syncthreads();
   // e2 currently holds e; Gershgorin bounds and squares
   if (tid == 0) {
     double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0;
@@ -206,7 +167,12 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
 
   // ---- 2. bisection: thread t finds the t-th smallest eigenvalue
   const double pivmin = s_pivmin, onenrm = s_onenrm;
-  for (int t = tid; t < m; t += nt) {
+  const int t0 = blockIdx.x * nt + tid, tstride = nt * gridDim.x;
+  if (phase == 2) {
+    for (int i = tid; i < m; i += nt) lam[i] = W[i];
+    __syncthreads();
+  }
+  for (int t = t0; phase != 2 && t < m; t += tstride) {
     double lo = s_gl, hi = s_gu;
     for (int it = 0; it < 200; ++it) {
       const double mid = 0.5 * (lo + hi);
@@ -218,6 +184,7 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
     W[t] = lam[t];
   }
   __syncthreads();
+  if (phase == 1) return;
 
   // ---- 3. inverse iteration (cluster leaders process their cluster in order)
   // Cluster threshold for reorthogonalization. LAPACK dstein uses 1e-3*||T||;
@@ -232,7 +199,7 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
   double* ml = work + 3 * MM;
   double* pv_ = work + 4 * MM;
   double* Z = work + 5 * MM;
-  for (int t = tid; t < m; t += nt) {
+  for (int t = t0; t < m; t += tstride) {
     if (t > 0 && lam[t] - lam[t - 1] <= ortol) continue;  // not a leader
     int last = t;
     while (last + 1 < m && lam[last + 1] - lam[last] <= ortol) ++last;
@@ -320,6 +287,107 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
   }
 }
 
+// ---- multi-workgroup Householder tridiagonalization (one launch per column)
+// Launch k applies the deferred rank-2 update of step k-1 to the workgroup's
+// columns of the trailing matrix, recomputes the updated column k (every
+// workgroup redundantly: O(m) work), forms the reflector of column k and
+// accumulates p_k = tau_k A22 v_k for its columns in the same pass over the
+// matrix. Reflector k is kept in Vh[k * m + i], i > k (Vh[k * m + k + 1] = 1).
+__device__ __forceinline__ double wg_sum256(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_tri_step(int m, int k, double* A, int64_t lda, double* Vh, double* tau,
+                                                  double* dd, double* ee, const double* __restrict__ pprev,
+                                                  double* pcur) {
+  __shared__ double vp[SYEV_MAX], wp[SYEV_MAX], col[SYEV_MAX], vk[SYEV_MAX];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool upd = k > 0;
+  double tprev = 0.0;
+  if (upd) {
+    // v_{k-1} lives on rows k.. ; w_{k-1} = p_{k-1} - (tau/2)(p^T v) v
+    tprev = tau[k - 1];
+    double part = 0.0;
+    for (int i = k + tid; i < m; i += 256) {
+      const double v = (i == k) ? 1.0 : Vh[(int64_t)(k - 1) * m + i];
+      vp[i] = v;
+      part += pprev[i] * v;
+    }
+    const double K = 0.5 * tprev * wg_sum256(part, red);
+    for (int i = k + tid; i < m; i += 256) wp[i] = pprev[i] - K * vp[i];
+    __syncthreads();
+  }
+  // updated column k (rows k..m-1)
+  for (int i = k + tid; i < m; i += 256) {
+    double x = A[(int64_t)k * lda + i];
+    if (upd) x -= vp[i] * wp[k] + wp[i] * vp[k];
+    col[i] = x;
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (int i = k + 2 + tid; i < m; i += 256) part += col[i] * col[i];
+  const double xn2 = wg_sum256(part, red);
+  const double alpha = col[k + 1];
+  double tk = 0.0, sc = 0.0, beta = alpha;
+  if (xn2 != 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+    tk = (beta - alpha) / beta;
+    sc = 1.0 / (alpha - beta);
+  }
+  for (int i = k + 1 + tid; i < m; i += 256) vk[i] = (i == k + 1) ? 1.0 : (tk != 0.0 ? col[i] * sc : 0.0);
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    if (tid == 0) {
+      tau[k] = tk;
+      dd[k] = col[k];
+      ee[k] = beta;
+    }
+    for (int i = k + 1 + tid; i < m; i += 256) Vh[(int64_t)k * m + i] = vk[i];
+  }
+  // columns j > k: apply the update of step k-1 (rows > k) and form p_k[j]
+  const int nwaves = gridDim.x * 4;
+  for (int j = k + 1 + blockIdx.x * 4 + wave; j < m; j += nwaves) {
+    double* cj = A + (int64_t)j * lda;
+    double s = 0.0;
+    for (int i = k + 1 + lane; i < m; i += 64) {
+      double a = cj[i];
+      if (upd) {
+        a -= vp[i] * wp[j] + wp[i] * vp[j];
+        cj[i] = a;
+      }
+      s += a * vk[i];
+    }
+    s = wave_sum(s);
+    if (lane == 0) pcur[j] = tk * s;
+  }
+}
+
+// Last 2 x 2 block: apply the final deferred update and read d, e.
+__global__ void k_tri_finish(int m, double* A, int64_t lda, const double* __restrict__ Vh,
+                             const double* __restrict__ tau, double* dd, double* ee, const double* __restrict__ plast) {
+  if (threadIdx.x != 0) return;
+  const int k = m - 2;  // rows/cols k, k+1 remain
+  double a00 = A[(int64_t)k * lda + k], a10 = A[(int64_t)k * lda + k + 1], a11 = A[(int64_t)(k + 1) * lda + k + 1];
+  if (m >= 3) {
+    const double t = tau[k - 1];
+    const double v0 = 1.0, v1 = Vh[(int64_t)(k - 1) * m + k + 1];
+    const double K = 0.5 * t * (plast[k] * v0 + plast[k + 1] * v1);
+    const double w0 = plast[k] - K * v0, w1 = plast[k + 1] - K * v1;
+    a00 -= 2.0 * v0 * w0;
+    a10 -= v1 * w0 + w1 * v0;
+    a11 -= 2.0 * v1 * w1;
+  }
+  dd[k] = a00;
+  dd[k + 1] = a11;
+  ee[k] = a10;
+}
+
 // V(:, cols) = H_0 ... H_{m-3} Z(:, cols); Householder vectors in A's lower part.
 // Each workgroup owns CB columns of Z, held in LDS as [row][CB].
 __global__ __launch_bounds__(256) void k_syev_back(int m, const double* __restrict__ A, int64_t lda,
@@ -359,42 +427,51 @@ __global__ __launch_bounds__(256) void k_syev_back(int m, const double* __restri
 
 using namespace fe;
 
-extern "C" int fe_potrf(fe_ctx ctx, int k, double* G, int64_t ldg) {
-  FE_ARG(ctx && G && k >= 0 && ldg >= k, "fe_potrf: bad argument");
-  if (k == 0) return 0;
-  hipLaunchKernelGGL(k_potrf, dim3(1), dim3(1024), 0, ctx->stream, k, G, ldg, ctx->info_dev);
-  FE_CHECK_LAUNCH();
-  int info = 0;
-  FE_HIP_TRY(hipMemcpyAsync(&info, ctx->info_dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  return info;
-}
-
-extern "C" int fe_chol_rinv(fe_ctx ctx, int k, const double* L, int64_t ldl, double* Rinv, int64_t ldr) {
-  FE_ARG(ctx && L && Rinv && k >= 0 && ldl >= k && ldr >= k, "fe_chol_rinv: bad argument");
-  if (k == 0) return 0;
-  hipLaunchKernelGGL(k_chol_rinv, dim3(blocks_for(k, 64)), dim3(64), 0, ctx->stream, k, L, ldl, Rinv, ldr);
-  FE_CHECK_LAUNCH();
-  return 0;
-}
-
 extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, double* V, int64_t ldv) {
   FE_ARG(ctx && A && W && V && m >= 0 && lda >= m && ldv >= m, "fe_syev: bad argument");
   FE_ARG(m <= SYEV_MAX, "fe_syev: matrix larger than 1024 is not supported");
   if (m == 0) return 0;
   const int64_t MM = (int64_t)m * m;
   void* ws = nullptr;
-  int rc = scratch(ctx, 1, sizeof(double) * (6 * MM + m + 64), &ws);
+  // tau | d | e | p (2 m) | Vh (m*m) | work (6 m*m)
+  int rc = scratch(ctx, 1, sizeof(double) * (7 * MM + 5 * (int64_t)m + 64), &ws);
   if (rc) return rc;
   double* tau = (double*)ws;
-  double* work = tau + m + 64;
-  hipLaunchKernelGGL(k_syev_tri, dim3(1), dim3(1024), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev);
+  double* dd = tau + m;
+  double* ee = dd + m;
+  double* pb = ee + m;
+  double* Vh = pb + 2 * m;
+  double* work = Vh + MM;
+  const bool multi = m > 128;  // small matrices stay in one workgroup
+  if (multi) {
+    rc = fe_sym_small(ctx, m, A, lda);
+    if (rc) return rc;
+    const int G = 64;
+    for (int k = 0; k + 2 < m; ++k) {
+      const double* pprev = pb + ((k + 1) & 1) * m;
+      double* pcur = pb + (k & 1) * m;
+      hipLaunchKernelGGL(k_tri_step, dim3(G), dim3(256), 0, ctx->stream, m, k, A, lda, Vh, tau, dd, ee, pprev,
+                         pcur);
+    }
+    hipLaunchKernelGGL(k_tri_finish, dim3(1), dim3(64), 0, ctx->stream, m, A, lda, Vh, tau, dd, ee,
+                       pb + ((m - 3) & 1) * m);
+    FE_CHECK_LAUNCH();
+  }
+  if (multi) {
+    const unsigned g = blocks_for(m, 64);
+    hipLaunchKernelGGL(k_syev_tri, dim3(g), dim3(64), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev, dd, ee, 1);
+    hipLaunchKernelGGL(k_syev_tri, dim3(g), dim3(64), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev, dd, ee, 2);
+  } else {
+    hipLaunchKernelGGL(k_syev_tri, dim3(1), dim3(1024), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev,
+                       nullptr, nullptr, 0);
+  }
   FE_CHECK_LAUNCH();
   int CB = 32;  // columns per workgroup, LDS slice m*CB doubles within 64 KiB
   while (CB > 1 && (size_t)m * CB * sizeof(double) > 65536) CB >>= 1;
   const size_t lds = sizeof(double) * (size_t)m * CB;
-  hipLaunchKernelGGL(k_syev_back, dim3(blocks_for(m, CB)), dim3(256), lds, ctx->stream, m, A, lda, tau,
-                     work + 5 * MM, V, ldv, CB);
+  hipLaunchKernelGGL(k_syev_back, dim3(blocks_for(m, CB)), dim3(256), lds, ctx->stream, m,
+                     multi ? (const double*)Vh : (const double*)A, multi ? (int64_t)m : lda, tau, work + 5 * MM, V,
+                     ldv, CB);
   FE_CHECK_LAUNCH();
   int info = 0;
   FE_HIP_TRY(hipMemcpyAsync(&info, ctx->info_dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

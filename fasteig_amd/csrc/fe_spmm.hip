@@ -97,98 +97,140 @@ __device__ __forceinline__ T bcast_val(T v, int src_lane) {
   }
 }
 
-// VEC elements per lane, LPR lanes per row, RS vectors per lane per row
-// (RS > 1 lets one lane carry 2 vectors so a wave covers 2*64*VEC columns).
-template <typename T, int VEC, int LPR, int RS>
+template <typename T, int RS>
+__device__ __forceinline__ void ld_runs(const T* p, int stride, bool full, int c, int k, T (&v)[RS][16 / sizeof(T)]) {
+  constexpr int V = 16 / sizeof(T);
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int cr = c + r * stride;
+    if (full || cr + V <= k) {
+      const float4 f = *reinterpret_cast<const float4*>(p + r * stride);
+      __builtin_memcpy(&v[r][0], &f, 16);
+    } else {
+#pragma unroll
+      for (int e = 0; e < V; ++e) v[r][e] = (cr + e < k) ? p[r * stride + e] : T(0);
+    }
+  }
+}
+
+// CSR kernel, one row per group of LPR lanes, RS 16-byte runs per lane spaced
+// LPR * 16 bytes apart (each load instruction of the wavefront reads one
+// contiguous 16 * LPR-byte stretch of the X row). Row tiles map to XCDs in
+// contiguous ranges. Scalar variant (V16 = 1 element runs) for unaligned blocks.
+template <typename T, int LPR, int RS, bool VECTOR>
 __global__ __launch_bounds__(256) void k_spmm_csr(
     int n, const int* __restrict__ rowptr, const int* __restrict__ colind, const T* __restrict__ vals,
     int k, const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
     const T* Z, int64_t ldz, T gamma, int64_t per_xcd, int64_t ntiles) {
-  constexpr int RPW = 64 / LPR;          // rows per wavefront
-  constexpr int RPB = 4 * RPW;           // rows per workgroup (4 waves)
-  constexpr int COLS = LPR * VEC * RS;   // columns per column tile
+  constexpr int V = VECTOR ? 16 / (int)sizeof(T) : 1;
+  constexpr int RPW = 64 / LPR;
+  constexpr int RPB = 4 * RPW;
+  constexpr int STRIDE = LPR * V;        // columns between a lane's runs
+  constexpr int COLS = STRIDE * RS;      // columns per column tile
+  constexpr int UNR = RS == 1 ? 8 : 4;
   const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
   if (tile >= ntiles) return;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t row = tile * RPB + wave * RPW + lane / LPR;
   const int sub = lane % LPR;
-  const int gbase = lane - sub;          // first lane of this row group
-  const int c0 = blockIdx.y * COLS + sub * VEC;
+  const int gbase = lane - sub;
+  const int c = blockIdx.y * COLS + sub * V;
+  // the lane's runs are all complete (no column tail) -> unpredicated loads
+  const bool full = c + (RS - 1) * STRIDE + V <= k;
   const bool rowok = row < n;
   const int beg = rowok ? rowptr[row] : 0;
   const int end = rowok ? rowptr[row + 1] : 0;
-
-  T acc[RS][VEC];
+  T acc[RS][V == 1 ? 1 : V];
 #pragma unroll
   for (int r = 0; r < RS; ++r)
 #pragma unroll
-    for (int v = 0; v < VEC; ++v) acc[r][v] = T(0);
-  bool colok[RS];
-#pragma unroll
-  for (int r = 0; r < RS; ++r) colok[r] = c0 + r * LPR * VEC < k;
-
+    for (int e = 0; e < V; ++e) acc[r][e] = T(0);
   for (int p = beg; p < end; p += LPR) {
     const int q = p + sub;
     const int mycol = q < end ? colind[q] : 0;
     const T myval = q < end ? vals[q] : T(0);
     const int cnt = end - p < LPR ? end - p : LPR;
     int t = 0;
-    for (; t + 4 <= cnt; t += 4) {
-      int cc[4];
-      T vv[4];
+    for (; t + UNR <= cnt; t += UNR) {
+      int cc[UNR];
+      T vv[UNR];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < UNR; ++u) {
         cc[u] = bcast_int<LPR>(mycol, gbase + t + u);
         vv[u] = bcast_val<LPR>(myval, gbase + t + u);
       }
-      T xv[4][RS][VEC];
+      T xv[UNR][RS][V == 1 ? 1 : V];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < UNR; ++u) {
+        const T* xp = X + (int64_t)cc[u] * ldx + c;
+        if constexpr (VECTOR) {
+          ld_runs<T, RS>(xp, STRIDE, full, c, k, xv[u]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < RS; ++r) xv[u][r][0] = (c + r * STRIDE < k) ? xp[r * STRIDE] : T(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u)
 #pragma unroll
         for (int r = 0; r < RS; ++r)
-          if (colok[r]) load_vec<T, VEC>(X + (int64_t)cc[u] * ldx + c0 + r * LPR * VEC, xv[u][r]);
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int r = 0; r < RS; ++r)
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) acc[r][v] += vv[u] * xv[u][r][v];
+          for (int e = 0; e < V; ++e) acc[r][e] += vv[u] * xv[u][r][e];
     }
     for (; t < cnt; ++t) {
       const int cc = bcast_int<LPR>(mycol, gbase + t);
       const T vv = bcast_val<LPR>(myval, gbase + t);
+      const T* xp = X + (int64_t)cc * ldx + c;
+      T xv[RS][V == 1 ? 1 : V];
+      if constexpr (VECTOR) {
+        ld_runs<T, RS>(xp, STRIDE, full, c, k, xv);
+      } else {
+#pragma unroll
+        for (int r = 0; r < RS; ++r) xv[r][0] = (c + r * STRIDE < k) ? xp[r * STRIDE] : T(0);
+      }
 #pragma unroll
       for (int r = 0; r < RS; ++r)
-        if (colok[r]) {
-          T xv[VEC];
-          load_vec<T, VEC>(X + (int64_t)cc * ldx + c0 + r * LPR * VEC, xv);
 #pragma unroll
-          for (int v = 0; v < VEC; ++v) acc[r][v] += vv * xv[v];
-        }
+        for (int e = 0; e < V; ++e) acc[r][e] += vv * xv[r][e];
     }
   }
   if (!rowok) return;
 #pragma unroll
   for (int r = 0; r < RS; ++r) {
-    if (!colok[r]) continue;
-    const int c = c0 + r * LPR * VEC;
-    T out[VEC];
+    const int cr = c + r * STRIDE;
+    if (cr >= k) continue;
+    T out[V == 1 ? 1 : V];
 #pragma unroll
-    for (int v = 0; v < VEC; ++v) out[v] = alpha * acc[r][v];
-    if (beta != T(0)) {
-      T xv[VEC];
-      load_vec<T, VEC>(X + row * ldx + c, xv);
+    for (int e = 0; e < V; ++e) out[e] = alpha * acc[r][e];
+    if (VECTOR && cr + V <= k) {
+      if (beta != T(0)) {
+        const float4 f = *reinterpret_cast<const float4*>(X + row * ldx + cr);
+        T xv[V];
+        __builtin_memcpy(xv, &f, 16);
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) out[v] += beta * xv[v];
+        for (int e = 0; e < V; ++e) out[e] += beta * xv[e];
+      }
+      if (Z && gamma != T(0)) {
+        const float4 f = *reinterpret_cast<const float4*>(Z + row * ldz + cr);
+        T zv[V];
+        __builtin_memcpy(zv, &f, 16);
+#pragma unroll
+        for (int e = 0; e < V; ++e) out[e] += gamma * zv[e];
+      }
+      float4 o;
+      __builtin_memcpy(&o, out, 16);
+      *reinterpret_cast<float4*>(Y + row * ldy + cr) = o;
+    } else {
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        if (cr + e >= k) continue;
+        T o = out[e];
+        if (beta != T(0)) o += beta * X[row * ldx + cr + e];
+        if (Z && gamma != T(0)) o += gamma * Z[row * ldz + cr + e];
+        Y[row * ldy + cr + e] = o;
+      }
     }
-    if (Z && gamma != T(0)) {
-      T zv[VEC];
-      load_vec<T, VEC>(Z + row * ldz + c, zv);
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) out[v] += gamma * zv[v];
-    }
-    store_vec<T, VEC>(Y + row * ldy + c, out);
   }
 }
 
@@ -259,17 +301,17 @@ __global__ __launch_bounds__(256) void k_spmm_sell(
   store_vec<T, VEC>(Y + row * ldy + c, out);
 }
 
-template <typename T, int VEC, int LPR, int RS>
+template <typename T, int LPR, int RS, bool VECTOR>
 static int launch_csr(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha,
                       T beta, const T* Z, int64_t ldz, T gamma, const T* vals) {
+  constexpr int V = VECTOR ? 16 / (int)sizeof(T) : 1;
   constexpr int RPB = 4 * (64 / LPR);
-  constexpr int COLS = LPR * VEC * RS;
+  constexpr int COLS = LPR * V * RS;
   const int64_t ntiles = (A->n + RPB - 1) / RPB;
   const int64_t nb8 = (ntiles + 7) / 8 * 8;
   const unsigned ycols = static_cast<unsigned>((k + COLS - 1) / COLS);
-  hipLaunchKernelGGL((k_spmm_csr<T, VEC, LPR, RS>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream,
-                     A->n, A->rowptr, A->colind, vals, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8,
-                     ntiles);
+  hipLaunchKernelGGL((k_spmm_csr<T, LPR, RS, VECTOR>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream, A->n,
+                     A->rowptr, A->colind, vals, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles);
   FE_CHECK_LAUNCH();
   return 0;
 }
@@ -289,6 +331,10 @@ static int launch_sell(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* 
   return 0;
 }
 
+// row-block gather kernel (fe_spmm_bg.hip); needs 16-byte aligned rows
+int spmm_blocked(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                 double alpha, double beta, const void* Z, int64_t ldz, double gamma);
+
 // Choose lanes-per-row so that one row group spans the k columns with
 // 16-byte vectors when the layout allows it.
 template <typename T>
@@ -300,6 +346,7 @@ static int dispatch(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, 
   };
   const bool vec = (k % V16 == 0) && aligned(X, ldx) && aligned(Y, ldy) && (!Z || aligned(Z, ldz));
   const T alpha = static_cast<T>(a), beta = static_cast<T>(b), gamma = static_cast<T>(g);
+  if (A->use_bg && vec) return spmm_blocked(ctx, A, prec_of<T>::value, k, X, ldx, Y, ldy, a, b, Z, ldz, g);
   const T* v = A->use_sell ? nullptr : (sizeof(T) == 8 ? (const T*)A->vals64 : (const T*)A->vals32);
   if (A->use_sell) {
     const T* sv = sizeof(T) == 8 ? (const T*)A->sell_v64 : (const T*)A->sell_v32;
@@ -315,24 +362,30 @@ static int dispatch(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, 
     if (groups <= 32) return launch_sell<T, 1, 32>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
     return launch_sell<T, 1, 64>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, sv);
   }
-  if (vec) {
-    const int groups = k / V16;
-    if (groups <= 1) return launch_csr<T, V16, 1, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-    if (groups <= 2) return launch_csr<T, V16, 2, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-    if (groups <= 4) return launch_csr<T, V16, 4, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-    if (groups <= 8) return launch_csr<T, V16, 8, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-    if (groups <= 16) return launch_csr<T, V16, 16, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-    if (groups <= 32) return launch_csr<T, V16, 32, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-    if (groups <= 64) return launch_csr<T, V16, 64, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-    return launch_csr<T, V16, 64, 2>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  // CSR: 16-byte runs per lane when the block layout allows (a partial last
+  // run is handled element-wise); up to 2 runs per lane for the widest blocks.
+  const bool al = aligned(X, ldx) && aligned(Y, ldy) && (!Z || aligned(Z, ldz));
+#define FE_CSR(LPR, RS, VEC) \
+  return launch_csr<T, LPR, RS, VEC>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v)
+  if (al) {
+    const int runs = (k + V16 - 1) / V16;
+    if (runs <= 1) FE_CSR(1, 1, true);
+    if (runs <= 2) FE_CSR(2, 1, true);
+    if (runs <= 4) FE_CSR(4, 1, true);
+    if (runs <= 8) FE_CSR(8, 1, true);
+    if (runs <= 16) FE_CSR(16, 1, true);
+    if (runs <= 32) FE_CSR(32, 1, true);
+    if (runs <= 64) FE_CSR(64, 1, true);
+    FE_CSR(64, 2, true);
   }
-  if (k <= 1) return launch_csr<T, 1, 1, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-  if (k <= 2) return launch_csr<T, 1, 2, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-  if (k <= 4) return launch_csr<T, 1, 4, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-  if (k <= 8) return launch_csr<T, 1, 8, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-  if (k <= 16) return launch_csr<T, 1, 16, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-  if (k <= 32) return launch_csr<T, 1, 32, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
-  return launch_csr<T, 1, 64, 1>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v);
+  if (k <= 1) FE_CSR(1, 1, false);
+  if (k <= 2) FE_CSR(2, 1, false);
+  if (k <= 4) FE_CSR(4, 1, false);
+  if (k <= 8) FE_CSR(8, 1, false);
+  if (k <= 16) FE_CSR(16, 1, false);
+  if (k <= 32) FE_CSR(32, 1, false);
+  FE_CSR(64, 1, false);
+#undef FE_CSR
 }
 
 // ------------------------------------------------------------ SELL-C-sigma build

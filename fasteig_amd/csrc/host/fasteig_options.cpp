@@ -89,7 +89,8 @@ bool fasteig_options::set(std::string_view key, std::string_view value) {
     gram_ = v;
   } else if (k == "format") {
     const std::string v = lower(value);
-    if (v != "csr" && v != "sell") throw std::invalid_argument("format must be 'csr' or 'sell'");
+    if (v != "csr" && v != "sell" && v != "blocked" && v != "auto")
+      throw std::invalid_argument("format must be 'csr', 'sell', 'blocked' or 'auto'");
     format_ = v;
   } else if (k == "cheb_bounds") {
     const std::string v = lower(value);

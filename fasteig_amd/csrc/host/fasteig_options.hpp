@@ -4,7 +4,9 @@
 //   precision  "fp64" (plain FP64 on the GPU) | "mixed" (FP32 filter and Gram,
 //              FP64 Cholesky, Rayleigh-Ritz, residuals and closing sweeps)
 //   gram       "native" (FP64/FP32 MFMA) | "bf16x3" (split-BF16 Gram, FP32 blocks)
-//   format     "csr" | "sell" (SELL-32-sigma sparse product)
+//   format     "csr" | "sell" (SELL-32-sigma sparse product) | "blocked"
+//              (row-block gather: X rows of 32-row blocks staged in LDS) |
+//              "auto" (blocked when the matrix has the column locality, else csr)
 //   device     HIP device index
 //   cheb_bounds "amplification" (the reference's fixed cutoff) | "ritz" (the
 //              damped interval starts at the largest Ritz value of the current

@@ -134,8 +134,18 @@ fasteig::hip::HipCsrMatrix& fasteig_problem::device_matrix(const fasteig_options
       ex->synchronize();
     }
   }
-  if (opts.format() == "sell") dA_->use_sell(32, 256);
-  else if (dA_->sell()) dA_->use_csr();
+  const std::string& f = opts.format();
+  if (f == "sell") {
+    dA_->use_sell(32, 256);
+  } else if (f == "blocked" || f == "auto") {
+    if (!dA_->use_blocked()) {
+      if (f == "blocked")
+        throw std::invalid_argument("format=blocked: the matrix has too little column locality for the row-block copy");
+      dA_->use_csr();
+    }
+  } else {
+    dA_->use_csr();
+  }
   return *dA_;
 }
 

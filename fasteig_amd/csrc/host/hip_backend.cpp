@@ -7,8 +7,8 @@
 namespace fasteig {
 namespace hip {
 
-void check(int rc, const char* what) {
-  if (rc >= 0) return;
+int check(int rc, const char* what) {
+  if (rc >= 0) return rc;
   const std::string msg = std::string(what) + ": " + fe_last_error();
   if (rc == -2) throw std::invalid_argument(msg);
   throw std::runtime_error(msg);
@@ -108,11 +108,25 @@ void HipCsrMatrix::use_sell(int C, int sigma) {
   }
   check(fe_csr_use(h_, FE_FMT_SELL), "fe_csr_use");
   sell_ = true;
+  blocked_ = false;
+}
+
+bool HipCsrMatrix::use_blocked() {
+  if (blocked_state_ == 0) {
+    double reuse = 0.0;
+    blocked_state_ = check(fe_csr_build_blocked(ex_.ctx(), h_, 32, &reuse), "fe_csr_build_blocked") == 0 ? 1 : -1;
+  }
+  if (blocked_state_ < 0) return false;
+  check(fe_csr_use(h_, FE_FMT_BLOCKED), "fe_csr_use");
+  blocked_ = true;
+  sell_ = false;
+  return true;
 }
 
 void HipCsrMatrix::use_csr() {
   check(fe_csr_use(h_, FE_FMT_CSR), "fe_csr_use");
   sell_ = false;
+  blocked_ = false;
 }
 
 // ---------------------------------------------------------------- executor

@@ -25,7 +25,7 @@
 namespace fasteig {
 namespace hip {
 
-void check(int rc, const char* what);
+int check(int rc, const char* what);  // throws on rc < 0, else returns rc
 
 class HipExecutor;
 
@@ -110,8 +110,12 @@ class HipCsrMatrix {
   int rows() const { return n_; }
   int64_t nonzeros() const { return nnz_; }
   void use_sell(int C = 32, int sigma = 256);
+  // Row-block gather copy (LDS-staged X rows, fe_csr_build_blocked); false
+  // when the matrix has too little column locality for it (CSR stays active).
+  bool use_blocked();
   void use_csr();
   bool sell() const { return sell_; }
+  bool blocked() const { return blocked_; }
 
  private:
   HipExecutor& ex_;
@@ -119,6 +123,8 @@ class HipCsrMatrix {
   int n_ = 0;
   int64_t nnz_ = 0;
   bool sell_built_ = false, sell_ = false;
+  int blocked_state_ = 0;  // 0 not tried, 1 built, -1 rejected
+  bool blocked_ = false;
 };
 
 class HipExecutor {

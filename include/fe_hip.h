@@ -47,7 +47,7 @@ typedef struct fe_csr_s* fe_csr;
 
 enum fe_prec { FE_F64 = 0, FE_F32 = 1 };
 enum fe_gram_mode { FE_GRAM_NATIVE = 0, FE_GRAM_BF16X3 = 1 };
-enum fe_format { FE_FMT_CSR = 0, FE_FMT_SELL = 1 };
+enum fe_format { FE_FMT_CSR = 0, FE_FMT_SELL = 1, FE_FMT_BLOCKED = 2 };
 
 /* ---- errors / version ------------------------------------------------- */
 const char* fe_last_error(void);
@@ -79,6 +79,7 @@ int fe_zero2d(fe_ctx ctx, void* ptr, int64_t rows, int64_t cols, int64_t ld, int
  * memory and keeps FP64 and FP32 copies of the values. */
 int fe_csr_create(fe_ctx ctx, int n, int64_t nnz, const int* rowptr, const int* colind,
                   const double* vals, fe_csr* out);
+/* Destroy a matrix before the context it was created on. */
 int fe_csr_destroy(fe_csr A);
 int fe_csr_info(fe_csr A, int* n, int64_t* nnz);
 /* A <- D A D with D = diag(d) (d on the device, FP64, n entries). Used for the
@@ -88,6 +89,13 @@ int fe_csr_scale_sym(fe_ctx ctx, fe_csr A, const double* d);
  * within windows of sigma rows); later products may use it (fe_csr_use). */
 int fe_csr_build_sell(fe_ctx ctx, fe_csr A, int C, int sigma);
 int fe_csr_use(fe_csr A, int format);
+/* Builds the row-block gather copy of A: rows in blocks of rb, each block with
+ * the sorted union of its column indices. The product then stages the union's
+ * X rows in LDS once per block and column tile, instead of gathering one X row
+ * from L2 per nonzero. Returns 0 when built, 1 when the matrix has too little
+ * reuse (union rows per block >= nonzeros / 2) or a union too large for LDS;
+ * *reuse receives nonzeros per union row. */
+int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse);
 
 /* ---- fused block SpMM (replaces LinearOperator::apply_block,
  *      include/core/LinearOperator.hpp:32-37; CUDA: k x cusparseSpMV,

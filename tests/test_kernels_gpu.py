@@ -28,6 +28,7 @@ def mats():
         "lap2d": G.laplacian_2d(37, 29),
         "cond3d": G.conductivity_3d(11, seed=2),
         "h27": G.hamiltonian_27pt(9, seed=4, wells=5),
+        "h27big": G.hamiltonian_27pt(23, seed=4, wells=9),
         "powlaw": G.power_law(3000, seed=5, dmax=300),
     }
     # ragged: empty rows and one dense row
@@ -84,6 +85,47 @@ def test_spmm_sell_matches_csr(dev, mats, k, prec):
     scale = np.max(np.abs(A).sum(axis=1))
     assert np.max(np.abs(Y1.download() - ref)) <= tol * scale
     assert np.max(np.abs(Y2.download() - ref)) <= tol * scale
+
+
+@pytest.mark.parametrize("name", ["h27", "h27big"])
+@pytest.mark.parametrize("k", [4, 12, 32, 48, 100, 256])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_spmm_blocked_matches_csr(dev, mats, name, k, prec):
+    """Row-block gather copy: same sums in the same order as CSR (bitwise),
+    fused epilogue included; k not a multiple of the 16-byte run falls back
+    to the CSR kernel."""
+    A = mats[name]
+    n = A.shape[0]
+    rng = np.random.default_rng(31 + k)
+    X = rng.uniform(-1, 1, (n, k))
+    Z = rng.uniform(-1, 1, (n, k))
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    C = dev.csr(A)
+    built, reuse = C.build_blocked(32)
+    assert built and reuse >= 2.0
+    Xd = dev.put(X, dt)
+    Zd = dev.put(Z, dt)
+    Y1 = dev.buffer((n, k), dt)
+    Y2 = dev.buffer((n, k), dt)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y1.ptr, k, 0.9, 0.4, Zd.ptr, k, -1.2))
+    C.use(_fe.FMT_BLOCKED)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y2.ptr, k, 0.9, 0.4, Zd.ptr, k, -1.2))
+    ref = orc.spmm(A, X, 0.9, 0.0) + 0.4 * X - 1.2 * Z
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    scale = np.max(np.abs(A).sum(axis=1)) + 2.0
+    assert np.max(np.abs(Y2.download().astype(np.float64) - ref)) <= tol * scale
+    assert np.array_equal(Y1.download(), Y2.download())
+
+
+@pytest.mark.parametrize("name", ["powlaw", "lap2d", "cond3d"])
+def test_spmm_blocked_rejects_low_reuse(dev, mats, name):
+    """Scattered columns (power law) and thin stencils (5- and 7-point: about
+    1.5 nonzeros per union row of a 32-row block) get no blocked copy."""
+    C = dev.csr(mats[name])
+    built, reuse = C.build_blocked(32)
+    assert not built and reuse < 2.0
+    with pytest.raises(_fe.FeError):
+        C.use(_fe.FMT_BLOCKED)
 
 
 def test_spmm_strided_subblock(dev, mats):
@@ -169,7 +211,7 @@ def test_gemm_nn(dev, n, m, p, prec):
     assert np.max(np.abs(Yd.download() - ref)) <= tol * m
 
 
-@pytest.mark.parametrize("k", [1, 7, 64, 130, 512])
+@pytest.mark.parametrize("k", [1, 7, 31, 32, 33, 64, 130, 512])
 def test_potrf_rinv_cholqr(dev, k):
     rng = np.random.default_rng(k)
     n = max(4 * k, 50)
@@ -192,12 +234,18 @@ def test_potrf_rinv_cholqr(dev, k):
     assert np.max(np.abs(Q - Qo)) < 1e-9
 
 
-def test_potrf_reports_failure(dev):
-    k = 6
-    Gm = np.eye(k)
-    Gm[3, 3] = -1.0
+@pytest.mark.parametrize("k,bad", [(6, 3), (100, 70), (64, 0)])
+def test_potrf_reports_failure(dev, k, bad):
+    """The first non-positive pivot (column bad + 1) is reported, also in a
+    later block of the blocked factorization."""
+    Gm = np.eye(k) * 2.0
+    Gm[bad, bad] = -1.0
     Gd = dev.put(Gm, np.float64, order="F")
-    assert L().fe_potrf(dev.ctx, k, Gd.ptr, k) == 4
+    assert L().fe_potrf(dev.ctx, k, Gd.ptr, k) == bad + 1
+    # the next factorization on the same context starts clean
+    Gd2 = dev.put(np.eye(k) * 4.0, np.float64, order="F")
+    assert L().fe_potrf(dev.ctx, k, Gd2.ptr, k) == 0
+    assert np.allclose(np.tril(Gd2.download()), np.eye(k) * 2.0)
 
 
 def _syev(dev, A):
@@ -209,7 +257,7 @@ def _syev(dev, A):
     return info, Wd.download(), Vd.download()
 
 
-@pytest.mark.parametrize("m", [1, 2, 3, 5, 17, 64, 100, 257, 512])
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 17, 64, 100, 128, 129, 130, 257, 512, 700])
 def test_syev_random(dev, m):
     rng = np.random.default_rng(m)
     A = rng.standard_normal((m, m))
