@@ -142,7 +142,7 @@ class Csr:
     def build_sell(self, C_=32, sigma=256):
         check(lib().fe_csr_build_sell(self.dev.ctx, self.h, C_, sigma))
 
-    def build_blocked(self, rb=32):
+    def build_blocked(self, rb=4):
         """Build the row-block gather copy; returns (built, reuse)."""
         r = C.c_double(0.0)
         rc = check(lib().fe_csr_build_blocked(self.dev.ctx, self.h, rb, C.byref(r)))

@@ -55,16 +55,17 @@ struct fe_csr_s {
   double* sell_v64 = nullptr;
   float* sell_v32 = nullptr;
   int use_sell = 0;
-  // row-block gather copy (optional): blocks of bg_rb rows, the sorted union of
-  // their column indices (bg_ucol, offsets bg_bptr) and per-nonzero positions
-  // in that union (bg_lcol, 16-bit)
+  // row-block copy (optional): blocks of bg_rb consecutive rows, each with the
+  // sorted union of its rows' columns (bg_ucol, offsets bg_bptr) and bg_rb
+  // values per union entry (bg_v64 / bg_v32, zero where a row lacks the column)
   int bg_rb = 0;
   int64_t bg_nblk = 0;
-  int bg_maxu = 0;
+  int64_t bg_nu = 0;
   int64_t* bg_bptr = nullptr;
   int* bg_ucol = nullptr;
-  uint16_t* bg_lcol = nullptr;
-  double bg_reuse = 0.0;      // nonzeros / union rows (gathers saved per staged row)
+  double* bg_v64 = nullptr;
+  float* bg_v32 = nullptr;
+  double bg_reuse = 0.0;      // nonzeros / union entries (gathers saved)
   int use_bg = 0;
 };
 

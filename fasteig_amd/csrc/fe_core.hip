@@ -367,7 +367,7 @@ int fe_csr_destroy(fe_csr A) {
   if (!A) return 0;
   if (A->ctx) hipStreamSynchronize(A->ctx->stream);
   void* ptrs[] = {A->rowptr, A->colind, A->vals64, A->vals32, A->sell_ptr, A->sell_len,
-                  A->sell_perm, A->sell_col, A->sell_v64, A->sell_v32, A->bg_bptr, A->bg_ucol, A->bg_lcol};
+                  A->sell_perm, A->sell_col, A->sell_v64, A->sell_v32, A->bg_bptr, A->bg_ucol, A->bg_v64, A->bg_v32};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete A;
@@ -393,7 +393,7 @@ int fe_csr_scale_sym(fe_ctx ctx, fe_csr A, const double* d) {
 
 int fe_csr_use(fe_csr A, int format) {
   FE_ARG(A, "fe_csr_use: null matrix");
-  FE_ARG(format == FE_FMT_CSR || (format == FE_FMT_SELL && A->sell_col) || (format == FE_FMT_BLOCKED && A->bg_lcol),
+  FE_ARG(format == FE_FMT_CSR || (format == FE_FMT_SELL && A->sell_col) || (format == FE_FMT_BLOCKED && A->bg_ucol),
          "fe_csr_use: requested copy not built");
   A->use_sell = format == FE_FMT_SELL;
   A->use_bg = format == FE_FMT_BLOCKED;

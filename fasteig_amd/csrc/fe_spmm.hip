@@ -113,10 +113,23 @@ __device__ __forceinline__ void ld_runs(const T* p, int stride, bool full, int c
   }
 }
 
+// gathers in flight per lane: rows of 16+ lanes with one run, and two runs
+#ifndef FE_CSR_UNR_WIDE
+#define FE_CSR_UNR_WIDE 16
+#endif
+#ifndef FE_CSR_UNR_RS2
+#define FE_CSR_UNR_RS2 8
+#endif
+
 // CSR kernel, one row per group of LPR lanes, RS 16-byte runs per lane spaced
 // LPR * 16 bytes apart (each load instruction of the wavefront reads one
 // contiguous 16 * LPR-byte stretch of the X row). Row tiles map to XCDs in
 // contiguous ranges. Scalar variant (V16 = 1 element runs) for unaligned blocks.
+//
+// Latency: the epilogue's own-row X and Z loads are issued before the gathers,
+// and the gathers of a row go out in batches of UNR loads; the last batch is
+// a full batch with the slots past the row's end masked off, so a 27-entry
+// row costs two dependent gather round trips at UNR = 16.
 template <typename T, int LPR, int RS, bool VECTOR>
 __global__ __launch_bounds__(256) void k_spmm_csr(
     int n, const int* __restrict__ rowptr, const int* __restrict__ colind, const T* __restrict__ vals,
@@ -127,7 +140,7 @@ __global__ __launch_bounds__(256) void k_spmm_csr(
   constexpr int RPB = 4 * RPW;
   constexpr int STRIDE = LPR * V;        // columns between a lane's runs
   constexpr int COLS = STRIDE * RS;      // columns per column tile
-  constexpr int UNR = RS == 1 ? 8 : 4;
+  constexpr int UNR = RS == 1 ? (LPR >= 16 ? FE_CSR_UNR_WIDE : 8) : FE_CSR_UNR_RS2;
   const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
   if (tile >= ntiles) return;
   const int lane = threadIdx.x & 63;
@@ -141,6 +154,15 @@ __global__ __launch_bounds__(256) void k_spmm_csr(
   const bool rowok = row < n;
   const int beg = rowok ? rowptr[row] : 0;
   const int end = rowok ? rowptr[row + 1] : 0;
+  // epilogue operands (own X row, Z row) of complete vector runs, loaded early
+  T xe[RS][V == 1 ? 1 : V], ze[RS][V == 1 ? 1 : V];
+  const bool early = VECTOR && rowok && full;
+  if constexpr (VECTOR) {
+    if (VECTOR && early) {
+      if (beta != T(0)) ld_runs<T, RS>(X + row * ldx + c, STRIDE, true, c, k, xe);
+      if (Z && gamma != T(0)) ld_runs<T, RS>(Z + row * ldz + c, STRIDE, true, c, k, ze);
+    }
+  }
   T acc[RS][V == 1 ? 1 : V];
 #pragma unroll
   for (int r = 0; r < RS; ++r)
@@ -151,48 +173,34 @@ __global__ __launch_bounds__(256) void k_spmm_csr(
     const int mycol = q < end ? colind[q] : 0;
     const T myval = q < end ? vals[q] : T(0);
     const int cnt = end - p < LPR ? end - p : LPR;
-    int t = 0;
-    for (; t + UNR <= cnt; t += UNR) {
+    for (int t = 0; t < cnt; t += UNR) {
       int cc[UNR];
       T vv[UNR];
 #pragma unroll
       for (int u = 0; u < UNR; ++u) {
-        cc[u] = bcast_int<LPR>(mycol, gbase + t + u);
-        vv[u] = bcast_val<LPR>(myval, gbase + t + u);
+        cc[u] = bcast_int<LPR>(mycol, gbase + ((t + u) & (LPR - 1)));
+        vv[u] = bcast_val<LPR>(myval, gbase + ((t + u) & (LPR - 1)));
       }
       T xv[UNR][RS][V == 1 ? 1 : V];
 #pragma unroll
       for (int u = 0; u < UNR; ++u) {
-        const T* xp = X + (int64_t)cc[u] * ldx + c;
-        if constexpr (VECTOR) {
-          ld_runs<T, RS>(xp, STRIDE, full, c, k, xv[u]);
-        } else {
+        if (t + u < cnt) {
+          const T* xp = X + (int64_t)cc[u] * ldx + c;
+          if constexpr (VECTOR) {
+            ld_runs<T, RS>(xp, STRIDE, full, c, k, xv[u]);
+          } else {
 #pragma unroll
-          for (int r = 0; r < RS; ++r) xv[u][r][0] = (c + r * STRIDE < k) ? xp[r * STRIDE] : T(0);
+            for (int r = 0; r < RS; ++r) xv[u][r][0] = (c + r * STRIDE < k) ? xp[r * STRIDE] : T(0);
+          }
         }
       }
 #pragma unroll
       for (int u = 0; u < UNR; ++u)
+        if (t + u < cnt)
 #pragma unroll
-        for (int r = 0; r < RS; ++r)
+          for (int r = 0; r < RS; ++r)
 #pragma unroll
-          for (int e = 0; e < V; ++e) acc[r][e] += vv[u] * xv[u][r][e];
-    }
-    for (; t < cnt; ++t) {
-      const int cc = bcast_int<LPR>(mycol, gbase + t);
-      const T vv = bcast_val<LPR>(myval, gbase + t);
-      const T* xp = X + (int64_t)cc * ldx + c;
-      T xv[RS][V == 1 ? 1 : V];
-      if constexpr (VECTOR) {
-        ld_runs<T, RS>(xp, STRIDE, full, c, k, xv);
-      } else {
-#pragma unroll
-        for (int r = 0; r < RS; ++r) xv[r][0] = (c + r * STRIDE < k) ? xp[r * STRIDE] : T(0);
-      }
-#pragma unroll
-      for (int r = 0; r < RS; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e) acc[r][e] += vv * xv[r][e];
+            for (int e = 0; e < V; ++e) acc[r][e] += vv[u] * xv[u][r][e];
     }
   }
   if (!rowok) return;
@@ -203,7 +211,17 @@ __global__ __launch_bounds__(256) void k_spmm_csr(
     T out[V == 1 ? 1 : V];
 #pragma unroll
     for (int e = 0; e < V; ++e) out[e] = alpha * acc[r][e];
-    if (VECTOR && cr + V <= k) {
+    if (VECTOR && early) {
+      if (beta != T(0))
+#pragma unroll
+        for (int e = 0; e < V; ++e) out[e] += beta * xe[r][e];
+      if (Z && gamma != T(0))
+#pragma unroll
+        for (int e = 0; e < V; ++e) out[e] += gamma * ze[r][e];
+      float4 o;
+      __builtin_memcpy(&o, out, 16);
+      *reinterpret_cast<float4*>(Y + row * ldy + cr) = o;
+    } else if (VECTOR && cr + V <= k) {
       if (beta != T(0)) {
         const float4 f = *reinterpret_cast<const float4*>(X + row * ldx + cr);
         T xv[V];

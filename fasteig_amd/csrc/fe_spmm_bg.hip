@@ -1,140 +1,179 @@
-// Row-block gather SpMM: Y = alpha A X + beta X + gamma Z with X staged in LDS.
+// Row-blocked SpMM: Y = alpha A X + beta X + gamma Z, one gather per column
+// of a row block instead of one per nonzero.
 //
-// For matrices with locality (stencils, FEM meshes) the rows of a block of RB
-// consecutive rows touch a union of X rows much smaller than their nonzero
-// count (27-point stencil, RB = 32: ~306 union rows for ~860 nonzeros). A
-// workgroup takes one block and one column tile of KT columns (128 bytes per
-// X row), copies the union's X row segments into LDS with coalesced 16-byte
-// loads, then runs the nonzeros of its 32 rows against LDS: each X row
-// segment crosses L2 once per block instead of once per nonzero. The
-// nonzeros keep their CSR order, so the sums are the same as the CSR kernel's.
-// Column tiles of one block are consecutive on one XCD, so the block's
-// nonzeros (values, 16-bit union positions) come from that XCD's L2 after the
-// first tile.
+// The rows are grouped in blocks of RB consecutive rows. A block stores the
+// sorted union of its rows' column indices and, per union entry, the RB
+// values of that column in the block's rows (zero where a row has no such
+// entry): a BCSR layout with RB x 1 blocks. The product gathers each union X
+// row once and feeds it to RB accumulators. For matrices with locality the
+// union is much smaller than the nonzero count (27-point stencil, RB = 4: 54
+// union columns for 108 nonzeros), which halves the gathered X bytes, the
+// dominant traffic of the block product through the vector memory path;
+// the explicit zeros cost FMAs and a little more A traffic. A row's sums run
+// over its entries in ascending column order, as the CSR kernel does on a
+// matrix with sorted rows (added explicit zeros leave the sums unchanged).
 #include "fe_common.hpp"
 
 #include <algorithm>
 
 namespace fe {
 
-constexpr int BG_RB = 32;          // rows per block
-constexpr int BG_MAXU = 448;       // union rows that fit the LDS tile
-constexpr int BG_ROWB = 128 + 16;  // LDS bytes per staged row (128 B + pad)
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_spmm_bg(int n, const int* __restrict__ rowptr, const uint16_t* __restrict__ lcol,
-                                                 const T* __restrict__ vals, const int64_t* __restrict__ bptr,
-                                                 const int* __restrict__ ucol, int k, const T* __restrict__ X,
-                                                 int64_t ldx, T* Y, int64_t ldy, T alpha, T beta, const T* Z,
-                                                 int64_t ldz, T gamma, int ntc, int64_t per_xcd, int64_t ntiles) {
-  constexpr int V = 16 / (int)sizeof(T);    // elements per 16-byte run
-  constexpr int KT = 128 / (int)sizeof(T);  // columns per tile
-  constexpr int RUNS = KT / V;              // 8 runs per staged row
-  constexpr int LPR = 256 / BG_RB;          // 8 lanes per row
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int64_t t = xcd_tile(blockIdx.x, per_xcd);
-  if (t >= ntiles) return;
-  const int64_t blk = t / ntc;
-  const int ct = static_cast<int>(t - blk * ntc);
-  const int c0 = ct * KT;
-  const int tid = threadIdx.x;
-  const int64_t u0 = bptr[blk];
-  const int nu = static_cast<int>(bptr[blk + 1] - u0);
-  // stage the union rows' column tile
-  for (int idx = tid; idx < nu * RUNS; idx += 256) {
-    const int u = idx / RUNS, r = idx % RUNS;
-    const int c = c0 + r * V;
-    const T* src = X + (int64_t)ucol[u0 + u] * ldx + c;
-    T* dst = reinterpret_cast<T*>(smem + u * BG_ROWB) + r * V;
-    if (c + V <= k) {
-      *reinterpret_cast<float4*>(dst) = *reinterpret_cast<const float4*>(src);
-    } else {
+// Same lane layout as the CSR kernel: a group of LPR lanes owns one row block
+// and RS 16-byte runs per lane, spaced LPR * 16 bytes apart. The group reads
+// the union entries' columns and values directly (one address per group:
+// scalar loads when the group is the whole wavefront).
+template <typename T, int RB, int LPR, int RS>
+__global__ __launch_bounds__(256) void k_spmm_rb(int n, const int64_t* __restrict__ bptr,
+                                                 const int* __restrict__ ucol, const T* __restrict__ bval, int k,
+                                                 const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha,
+                                                 T beta, const T* Z, int64_t ldz, T gamma, int64_t per_xcd,
+                                                 int64_t ntiles, int64_t nblk) {
+  constexpr int V = 16 / (int)sizeof(T);
+  constexpr int GPW = 64 / LPR;  // row blocks per wavefront
+  constexpr int GPB = 4 * GPW;   // row blocks per workgroup
+  constexpr int STRIDE = LPR * V;
+  constexpr int COLS = STRIDE * RS;
+  constexpr int UNR = (RS == 1 && RB == 4) ? 8 : 4;
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  // wave index made wavefront-uniform so that whole-wave groups use scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int grp = LPR == 64 ? 0 : lane / LPR;
+  const int64_t blk = tile * GPB + wave * GPW + grp;
+  const int sub = LPR == 64 ? lane : lane % LPR;
+  const int c = blockIdx.y * COLS + sub * V;
+  const bool full = c + (RS - 1) * STRIDE + V <= k;
+  const bool blkok = blk < nblk;
+  const int64_t e0 = blkok ? bptr[blk] : 0;
+  const int64_t e1 = blkok ? bptr[blk + 1] : 0;
+  T acc[RB][RS][V];
 #pragma unroll
-      for (int e = 0; e < V; ++e) dst[e] = (c + e < k) ? src[e] : T(0);
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int s = 0; s < RS; ++s)
+#pragma unroll
+      for (int e = 0; e < V; ++e) acc[r][s][e] = T(0);
+  for (int64_t e = e0; e < e1; e += UNR) {
+    T xv[UNR][RS][V];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      if (e + u < e1) {
+        const T* xp = X + (int64_t)ucol[e + u] * ldx + c;
+#pragma unroll
+        for (int s = 0; s < RS; ++s) {
+          const int cs = c + s * STRIDE;
+          if (full || cs + V <= k) {
+            const float4 f = *reinterpret_cast<const float4*>(xp + s * STRIDE);
+            __builtin_memcpy(&xv[u][s][0], &f, 16);
+          } else {
+#pragma unroll
+            for (int q = 0; q < V; ++q) xv[u][s][q] = (cs + q < k) ? xp[s * STRIDE + q] : T(0);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      if (e + u < e1) {
+        const T* w = bval + (e + u) * RB;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          const T wr = w[r];
+#pragma unroll
+          for (int s = 0; s < RS; ++s)
+#pragma unroll
+            for (int q = 0; q < V; ++q) acc[r][s][q] += wr * xv[u][s][q];
+        }
+      }
     }
   }
-  __syncthreads();
-  const int lane = tid & 63;
-  const int sub = lane % LPR, gbase = lane - sub;
-  const int64_t row = blk * BG_RB + tid / LPR;
-  const int c = c0 + sub * V;
-  const bool rowok = row < n;
-  const int beg = rowok ? rowptr[row] : 0;
-  const int end = rowok ? rowptr[row + 1] : 0;
-  T acc[V];
+  if (!blkok) return;
 #pragma unroll
-  for (int e = 0; e < V; ++e) acc[e] = T(0);
-  for (int p = beg; p < end; p += LPR) {
-    const int q = p + sub;
-    const int myl = q < end ? (int)lcol[q] : 0;
-    const T myv = q < end ? vals[q] : T(0);
-    const int cnt = end - p < LPR ? end - p : LPR;
-    for (int s = 0; s < cnt; ++s) {
-      const int j = __shfl(myl, gbase + s, 64);
-      const T v = __shfl(myv, gbase + s, 64);
-      const float4 f = *reinterpret_cast<const float4*>(smem + j * BG_ROWB + sub * 16);
-      T x[V];
-      __builtin_memcpy(x, &f, 16);
+  for (int r = 0; r < RB; ++r) {
+    const int64_t row = blk * RB + r;
+    if (row >= n) break;
 #pragma unroll
-      for (int e = 0; e < V; ++e) acc[e] += v * x[e];
-    }
-  }
-  if (!rowok || c >= k) return;
-  T out[V];
+    for (int s = 0; s < RS; ++s) {
+      const int cs = c + s * STRIDE;
+      if (cs >= k) continue;
+      T out[V];
 #pragma unroll
-  for (int e = 0; e < V; ++e) out[e] = alpha * acc[e];
-  if (c + V <= k) {
-    if (beta != T(0)) {
-      const float4 f = *reinterpret_cast<const float4*>(X + row * ldx + c);
-      T xv[V];
-      __builtin_memcpy(xv, &f, 16);
+      for (int q = 0; q < V; ++q) out[q] = alpha * acc[r][s][q];
+      if (cs + V <= k) {
+        if (beta != T(0)) {
+          const float4 f = *reinterpret_cast<const float4*>(X + row * ldx + cs);
+          T xv[V];
+          __builtin_memcpy(xv, &f, 16);
 #pragma unroll
-      for (int e = 0; e < V; ++e) out[e] += beta * xv[e];
-    }
-    if (Z && gamma != T(0)) {
-      const float4 f = *reinterpret_cast<const float4*>(Z + row * ldz + c);
-      T zv[V];
-      __builtin_memcpy(zv, &f, 16);
+          for (int q = 0; q < V; ++q) out[q] += beta * xv[q];
+        }
+        if (Z && gamma != T(0)) {
+          const float4 f = *reinterpret_cast<const float4*>(Z + row * ldz + cs);
+          T zv[V];
+          __builtin_memcpy(zv, &f, 16);
 #pragma unroll
-      for (int e = 0; e < V; ++e) out[e] += gamma * zv[e];
-    }
-    float4 o;
-    __builtin_memcpy(&o, out, 16);
-    *reinterpret_cast<float4*>(Y + row * ldy + c) = o;
-  } else {
+          for (int q = 0; q < V; ++q) out[q] += gamma * zv[q];
+        }
+        float4 o;
+        __builtin_memcpy(&o, out, 16);
+        *reinterpret_cast<float4*>(Y + row * ldy + cs) = o;
+      } else {
 #pragma unroll
-    for (int e = 0; e < V; ++e) {
-      if (c + e >= k) continue;
-      T o = out[e];
-      if (beta != T(0)) o += beta * X[row * ldx + c + e];
-      if (Z && gamma != T(0)) o += gamma * Z[row * ldz + c + e];
-      Y[row * ldy + c + e] = o;
+        for (int q = 0; q < V; ++q) {
+          if (cs + q >= k) continue;
+          T o = out[q];
+          if (beta != T(0)) o += beta * X[row * ldx + cs + q];
+          if (Z && gamma != T(0)) o += gamma * Z[row * ldz + cs + q];
+          Y[row * ldy + cs + q] = o;
+        }
+      }
     }
   }
 }
 
-template <typename T>
-int spmm_blocked_t(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
-                   const T* Z, int64_t ldz, T gamma) {
-  constexpr int KT = 128 / (int)sizeof(T);
-  const int ntc = (k + KT - 1) / KT;
-  const int64_t ntiles = A->bg_nblk * ntc;
+template <typename T, int RB, int LPR, int RS>
+static int launch_rb(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
+                     const T* Z, int64_t ldz, T gamma) {
+  constexpr int V = 16 / (int)sizeof(T);
+  constexpr int GPB = 4 * (64 / LPR);
+  constexpr int COLS = LPR * V * RS;
+  const int64_t ntiles = (A->bg_nblk + GPB - 1) / GPB;
   const int64_t nb8 = (ntiles + 7) / 8 * 8;
-  const size_t lds = static_cast<size_t>(A->bg_maxu) * BG_ROWB;
-  const T* v = sizeof(T) == 8 ? (const T*)A->vals64 : (const T*)A->vals32;
-  hipLaunchKernelGGL(k_spmm_bg<T>, dim3((unsigned)nb8), dim3(256), lds, ctx->stream, A->n, A->rowptr, A->bg_lcol, v,
-                     A->bg_bptr, A->bg_ucol, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, ntc, nb8 / 8, ntiles);
+  const unsigned ycols = static_cast<unsigned>((k + COLS - 1) / COLS);
+  const T* bv = sizeof(T) == 8 ? (const T*)A->bg_v64 : (const T*)A->bg_v32;
+  hipLaunchKernelGGL((k_spmm_rb<T, RB, LPR, RS>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream, A->n,
+                     A->bg_bptr, A->bg_ucol, bv, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles,
+                     A->bg_nblk);
   FE_CHECK_LAUNCH();
   return 0;
 }
 
+template <typename T, int RB>
+static int dispatch_rb(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
+                       const T* Z, int64_t ldz, T gamma) {
+  constexpr int V = 16 / (int)sizeof(T);
+  const int runs = (k + V - 1) / V;
+#define FE_RB(LPR, RS) return launch_rb<T, RB, LPR, RS>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma)
+  if (runs <= 4) FE_RB(4, 1);
+  if (runs <= 8) FE_RB(8, 1);
+  if (runs <= 16) FE_RB(16, 1);
+  if (runs <= 32) FE_RB(32, 1);
+  if (runs <= 64) FE_RB(64, 1);
+  FE_RB(64, 2);
+#undef FE_RB
+}
+
+// Called by fe_spmm for 16-byte aligned blocks when the row-block copy is active.
 int spmm_blocked(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
                  double alpha, double beta, const void* Z, int64_t ldz, double gamma) {
-  if (prec == FE_F64)
-    return spmm_blocked_t<double>(ctx, A, k, (const double*)X, ldx, (double*)Y, ldy, alpha, beta, (const double*)Z,
-                                  ldz, gamma);
-  return spmm_blocked_t<float>(ctx, A, k, (const float*)X, ldx, (float*)Y, ldy, (float)alpha, (float)beta,
-                               (const float*)Z, ldz, (float)gamma);
+  if (prec == FE_F64) {
+    auto f = A->bg_rb == 8 ? dispatch_rb<double, 8> : dispatch_rb<double, 4>;
+    return f(ctx, A, k, (const double*)X, ldx, (double*)Y, ldy, alpha, beta, (const double*)Z, ldz, gamma);
+  }
+  auto f = A->bg_rb == 8 ? dispatch_rb<float, 8> : dispatch_rb<float, 4>;
+  return f(ctx, A, k, (const float*)X, ldx, (float*)Y, ldy, (float)alpha, (float)beta, (const float*)Z, ldz,
+           (float)gamma);
 }
 
 }  // namespace fe
@@ -143,53 +182,64 @@ using namespace fe;
 
 extern "C" int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse) {
   FE_ARG(ctx && A, "fe_csr_build_blocked: null argument");
-  FE_ARG(rb == BG_RB, "fe_csr_build_blocked: the row block size is 32");
-  if (A->bg_lcol) {
+  FE_ARG(rb == 4 || rb == 8, "fe_csr_build_blocked: the row block size is 4 or 8");
+  if (A->bg_ucol) {
+    FE_ARG(A->bg_rb == rb, "fe_csr_build_blocked: already built with another row block size");
     if (reuse) *reuse = A->bg_reuse;
     return 0;
   }
   const int n = A->n;
   std::vector<int> rp(n + 1), ci(A->nnz);
+  std::vector<double> v(A->nnz);
   FE_HIP_TRY(hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
-  if (A->nnz)
+  if (A->nnz) {
     FE_HIP_TRY(hipMemcpyAsync(ci.data(), A->colind, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    FE_HIP_TRY(hipMemcpyAsync(v.data(), A->vals64, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+  }
   FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
   const int64_t nblk = (n + rb - 1) / rb;
+  // pass 1: union sizes
   std::vector<int64_t> bptr(nblk + 1, 0);
-  std::vector<int> ucol;
-  std::vector<uint16_t> lcol(A->nnz > 0 ? A->nnz : 1);
-  ucol.reserve(static_cast<size_t>(A->nnz / 2 + 16));
   std::vector<int> u;
-  int maxu = 0;
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int r0 = static_cast<int>(b * rb), r1 = std::min(n, r0 + rb);
+    u.assign(ci.begin() + rp[r0], ci.begin() + rp[r1]);
+    std::sort(u.begin(), u.end());
+    bptr[b + 1] = bptr[b] + (std::unique(u.begin(), u.end()) - u.begin());
+  }
+  const int64_t nu = bptr[nblk];
+  const double r = nu ? static_cast<double>(A->nnz) / static_cast<double>(nu) : 0.0;
+  if (reuse) *reuse = r;
+  if (r < 1.5) return 1;  // too little column sharing: the CSR gather is as good
+  // pass 2: union columns and the rb values per union entry
+  std::vector<int> ucol(static_cast<size_t>(std::max<int64_t>(nu, 1)));
+  std::vector<double> bv(static_cast<size_t>(std::max<int64_t>(nu, 1)) * rb, 0.0);
   for (int64_t b = 0; b < nblk; ++b) {
     const int r0 = static_cast<int>(b * rb), r1 = std::min(n, r0 + rb);
     u.assign(ci.begin() + rp[r0], ci.begin() + rp[r1]);
     std::sort(u.begin(), u.end());
     u.erase(std::unique(u.begin(), u.end()), u.end());
-    maxu = std::max(maxu, static_cast<int>(u.size()));
-    if (maxu > BG_MAXU) {
-      if (reuse) *reuse = 0.0;
-      return 1;  // union too large for the LDS tile
-    }
-    for (int p = rp[r0]; p < rp[r1]; ++p)
-      lcol[p] = static_cast<uint16_t>(std::lower_bound(u.begin(), u.end(), ci[p]) - u.begin());
-    ucol.insert(ucol.end(), u.begin(), u.end());
-    bptr[b + 1] = static_cast<int64_t>(ucol.size());
+    std::copy(u.begin(), u.end(), ucol.begin() + bptr[b]);
+    for (int row = r0; row < r1; ++row)
+      for (int p = rp[row]; p < rp[row + 1]; ++p) {
+        const int64_t pos = bptr[b] + (std::lower_bound(u.begin(), u.end(), ci[p]) - u.begin());
+        bv[pos * rb + (row - r0)] += v[p];  // duplicates in a CSR row add up
+      }
   }
-  const double r = ucol.empty() ? 0.0 : static_cast<double>(A->nnz) / static_cast<double>(ucol.size());
-  if (reuse) *reuse = r;
-  if (r < 2.0) return 1;  // too little reuse: the direct gather is as good
+  std::vector<float> bv32(bv.size());
+  for (size_t i = 0; i < bv.size(); ++i) bv32[i] = static_cast<float>(bv[i]);
   FE_HIP_TRY(hipMalloc(&A->bg_bptr, sizeof(int64_t) * (nblk + 1)));
-  FE_HIP_TRY(hipMalloc(&A->bg_ucol, sizeof(int) * std::max<size_t>(ucol.size(), 1)));
-  FE_HIP_TRY(hipMalloc(&A->bg_lcol, sizeof(uint16_t) * lcol.size()));
+  FE_HIP_TRY(hipMalloc(&A->bg_ucol, sizeof(int) * ucol.size()));
+  FE_HIP_TRY(hipMalloc(&A->bg_v64, sizeof(double) * bv.size()));
+  FE_HIP_TRY(hipMalloc(&A->bg_v32, sizeof(float) * bv32.size()));
   FE_HIP_TRY(hipMemcpyAsync(A->bg_bptr, bptr.data(), sizeof(int64_t) * (nblk + 1), hipMemcpyHostToDevice, ctx->stream));
-  if (!ucol.empty())
-    FE_HIP_TRY(hipMemcpyAsync(A->bg_ucol, ucol.data(), sizeof(int) * ucol.size(), hipMemcpyHostToDevice, ctx->stream));
-  FE_HIP_TRY(hipMemcpyAsync(A->bg_lcol, lcol.data(), sizeof(uint16_t) * lcol.size(), hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipMemcpyAsync(A->bg_ucol, ucol.data(), sizeof(int) * ucol.size(), hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipMemcpyAsync(A->bg_v64, bv.data(), sizeof(double) * bv.size(), hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipMemcpyAsync(A->bg_v32, bv32.data(), sizeof(float) * bv32.size(), hipMemcpyHostToDevice, ctx->stream));
   FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
   A->bg_rb = rb;
   A->bg_nblk = nblk;
-  A->bg_maxu = std::max(maxu, 1);
+  A->bg_nu = nu;
   A->bg_reuse = r;
   return 0;
 }

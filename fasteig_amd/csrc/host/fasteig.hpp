@@ -117,5 +117,5 @@ class fasteig_params {
   // MI355X backend extensions (deck keys `precision`, `gram`, `format`)
   std::string precision = "fp64";
   std::string gram = "native";
-  std::string format = "csr";
+  std::string format = "auto";
 };

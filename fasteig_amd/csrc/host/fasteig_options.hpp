@@ -5,7 +5,8 @@
 //              FP64 Cholesky, Rayleigh-Ritz, residuals and closing sweeps)
 //   gram       "native" (FP64/FP32 MFMA) | "bf16x3" (split-BF16 Gram, FP32 blocks)
 //   format     "csr" | "sell" (SELL-32-sigma sparse product) | "blocked"
-//              (row-block gather: X rows of 32-row blocks staged in LDS) |
+//              (4-row blocks sharing one column list: one X gather per column of
+//              a block instead of one per nonzero) |
 //              "auto" (blocked when the matrix has the column locality, else csr)
 //   device     HIP device index
 //   cheb_bounds "amplification" (the reference's fixed cutoff) | "ritz" (the
@@ -91,7 +92,7 @@ class fasteig_options {
   bool gpu_backend_ = true;
   std::string precision_ = "fp64";
   std::string gram_ = "native";
-  std::string format_ = "csr";
+  std::string format_ = "auto";
   std::string cheb_bounds_ = "amplification";
   int device_ = 0;
   MatrixXd v0_;

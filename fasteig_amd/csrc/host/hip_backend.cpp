@@ -114,7 +114,7 @@ void HipCsrMatrix::use_sell(int C, int sigma) {
 bool HipCsrMatrix::use_blocked() {
   if (blocked_state_ == 0) {
     double reuse = 0.0;
-    blocked_state_ = check(fe_csr_build_blocked(ex_.ctx(), h_, 32, &reuse), "fe_csr_build_blocked") == 0 ? 1 : -1;
+    blocked_state_ = check(fe_csr_build_blocked(ex_.ctx(), h_, 4, &reuse), "fe_csr_build_blocked") == 0 ? 1 : -1;
   }
   if (blocked_state_ < 0) return false;
   check(fe_csr_use(h_, FE_FMT_BLOCKED), "fe_csr_use");

@@ -110,7 +110,7 @@ class HipCsrMatrix {
   int rows() const { return n_; }
   int64_t nonzeros() const { return nnz_; }
   void use_sell(int C = 32, int sigma = 256);
-  // Row-block gather copy (LDS-staged X rows, fe_csr_build_blocked); false
+  // Row-block copy (4-row blocks sharing a column list, fe_csr_build_blocked); false
   // when the matrix has too little column locality for it (CSR stays active).
   bool use_blocked();
   void use_csr();

@@ -89,12 +89,14 @@ int fe_csr_scale_sym(fe_ctx ctx, fe_csr A, const double* d);
  * within windows of sigma rows); later products may use it (fe_csr_use). */
 int fe_csr_build_sell(fe_ctx ctx, fe_csr A, int C, int sigma);
 int fe_csr_use(fe_csr A, int format);
-/* Builds the row-block gather copy of A: rows in blocks of rb, each block with
- * the sorted union of its column indices. The product then stages the union's
- * X rows in LDS once per block and column tile, instead of gathering one X row
- * from L2 per nonzero. Returns 0 when built, 1 when the matrix has too little
- * reuse (union rows per block >= nonzeros / 2) or a union too large for LDS;
- * *reuse receives nonzeros per union row. */
+/* Builds the row-block copy of A: blocks of rb (4 or 8) consecutive rows,
+ * each with the sorted union of its rows' column indices and rb values per
+ * union entry (BCSR with rb x 1 blocks, explicit zeros). The product then
+ * gathers each union X row once per block instead of once per nonzero.
+ * Returns 0 when built, 1 when the matrix shares too few columns between
+ * neighbouring rows (nonzeros per union entry below 1.5: 5- and 7-point
+ * stencils, scattered graphs); *reuse receives nonzeros per union entry.
+ * Select it with fe_csr_use(A, FE_FMT_BLOCKED). */
 int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse);
 
 /* ---- fused block SpMM (replaces LinearOperator::apply_block,

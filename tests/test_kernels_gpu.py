@@ -88,12 +88,13 @@ def test_spmm_sell_matches_csr(dev, mats, k, prec):
 
 
 @pytest.mark.parametrize("name", ["h27", "h27big"])
-@pytest.mark.parametrize("k", [4, 12, 32, 48, 100, 256])
+@pytest.mark.parametrize("rb", [4, 8])
+@pytest.mark.parametrize("k", [4, 12, 32, 48, 100, 256, 512])
 @pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
-def test_spmm_blocked_matches_csr(dev, mats, name, k, prec):
-    """Row-block gather copy: same sums in the same order as CSR (bitwise),
-    fused epilogue included; k not a multiple of the 16-byte run falls back
-    to the CSR kernel."""
+def test_spmm_blocked_matches_csr(dev, mats, name, rb, k, prec):
+    """Row-block copy (BCSR rb x 1 with explicit zeros): the same sums in the
+    same ascending-column order as the CSR kernel (bitwise), fused epilogue
+    included; k not a multiple of the 16-byte run falls back to CSR."""
     A = mats[name]
     n = A.shape[0]
     rng = np.random.default_rng(31 + k)
@@ -101,8 +102,8 @@ def test_spmm_blocked_matches_csr(dev, mats, name, k, prec):
     Z = rng.uniform(-1, 1, (n, k))
     dt = np.float64 if prec == _fe.F64 else np.float32
     C = dev.csr(A)
-    built, reuse = C.build_blocked(32)
-    assert built and reuse >= 2.0
+    built, reuse = C.build_blocked(rb)
+    assert built and reuse >= 1.5
     Xd = dev.put(X, dt)
     Zd = dev.put(Z, dt)
     Y1 = dev.buffer((n, k), dt)
@@ -117,13 +118,37 @@ def test_spmm_blocked_matches_csr(dev, mats, name, k, prec):
     assert np.array_equal(Y1.download(), Y2.download())
 
 
+def test_spmm_blocked_ragged_tail(dev):
+    """n not a multiple of the row block, a row with no entries in the last
+    block, and a strided (ld > k) aligned block."""
+    A = G.hamiltonian_27pt(7, seed=1, wells=3).tolil()
+    n = A.shape[0]  # 343 = 4 * 85 + 3
+    A[n - 2, :] = 0.0
+    A[:, n - 2] = 0.0
+    A = A.tocsr()
+    A.eliminate_zeros()
+    k, ld = 40, 48
+    X = np.random.default_rng(5).uniform(-1, 1, (n, ld))
+    C = dev.csr(A)
+    built, _ = C.build_blocked(4)
+    assert built
+    C.use(_fe.FMT_BLOCKED)
+    Xd = dev.put(X)
+    Yd = dev.put(np.full((n, ld), 7.0))
+    _fe.check(L().fe_spmm(dev.ctx, C.h, _fe.F64, k, Xd.ptr, ld, Yd.ptr, ld, 1.0, 0.0, None, 0, 0.0))
+    got = Yd.download()
+    assert np.max(np.abs(got[:, :k] - orc.spmm(A, X[:, :k]))) <= 1e-13 * np.max(np.abs(A).sum(axis=1))
+    assert np.all(got[:, k:] == 7.0)
+    assert np.all(got[n - 2, :k] == 0.0)
+
+
 @pytest.mark.parametrize("name", ["powlaw", "lap2d", "cond3d"])
 def test_spmm_blocked_rejects_low_reuse(dev, mats, name):
-    """Scattered columns (power law) and thin stencils (5- and 7-point: about
-    1.5 nonzeros per union row of a 32-row block) get no blocked copy."""
+    """Scattered columns (power law) and thin stencils (5- and 7-point: under
+    1.5 nonzeros per union entry of a 4-row block) get no row-block copy."""
     C = dev.csr(mats[name])
-    built, reuse = C.build_blocked(32)
-    assert not built and reuse < 2.0
+    built, reuse = C.build_blocked(4)
+    assert not built and reuse < 1.5
     with pytest.raises(_fe.FeError):
         C.use(_fe.FMT_BLOCKED)
 
