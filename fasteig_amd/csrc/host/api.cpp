@@ -2,6 +2,7 @@
 #include "fasteig_hip.h"
 
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -9,6 +10,7 @@
 
 #include "fasteig_problem.hpp"
 #include "parseinput.hpp"
+#include "serving.hpp"
 
 struct feh_problem_s {
   std::unique_ptr<fasteig_problem> prob;
@@ -16,8 +18,51 @@ struct feh_problem_s {
   bool from_deck = false;
 };
 
+struct feh_server_s {
+  std::unique_ptr<fasteig::hip::BatchServer> srv;
+  std::deque<fasteig::hip::ServeResult> ready;
+};
+
 namespace {
 thread_local std::string g_err;
+
+fasteig::hip::ServeOptions parse_serve_options(const char* text) {
+  fasteig::hip::ServeOptions o;
+  std::string s = text ? text : "";
+  size_t pos = 0;
+  while (pos < s.size()) {
+    size_t end = s.find(';', pos);
+    if (end == std::string::npos) end = s.size();
+    const std::string kv = s.substr(pos, end - pos);
+    pos = end + 1;
+    if (kv.find_first_not_of(" \t") == std::string::npos) continue;
+    const size_t eq = kv.find('=');
+    if (eq == std::string::npos) throw std::invalid_argument("serve option without '=': " + kv);
+    std::string k = kv.substr(0, eq), v = kv.substr(eq + 1);
+    auto trim = [](std::string& x) {
+      x.erase(0, x.find_first_not_of(" \t"));
+      x.erase(x.find_last_not_of(" \t") + 1);
+    };
+    trim(k);
+    trim(v);
+    try {
+      if (k == "max_slots") o.max_slots = std::stoi(v);
+      else if (k == "degree") o.degree = std::stoi(v);
+      else if (k == "maxiter") o.maxiter = std::stoi(v);
+      else if (k == "hbm_budget_mb") o.hbm_budget = static_cast<size_t>(std::stod(v) * 1048576.0);
+      else if (k == "switch_tol") o.switch_tol = std::stod(v);
+      else if (k == "graph") o.use_graph = std::stoi(v) != 0;
+      else if (k == "keep_vectors") o.keep_vectors = std::stoi(v) != 0;
+      else if (k == "seed") o.seed = std::stoull(v);
+      else throw std::invalid_argument("unknown serve option: " + k);
+    } catch (const std::invalid_argument&) {
+      throw;
+    } catch (const std::exception&) {
+      throw std::invalid_argument("bad value for serve option " + k + ": " + v);
+    }
+  }
+  return o;
+}
 
 template <class F>
 int guarded(F&& f) {
@@ -121,6 +166,112 @@ int feh_problem_eigenvectors(feh_problem p, double* out) {
     if (!p || !out) throw std::invalid_argument("null argument");
     const MatrixXd& V = p->prob->eigenvectors();
     std::memcpy(out, V.data(), sizeof(double) * static_cast<size_t>(V.size()));
+    return 0;
+  });
+}
+
+size_t feh_serve_footprint(int n, int64_t nnz, int nev) {
+  return n > 0 && nev > 0 ? fasteig::hip::serve_footprint(n, nnz, nev) : 0;
+}
+
+int feh_server_create(int device, const char* options, feh_server* out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null argument");
+    const auto o = parse_serve_options(options);
+    auto h = std::make_unique<feh_server_s>();
+    h->srv = std::make_unique<fasteig::hip::BatchServer>(device, o);
+    *out = h.release();
+    return 0;
+  });
+}
+
+int feh_server_destroy(feh_server s) {
+  delete s;
+  return 0;
+}
+
+int feh_server_submit(feh_server s, int64_t id, int n, int64_t nnz, const int* rowptr, const int* colind,
+                      const double* vals, int nev, double tol) {
+  return guarded([&] {
+    if (!s || !rowptr || (nnz > 0 && (!colind || !vals)) || n <= 0 || nnz < 0)
+      throw std::invalid_argument("feh_server_submit: bad argument");
+    fasteig::hip::ServeRequest r;
+    r.id = id;
+    r.n = n;
+    r.rowptr.assign(rowptr, rowptr + n + 1);
+    r.colind.assign(colind, colind + nnz);
+    r.values.assign(vals, vals + nnz);
+    r.nev = nev;
+    r.tol = tol;
+    s->srv->submit(std::move(r));
+    return 0;
+  });
+}
+
+int feh_server_step(feh_server s) {
+  return guarded([&] {
+    if (!s) throw std::invalid_argument("null server");
+    const int left = s->srv->step();
+    for (auto& r : s->srv->take_results()) s->ready.push_back(std::move(r));
+    return left;
+  });
+}
+
+int feh_server_counts(feh_server s, int* active, int* queued, int* ready) {
+  return guarded([&] {
+    if (!s) throw std::invalid_argument("null server");
+    if (active) *active = s->srv->active();
+    if (queued) *queued = s->srv->queued();
+    if (ready) *ready = static_cast<int>(s->ready.size());
+    return 0;
+  });
+}
+
+int feh_server_peek(feh_server s, int64_t* id, int* status, int* n, int* nev, int* cycles, int* fp64_cycles,
+                    double* times) {
+  return guarded([&] {
+    if (!s) throw std::invalid_argument("null server");
+    if (s->ready.empty()) return 0;
+    const auto& r = s->ready.front();
+    if (id) *id = r.id;
+    if (status) *status = r.status;
+    if (n) *n = r.n;
+    if (nev) *nev = r.nev;
+    if (cycles) *cycles = r.cycles;
+    if (fp64_cycles) *fp64_cycles = r.fp64_cycles;
+    if (times) {
+      times[0] = r.queued_s;
+      times[1] = r.admitted_s;
+      times[2] = r.finished_s;
+    }
+    return 1;
+  });
+}
+
+int feh_server_pop(feh_server s, double* eigenvalues, double* residuals, double* eigenvectors) {
+  return guarded([&] {
+    if (!s) throw std::invalid_argument("null server");
+    if (s->ready.empty()) return 0;
+    auto& r = s->ready.front();
+    if (eigenvalues) std::copy(r.eigenvalues.begin(), r.eigenvalues.end(), eigenvalues);
+    if (residuals) std::copy(r.residuals.begin(), r.residuals.end(), residuals);
+    if (eigenvectors) std::copy(r.eigenvectors.begin(), r.eigenvectors.end(), eigenvectors);
+    s->ready.pop_front();
+    return 1;
+  });
+}
+
+int feh_server_stats(feh_server s, int64_t* cycles, int64_t* admitted, int64_t* retired, int64_t* peak_active,
+                     double* cycle_seconds, size_t* peak_bytes) {
+  return guarded([&] {
+    if (!s) throw std::invalid_argument("null server");
+    const auto& st = s->srv->stats();
+    if (cycles) *cycles = st.cycles;
+    if (admitted) *admitted = st.admitted;
+    if (retired) *retired = st.retired;
+    if (peak_active) *peak_active = st.peak_active;
+    if (cycle_seconds) *cycle_seconds = st.device_s;
+    if (peak_bytes) *peak_bytes = st.peak_bytes;
     return 0;
   });
 }

@@ -207,3 +207,21 @@ def congruence(K: sp.csr_matrix, mass: np.ndarray) -> sp.csr_matrix:
     d = 1.0 / np.sqrt(mass)
     A = sp.diags(d) @ K @ sp.diags(d)
     return _finish(A)
+
+
+def serving_stream(count: int = 512, seed: int = 11, rate: float = 200.0, n_min: float = 1e3,
+                   n_max: float = 2e5):
+    """Request stream of the serving workload: ``count`` requests with Poisson
+    arrivals at ``rate`` per second; each a 2D 5-point or 3D 7-point Laplacian
+    plus an N(0,1) diagonal with n log-uniform in [n_min, n_max], nev in
+    {8, 16, 32}, tol in {1e-6, 1e-8}. Returns dicts (id, arrival, kind, n_target,
+    nev, tol, seed); build the matrix with ``stencil_with_potential``."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count)) if rate > 0 else np.zeros(count)
+    out = []
+    for i in range(count):
+        out.append(dict(id=i, arrival=float(t[i]), kind="2d" if rng.random() < 0.5 else "3d",
+                        n_target=int(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))),
+                        nev=int(rng.choice([8, 16, 32])), tol=float(rng.choice([1e-6, 1e-8])),
+                        seed=int(rng.integers(1 << 30))))
+    return out

@@ -15,6 +15,7 @@
 #ifndef FASTEIG_HIP_H
 #define FASTEIG_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -48,6 +49,43 @@ int feh_problem_results(feh_problem p, double* eigenvalues, double* residual_nor
 int feh_problem_eigenvectors(feh_problem p, double* out);
 /* RAW RESULTS / DIAGNOSTICS text of the last solve into buf (truncated). */
 int feh_problem_results_text(feh_problem p, char* buf, int len);
+
+/* ---- continuous-batching solve server (no reference counterpart: the
+ * reference runs one solve per fasteig_problem::solve call) ------------------
+ * Requests (CSR matrix, nev, tol; smallest eigenpairs) queue on the host and
+ * are admitted in arrival order while their device footprint fits the HBM
+ * budget and a slot is free. Each step runs one cycle for every active
+ * problem with grouped launches (Chebyshev filter in FP32, FP64 after the
+ * precision switch; CholQR; Rayleigh-Ritz with FP64 Gram, Jacobi eigensolver
+ * and residuals), then retires converged problems. A problem's arithmetic is
+ * independent of the batch, so its results are bitwise the same alone and in
+ * any mix. Options "key=value;...": max_slots (128), degree (16), maxiter
+ * (300), hbm_budget_mb (75% of device memory), switch_tol (2e-5), graph (1:
+ * replay one captured graph per cycle), keep_vectors (1), seed (1). nev may be
+ * 1..32 and the subspace (2 nev rounded up to 4 columns) must be below n;
+ * other requests come back with status 2. */
+typedef struct feh_server_s* feh_server;
+
+/* Device bytes one request holds while resident (admission unit); host-only. */
+size_t feh_serve_footprint(int n, int64_t nnz, int nev);
+int feh_server_create(int device, const char* options, feh_server* out);
+int feh_server_destroy(feh_server s);
+/* Queue a request (arrays are copied). */
+int feh_server_submit(feh_server s, int64_t id, int n, int64_t nnz, const int* rowptr, const int* colind,
+                      const double* vals, int nev, double tol);
+/* One cycle; returns the number of requests still active or queued. */
+int feh_server_step(feh_server s);
+int feh_server_counts(feh_server s, int* active, int* queued, int* ready);
+/* Front finished result: returns 1 and its metadata (status 0 converged,
+ * 1 maxiter, 2 rejected; times = queued, admitted, finished on the server
+ * clock in seconds), or 0 when none is ready. */
+int feh_server_peek(feh_server s, int64_t* id, int* status, int* n, int* nev, int* cycles, int* fp64_cycles,
+                    double* times);
+/* Copy the front result (nev values, nev residuals, n x nev column-major
+ * vectors; NULL to skip) and drop it; returns 1, or 0 when none is ready. */
+int feh_server_pop(feh_server s, double* eigenvalues, double* residuals, double* eigenvectors);
+int feh_server_stats(feh_server s, int64_t* cycles, int64_t* admitted, int64_t* retired, int64_t* peak_active,
+                     double* cycle_seconds, size_t* peak_bytes);
 
 #ifdef __cplusplus
 }

@@ -191,6 +191,135 @@ int fe_zero_rows_small(fe_ctx ctx, int r, int p, double* C, int64_t ldc);
 int fe_event_record(fe_ctx ctx, int slot);
 int fe_event_elapsed_ms(fe_ctx ctx, int slot0, int slot1, float* ms);
 
+/* ---- runtime helpers for host loops that avoid per-call synchronization --- */
+/* Device arrays of a matrix (for grouped launch tables); any may be NULL. */
+int fe_csr_arrays(fe_csr A, const int** rowptr, const int** colind, const double** vals64,
+                  const float** vals32);
+/* Copies ordered on the context stream, without synchronizing. */
+int fe_memcpy_h2d_async(fe_ctx ctx, void* dst, const void* src, size_t bytes);
+int fe_memcpy_d2h_async(fe_ctx ctx, void* dst, const void* src, size_t bytes);
+/* Pinned host memory (staging for the async copies). */
+int fe_host_alloc(size_t bytes, void** ptr);
+int fe_host_free(void* ptr);
+/* Capture the work issued on the context stream between begin and end into
+ * a graph, and replay it (same kernels, same arguments) on that stream. */
+typedef struct fe_graph_s* fe_graph;
+int fe_graph_begin(fe_ctx ctx);
+int fe_graph_end(fe_ctx ctx, fe_graph* out);
+int fe_graph_launch(fe_ctx ctx, fe_graph g);
+int fe_graph_destroy(fe_graph g);
+
+/* X = uniform(-1, 1) from a counter-based generator (splitmix64 of seed, row
+ * and column): the same block for the same (seed, n, k) on any launch. */
+int fe_fill_random(fe_ctx ctx, int prec, int64_t n, int k, void* X, int64_t ldx, uint64_t seed);
+
+/* ---- grouped kernels of the serving loop (no reference counterpart: the
+ * reference solves one problem at a time) -------------------------------------
+ * A table is one device buffer: an fe_grp_header followed by `nitems` items of
+ * one kind. The kernels run on fixed grids and read the item count and tile
+ * counts from the table, so one captured graph serves any batch composition;
+ * each item's arithmetic depends only on that item (its own tiling and
+ * summation order), so a problem's results are bitwise the same alone and in
+ * any mix. The *_plan functions are host-only: they validate the items and
+ * fill the tiling fields and the header before the table is copied to the
+ * device. Blocks are row-major with 16-byte aligned rows (ld a multiple of
+ * 16 / element bytes); small matrices are FP64 column-major. */
+enum { FE_GRP_MAXCOLS = 64,   /* columns of a grouped block (m, p) */
+       FE_GRP_CHUNK = 256 };  /* rows per partial-sum chunk of gram / resid */
+
+typedef struct fe_grp_header {
+  int32_t nitems;
+  int32_t kind;                /* FE_GRP_KIND_* of the items */
+  int64_t ntiles;              /* work units over all items (filled by plan) */
+} fe_grp_header;
+enum { FE_GRP_KIND_SPMM = 1, FE_GRP_KIND_GRAM = 2, FE_GRP_KIND_RESID = 3, FE_GRP_KIND_GEMM = 4,
+       FE_GRP_KIND_SMALL = 5 };
+
+/* Y = alpha A X + beta X + gamma Z for each item (the fe_spmm operation). */
+typedef struct fe_grp_spmm {
+  int32_t n, k, prec, lpr;     /* lpr: lanes per row (filled by plan) */
+  int64_t tile0;               /* first tile (256 / lpr rows each; filled by plan) */
+  const int* rowptr;
+  const int* colind;
+  const void* vals;            /* FP64 or FP32 values, matching prec */
+  const void* X;
+  void* Y;
+  const void* Z;               /* may be NULL */
+  int64_t ldx, ldy, ldz;
+  double alpha, beta, gamma;
+} fe_grp_spmm;
+
+/* G = X^T Y (m x p, FP64, column-major ldg); part: scratch of
+ * ceil(n / FE_GRP_CHUNK) * m * p doubles. */
+typedef struct fe_grp_gram {
+  int32_t n, m, p, prec;
+  int64_t tile0;               /* first chunk (filled by plan) */
+  const void* X;
+  const void* Y;
+  int64_t ldx, ldy;
+  double* part;
+  double* G;
+  int64_t ldg;
+} fe_grp_gram;
+
+/* rnorm[j] = ||W[:, j] - theta[j] Y[:, j]||, ynorm[j] = ||Y[:, j]||, j < p;
+ * part: scratch of ceil(n / FE_GRP_CHUNK) * 2p doubles. */
+typedef struct fe_grp_resid {
+  int32_t n, p, prec, pad;
+  int64_t tile0;               /* first chunk (filled by plan) */
+  const void* Y;
+  const void* W;
+  int64_t ldy, ldw;
+  const double* theta;
+  double* part;
+  double* rnorm;
+  double* ynorm;
+} fe_grp_resid;
+
+/* Y = X C, X n x m (prec), C m x p FP64 column-major, Y n x p (prec); Y must
+ * not overlap X. */
+typedef struct fe_grp_gemm {
+  int32_t n, m, p, prec;
+  int64_t tile0;               /* first tile of 64 rows (filled by plan) */
+  const void* X;
+  int64_t ldx;
+  const double* C;
+  int64_t ldc;
+  void* Y;
+  int64_t ldy;
+} fe_grp_gemm;
+
+/* One small m x m FP64 problem per item (m <= FE_GRP_MAXCOLS):
+ *   fe_grp_cholinv: A = X^T X (overwritten); with D = diag(A)^(-1/2) and
+ *     D A D + s I = R^T R (s = 0 unless a pivot fails, then growing shifts),
+ *     C = D R^(-1), so X C has orthonormal columns to working precision;
+ *     info[0] = number of shifted retries (negative: no factorization).
+ *   fe_grp_eig: A symmetric (overwritten): eigenvalues ascending in w,
+ *     eigenvectors in the columns of C (cyclic Jacobi; info[0] = sweeps). */
+typedef struct fe_grp_small {
+  int32_t m, pad;
+  double* A;
+  int64_t lda;
+  double* C;
+  int64_t ldc;
+  double* w;
+  int* info;
+} fe_grp_small;
+
+int fe_grp_spmm_plan(fe_grp_header* hdr, fe_grp_spmm* items, int nitems);
+int fe_grp_gram_plan(fe_grp_header* hdr, fe_grp_gram* items, int nitems);
+int fe_grp_resid_plan(fe_grp_header* hdr, fe_grp_resid* items, int nitems);
+int fe_grp_gemm_plan(fe_grp_header* hdr, fe_grp_gemm* items, int nitems);
+int fe_grp_small_plan(fe_grp_header* hdr, fe_grp_small* items, int nitems);
+/* Launch on a device table (header + items). max_items bounds the grid of the
+ * per-item phases (the table may hold fewer). */
+int fe_grp_spmm_run(fe_ctx ctx, const void* table);
+int fe_grp_gram_run(fe_ctx ctx, const void* table, int max_items);
+int fe_grp_resid_run(fe_ctx ctx, const void* table, int max_items);
+int fe_grp_gemm_run(fe_ctx ctx, const void* table);
+int fe_grp_cholinv_run(fe_ctx ctx, const void* table, int max_items);
+int fe_grp_eig_run(fe_ctx ctx, const void* table, int max_items);
+
 #ifdef __cplusplus
 }
 #endif

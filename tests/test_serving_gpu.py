@@ -1,0 +1,124 @@
+"""Continuous-batching server (fasteig_amd.serving, feh_server_* C ABI).
+
+Parity: every result against a dense FP64 eigensolver (numpy eigh) on the
+same matrix: eigenvalues within 1e-9 (1 + |l|), residuals at or below the
+requested tol, eigenvector subspace angles within 1e-6 per non-degenerate
+pair. Batch independence: a request's eigenvalues, residuals, vectors and
+cycle counts are bitwise the same alone and in a mix, with graph replay and
+with eager launches.
+"""
+import numpy as np
+import pytest
+
+from fasteig_amd import generators as G
+from fasteig_amd.serving import Server, footprint
+
+pytestmark = pytest.mark.gpu
+
+
+def reqs():
+    return [
+        (101, G.stencil_with_potential("2d", 1600, 3), 8, 1e-8),
+        (102, G.stencil_with_potential("3d", 2744, 4), 16, 1e-8),
+        (103, G.stencil_with_potential("2d", 2500, 5), 16, 1e-6),
+        (104, G.stencil_with_potential("3d", 1728, 6), 32, 1e-8),
+        (105, G.stencil_with_potential("2d", 3600, 7), 8, 1e-6),
+        (106, G.stencil_with_potential("3d", 3375, 8), 8, 1e-8),
+    ]
+
+
+def run(server, items, stagger=0):
+    """Submit items (stagger: cycles between submissions) and drain."""
+    out = {}
+    pending = list(items)
+    while pending or server.pending():
+        if pending:
+            i, A, nev, tol = pending.pop(0)
+            server.submit(i, A, nev, tol)
+            for _ in range(stagger):
+                server.step()
+                out.update({r.id: r for r in server.results()})
+            continue
+        server.step()
+        out.update({r.id: r for r in server.results()})
+    return out
+
+
+def test_results_match_dense_eigh(dev):
+    items = reqs()
+    with Server(max_slots=16, degree=12) as srv:
+        res = run(srv, items)
+    for i, A, nev, tol in items:
+        r = res[i]
+        assert r.status == 0, (i, r.cycles)
+        w, V = np.linalg.eigh(A.toarray())
+        assert np.all(np.abs(r.eigenvalues - w[:nev]) <= 1e-9 * (1 + np.abs(w[:nev]))), i
+        assert r.residuals.max() <= tol
+        # true residuals of the returned vectors
+        X = r.eigenvectors
+        R = A @ X - X * r.eigenvalues
+        true = np.linalg.norm(R, axis=0) / ((1 + np.abs(r.eigenvalues)) * np.linalg.norm(X, axis=0))
+        assert true.max() <= 1.01 * tol + 1e-12
+        # subspace angles for pairs separated from their neighbours
+        gap = np.minimum(np.abs(np.diff(w[:nev + 1]))[:nev], np.r_[np.inf, np.abs(np.diff(w[:nev]))])
+        for j in np.where(gap > 1e-3)[0]:
+            c = abs(V[:, j] @ X[:, j]) / np.linalg.norm(X[:, j])
+            assert 1 - c <= 1e-6 + 10 * tol / gap[j], (i, j)
+
+
+def test_alone_equals_in_mix_bitwise(dev):
+    items = reqs()
+    probe = items[1]
+    with Server(max_slots=16, degree=12) as srv:
+        alone = run(srv, [probe])[probe[0]]
+    mix = items[2:] + [probe] + items[:1]
+    with Server(max_slots=16, degree=12) as srv:
+        res = run(srv, mix, stagger=3)
+    got = res[probe[0]]
+    assert got.cycles == alone.cycles and got.fp64_cycles == alone.fp64_cycles
+    assert np.array_equal(got.eigenvalues, alone.eigenvalues)
+    assert np.array_equal(got.residuals, alone.residuals)
+    assert np.array_equal(got.eigenvectors, alone.eigenvectors)
+
+
+def test_graph_replay_equals_eager(dev):
+    items = reqs()[:4]
+    with Server(max_slots=8, degree=10, graph=True) as srv:
+        a = run(srv, items, stagger=2)
+    with Server(max_slots=8, degree=10, graph=False) as srv:
+        b = run(srv, items, stagger=2)
+    for i, *_ in items:
+        assert a[i].cycles == b[i].cycles
+        assert np.array_equal(a[i].eigenvalues, b[i].eigenvalues)
+        assert np.array_equal(a[i].eigenvectors, b[i].eigenvectors)
+
+
+def test_admission_by_hbm_footprint(dev):
+    items = reqs()
+    fps = [footprint(A.shape[0], A.nnz, nev) for _, A, nev, _ in items]
+    budget = max(fps) + min(fps)  # never room for more than two at once
+    with Server(max_slots=16, degree=12, hbm_budget_mb=budget / 2**20) as srv:
+        res = run(srv, items)
+        st = srv.stats()
+    assert st["peak_active"] <= 2
+    assert st["peak_bytes"] <= budget
+    assert all(res[i].status == 0 for i, *_ in items)
+
+
+def test_continuous_admission(dev):
+    """A request that arrives while others run joins before they finish."""
+    items = reqs()
+    with Server(max_slots=16, degree=12) as srv:
+        res = run(srv, items, stagger=2)
+    first, last = res[items[0][0]], res[items[-1][0]]
+    assert last.admitted < first.finished
+
+
+def test_rejected_requests(dev):
+    A = G.stencil_with_potential("2d", 900, 1)
+    with Server(max_slots=4) as srv:
+        srv.submit(1, A, 40, 1e-8)          # nev too large for the grouped kernels
+        srv.submit(2, A[:40, :40].tocsr(), 32, 1e-8)  # subspace not smaller than n
+        srv.submit(3, A, 4, 1e-8)
+        res = run(srv, [])
+    assert res[1].status == 2 and res[2].status == 2 and res[3].status == 0
