@@ -10,12 +10,15 @@ block already resident in HBM. `value` is mixed-precision (FP32 filter and
 Gram, FP64 Cholesky / Rayleigh-Ritz / residuals / closing sweeps) solves per
 second; the plain-FP64 GPU path is timed beside it.
 
-Also measured: the fused Chebyshev step (the dominant HBM-bound kernel) with
-HIP events on the library's own stream, against algorithmic bytes
-nnz*(s+4) + 4(n+1) + 3 n k s; and the CPU oracle's FP64 fused step on the
-host cores (a bounded sample, extrapolated to solves/s).
+Also measured: the fused Chebyshev step (the dominant HBM-bound kernel, in
+the solve's own sparse format) with HIP events on the library's own stream,
+against algorithmic bytes nnz*(s+4) + 4(n+1) + 3 n k s; its HBM traffic from
+two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) calibrated on a 1 GiB
+copy; the CPU oracle's FP64 fused step on the host cores (a bounded sample,
+extrapolated to solves/s); the generalized 64^3 problem in plain FP64; and the
+serving stream (512 requests, continuous batching vs one at a time in FP64).
 
-    python bench.py [--steps K] [--warmup W] [--config c3|c2|small] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--config c3|c2|small|c5] [--gpus 1]
 """
 from __future__ import annotations
 
@@ -126,12 +129,159 @@ def cpu_sample(A, k, target_s=15.0):
     return (time.perf_counter() - t0) / reps, orc.max_threads()
 
 
+def measure_traffic(config, k, fmt, timeout=180):
+    """HBM bytes per launch of the fused step (FP32, k columns, format fmt):
+    two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch) over a child `bench.py --only-kernels` run, each scaled by the
+    same counter on the 1 GiB reference copy in that run (a copy reads and
+    writes exactly 1 GiB). Returns (bytes, detail) or (None, reason)."""
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    vals = {}
+    with tempfile.TemporaryDirectory(prefix="fe_pmc_") as tmp:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(tmp, ctr)
+            cmd = [prof, "--pmc", ctr, "--kernel-include-regex", "k_spmm|k_copy2d", "-d", out, "-o", "pmc", "--",
+                   sys.executable, os.path.abspath(__file__), "--config", config, "--only-kernels",
+                   "--kernel-k", str(k), "--kernel-precs", "fp32", "--kernel-formats", fmt]
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+            except subprocess.TimeoutExpired:
+                return None, f"{ctr} pass timed out"
+            dbs = [os.path.join(dp, f) for dp, _, fs in os.walk(out) for f in fs if f.endswith("_results.db")]
+            if r.returncode != 0 or not dbs:
+                return None, f"{ctr} pass failed (rc {r.returncode})"
+            con = sqlite3.connect(dbs[0])
+            rows = con.execute("select kernel_name, avg(value), count(*) from counters_collection "
+                               "where counter_name = ? group by kernel_name", (ctr,)).fetchall()
+            con.close()
+            spmm = [v for name, v, _ in rows if "k_spmm" in name and "float" in name]
+            copy = [v for name, v, _ in rows if "k_copy2d<float, float>" in name]
+            if not spmm or not copy:
+                return None, f"{ctr}: kernels not found in the counter database"
+            vals[ctr] = (spmm[0], copy[0])
+    gib = float(1 << 30)
+    rd = vals["FETCH_SIZE"][0] * gib / vals["FETCH_SIZE"][1]
+    wr = vals["WRITE_SIZE"][0] * gib / vals["WRITE_SIZE"][1]
+    detail = {"fetch_kib": vals["FETCH_SIZE"][0], "write_kib": vals["WRITE_SIZE"][0],
+              "copy_fetch_kib_per_gib": vals["FETCH_SIZE"][1], "copy_write_kib_per_gib": vals["WRITE_SIZE"][1],
+              "read_bytes": rd, "write_bytes": wr}
+    return rd + wr, detail
+
+
+def c2_bench(steps=2):
+    """Generalized K u = lambda M u, 7-point conductivity stencil on 64^3
+    (n = 262,144), diagonal M, lowest 64 pairs, Chebyshev block 96 degree 16,
+    tol 1e-8, plain FP64 on the GPU: seconds per solve and accuracy."""
+    from fasteig_amd import generators as G
+    from fasteig_amd.api import Problem
+    K, M = G.conductivity_3d(64, seed=2, mass_seed=3)
+    p = Problem(K, mass=M)
+    opts = dict(solver="Chebyshev", nev=64, blksize=96, degree=16, tol=1e-8, which="SR", maxiter=300, seed=1,
+                cheb_bounds="ritz", precision="fp64")
+    p.solve(**opts)
+    ts = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        r = p.solve(**opts)
+        ts.append(time.perf_counter() - t)
+    out = {"solves_per_s": steps / sum(ts), "s_per_solve": float(np.median(ts)), "iterations": r.iterations,
+           "max_resid": float(r.residual_norms.max()), "status": r.status}
+    log(f"c2 generalized fp64: {out['s_per_solve']:.3f} s/solve, {r.iterations} cycles, "
+        f"max resid {out['max_resid']:.2e}")
+    return out
+
+
+def serving_bench(count=512, rate=200.0, baseline=48, degree=16, max_slots=128, seed=11):
+    """Serving workload: `count` requests (2D 5-point / 3D 7-point Laplacian +
+    N(0,1) diagonal, n log-uniform in [1e3, 2e5], nev in {8,16,32}, tol in
+    {1e-6,1e-8}) with Poisson arrivals at `rate`/s, served by the
+    continuous-batching server in real time; then the first `baseline`
+    requests one at a time with the plain-FP64 Chebyshev solver (same degree,
+    ritz bounds), their latencies from a FIFO queue over the measured solve
+    times at the same arrivals. Eigenvectors are returned by both."""
+    from fasteig_amd import generators as G
+    from fasteig_amd.api import Problem
+    from fasteig_amd.serving import Server
+    stream = G.serving_stream(count, seed=seed, rate=rate)
+    t = time.time()
+    mats = [G.stencil_with_potential(r["kind"], r["n_target"], r["seed"]) for r in stream]
+    log(f"serving: {count} requests, n {min(A.shape[0] for A in mats)}..{max(A.shape[0] for A in mats)}, "
+        f"built in {time.time() - t:.1f}s")
+    # warm the kernels and the graph on two small requests
+    with Server(max_slots=max_slots, degree=degree) as w:
+        for j in range(2):
+            w.submit(10**6 + j, mats[j], stream[j]["nev"], stream[j]["tol"])
+        while w.pending():
+            w.step()
+            w.results()
+    srv = Server(max_slots=max_slots, degree=degree)
+    lat, status, cycles = {}, {}, {}
+    t0 = time.perf_counter()
+    i = 0
+    while len(lat) < count:
+        now = time.perf_counter() - t0
+        while i < count and stream[i]["arrival"] <= now:
+            srv.submit(i, mats[i], stream[i]["nev"], stream[i]["tol"])
+            i += 1
+        active, queued, _ = srv.counts()
+        if active == 0 and queued == 0 and i < count:
+            time.sleep(max(0.0, stream[i]["arrival"] - now))
+            continue
+        srv.step()
+        for r in srv.results():
+            lat[r.id] = time.perf_counter() - t0 - stream[r.id]["arrival"]
+            status[r.id] = r.status
+            cycles[r.id] = r.cycles
+    span = time.perf_counter() - t0 - stream[0]["arrival"]
+    st = srv.stats()
+    srv.close()
+    lats = np.array([lat[j] for j in range(count)])
+    out = {"requests": count, "arrival_rate_per_s": rate, "solves_per_s": count / span,
+           "p50_s": float(np.percentile(lats, 50)), "p99_s": float(np.percentile(lats, 99)),
+           "converged": int(sum(1 for v in status.values() if v == 0)), "cycles": st["cycles"],
+           "peak_active": st["peak_active"], "peak_hbm_gb": st["peak_bytes"] / 1e9,
+           "mean_problem_cycles": float(np.mean(list(cycles.values())))}
+    log(f"serving batched: {out['solves_per_s']:.1f} solves/s p50 {out['p50_s']:.3f}s p99 {out['p99_s']:.3f}s "
+        f"converged {out['converged']}/{count}, {st['cycles']} cycles, peak {st['peak_active']} active")
+    # one at a time, plain FP64
+    durs = []
+    p0 = None
+    for j in range(min(baseline, count)):
+        r = stream[j]
+        ts = time.perf_counter()
+        p = Problem(mats[j], share_with=p0)
+        res = p.solve(solver="Chebyshev", nev=r["nev"], tol=r["tol"], degree=degree, which="SR", maxiter=300,
+                      seed=1, cheb_bounds="ritz", precision="fp64")
+        p.eigenvectors()
+        durs.append(time.perf_counter() - ts)
+        if p0 is None:
+            p0 = p
+    durs = np.array(durs)
+    fin, lat1 = 0.0, []
+    for j, d in enumerate(durs):
+        fin = max(fin, stream[j]["arrival"]) + d
+        lat1.append(fin - stream[j]["arrival"])
+    one = {"requests": len(durs), "solves_per_s": len(durs) / durs.sum(),
+           "p50_s": float(np.percentile(lat1, 50)), "p99_s": float(np.percentile(lat1, 99)),
+           "mean_solve_s": float(durs.mean())}
+    log(f"serving one-at-a-time fp64 ({len(durs)} requests): {one['solves_per_s']:.1f} solves/s "
+        f"p50 {one['p50_s']:.3f}s p99 {one['p99_s']:.3f}s")
+    return {"batched": out, "one_at_a_time_fp64": one,
+            "throughput_ratio": out["solves_per_s"] / one["solves_per_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=["c3", "c2", "small"])
+    ap.add_argument("--config", default="c3", choices=["c3", "c2", "small", "c5"])
     ap.add_argument("--gram", default="native", choices=["native", "bf16x3"])
     ap.add_argument("--format", default="auto", choices=["auto", "csr", "blocked", "sell"],
                     help="sparse format of the solve (auto: row-block gather when the matrix allows)")
@@ -139,6 +289,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--only-kernels", action="store_true",
                     help="time only the fused Chebyshev step and a reference copy (for counter runs)")
+    ap.add_argument("--kernel-k", default="", help="block widths timed by --only-kernels (default nev, 2 nev)")
+    ap.add_argument("--kernel-precs", default="fp32,fp64", help="precisions timed by --only-kernels")
+    ap.add_argument("--no-extra", action="store_true", help="skip the generalized (C2) and serving (C5) runs")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--kernel-formats", default="csr,blocked,blocked8",
                     help="sparse formats timed by --only-kernels (csr, blocked = 4-row blocks, blocked8)")
     args = ap.parse_args()
@@ -151,13 +305,19 @@ def main():
     from fasteig_amd.api import Problem
 
     t0 = time.time()
+    if args.config == "c5":
+        print(json.dumps(serving_bench()), flush=True)
+        return
     if args.only_kernels:
         M, cfg = build_matrix(args.config)
         A = M[0] if isinstance(M, tuple) else M
         out = {}
-        for kk in (cfg["nev"], 2 * cfg["nev"]):
+        ks = [int(x) for x in args.kernel_k.split(",")] if args.kernel_k else [cfg["nev"], 2 * cfg["nev"]]
+        for kk in ks:
             for fmt in args.kernel_formats.split(","):
                 for pname, prec in (("fp32", _fe.F32), ("fp64", _fe.F64)):
+                    if pname not in args.kernel_precs.split(","):
+                        continue
                     per, algo = spmm_roofline(A, kk, prec, fmt=fmt)
                     out[f"{fmt}_{pname}_k{kk}_ms"] = per
                     out[f"{fmt}_{pname}_k{kk}_gbs"] = algo / (per * 1e-3) / 1e9
@@ -242,6 +402,13 @@ def main():
                          f"{cycles} cycles x {cfg['degree'] + 2} block products; dense work excluded"}
         log(f"cpu sample: {t_step:.2f}s per k={kc} step on {cores} threads")
 
+    if not args.no_extra:
+        extra["c2_generalized_fp64"] = c2_bench()
+        extra["c5_serving"] = serving_bench()
+    traffic, tdetail = (None, "skipped") if args.no_traffic else measure_traffic(args.config, k, fmt)
+    extra["traffic_detail"] = tdetail
+    log(f"traffic: {traffic} ({tdetail})")
+
     line = {
         "metric": "solves/s + p50/p99 latency at resid<=1e-8, mixed vs plain FP64; SpMM HBM GB/s",
         "value": value,
@@ -263,7 +430,7 @@ def main():
                       for m, v in lat.items()},
         "accuracy": out_acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"{'k_spmm_rb' if fmt == 'blocked' else 'k_spmm_csr'} fused Chebyshev step FP32 k={k}",
                      "algo_bytes": algo32},
         "cpu_baseline": cpu,
