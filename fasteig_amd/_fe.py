@@ -56,6 +56,25 @@ def lib():
         L.fe_csr_build_sell.argtypes = [vp, vp, i32, i32]
         L.fe_csr_use.argtypes = [vp, i32]
         L.fe_csr_build_blocked.argtypes = [vp, vp, i32, C.POINTER(dbl)]
+        L.fe_fill_random.argtypes = [vp, i32, i64, i32, vp, i64, C.c_uint64]
+        L.fe_scale_rows.argtypes = [vp, i32, i64, i32, vp, i64, vp]
+        L.fe_last_error.argtypes = []
+        L.fe_version.argtypes = []
+        L.fe_csr_arrays.argtypes = [vp, vp, vp, vp, vp]
+        L.fe_memcpy_h2d_async.argtypes = [vp, vp, vp, C.c_size_t]
+        L.fe_memcpy_d2h_async.argtypes = [vp, vp, vp, C.c_size_t]
+        L.fe_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+        L.fe_host_free.argtypes = [vp]
+        L.fe_graph_begin.argtypes = [vp]
+        L.fe_graph_end.argtypes = [vp, C.POINTER(vp)]
+        L.fe_graph_launch.argtypes = [vp, vp]
+        L.fe_graph_destroy.argtypes = [vp]
+        for nm in ("spmm", "gram", "resid", "gemm", "small"):
+            getattr(L, f"fe_grp_{nm}_plan").argtypes = [vp, vp, i32]
+        L.fe_grp_spmm_run.argtypes = [vp, vp]
+        L.fe_grp_gemm_run.argtypes = [vp, vp]
+        for nm in ("gram", "resid", "cholinv", "eig"):
+            getattr(L, f"fe_grp_{nm}_run").argtypes = [vp, vp, i32]
         L.fe_spmm.argtypes = [vp, vp, i32, i32, vp, i64, vp, i64, dbl, dbl, vp, i64, dbl]
         L.fe_gram.argtypes = [vp, i32, i32, i64, i32, i32, vp, i64, vp, i64, vp, vp, i64]
         L.fe_gemm_nn.argtypes = [vp, i32, i64, i32, i32, vp, i64, vp, i64, vp, i64, dbl, dbl]

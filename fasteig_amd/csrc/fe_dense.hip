@@ -62,71 +62,116 @@ __device__ __forceinline__ void gram_tile(int t, int tiles_b, int sym, int& ta, 
 }
 
 // ---------------------------------------------------------------- Gram (native)
+// Work unit u = chunk * ntiles + tile; units map to XCDs in contiguous ranges,
+// so all tiles of a row chunk run on one XCD and share its X / Y rows in L2.
+// A workgroup computes a 128 x 128 tile (4 wavefronts of 64 x 64 = 4 x 4 MFMA
+// blocks) over the chunk's rows, staging GKS rows of X and Y per pass with
+// 16-byte loads; the next pass's rows are loaded into registers while the
+// current one runs on MFMA.
+constexpr int WT = 128;   // workgroup tile edge (Gram, GEMM)
+constexpr int GKS = 16;   // rows per LDS stage (Gram)
+constexpr int GPAD = 4;   // LDS row padding (elements)
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_gram(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
+__device__ __forceinline__ void load8(const T* p, bool vec, int valid, T (&v)[8]) {
+  if (vec && valid >= 8) {
+    if constexpr (sizeof(T) == 4) {
+      const float4 a = *reinterpret_cast<const float4*>(p), b = *reinterpret_cast<const float4*>(p + 4);
+      __builtin_memcpy(&v[0], &a, 16);
+      __builtin_memcpy(&v[4], &b, 16);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double2 a = *reinterpret_cast<const double2*>(p + 2 * q);
+        v[2 * q] = a.x;
+        v[2 * q + 1] = a.y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = e < valid ? p[e] : T(0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_gram(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
                                               const T* __restrict__ Y, int64_t ldy,
                                               const double* __restrict__ w, int tiles_b, int ntiles,
-                                              int sym, int64_t rows_per,
-                                              double* __restrict__ part) {
-  __shared__ T Xs[KCG][TILE + PAD];
-  __shared__ T Ys[KCG][TILE + PAD];
+                                              int sym, int64_t rows_per, int64_t per_xcd, int64_t nunits,
+                                              int vec, double* __restrict__ part) {
+  __shared__ T Xs[GKS][WT + GPAD];
+  __shared__ T Ys[GKS][WT + GPAD];
+  const int64_t u = xcd_tile(blockIdx.x, per_xcd);
+  if (u >= nunits) return;
+  const int64_t chunk = u / ntiles;
+  const int tile = static_cast<int>(u - chunk * ntiles);
   int ta, tb;
-  gram_tile(blockIdx.x, tiles_b, sym, ta, tb);
-  const int a0 = ta * TILE, b0 = tb * TILE;
-  const int64_t chunk = blockIdx.y;
+  gram_tile(tile, tiles_b, sym, ta, tb);
+  const int a0 = ta * WT, b0 = tb * WT;
   const int64_t r0 = chunk * rows_per;
   const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   using acc_t = typename mfma<T>::acc;
-  acc_t acc[2][2];
+  acc_t acc[4][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{};
-  // each thread stages 8 elements of X and 8 of Y per stage: row = tid / 8, cols (tid % 8) * 8 ..
-  const int lr = tid >> 3, lc = (tid & 7) * 8;
-  for (int64_t rb = r0; rb < r1; rb += KCG) {
-    {
-      const int64_t r = rb + lr;
-      const bool rok = r < r1;
-      const double wr = (w && rok) ? w[r] : 1.0;
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{};
+  // staging: thread -> row tid / 16, columns (tid % 16) * 8 .. + 7 of the tile
+  const int sr = tid >> 4, sc = (tid & 15) * 8;
+  const int va = m - (a0 + sc), vb = p - (b0 + sc);
+  T xr[8], yr[8];
+  auto fetch = [&](int64_t rb) {
+    const int64_t r = rb + sr;
+    if (r < r1) {
+      load8(X + r * ldx + a0 + sc, vec, va, xr);
+      load8(Y + r * ldy + b0 + sc, vec, vb, yr);
+      if (w) {
+        const double wr = w[r];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int ca = a0 + lc + e, cb = b0 + lc + e;
-        Xs[lr][lc + e] = (rok && ca < m) ? X[r * ldx + ca] : T(0);
-        T yv = (rok && cb < p) ? Y[r * ldy + cb] : T(0);
-        if (w) yv = static_cast<T>(static_cast<double>(yv) * wr);
-        Ys[lr][lc + e] = yv;
+        for (int e = 0; e < 8; ++e) yr[e] = static_cast<T>(static_cast<double>(yr[e]) * wr);
       }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xr[e] = yr[e] = T(0);
     }
+  };
+  fetch(r0);
+  for (int64_t rb = r0; rb < r1; rb += GKS) {
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < KCG; kk += 4) {
+    for (int e = 0; e < 8; ++e) {
+      Xs[sr][sc + e] = xr[e];
+      Ys[sr][sc + e] = yr[e];
+    }
+    __syncthreads();
+    if (rb + GKS < r1) fetch(rb + GKS);
+#pragma unroll
+    for (int kk = 0; kk < GKS; kk += 4) {
       const int kr = kk + (lane >> 4);
-      T af[2], bf[2];
+      T af[4], bf[4];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = Xs[kr][wm * 32 + i * 16 + (lane & 15)];
+      for (int i = 0; i < 4; ++i) af[i] = Xs[kr][wm * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = Ys[kr][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 4; ++j) bf[j] = Ys[kr][wn * 64 + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
     }
-    __syncthreads();
   }
-  // partial tile, column-major 64 x 64: part[((chunk * ntiles) + blockIdx.x) * 4096 + col * 64 + row]
-  double* P = part + ((int64_t)chunk * ntiles + blockIdx.x) * (TILE * TILE);
+  // partial tile, column-major 128 x 128
+  double* P = part + u * (WT * WT);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = wm * 32 + i * 16 + mfma<T>::row(lane, e);
-        const int col = wn * 32 + j * 16 + (lane & 15);
-        P[col * TILE + row] = static_cast<double>(acc[i][j][e]);
+        const int row = wm * 64 + i * 16 + mfma<T>::row(lane, e);
+        const int col = wn * 64 + j * 16 + (lane & 15);
+        P[col * WT + row] = static_cast<double>(acc[i][j][e]);
       }
 }
 
@@ -222,17 +267,20 @@ __global__ __launch_bounds__(256) void k_gram_bf16x3(int64_t n, int m, int p, co
 }
 
 // Fixed-order sum over chunks; mirrors the lower triangle for symmetric Grams.
+// Grid: (tile slots, tsz * tsz / 256 element slices); one thread per element.
 __global__ void k_gram_reduce(int m, int p, int tiles_b, int ntiles, int chunks, const double* __restrict__ part,
-                              double* G, int64_t ldg, int sym) {
+                              double* G, int64_t ldg, int sym, int tsz) {
   const int t = blockIdx.x;  // computed tile slot
   int ta, tb;
   gram_tile(t, tiles_b, sym, ta, tb);
-  for (int e = threadIdx.x; e < TILE * TILE; e += blockDim.x) {
-    const int col = e / TILE, row = e % TILE;
-    const int a = ta * TILE + row, b = tb * TILE + col;
-    if (a >= m || b >= p) continue;
+  {
+    const int e = blockIdx.y * blockDim.x + threadIdx.x;
+    if (e >= tsz * tsz) return;
+    const int col = e / tsz, row = e % tsz;
+    const int a = ta * tsz + row, b = tb * tsz + col;
+    if (a >= m || b >= p) return;
     double s = 0.0;
-    for (int c = 0; c < chunks; ++c) s += part[((int64_t)c * ntiles + t) * (TILE * TILE) + e];
+    for (int c = 0; c < chunks; ++c) s += part[((int64_t)c * ntiles + t) * (tsz * tsz) + e];
     if (sym) {
       if (ta == tb) {
         if (a <= b) {
@@ -254,69 +302,83 @@ __global__ void k_gram_reduce(int m, int p, int tiles_b, int ntiles, int chunks,
 // halves makes G exactly symmetric.
 
 // ---------------------------------------------------------------- GEMM NN
+// Y[r0:r0+128, c0:c0+128] = alpha X[r0:r0+128, :] C[:, c0:c0+128] + beta Y.
+// Tiles map to XCDs in contiguous ranges with the column tiles of a row slab
+// consecutive, so the slab is read from HBM once and from L2 for the other
+// column tiles. X is staged [row][k] and C (column-major FP64, rounded to T)
+// [k][col], MKS values of k per stage, the next stage prefetched into
+// registers during the MFMA work.
+constexpr int MKS = 16;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
+__global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
                                                  const double* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
-                                                 T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles) {
-  __shared__ T Xs[TILE][KCM + PAD];
-  __shared__ T Cs[KCM][TILE + PAD];
+                                                 T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles,
+                                                 int vec) {
+  __shared__ T Xs[WT][MKS + 1];
+  __shared__ T Cs[MKS][WT + GPAD];
   const int64_t t = xcd_tile(blockIdx.x, per_xcd);
   if (t >= ntiles) return;
   const int64_t rt = t / tiles_c;
-  const int ct = static_cast<int>(t % tiles_c);
-  const int64_t r0 = rt * TILE;
-  const int c0 = ct * TILE;
+  const int ct = static_cast<int>(t - rt * tiles_c);
+  const int64_t r0 = rt * WT;
+  const int c0 = ct * WT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   using acc_t = typename mfma<T>::acc;
-  acc_t acc[2][2];
+  acc_t acc[4][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{};
-  // X stage: 64 rows x 16 k: thread -> row tid/4, k (tid%4)*4 .. +3
-  const int xr = tid >> 2, xk = (tid & 3) * 4;
-  // C stage: 16 k x 64 cols: thread -> col tid/4, k (tid%4)*4 .. +3 (C is column-major)
-  const int cc = tid >> 2, ck = (tid & 3) * 4;
-  for (int k0 = 0; k0 < m; k0 += KCM) {
-    {
-      const int64_t r = r0 + xr;
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{};
+  // staging: X row tid / 2, k (tid % 2) * 8 .. + 7; C column tid / 2, same k
+  const int sr = tid >> 1, sk = (tid & 1) * 8;
+  const int64_t xrow = r0 + sr;
+  const int ccol = c0 + sr;
+  T xr[8], cr[8];
+  auto fetch = [&](int k0) {
+    if (xrow < n) load8(X + xrow * ldx + k0 + sk, vec, m - (k0 + sk), xr);
+    else
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int kx = k0 + xk + e;
-        Xs[xr][xk + e] = (r < n && kx < m) ? X[r * ldx + kx] : T(0);
-      }
-      const int col = c0 + cc;
+      for (int e = 0; e < 8; ++e) xr[e] = T(0);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int kx = k0 + ck + e;
-        Cs[ck + e][cc] = (col < p && kx < m) ? static_cast<T>(C[(int64_t)col * ldc + kx]) : T(0);
-      }
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + sk + e;
+      cr[e] = (ccol < p && k < m) ? static_cast<T>(C[(int64_t)ccol * ldc + k]) : T(0);
     }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < m; k0 += MKS) {
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < KCM; kk += 4) {
+    for (int e = 0; e < 8; ++e) {
+      Xs[sr][sk + e] = xr[e];
+      Cs[sk + e][sr] = cr[e];
+    }
+    __syncthreads();
+    if (k0 + MKS < m) fetch(k0 + MKS);
+#pragma unroll
+    for (int kk = 0; kk < MKS; kk += 4) {
       const int kr = kk + (lane >> 4);
-      T af[2], bf[2];
+      T af[4], bf[4];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = Xs[wm * 32 + i * 16 + (lane & 15)][kr];
+      for (int i = 0; i < 4; ++i) af[i] = Xs[wm * 64 + i * 16 + (lane & 15)][kr];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = Cs[kr][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 4; ++j) bf[j] = Cs[kr][wn * 64 + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
     }
-    __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int64_t row = r0 + wm * 32 + i * 16 + mfma<T>::row(lane, e);
-        const int col = c0 + wn * 32 + j * 16 + (lane & 15);
+        const int64_t row = r0 + wm * 64 + i * 16 + mfma<T>::row(lane, e);
+        const int col = c0 + wn * 64 + j * 16 + (lane & 15);
         if (row < n && col < p) {
           T& y = Y[row * ldy + col];
           const T v = alpha * static_cast<T>(acc[i][j][e]);
@@ -336,13 +398,43 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
   FE_ARG(prec == FE_F64 || prec == FE_F32, "fe_gram: bad precision");
   FE_ARG(mode == FE_GRAM_NATIVE || (mode == FE_GRAM_BF16X3 && prec == FE_F32), "fe_gram: bad mode");
   if (m == 0 || p == 0) return 0;
-  const int tiles_a = (m + TILE - 1) / TILE, tiles_b = (p + TILE - 1) / TILE;
   const bool sym = (X == Y) && (m == p) && (ldx == ldy) && !w;
-  const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
+  const int symi = sym ? 1 : 0;
   if (n == 0) return fe_zero2d(ctx, G, p, m, ldg, 8);
+  const int cus = ctx->cus > 0 ? ctx->cus : 256;
+  if (mode == FE_GRAM_NATIVE) {
+    const int tiles_a = (m + WT - 1) / WT, tiles_b = (p + WT - 1) / WT;
+    const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
+    // about 4 work units per CU; chunks of whole stages
+    int64_t chunks = (4 * cus + ntiles - 1) / ntiles;
+    int64_t rows_per = (n + chunks - 1) / chunks;
+    rows_per = (rows_per + GKS - 1) / GKS * GKS;
+    chunks = (n + rows_per - 1) / rows_per;
+    void* ws = nullptr;
+    int rc = scratch(ctx, 0, sizeof(double) * (size_t)chunks * ntiles * WT * WT, &ws);
+    if (rc) return rc;
+    const int64_t units = chunks * ntiles, nb8 = (units + 7) / 8 * 8;
+    const int V = prec == FE_F64 ? 2 : 4;
+    const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && (reinterpret_cast<uintptr_t>(Y) % 16 == 0) &&
+                    ldx % V == 0 && ldy % V == 0;
+    if (prec == FE_F64)
+      hipLaunchKernelGGL(k_gram<double>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const double*)X,
+                         ldx, (const double*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec,
+                         (double*)ws);
+    else
+      hipLaunchKernelGGL(k_gram<float>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const float*)X,
+                         ldx, (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec,
+                         (double*)ws);
+    FE_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles, WT * WT / 256), dim3(256), 0, ctx->stream, m, p,
+                       tiles_b, ntiles, (int)chunks, (const double*)ws, G, ldg, symi, WT);
+    FE_CHECK_LAUNCH();
+    return 0;
+  }
+  const int tiles_a = (m + TILE - 1) / TILE, tiles_b = (p + TILE - 1) / TILE;
+  const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
   // split rows so that about 4 workgroups per CU are live
-  const int target = 4 * (ctx->cus > 0 ? ctx->cus : 256);
-  int64_t chunks = (target + ntiles - 1) / ntiles;
+  int64_t chunks = (4 * cus + ntiles - 1) / ntiles;
   int64_t rows_per = (n + chunks - 1) / chunks;
   if (rows_per < KCG) rows_per = KCG;
   rows_per = (rows_per + KCG - 1) / KCG * KCG;
@@ -352,20 +444,12 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
   int rc = scratch(ctx, 0, part_bytes, &ws);
   if (rc) return rc;
   double* part = (double*)ws;
-  const int symi = sym ? 1 : 0;
   const dim3 grid((unsigned)ntiles, (unsigned)chunks);
-  if (prec == FE_F64)
-    hipLaunchKernelGGL(k_gram<double>, grid, dim3(256), 0, ctx->stream, n, m, p, (const double*)X, ldx,
-                       (const double*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
-  else if (mode == FE_GRAM_NATIVE)
-    hipLaunchKernelGGL(k_gram<float>, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
-                       (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
-  else
-    hipLaunchKernelGGL(k_gram_bf16x3, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
-                       (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
+  hipLaunchKernelGGL(k_gram_bf16x3, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                     (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
   FE_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles), dim3(256), 0, ctx->stream, m, p, tiles_b, ntiles,
-                     (int)chunks, part, G, ldg, symi);
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles, TILE * TILE / 256), dim3(256), 0, ctx->stream, m, p,
+                     tiles_b, ntiles, (int)chunks, part, G, ldg, symi, TILE);
   FE_CHECK_LAUNCH();
   return 0;
 }
@@ -380,17 +464,19 @@ extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const v
     if (beta == 0.0) return fe_zero2d(ctx, Y, n, p, ldy, (int)prec_bytes(prec));
     return fe_axpby_cols(ctx, prec, n, p, Y, ldy, Y, ldy, 0.0, beta, nullptr);
   }
-  const int tiles_c = (p + TILE - 1) / TILE;
-  const int64_t tiles_r = (n + TILE - 1) / TILE;
+  const int tiles_c = (p + WT - 1) / WT;
+  const int64_t tiles_r = (n + WT - 1) / WT;
   const int64_t ntiles = tiles_r * tiles_c;
   const int64_t nb8 = (ntiles + 7) / 8 * 8;
+  const int V = prec == FE_F64 ? 2 : 4;
+  const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && ldx % V == 0;
   if (prec == FE_F64)
     hipLaunchKernelGGL(k_gemm_nn<double>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p,
-                       (const double*)X, ldx, C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles);
+                       (const double*)X, ldx, C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
   else
     hipLaunchKernelGGL(k_gemm_nn<float>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p,
                        (const float*)X, ldx, C, ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8,
-                       ntiles);
+                       ntiles, vec);
   FE_CHECK_LAUNCH();
   return 0;
 }

@@ -362,9 +362,11 @@ static int dispatch(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, 
   auto aligned = [&](const void* p, int64_t ld) {
     return ((reinterpret_cast<uintptr_t>(p) % 16) == 0) && (ld % V16 == 0);
   };
-  const bool vec = (k % V16 == 0) && aligned(X, ldx) && aligned(Y, ldy) && (!Z || aligned(Z, ldz));
+  const bool al = aligned(X, ldx) && aligned(Y, ldy) && (!Z || aligned(Z, ldz));
+  const bool vec = (k % V16 == 0) && al;
   const T alpha = static_cast<T>(a), beta = static_cast<T>(b), gamma = static_cast<T>(g);
-  if (A->use_bg && vec) return spmm_blocked(ctx, A, prec_of<T>::value, k, X, ldx, Y, ldy, a, b, Z, ldz, g);
+  // the row-block kernel handles a partial last run element-wise
+  if (A->use_bg && al) return spmm_blocked(ctx, A, prec_of<T>::value, k, X, ldx, Y, ldy, a, b, Z, ldz, g);
   const T* v = A->use_sell ? nullptr : (sizeof(T) == 8 ? (const T*)A->vals64 : (const T*)A->vals32);
   if (A->use_sell) {
     const T* sv = sizeof(T) == 8 ? (const T*)A->sell_v64 : (const T*)A->sell_v32;
@@ -382,7 +384,6 @@ static int dispatch(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, 
   }
   // CSR: 16-byte runs per lane when the block layout allows (a partial last
   // run is handled element-wise); up to 2 runs per lane for the widest blocks.
-  const bool al = aligned(X, ldx) && aligned(Y, ldy) && (!Z || aligned(Z, ldz));
 #define FE_CSR(LPR, RS, VEC) \
   return launch_csr<T, LPR, RS, VEC>(ctx, A, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, v)
   if (al) {

@@ -74,3 +74,11 @@ def test_deck_parser_reads_keys_and_matrix(tmp_path):
     with pytest.raises(ValueError):
         p.solve(nev=30)
     p.close()
+
+
+def test_python_binding_declares_every_kernel_entry_point():
+    """ctypes passes undeclared integers as 32-bit C ints: every entry point of
+    fe_hip.h must have its argument types declared in the binding."""
+    L = _fe.lib()
+    missing = [n for n in declared("fe_hip.h") if getattr(L, n).argtypes is None]
+    assert not missing, missing

@@ -127,19 +127,26 @@ def test_spmm_blocked_ragged_tail(dev):
     A[:, n - 2] = 0.0
     A = A.tocsr()
     A.eliminate_zeros()
-    k, ld = 40, 48
+    ld = 48
     X = np.random.default_rng(5).uniform(-1, 1, (n, ld))
     C = dev.csr(A)
     built, _ = C.build_blocked(4)
     assert built
-    C.use(_fe.FMT_BLOCKED)
-    Xd = dev.put(X)
-    Yd = dev.put(np.full((n, ld), 7.0))
-    _fe.check(L().fe_spmm(dev.ctx, C.h, _fe.F64, k, Xd.ptr, ld, Yd.ptr, ld, 1.0, 0.0, None, 0, 0.0))
-    got = Yd.download()
-    assert np.max(np.abs(got[:, :k] - orc.spmm(A, X[:, :k]))) <= 1e-13 * np.max(np.abs(A).sum(axis=1))
-    assert np.all(got[:, k:] == 7.0)
-    assert np.all(got[n - 2, :k] == 0.0)
+    for k in (40, 37, 3):  # 37, 3: partial last 16-byte run
+        for prec, dt, tol in ((_fe.F64, np.float64, 1e-13), (_fe.F32, np.float32, 2e-6)):
+            C.use(_fe.FMT_CSR)
+            Xd = dev.put(X, dt)
+            Y1 = dev.put(np.full((n, ld), 7.0), dt)
+            Y2 = dev.put(np.full((n, ld), 7.0), dt)
+            _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, ld, Y1.ptr, ld, 1.0, 0.5, Y1.ptr, ld, 0.25))
+            C.use(_fe.FMT_BLOCKED)
+            _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, ld, Y2.ptr, ld, 1.0, 0.5, Y2.ptr, ld, 0.25))
+            got = Y2.download().astype(np.float64)
+            Xr = X.astype(dt).astype(np.float64)
+            ref = orc.spmm(A, Xr[:, :k]) + 0.5 * Xr[:, :k] + 0.25 * 7.0
+            assert np.max(np.abs(got[:, :k] - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 2)
+            assert np.all(got[:, k:] == 7.0)
+            assert np.array_equal(Y1.download(), Y2.download())
 
 
 @pytest.mark.parametrize("name", ["powlaw", "lap2d", "cond3d"])
