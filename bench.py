@@ -59,6 +59,9 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
     L = _fe.lib()
     dev = _fe.Device(0)
     C = dev.csr(A)
+    if fmt == "sell":
+        C.build_sell(32, 256)
+        C.use(_fe.FMT_SELL)
     if fmt.startswith("blocked"):
         built, reuse = C.build_blocked(8 if fmt == "blocked8" else 4)
         if not built:
@@ -197,6 +200,39 @@ def c2_bench(steps=2):
     return out
 
 
+def c4_bench(maxiter=400):
+    """Power-law SPD matrix, n = 2,000,000 (row degrees alpha 2.1 on [4, 2000],
+    symmetrized, diagonal = row abs-sum + 1): the fused step at k = 48 in CSR
+    and SELL-32-sigma (load balance), then the lowest 32 pairs by LOBPCG
+    block 48 (plain FP64) and by Chebyshev block 48 with the mixed FP32 filter
+    and FP64 refinement, tol 1e-8."""
+    from fasteig_amd import _fe
+    from fasteig_amd import generators as G
+    from fasteig_amd.api import Problem
+    t = time.time()
+    A = G.power_law(2_000_000, seed=5)
+    log(f"c4: n={A.shape[0]} nnz={A.nnz} built in {time.time() - t:.1f}s")
+    out = {"n": A.shape[0], "nnz": int(A.nnz)}
+    for fmt in ("csr", "sell"):
+        for pname, prec in (("fp32", _fe.F32), ("fp64", _fe.F64)):
+            per, algo = spmm_roofline(A, 48, prec, fmt=fmt)
+            out[f"step_{fmt}_{pname}_k48_ms"] = per
+            out[f"step_{fmt}_{pname}_k48_gbs"] = algo / (per * 1e-3) / 1e9
+            log(f"c4 fused step {fmt} {pname} k=48: {per:.3f} ms {algo / (per * 1e-3) / 1e9:.0f} GB/s")
+    p = Problem(A)
+    for name, opts in (("lobpcg_fp64", dict(solver="LOBPCG", blksize=48, precision="fp64")),
+                       ("chebyshev_mixed", dict(solver="Chebyshev", blksize=48, degree=20, precision="mixed",
+                                                cheb_bounds="ritz"))):
+        t = time.perf_counter()
+        r = p.solve(nev=32, tol=1e-8, which="SR", maxiter=maxiter, seed=1, **opts)
+        dt = time.perf_counter() - t
+        out[name] = {"s": dt, "iterations": r.iterations, "status": r.status,
+                     "max_resid": float(r.residual_norms.max()), "lowest": float(r.eigenvalues[0])}
+        log(f"c4 {name}: {dt:.2f}s, {r.iterations} iterations, status {r.status}, "
+            f"max resid {r.residual_norms.max():.2e}")
+    return out
+
+
 def serving_bench(count=512, rate=200.0, baseline=48, degree=16, max_slots=128, seed=11):
     """Serving workload: `count` requests (2D 5-point / 3D 7-point Laplacian +
     N(0,1) diagonal, n log-uniform in [1e3, 2e5], nev in {8,16,32}, tol in
@@ -281,7 +317,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=["c3", "c2", "small", "c5"])
+    ap.add_argument("--config", default="c3", choices=["c3", "c2", "small", "c4", "c5"])
     ap.add_argument("--gram", default="native", choices=["native", "bf16x3"])
     ap.add_argument("--format", default="auto", choices=["auto", "csr", "blocked", "sell"],
                     help="sparse format of the solve (auto: row-block gather when the matrix allows)")
@@ -307,6 +343,9 @@ def main():
     t0 = time.time()
     if args.config == "c5":
         print(json.dumps(serving_bench()), flush=True)
+        return
+    if args.config == "c4":
+        print(json.dumps(c4_bench()), flush=True)
         return
     if args.only_kernels:
         M, cfg = build_matrix(args.config)

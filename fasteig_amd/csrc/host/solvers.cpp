@@ -38,13 +38,19 @@ bool all_finite(const std::vector<double>& v) {
     if (!std::isfinite(x)) return false;
   return true;
 }
+// Per-iteration progress (print_every > 0, verbosity >= 2): the largest
+// relative residual of the wanted pairs and the block precision; verbosity >= 3
+// adds every wanted value and residual.
 void progress(const fasteig_options& o, long long iter, const std::vector<double>& lam, const std::vector<double>& rr,
-              int count, double sign) {
+              int count, double sign, int prec = FE_F64) {
   if (o.print_every() <= 0 || o.verbosity() < 2 || iter % o.print_every() != 0) return;
-  std::cout << "  iter " << iter << ":";
-  for (int j = 0; j < count && j < static_cast<int>(lam.size()); ++j)
-    std::cout << " " << sign * lam[j] << "(" << rr[j] << ")";
-  std::cout << "\n";
+  double mx = 0.0;
+  for (int j = 0; j < count && j < static_cast<int>(rr.size()); ++j) mx = std::max(mx, rr[j]);
+  std::cout << "  iter " << iter << " [" << (prec == FE_F32 ? "fp32" : "fp64") << "] max relres " << mx;
+  if (o.verbosity() >= 3)
+    for (int j = 0; j < count && j < static_cast<int>(lam.size()); ++j)
+      std::cout << " " << sign * lam[j] << "(" << rr[j] << ")";
+  std::cout << std::endl;
 }
 // Start block of `cols` columns in precision prec: the cached seeded block,
 // with the first columns of options.v0 when provided (reference:
@@ -113,6 +119,9 @@ FASTEIG_INT ChebyshevSolver::solve() {
   hip::RitzPairs ritz;
   long long it_done = 0;
   double prev_max = std::numeric_limits<double>::infinity();
+  double last_mx = std::numeric_limits<double>::quiet_NaN();  // previous cycle's max residual
+  double best_rate = 0.0;  // fastest residual decay seen, decades per filter step
+  int cycle_degree = degree;  // filter degree of the coming cycle
   double cutoff = std::numeric_limits<double>::quiet_NaN();  // adaptive bound (ritz_bounds)
   hip::Small inv_norms(ex, capacity, 1);
 
@@ -130,7 +139,7 @@ FASTEIG_INT ChebyshevSolver::solve() {
         double sigma = e / (lower - c);
         const double s1 = sigma;
         hip::spmm(ex, A, prev, cur, sign * s1 / e, -c * s1 / e);
-        for (int k = 2; k <= degree; ++k) {
+        for (int k = 2; k <= cycle_degree; ++k) {
           const double sn = 1.0 / (2.0 / s1 - sigma);
           hip::spmm(ex, A, cur, prev, sign * 2.0 * sn / e, -c * 2.0 * sn / e, &prev, -sigma * sn);
           std::swap(prev, cur);
@@ -150,14 +159,14 @@ FASTEIG_INT ChebyshevSolver::solve() {
       }
       // 2. deflate against the locked Ritz vectors (ChebyshevSolver.cpp:159-166)
       if (locked > 0) hip::block_gram_schmidt(ws, V.cols(0, locked), cur, cur);
-      if (prec == FE_F32) {
-        // FP32 blocks: equilibrate the columns so the FP32 Gram of the filtered
-        // block is not squared-condition-number limited by their norm spread
+      if (prec == FE_F32 || cycle_degree > degree) {
+        // FP32 blocks and the long closing sweep: equilibrate the columns so
+        // the Gram of the filtered block is not limited by their norm spread
         const std::vector<double> nr = hip::column_norms(ex, cur);
         std::vector<double> inv(active);
         for (int j = 0; j < active; ++j) inv[j] = nr[j] > 0.0 ? 1.0 / nr[j] : 1.0;
         hip::check(fe_memcpy_h2d(ex.ctx(), inv_norms.data(), inv.data(), sizeof(double) * active), "h2d");
-        hip::check(fe_axpby_cols(ex.ctx(), FE_F32, n, active, cur.ptr, cur.ld, cur.ptr, cur.ld, 0.0, 0.0,
+        hip::check(fe_axpby_cols(ex.ctx(), prec, n, active, cur.ptr, cur.ld, cur.ptr, cur.ld, 0.0, 0.0,
                                  inv_norms.data()),
                    "scale");
       }
@@ -165,14 +174,37 @@ FASTEIG_INT ChebyshevSolver::solve() {
     }
     // CholQR of V[:, :subspace_cols] into Q (ChebyshevSolver.cpp:167-171); the
     // FP32 phase runs it twice (CholQR2) for orthogonality at FP32 accuracy
-    for (int pass = 0; pass < (prec == FE_F32 ? 2 : 1); ++pass) {
+    // (and CholQR2 after the long closing sweep); a failed first factorization
+    // is retried with a shift (shifted CholQR, Fukaya et al. 2020) and two
+    // more passes
+    int passes = (prec == FE_F32 || cycle_degree > degree) ? 2 : 1;
+    for (int pass = 0; pass < passes; ++pass) {
       ws.ensure(subspace_cols);
       const BlockRef Vb = pass == 0 ? V.cols(0, subspace_cols) : AY.cols(0, subspace_cols);
-      if (pass == 1) hip::copy(ex, Q.cols(0, subspace_cols), Vb);
+      if (pass > 0) hip::copy(ex, Q.cols(0, subspace_cols), Vb);
       hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap);
-      const int info = fe_potrf(ex.ctx(), subspace_cols, ws.G.data(), ws.cap);
+      int info = fe_potrf(ex.ctx(), subspace_cols, ws.G.data(), ws.cap);
       hip::check(info, "fe_potrf");
-      if (info != 0) throw std::runtime_error("ChebyshevSolver: Cholesky QR failed during filtering.");
+      if (info != 0 && passes < 4) {
+        hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap);
+        MatrixXd Gh = ws.G.to_host();
+        double tr = 0.0;
+        for (int i = 0; i < subspace_cols; ++i) tr += Gh(i, i);
+        const double u = prec == FE_F64 ? 0.5 * std::numeric_limits<double>::epsilon()
+                                        : 0.5 * std::numeric_limits<float>::epsilon();
+        const double sh = 11.0 * (static_cast<double>(n) * subspace_cols +
+                                  static_cast<double>(subspace_cols) * (subspace_cols + 1)) * u * tr;
+        for (int i = 0; i < subspace_cols; ++i) Gh(i, i) += sh;
+        ws.G.from_host(Gh);
+        info = fe_potrf(ex.ctx(), subspace_cols, ws.G.data(), ws.cap);
+        hip::check(info, "fe_potrf");
+        passes = std::min(4, pass + 3);
+      }
+      if (info != 0)
+        throw std::runtime_error("ChebyshevSolver: Cholesky QR failed during filtering (cycle " +
+                                 std::to_string(iter + 1) + ", pass " + std::to_string(pass + 1) + ", " +
+                                 (prec == FE_F32 ? "fp32" : "fp64") + ", degree " + std::to_string(cycle_degree) +
+                                 ", column " + std::to_string(info) + ").");
       hip::check(fe_chol_rinv(ex.ctx(), subspace_cols, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
       hip::gemm(ex, Vb, ws.R.data(), ws.cap, Q.cols(0, subspace_cols), 1.0, 0.0);
     }
@@ -186,7 +218,7 @@ FASTEIG_INT ChebyshevSolver::solve() {
     it_done = iter + 1;
     cutoff = ritz.values[subspace_cols - 1];
     const double mx = max_head(ritz.relres, nev);
-    progress(options_, it_done, ritz.values, ritz.relres, nev, sign);
+    progress(options_, it_done, ritz.values, ritz.relres, nev, sign, prec);
 
     if (prec == FE_F32) {
       // Mixed precision: FP32 cycles until the FP32 residuals stall or reach the
@@ -195,7 +227,17 @@ FASTEIG_INT ChebyshevSolver::solve() {
       const bool floor_reached = mx <= std::max(tol, 2e-5);
       const bool stalled = iter >= 2 && mx > 0.5 * prev_max;
       prev_max = std::min(prev_max, mx);
+      if (std::isfinite(last_mx) && last_mx > mx) best_rate = std::max(best_rate, std::log10(last_mx / mx) / degree);
       if (floor_reached || stalled || iter + 1 == maxiter) {
+        // The closing FP64 sweep: its filter degree is sized from the fastest
+        // residual decay per filter step of the FP32 cycles (a stall reflects
+        // the FP32 floor, not the filter), to bring the residual from mx to
+        // half the tolerance in one cycle (between degree and 2 x degree;
+        // further FP64 cycles follow at the base degree if needed).
+        if (ritz_bounds && best_rate > 0.0 && mx > tol) {
+          const double need = std::log10(mx / (0.5 * tol)) / best_rate;
+          cycle_degree = std::clamp(static_cast<int>(std::ceil(need)), degree, 2 * degree);
+        }
         prec = FE_F64;
         Block V64(ex, n, capacity, FE_F64);
         hip::copy(ex, V.ref(), V64.ref());
@@ -209,8 +251,10 @@ FASTEIG_INT ChebyshevSolver::solve() {
         locked = 0;
         if (iter + 1 == maxiter) --iter;  // always finish with an FP64 cycle
       }
+      last_mx = mx;
       continue;  // no locking on FP32 residuals
     }
+    cycle_degree = degree;
 
     // 4. ordered soft locking (ChebyshevSolver.cpp:188-192)
     locked = 0;
