@@ -126,7 +126,10 @@ FASTEIG_INT ChebyshevSolver::solve() {
   hip::Small inv_norms(ex, capacity, 1);
 
   for (int iter = 0; iter < maxiter; ++iter) {
-    const int active = subspace_cols - locked;
+    // the FP64 cycles of the mixed path refine the wanted columns only; the
+    // guard columns keep their FP32-phase vectors and stay in the
+    // Rayleigh-Ritz subspace
+    const int active = (mixed && prec == FE_F64) ? std::max(0, nev - locked) : subspace_cols - locked;
     // 1. filter the active columns (ChebyshevSolver.cpp:151-155, 223-258)
     if (active > 0 && upper != lower) {
       BlockRef prev = F.cols(0, active), cur = P.cols(0, active);
@@ -232,10 +235,10 @@ FASTEIG_INT ChebyshevSolver::solve() {
         // The closing FP64 sweep: its filter degree is sized from the fastest
         // residual decay per filter step of the FP32 cycles (a stall reflects
         // the FP32 floor, not the filter), to bring the residual from mx to
-        // half the tolerance in one cycle (between degree and 2 x degree;
+        // a quarter of the tolerance in one cycle (between degree and 2 x degree;
         // further FP64 cycles follow at the base degree if needed).
         if (ritz_bounds && best_rate > 0.0 && mx > tol) {
-          const double need = std::log10(mx / (0.5 * tol)) / best_rate;
+          const double need = std::log10(mx / (0.25 * tol)) / best_rate;
           cycle_degree = std::clamp(static_cast<int>(std::ceil(need)), degree, 2 * degree);
         }
         prec = FE_F64;
