@@ -70,7 +70,7 @@ __device__ __forceinline__ void gram_tile(int t, int tiles_b, int sym, int& ta, 
 // current one runs on MFMA.
 constexpr int WT = 128;   // workgroup tile edge (Gram, GEMM)
 constexpr int GKS = 16;   // rows per LDS stage (Gram)
-constexpr int GPAD = 4;   // LDS row padding (elements)
+constexpr int GPAD = 16;  // LDS row padding (elements): row stride = 16 banks mod 64
 
 template <typename T>
 __device__ __forceinline__ void load8(const T* p, bool vec, int valid, T (&v)[8]) {

@@ -349,6 +349,46 @@ static int launch_sell(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* 
   return 0;
 }
 
+// Sparse matrix-vector product (k = 1): a group of G lanes per row walks the
+// row's nonzeros with coalesced loads of values and indices, gathers x, and
+// reduces across the group with lane exchanges (CSR-vector layout).
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_spmv_csr(int n, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ colind, const T* __restrict__ vals,
+                                                  const T* __restrict__ x, int64_t ldx, T* y, int64_t ldy, T alpha,
+                                                  T beta, const T* z, int64_t ldz, T gamma, int64_t per_xcd,
+                                                  int64_t ntiles) {
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= ntiles) return;
+  const int sub = threadIdx.x % G;
+  const int64_t row = tile * (256 / G) + threadIdx.x / G;
+  const bool rowok = row < n;
+  const int beg = rowok ? rowptr[row] : 0;
+  const int end = rowok ? rowptr[row + 1] : 0;
+  T acc = T(0);
+  for (int p = beg + sub; p < end; p += G) acc += vals[p] * x[(int64_t)colind[p] * ldx];
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (!rowok || sub != 0) return;
+  T out = alpha * acc;
+  if (beta != T(0)) out += beta * x[row * ldx];
+  if (z && gamma != T(0)) out += gamma * z[row * ldz];
+  y[row * ldy] = out;
+}
+
+template <typename T>
+static int launch_spmv(fe_ctx ctx, fe_csr A, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
+                       const T* Z, int64_t ldz, T gamma) {
+  constexpr int G = 8;
+  const int64_t ntiles = (A->n + 256 / G - 1) / (256 / G);
+  const int64_t nb8 = (ntiles + 7) / 8 * 8;
+  const T* v = sizeof(T) == 8 ? (const T*)A->vals64 : (const T*)A->vals32;
+  hipLaunchKernelGGL((k_spmv_csr<T, G>), dim3((unsigned)nb8), dim3(256), 0, ctx->stream, A->n, A->rowptr,
+                     A->colind, v, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
 // row-block gather kernel (fe_spmm_bg.hip); needs 16-byte aligned rows
 int spmm_blocked(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
                  double alpha, double beta, const void* Z, int64_t ldz, double gamma);
@@ -365,6 +405,7 @@ static int dispatch(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, 
   const bool al = aligned(X, ldx) && aligned(Y, ldy) && (!Z || aligned(Z, ldz));
   const bool vec = (k % V16 == 0) && al;
   const T alpha = static_cast<T>(a), beta = static_cast<T>(b), gamma = static_cast<T>(g);
+  if (k == 1) return launch_spmv<T>(ctx, A, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma);
   // the row-block kernel handles a partial last run element-wise
   if (A->use_bg && al) return spmm_blocked(ctx, A, prec_of<T>::value, k, X, ldx, Y, ldy, a, b, Z, ldz, g);
   const T* v = A->use_sell ? nullptr : (sizeof(T) == 8 ? (const T*)A->vals64 : (const T*)A->vals32);

@@ -220,7 +220,8 @@ void lanczos_bounds(HipExecutor& ex, const HipCsrMatrix& A, double sign, unsigne
   const int64_t n = A.rows();
   const int steps = static_cast<int>(std::min<int64_t>(20, n));
   Workspace ws(ex, std::max(steps, 2), FE_GRAM_NATIVE);
-  Block r(ex, n, 1, FE_F64), av(ex, n, 1, FE_F64), basis(ex, n, steps, FE_F64);
+  Block r(ex, n, 1, FE_F64), av(ex, n, 1, FE_F64), basis(ex, n, steps, FE_F64), xb(ex, n, 1, FE_F64);
+  const BlockRef xv{xb.data(), n, 1, 1, FE_F64};  // unit-stride vector
   copy(ex, ex.random_block(n, 1, seed).ref(), r.ref());
   MatrixXd T(steps, steps);
   const double eps = std::numeric_limits<double>::epsilon();
@@ -254,7 +255,10 @@ void lanczos_bounds(HipExecutor& ex, const HipCsrMatrix& A, double sign, unsigne
     check(fe_axpby_cols(ex.ctx(), FE_F64, n, 1, r.data(), r.ld(), r.data(), r.ld(), 1.0 / beta, 0.0, nullptr),
           "scale");
     copy(ex, r.ref(), basis.cols(it, 1));
-    spmm(ex, A, basis.cols(it, 1), av.ref(), sign);
+    // the product reads the contiguous copy: its gathers then touch neighbouring
+    // entries instead of one entry per basis row
+    copy(ex, r.ref(), xv);
+    spmm(ex, A, xv, av.ref(), sign);
     const double alpha = two_pass(it + 1, av.ref(), r.ref());
     beta_next = column_norms(ex, r.ref())[0];
     if (!std::isfinite(beta_next))

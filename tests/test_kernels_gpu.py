@@ -160,6 +160,27 @@ def test_spmm_blocked_rejects_low_reuse(dev, mats, name):
         C.use(_fe.FMT_BLOCKED)
 
 
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_spmv_strided_column(dev, mats, prec):
+    """k = 1 (the Lanczos product) on one column of a wider block, fused terms."""
+    A = mats["ragged"]
+    n = A.shape[0]
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    X = np.random.default_rng(9).uniform(-1, 1, (n, 3)).astype(dt)
+    Z = np.random.default_rng(10).uniform(-1, 1, (n, 3)).astype(dt)
+    Xd, Yd = dev.put(X, dt), dev.put(Z, dt)
+    C = dev.csr(A)
+    es = np.dtype(dt).itemsize
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, 1, Xd.ptr + es, 3, Yd.ptr + 2 * es, 3, 0.5, -2.0, Yd.ptr + 2 * es, 3,
+                          0.75))
+    got = Yd.download().astype(np.float64)
+    x = X[:, 1].astype(np.float64)
+    ref = 0.5 * (A @ x) - 2.0 * x + 0.75 * Z[:, 2]
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    assert np.max(np.abs(got[:, 2] - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 3)
+    assert np.array_equal(got[:, :2], Z[:, :2].astype(np.float64))
+
+
 def test_spmm_strided_subblock(dev, mats):
     """Column sub-ranges with ld > k (unaligned) take the scalar path."""
     A = mats["lap2d"]
