@@ -921,6 +921,263 @@ Result davidson(const Csr& A, int nev, int maxblks, int maxiter, double tol, con
 
 // Final residuals rho_i = ||A x - lambda x|| / ((1+|lambda|) ||x||).
 // Reference: src/fasteig_solver.cpp:89-129.
+// ---------------------------------------------------------------- block Krylov-Schur
+
+Vec residual_norms(const Csr& A, const Vec& evals, const Mat& X);
+
+// Two-pass projection of one vector against the first `active` columns of V.
+// Reference: src/core/MultiVector.cpp:119-151 (orthogonalize_active_columns).
+void orth_vector(const Mat& V, int active, Vec& v) {
+  const int n = V.r;
+  for (int pass = 0; pass < 2; ++pass) {
+    Vec h(active, 0.0);
+    for (int j = 0; j < active; ++j) {
+      const double* c = V.col(j);
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) s += c[i] * v[i];
+      h[j] = s;
+    }
+    for (int j = 0; j < active; ++j) {
+      const double* c = V.col(j);
+      for (int i = 0; i < n; ++i) v[i] -= h[j] * c[i];
+    }
+  }
+}
+
+double norm2(const Vec& v) {
+  double s = 0.0;
+  for (double x : v) s += x * x;
+  return std::sqrt(s);
+}
+
+// Rank-aware block QR of `source` against basis[:, :active]: Cholesky QR when
+// it is accurate (R diagonal above 64 eps * reference_norm, Q^T Q = I and
+// basis^T Q = 0 to 1e-10), else column-by-column with pivoting over the source
+// columns and seeded random completion of missing directions; R = Q^T source
+// with roundoff-sized columns zeroed. Returns true when directions were
+// replenished. Reference: src/core/Orthogonalization.cpp:609-715.
+bool complete_block_qr(const Mat& basis, int active, const Mat& source, Mat& Q, Mat& R, unsigned seed,
+                       double reference_norm) {
+  const int n = source.r, b = source.c;
+  const int capacity = std::min(b, n - active);
+  const double threshold = 64.0 * std::numeric_limits<double>::epsilon() * reference_norm;
+  if (capacity == b) {
+    Mat Qc = source, Rc;
+    if (cholesky_qr(Qc, Rc) == 0) {
+      bool ok = true;
+      double dmin = std::numeric_limits<double>::infinity();
+      for (int j = 0; j < b; ++j) {
+        dmin = std::min(dmin, std::fabs(Rc(j, j)));
+        for (int i = 0; i < b; ++i) ok = ok && std::isfinite(Rc(i, j));
+      }
+      Mat G = AtB(Qc, b, Qc, b);
+      double ge = 0.0;
+      for (int j = 0; j < b; ++j)
+        for (int i = 0; i < b; ++i) {
+          const double d = G(i, j) - (i == j ? 1.0 : 0.0);
+          ge += d * d;
+        }
+      double ce = 0.0;
+      if (active > 0) {
+        Mat Cr = AtB(basis, active, Qc, b);
+        for (double x : Cr.a) ce += x * x;
+      }
+      if (ok && dmin > threshold && std::isfinite(ge) && std::isfinite(ce) && std::sqrt(ge) < 1e-10 &&
+          std::sqrt(ce) < 1e-10) {
+        Q = std::move(Qc);
+        R = std::move(Rc);
+        return false;
+      }
+    }
+  }
+  bool replenished = false;
+  Q = Mat(n, b);
+  auto project = [&](int j, Vec& v) {
+    orth_vector(basis, active, v);
+    orth_vector(Q, j, v);
+  };
+  for (int j = 0; j < capacity; ++j) {
+    double length = 0.0, original_scale = 0.0;
+    Vec best(n, 0.0), v(n);
+    for (int col = 0; col < b; ++col) {
+      std::copy(source.col(col), source.col(col) + n, v.begin());
+      original_scale = std::max(original_scale, norm2(v));
+      project(j, v);
+      const double cn = norm2(v);
+      if (cn > length) {
+        length = cn;
+        best = v;
+      }
+    }
+    const double cutoff = std::max(threshold, 1e-12 * original_scale);
+    if (length > cutoff) {
+      v = best;
+      length = norm2(v);
+    } else {
+      replenished = true;
+      length = 0.0;
+      for (unsigned attempt = 0; attempt < 4; ++attempt) {
+        Mat rv = random_block(n, 1, seed + 17u * static_cast<unsigned>(j + 1) + attempt);
+        std::copy(rv.col(0), rv.col(0) + n, v.begin());
+        const double rn = norm2(v);
+        project(j, v);
+        length = norm2(v);
+        if (length > 1e-12 * rn) break;
+        length = 0.0;
+      }
+      if (length == 0.0) throw std::runtime_error("complete_block_qr: unable to complete the basis.");
+    }
+    for (int i = 0; i < n; ++i) Q(i, j) = v[i] / length;
+  }
+  R = AtB(Q, b, source, b);
+  for (int j = 0; j < b; ++j) {
+    double cn = 0.0;
+    for (int i = 0; i < b; ++i) cn += R(i, j) * R(i, j);
+    if (std::sqrt(cn) <= threshold)
+      for (int i = 0; i < b; ++i) R(i, j) = 0.0;
+  }
+  return replenished;
+}
+
+// Block Krylov-Schur with thick restart (standard problems, which = sr/lr/sm/
+// lm/be). Reference: src/BlockKrylovSchurSolver.cpp:26-214 with
+// src/core/KrylovSubspace.cpp:261-433 (basis expansion and restart),
+// src/core/LanczosWorkspace.cpp:42-73,109-135 (projected matrix),
+// src/core/RitzExtraction.cpp:272-380 (block Ritz estimates),
+// src/core/RitzExtraction.cpp:34-63 (final residuals).
+Result block_krylov_schur(const Csr& A, int nev, int bsz, int maxdim, int maxiter, double tol,
+                          const std::string& which, unsigned seed) {
+  const int n = A.n;
+  if (n <= 0) throw std::invalid_argument("BlockKrylovSchurSolver: n must be > 0.");
+  if (nev <= 0 || nev > n) throw std::invalid_argument("BlockKrylovSchurSolver: nev must be in [1,n].");
+  if (maxiter <= 0) throw std::invalid_argument("BlockKrylovSchurSolver: maxiter must be > 0.");
+  if (bsz <= 0 || bsz > n) throw std::invalid_argument("BlockKrylovSchurSolver: blksize must be in [1,n].");
+  const int requested_dim = maxdim > 0 ? maxdim : std::max(2 * nev, 20);
+  const int block_count = requested_dim >= n ? 1 + (n - 1) / bsz : requested_dim / bsz;
+  const int cycle_dim = std::min(n, block_count * bsz);
+  const int wanted_blocks = 1 + (nev - 1) / bsz;
+  if (cycle_dim < n && block_count <= wanted_blocks)
+    throw std::invalid_argument("BlockKrylovSchurSolver: maxdim must accommodate ceil(nev/blksize) retained blocks "
+                                "plus at least one expansion block (or the full matrix dimension).");
+  const int pdim = (block_count + 1) * bsz;
+  const int retained_blocks = cycle_dim == n ? 0 : std::max(wanted_blocks, block_count / 2);
+  const int retained_dim = retained_blocks * bsz;
+  const int snap_count = std::max(nev, retained_dim);
+
+  Mat Qb(n, pdim);   // basis
+  Mat T(pdim, pdim); // projected matrix
+  Mat beta(bsz, bsz);
+  Mat init = random_block(n, bsz, seed);
+  {
+    Mat Q0, R0;
+    complete_block_qr(Qb, 0, init, Q0, R0, seed, 0.0);
+    copy_cols(Qb, 0, Q0, 0, bsz);
+  }
+  auto expand = [&](int first) {
+    for (int blk = first; blk < block_count; ++blk) {
+      Mat Z(n, bsz);
+      spmm(A, Qb, blk * bsz, bsz, Z, 1.0, 0.0);
+      const int active = std::min(n, (blk + 1) * bsz);
+      Mat Rz, H;
+      block_gram_schmidt(Qb, active, Z, Rz, H);
+      const int width = 1 + (active - 1) % bsz;
+      Mat D(bsz, bsz);
+      for (int j = 0; j < width; ++j)
+        for (int i = 0; i < width; ++i) D(i, j) = H(active - width + i, j);
+      double scale = 0.0;
+      for (double x : column_norms(Z, bsz)) scale += x * x;
+      scale = std::sqrt(scale);
+      Mat Qn, B;
+      const int nf = std::min((blk + 1) * bsz, n);
+      complete_block_qr(Qb, nf, Rz, Qn, B, seed + static_cast<unsigned>((blk + 1) * bsz), scale);
+      copy_cols(Qb, (blk + 1) * bsz, Qn, 0, bsz);
+      const int row = blk * bsz, nrow = row + bsz;
+      for (int j = 0; j < bsz; ++j)
+        for (int i = 0; i < bsz; ++i) {
+          T(row + i, row + j) = D(i, j);
+          T(nrow + i, row + j) = B(i, j);
+          T(row + j, nrow + i) = B(i, j);
+        }
+      beta = B;
+    }
+  };
+  expand(0);
+  Result res;
+  Vec lam_keep;
+  Mat C_keep;
+  for (int iter = 1; iter <= maxiter; ++iter) {
+    // block Ritz snapshot of T[:m, :m]
+    const int m = cycle_dim;
+    Mat P(m, m);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) P(i, j) = T(i, j);
+    Vec w;
+    Mat V;
+    if (syev(P, w, V) != 0) throw std::runtime_error("BlockKrylovSchurSolver: projected eigensolve failed.");
+    const auto idx = sort_indices(w, which);
+    const int cnt = std::min(snap_count, m);
+    const int tail = 1 + (m - 1) % bsz;
+    Vec lam(cnt), rr(cnt);
+    Mat Csel(m, cnt);
+    for (int j = 0; j < cnt; ++j) {
+      const int c = idx[j];
+      lam[j] = w[c];
+      Vec ylast(bsz, 0.0);
+      for (int t = 0; t < tail; ++t) ylast[t] = V(m - tail + t, c);
+      double a = 0.0;
+      for (int i = 0; i < bsz; ++i) {
+        double s = 0.0;
+        for (int t = 0; t < bsz; ++t) s += beta(i, t) * ylast[t];
+        a += s * s;
+      }
+      rr[j] = std::sqrt(a) / (1.0 + std::fabs(w[c]));
+      std::copy(V.col(c), V.col(c) + m, Csel.col(j));
+    }
+    res.iterations = iter;
+    double mx = 0.0;
+    for (int j = 0; j < std::min(nev, cnt); ++j) mx = std::max(mx, rr[j]);
+    bool converged = mx < tol;
+    const bool at_limit = iter == maxiter || cycle_dim == n;
+    if (converged || at_limit) {
+      Mat Cn(m, nev);
+      copy_cols(Cn, 0, Csel, 0, nev);
+      Mat X = AC(Qb, m, Cn);
+      Vec ev(lam.begin(), lam.begin() + nev);
+      Vec r = residual_norms(A, ev, X);
+      converged = true;
+      for (double x : r) converged = converged && x <= tol;
+      res.evals = ev;
+      res.relres = r;
+      res.evecs = X;
+      res.status = converged ? 0 : 1;
+      break;
+    }
+    // thick restart: Q <- [Q C_keep, Q_residual], T <- [diag(theta), F^T; F, *]
+    Mat coupling(bsz, cnt);
+    for (int j = 0; j < cnt; ++j)
+      for (int i = 0; i < bsz; ++i) {
+        double s = 0.0;
+        for (int t = 0; t < bsz; ++t) s += beta(i, t) * Csel(m - bsz + t, j);
+        coupling(i, j) = s;
+      }
+    Mat U = AC(Qb, m, Csel);
+    Mat Qres(n, bsz);
+    copy_cols(Qres, 0, Qb, block_count * bsz, bsz);
+    Qb = Mat(n, pdim);
+    copy_cols(Qb, 0, U, 0, cnt);
+    copy_cols(Qb, cnt, Qres, 0, bsz);
+    T = Mat(pdim, pdim);
+    for (int j = 0; j < cnt; ++j) T(j, j) = lam[j];
+    for (int j = 0; j < cnt; ++j)
+      for (int i = 0; i < bsz; ++i) {
+        T(cnt + i, j) = coupling(i, j);
+        T(j, cnt + i) = coupling(i, j);
+      }
+    expand(retained_blocks);
+  }
+  return res;
+}
+
 Vec residual_norms(const Csr& A, const Vec& evals, const Mat& X) {
   const int k = static_cast<int>(evals.size());
   Mat AX(X.r, k);
@@ -1104,6 +1361,15 @@ int orc_davidson(int n, const int* rp, const int* ci, const double* v, int nev, 
                  long* iters) {
   return guarded([&] {
     auto r = orc::davidson(csr(n, rp, ci, v), nev, maxblks, maxiter, tol, which, seed);
+    return finish(r, nev, n, ev, evec, res, iters);
+  });
+}
+
+int orc_block_krylov_schur(int n, const int* rp, const int* ci, const double* v, int nev, int block, int maxdim,
+                           int maxiter, double tol, const char* which, unsigned seed, double* ev, double* evec,
+                           double* res, long* iters) {
+  return guarded([&] {
+    auto r = orc::block_krylov_schur(csr(n, rp, ci, v), nev, block, maxdim, maxiter, tol, which, seed);
     return finish(r, nev, n, ev, evec, res, iters);
   });
 }

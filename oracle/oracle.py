@@ -50,6 +50,8 @@ def lib():
         L.orc_chebyshev.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_lobpcg.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_davidson.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
+        L.orc_block_krylov_schur.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int] + \
+            solver_tail
         L.orc_residual_norms.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_max_threads.restype = C.c_int
@@ -193,6 +195,12 @@ def lobpcg(A, nev, block=0, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vec
 
 def davidson(A, nev, maxblks=10, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True):
     return _solve(lib().orc_davidson, A, nev, (maxblks,), maxiter, tol, which, seed, want_vectors)
+
+
+def block_krylov_schur(A, nev, block=10, maxdim=0, maxiter=1000, tol=1e-8, which="sr", seed=1,
+                       want_vectors=True):
+    """Block Krylov-Schur with thick restart (maxdim 0: max(2 nev, 20))."""
+    return _solve(lib().orc_block_krylov_schur, A, nev, (block, maxdim), maxiter, tol, which, seed, want_vectors)
 
 
 def residual_norms(A, evals, X):

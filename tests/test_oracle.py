@@ -185,3 +185,52 @@ def test_sio_golden_spectrum_lobpcg():
     assert np.all(np.abs(ev - ref) <= 1e-10 + 1e-10 * np.abs(ref))
     rn = orc.residual_norms(A, ev, r["eigenvectors"])
     assert rn.max() <= 1e-8
+
+
+def test_sio_golden_spectrum_block_krylov_schur():
+    """The reference's CTest case reference_cpu_SiO_BlockKrylovSchur_SR (k=5,
+    block 10, maxdim 40, tol 1e-8, seed 1; tests/scripts/run_case.py defaults)
+    against the ScaLAPACK spectrum."""
+    A = load_sio()
+    ref = json.load(open(os.path.join(GOLD, "sio_eigs.json")))["eigenvalues"][:5]
+    r = orc.block_krylov_schur(A, 5, block=10, maxdim=40, maxiter=2000, tol=1e-8, which="sr", seed=1)
+    assert r["status"] == 0
+    assert np.all(np.abs(r["eigenvalues"] - ref) <= 1e-10 + 1e-10 * np.abs(ref))
+    assert orc.residual_norms(A, r["eigenvalues"], r["eigenvectors"]).max() <= 1e-8
+
+
+@pytest.mark.parametrize("which", ["sr", "lr"])
+def test_block_krylov_schur_closed_form(which):
+    A = G.laplacian_2d(24, 20)
+    ex = np.sort(G.laplacian_2d_eigenvalues(24, 20))
+    ex = ex[:6] if which == "sr" else ex[::-1][:6]
+    r = orc.block_krylov_schur(A, 6, block=4, maxdim=32, maxiter=3000, tol=1e-10, which=which, seed=3)
+    assert r["status"] == 0
+    assert np.all(np.abs(r["eigenvalues"] - ex) <= 1e-9 * np.abs(ex))
+
+
+def test_block_krylov_schur_full_dimension_partial_block():
+    """maxdim >= n: the basis covers the whole space with a partial last block
+    (n = 25, block 4) and the Ritz values are the exact spectrum."""
+    rng = np.random.default_rng(8)
+    M = rng.standard_normal((25, 25))
+    A = sp.csr_matrix(M + M.T)
+    w = np.linalg.eigvalsh(A.toarray())
+    r = orc.block_krylov_schur(A, 5, block=4, maxdim=40, maxiter=5, tol=1e-10, which="sr", seed=1)
+    assert r["iterations"] == 1
+    assert np.all(np.abs(r["eigenvalues"] - w[:5]) <= 1e-11 * np.abs(w).max())
+    assert orc.residual_norms(A, r["eigenvalues"], r["eigenvectors"]).max() <= 1e-12
+
+
+def test_block_krylov_schur_invariant_start_replenishes():
+    """A = diag(1 x 10, 2 x 10, 3 x 10): the Krylov space of a 3-column start
+    block has dimension at most 9 (3 directions per distinct eigenvalue), so
+    the block QR must complete the basis with seeded random directions to find
+    four copies of the eigenvalue 1."""
+    d = np.r_[np.ones(10), 2.0 * np.ones(10), 3.0 * np.ones(10)]
+    A = sp.diags(d).tocsr()
+    r = orc.block_krylov_schur(A, 4, block=3, maxdim=15, maxiter=200, tol=1e-10, which="sr", seed=2)
+    assert r["status"] == 0
+    assert np.all(np.abs(r["eigenvalues"] - 1.0) <= 1e-12)
+    X = r["eigenvectors"]
+    assert np.max(np.abs(X.T @ X - np.eye(4))) <= 1e-10
