@@ -46,6 +46,18 @@ class ChebyshevSolver final : public fasteig_solver {
   FASTEIG_INT solve() override;
 };
 
+// Block Krylov-Schur with thick restart (src/BlockKrylovSchurSolver.cpp:26-214):
+// block Lanczos expansion with full two-pass block Gram-Schmidt and a
+// rank-aware block QR, block Ritz estimates from the coupling block, restart
+// keeping max(ceil(nev/blksize), blocks/2) blocks of Ritz vectors. Standard
+// problems, which = SR, LR, SM, LM, BE (shift-invert needs a sparse direct
+// solver and is not part of the HIP backend).
+class BlockKrylovSchurSolver final : public fasteig_solver {
+ public:
+  BlockKrylovSchurSolver(::fasteig_problem& problem, const fasteig_options& options);
+  FASTEIG_INT solve() override;
+};
+
 // LOBPCG without preconditioner (src/LOBPCGSolver.cpp:25-173). Extension:
 // blksize > nev widens the block; convergence and output use the first nev.
 class LOBPCGSolver final : public fasteig_solver {

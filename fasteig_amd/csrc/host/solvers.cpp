@@ -470,14 +470,317 @@ FASTEIG_INT DavidsonSolver::solve() {
   return status_;
 }
 
+// ============================================================== Block Krylov-Schur
+// Reference: src/BlockKrylovSchurSolver.cpp:26-214, src/core/KrylovSubspace.cpp:261-433,
+// src/core/Orthogonalization.cpp:609-715 (complete_block_qr),
+// src/core/LanczosWorkspace.cpp:42-73,109-135, src/core/RitzExtraction.cpp:34-63,272-380.
+BlockKrylovSchurSolver::BlockKrylovSchurSolver(::fasteig_problem& problem, const fasteig_options& options)
+    : fasteig_solver(problem, options) {}
+
+FASTEIG_INT BlockKrylovSchurSolver::solve() {
+  options_.validate();
+  const int64_t n = problem_.n();
+  const int nev = options_.nev();
+  const int maxiter = options_.maxiter();
+  const double tol = options_.tol();
+  const std::string which = lower_copy(options_.which());
+  if (n <= 0) throw std::invalid_argument("BlockKrylovSchurSolver: n must be > 0.");
+  if (nev <= 0 || nev > n) throw std::invalid_argument("BlockKrylovSchurSolver: nev must be in [1,n].");
+  if (maxiter <= 0) throw std::invalid_argument("BlockKrylovSchurSolver: maxiter must be > 0.");
+  if (which != "sr" && which != "lr" && which != "sm" && which != "lm" && which != "be")
+    throw std::invalid_argument("BlockKrylovSchurSolver: which must be SR, LR, SM, LM or BE on the HIP backend.");
+  const int b = static_cast<int>(options_.blksize());
+  if (b <= 0 || b > n) throw std::invalid_argument("BlockKrylovSchurSolver: blksize must be in [1,n].");
+  const int64_t requested_dim = options_.maxdim() > 0 ? options_.maxdim() : std::max<int64_t>(2 * nev, 20);
+  const int block_count = static_cast<int>(requested_dim >= n ? 1 + (n - 1) / b : requested_dim / b);
+  const int cycle_dim = static_cast<int>(std::min<int64_t>(n, static_cast<int64_t>(block_count) * b));
+  const int wanted_blocks = 1 + (nev - 1) / b;
+  if (cycle_dim < n && block_count <= wanted_blocks)
+    throw std::invalid_argument("BlockKrylovSchurSolver: maxdim must accommodate ceil(nev/blksize) retained blocks "
+                                "plus at least one expansion block (or the full matrix dimension).");
+  const int pdim = (block_count + 1) * b;
+  const int retained_blocks = cycle_dim == n ? 0 : std::max(wanted_blocks, block_count / 2);
+  const int retained_dim = retained_blocks * b;
+  const int snap_count = std::max(nev, retained_dim);
+  const unsigned seed = options_.seed().value_or(1u);
+
+  auto exp = problem_.executor(options_);
+  HipExecutor& ex = *exp;
+  HipCsrMatrix& A = problem_.device_matrix(options_);
+  Workspace ws(ex, pdim, FE_GRAM_NATIVE);
+  Block Qb(ex, n, pdim, FE_F64), Z(ex, n, b, FE_F64), Rz(ex, n, b, FE_F64), Qn(ex, n, b, FE_F64);
+  Block v(ex, n, 1, FE_F64), vbest(ex, n, 1, FE_F64);
+  hip::zero(ex, Qb.ref());
+  MatrixXd T(pdim, pdim), beta(b, b);
+  const double eps = std::numeric_limits<double>::epsilon();
+
+  auto host_small = [&](const hip::Small& S, int rows, int cols) {
+    MatrixXd full = S.to_host();
+    MatrixXd out(rows, cols);
+    for (int j = 0; j < cols; ++j)
+      for (int i = 0; i < rows; ++i) out(i, j) = full(i, j);
+    return out;
+  };
+  auto frob = [](const MatrixXd& M) {
+    double s = 0.0;
+    for (int j = 0; j < M.cols(); ++j)
+      for (int i = 0; i < M.rows(); ++i) s += M(i, j) * M(i, j);
+    return std::sqrt(s);
+  };
+  auto vnorm = [&](const BlockRef& x) { return hip::column_norms(ex, x)[0]; };
+
+  // rank-aware block QR of src against Qb[:, :active] into Qn (reference
+  // complete_block_qr): Cholesky QR when accurate, else column by column with
+  // pivoting and seeded random completion; R = Qn^T src, tiny columns zeroed
+  auto complete_qr = [&](int active, const BlockRef& src, MatrixXd& R, unsigned qseed, double refnorm) {
+    const int capacity = static_cast<int>(std::min<int64_t>(b, n - active));
+    const double threshold = 64.0 * eps * refnorm;
+    if (capacity == b) {
+      hip::gram(ex, FE_GRAM_NATIVE, src, src, ws.G.data(), ws.cap);
+      const int info = fe_potrf(ex.ctx(), b, ws.G.data(), ws.cap);
+      hip::check(info, "fe_potrf");
+      if (info == 0) {
+        const MatrixXd L = host_small(ws.G, b, b);
+        hip::check(fe_chol_rinv(ex.ctx(), b, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
+        hip::gemm(ex, src, ws.R.data(), ws.cap, Qn.ref(), 1.0, 0.0);
+        hip::gram(ex, FE_GRAM_NATIVE, Qn.ref(), Qn.ref(), ws.W.data(), ws.cap);
+        MatrixXd Gq = host_small(ws.W, b, b);
+        for (int i = 0; i < b; ++i) Gq(i, i) -= 1.0;
+        double cross = 0.0;
+        if (active > 0) {
+          hip::gram(ex, FE_GRAM_NATIVE, Qb.cols(0, active), Qn.ref(), ws.V.data(), ws.cap);
+          cross = frob(host_small(ws.V, active, b));
+        }
+        double dmin = std::numeric_limits<double>::infinity();
+        bool finite = true;
+        R = MatrixXd(b, b);
+        for (int j = 0; j < b; ++j)
+          for (int i = 0; i <= j; ++i) {
+            R(i, j) = L(j, i);
+            finite = finite && std::isfinite(R(i, j));
+          }
+        for (int j = 0; j < b; ++j) dmin = std::min(dmin, std::fabs(R(j, j)));
+        const double ge = frob(Gq);
+        if (finite && dmin > threshold && std::isfinite(ge) && std::isfinite(cross) && ge < 1e-10 && cross < 1e-10)
+          return false;
+      }
+    }
+    bool replenished = false;
+    hip::zero(ex, Qn.ref());
+    auto project = [&](int j) {
+      if (active > 0) hip::block_gram_schmidt(ws, Qb.cols(0, active), v.ref(), v.ref());
+      if (j > 0) hip::block_gram_schmidt(ws, Qn.cols(0, j), v.ref(), v.ref());
+    };
+    for (int j = 0; j < capacity; ++j) {
+      double length = 0.0, original = 0.0;
+      for (int col = 0; col < b; ++col) {
+        hip::copy(ex, src.range(col, 1), v.ref());
+        original = std::max(original, vnorm(v.ref()));
+        project(j);
+        const double cn = vnorm(v.ref());
+        if (cn > length) {
+          length = cn;
+          hip::copy(ex, v.ref(), vbest.ref());
+        }
+      }
+      const double cutoff = std::max(threshold, 1e-12 * original);
+      if (length > cutoff) {
+        hip::copy(ex, vbest.ref(), v.ref());
+        length = vnorm(v.ref());
+      } else {
+        replenished = true;
+        length = 0.0;
+        for (unsigned attempt = 0; attempt < 4; ++attempt) {
+          hip::copy(ex, ex.random_block(n, 1, qseed + 17u * static_cast<unsigned>(j + 1) + attempt).ref(), v.ref());
+          const double rn = vnorm(v.ref());
+          project(j);
+          length = vnorm(v.ref());
+          if (length > 1e-12 * rn) break;
+          length = 0.0;
+        }
+        if (length == 0.0) throw std::runtime_error("BlockKrylovSchurSolver: unable to complete the basis.");
+      }
+      hip::check(fe_axpby_cols(ex.ctx(), FE_F64, n, 1, v.data(), v.ld(), Qn.cols(j, 1).ptr, Qn.ld(), 1.0 / length,
+                               0.0, nullptr),
+                 "scale");
+    }
+    hip::gram(ex, FE_GRAM_NATIVE, Qn.ref(), src, ws.G.data(), ws.cap);
+    R = host_small(ws.G, b, b);
+    for (int j = 0; j < b; ++j) {
+      double cn = 0.0;
+      for (int i = 0; i < b; ++i) cn += R(i, j) * R(i, j);
+      if (std::sqrt(cn) <= threshold)
+        for (int i = 0; i < b; ++i) R(i, j) = 0.0;
+    }
+    return replenished;
+  };
+
+  // start block (Executor::create_random_multivector or v0) and its QR
+  if (options_.v0().size() > 0) {
+    if (options_.v0().rows() != n || options_.v0().cols() < b)
+      throw std::invalid_argument("BlockKrylovSchurSolver: v0 must have n rows and at least blksize columns.");
+    MatrixXd head(static_cast<int>(n), b);
+    std::copy(options_.v0().data(), options_.v0().data() + n * b, head.data());
+    hip::upload(ex, head, Rz.ref());
+  } else {
+    hip::copy(ex, ex.random_block(n, b, seed).ref(), Rz.ref());
+  }
+  {
+    MatrixXd R0;
+    complete_qr(0, Rz.ref(), R0, seed, 0.0);
+    hip::copy(ex, Qn.ref(), Qb.cols(0, b));
+  }
+
+  auto expand = [&](int first) {
+    for (int blk = first; blk < block_count; ++blk) {
+      hip::spmm(ex, A, Qb.cols(blk * b, b), Z.ref(), 1.0);
+      const int active = static_cast<int>(std::min<int64_t>(n, static_cast<int64_t>(blk + 1) * b));
+      double scale = 0.0;
+      for (double c : hip::column_norms(ex, Z.ref())) scale += c * c;
+      scale = std::sqrt(scale);
+      // two-pass block Gram-Schmidt keeping the coefficients H = H1 + H2
+      hip::copy(ex, Z.ref(), Rz.ref());
+      const BlockRef Vb = Qb.cols(0, active);
+      hip::gram(ex, FE_GRAM_NATIVE, Vb, Rz.ref(), ws.G.data(), ws.cap);
+      hip::gemm(ex, Vb, ws.G.data(), ws.cap, Rz.ref(), -1.0, 1.0);
+      hip::gram(ex, FE_GRAM_NATIVE, Vb, Rz.ref(), ws.R.data(), ws.cap);
+      hip::gemm(ex, Vb, ws.R.data(), ws.cap, Rz.ref(), -1.0, 1.0);
+      const MatrixXd H1 = host_small(ws.G, active, b), H2 = host_small(ws.R, active, b);
+      const int width = 1 + (active - 1) % b;
+      MatrixXd D(b, b);
+      for (int j = 0; j < width; ++j)
+        for (int i = 0; i < width; ++i) D(i, j) = H1(active - width + i, j) + H2(active - width + i, j);
+      MatrixXd B;
+      const int nf = static_cast<int>(std::min<int64_t>(static_cast<int64_t>(blk + 1) * b, n));
+      complete_qr(nf, Rz.ref(), B, seed + static_cast<unsigned>((blk + 1) * b), scale);
+      hip::copy(ex, Qn.ref(), Qb.cols((blk + 1) * b, b));
+      const int row = blk * b, nrow = row + b;
+      for (int j = 0; j < b; ++j)
+        for (int i = 0; i < b; ++i) {
+          T(row + i, row + j) = D(i, j);
+          T(nrow + i, row + j) = B(i, j);
+          T(row + j, nrow + i) = B(i, j);
+        }
+      beta = B;
+    }
+  };
+  expand(0);
+
+  const int m = cycle_dim;
+  std::vector<double> lam, rr;
+  MatrixXd Csel;
+  long long it_done = 0;
+  bool converged = false;
+  Block X(ex, n, nev, FE_F64), AX(ex, n, nev, FE_F64), Rr(ex, n, nev, FE_F64);
+  for (int iter = 1; iter <= maxiter; ++iter) {
+    // block Ritz snapshot of T[:m, :m] (RitzExtraction.cpp:272-380)
+    MatrixXd Tp(ws.cap, ws.cap);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) Tp(i, j) = 0.5 * (T(i, j) + T(j, i));
+    ws.G.from_host(Tp);
+    const int info = fe_syev(ex.ctx(), m, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap);
+    hip::check(info, "fe_syev(block Krylov-Schur)");
+    if (info != 0) throw std::runtime_error("BlockKrylovSchurSolver: projected eigensolve failed.");
+    std::vector<double> w(m);
+    hip::check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * m), "d2h");
+    const MatrixXd Vh = host_small(ws.V, m, m);
+    const std::vector<int> idx = hip::select_indices(w, which);
+    const int cnt = std::min(snap_count, m);
+    const int tail = 1 + (m - 1) % b;
+    lam.assign(cnt, 0.0);
+    rr.assign(cnt, 0.0);
+    Csel.resize(m, cnt);
+    for (int j = 0; j < cnt; ++j) {
+      const int c = idx[j];
+      lam[j] = w[c];
+      double a2 = 0.0;
+      for (int i = 0; i < b; ++i) {
+        double sacc = 0.0;
+        for (int t = 0; t < tail; ++t) sacc += beta(i, t) * Vh(m - tail + t, c);
+        a2 += sacc * sacc;
+      }
+      rr[j] = std::sqrt(a2) / (1.0 + std::fabs(w[c]));
+      for (int i = 0; i < m; ++i) Csel(i, j) = Vh(i, c);
+    }
+    it_done = iter;
+    progress(options_, it_done, lam, rr, nev, 1.0);
+    converged = max_head(rr, nev) < tol;
+    const bool at_limit = iter == maxiter || cycle_dim == n;
+    if (converged || at_limit) {
+      // Ritz vectors and true residuals (RitzExtraction.cpp:34-63)
+      hip::Small Cs(ex, m, nev), th(ex, nev, 1), rn(ex, nev, 1), yn(ex, nev, 1);
+      MatrixXd Ch(m, nev);
+      for (int j = 0; j < nev; ++j)
+        for (int i = 0; i < m; ++i) Ch(i, j) = Csel(i, j);
+      Cs.from_host(Ch);
+      hip::gemm(ex, Qb.cols(0, m), Cs.data(), m, X.ref(), 1.0, 0.0);
+      hip::spmm(ex, A, X.ref(), AX.ref(), 1.0);
+      hip::check(fe_memcpy_h2d(ex.ctx(), th.data(), lam.data(), sizeof(double) * nev), "h2d");
+      hip::check(fe_ritz_residuals(ex.ctx(), FE_F64, n, nev, X.data(), X.ld(), AX.data(), AX.ld(), th.data(),
+                                   Rr.data(), Rr.ld(), rn.data(), yn.data()),
+                 "fe_ritz_residuals");
+      std::vector<double> rh(nev), yh(nev);
+      hip::check(fe_memcpy_d2h(ex.ctx(), rh.data(), rn.data(), sizeof(double) * nev), "d2h");
+      hip::check(fe_memcpy_d2h(ex.ctx(), yh.data(), yn.data(), sizeof(double) * nev), "d2h");
+      converged = true;
+      for (int j = 0; j < nev; ++j) {
+        const double den = (1.0 + std::fabs(lam[j])) * yh[j];
+        if (!(den > 0.0) || !std::isfinite(rh[j]))
+          throw std::runtime_error("BlockKrylovSchurSolver: invalid vector or non-finite residual.");
+        rr[j] = rh[j] / den;
+        converged = converged && rr[j] <= tol;
+      }
+      break;
+    }
+    // thick restart: Q <- [Q C_keep, Q_residual]; T <- [diag(theta) F^T; F *]
+    MatrixXd coupling(b, cnt);
+    for (int j = 0; j < cnt; ++j)
+      for (int i = 0; i < b; ++i) {
+        double sacc = 0.0;
+        for (int t = 0; t < b; ++t) sacc += beta(i, t) * Csel(m - b + t, j);
+        coupling(i, j) = sacc;
+      }
+    {
+      hip::Small Cs(ex, m, cnt);
+      Cs.from_host(Csel);
+      Block U(ex, n, cnt, FE_F64), Qres(ex, n, b, FE_F64);
+      hip::gemm(ex, Qb.cols(0, m), Cs.data(), m, U.ref(), 1.0, 0.0);
+      hip::copy(ex, Qb.cols(block_count * b, b), Qres.ref());
+      hip::zero(ex, Qb.ref());
+      hip::copy(ex, U.ref(), Qb.cols(0, cnt));
+      hip::copy(ex, Qres.ref(), Qb.cols(cnt, b));
+    }
+    T.setZero();
+    for (int j = 0; j < cnt; ++j) T(j, j) = lam[j];
+    for (int j = 0; j < cnt; ++j)
+      for (int i = 0; i < b; ++i) {
+        T(cnt + i, j) = coupling(i, j);
+        T(j, cnt + i) = coupling(i, j);
+      }
+    expand(retained_blocks);
+  }
+
+  eigvals_.resize(nev);
+  resnrms_.resize(nev);
+  for (int j = 0; j < nev; ++j) {
+    eigvals_(j) = lam[j];
+    resnrms_(j) = rr[j];
+  }
+  problem_.set_device_eigenvectors(std::move(X));
+  status_ = converged ? 0 : 1;
+  set_iterations(it_done);
+  return status_;
+}
+
 std::unique_ptr<fasteig_solver> make_fasteig_solver(const std::string& solver, ::fasteig_problem& problem,
                                                     const fasteig_options& options) {
+  if (has_prefix(solver, "BlockKrylovSchur", 11)) return std::make_unique<BlockKrylovSchurSolver>(problem, options);
   if (has_prefix(solver, "Davidson", 3)) return std::make_unique<DavidsonSolver>(problem, options);
   if (has_prefix(solver, "LOBPCG", 3)) return std::make_unique<LOBPCGSolver>(problem, options);
   if (has_prefix(solver, "Chebyshev", 3)) return std::make_unique<ChebyshevSolver>(problem, options);
   if (has_prefix(solver, "auto", 3)) return nullptr;
   throw std::invalid_argument("solver '" + solver +
-                              "' is not available on the HIP backend (Chebyshev, LOBPCG, Davidson)");
+                              "' is not available on the HIP backend (Chebyshev, LOBPCG, Davidson, BlockKrylovSchur)");
 }
 
 }  // namespace fasteig

@@ -112,6 +112,52 @@ def test_sio_golden(solver):
     assert r.residual_norms.max() <= 1e-8
 
 
+def test_sio_golden_block_krylov_schur():
+    """reference_gpu_SiO_BlockKrylovSchur_SR: k=5, block 10, maxdim 40 (the
+    reference harness defaults, tests/scripts/run_case.py:21-33)."""
+    ref = np.array(json.load(open(os.path.join(GOLD, "sio_eigs.json")))["eigenvalues"][:5])
+    r = Problem(load_sio()).solve(solver="BlockKrylovSchur", nev=5, tol=1e-8, maxiter=2000, seed=1, which="SR",
+                                  blksize=10, maxdim=40)
+    assert r.status == 0
+    assert np.all(np.abs(r.eigenvalues - ref) <= 1e-10 + 1e-10 * np.abs(ref)), r.eigenvalues - ref
+    assert r.residual_norms.max() <= 1e-8
+
+
+@pytest.mark.parametrize("which", ["SR", "LR"])
+def test_block_krylov_schur_matches_oracle_and_closed_form(which):
+    A = G.laplacian_2d(24, 20)
+    ex = np.sort(G.laplacian_2d_eigenvalues(24, 20))
+    ex = ex[:6] if which == "SR" else ex[::-1][:6]
+    p = Problem(A)
+    r = p.solve(solver="BlockKrylovSchur", nev=6, blksize=4, maxdim=32, maxiter=3000, tol=1e-10, which=which,
+                seed=3)
+    o = orc.block_krylov_schur(A, 6, block=4, maxdim=32, maxiter=3000, tol=1e-10, which=which.lower(), seed=3)
+    assert r.status == 0
+    assert np.all(np.abs(r.eigenvalues - ex) <= 1e-9 * np.abs(ex))
+    assert np.all(np.abs(r.eigenvalues - o["eigenvalues"]) <= 1e-10 * np.abs(ex))
+    # same restart sequence up to summation order
+    assert abs(r.iterations - o["iterations"]) <= max(2, o["iterations"] // 20)
+    X = p.eigenvectors()
+    assert np.max(np.abs(X.T @ X - np.eye(6))) <= 1e-9
+
+
+def test_block_krylov_schur_replenishes_invariant_subspace():
+    d = np.r_[np.ones(10), 2.0 * np.ones(10), 3.0 * np.ones(10)]
+    A = sp.diags(d).tocsr()
+    r = Problem(A).solve(solver="BlockKrylovSchur", nev=4, blksize=3, maxdim=15, maxiter=200, tol=1e-10,
+                         which="SR", seed=2)
+    assert r.status == 0
+    assert np.all(np.abs(r.eigenvalues - 1.0) <= 1e-12)
+
+
+def test_block_krylov_schur_rejects_bad_dimensions():
+    p = Problem(G.laplacian_2d(10))
+    with pytest.raises(ValueError):
+        p.solve(solver="BlockKrylovSchur", nev=8, blksize=8, maxdim=12, which="SR")
+    with pytest.raises(ValueError):
+        p.solve(solver="BlockKrylovSchur", nev=4, blksize=4, which="SAI")
+
+
 def test_generalized_diagonal_mass_matches_dense_pencil():
     K, M = G.conductivity_3d(10, seed=2, mass_seed=3)
     nev = 6
