@@ -4,9 +4,9 @@
 //   k_syev_tri    one workgroup: Householder tridiagonalization (Golub & Van
 //                 Loan, Alg. 8.3.1), Sturm-count bisection for every eigenvalue
 //                 (GvL 8.4.1), inverse iteration for the eigenvectors of the
-//                 tridiagonal with modified Gram-Schmidt inside clusters of close
-//                 eigenvalues (the strategy of LAPACK dstein; Wilkinson's
-//                 inverse iteration)
+//                 tridiagonal (Wilkinson; shifts and clusters as in LAPACK
+//                 dstein), clusters of close eigenvalues orthogonalized by a
+//                 workgroup between the steps
 //   k_syev_back   multi-workgroup back-transformation V = H_0 ... H_{m-3} Z with
 //                 the eigenvector slice of each workgroup held in LDS
 #include "fe_common.hpp"
@@ -51,7 +51,137 @@ __device__ __forceinline__ double hash_unit(uint32_t a, uint32_t b) {
   return (static_cast<double>(h) / 4294967295.0) * 2.0 - 1.0;
 }
 
-// work layout (doubles): u1, u2, u3, ml, piv, Z  -- each m*m, interleaved [i * m + t]
+__device__ __forceinline__ double block_max(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nw; ++w) s = fmax(s, red[w]);
+  return s;
+}
+
+// Solve (T - x_j I) z = z in place with the LU factors of slot j (interleaved
+// [i * m + j]), then scale z to max-norm one.
+__device__ __forceinline__ void tri_lu_solve(int m, int j, const double* u1, const double* u2, const double* u3,
+                                             const double* ml, const double* pv, double tiny, double* Z) {
+  for (int i = 0; i + 1 < m; ++i) {
+    double yi = Z[(int64_t)i * m + j], yi1 = Z[(int64_t)(i + 1) * m + j];
+    if (pv[(int64_t)i * m + j] != 0.0) {
+      const double tmp = yi;
+      yi = yi1;
+      yi1 = tmp;
+    }
+    yi1 -= ml[(int64_t)i * m + j] * yi;
+    Z[(int64_t)i * m + j] = yi;
+    Z[(int64_t)(i + 1) * m + j] = yi1;
+  }
+  double mx = 0.0;
+  for (int i = m - 1; i >= 0; --i) {
+    double s = Z[(int64_t)i * m + j];
+    if (i + 1 < m) s -= u2[(int64_t)i * m + j] * Z[(int64_t)(i + 1) * m + j];
+    if (i + 2 < m) s -= u3[(int64_t)i * m + j] * Z[(int64_t)(i + 2) * m + j];
+    double yi = s / u1[(int64_t)i * m + j];
+    if (!isfinite(yi)) yi = s / tiny;
+    Z[(int64_t)i * m + j] = yi;
+    mx = fmax(mx, fabs(yi));
+  }
+  if (mx > 0.0)
+    for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] /= mx;
+}
+
+// Columns t..last of Z in order, each made orthonormal to the cluster's earlier
+// columns by two classical Gram-Schmidt passes; the whole workgroup works on
+// one column at a time (the dot products one wavefront each). The cluster is
+// copied to C with its columns contiguous (C[q * m + i]) for the sweep, so that
+// both the dot products and the updates read whole cache lines.
+__device__ void cluster_orth(int m, int t, int last, double* Z, double* C, double* zs, double* dots, double* red,
+                             int* info) {
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
+  const int c = last - t + 1;
+  for (int64_t idx = tid; idx < (int64_t)c * m; idx += nt) {
+    const int i = static_cast<int>(idx / c), q = t + static_cast<int>(idx % c);
+    C[(int64_t)q * m + i] = Z[(int64_t)i * m + q];
+  }
+  __syncthreads();
+  for (int j = t; j <= last; ++j) {
+    double* cj = C + (int64_t)j * m;
+    for (int i = tid; i < m; i += nt) zs[i] = cj[i];
+    __syncthreads();
+    for (int pass = 0; pass < 2 && j > t; ++pass) {
+      for (int q = t + wave; q < j; q += nw) {
+        const double* cq = C + (int64_t)q * m;
+        double sv = 0.0;
+        for (int i = lane; i < m; i += 64) sv += cq[i] * zs[i];
+        sv = wave_sum(sv);
+        if (lane == 0) dots[q - t] = sv;
+      }
+      __syncthreads();
+      for (int i = tid; i < m; i += nt) {
+        double acc = zs[i];
+        for (int q = t; q < j; ++q) acc -= dots[q - t] * C[(int64_t)q * m + i];
+        zs[i] = acc;
+      }
+      __syncthreads();
+    }
+    double nr = 0.0;
+    for (int i = tid; i < m; i += nt) nr += zs[i] * zs[i];
+    nr = sqrt(block_sum(nr, red));
+    if (!(nr > 0.0)) {
+      if (tid == 0) info[0] = 2;
+      nr = 1.0;
+    }
+    for (int i = tid; i < m; i += nt) cj[i] = zs[i] / nr;
+    __syncthreads();
+  }
+  for (int64_t idx = tid; idx < (int64_t)c * m; idx += nt) {
+    const int i = static_cast<int>(idx / c), q = t + static_cast<int>(idx % c);
+    Z[(int64_t)i * m + q] = C[(int64_t)q * m + i];
+  }
+  __syncthreads();
+}
+
+// Eigenvectors of clustered eigenvalues (consecutive gaps <= ortol). Every
+// member has had three inverse-iteration steps on its own (perturbed) shift,
+// independently of the other members, with its LU factors kept in slot j of the
+// work arrays. One workgroup per cluster (clusters dealt round-robin over the
+// launch's workgroups) then orthogonalizes the members in order, runs one more
+// inverse-iteration step on every member at once (one thread each), which
+// removes what the orthogonalization left outside the cluster's invariant
+// subspace, and orthogonalizes again. The members' order and shifts are those
+// of LAPACK dstein; dstein reorthogonalizes after every step, one member at a
+// time, which leaves the O(m) triangular solves on a single thread.
+__device__ void syev_clusters(int m, const double* lam, double onenrm, double pivmin, double* work, double* zs,
+                              double* dots, double* red, int* info) {
+  const double ortol = 1.0e-5 * onenrm;
+  const double tiny = fmax(DBL_EPSILON * onenrm, pivmin);
+  const int64_t MM = (int64_t)m * m;
+  const double* u1 = work;
+  const double* u2 = work + MM;
+  const double* u3 = work + 2 * MM;
+  const double* ml = work + 3 * MM;
+  const double* pv = work + 4 * MM;
+  double* Z = work + 5 * MM;
+  double* C = work + 6 * MM;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  int ordinal = 0;
+  for (int t = 0; t < m;) {
+    int last = t;
+    while (last + 1 < m && lam[last + 1] - lam[last] <= ortol) ++last;
+    if (last > t && (ordinal++ % static_cast<int>(gridDim.x)) == static_cast<int>(blockIdx.x)) {
+      cluster_orth(m, t, last, Z, C, zs, dots, red, info);
+      for (int j = t + tid; j <= last; j += nt) tri_lu_solve(m, j, u1, u2, u3, ml, pv, tiny, Z);
+      __syncthreads();
+      cluster_orth(m, t, last, Z, C, zs, dots, red, info);
+    }
+    t = last + 1;
+  }
+}
+
+// work layout (doubles): u1, u2, u3, ml, piv, Z  -- each m*m, interleaved [i * m + t];
+// then m*m for the cluster copies (column q at C[q * m])
 __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda, double* W, double* tau,
                                                    double* work, int* info, const double* __restrict__ dg,
                                                    const double* __restrict__ eg, int phase) {
@@ -60,12 +190,13 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
   // over the launch's workgroups (phase 2 reads the eigenvalues back from W).
   __shared__ double vs[SYEV_MAX], ps[SYEV_MAX], wsv[SYEV_MAX];
   __shared__ double d[SYEV_MAX], e2[SYEV_MAX], lam[SYEV_MAX];
+  __shared__ double zs[SYEV_MAX], dots[SYEV_MAX];
   __shared__ double red[16];
   __shared__ double s_tau, s_scale, s_gl, s_gu, s_pivmin, s_onenrm;
   __shared__ int s_skip;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-  if (tid == 0 && blockIdx.x == 0 && phase != 2) info[0] = 0;
+  if (tid == 0 && blockIdx.x == 0 && phase < 2) info[0] = 0;
 
   if (dg) {
     // tridiagonal already formed by the multi-workgroup reduction
@@ -168,11 +299,11 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
   // ---- 2. bisection: thread t finds the t-th smallest eigenvalue
   const double pivmin = s_pivmin, onenrm = s_onenrm;
   const int t0 = blockIdx.x * nt + tid, tstride = nt * gridDim.x;
-  if (phase == 2) {
+  if (phase >= 2) {
     for (int i = tid; i < m; i += nt) lam[i] = W[i];
     __syncthreads();
   }
-  for (int t = t0; phase != 2 && t < m; t += tstride) {
+  for (int t = t0; phase < 2 && t < m; t += tstride) {
     double lo = s_gl, hi = s_gu;
     for (int it = 0; it < 200; ++it) {
       const double mid = 0.5 * (lo + hi);
@@ -199,91 +330,65 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
   double* ml = work + 3 * MM;
   double* pv_ = work + 4 * MM;
   double* Z = work + 5 * MM;
+  if (phase == 3) {  // clustered eigenvalues, workgroup-cooperative
+    syev_clusters(m, lam, onenrm, pivmin, work, zs, dots, red, info);
+    return;
+  }
+  const double tiny = fmax(DBL_EPSILON * onenrm, pivmin);
   for (int t = t0; t < m; t += tstride) {
-    if (t > 0 && lam[t] - lam[t - 1] <= ortol) continue;  // not a leader
-    int last = t;
-    while (last + 1 < m && lam[last + 1] - lam[last] <= ortol) ++last;
-    double prev_shift = 0.0;
-    for (int j = t; j <= last; ++j) {
-      double xj = lam[j];
-      if (j > t && xj - prev_shift < pertol) xj = prev_shift + pertol;
-      prev_shift = xj;
-      // LU with partial pivoting of T - xj I into slot t (interleaved)
-      const double tiny = fmax(DBL_EPSILON * onenrm, pivmin);
-      u1[0 * m + t] = d[0] - xj;
-      if (m > 1) u2[0 * m + t] = ps[0];
-      for (int i = 0; i + 1 < m; ++i) {
-        const double ui = u1[(int64_t)i * m + t];
-        const double bi = ps[i];
-        const double a1 = d[i + 1] - xj;
-        const double c1 = i + 2 < m ? ps[i + 1] : 0.0;
-        const double u2i = u2[(int64_t)i * m + t];
-        if (fabs(ui) >= fabs(bi)) {
-          const double piv = ui != 0.0 ? ui : tiny;
-          u1[(int64_t)i * m + t] = piv;
-          const double l = bi / piv;
-          ml[(int64_t)i * m + t] = l;
-          pv_[(int64_t)i * m + t] = 0.0;
-          u3[(int64_t)i * m + t] = 0.0;
-          u1[(int64_t)(i + 1) * m + t] = a1 - l * u2i;
-          if (i + 2 < m) u2[(int64_t)(i + 1) * m + t] = c1;
-        } else {
-          const double l = ui / bi;
-          ml[(int64_t)i * m + t] = l;
-          pv_[(int64_t)i * m + t] = 1.0;
-          u1[(int64_t)i * m + t] = bi;
-          u2[(int64_t)i * m + t] = a1;
-          u3[(int64_t)i * m + t] = c1;
-          u1[(int64_t)(i + 1) * m + t] = u2i - l * a1;
-          if (i + 2 < m) u2[(int64_t)(i + 1) * m + t] = -l * c1;
-        }
-      }
-      if (u1[(int64_t)(m - 1) * m + t] == 0.0) u1[(int64_t)(m - 1) * m + t] = tiny;
-      // start vector
-      for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] = hash_unit(j + 1, i + 1);
-      for (int iter = 0; iter < 3; ++iter) {
-        // forward elimination with the recorded interchanges
-        for (int i = 0; i + 1 < m; ++i) {
-          double yi = Z[(int64_t)i * m + j], yi1 = Z[(int64_t)(i + 1) * m + j];
-          if (pv_[(int64_t)i * m + t] != 0.0) {
-            const double tmp = yi;
-            yi = yi1;
-            yi1 = tmp;
-          }
-          yi1 -= ml[(int64_t)i * m + t] * yi;
-          Z[(int64_t)i * m + j] = yi;
-          Z[(int64_t)(i + 1) * m + j] = yi1;
-        }
-        // back substitution
-        double nrm = 0.0;
-        for (int i = m - 1; i >= 0; --i) {
-          double s = Z[(int64_t)i * m + j];
-          if (i + 1 < m) s -= u2[(int64_t)i * m + t] * Z[(int64_t)(i + 1) * m + j];
-          if (i + 2 < m) s -= u3[(int64_t)i * m + t] * Z[(int64_t)(i + 2) * m + j];
-          const double piv = u1[(int64_t)i * m + t];
-          double yi = s / piv;
-          if (!isfinite(yi)) yi = s / tiny;
-          Z[(int64_t)i * m + j] = yi;
-        }
-        // scale down to avoid overflow, reorthogonalize inside the cluster, normalize
-        double mx = 0.0;
-        for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(Z[(int64_t)i * m + j]));
-        if (mx > 0.0)
-          for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] /= mx;
-        for (int q = t; q < j; ++q) {
-          double dot = 0.0;
-          for (int i = 0; i < m; ++i) dot += Z[(int64_t)i * m + q] * Z[(int64_t)i * m + j];
-          for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] -= dot * Z[(int64_t)i * m + q];
-        }
-        for (int i = 0; i < m; ++i) nrm += Z[(int64_t)i * m + j] * Z[(int64_t)i * m + j];
-        nrm = sqrt(nrm);
-        if (!(nrm > 0.0)) {
-          info[0] = 2;
-          nrm = 1.0;
-        }
-        for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] /= nrm;
+    // shift: inside a cluster each member sits at least pertol above the
+    // previous member's shift, so that no two factorizations coincide
+    int s0 = t;
+    while (s0 > 0 && lam[s0] - lam[s0 - 1] <= ortol) --s0;
+    double xj = lam[s0];
+    for (int q = s0 + 1; q <= t; ++q) xj = fmax(lam[q], xj + pertol);
+    // LU with partial pivoting of T - xj I into slot t (interleaved)
+    u1[0 * m + t] = d[0] - xj;
+    if (m > 1) u2[0 * m + t] = ps[0];
+    for (int i = 0; i + 1 < m; ++i) {
+      const double ui = u1[(int64_t)i * m + t];
+      const double bi = ps[i];
+      const double a1 = d[i + 1] - xj;
+      const double c1 = i + 2 < m ? ps[i + 1] : 0.0;
+      const double u2i = u2[(int64_t)i * m + t];
+      if (fabs(ui) >= fabs(bi)) {
+        const double piv = ui != 0.0 ? ui : tiny;
+        u1[(int64_t)i * m + t] = piv;
+        const double l = bi / piv;
+        ml[(int64_t)i * m + t] = l;
+        pv_[(int64_t)i * m + t] = 0.0;
+        u3[(int64_t)i * m + t] = 0.0;
+        u1[(int64_t)(i + 1) * m + t] = a1 - l * u2i;
+        if (i + 2 < m) u2[(int64_t)(i + 1) * m + t] = c1;
+      } else {
+        const double l = ui / bi;
+        ml[(int64_t)i * m + t] = l;
+        pv_[(int64_t)i * m + t] = 1.0;
+        u1[(int64_t)i * m + t] = bi;
+        u2[(int64_t)i * m + t] = a1;
+        u3[(int64_t)i * m + t] = c1;
+        u1[(int64_t)(i + 1) * m + t] = u2i - l * a1;
+        if (i + 2 < m) u2[(int64_t)(i + 1) * m + t] = -l * c1;
       }
     }
+    if (u1[(int64_t)(m - 1) * m + t] == 0.0) u1[(int64_t)(m - 1) * m + t] = tiny;
+    // seeded start vector, three inverse-iteration steps
+    for (int i = 0; i < m; ++i) Z[(int64_t)i * m + t] = hash_unit(t + 1, i + 1);
+    for (int iter = 0; iter < 3; ++iter) {
+      tri_lu_solve(m, t, u1, u2, u3, ml, pv_, tiny, Z);
+      double nrm = 0.0;
+      for (int i = 0; i < m; ++i) nrm += Z[(int64_t)i * m + t] * Z[(int64_t)i * m + t];
+      nrm = sqrt(nrm);
+      if (!(nrm > 0.0)) {
+        info[0] = 2;
+        nrm = 1.0;
+      }
+      for (int i = 0; i < m; ++i) Z[(int64_t)i * m + t] /= nrm;
+    }
+  }
+  if (phase == 0) {  // one workgroup: the clusters follow on the same launch
+    __syncthreads();
+    syev_clusters(m, lam, onenrm, pivmin, work, zs, dots, red, info);
   }
 }
 
@@ -433,8 +538,8 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
   if (m == 0) return 0;
   const int64_t MM = (int64_t)m * m;
   void* ws = nullptr;
-  // tau | d | e | p (2 m) | Vh (m*m) | work (6 m*m)
-  int rc = scratch(ctx, 1, sizeof(double) * (7 * MM + 5 * (int64_t)m + 64), &ws);
+  // tau | d | e | p (2 m) | Vh (m*m) | work (7 m*m)
+  int rc = scratch(ctx, 1, sizeof(double) * (8 * MM + 5 * (int64_t)m + 64), &ws);
   if (rc) return rc;
   double* tau = (double*)ws;
   double* dd = tau + m;
@@ -461,6 +566,8 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
     const unsigned g = blocks_for(m, 64);
     hipLaunchKernelGGL(k_syev_tri, dim3(g), dim3(64), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev, dd, ee, 1);
     hipLaunchKernelGGL(k_syev_tri, dim3(g), dim3(64), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev, dd, ee, 2);
+    hipLaunchKernelGGL(k_syev_tri, dim3(16), dim3(256), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev, dd, ee,
+                       3);
   } else {
     hipLaunchKernelGGL(k_syev_tri, dim3(1), dim3(1024), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev,
                        nullptr, nullptr, 0);
