@@ -13,6 +13,8 @@
 // A workgroup is 4 wavefronts computing a 64 x 64 output tile (32 x 32 each).
 #include "fe_common.hpp"
 
+#include <algorithm>
+
 namespace fe {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
@@ -112,6 +114,8 @@ __global__ __launch_bounds__(256, 2) void k_gram(int64_t n, int m, int p, const 
   const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
+  // valid 16-wide blocks of the wavefront's 64 x 64 tile (wave-uniform)
+  const int vi = min(4, max(0, (m - a0 - wm * 64 + 15) >> 4)), vj = min(4, max(0, (p - b0 - wn * 64 + 15) >> 4));
   using acc_t = typename mfma<T>::acc;
   acc_t acc[4][4];
 #pragma unroll
@@ -158,7 +162,8 @@ __global__ __launch_bounds__(256, 2) void k_gram(int64_t n, int m, int p, const 
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j)
+          if (i < vi && j < vj) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
     }
   }
   // partial tile, column-major 128 x 128
@@ -173,6 +178,91 @@ __global__ __launch_bounds__(256, 2) void k_gram(int64_t n, int m, int p, const 
         const int col = wn * 64 + j * 16 + (lane & 15);
         P[col * WT + row] = static_cast<double>(acc[i][j][e]);
       }
+}
+
+// Gram with 64 x 64 tiles for thin or small operands: wavefront w owns the
+// 16-row strip 16w..16w+15 of the tile (four 16 x 16 MFMA blocks, blocks
+// beyond the valid columns skipped), so the accumulators take 32 VGPRs and
+// four workgroups fit on a CU to hide the load latency of the row stream.
+constexpr int G64 = 64;   // tile edge
+constexpr int G64R = 32;  // rows per LDS stage
+
+template <typename T>
+__global__ __launch_bounds__(256, 4) void k_gram64(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
+                                                const T* __restrict__ Y, int64_t ldy,
+                                                const double* __restrict__ w, int tiles_b, int ntiles, int sym,
+                                                int64_t rows_per, int64_t per_xcd, int64_t nunits, int vec,
+                                                double* __restrict__ part) {
+  __shared__ T Xs[G64R][G64 + GPAD];
+  __shared__ T Ys[G64R][G64 + GPAD];
+  const int64_t u = xcd_tile(blockIdx.x, per_xcd);
+  if (u >= nunits) return;
+  const int64_t chunk = u / ntiles;
+  const int tile = static_cast<int>(u - chunk * ntiles);
+  int ta, tb;
+  gram_tile(tile, tiles_b, sym, ta, tb);
+  const int a0 = ta * G64, b0 = tb * G64;
+  const int64_t r0 = chunk * rows_per;
+  const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // valid 16-wide blocks (wave-uniform): this wave's strip, and the columns
+  const bool strip = m - a0 - 16 * wave > 0;
+  const int vj = strip ? min(4, max(0, (p - b0 + 15) >> 4)) : 0;
+  using acc_t = typename mfma<T>::acc;
+  acc_t acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = acc_t{};
+  // staging: thread -> row tid / 8, columns (tid % 8) * 8 .. + 7 of the tile
+  const int sr = tid >> 3, sc = (tid & 7) * 8;
+  const int va = m - (a0 + sc), vb = p - (b0 + sc);
+  T xr[8], yr[8];
+  auto fetch = [&](int64_t rb) {
+    const int64_t r = rb + sr;
+    if (r < r1) {
+      load8(X + r * ldx + a0 + sc, vec, va, xr);
+      load8(Y + r * ldy + b0 + sc, vec, vb, yr);
+      if (w) {
+        const double wr = w[r];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) yr[e] = static_cast<T>(static_cast<double>(yr[e]) * wr);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xr[e] = yr[e] = T(0);
+    }
+  };
+  fetch(r0);
+  for (int64_t rb = r0; rb < r1; rb += G64R) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      Xs[sr][sc + e] = xr[e];
+      Ys[sr][sc + e] = yr[e];
+    }
+    __syncthreads();
+    if (rb + G64R < r1) fetch(rb + G64R);
+#pragma unroll
+    for (int kk = 0; kk < G64R; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      const T af = Xs[kr][16 * wave + (lane & 15)];
+      T bf[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Ys[kr][j * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < vj) acc[j] = mfma<T>::run(af, bf[j], acc[j]);
+    }
+  }
+  // partial tile, column-major 64 x 64
+  double* P = part + u * (G64 * G64);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 16 * wave + mfma<T>::row(lane, e);
+      const int col = j * 16 + (lane & 15);
+      P[col * G64 + row] = static_cast<double>(acc[j][e]);
+    }
 }
 
 // ---------------------------------------------------------------- Gram (BF16x3)
@@ -267,33 +357,40 @@ __global__ __launch_bounds__(256) void k_gram_bf16x3(int64_t n, int m, int p, co
 }
 
 // Fixed-order sum over chunks; mirrors the lower triangle for symmetric Grams.
-// Grid: (tile slots, tsz * tsz / 256 element slices); one thread per element.
-__global__ void k_gram_reduce(int m, int p, int tiles_b, int ntiles, int chunks, const double* __restrict__ part,
-                              double* G, int64_t ldg, int sym, int tsz) {
+// Grid: (tile slots, tsz * tsz / 64 element slices), 256 threads: 64 elements
+// x 4 chunk residues mod 4, each summed in chunk order and then the four
+// residues in order.
+__global__ __launch_bounds__(256) void k_gram_reduce(int m, int p, int tiles_b, int ntiles, int chunks,
+                                                     const double* __restrict__ part, double* G, int64_t ldg, int sym,
+                                                     int tsz) {
+  __shared__ double red[4][64];
   const int t = blockIdx.x;  // computed tile slot
   int ta, tb;
   gram_tile(t, tiles_b, sym, ta, tb);
-  {
-    const int e = blockIdx.y * blockDim.x + threadIdx.x;
-    if (e >= tsz * tsz) return;
-    const int col = e / tsz, row = e % tsz;
-    const int a = ta * tsz + row, b = tb * tsz + col;
-    if (a >= m || b >= p) return;
-    double s = 0.0;
-    for (int c = 0; c < chunks; ++c) s += part[((int64_t)c * ntiles + t) * (tsz * tsz) + e];
-    if (sym) {
-      if (ta == tb) {
-        if (a <= b) {
-          G[(int64_t)b * ldg + a] = s;
-          G[(int64_t)a * ldg + b] = s;
-        }
-      } else {
+  const int el = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int e = blockIdx.y * 64 + el;
+  const int col = e / tsz, row = e % tsz;
+  const int a = ta * tsz + row, b = tb * tsz + col;
+  const bool ok = e < tsz * tsz && a < m && b < p;
+  double s = 0.0;
+  if (ok)
+    for (int c = q; c < chunks; c += 4) s += part[((int64_t)c * ntiles + t) * (tsz * tsz) + e];
+  red[q][el] = s;
+  __syncthreads();
+  if (q != 0 || !ok) return;
+  s = red[0][el] + red[1][el] + red[2][el] + red[3][el];
+  if (sym) {
+    if (ta == tb) {
+      if (a <= b) {
         G[(int64_t)b * ldg + a] = s;
         G[(int64_t)a * ldg + b] = s;
       }
     } else {
       G[(int64_t)b * ldg + a] = s;
+      G[(int64_t)a * ldg + b] = s;
     }
+  } else {
+    G[(int64_t)b * ldg + a] = s;
   }
 }
 
@@ -310,40 +407,52 @@ __global__ void k_gram_reduce(int m, int p, int tiles_b, int ntiles, int chunks,
 // registers during the MFMA work.
 constexpr int MKS = 16;
 
-template <typename T>
+template <typename T, int TR, int TC>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
                                                  const double* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
                                                  T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles,
                                                  int vec) {
-  __shared__ T Xs[WT][MKS + 1];
-  __shared__ T Cs[MKS][WT + GPAD];
+  static_assert((TR / 64) * (TC / 64) == 4, "four 64 x 64 wavefront tiles");
+  constexpr int TPR = 256 / TR, XK = MKS / TPR;  // X staging: threads per row, k values per thread
+  constexpr int TPC = 256 / TC, CK = MKS / TPC;  // C staging: threads per column, k values per thread
+  __shared__ T Xs[TR][MKS + 1];
+  __shared__ T Cs[MKS][TC + GPAD];
   const int64_t t = xcd_tile(blockIdx.x, per_xcd);
   if (t >= ntiles) return;
   const int64_t rt = t / tiles_c;
   const int ct = static_cast<int>(t - rt * tiles_c);
-  const int64_t r0 = rt * WT;
-  const int c0 = ct * WT;
+  const int64_t r0 = rt * TR;
+  const int c0 = ct * TC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / (TC / 64), wn = wave % (TC / 64);
+  // valid 16-wide blocks of the wavefront's 64 x 64 tile (wave-uniform)
+  const int64_t rows_left = n - r0 - wm * 64;
+  const int vi = rows_left >= 64 ? 4 : static_cast<int>(max<int64_t>(0, (rows_left + 15) >> 4));
+  const int vj = min(4, max(0, (p - c0 - wn * 64 + 15) >> 4));
   using acc_t = typename mfma<T>::acc;
   acc_t acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{};
-  // staging: X row tid / 2, k (tid % 2) * 8 .. + 7; C column tid / 2, same k
-  const int sr = tid >> 1, sk = (tid & 1) * 8;
+  // staging: X row tid / TPR, k (tid % TPR) * XK .. + XK - 1; C column tid / TPC, k (tid % TPC) * CK ..
+  const int sr = tid / TPR, sk = (tid % TPR) * XK;
+  const int scc = tid / TPC, sck = (tid % TPC) * CK;
   const int64_t xrow = r0 + sr;
-  const int ccol = c0 + sr;
-  T xr[8], cr[8];
+  const int ccol = c0 + scc;
+  T xr[XK], cr[CK];
   auto fetch = [&](int k0) {
-    if (xrow < n) load8(X + xrow * ldx + k0 + sk, vec, m - (k0 + sk), xr);
-    else
 #pragma unroll
-      for (int e = 0; e < 8; ++e) xr[e] = T(0);
+    for (int h = 0; h < XK / 8; ++h) {
+      T* dst = xr + 8 * h;
+      if (xrow < n) load8(X + xrow * ldx + k0 + sk + 8 * h, vec, m - (k0 + sk + 8 * h), *reinterpret_cast<T(*)[8]>(dst));
+      else
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = k0 + sk + e;
+        for (int e = 0; e < 8; ++e) dst[e] = T(0);
+    }
+#pragma unroll
+    for (int e = 0; e < CK; ++e) {
+      const int k = k0 + sck + e;
       cr[e] = (ccol < p && k < m) ? static_cast<T>(C[(int64_t)ccol * ldc + k]) : T(0);
     }
   };
@@ -351,10 +460,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
   for (int k0 = 0; k0 < m; k0 += MKS) {
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      Xs[sr][sk + e] = xr[e];
-      Cs[sk + e][sr] = cr[e];
-    }
+    for (int e = 0; e < XK; ++e) Xs[sr][sk + e] = xr[e];
+#pragma unroll
+    for (int e = 0; e < CK; ++e) Cs[sck + e][scc] = cr[e];
     __syncthreads();
     if (k0 + MKS < m) fetch(k0 + MKS);
 #pragma unroll
@@ -368,7 +476,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j)
+          if (i < vi && j < vj) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
     }
   }
 #pragma unroll
@@ -403,31 +512,47 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
   if (n == 0) return fe_zero2d(ctx, G, p, m, ldg, 8);
   const int cus = ctx->cus > 0 ? ctx->cus : 256;
   if (mode == FE_GRAM_NATIVE) {
-    const int tiles_a = (m + WT - 1) / WT, tiles_b = (p + WT - 1) / WT;
+    // 64 x 64 tiles (rows split over the waves) for thin or small products,
+    // 128 x 128 tiles otherwise
+    const bool small = std::min(m, p) <= 64 || std::max(m, p) <= 160;
+    const int TT = small ? G64 : WT, RS = small ? G64R : GKS;
+    const int tiles_a = (m + TT - 1) / TT, tiles_b = (p + TT - 1) / TT;
     const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
     // about 4 work units per CU; chunks of whole stages
     int64_t chunks = (4 * cus + ntiles - 1) / ntiles;
     int64_t rows_per = (n + chunks - 1) / chunks;
-    rows_per = (rows_per + GKS - 1) / GKS * GKS;
+    rows_per = (rows_per + RS - 1) / RS * RS;
     chunks = (n + rows_per - 1) / rows_per;
     void* ws = nullptr;
-    int rc = scratch(ctx, 0, sizeof(double) * (size_t)chunks * ntiles * WT * WT, &ws);
+    int rc = scratch(ctx, 0, sizeof(double) * (size_t)chunks * ntiles * TT * TT, &ws);
     if (rc) return rc;
     const int64_t units = chunks * ntiles, nb8 = (units + 7) / 8 * 8;
     const int V = prec == FE_F64 ? 2 : 4;
     const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && (reinterpret_cast<uintptr_t>(Y) % 16 == 0) &&
                     ldx % V == 0 && ldy % V == 0;
-    if (prec == FE_F64)
-      hipLaunchKernelGGL(k_gram<double>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const double*)X,
-                         ldx, (const double*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec,
-                         (double*)ws);
-    else
-      hipLaunchKernelGGL(k_gram<float>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const float*)X,
-                         ldx, (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec,
-                         (double*)ws);
+    const dim3 grid((unsigned)nb8), block(256);
+    if (prec == FE_F64) {
+      if (small)
+        hipLaunchKernelGGL(k_gram64<double>, grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx,
+                           (const double*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec,
+                           (double*)ws);
+      else
+        hipLaunchKernelGGL(k_gram<double>, grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx,
+                           (const double*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec,
+                           (double*)ws);
+    } else {
+      if (small)
+        hipLaunchKernelGGL(k_gram64<float>, grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                           (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec,
+                           (double*)ws);
+      else
+        hipLaunchKernelGGL(k_gram<float>, grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                           (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec,
+                           (double*)ws);
+    }
     FE_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles, WT * WT / 256), dim3(256), 0, ctx->stream, m, p,
-                       tiles_b, ntiles, (int)chunks, (const double*)ws, G, ldg, symi, WT);
+    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles, TT * TT / 64), dim3(256), 0, ctx->stream, m, p,
+                       tiles_b, ntiles, (int)chunks, (const double*)ws, G, ldg, symi, TT);
     FE_CHECK_LAUNCH();
     return 0;
   }
@@ -448,7 +573,7 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
   hipLaunchKernelGGL(k_gram_bf16x3, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
                      (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
   FE_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles, TILE * TILE / 256), dim3(256), 0, ctx->stream, m, p,
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles, TILE * TILE / 64), dim3(256), 0, ctx->stream, m, p,
                      tiles_b, ntiles, (int)chunks, part, G, ldg, symi, TILE);
   FE_CHECK_LAUNCH();
   return 0;
@@ -464,19 +589,31 @@ extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const v
     if (beta == 0.0) return fe_zero2d(ctx, Y, n, p, ldy, (int)prec_bytes(prec));
     return fe_axpby_cols(ctx, prec, n, p, Y, ldy, Y, ldy, 0.0, beta, nullptr);
   }
-  const int tiles_c = (p + WT - 1) / WT;
-  const int64_t tiles_r = (n + WT - 1) / WT;
+  // 256 x 64 tiles when the output has at most 64 columns, else 128 x 128
+  const bool thin = p <= 64;
+  const int TR = thin ? 256 : WT, TC = thin ? 64 : WT;
+  const int tiles_c = (p + TC - 1) / TC;
+  const int64_t tiles_r = (n + TR - 1) / TR;
   const int64_t ntiles = tiles_r * tiles_c;
   const int64_t nb8 = (ntiles + 7) / 8 * 8;
   const int V = prec == FE_F64 ? 2 : 4;
   const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && ldx % V == 0;
-  if (prec == FE_F64)
-    hipLaunchKernelGGL(k_gemm_nn<double>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p,
-                       (const double*)X, ldx, C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
-  else
-    hipLaunchKernelGGL(k_gemm_nn<float>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p,
-                       (const float*)X, ldx, C, ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8,
-                       ntiles, vec);
+  const dim3 grid((unsigned)nb8), block(256);
+  if (prec == FE_F64) {
+    if (thin)
+      hipLaunchKernelGGL((k_gemm_nn<double, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx, C,
+                         ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
+    else
+      hipLaunchKernelGGL((k_gemm_nn<double, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx,
+                         C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
+  } else {
+    if (thin)
+      hipLaunchKernelGGL((k_gemm_nn<float, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
+                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec);
+    else
+      hipLaunchKernelGGL((k_gemm_nn<float, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
+                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec);
+  }
   FE_CHECK_LAUNCH();
   return 0;
 }
