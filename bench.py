@@ -203,9 +203,10 @@ def c2_bench(steps=2):
 def c4_bench(maxiter=400):
     """Power-law SPD matrix, n = 2,000,000 (row degrees alpha 2.1 on [4, 2000],
     symmetrized, diagonal = row abs-sum + 1): the fused step at k = 48 in CSR
-    and SELL-32-sigma (load balance), then the lowest 32 pairs by LOBPCG
-    block 48 (plain FP64) and by Chebyshev block 48 with the mixed FP32 filter
-    and FP64 refinement, tol 1e-8."""
+    and SELL-32-sigma (load balance), then the lowest 32 pairs by block
+    Krylov-Schur (block 48, maxdim 192) and LOBPCG block 48 (plain FP64) and
+    by Chebyshev block 48 with the mixed FP32 filter and FP64 refinement,
+    tol 1e-8."""
     from fasteig_amd import _fe
     from fasteig_amd import generators as G
     from fasteig_amd.api import Problem
@@ -220,11 +221,12 @@ def c4_bench(maxiter=400):
             out[f"step_{fmt}_{pname}_k48_gbs"] = algo / (per * 1e-3) / 1e9
             log(f"c4 fused step {fmt} {pname} k=48: {per:.3f} ms {algo / (per * 1e-3) / 1e9:.0f} GB/s")
     p = Problem(A)
-    for name, opts in (("lobpcg_fp64", dict(solver="LOBPCG", blksize=48, precision="fp64")),
+    for name, opts in (("block_krylov_schur_fp64", dict(solver="BlockKrylovSchur", blksize=48, maxdim=192)),
+                       ("lobpcg_fp64", dict(solver="LOBPCG", blksize=48, precision="fp64")),
                        ("chebyshev_mixed", dict(solver="Chebyshev", blksize=48, degree=20, precision="mixed",
                                                 cheb_bounds="ritz"))):
         t = time.perf_counter()
-        r = p.solve(nev=32, tol=1e-8, which="SR", maxiter=maxiter, seed=1, **opts)
+        r = p.solve(nev=32, tol=1e-8, which="SR", maxiter=maxiter, seed=1, verbosity=2, print_every=10, **opts)
         dt = time.perf_counter() - t
         out[name] = {"s": dt, "iterations": r.iterations, "status": r.status,
                      "max_resid": float(r.residual_norms.max()), "lowest": float(r.eigenvalues[0])}

@@ -85,6 +85,7 @@ def lib():
         L.fe_colnorms.argtypes = [vp, i32, i64, i32, vp, i64, vp]
         L.fe_ritz_residuals.argtypes = [vp, i32, i64, i32, vp, i64, vp, i64, vp, vp, i64, vp, vp]
         L.fe_copy2d.argtypes = [vp, i64, i32, vp, i32, i64, vp, i32, i64]
+        L.fe_block_gather_cols.argtypes = [vp, i32, i64, i32, vp, i64, vp, vp, i64]
         L.fe_upload_colmajor.argtypes = [vp, i64, i32, vp, vp, i32, i64]
         L.fe_download_colmajor.argtypes = [vp, i64, i32, vp, i32, i64, vp]
         L.fe_sym_small.argtypes = [vp, i32, vp, i64]

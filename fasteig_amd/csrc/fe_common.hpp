@@ -5,7 +5,10 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/fe_hip.h"
@@ -23,6 +26,18 @@ struct Scratch {
   size_t bytes = 0;
 };
 
+// Stream-ordered reuse of fe_alloc'd blocks: fe_free parks a block here
+// instead of synchronizing and calling hipFree; a later fe_alloc of a size
+// within 25% takes it back. All work of a context runs on its one stream, so
+// a parked block is reused only by work queued after its last use. The cache
+// is bounded (an eighth of HBM) and emptied when hipMalloc runs out.
+struct BlockCache {
+  std::mutex mu;
+  std::multimap<size_t, void*> parked;       // size -> block
+  std::unordered_map<void*, size_t> live;    // block -> rounded size
+  size_t parked_bytes = 0;
+};
+
 }  // namespace fe
 
 struct fe_ctx_s {
@@ -32,6 +47,7 @@ struct fe_ctx_s {
   size_t hbm = 0;
   char name[128] = {0};
   fe::Scratch ws[4];           // independent scratch slots
+  fe::BlockCache cache;        // fe_alloc / fe_free reuse
   int* info_dev = nullptr;     // small device status words
   hipEvent_t events[16] = {};
 };

@@ -81,40 +81,58 @@ __global__ void k_axpby_cols(int64_t n, int k, const T* __restrict__ X, int64_t 
   }
 }
 
-// Column sums of squares over row chunks: part[chunk][j]. One thread per column
-// within a chunk of `rows_per` rows; threads of a row are adjacent columns, so
-// the row-major block is read coalesced. The chunk partials are reduced in a
-// fixed order by k_reduce_cols.
+// Column sums of squares over row chunks: part[chunk][j]. Within a chunk of
+// `rows_per` rows a column is summed by four threads (rows i = 4 q + phase),
+// and the threads of a row are adjacent columns, so the row-major block is
+// read coalesced. The chunk partials are reduced in a fixed order by
+// k_reduce_cols.
 template <typename T, bool RES>
-__global__ void k_colsq(int64_t n, int k, const T* __restrict__ Y, int64_t ldy,
-                        const T* __restrict__ AY, int64_t lday, const double* __restrict__ theta,
-                        T* R, int64_t ldr, double* part_r, double* part_y, int64_t rows_per) {
-  const int j = blockIdx.y * blockDim.x + threadIdx.x;
-  if (j >= k) return;
+__global__ __launch_bounds__(256) void k_colsq(int64_t n, int k, const T* __restrict__ Y, int64_t ldy,
+                                               const T* __restrict__ AY, int64_t lday,
+                                               const double* __restrict__ theta, T* R, int64_t ldr, double* part_r,
+                                               double* part_y, int64_t rows_per) {
+  // 64 columns x 4 row phases per workgroup; the phases are summed in order
+  __shared__ double ry[4][64], rr[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int j = blockIdx.y * 64 + tx;
   const int64_t r0 = blockIdx.x * rows_per;
   const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
   double sy = 0.0, sr = 0.0;
-  const double th = RES ? theta[j] : 0.0;
-  for (int64_t i = r0; i < r1; ++i) {
-    const double y = static_cast<double>(Y[i * ldy + j]);
-    sy += y * y;
-    if (RES) {
-      const T rv = static_cast<T>(static_cast<double>(AY[i * lday + j]) - th * y);
-      if (R) R[i * ldr + j] = rv;
-      const double rd = static_cast<double>(rv);
-      sr += rd * rd;
+  if (j < k) {
+    const double th = RES ? theta[j] : 0.0;
+#pragma unroll 4
+    for (int64_t i = r0 + ty; i < r1; i += 4) {
+      const double y = static_cast<double>(Y[i * ldy + j]);
+      sy += y * y;
+      if (RES) {
+        const T rv = static_cast<T>(static_cast<double>(AY[i * lday + j]) - th * y);
+        if (R) R[i * ldr + j] = rv;
+        const double rd = static_cast<double>(rv);
+        sr += rd * rd;
+      }
     }
   }
-  part_y[blockIdx.x * (int64_t)k + j] = sy;
-  if (RES) part_r[blockIdx.x * (int64_t)k + j] = sr;
+  ry[ty][tx] = sy;
+  rr[ty][tx] = sr;
+  __syncthreads();
+  if (ty != 0 || j >= k) return;
+  part_y[blockIdx.x * (int64_t)k + j] = ry[0][tx] + ry[1][tx] + ry[2][tx] + ry[3][tx];
+  if (RES) part_r[blockIdx.x * (int64_t)k + j] = rr[0][tx] + rr[1][tx] + rr[2][tx] + rr[3][tx];
 }
 
-__global__ void k_reduce_cols(int chunks, int k, const double* __restrict__ part, double* out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= k) return;
+// sqrt of the chunk sums: 64 columns x 4 chunk residues per workgroup, each
+// residue summed in chunk order, then the residues in order
+__global__ __launch_bounds__(256) void k_reduce_cols(int chunks, int k, const double* __restrict__ part,
+                                                     double* out) {
+  __shared__ double red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + tx;
   double s = 0.0;
-  for (int c = 0; c < chunks; ++c) s += part[(int64_t)c * k + j];
-  out[j] = sqrt(s);
+  if (j < k)
+    for (int c = ty; c < chunks; c += 4) s += part[(int64_t)c * k + j];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && j < k) out[j] = sqrt(red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx]);
 }
 
 template <typename T>
@@ -137,6 +155,27 @@ __global__ void k_copy2d(int64_t rows, int cols, const S* __restrict__ src, int6
     const int64_t r = t / cols;
     const int c = static_cast<int>(t - r * cols);
     dst[r * ldd + c] = static_cast<D>(src[r * lds + c]);
+  }
+}
+
+// dst[:, j] = src[:, idx[j]] for j < p (p <= FE_GATHER_MAX), row-major blocks;
+// the indices travel in the kernel arguments and are staged in LDS.
+constexpr int FE_GATHER_MAX = 256;
+struct ColIdx {
+  int v[FE_GATHER_MAX];
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_cols(int64_t rows, int p, const T* __restrict__ src, int64_t lds,
+                                                     const ColIdx idx, T* __restrict__ dst, int64_t ldd) {
+  __shared__ int ci[FE_GATHER_MAX];
+  for (int j = threadIdx.x; j < p; j += blockDim.x) ci[j] = idx.v[j];
+  __syncthreads();
+  const int64_t total = rows * p;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / p;
+    const int c = static_cast<int>(t - r * p);
+    dst[r * ldd + c] = src[r * lds + ci[c]];
   }
 }
 
@@ -249,6 +288,8 @@ int fe_ctx_destroy(fe_ctx ctx) {
   hipStreamSynchronize(ctx->stream);
   for (auto& s : ctx->ws)
     if (s.ptr) hipFree(s.ptr);
+  for (auto& kv : ctx->cache.parked) hipFree(kv.second);
+  ctx->cache.parked.clear();
   if (ctx->info_dev) hipFree(ctx->info_dev);
   for (auto& e : ctx->events)
     if (e) hipEventDestroy(e);
@@ -273,20 +314,62 @@ int fe_ctx_info(fe_ctx ctx, int* cus, size_t* hbm, char* name, int name_len) {
   return 0;
 }
 
+namespace {
+// release parked blocks (largest first) until at most `keep` bytes stay parked
+int release_parked(fe_ctx ctx, size_t keep) {
+  auto& c = ctx->cache;
+  if (c.parked_bytes <= keep) return 0;
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  while (c.parked_bytes > keep && !c.parked.empty()) {
+    auto it = std::prev(c.parked.end());
+    c.parked_bytes -= it->first;
+    const hipError_t e = hipFree(it->second);
+    c.parked.erase(it);
+    if (e != hipSuccess) return fail(-3, std::string("hipFree: ") + hipGetErrorString(e));
+  }
+  return 0;
+}
+}  // namespace
+
 int fe_alloc(fe_ctx ctx, size_t bytes, void** ptr) {
   FE_ARG(ctx && ptr, "fe_alloc: null argument");
   *ptr = nullptr;
   if (bytes == 0) return 0;
-  FE_HIP_TRY(hipMalloc(ptr, bytes));
+  // 256-byte granules below 2 MiB, 2 MiB granules above
+  const size_t g = bytes < (size_t(2) << 20) ? 256 : (size_t(2) << 20);
+  const size_t want = (bytes + g - 1) / g * g;
+  auto& c = ctx->cache;
+  std::lock_guard<std::mutex> lock(c.mu);
+  auto it = c.parked.lower_bound(want);
+  if (it != c.parked.end() && it->first <= want + want / 4) {
+    *ptr = it->second;
+    c.live[*ptr] = it->first;
+    c.parked_bytes -= it->first;
+    c.parked.erase(it);
+    return 0;
+  }
+  if (hipMalloc(ptr, want) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    const int rc = release_parked(ctx, 0);
+    if (rc) return rc;
+    FE_HIP_TRY(hipMalloc(ptr, want));
+  }
+  c.live[*ptr] = want;
   return 0;
 }
 
 int fe_free(fe_ctx ctx, void* ptr) {
   FE_ARG(ctx, "fe_free: null context");
   if (!ptr) return 0;
-  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  FE_HIP_TRY(hipFree(ptr));
-  return 0;
+  auto& c = ctx->cache;
+  std::lock_guard<std::mutex> lock(c.mu);
+  auto it = c.live.find(ptr);
+  FE_ARG(it != c.live.end(), "fe_free: pointer not allocated by fe_alloc on this context");
+  c.parked.emplace(it->second, ptr);
+  c.parked_bytes += it->second;
+  c.live.erase(it);
+  return release_parked(ctx, ctx->hbm / 8);
 }
 
 int fe_memcpy_h2d(fe_ctx ctx, void* dst, const void* src, size_t bytes) {
@@ -418,8 +501,8 @@ static int colsq_launch(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, i
                         const void* AY, int64_t lday, const double* theta, void* R, int64_t ldr,
                         double* rnorm, double* ynorm) {
   if (k == 0) return 0;
-  // chunk rows so that ~2048 x ceil(k/64) workgroups exist
-  int64_t chunks = 2048 / ((k + 63) / 64);
+  // chunk rows so that ~1024 workgroups (of 4 row phases) exist
+  int64_t chunks = 1024 / ((k + 63) / 64);
   if (chunks < 64) chunks = 64;
   if (chunks > n) chunks = n > 0 ? n : 1;
   const int64_t rows_per = (n + chunks - 1) / chunks;
@@ -434,24 +517,24 @@ static int colsq_launch(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, i
   const bool res = AY != nullptr;
   if (prec == FE_F64) {
     if (res)
-      hipLaunchKernelGGL((k_colsq<double, true>), grid, dim3(64), 0, ctx->stream, n, k, (const double*)Y,
+      hipLaunchKernelGGL((k_colsq<double, true>), grid, dim3(256), 0, ctx->stream, n, k, (const double*)Y,
                          ldy, (const double*)AY, lday, theta, (double*)R, ldr, part_r, part_y, rows_per);
     else
-      hipLaunchKernelGGL((k_colsq<double, false>), grid, dim3(64), 0, ctx->stream, n, k, (const double*)Y,
+      hipLaunchKernelGGL((k_colsq<double, false>), grid, dim3(256), 0, ctx->stream, n, k, (const double*)Y,
                          ldy, nullptr, 0, nullptr, nullptr, 0, nullptr, part_y, rows_per);
   } else {
     if (res)
-      hipLaunchKernelGGL((k_colsq<float, true>), grid, dim3(64), 0, ctx->stream, n, k, (const float*)Y, ldy,
+      hipLaunchKernelGGL((k_colsq<float, true>), grid, dim3(256), 0, ctx->stream, n, k, (const float*)Y, ldy,
                          (const float*)AY, lday, theta, (float*)R, ldr, part_r, part_y, rows_per);
     else
-      hipLaunchKernelGGL((k_colsq<float, false>), grid, dim3(64), 0, ctx->stream, n, k, (const float*)Y, ldy,
+      hipLaunchKernelGGL((k_colsq<float, false>), grid, dim3(256), 0, ctx->stream, n, k, (const float*)Y, ldy,
                          nullptr, 0, nullptr, nullptr, 0, nullptr, part_y, rows_per);
   }
   FE_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 256)), dim3(256), 0, ctx->stream, (int)chunks, k,
+  hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 64)), dim3(256), 0, ctx->stream, (int)chunks, k,
                      part_y, ynorm);
   if (res)
-    hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 256)), dim3(256), 0, ctx->stream, (int)chunks, k,
+    hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 64)), dim3(256), 0, ctx->stream, (int)chunks, k,
                        part_r, rnorm);
   FE_CHECK_LAUNCH();
   return 0;
@@ -498,6 +581,30 @@ int fe_copy2d(fe_ctx ctx, int64_t rows, int cols, const void* src, int sp, int64
     hipLaunchKernelGGL((k_copy2d<float, float>), g, b, 0, ctx->stream, rows, cols, (const float*)src, lds,
                        (float*)dst, ldd);
   FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_block_gather_cols(fe_ctx ctx, int prec, int64_t rows, int p, const void* src, int64_t lds,
+                         const int* idx_host, void* dst, int64_t ldd) {
+  FE_ARG(ctx && src && dst && (p == 0 || idx_host), "fe_block_gather_cols: null argument");
+  FE_ARG(prec == FE_F64 || prec == FE_F32, "fe_block_gather_cols: bad precision");
+  FE_ARG(rows >= 0 && p >= 0 && ldd >= p, "fe_block_gather_cols: bad dimensions");
+  for (int j = 0; j < p; ++j) FE_ARG(idx_host[j] >= 0 && idx_host[j] < lds, "fe_block_gather_cols: column out of range");
+  FE_ARG(src != dst, "fe_block_gather_cols: dst must not alias src");
+  const size_t eb = prec_bytes(prec);
+  for (int j0 = 0; j0 < p; j0 += FE_GATHER_MAX) {
+    const int pc = std::min(FE_GATHER_MAX, p - j0);
+    ColIdx idx{};
+    for (int j = 0; j < pc; ++j) idx.v[j] = idx_host[j0 + j];
+    const dim3 g(grid_for(rows * pc)), b(256);
+    if (prec == FE_F64)
+      hipLaunchKernelGGL(k_gather_cols<double>, g, b, 0, ctx->stream, rows, pc, (const double*)src, lds, idx,
+                         (double*)((char*)dst + eb * j0), ldd);
+    else
+      hipLaunchKernelGGL(k_gather_cols<float>, g, b, 0, ctx->stream, rows, pc, (const float*)src, lds, idx,
+                         (float*)((char*)dst + eb * j0), ldd);
+    FE_CHECK_LAUNCH();
+  }
   return 0;
 }
 

@@ -119,7 +119,7 @@ Block orthonormalize_block(Workspace& ws, const BlockRef& basis, const BlockRef&
     if (keep.empty() || capacity <= 0) return Block();
     // compact the kept columns and scale them to unit norm
     Block kept(ex, src.rows, static_cast<int>(keep.size()), src.prec);
-    for (size_t j = 0; j < keep.size(); ++j) copy(ex, cur.cols(keep[j], 1), kept.cols(static_cast<int>(j), 1));
+    gather_cols(ex, cur.ref(), keep, kept.ref());
     const int c = kept.ncols();
     ws.ensure(c);
     {

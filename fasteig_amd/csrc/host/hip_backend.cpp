@@ -185,6 +185,15 @@ void copy(HipExecutor& ex, const BlockRef& src, const BlockRef& dst) {
   check(fe_copy2d(ex.ctx(), src.rows, src.cols, src.ptr, src.prec, src.ld, dst.ptr, dst.prec, dst.ld), "fe_copy2d");
 }
 
+void gather_cols(HipExecutor& ex, const BlockRef& src, const std::vector<int>& idx, const BlockRef& dst) {
+  if (src.rows != dst.rows || static_cast<int>(idx.size()) != dst.cols || src.prec != dst.prec)
+    throw std::invalid_argument("gather_cols: shape mismatch");
+  for (int c : idx)
+    if (c < 0 || c >= src.cols) throw std::invalid_argument("gather_cols: column out of range");
+  check(fe_block_gather_cols(ex.ctx(), src.prec, src.rows, dst.cols, src.ptr, src.ld, idx.data(), dst.ptr, dst.ld),
+        "fe_block_gather_cols");
+}
+
 void zero(HipExecutor& ex, const BlockRef& X) {
   check(fe_zero2d(ex.ctx(), X.ptr, X.rows, X.cols, X.ld, static_cast<int>(X.esize())), "fe_zero2d");
 }

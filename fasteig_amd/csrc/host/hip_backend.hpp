@@ -163,6 +163,8 @@ void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, doubl
 void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, double alpha,
           double beta);
 void copy(HipExecutor& ex, const BlockRef& src, const BlockRef& dst);
+// dst[:, j] = src[:, idx[j]] in one pass (same precision)
+void gather_cols(HipExecutor& ex, const BlockRef& src, const std::vector<int>& idx, const BlockRef& dst);
 void zero(HipExecutor& ex, const BlockRef& X);
 std::vector<double> column_norms(HipExecutor& ex, const BlockRef& X);
 void upload(HipExecutor& ex, const MatrixXd& host, const BlockRef& dst);

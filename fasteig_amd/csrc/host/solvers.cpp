@@ -122,14 +122,16 @@ FASTEIG_INT ChebyshevSolver::solve() {
   double last_mx = std::numeric_limits<double>::quiet_NaN();  // previous cycle's max residual
   double best_rate = 0.0;  // fastest residual decay seen, decades per filter step
   int cycle_degree = degree;  // filter degree of the coming cycle
+  bool refine_wanted = false;  // FP64 cycles filter the wanted columns only
   double cutoff = std::numeric_limits<double>::quiet_NaN();  // adaptive bound (ritz_bounds)
   hip::Small inv_norms(ex, capacity, 1);
 
   for (int iter = 0; iter < maxiter; ++iter) {
-    // the FP64 cycles of the mixed path refine the wanted columns only; the
-    // guard columns keep their FP32-phase vectors and stay in the
-    // Rayleigh-Ritz subspace
-    const int active = (mixed && prec == FE_F64) ? std::max(0, nev - locked) : subspace_cols - locked;
+    // the FP64 cycles of the mixed path refine the wanted columns only when
+    // the FP32 phase brought the whole block near its floor; the guard
+    // columns then keep their FP32-phase vectors and stay in the Rayleigh-Ritz
+    // subspace
+    const int active = (mixed && prec == FE_F64 && refine_wanted) ? std::max(0, nev - locked) : subspace_cols - locked;
     // 1. filter the active columns (ChebyshevSolver.cpp:151-155, 223-258)
     if (active > 0 && upper != lower) {
       BlockRef prev = F.cols(0, active), cur = P.cols(0, active);
@@ -227,8 +229,11 @@ FASTEIG_INT ChebyshevSolver::solve() {
       // Mixed precision: FP32 cycles until the FP32 residuals stall or reach the
       // FP32 floor; then the subspace is promoted to FP64 and the remaining
       // cycles (filter, CholQR, Rayleigh-Ritz) run in FP64.
+      // A slow decay far above the FP32 floor is the filter's convergence
+      // rate, not a precision limit: the FP32 cycles continue while the
+      // residual still falls by 1% per cycle there.
       const bool floor_reached = mx <= std::max(tol, 2e-5);
-      const bool stalled = iter >= 2 && mx > 0.5 * prev_max;
+      const bool stalled = iter >= 2 && mx > (mx <= 1e-3 ? 0.5 : 0.99) * prev_max;
       prev_max = std::min(prev_max, mx);
       if (std::isfinite(last_mx) && last_mx > mx) best_rate = std::max(best_rate, std::log10(last_mx / mx) / degree);
       if (floor_reached || stalled || iter + 1 == maxiter) {
@@ -242,6 +247,7 @@ FASTEIG_INT ChebyshevSolver::solve() {
           cycle_degree = std::clamp(static_cast<int>(std::ceil(need)), degree, 2 * degree);
         }
         prec = FE_F64;
+        refine_wanted = mx <= 1e-4;
         Block V64(ex, n, capacity, FE_F64);
         hip::copy(ex, V.ref(), V64.ref());
         V = std::move(V64);
@@ -323,7 +329,7 @@ FASTEIG_INT LOBPCGSolver::solve() {
       if (ritz.relres[j] > tol) act.push_back(j);
     const int na = static_cast<int>(act.size());
     Block Wsrc(ex, n, na, prec);
-    for (int j = 0; j < na; ++j) hip::copy(ex, Res.cols(act[j], 1), Wsrc.cols(j, 1));
+    hip::gather_cols(ex, Res.ref(), act, Wsrc.ref());
     // basis [X, orth(W), orth(P)] (LOBPCGSolver.cpp:105-129)
     hip::orthonormal_qr(ws, X, tmp);
     hip::copy(ex, X.ref(), basis.cols(0, bw));
@@ -336,7 +342,7 @@ FASTEIG_INT LOBPCGSolver::solve() {
     }
     if (haveP && sc < capacity) {
       Block Psrc(ex, n, na, prec);
-      for (int j = 0; j < na; ++j) hip::copy(ex, P.cols(act[j], 1), Psrc.cols(j, 1));
+      hip::gather_cols(ex, P.ref(), act, Psrc.ref());
       Block Po = hip::orthonormalize_block(ws, basis.cols(0, sc), Psrc.ref());
       if (!Po.empty()) {
         const int add = std::min(Po.ncols(), capacity - sc);
@@ -437,7 +443,7 @@ FASTEIG_INT DavidsonSolver::solve() {
       if (cc == 0 || act + cc > maxcols) break;
       // correction block from raw residuals (DavidsonSolver.cpp:172-198)
       Block corr(ex, n, cc, prec), ctmp(ex, n, cc, prec), acorr(ex, n, cc, prec);
-      for (int j = 0; j < cc; ++j) hip::copy(ex, Res.cols(ai[j], 1), corr.cols(j, 1));
+      hip::gather_cols(ex, Res.ref(), ai, corr.ref());
       hip::block_gram_schmidt(ws, basis.cols(0, act), corr.ref(), corr.ref());
       hip::orthonormal_qr(ws, corr, ctmp);
       hip::spmm(ex, A, corr.ref(), acorr.ref(), 1.0);
@@ -509,12 +515,13 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
   HipCsrMatrix& A = problem_.device_matrix(options_);
   Workspace ws(ex, pdim, FE_GRAM_NATIVE);
   Block Qb(ex, n, pdim, FE_F64), Z(ex, n, b, FE_F64), Rz(ex, n, b, FE_F64), Qn(ex, n, b, FE_F64);
-  Block v(ex, n, 1, FE_F64), vbest(ex, n, 1, FE_F64);
+  Block v(ex, n, 1, FE_F64), Sw(ex, n, b, FE_F64);
   hip::zero(ex, Qb.ref());
   MatrixXd T(pdim, pdim), beta(b, b);
   const double eps = std::numeric_limits<double>::epsilon();
 
   auto host_small = [&](const hip::Small& S, int rows, int cols) {
+    if (rows > S.rows() || cols > S.cols()) throw std::logic_error("BlockKrylovSchurSolver: small matrix too small.");
     MatrixXd full = S.to_host();
     MatrixXd out(rows, cols);
     for (int j = 0; j < cols; ++j)
@@ -530,66 +537,98 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
   auto vnorm = [&](const BlockRef& x) { return hip::column_norms(ex, x)[0]; };
 
   // rank-aware block QR of src against Qb[:, :active] into Qn (reference
-  // complete_block_qr): Cholesky QR when accurate, else column by column with
-  // pivoting and seeded random completion; R = Qn^T src, tiny columns zeroed
+  // complete_block_qr): R = Qn^T src with columns at or below the threshold
+  // zeroed; returns whether random directions had to complete the basis.
+  // Fast path: Cholesky QR twice on the column-equilibrated block, accepted
+  // when every diagonal of R clears the threshold and Qn is orthonormal and
+  // orthogonal to the basis to 1e-10. Otherwise Gram-Schmidt with column
+  // pivoting (the reference's greedy choice of the longest projected column)
+  // on a working copy deflated one accepted direction at a time; a direction
+  // at or below max(threshold, 1e-12 * longest source column) is replaced by
+  // a seeded random vector projected against the basis.
   auto complete_qr = [&](int active, const BlockRef& src, MatrixXd& R, unsigned qseed, double refnorm) {
     const int capacity = static_cast<int>(std::min<int64_t>(b, n - active));
     const double threshold = 64.0 * eps * refnorm;
-    if (capacity == b) {
+    const std::vector<double> cn0 = hip::column_norms(ex, src);
+    double original = 0.0;
+    for (double c : cn0) original = std::max(original, c);
+    auto finish_r = [&]() {
+      hip::gram(ex, FE_GRAM_NATIVE, Qn.ref(), src, ws.G.data(), ws.cap);
+      R = host_small(ws.G, b, b);
+    };
+    auto rinv_into_r = [&](const std::vector<double>* rowscale) {
+      if (fe_potrf(ex.ctx(), b, ws.G.data(), ws.cap) != 0) return false;
+      hip::check(fe_chol_rinv(ex.ctx(), b, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
+      if (rowscale) {
+        MatrixXd Ri = ws.R.to_host();
+        for (int j = 0; j < b; ++j)
+          for (int i = 0; i < b; ++i) Ri(i, j) /= (*rowscale)[i];
+        ws.R.from_host(Ri);
+      }
+      return true;
+    };
+    bool finite = std::isfinite(original);
+    for (double c : cn0) finite = finite && c > 0.0;
+    if (capacity == b && finite) {
       hip::gram(ex, FE_GRAM_NATIVE, src, src, ws.G.data(), ws.cap);
-      const int info = fe_potrf(ex.ctx(), b, ws.G.data(), ws.cap);
-      hip::check(info, "fe_potrf");
-      if (info == 0) {
-        const MatrixXd L = host_small(ws.G, b, b);
-        hip::check(fe_chol_rinv(ex.ctx(), b, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
-        hip::gemm(ex, src, ws.R.data(), ws.cap, Qn.ref(), 1.0, 0.0);
-        hip::gram(ex, FE_GRAM_NATIVE, Qn.ref(), Qn.ref(), ws.W.data(), ws.cap);
-        MatrixXd Gq = host_small(ws.W, b, b);
+      MatrixXd G = ws.G.to_host();
+      for (int j = 0; j < b; ++j)
+        for (int i = 0; i < b; ++i) G(i, j) /= cn0[i] * cn0[j];
+      ws.G.from_host(G);
+      bool ok = rinv_into_r(&cn0);
+      if (ok) {
+        hip::gemm(ex, src, ws.R.data(), ws.cap, Sw.ref(), 1.0, 0.0);
+        hip::gram(ex, FE_GRAM_NATIVE, Sw.ref(), Sw.ref(), ws.G.data(), ws.cap);
+        ok = rinv_into_r(nullptr);
+      }
+      if (ok) {
+        hip::gemm(ex, Sw.ref(), ws.R.data(), ws.cap, Qn.ref(), 1.0, 0.0);
+        hip::gram(ex, FE_GRAM_NATIVE, Qn.ref(), Qn.ref(), ws.C.data(), ws.cap);
+        MatrixXd Gq = host_small(ws.C, b, b);
         for (int i = 0; i < b; ++i) Gq(i, i) -= 1.0;
         double cross = 0.0;
         if (active > 0) {
           hip::gram(ex, FE_GRAM_NATIVE, Qb.cols(0, active), Qn.ref(), ws.V.data(), ws.cap);
           cross = frob(host_small(ws.V, active, b));
         }
+        finish_r();
         double dmin = std::numeric_limits<double>::infinity();
-        bool finite = true;
-        R = MatrixXd(b, b);
-        for (int j = 0; j < b; ++j)
-          for (int i = 0; i <= j; ++i) {
-            R(i, j) = L(j, i);
-            finite = finite && std::isfinite(R(i, j));
-          }
-        for (int j = 0; j < b; ++j) dmin = std::min(dmin, std::fabs(R(j, j)));
+        bool rfin = true;
+        for (int j = 0; j < b; ++j) {
+          dmin = std::min(dmin, std::fabs(R(j, j)));
+          for (int i = 0; i < b; ++i) rfin = rfin && std::isfinite(R(i, j));
+        }
         const double ge = frob(Gq);
-        if (finite && dmin > threshold && std::isfinite(ge) && std::isfinite(cross) && ge < 1e-10 && cross < 1e-10)
+        if (rfin && dmin > threshold && std::isfinite(ge) && std::isfinite(cross) && ge < 1e-10 && cross < 1e-10) {
+          for (int j = 0; j < b; ++j)
+            for (int i = j + 1; i < b; ++i) R(i, j) = 0.0;
           return false;
+        }
       }
     }
     bool replenished = false;
     hip::zero(ex, Qn.ref());
+    hip::copy(ex, src, Sw.ref());
+    if (active > 0) hip::block_gram_schmidt(ws, Qb.cols(0, active), Sw.ref(), Sw.ref());
     auto project = [&](int j) {
       if (active > 0) hip::block_gram_schmidt(ws, Qb.cols(0, active), v.ref(), v.ref());
       if (j > 0) hip::block_gram_schmidt(ws, Qn.cols(0, j), v.ref(), v.ref());
     };
+    const double cutoff = std::max(threshold, 1e-12 * original);
     for (int j = 0; j < capacity; ++j) {
-      double length = 0.0, original = 0.0;
-      for (int col = 0; col < b; ++col) {
-        hip::copy(ex, src.range(col, 1), v.ref());
-        original = std::max(original, vnorm(v.ref()));
+      const std::vector<double> cn = hip::column_norms(ex, Sw.ref());
+      int best = 0;
+      for (int c = 1; c < b; ++c)
+        if (cn[c] > cn[best]) best = c;
+      double length = 0.0;
+      if (cn[best] > cutoff) {
+        hip::copy(ex, Sw.ref().range(best, 1), v.ref());
         project(j);
-        const double cn = vnorm(v.ref());
-        if (cn > length) {
-          length = cn;
-          hip::copy(ex, v.ref(), vbest.ref());
-        }
-      }
-      const double cutoff = std::max(threshold, 1e-12 * original);
-      if (length > cutoff) {
-        hip::copy(ex, vbest.ref(), v.ref());
         length = vnorm(v.ref());
-      } else {
+        if (!(length > cutoff)) length = 0.0;
+      }
+      if (length == 0.0) {
         replenished = true;
-        length = 0.0;
         for (unsigned attempt = 0; attempt < 4; ++attempt) {
           hip::copy(ex, ex.random_block(n, 1, qseed + 17u * static_cast<unsigned>(j + 1) + attempt).ref(), v.ref());
           const double rn = vnorm(v.ref());
@@ -600,12 +639,15 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
         }
         if (length == 0.0) throw std::runtime_error("BlockKrylovSchurSolver: unable to complete the basis.");
       }
-      hip::check(fe_axpby_cols(ex.ctx(), FE_F64, n, 1, v.data(), v.ld(), Qn.cols(j, 1).ptr, Qn.ld(), 1.0 / length,
-                               0.0, nullptr),
+      const BlockRef qj = Qn.cols(j, 1);
+      hip::check(fe_axpby_cols(ex.ctx(), FE_F64, n, 1, v.data(), v.ld(), qj.ptr, Qn.ld(), 1.0 / length, 0.0,
+                               nullptr),
                  "scale");
+      // deflate the working copy by the accepted direction
+      hip::gram(ex, FE_GRAM_NATIVE, qj, Sw.ref(), ws.G.data(), ws.cap);
+      hip::gemm(ex, qj, ws.G.data(), ws.cap, Sw.ref(), -1.0, 1.0);
     }
-    hip::gram(ex, FE_GRAM_NATIVE, Qn.ref(), src, ws.G.data(), ws.cap);
-    R = host_small(ws.G, b, b);
+    finish_r();
     for (int j = 0; j < b; ++j) {
       double cn = 0.0;
       for (int i = 0; i < b; ++i) cn += R(i, j) * R(i, j);
@@ -672,6 +714,7 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
   long long it_done = 0;
   bool converged = false;
   Block X(ex, n, nev, FE_F64), AX(ex, n, nev, FE_F64), Rr(ex, n, nev, FE_F64);
+  Block U(ex, n, std::min(snap_count, m), FE_F64);
   for (int iter = 1; iter <= maxiter; ++iter) {
     // block Ritz snapshot of T[:m, :m] (RitzExtraction.cpp:272-380)
     MatrixXd Tp(ws.cap, ws.cap);
@@ -743,12 +786,11 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
     {
       hip::Small Cs(ex, m, cnt);
       Cs.from_host(Csel);
-      Block U(ex, n, cnt, FE_F64), Qres(ex, n, b, FE_F64);
-      hip::gemm(ex, Qb.cols(0, m), Cs.data(), m, U.ref(), 1.0, 0.0);
-      hip::copy(ex, Qb.cols(block_count * b, b), Qres.ref());
+      hip::gemm(ex, Qb.cols(0, m), Cs.data(), m, U.cols(0, cnt), 1.0, 0.0);
+      hip::copy(ex, Qb.cols(block_count * b, b), Rz.ref());
       hip::zero(ex, Qb.ref());
-      hip::copy(ex, U.ref(), Qb.cols(0, cnt));
-      hip::copy(ex, Qres.ref(), Qb.cols(cnt, b));
+      hip::copy(ex, U.cols(0, cnt), Qb.cols(0, cnt));
+      hip::copy(ex, Rz.ref(), Qb.cols(cnt, b));
     }
     T.setZero();
     for (int j = 0; j < cnt; ++j) T(j, j) = lam[j];

@@ -173,6 +173,12 @@ int fe_scale_rows(fe_ctx ctx, int prec, int64_t n, int k, void* X, int64_t ldx, 
 /* Copy a rows x cols block between (possibly different) precisions. */
 int fe_copy2d(fe_ctx ctx, int64_t rows, int cols, const void* src, int src_prec, int64_t lds,
               void* dst, int dst_prec, int64_t ldd);
+/* Column gather dst[:, j] = src[:, idx[j]] (j < p, same precision, idx a host
+ * array of columns of src, below lds): one pass over the rows instead of one
+ * strided copy per column (the per-column copies of the reference's soft
+ * locking, src/LOBPCGSolver.cpp:95-103, src/DavidsonSolver.cpp:173-181). */
+int fe_block_gather_cols(fe_ctx ctx, int prec, int64_t rows, int p, const void* src, int64_t lds,
+                         const int* idx_host, void* dst, int64_t ldd);
 /* Host column-major FP64 (n x k, ld n) <-> device row-major block. */
 int fe_upload_colmajor(fe_ctx ctx, int64_t n, int k, const double* host, void* dst, int prec,
                        int64_t ldd);

@@ -5,6 +5,8 @@ inputs and compares with the oracle (oracle/oracle.cpp) or with numpy closed
 forms. Tolerances: FP64 kernels 1e-13 relative (different summation order
 only); FP32 kernels 1e-5 relative to the operand scale; BF16x3 Gram 2e-4.
 """
+import ctypes
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -198,7 +200,8 @@ def test_spmm_strided_subblock(dev, mats):
 
 
 @pytest.mark.parametrize("n,m,p", [(1000, 16, 16), (5000, 48, 40), (20000, 96, 96), (3000, 200, 70),
-                                   (777, 5, 3), (100000, 256, 256), (40000, 512, 512)])
+                                   (777, 5, 3), (100000, 256, 256), (40000, 512, 512), (200003, 192, 48),
+                                   (50000, 144, 144), (3001, 65, 64), (4000, 161, 150)])
 @pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
 def test_gram(dev, n, m, p, prec):
     rng = np.random.default_rng(n + m)
@@ -247,7 +250,8 @@ def test_gram_reproducible(dev):
 
 
 @pytest.mark.parametrize("n,m,p", [(1000, 16, 16), (5000, 48, 10), (20000, 96, 96), (3333, 200, 70),
-                                   (100000, 256, 256), (7, 3, 2)])
+                                   (100000, 256, 256), (7, 3, 2), (200001, 192, 48), (1001, 48, 64),
+                                   (999, 20, 65), (300, 33, 1)])
 @pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
 def test_gemm_nn(dev, n, m, p, prec):
     rng = np.random.default_rng(n + p)
@@ -384,3 +388,23 @@ def test_axpby_copy_upload(dev):
     B32 = dev.buffer((n, k), np.float32)
     _fe.check(L().fe_copy2d(dev.ctx, n, k, Xd.ptr, _fe.F64, k, B32.ptr, _fe.F32, k))
     assert np.array_equal(B32.download(), X.astype(np.float32))
+
+
+@pytest.mark.parametrize("n,k,idx", [(1000, 48, [3, 0, 47, 5, 5]), (70001, 300, list(range(299, -1, -1))),
+                                     (13, 4, [])])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_block_gather_cols(dev, n, k, idx, prec):
+    rng = np.random.default_rng(n + k)
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    X = rng.standard_normal((n, k)).astype(dt)
+    Xd = dev.put(X, dt)
+    p = len(idx)
+    ld = p + 3
+    Dd = dev.put(np.full((n, ld), -7.0, dt), dt)
+    ia = (ctypes.c_int * max(p, 1))(*idx)
+    _fe.check(L().fe_block_gather_cols(dev.ctx, prec, n, p, Xd.ptr, k, ia, Dd.ptr, ld))
+    got = Dd.download()
+    assert np.array_equal(got[:, :p], X[:, idx])
+    assert np.all(got[:, p:] == -7.0)
+    bad = (ctypes.c_int * 1)(k)
+    assert L().fe_block_gather_cols(dev.ctx, prec, n, 1, Xd.ptr, k, bad, Dd.ptr, ld) == -2
