@@ -101,8 +101,10 @@ __global__ __launch_bounds__(256, 2) void k_gram(int64_t n, int m, int p, const 
                                               const double* __restrict__ w, int tiles_b, int ntiles,
                                               int sym, int64_t rows_per, int64_t per_xcd, int64_t nunits,
                                               int vec, double* __restrict__ part) {
-  __shared__ T Xs[GKS][WT + GPAD];
-  __shared__ T Ys[GKS][WT + GPAD];
+  // rows per stage: 32 for FP32 (two per staging thread), 16 for FP64
+  constexpr int GK = sizeof(T) == 4 ? 2 * GKS : GKS, NR = GK / 16;
+  __shared__ T Xs[GK][WT + GPAD];
+  __shared__ T Ys[GK][WT + GPAD];
   const int64_t u = xcd_tile(blockIdx.x, per_xcd);
   if (u >= nunits) return;
   const int64_t chunk = u / ntiles;
@@ -122,37 +124,42 @@ __global__ __launch_bounds__(256, 2) void k_gram(int64_t n, int m, int p, const 
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{};
-  // staging: thread -> row tid / 16, columns (tid % 16) * 8 .. + 7 of the tile
+  // staging: thread -> rows tid / 16 + 16 h (h < NR), columns (tid % 16) * 8 .. + 7 of the tile
   const int sr = tid >> 4, sc = (tid & 15) * 8;
   const int va = m - (a0 + sc), vb = p - (b0 + sc);
-  T xr[8], yr[8];
+  T xr[NR][8], yr[NR][8];
   auto fetch = [&](int64_t rb) {
-    const int64_t r = rb + sr;
-    if (r < r1) {
-      load8(X + r * ldx + a0 + sc, vec, va, xr);
-      load8(Y + r * ldy + b0 + sc, vec, vb, yr);
-      if (w) {
-        const double wr = w[r];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) yr[e] = static_cast<T>(static_cast<double>(yr[e]) * wr);
+    for (int h = 0; h < NR; ++h) {
+      const int64_t r = rb + sr + 16 * h;
+      if (r < r1) {
+        load8(X + r * ldx + a0 + sc, vec, va, xr[h]);
+        load8(Y + r * ldy + b0 + sc, vec, vb, yr[h]);
+        if (w) {
+          const double wr = w[r];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) yr[h][e] = static_cast<T>(static_cast<double>(yr[h][e]) * wr);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xr[h][e] = yr[h][e] = T(0);
       }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) xr[e] = yr[e] = T(0);
     }
   };
   fetch(r0);
-  for (int64_t rb = r0; rb < r1; rb += GKS) {
+  for (int64_t rb = r0; rb < r1; rb += GK) {
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      Xs[sr][sc + e] = xr[e];
-      Ys[sr][sc + e] = yr[e];
-    }
-    __syncthreads();
-    if (rb + GKS < r1) fetch(rb + GKS);
+    for (int h = 0; h < NR; ++h)
 #pragma unroll
-    for (int kk = 0; kk < GKS; kk += 4) {
+      for (int e = 0; e < 8; ++e) {
+        Xs[sr + 16 * h][sc + e] = xr[h][e];
+        Ys[sr + 16 * h][sc + e] = yr[h][e];
+      }
+    __syncthreads();
+    if (rb + GK < r1) fetch(rb + GK);
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 4) {
       const int kr = kk + (lane >> 4);
       T af[4], bf[4];
 #pragma unroll
@@ -413,10 +420,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
                                                  T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles,
                                                  int vec) {
   static_assert((TR / 64) * (TC / 64) == 4, "four 64 x 64 wavefront tiles");
-  constexpr int TPR = 256 / TR, XK = MKS / TPR;  // X staging: threads per row, k values per thread
-  constexpr int TPC = 256 / TC, CK = MKS / TPC;  // C staging: threads per column, k values per thread
-  __shared__ T Xs[TR][MKS + 1];
-  __shared__ T Cs[MKS][TC + GPAD];
+  constexpr int MK = MKS;  // k values per stage
+  constexpr int TPR = 256 / TR, XK = MK / TPR;  // X staging: threads per row, k values per thread
+  constexpr int TPC = 256 / TC, CK = MK / TPC;  // C staging: threads per column, k values per thread
+  __shared__ T Xs[TR][MK + 1];
+  __shared__ T Cs[MK][TC + GPAD];
   const int64_t t = xcd_tile(blockIdx.x, per_xcd);
   if (t >= ntiles) return;
   const int64_t rt = t / tiles_c;
@@ -457,16 +465,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
     }
   };
   fetch(0);
-  for (int k0 = 0; k0 < m; k0 += MKS) {
+  for (int k0 = 0; k0 < m; k0 += MK) {
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < XK; ++e) Xs[sr][sk + e] = xr[e];
 #pragma unroll
     for (int e = 0; e < CK; ++e) Cs[sck + e][scc] = cr[e];
     __syncthreads();
-    if (k0 + MKS < m) fetch(k0 + MKS);
+    if (k0 + MK < m) fetch(k0 + MK);
 #pragma unroll
-    for (int kk = 0; kk < MKS; kk += 4) {
+    for (int kk = 0; kk < MK; kk += 4) {
       const int kr = kk + (lane >> 4);
       T af[4], bf[4];
 #pragma unroll
@@ -515,7 +523,7 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
     // 64 x 64 tiles (rows split over the waves) for thin or small products,
     // 128 x 128 tiles otherwise
     const bool small = std::min(m, p) <= 64 || std::max(m, p) <= 160;
-    const int TT = small ? G64 : WT, RS = small ? G64R : GKS;
+    const int TT = small ? G64 : WT, RS = small ? G64R : 2 * GKS;
     const int tiles_a = (m + TT - 1) / TT, tiles_b = (p + TT - 1) / TT;
     const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
     // about 4 work units per CU; chunks of whole stages
