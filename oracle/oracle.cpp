@@ -26,6 +26,11 @@
 //                       2D Laplacian closed form, SiO spectrum from the
 //                       reference's tests/reference/SiO.out.121mpi (golden
 //                       fixture), dense eigh on small random problems.
+//   orc_lanczos / orc_krylov_schur
+//                       2D Laplacian closed form (SR, LR, BE ends), SiO golden
+//                       spectrum, dense eigh on small random problems, a
+//                       matrix with a repeated eigenvalue (invariant-subspace
+//                       restart), Lanczos at m = n against eigh exactly.
 //   orc_random_block    reference's seeded mt19937/uniform(-1,1) fill: pinned by
 //                       the C++ standard's required 10000th mt19937 output
 //                       (4123659995) and by a golden vector.
@@ -1178,6 +1183,208 @@ Result block_krylov_schur(const Csr& A, int nev, int bsz, int maxdim, int maxite
   return res;
 }
 
+// Scalar Krylov basis with full two-pass reorthogonalization. Reference:
+// src/core/KrylovSubspace.cpp:124-260 (normalize_residual_into_basis with the
+// random restart of normalize_or_restart / restart_residual,
+// apply_operator_to_basis_column, orthogonalize_operator_result, residual_norm,
+// rotate_basis) and src/core/MultiVector.cpp:153-187 (two-pass projection
+// returning the coefficient of the newest column).
+struct ScalarKrylov {
+  const Csr& A;
+  int n;
+  unsigned seed;
+  Mat basis, r, av;
+  ScalarKrylov(const Csr& a, int cols, const Mat& start, unsigned s)
+      : A(a), n(a.n), seed(s), basis(a.n, cols), r(start), av(a.n, 1) {}
+  double project(int active, const Mat& src, Mat& res) {
+    res = src;
+    if (active == 0) return 0.0;
+    Mat h = AtB(basis, active, src, 1);
+    Mat vh = AC(basis, active, h);
+    for (int t = 0; t < n; ++t) res.a[t] -= vh.a[t];
+    Mat s2 = AtB(basis, active, res, 1);
+    Mat vs = AC(basis, active, s2);
+    for (int t = 0; t < n; ++t) res.a[t] -= vs.a[t];
+    return h.a[active - 1] + s2.a[active - 1];
+  }
+  double norm() const {
+    const double b = column_norms(r, 1)[0];
+    if (!std::isfinite(b)) throw std::runtime_error("core::KrylovSubspace: non-finite residual norm encountered.");
+    return b;
+  }
+  // the recorded coupling: 0 after a random restart (invariant subspace)
+  double normalize_into(int col) {
+    const double eps = std::numeric_limits<double>::epsilon();
+    double beta = norm();
+    bool restarted = false;
+    if (!(beta > eps)) {
+      Mat cand = random_block(n, 1, seed + static_cast<unsigned>(col + 1));
+      project(col, cand, r);
+      restarted = true;
+      beta = norm();
+      if (beta <= eps) throw std::runtime_error("core::KrylovSubspace: breakdown, unable to generate new basis vector.");
+    }
+    for (int t = 0; t < n; ++t) r.a[t] /= beta;
+    std::copy(r.a.begin(), r.a.end(), basis.col(col));
+    return restarted ? 0.0 : beta;
+  }
+  // z = A q_col, orthogonalized against q_0..q_col into r; returns alpha
+  double step(int col) {
+    spmm(A, basis, col, 1, av, 1.0, 0.0);
+    return project(col + 1, av, r);
+  }
+  void rotate(int active, const Mat& C) {
+    Mat U = AC(basis, active, C);
+    copy_cols(basis, 0, U, 0, C.c);
+  }
+};
+
+// Projected tridiagonal (arrowhead after a restart). Reference:
+// src/core/LanczosWorkspace.cpp:25-40 (record_step), :97-108
+// (prepare_krylov_schur_restart), src/core/RitzExtraction.cpp:195-230,252-268
+// (snapshot: |beta_next * y_last| / (1 + |theta|)), :470-492 (restart matrix).
+struct Projection {
+  Mat T;
+  int coupling_row = -1;
+  double beta_next = 0.0;
+  explicit Projection(int m) : T(m, m) {}
+  void record(int it, double beta, double alpha, double bn) {
+    if (it != 0 && it != coupling_row) {
+      T(it, it - 1) = beta;
+      T(it - 1, it) = beta;
+    }
+    T(it, it) = alpha;
+    beta_next = bn;
+    if (it == coupling_row) coupling_row = -1;
+  }
+  void snapshot(int m, int count, const std::string& which, Vec& lam, Vec& rel, Mat& Csel) const {
+    Mat P(m, m);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) P(i, j) = T(i, j);
+    Vec w;
+    Mat V;
+    if (syev(P, w, V) != 0) throw std::runtime_error("projected eigensolve failed.");
+    const auto idx = sort_indices(w, which);
+    const int cnt = std::min(count, m);
+    lam.assign(cnt, 0.0);
+    rel.assign(cnt, 0.0);
+    Csel = Mat(m, cnt);
+    for (int j = 0; j < cnt; ++j) {
+      const int c = idx[j];
+      lam[j] = w[c];
+      rel[j] = std::fabs(beta_next * V(m - 1, c)) / (1.0 + std::fabs(w[c]));
+      std::copy(V.col(c), V.col(c) + m, Csel.col(j));
+    }
+  }
+  void restart(int retained, const Vec& lam, const Mat& Csel) {
+    const int m = T.r;
+    T = Mat(m, m);
+    for (int j = 0; j < retained; ++j) {
+      T(j, j) = lam[j];
+      const double coupling = beta_next * Csel(Csel.r - 1, j);
+      T(retained, j) = coupling;
+      T(j, retained) = coupling;
+    }
+    coupling_row = retained;
+  }
+};
+
+void check_lanczos_which(const std::string& which, const char* who) {
+  if (which != "sr" && which != "lr" && which != "sm" && which != "lm" && which != "be")
+    throw std::invalid_argument(std::string(who) + ": invalid which option (use SR/LR/SM/LM/BE).");
+}
+
+// Scalar Lanczos with full reorthogonalization, no restart (standard problems).
+// Reference: src/LanczosSolver.cpp:28-164; convergence checks every 5 steps
+// (print_every = 0) once m >= nev, and at maxiter.
+Result lanczos(const Csr& A, int nev, int maxiter, double tol, const std::string& which, unsigned seed) {
+  const int n = A.n;
+  if (maxiter <= 0) throw std::invalid_argument("LanczosSolver: maxiter must be > 0.");
+  if (nev <= 0) throw std::invalid_argument("LanczosSolver: nev must be > 0.");
+  if (maxiter < nev) throw std::invalid_argument("LanczosSolver: maxiter must be >= nev.");
+  if (maxiter > n) throw std::invalid_argument("LanczosSolver: maxiter must be <= n.");
+  check_lanczos_which(which, "LanczosSolver");
+  ScalarKrylov K(A, maxiter, random_block(n, 1, seed), seed);
+  Projection P(maxiter);
+  Vec lam, rel;
+  Mat Csel;
+  int it_stop = maxiter;
+  bool converged = false;
+  for (int it = 0; it < maxiter; ++it) {
+    const double beta = K.normalize_into(it);
+    const double alpha = K.step(it);
+    P.record(it, beta, alpha, K.norm());
+    const int m = it + 1;
+    if (!(m >= nev && (m % 5 == 0 || m == maxiter))) continue;
+    P.snapshot(m, nev, which, lam, rel, Csel);
+    double mx = 0.0;
+    for (double x : rel) mx = std::max(mx, x);
+    if (!rel.empty() && mx < tol) {
+      converged = true;
+      it_stop = m;
+      break;
+    }
+  }
+  if (!converged) P.snapshot(maxiter, nev, which, lam, rel, Csel);
+  Result res;
+  res.evals = lam;
+  res.evecs = AC(K.basis, it_stop, Csel);
+  res.relres = residual_norms(A, res.evals, res.evecs);
+  res.iterations = it_stop;
+  res.status = 0;
+  return res;
+}
+
+// Scalar Krylov-Schur with thick restart (standard problems).
+// Reference: src/KrylovSchurSolver.cpp:28-167.
+Result krylov_schur(const Csr& A, int nev, int maxdim, int maxiter, double tol, const std::string& which,
+                    unsigned seed) {
+  const int n = A.n;
+  if (n <= 0) throw std::invalid_argument("KrylovSchurSolver: n must be > 0.");
+  if (nev <= 0 || nev > n) throw std::invalid_argument("KrylovSchurSolver: nev must be in [1,n].");
+  if (maxiter <= 0) throw std::invalid_argument("KrylovSchurSolver: maxiter must be > 0.");
+  check_lanczos_which(which, "KrylovSchurSolver");
+  const int m = std::min(std::max({2 * nev, 20, maxdim}), n);
+  ScalarKrylov K(A, m, random_block(n, 1, seed), seed);
+  Projection P(m);
+  auto expand = [&](int begin) {
+    for (int col = begin; col < m; ++col) {
+      const double beta = K.normalize_into(col);
+      const double alpha = K.step(col);
+      P.record(col, beta, alpha, K.norm());
+    }
+  };
+  expand(0);
+  Result res;
+  for (int iter = 1; iter <= maxiter; ++iter) {
+    Vec lam, rel;
+    Mat Csel;
+    P.snapshot(m, m, which, lam, rel, Csel);
+    int nconv = 0;
+    for (int j = 0; j < nev; ++j)
+      if (rel[j] * (1.0 + std::fabs(lam[j])) < tol * std::fabs(lam[j])) ++nconv;
+    if (nconv >= nev || m == n || iter == maxiter) {
+      Mat Ch(m, nev);
+      copy_cols(Ch, 0, Csel, 0, nev);
+      res.evals.assign(lam.begin(), lam.begin() + nev);
+      res.evecs = AC(K.basis, m, Ch);
+      res.relres = residual_norms(A, res.evals, res.evecs);
+      res.iterations = iter;
+      res.status = 0;
+      return res;
+    }
+    const int spare = (m - nev) / 2;
+    int retained = nev + std::min(nconv, spare);
+    if (retained == 1 && m > 3) retained = m / 2;
+    Mat Ck(m, retained);
+    copy_cols(Ck, 0, Csel, 0, retained);
+    K.rotate(m, Ck);
+    P.restart(retained, lam, Csel);
+    expand(retained);
+  }
+  throw std::logic_error("KrylovSchurSolver: unreachable solve termination.");
+}
+
 Vec residual_norms(const Csr& A, const Vec& evals, const Mat& X) {
   const int k = static_cast<int>(evals.size());
   Mat AX(X.r, k);
@@ -1370,6 +1577,23 @@ int orc_block_krylov_schur(int n, const int* rp, const int* ci, const double* v,
                            double* res, long* iters) {
   return guarded([&] {
     auto r = orc::block_krylov_schur(csr(n, rp, ci, v), nev, block, maxdim, maxiter, tol, which, seed);
+    return finish(r, nev, n, ev, evec, res, iters);
+  });
+}
+
+int orc_lanczos(int n, const int* rp, const int* ci, const double* v, int nev, int maxiter, double tol,
+                const char* which, unsigned seed, double* ev, double* evec, double* res, long* iters) {
+  return guarded([&] {
+    auto r = orc::lanczos(csr(n, rp, ci, v), nev, maxiter, tol, which, seed);
+    return finish(r, nev, n, ev, evec, res, iters);
+  });
+}
+
+int orc_krylov_schur(int n, const int* rp, const int* ci, const double* v, int nev, int maxdim, int maxiter,
+                     double tol, const char* which, unsigned seed, double* ev, double* evec, double* res,
+                     long* iters) {
+  return guarded([&] {
+    auto r = orc::krylov_schur(csr(n, rp, ci, v), nev, maxdim, maxiter, tol, which, seed);
     return finish(r, nev, n, ev, evec, res, iters);
   });
 }

@@ -52,6 +52,8 @@ def lib():
         L.orc_davidson.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_block_krylov_schur.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int] + \
             solver_tail
+        L.orc_lanczos.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int] + solver_tail
+        L.orc_krylov_schur.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_residual_norms.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_max_threads.restype = C.c_int
@@ -201,6 +203,16 @@ def block_krylov_schur(A, nev, block=10, maxdim=0, maxiter=1000, tol=1e-8, which
                        want_vectors=True):
     """Block Krylov-Schur with thick restart (maxdim 0: max(2 nev, 20))."""
     return _solve(lib().orc_block_krylov_schur, A, nev, (block, maxdim), maxiter, tol, which, seed, want_vectors)
+
+
+def lanczos(A, nev, maxiter=100, tol=1e-8, which="sr", seed=1, want_vectors=True):
+    """Scalar Lanczos with full reorthogonalization; maxiter steps at most."""
+    return _solve(lib().orc_lanczos, A, nev, (), maxiter, tol, which, seed, want_vectors)
+
+
+def krylov_schur(A, nev, maxdim=0, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True):
+    """Scalar Krylov-Schur with thick restart; subspace max(2 nev, 20, maxdim)."""
+    return _solve(lib().orc_krylov_schur, A, nev, (maxdim,), maxiter, tol, which, seed, want_vectors)
 
 
 def residual_norms(A, evals, X):

@@ -234,3 +234,75 @@ def test_block_krylov_schur_invariant_start_replenishes():
     assert np.all(np.abs(r["eigenvalues"] - 1.0) <= 1e-12)
     X = r["eigenvectors"]
     assert np.max(np.abs(X.T @ X - np.eye(4))) <= 1e-10
+
+
+def test_sio_golden_spectrum_krylov_schur():
+    """The reference's CTest case reference_cpu_SiO_KrylovSchur_SR (k=5, maxdim
+    40, tol 1e-8, seed 1; tests/scripts/run_case.py defaults) against the
+    ScaLAPACK spectrum."""
+    A = load_sio()
+    ref = json.load(open(os.path.join(GOLD, "sio_eigs.json")))["eigenvalues"][:5]
+    r = orc.krylov_schur(A, 5, maxdim=40, maxiter=2000, tol=1e-8, which="sr", seed=1)
+    assert r["status"] == 0
+    assert np.all(np.abs(r["eigenvalues"] - ref) <= 1e-10 + 1e-10 * np.abs(ref))
+    assert orc.residual_norms(A, r["eigenvalues"], r["eigenvectors"]).max() <= 1e-8
+
+
+def test_sio_golden_spectrum_lanczos():
+    """reference_cpu_SiO_Lanczos_SR: maxiter min(2000, n), k=5, tol 1e-8."""
+    A = load_sio()
+    ref = json.load(open(os.path.join(GOLD, "sio_eigs.json")))["eigenvalues"][:5]
+    r = orc.lanczos(A, 5, maxiter=min(2000, A.shape[0]), tol=1e-8, which="sr", seed=1)
+    assert r["status"] == 0 and r["iterations"] % 5 == 0
+    assert np.all(np.abs(r["eigenvalues"] - ref) <= 1e-10 + 1e-10 * np.abs(ref))
+    assert orc.residual_norms(A, r["eigenvalues"], r["eigenvectors"]).max() <= 1e-8
+
+
+@pytest.mark.parametrize("which", ["sr", "lr", "be"])
+def test_scalar_krylov_closed_form(which):
+    """30 x 17 Laplacian (simple spectrum): both scalar solvers return the
+    closed-form ends in the reference's order (BE alternates largest,
+    smallest)."""
+    A = G.laplacian_2d(30, 17)
+    ex = np.sort(G.laplacian_2d_eigenvalues(30, 17))
+    want = {"sr": ex[:6], "lr": ex[::-1][:6], "be": ex[[-1, 0, -2, 1, -3, 2]]}[which]
+    for r in (orc.lanczos(A, 6, maxiter=300, tol=1e-9, which=which, seed=1),
+              orc.krylov_schur(A, 6, maxdim=30, maxiter=300, tol=1e-9, which=which, seed=1)):
+        assert r["status"] == 0
+        assert np.all(np.abs(r["eigenvalues"] - want) <= 1e-9 * np.abs(want))
+
+
+def test_lanczos_full_dimension_is_exact():
+    """m = n steps: the Lanczos basis spans the space and the Ritz values are
+    the eigenvalues (dense eigh)."""
+    A = G.laplacian_2d(7, 5)
+    w = np.linalg.eigvalsh(A.toarray())
+    r = orc.lanczos(A, 5, maxiter=35, tol=1e-300, which="sr", seed=1)
+    assert r["iterations"] == 35
+    assert np.all(np.abs(r["eigenvalues"] - w[:5]) <= 1e-12 * np.abs(w).max())
+
+
+def test_scalar_krylov_invariant_subspace_restart():
+    """diag(1 x 10, 2 x 10, 3 x 10): one start vector spans a 3-dimensional
+    Krylov space, so the chain breaks down and continues from seeded random
+    vectors. Krylov-Schur finds four copies of 1 with orthonormal vectors;
+    Lanczos stops at its first check (m = 10) with the values the broken
+    chain holds, as the reference's scalar recurrence does."""
+    A = sp.diags(np.r_[np.ones(10), 2.0 * np.ones(10), 3.0 * np.ones(10)]).tocsr()
+    r = orc.krylov_schur(A, 4, maxdim=12, maxiter=200, tol=1e-10, which="sr", seed=2)
+    assert r["status"] == 0 and np.all(np.abs(r["eigenvalues"] - 1.0) <= 1e-12)
+    X = r["eigenvectors"]
+    assert np.max(np.abs(X.T @ X - np.eye(4))) <= 1e-10
+    r = orc.lanczos(A, 4, maxiter=30, tol=1e-10, which="sr", seed=2)
+    assert r["iterations"] == 10
+    assert np.allclose(r["eigenvalues"], [1.0, 1.0, 1.0, 2.0], atol=1e-12)
+
+
+def test_scalar_krylov_rejects_bad_arguments():
+    A = G.laplacian_2d(6)
+    with pytest.raises(ValueError, match="maxiter must be >= nev"):
+        orc.lanczos(A, 4, maxiter=3)
+    with pytest.raises(ValueError, match="maxiter must be <= n"):
+        orc.lanczos(A, 4, maxiter=37)
+    with pytest.raises(ValueError, match="invalid which"):
+        orc.krylov_schur(A, 4, which="xx")
