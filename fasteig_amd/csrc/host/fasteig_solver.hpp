@@ -58,6 +58,25 @@ class BlockKrylovSchurSolver final : public fasteig_solver {
   FASTEIG_INT solve() override;
 };
 
+// Scalar Lanczos with full two-pass reorthogonalization and no restart
+// (src/LanczosSolver.cpp:28-164): convergence checked every print_every steps
+// (5 when 0) once m >= nev. Standard problems, which = SR, LR, SM, LM, BE; the
+// projected dimension is limited to 1024 on the HIP backend.
+class LanczosSolver final : public fasteig_solver {
+ public:
+  LanczosSolver(::fasteig_problem& problem, const fasteig_options& options);
+  FASTEIG_INT solve() override;
+};
+
+// Scalar Krylov-Schur with thick restart (src/KrylovSchurSolver.cpp:28-167):
+// subspace max(2 nev, 20, maxdim) (at most 1024 here), restart keeping
+// nev + min(nconv, (m - nev) / 2) Ritz vectors.
+class KrylovSchurSolver final : public fasteig_solver {
+ public:
+  KrylovSchurSolver(::fasteig_problem& problem, const fasteig_options& options);
+  FASTEIG_INT solve() override;
+};
+
 // LOBPCG without preconditioner (src/LOBPCGSolver.cpp:25-173). Extension:
 // blksize > nev widens the block; convergence and output use the first nev.
 class LOBPCGSolver final : public fasteig_solver {

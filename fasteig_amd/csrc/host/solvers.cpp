@@ -814,15 +814,339 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
   return status_;
 }
 
+// ============================================================== scalar Krylov
+namespace {
+constexpr int kMaxProjected = 1024;  // largest projected matrix fe_syev takes
+
+// Scalar Krylov basis on the device with full two-pass reorthogonalization.
+// Reference: src/core/KrylovSubspace.cpp:124-260 (normalize_residual_into_basis
+// with the random restart of normalize_or_restart / restart_residual,
+// apply_operator_to_basis_column, orthogonalize_operator_result, residual_norm,
+// rotate_basis) and src/core/MultiVector.cpp:153-187 (the two-pass projection
+// returns the coefficient of the newest column).
+class DeviceKrylov {
+ public:
+  DeviceKrylov(HipExecutor& ex, HipCsrMatrix& A, const fasteig_options& o, int cols, unsigned seed)
+      : ex_(ex), A_(A), n_(A.rows()), seed_(seed), ws_(ex, cols, FE_GRAM_NATIVE), basis_(ex, A.rows(), cols, FE_F64),
+        r_(ex, A.rows(), 1, FE_F64), av_(ex, A.rows(), 1, FE_F64), xb_(ex, A.rows(), 1, FE_F64),
+        xv_{xb_.data(), A.rows(), 1, 1, FE_F64} {
+    if (o.v0().size() > 0) {
+      if (o.v0().rows() != n_) throw std::invalid_argument("v0 row count does not match the problem.");
+      MatrixXd head(static_cast<int>(n_), 1);
+      std::copy(o.v0().data(), o.v0().data() + n_, head.data());
+      hip::upload(ex_, head, r_.ref());
+    } else {
+      hip::copy(ex_, ex_.random_block(n_, 1, seed_).ref(), r_.ref());
+    }
+  }
+  Workspace& ws() { return ws_; }
+  const Block& basis() const { return basis_; }
+  double norm() {
+    const double b = hip::column_norms(ex_, r_.ref())[0];
+    if (!std::isfinite(b)) throw std::runtime_error("core::KrylovSubspace: non-finite residual norm encountered.");
+    return b;
+  }
+  // q_col = r / ||r|| (a seeded random vector projected against q_0..q_col-1
+  // when ||r|| <= eps); returns the coupling to record (0 after such a restart)
+  double normalize_into(int col) {
+    const double eps = std::numeric_limits<double>::epsilon();
+    double beta = norm();
+    bool restarted = false;
+    if (!(beta > eps)) {
+      project(col, ex_.random_block(n_, 1, seed_ + static_cast<unsigned>(col + 1)).ref(), r_.ref());
+      restarted = true;
+      beta = norm();
+      if (beta <= eps) throw std::runtime_error("core::KrylovSubspace: breakdown, unable to generate new basis vector.");
+    }
+    hip::check(fe_axpby_cols(ex_.ctx(), FE_F64, n_, 1, r_.data(), r_.ld(), r_.data(), r_.ld(), 1.0 / beta, 0.0,
+                             nullptr),
+               "scale");
+    hip::copy(ex_, r_.ref(), basis_.cols(col, 1));
+    hip::copy(ex_, r_.ref(), xv_);  // unit-stride copy for the product
+    return restarted ? 0.0 : beta;
+  }
+  // r = A q_col orthogonalized against q_0..q_col; returns alpha
+  double step(int col) {
+    hip::spmm(ex_, A_, xv_, av_.ref(), 1.0);
+    return project(col + 1, av_.ref(), r_.ref());
+  }
+  // basis[:, :C.cols()] = basis[:, :active] C
+  void rotate(int active, const MatrixXd& C) {
+    hip::Small Cs(ex_, active, C.cols());
+    Cs.from_host(C);
+    Block U(ex_, n_, C.cols(), FE_F64);
+    hip::gemm(ex_, basis_.cols(0, active), Cs.data(), active, U.ref(), 1.0, 0.0);
+    hip::copy(ex_, U.ref(), basis_.cols(0, C.cols()));
+  }
+  // Ritz vectors X = basis[:, :active] C and their residual norms
+  // ||A x - theta x|| / ((1 + |theta|) ||x||) (src/core/RitzExtraction.cpp:34-63)
+  std::vector<double> ritz(int active, const MatrixXd& C, const std::vector<double>& theta, Block& X) {
+    const int k = C.cols();
+    hip::Small Cs(ex_, active, k), th(ex_, k, 1), rn(ex_, k, 1), yn(ex_, k, 1);
+    Cs.from_host(C);
+    X = Block(ex_, n_, k, FE_F64);
+    Block AX(ex_, n_, k, FE_F64);
+    hip::gemm(ex_, basis_.cols(0, active), Cs.data(), active, X.ref(), 1.0, 0.0);
+    hip::spmm(ex_, A_, X.ref(), AX.ref(), 1.0);
+    hip::check(fe_memcpy_h2d(ex_.ctx(), th.data(), theta.data(), sizeof(double) * k), "h2d");
+    hip::check(fe_ritz_residuals(ex_.ctx(), FE_F64, n_, k, X.data(), X.ld(), AX.data(), AX.ld(), th.data(), nullptr,
+                                 0, rn.data(), yn.data()),
+               "fe_ritz_residuals");
+    std::vector<double> rh(k), yh(k), out(k);
+    hip::check(fe_memcpy_d2h(ex_.ctx(), rh.data(), rn.data(), sizeof(double) * k), "d2h");
+    hip::check(fe_memcpy_d2h(ex_.ctx(), yh.data(), yn.data(), sizeof(double) * k), "d2h");
+    for (int j = 0; j < k; ++j) {
+      const double den = (1.0 + std::fabs(theta[j])) * yh[j];
+      if (!(den > 0.0) || !std::isfinite(den) || !std::isfinite(rh[j]))
+        throw std::runtime_error("evaluate_ritz_pairs: invalid vector or non-finite residual.");
+      out[j] = rh[j] / den;
+    }
+    return out;
+  }
+
+ private:
+  double project(int active, const BlockRef& src, const BlockRef& res) {
+    if (src.ptr != res.ptr) hip::copy(ex_, src, res);
+    if (active == 0) return 0.0;
+    const BlockRef Vb = basis_.cols(0, active);
+    hip::gram(ex_, FE_GRAM_NATIVE, Vb, res, ws_.G.data(), ws_.cap);
+    hip::gemm(ex_, Vb, ws_.G.data(), ws_.cap, res, -1.0, 1.0);
+    hip::gram(ex_, FE_GRAM_NATIVE, Vb, res, ws_.R.data(), ws_.cap);
+    hip::gemm(ex_, Vb, ws_.R.data(), ws_.cap, res, -1.0, 1.0);
+    double h = 0.0, s = 0.0;
+    hip::check(fe_memcpy_d2h(ex_.ctx(), &h, ws_.G.data() + (active - 1), sizeof(double)), "d2h");
+    hip::check(fe_memcpy_d2h(ex_.ctx(), &s, ws_.R.data() + (active - 1), sizeof(double)), "d2h");
+    return h + s;
+  }
+
+  HipExecutor& ex_;
+  HipCsrMatrix& A_;
+  int64_t n_;
+  unsigned seed_;
+  Workspace ws_;
+  Block basis_, r_, av_, xb_;
+  BlockRef xv_;
+};
+
+// Projected tridiagonal (an arrowhead after a restart), kept on the host and
+// diagonalized on the device. Reference: src/core/LanczosWorkspace.cpp:25-40
+// (record_step), :97-108 (prepare_krylov_schur_restart),
+// src/core/RitzExtraction.cpp:195-230,252-268 (snapshot estimates
+// |beta_next y_last| / (1 + |theta|)), :470-492 (restart matrix).
+struct Projection {
+  MatrixXd T;
+  int coupling_row = -1;
+  double beta_next = 0.0;
+  explicit Projection(int m) : T(m, m) {}
+  void record(int it, double beta, double alpha, double bn) {
+    if (it != 0 && it != coupling_row) {
+      T(it, it - 1) = beta;
+      T(it - 1, it) = beta;
+    }
+    T(it, it) = alpha;
+    beta_next = bn;
+    if (it == coupling_row) coupling_row = -1;
+  }
+  // the `count` leading Ritz pairs of T[:m, :m] in `which` order
+  void snapshot(HipExecutor& ex, Workspace& ws, int m, int count, const std::string& which, bool vectors,
+                std::vector<double>& lam, std::vector<double>& rel, MatrixXd& Csel, const char* who) {
+    if (m > kMaxProjected)
+      throw std::runtime_error(std::string(who) + ": projected dimension above 1024 is not supported on the HIP backend.");
+    MatrixXd Tp(ws.cap, ws.cap);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) Tp(i, j) = T(i, j);
+    ws.G.from_host(Tp);
+    const int info = fe_syev(ex.ctx(), m, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap);
+    hip::check(info, "fe_syev");
+    if (info != 0) throw std::runtime_error(std::string(who) + ": projected eigensolve failed.");
+    std::vector<double> w(m);
+    hip::check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * m), "d2h");
+    const MatrixXd V = ws.V.to_host();
+    const std::vector<int> idx = hip::select_indices(w, which);
+    const int cnt = std::min(count, m);
+    lam.assign(cnt, 0.0);
+    rel.assign(cnt, 0.0);
+    if (vectors) Csel = MatrixXd(m, cnt);
+    for (int j = 0; j < cnt; ++j) {
+      const int c = idx[j];
+      lam[j] = w[c];
+      rel[j] = std::fabs(beta_next * V(m - 1, c)) / (1.0 + std::fabs(w[c]));
+      if (vectors)
+        for (int i = 0; i < m; ++i) Csel(i, j) = V(i, c);
+    }
+  }
+  void restart(int retained, const std::vector<double>& lam, const MatrixXd& Csel) {
+    const int m = T.rows();
+    T = MatrixXd(m, m);
+    for (int j = 0; j < retained; ++j) {
+      T(j, j) = lam[j];
+      const double coupling = beta_next * Csel(Csel.rows() - 1, j);
+      T(retained, j) = coupling;
+      T(j, retained) = coupling;
+    }
+    coupling_row = retained;
+  }
+};
+
+void check_lanczos_which(const std::string& which, const char* who) {
+  if (which == "sai")
+    throw std::invalid_argument(std::string(who) +
+                                ": SAI needs a sparse direct solver and is not part of the HIP backend.");
+  if (which != "sr" && which != "lr" && which != "sm" && which != "lm" && which != "be")
+    throw std::invalid_argument(std::string(who) + ": invalid which option (use SR/LR/SM/LM/BE).");
+}
+}  // namespace
+
+LanczosSolver::LanczosSolver(::fasteig_problem& p, const fasteig_options& o) : fasteig_solver(p, o) {}
+KrylovSchurSolver::KrylovSchurSolver(::fasteig_problem& p, const fasteig_options& o) : fasteig_solver(p, o) {}
+
+// Reference: src/LanczosSolver.cpp:28-164.
+FASTEIG_INT LanczosSolver::solve() {
+  options_.validate();
+  const int64_t n = problem_.n();
+  const int nev = options_.nev();
+  const int maxiter = options_.maxiter();
+  const double tol = options_.tol();
+  const std::string which = lower_copy(options_.which());
+  if (maxiter <= 0) throw std::invalid_argument("LanczosSolver: maxiter must be > 0.");
+  if (nev <= 0) throw std::invalid_argument("LanczosSolver: nev must be > 0.");
+  if (maxiter < nev) throw std::invalid_argument("LanczosSolver: maxiter must be >= nev.");
+  if (maxiter > n) throw std::invalid_argument("LanczosSolver: maxiter must be <= n.");
+  check_lanczos_which(which, "LanczosSolver");
+  auto exp = problem_.executor(options_);
+  HipExecutor& ex = *exp;
+  HipCsrMatrix& A = problem_.device_matrix(options_);
+  const unsigned seed = options_.seed().value_or(1u);
+  const int pe = static_cast<int>(options_.print_every());
+  const int check_every = pe > 0 ? pe : 5;
+  const int cap = std::min(maxiter, kMaxProjected);  // basis columns kept
+  DeviceKrylov K(ex, A, options_, cap, seed);
+  Projection P(cap);
+  std::vector<double> lam, rel;
+  MatrixXd Csel;
+  int it_stop = maxiter;
+  bool converged = false;
+  for (int it = 0; it < maxiter; ++it) {
+    if (it >= kMaxProjected)
+      throw std::runtime_error("LanczosSolver: projected dimension above 1024 is not supported on the HIP backend.");
+    const double beta = K.normalize_into(it);
+    const double alpha = K.step(it);
+    P.record(it, beta, alpha, K.norm());
+    const int m = it + 1;
+    const bool do_progress = pe > 0 && m % pe == 0;
+    const bool do_check = m >= nev && (m % check_every == 0 || m == maxiter);
+    if (!do_progress && !do_check) continue;
+    P.snapshot(ex, K.ws(), m, nev, which, false, lam, rel, Csel, "LanczosSolver");
+    progress(options_, m, lam, rel, static_cast<int>(lam.size()), 1.0);
+    if (do_check && !rel.empty() && max_head(rel, static_cast<int>(rel.size())) < tol) {
+      converged = true;
+      it_stop = m;
+      break;
+    }
+  }
+  P.snapshot(ex, K.ws(), it_stop, nev, which, true, lam, rel, Csel, "LanczosSolver");
+  Block X;
+  const std::vector<double> rr = K.ritz(it_stop, Csel, lam, X);
+  const int cnt = static_cast<int>(lam.size());
+  eigvals_.resize(cnt);
+  resnrms_.resize(cnt);
+  for (int j = 0; j < cnt; ++j) {
+    eigvals_(j) = lam[j];
+    resnrms_(j) = rr[j];
+  }
+  (void)converged;
+  problem_.set_device_eigenvectors(std::move(X));
+  status_ = 0;
+  set_iterations(it_stop);
+  return status_;
+}
+
+// Reference: src/KrylovSchurSolver.cpp:28-167.
+FASTEIG_INT KrylovSchurSolver::solve() {
+  options_.validate();
+  const int64_t n = problem_.n();
+  const int nev = options_.nev();
+  const int maxiter = options_.maxiter();
+  const double tol = options_.tol();
+  const std::string which = lower_copy(options_.which());
+  if (n <= 0) throw std::invalid_argument("KrylovSchurSolver: n must be > 0.");
+  if (nev <= 0 || nev > n) throw std::invalid_argument("KrylovSchurSolver: nev must be in [1,n].");
+  if (maxiter <= 0) throw std::invalid_argument("KrylovSchurSolver: maxiter must be > 0.");
+  check_lanczos_which(which, "KrylovSchurSolver");
+  const int64_t want = std::max<int64_t>({2LL * nev, 20LL, static_cast<int64_t>(options_.maxdim())});
+  const int m = static_cast<int>(std::min<int64_t>(want, n));
+  if (m > kMaxProjected)
+    throw std::invalid_argument("KrylovSchurSolver: subspace dimension above 1024 is not supported on the HIP backend.");
+  auto exp = problem_.executor(options_);
+  HipExecutor& ex = *exp;
+  HipCsrMatrix& A = problem_.device_matrix(options_);
+  const unsigned seed = options_.seed().value_or(1u);
+  DeviceKrylov K(ex, A, options_, m, seed);
+  Projection P(m);
+  auto expand = [&](int begin) {
+    for (int col = begin; col < m; ++col) {
+      const double beta = K.normalize_into(col);
+      const double alpha = K.step(col);
+      P.record(col, beta, alpha, K.norm());
+    }
+  };
+  expand(0);
+  for (int iter = 1; iter <= maxiter; ++iter) {
+    std::vector<double> lam, rel;
+    MatrixXd Csel;
+    P.snapshot(ex, K.ws(), m, m, which, true, lam, rel, Csel, "KrylovSchurSolver");
+    int nconv = 0;
+    for (int j = 0; j < nev; ++j)
+      if (rel[j] * (1.0 + std::fabs(lam[j])) < tol * std::fabs(lam[j])) ++nconv;
+    {
+      // progress reports absolute estimates, like krylov_schur_progress_snapshot
+      std::vector<double> ab(nev);
+      for (int j = 0; j < nev; ++j) ab[j] = rel[j] * (1.0 + std::fabs(lam[j]));
+      progress(options_, iter, lam, ab, nev, 1.0);
+    }
+    if (nconv >= nev || m == n || iter == maxiter) {
+      MatrixXd Ch(m, nev);
+      for (int j = 0; j < nev; ++j)
+        for (int i = 0; i < m; ++i) Ch(i, j) = Csel(i, j);
+      std::vector<double> head(lam.begin(), lam.begin() + nev);
+      Block X;
+      const std::vector<double> rr = K.ritz(m, Ch, head, X);
+      eigvals_.resize(nev);
+      resnrms_.resize(nev);
+      for (int j = 0; j < nev; ++j) {
+        eigvals_(j) = head[j];
+        resnrms_(j) = rr[j];
+      }
+      problem_.set_device_eigenvectors(std::move(X));
+      status_ = 0;
+      set_iterations(iter);
+      return status_;
+    }
+    const int spare = (m - nev) / 2;
+    int retained = nev + std::min(nconv, spare);
+    if (retained == 1 && m > 3) retained = m / 2;
+    MatrixXd Ck(m, retained);
+    for (int j = 0; j < retained; ++j)
+      for (int i = 0; i < m; ++i) Ck(i, j) = Csel(i, j);
+    K.rotate(m, Ck);
+    P.restart(retained, lam, Csel);
+    expand(retained);
+  }
+  throw std::logic_error("KrylovSchurSolver: unreachable solve termination.");
+}
+
 std::unique_ptr<fasteig_solver> make_fasteig_solver(const std::string& solver, ::fasteig_problem& problem,
                                                     const fasteig_options& options) {
+  if (has_prefix(solver, "Lanczos", 3)) return std::make_unique<LanczosSolver>(problem, options);
   if (has_prefix(solver, "BlockKrylovSchur", 11)) return std::make_unique<BlockKrylovSchurSolver>(problem, options);
+  if (has_prefix(solver, "KrylovSchur", 7)) return std::make_unique<KrylovSchurSolver>(problem, options);
   if (has_prefix(solver, "Davidson", 3)) return std::make_unique<DavidsonSolver>(problem, options);
   if (has_prefix(solver, "LOBPCG", 3)) return std::make_unique<LOBPCGSolver>(problem, options);
   if (has_prefix(solver, "Chebyshev", 3)) return std::make_unique<ChebyshevSolver>(problem, options);
   if (has_prefix(solver, "auto", 3)) return nullptr;
   throw std::invalid_argument("solver '" + solver +
-                              "' is not available on the HIP backend (Chebyshev, LOBPCG, Davidson, BlockKrylovSchur)");
+                              "' is not available on the HIP backend (Chebyshev, LOBPCG, Davidson, Lanczos, "
+                              "KrylovSchur, BlockKrylovSchur)");
 }
 
 }  // namespace fasteig

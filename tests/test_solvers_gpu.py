@@ -158,6 +158,70 @@ def test_block_krylov_schur_rejects_bad_dimensions():
         p.solve(solver="BlockKrylovSchur", nev=4, blksize=4, which="SAI")
 
 
+@pytest.mark.parametrize("solver", ["Lanczos", "KrylovSchur"])
+def test_sio_golden_scalar_krylov(solver):
+    """reference_gpu_SiO_{Lanczos,KrylovSchur}_SR: k=5, tol 1e-8, maxiter
+    min(2000, n) for Lanczos, maxdim 40 for Krylov-Schur (the reference
+    harness defaults, tests/scripts/run_case.py:21-33,225-241)."""
+    A = load_sio()
+    ref = np.array(json.load(open(os.path.join(GOLD, "sio_eigs.json")))["eigenvalues"][:5])
+    extra = {"maxiter": min(2000, A.shape[0])} if solver == "Lanczos" else {"maxiter": 2000, "maxdim": 40}
+    r = Problem(A).solve(solver=solver, nev=5, tol=1e-8, seed=1, which="SR", **extra)
+    assert r.status == 0
+    assert np.all(np.abs(r.eigenvalues - ref) <= 1e-10 + 1e-10 * np.abs(ref)), r.eigenvalues - ref
+    assert r.residual_norms.max() <= 1e-8
+
+
+@pytest.mark.parametrize("which", ["SR", "LR", "BE"])
+@pytest.mark.parametrize("solver", ["Lanczos", "KrylovSchur"])
+def test_scalar_krylov_matches_oracle_and_closed_form(solver, which):
+    """30 x 17 Laplacian (simple spectrum): the closed form, the oracle's
+    values, and the oracle's step / restart count (same seeded recurrence)."""
+    A = G.laplacian_2d(30, 17)
+    ex = np.sort(G.laplacian_2d_eigenvalues(30, 17))
+    want = {"SR": ex[:6], "LR": ex[::-1][:6], "BE": ex[[-1, 0, -2, 1, -3, 2]]}[which]
+    p = Problem(A)
+    if solver == "Lanczos":
+        r = p.solve(solver=solver, nev=6, maxiter=300, tol=1e-9, which=which, seed=1)
+        o = orc.lanczos(A, 6, maxiter=300, tol=1e-9, which=which.lower(), seed=1)
+    else:
+        r = p.solve(solver=solver, nev=6, maxdim=30, maxiter=300, tol=1e-9, which=which, seed=1)
+        o = orc.krylov_schur(A, 6, maxdim=30, maxiter=300, tol=1e-9, which=which.lower(), seed=1)
+    assert r.status == 0
+    assert np.all(np.abs(r.eigenvalues - want) <= 1e-9 * np.abs(want))
+    assert np.all(np.abs(r.eigenvalues - o["eigenvalues"]) <= 1e-10 * np.abs(want))
+    assert r.iterations == o["iterations"]
+    assert r.residual_norms.max() <= 1e-8
+    X = p.eigenvectors()
+    assert np.max(np.abs(X.T @ X - np.eye(6))) <= 1e-8
+
+
+def test_scalar_krylov_invariant_subspace_restart():
+    """diag(1 x 10, 2 x 10, 3 x 10): the scalar chain breaks down after three
+    steps and continues from seeded random vectors."""
+    A = sp.diags(np.r_[np.ones(10), 2.0 * np.ones(10), 3.0 * np.ones(10)]).tocsr()
+    r = Problem(A).solve(solver="KrylovSchur", nev=4, maxdim=12, maxiter=200, tol=1e-10, which="SR", seed=2)
+    assert r.status == 0 and np.all(np.abs(r.eigenvalues - 1.0) <= 1e-12)
+    # where the broken chain stops depends on rounding-level residual
+    # estimates (the oracle stops at 10 steps); the pairs it returns are exact
+    r = Problem(A).solve(solver="Lanczos", nev=4, maxiter=30, tol=1e-10, which="SR", seed=2)
+    assert r.status == 0 and r.iterations % 5 == 0
+    ev = np.asarray(r.eigenvalues)
+    assert np.all(np.abs(ev[:3] - 1.0) <= 1e-12)
+    assert np.all(np.minimum(np.abs(ev - 1.0), np.abs(ev - 2.0)) <= 1e-12)
+    assert r.residual_norms.max() <= 1e-10
+
+
+def test_scalar_krylov_rejects_bad_arguments():
+    p = Problem(G.laplacian_2d(6))
+    with pytest.raises(ValueError, match="maxiter must be >= nev"):
+        p.solve(solver="Lanczos", nev=4, maxiter=3, which="SR")
+    with pytest.raises(ValueError, match="maxiter must be <= n"):
+        p.solve(solver="Lanczos", nev=4, maxiter=37, which="SR")
+    with pytest.raises(ValueError, match="SAI"):
+        p.solve(solver="KrylovSchur", nev=4, which="SAI")
+
+
 def test_generalized_diagonal_mass_matches_dense_pencil():
     K, M = G.conductivity_3d(10, seed=2, mass_seed=3)
     nev = 6
