@@ -15,7 +15,7 @@
 
 namespace fe {
 
-constexpr int SYEV_MAX = 1024;  // largest projected matrix handled on the device
+constexpr int SYEV_MAX = 2048;  // largest projected matrix handled on the device
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
@@ -534,7 +534,7 @@ using namespace fe;
 
 extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, double* V, int64_t ldv) {
   FE_ARG(ctx && A && W && V && m >= 0 && lda >= m && ldv >= m, "fe_syev: bad argument");
-  FE_ARG(m <= SYEV_MAX, "fe_syev: matrix larger than 1024 is not supported");
+  FE_ARG(m <= SYEV_MAX, "fe_syev: matrix larger than 2048 is not supported");
   if (m == 0) return 0;
   const int64_t MM = (int64_t)m * m;
   void* ws = nullptr;
