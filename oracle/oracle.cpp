@@ -26,7 +26,7 @@
 //                       2D Laplacian closed form, SiO spectrum from the
 //                       reference's tests/reference/SiO.out.121mpi (golden
 //                       fixture), dense eigh on small random problems.
-//   orc_lanczos / orc_krylov_schur
+//   orc_lanczos / orc_krylov_schur / orc_block_lanczos
 //                       2D Laplacian closed form (SR, LR, BE ends), SiO golden
 //                       spectrum, dense eigh on small random problems, a
 //                       matrix with a repeated eigenvalue (invariant-subspace
@@ -1183,6 +1183,103 @@ Result block_krylov_schur(const Csr& A, int nev, int bsz, int maxdim, int maxite
   return res;
 }
 
+// Block Lanczos without restart (standard problems). Reference:
+// src/BlockLanczosSolver.cpp:24-172: Q_0 from the rank-aware QR of the start
+// block; block iter expands Q_{iter-1} (two-pass block Gram-Schmidt, block QR
+// of the residual into Q_iter), then checks the block Ritz estimates of the
+// nev wanted pairs once iter * bsz >= nev; at most maxiter blocks (and the
+// whole space).
+Result block_lanczos(const Csr& A, int nev, int bsz, int maxiter, double tol, const std::string& which,
+                     unsigned seed) {
+  const int n = A.n;
+  if (n <= 0) throw std::invalid_argument("BlockLanczosSolver: n must be > 0.");
+  if (nev <= 0) throw std::invalid_argument("BlockLanczosSolver: nev must be > 0.");
+  if (maxiter <= 0) throw std::invalid_argument("BlockLanczosSolver: maxiter must be > 0.");
+  if (bsz <= 0) throw std::invalid_argument("BlockLanczosSolver: blksize must be > 0.");
+  if (bsz > n || nev > n) throw std::invalid_argument("BlockLanczosSolver: blksize and nev must be <= n.");
+  if (static_cast<long>(maxiter) * bsz < nev) throw std::invalid_argument("BlockLanczosSolver: need maxiter*blksize >= nev.");
+  if (which != "sr" && which != "lr" && which != "sm" && which != "lm" && which != "be")
+    throw std::invalid_argument("BlockLanczosSolver: invalid which option (use SR/LR/SM/LM/BE).");
+  const int blocks = std::min(maxiter, 1 + (n - 1) / bsz);
+  const int pdim = (blocks + 1) * bsz;
+  Mat Qb(n, pdim), T(pdim, pdim), beta(bsz, bsz);
+  {
+    Mat Q0, R0;
+    complete_block_qr(Qb, 0, random_block(n, bsz, seed), Q0, R0, seed, 0.0);
+    copy_cols(Qb, 0, Q0, 0, bsz);
+  }
+  auto snapshot = [&](int m, Vec& lam, Vec& rr, Mat& Csel) {
+    Mat P(m, m);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) P(i, j) = T(i, j);
+    Vec w;
+    Mat V;
+    if (syev(P, w, V) != 0) throw std::runtime_error("BlockLanczosSolver: projected eigensolve failed.");
+    const auto idx = sort_indices(w, which);
+    const int cnt = std::min(nev, m), tail = 1 + (m - 1) % bsz;
+    lam.assign(cnt, 0.0);
+    rr.assign(cnt, 0.0);
+    Csel = Mat(m, cnt);
+    for (int j = 0; j < cnt; ++j) {
+      const int c = idx[j];
+      lam[j] = w[c];
+      double a = 0.0;
+      for (int i = 0; i < bsz; ++i) {
+        double s = 0.0;
+        for (int t = 0; t < tail; ++t) s += beta(i, t) * V(m - tail + t, c);
+        a += s * s;
+      }
+      rr[j] = std::sqrt(a) / (1.0 + std::fabs(w[c]));
+      std::copy(V.col(c), V.col(c) + m, Csel.col(j));
+    }
+  };
+  Vec lam, rr;
+  Mat Csel;
+  int it_stop = blocks;
+  for (int iter = 1; iter <= blocks; ++iter) {
+    const int blk = iter - 1;
+    Mat Z(n, bsz);
+    spmm(A, Qb, blk * bsz, bsz, Z, 1.0, 0.0);
+    const int active = std::min(n, iter * bsz);
+    Mat Rz, H;
+    block_gram_schmidt(Qb, active, Z, Rz, H);
+    const int width = 1 + (active - 1) % bsz;
+    Mat D(bsz, bsz);
+    for (int j = 0; j < width; ++j)
+      for (int i = 0; i < width; ++i) D(i, j) = H(active - width + i, j);
+    double scale = 0.0;
+    for (double x : column_norms(Z, bsz)) scale += x * x;
+    scale = std::sqrt(scale);
+    Mat Qn, B;
+    complete_block_qr(Qb, active, Rz, Qn, B, seed + static_cast<unsigned>(iter * bsz), scale);
+    copy_cols(Qb, iter * bsz, Qn, 0, bsz);
+    const int row = blk * bsz, nrow = row + bsz;
+    for (int j = 0; j < bsz; ++j)
+      for (int i = 0; i < bsz; ++i) {
+        T(row + i, row + j) = D(i, j);
+        T(nrow + i, row + j) = B(i, j);
+        T(row + j, nrow + i) = B(i, j);
+      }
+    beta = B;
+    snapshot(active, lam, rr, Csel);
+    double mx = 0.0;
+    for (double x : rr) mx = std::max(mx, x);
+    if (static_cast<long>(iter) * bsz >= nev && mx < tol) {
+      it_stop = iter;
+      break;
+    }
+  }
+  const int m = std::min(n, it_stop * bsz);
+  snapshot(m, lam, rr, Csel);
+  Result res;
+  res.evals = lam;
+  res.evecs = AC(Qb, m, Csel);
+  res.relres = residual_norms(A, res.evals, res.evecs);
+  res.iterations = it_stop;
+  res.status = 0;
+  return res;
+}
+
 // Scalar Krylov basis with full two-pass reorthogonalization. Reference:
 // src/core/KrylovSubspace.cpp:124-260 (normalize_residual_into_basis with the
 // random restart of normalize_or_restart / restart_residual,
@@ -1585,6 +1682,15 @@ int orc_lanczos(int n, const int* rp, const int* ci, const double* v, int nev, i
                 const char* which, unsigned seed, double* ev, double* evec, double* res, long* iters) {
   return guarded([&] {
     auto r = orc::lanczos(csr(n, rp, ci, v), nev, maxiter, tol, which, seed);
+    return finish(r, nev, n, ev, evec, res, iters);
+  });
+}
+
+int orc_block_lanczos(int n, const int* rp, const int* ci, const double* v, int nev, int block, int maxiter,
+                      double tol, const char* which, unsigned seed, double* ev, double* evec, double* res,
+                      long* iters) {
+  return guarded([&] {
+    auto r = orc::block_lanczos(csr(n, rp, ci, v), nev, block, maxiter, tol, which, seed);
     return finish(r, nev, n, ev, evec, res, iters);
   });
 }

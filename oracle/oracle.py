@@ -54,6 +54,7 @@ def lib():
             solver_tail
         L.orc_lanczos.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int] + solver_tail
         L.orc_krylov_schur.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
+        L.orc_block_lanczos.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_residual_norms.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_max_threads.restype = C.c_int
@@ -213,6 +214,11 @@ def lanczos(A, nev, maxiter=100, tol=1e-8, which="sr", seed=1, want_vectors=True
 def krylov_schur(A, nev, maxdim=0, maxiter=1000, tol=1e-8, which="sr", seed=1, want_vectors=True):
     """Scalar Krylov-Schur with thick restart; subspace max(2 nev, 20, maxdim)."""
     return _solve(lib().orc_krylov_schur, A, nev, (maxdim,), maxiter, tol, which, seed, want_vectors)
+
+
+def block_lanczos(A, nev, block=10, maxiter=100, tol=1e-8, which="sr", seed=1, want_vectors=True):
+    """Block Lanczos without restart; at most maxiter blocks."""
+    return _solve(lib().orc_block_lanczos, A, nev, (block,), maxiter, tol, which, seed, want_vectors)
 
 
 def residual_norms(A, evals, X):

@@ -306,3 +306,36 @@ def test_scalar_krylov_rejects_bad_arguments():
         orc.lanczos(A, 4, maxiter=37)
     with pytest.raises(ValueError, match="invalid which"):
         orc.krylov_schur(A, 4, which="xx")
+
+
+@pytest.mark.parametrize("which", ["sr", "lr", "be"])
+def test_block_lanczos_closed_form(which):
+    A = G.laplacian_2d(30, 17)
+    ex = np.sort(G.laplacian_2d_eigenvalues(30, 17))
+    want = {"sr": ex[:6], "lr": ex[::-1][:6], "be": ex[[-1, 0, -2, 1, -3, 2]]}[which]
+    r = orc.block_lanczos(A, 6, block=4, maxiter=200, tol=1e-9, which=which, seed=1)
+    assert r["status"] == 0
+    assert np.all(np.abs(r["eigenvalues"] - want) <= 1e-9 * np.abs(want))
+    assert r["residuals"].max() <= 1e-8
+
+
+def test_block_lanczos_full_space_is_exact():
+    """35 = 8 blocks of 4 + a partial block: after 9 blocks the basis spans the
+    space and the Ritz values are the eigenvalues."""
+    A = G.laplacian_2d(7, 5)
+    w = np.linalg.eigvalsh(A.toarray())
+    r = orc.block_lanczos(A, 5, block=4, maxiter=100, tol=1e-300, which="sr", seed=1)
+    assert r["iterations"] == 9
+    assert np.all(np.abs(r["eigenvalues"] - w[:5]) <= 1e-12 * np.abs(w).max())
+
+
+def test_block_lanczos_narrow_block_misses_multiplicity():
+    """diag(1 x 10, 2 x 10, 3 x 10) with a 3-column block: the block Krylov
+    space closes after three blocks holding three copies of each eigenvalue,
+    so the converged answer is (1, 1, 1, 2) -- the limitation the reference
+    documents for blocks narrower than nev (src/BlockLanczosSolver.cpp:129-131)."""
+    A = sp.diags(np.r_[np.ones(10), 2.0 * np.ones(10), 3.0 * np.ones(10)]).tocsr()
+    r = orc.block_lanczos(A, 4, block=3, maxiter=50, tol=1e-10, which="sr", seed=2)
+    assert r["iterations"] == 3
+    assert np.allclose(r["eigenvalues"], [1.0, 1.0, 1.0, 2.0], atol=1e-12)
+    assert r["residuals"].max() <= 1e-12
