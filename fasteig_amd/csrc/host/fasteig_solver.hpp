@@ -52,16 +52,32 @@ class ChebyshevSolver final : public fasteig_solver {
 // keeping max(ceil(nev/blksize), blocks/2) blocks of Ritz vectors. Standard
 // problems, which = SR, LR, SM, LM, BE (shift-invert needs a sparse direct
 // solver and is not part of the HIP backend).
-class BlockKrylovSchurSolver final : public fasteig_solver {
+class BlockKrylovBase : public fasteig_solver {
+ protected:
+  using fasteig_solver::fasteig_solver;
+  // restarted: block Krylov-Schur; otherwise block Lanczos
+  FASTEIG_INT run(bool restarted);
+};
+class BlockKrylovSchurSolver final : public BlockKrylovBase {
  public:
   BlockKrylovSchurSolver(::fasteig_problem& problem, const fasteig_options& options);
+  FASTEIG_INT solve() override;
+};
+
+// Block Lanczos without restart (src/BlockLanczosSolver.cpp:24-172): the same
+// block expansion and block QR as block Krylov-Schur, a convergence check
+// after every block once iter * blksize >= nev, at most maxiter blocks (the
+// projected dimension at most 2048 here).
+class BlockLanczosSolver final : public BlockKrylovBase {
+ public:
+  BlockLanczosSolver(::fasteig_problem& problem, const fasteig_options& options);
   FASTEIG_INT solve() override;
 };
 
 // Scalar Lanczos with full two-pass reorthogonalization and no restart
 // (src/LanczosSolver.cpp:28-164): convergence checked every print_every steps
 // (5 when 0) once m >= nev. Standard problems, which = SR, LR, SM, LM, BE; the
-// projected dimension is limited to 1024 on the HIP backend.
+// projected dimension is limited to 2048 on the HIP backend.
 class LanczosSolver final : public fasteig_solver {
  public:
   LanczosSolver(::fasteig_problem& problem, const fasteig_options& options);
@@ -69,7 +85,7 @@ class LanczosSolver final : public fasteig_solver {
 };
 
 // Scalar Krylov-Schur with thick restart (src/KrylovSchurSolver.cpp:28-167):
-// subspace max(2 nev, 20, maxdim) (at most 1024 here), restart keeping
+// subspace max(2 nev, 20, maxdim) (at most 2048 here), restart keeping
 // nev + min(nconv, (m - nev) / 2) Ritz vectors.
 class KrylovSchurSolver final : public fasteig_solver {
  public:

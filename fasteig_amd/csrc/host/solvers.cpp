@@ -20,6 +20,7 @@ using hip::HipExecutor;
 using hip::Workspace;
 
 namespace {
+constexpr int kMaxProjected = 2048;  // largest projected matrix fe_syev takes
 std::string lower_copy(std::string s) {
   std::transform(s.begin(), s.end(), s.begin(), [](unsigned char c) { return static_cast<char>(std::tolower(c)); });
   return s;
@@ -481,33 +482,62 @@ FASTEIG_INT DavidsonSolver::solve() {
 // src/core/Orthogonalization.cpp:609-715 (complete_block_qr),
 // src/core/LanczosWorkspace.cpp:42-73,109-135, src/core/RitzExtraction.cpp:34-63,272-380.
 BlockKrylovSchurSolver::BlockKrylovSchurSolver(::fasteig_problem& problem, const fasteig_options& options)
-    : fasteig_solver(problem, options) {}
+    : BlockKrylovBase(problem, options) {}
+BlockLanczosSolver::BlockLanczosSolver(::fasteig_problem& problem, const fasteig_options& options)
+    : BlockKrylovBase(problem, options) {}
+FASTEIG_INT BlockKrylovSchurSolver::solve() { return run(true); }
+FASTEIG_INT BlockLanczosSolver::solve() { return run(false); }
 
-FASTEIG_INT BlockKrylovSchurSolver::solve() {
+// Block Krylov-Schur (restarted) and block Lanczos (not restarted) share the
+// basis expansion, the rank-aware block QR and the block Ritz estimates.
+FASTEIG_INT BlockKrylovBase::run(bool restarted) {
   options_.validate();
   const int64_t n = problem_.n();
   const int nev = options_.nev();
   const int maxiter = options_.maxiter();
   const double tol = options_.tol();
   const std::string which = lower_copy(options_.which());
-  if (n <= 0) throw std::invalid_argument("BlockKrylovSchurSolver: n must be > 0.");
-  if (nev <= 0 || nev > n) throw std::invalid_argument("BlockKrylovSchurSolver: nev must be in [1,n].");
-  if (maxiter <= 0) throw std::invalid_argument("BlockKrylovSchurSolver: maxiter must be > 0.");
-  if (which != "sr" && which != "lr" && which != "sm" && which != "lm" && which != "be")
-    throw std::invalid_argument("BlockKrylovSchurSolver: which must be SR, LR, SM, LM or BE on the HIP backend.");
+  const std::string who = restarted ? "BlockKrylovSchurSolver" : "BlockLanczosSolver";
+  if (n <= 0) throw std::invalid_argument(who + ": n must be > 0.");
   const int b = static_cast<int>(options_.blksize());
-  if (b <= 0 || b > n) throw std::invalid_argument("BlockKrylovSchurSolver: blksize must be in [1,n].");
-  const int64_t requested_dim = options_.maxdim() > 0 ? options_.maxdim() : std::max<int64_t>(2 * nev, 20);
-  const int block_count = static_cast<int>(requested_dim >= n ? 1 + (n - 1) / b : requested_dim / b);
-  const int cycle_dim = static_cast<int>(std::min<int64_t>(n, static_cast<int64_t>(block_count) * b));
-  const int wanted_blocks = 1 + (nev - 1) / b;
-  if (cycle_dim < n && block_count <= wanted_blocks)
-    throw std::invalid_argument("BlockKrylovSchurSolver: maxdim must accommodate ceil(nev/blksize) retained blocks "
-                                "plus at least one expansion block (or the full matrix dimension).");
+  int block_count = 0, cycle_dim = 0, retained_blocks = 0, snap_count = nev;
+  if (restarted) {
+    if (nev <= 0 || nev > n) throw std::invalid_argument("BlockKrylovSchurSolver: nev must be in [1,n].");
+    if (maxiter <= 0) throw std::invalid_argument("BlockKrylovSchurSolver: maxiter must be > 0.");
+    if (which != "sr" && which != "lr" && which != "sm" && which != "lm" && which != "be")
+      throw std::invalid_argument("BlockKrylovSchurSolver: which must be SR, LR, SM, LM or BE on the HIP backend.");
+    if (b <= 0 || b > n) throw std::invalid_argument("BlockKrylovSchurSolver: blksize must be in [1,n].");
+    const int64_t requested_dim = options_.maxdim() > 0 ? options_.maxdim() : std::max<int64_t>(2 * nev, 20);
+    block_count = static_cast<int>(requested_dim >= n ? 1 + (n - 1) / b : requested_dim / b);
+    cycle_dim = static_cast<int>(std::min<int64_t>(n, static_cast<int64_t>(block_count) * b));
+    const int wanted_blocks = 1 + (nev - 1) / b;
+    if (cycle_dim < n && block_count <= wanted_blocks)
+      throw std::invalid_argument("BlockKrylovSchurSolver: maxdim must accommodate ceil(nev/blksize) retained blocks "
+                                  "plus at least one expansion block (or the full matrix dimension).");
+    retained_blocks = cycle_dim == n ? 0 : std::max(wanted_blocks, block_count / 2);
+    snap_count = std::max(nev, retained_blocks * b);
+  } else {
+    // src/BlockLanczosSolver.cpp:33-49
+    if (nev <= 0) throw std::invalid_argument("BlockLanczosSolver: nev must be > 0.");
+    if (maxiter <= 0) throw std::invalid_argument("BlockLanczosSolver: maxiter must be > 0.");
+    if (b <= 0) throw std::invalid_argument("BlockLanczosSolver: blksize must be > 0.");
+    if (b > n || nev > n) throw std::invalid_argument("BlockLanczosSolver: blksize and nev must be <= n.");
+    if (static_cast<int64_t>(maxiter) * b < nev)
+      throw std::invalid_argument("BlockLanczosSolver: need maxiter*blksize >= nev.");
+    if (which == "sai")
+      throw std::invalid_argument("BlockLanczosSolver: SAI needs a sparse direct solver and is not part of the HIP "
+                                  "backend.");
+    if (which != "sr" && which != "lr" && which != "sm" && which != "lm" && which != "be")
+      throw std::invalid_argument("BlockLanczosSolver: invalid which option (use SR/LR/SM/LM/BE).");
+    // blocks that can be held: the whole space, maxiter blocks, or the
+    // largest projected matrix the device eigensolver takes (2048)
+    const int64_t span_blocks = 1 + (n - 1) / b;
+    block_count = static_cast<int>(std::min<int64_t>({static_cast<int64_t>(maxiter), span_blocks,
+                                                      static_cast<int64_t>(kMaxProjected / b)}));
+    if (block_count < 1) throw std::invalid_argument("BlockLanczosSolver: blksize above 2048 is not supported.");
+    cycle_dim = static_cast<int>(std::min<int64_t>(n, static_cast<int64_t>(block_count) * b));
+  }
   const int pdim = (block_count + 1) * b;
-  const int retained_blocks = cycle_dim == n ? 0 : std::max(wanted_blocks, block_count / 2);
-  const int retained_dim = retained_blocks * b;
-  const int snap_count = std::max(nev, retained_dim);
   const unsigned seed = options_.seed().value_or(1u);
 
   auto exp = problem_.executor(options_);
@@ -673,62 +703,57 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
     hip::copy(ex, Qn.ref(), Qb.cols(0, b));
   }
 
-  auto expand = [&](int first) {
-    for (int blk = first; blk < block_count; ++blk) {
-      hip::spmm(ex, A, Qb.cols(blk * b, b), Z.ref(), 1.0);
-      const int active = static_cast<int>(std::min<int64_t>(n, static_cast<int64_t>(blk + 1) * b));
-      double scale = 0.0;
-      for (double c : hip::column_norms(ex, Z.ref())) scale += c * c;
-      scale = std::sqrt(scale);
-      // two-pass block Gram-Schmidt keeping the coefficients H = H1 + H2
-      hip::copy(ex, Z.ref(), Rz.ref());
-      const BlockRef Vb = Qb.cols(0, active);
-      hip::gram(ex, FE_GRAM_NATIVE, Vb, Rz.ref(), ws.G.data(), ws.cap);
-      hip::gemm(ex, Vb, ws.G.data(), ws.cap, Rz.ref(), -1.0, 1.0);
-      hip::gram(ex, FE_GRAM_NATIVE, Vb, Rz.ref(), ws.R.data(), ws.cap);
-      hip::gemm(ex, Vb, ws.R.data(), ws.cap, Rz.ref(), -1.0, 1.0);
-      const MatrixXd H1 = host_small(ws.G, active, b), H2 = host_small(ws.R, active, b);
-      const int width = 1 + (active - 1) % b;
-      MatrixXd D(b, b);
-      for (int j = 0; j < width; ++j)
-        for (int i = 0; i < width; ++i) D(i, j) = H1(active - width + i, j) + H2(active - width + i, j);
-      MatrixXd B;
-      const int nf = static_cast<int>(std::min<int64_t>(static_cast<int64_t>(blk + 1) * b, n));
-      complete_qr(nf, Rz.ref(), B, seed + static_cast<unsigned>((blk + 1) * b), scale);
-      hip::copy(ex, Qn.ref(), Qb.cols((blk + 1) * b, b));
-      const int row = blk * b, nrow = row + b;
-      for (int j = 0; j < b; ++j)
-        for (int i = 0; i < b; ++i) {
-          T(row + i, row + j) = D(i, j);
-          T(nrow + i, row + j) = B(i, j);
-          T(row + j, nrow + i) = B(i, j);
-        }
-      beta = B;
-    }
+  auto expand_block = [&](int blk) {
+    hip::spmm(ex, A, Qb.cols(blk * b, b), Z.ref(), 1.0);
+    const int active = static_cast<int>(std::min<int64_t>(n, static_cast<int64_t>(blk + 1) * b));
+    double scale = 0.0;
+    for (double c : hip::column_norms(ex, Z.ref())) scale += c * c;
+    scale = std::sqrt(scale);
+    // two-pass block Gram-Schmidt keeping the coefficients H = H1 + H2
+    hip::copy(ex, Z.ref(), Rz.ref());
+    const BlockRef Vb = Qb.cols(0, active);
+    hip::gram(ex, FE_GRAM_NATIVE, Vb, Rz.ref(), ws.G.data(), ws.cap);
+    hip::gemm(ex, Vb, ws.G.data(), ws.cap, Rz.ref(), -1.0, 1.0);
+    hip::gram(ex, FE_GRAM_NATIVE, Vb, Rz.ref(), ws.R.data(), ws.cap);
+    hip::gemm(ex, Vb, ws.R.data(), ws.cap, Rz.ref(), -1.0, 1.0);
+    const MatrixXd H1 = host_small(ws.G, active, b), H2 = host_small(ws.R, active, b);
+    const int width = 1 + (active - 1) % b;
+    MatrixXd D(b, b);
+    for (int j = 0; j < width; ++j)
+      for (int i = 0; i < width; ++i) D(i, j) = H1(active - width + i, j) + H2(active - width + i, j);
+    MatrixXd B;
+    const int nf = static_cast<int>(std::min<int64_t>(static_cast<int64_t>(blk + 1) * b, n));
+    complete_qr(nf, Rz.ref(), B, seed + static_cast<unsigned>((blk + 1) * b), scale);
+    hip::copy(ex, Qn.ref(), Qb.cols((blk + 1) * b, b));
+    const int row = blk * b, nrow = row + b;
+    for (int j = 0; j < b; ++j)
+      for (int i = 0; i < b; ++i) {
+        T(row + i, row + j) = D(i, j);
+        T(nrow + i, row + j) = B(i, j);
+        T(row + j, nrow + i) = B(i, j);
+      }
+    beta = B;
   };
-  expand(0);
-
-  const int m = cycle_dim;
-  std::vector<double> lam, rr;
-  MatrixXd Csel;
-  long long it_done = 0;
-  bool converged = false;
+  auto expand = [&](int first) {
+    for (int blk = first; blk < block_count; ++blk) expand_block(blk);
+  };
   Block X(ex, n, nev, FE_F64), AX(ex, n, nev, FE_F64), Rr(ex, n, nev, FE_F64);
-  Block U(ex, n, std::min(snap_count, m), FE_F64);
-  for (int iter = 1; iter <= maxiter; ++iter) {
-    // block Ritz snapshot of T[:m, :m] (RitzExtraction.cpp:272-380)
+  // block Ritz snapshot of T[:m, :m]: the `count` leading pairs in `which`
+  // order with estimates ||beta y_last|| / (1 + |theta|)
+  // (src/core/RitzExtraction.cpp:272-380)
+  auto snapshot = [&](int m, int count, std::vector<double>& lam, std::vector<double>& rr, MatrixXd& Csel) {
     MatrixXd Tp(ws.cap, ws.cap);
     for (int j = 0; j < m; ++j)
       for (int i = 0; i < m; ++i) Tp(i, j) = 0.5 * (T(i, j) + T(j, i));
     ws.G.from_host(Tp);
     const int info = fe_syev(ex.ctx(), m, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap);
-    hip::check(info, "fe_syev(block Krylov-Schur)");
-    if (info != 0) throw std::runtime_error("BlockKrylovSchurSolver: projected eigensolve failed.");
+    hip::check(info, "fe_syev(block Krylov)");
+    if (info != 0) throw std::runtime_error(who + ": projected eigensolve failed.");
     std::vector<double> w(m);
     hip::check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * m), "d2h");
     const MatrixXd Vh = host_small(ws.V, m, m);
     const std::vector<int> idx = hip::select_indices(w, which);
-    const int cnt = std::min(snap_count, m);
+    const int cnt = std::min(count, m);
     const int tail = 1 + (m - 1) % b;
     lam.assign(cnt, 0.0);
     rr.assign(cnt, 0.0);
@@ -745,34 +770,94 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
       rr[j] = std::sqrt(a2) / (1.0 + std::fabs(w[c]));
       for (int i = 0; i < m; ++i) Csel(i, j) = Vh(i, c);
     }
+  };
+  // Ritz vectors X = Q[:, :m] C[:, :nev] and true residuals into rr
+  // (src/core/RitzExtraction.cpp:34-63); true when all meet tol
+  auto finish = [&](int m, const MatrixXd& Csel, const std::vector<double>& lam, std::vector<double>& rr) {
+    hip::Small Cs(ex, m, nev), th(ex, nev, 1), rn(ex, nev, 1), yn(ex, nev, 1);
+    MatrixXd Ch(m, nev);
+    for (int j = 0; j < nev; ++j)
+      for (int i = 0; i < m; ++i) Ch(i, j) = Csel(i, j);
+    Cs.from_host(Ch);
+    hip::gemm(ex, Qb.cols(0, m), Cs.data(), m, X.ref(), 1.0, 0.0);
+    hip::spmm(ex, A, X.ref(), AX.ref(), 1.0);
+    hip::check(fe_memcpy_h2d(ex.ctx(), th.data(), lam.data(), sizeof(double) * nev), "h2d");
+    hip::check(fe_ritz_residuals(ex.ctx(), FE_F64, n, nev, X.data(), X.ld(), AX.data(), AX.ld(), th.data(),
+                                 Rr.data(), Rr.ld(), rn.data(), yn.data()),
+               "fe_ritz_residuals");
+    std::vector<double> rh(nev), yh(nev);
+    hip::check(fe_memcpy_d2h(ex.ctx(), rh.data(), rn.data(), sizeof(double) * nev), "d2h");
+    hip::check(fe_memcpy_d2h(ex.ctx(), yh.data(), yn.data(), sizeof(double) * nev), "d2h");
+    bool ok = true;
+    rr.assign(nev, 0.0);
+    for (int j = 0; j < nev; ++j) {
+      const double den = (1.0 + std::fabs(lam[j])) * yh[j];
+      if (!(den > 0.0) || !std::isfinite(rh[j])) throw std::runtime_error(who + ": invalid vector or non-finite residual.");
+      rr[j] = rh[j] / den;
+      ok = ok && rr[j] <= tol;
+    }
+    return ok;
+  };
+  auto publish = [&](const std::vector<double>& lam, const std::vector<double>& rr, long long iterations,
+                     int status) {
+    eigvals_.resize(nev);
+    resnrms_.resize(nev);
+    for (int j = 0; j < nev; ++j) {
+      eigvals_(j) = lam[j];
+      resnrms_(j) = rr[j];
+    }
+    problem_.set_device_eigenvectors(std::move(X));
+    status_ = status;
+    set_iterations(iterations);
+    return status_;
+  };
+
+  if (!restarted) {
+    // block Lanczos: one block per iteration, checked once iter * b >= nev
+    // (src/BlockLanczosSolver.cpp:95-170)
+    const int64_t span_blocks = 1 + (n - 1) / b;
+    const bool device_limited = block_count < maxiter && block_count < span_blocks;
+    std::vector<double> lam, rr;
+    MatrixXd Csel;
+    int it_stop = block_count;
+    bool converged = false;
+    for (int iter = 1; iter <= block_count; ++iter) {
+      expand_block(iter - 1);
+      const int m = static_cast<int>(std::min<int64_t>(static_cast<int64_t>(iter) * b, n));
+      snapshot(m, nev, lam, rr, Csel);
+      progress(options_, iter, lam, rr, static_cast<int>(lam.size()), 1.0);
+      if (static_cast<int64_t>(iter) * b >= nev && max_head(rr, static_cast<int>(rr.size())) < tol) {
+        converged = true;
+        it_stop = iter;
+        break;
+      }
+    }
+    if (!converged && device_limited)
+      throw std::runtime_error("BlockLanczosSolver: projected dimension above 2048 is not supported on the HIP "
+                               "backend.");
+    const int m = static_cast<int>(std::min<int64_t>(static_cast<int64_t>(it_stop) * b, n));
+    snapshot(m, nev, lam, rr, Csel);
+    if (static_cast<int>(lam.size()) < nev) throw std::runtime_error("BlockLanczosSolver: subspace smaller than nev.");
+    finish(m, Csel, lam, rr);
+    return publish(lam, rr, it_stop, 0);
+  }
+
+  expand(0);
+  const int m = cycle_dim;
+  std::vector<double> lam, rr;
+  MatrixXd Csel;
+  long long it_done = 0;
+  bool converged = false;
+  Block U(ex, n, std::min(snap_count, m), FE_F64);
+  for (int iter = 1; iter <= maxiter; ++iter) {
+    snapshot(m, snap_count, lam, rr, Csel);
+    const int cnt = static_cast<int>(lam.size());
     it_done = iter;
     progress(options_, it_done, lam, rr, nev, 1.0);
     converged = max_head(rr, nev) < tol;
     const bool at_limit = iter == maxiter || cycle_dim == n;
     if (converged || at_limit) {
-      // Ritz vectors and true residuals (RitzExtraction.cpp:34-63)
-      hip::Small Cs(ex, m, nev), th(ex, nev, 1), rn(ex, nev, 1), yn(ex, nev, 1);
-      MatrixXd Ch(m, nev);
-      for (int j = 0; j < nev; ++j)
-        for (int i = 0; i < m; ++i) Ch(i, j) = Csel(i, j);
-      Cs.from_host(Ch);
-      hip::gemm(ex, Qb.cols(0, m), Cs.data(), m, X.ref(), 1.0, 0.0);
-      hip::spmm(ex, A, X.ref(), AX.ref(), 1.0);
-      hip::check(fe_memcpy_h2d(ex.ctx(), th.data(), lam.data(), sizeof(double) * nev), "h2d");
-      hip::check(fe_ritz_residuals(ex.ctx(), FE_F64, n, nev, X.data(), X.ld(), AX.data(), AX.ld(), th.data(),
-                                   Rr.data(), Rr.ld(), rn.data(), yn.data()),
-                 "fe_ritz_residuals");
-      std::vector<double> rh(nev), yh(nev);
-      hip::check(fe_memcpy_d2h(ex.ctx(), rh.data(), rn.data(), sizeof(double) * nev), "d2h");
-      hip::check(fe_memcpy_d2h(ex.ctx(), yh.data(), yn.data(), sizeof(double) * nev), "d2h");
-      converged = true;
-      for (int j = 0; j < nev; ++j) {
-        const double den = (1.0 + std::fabs(lam[j])) * yh[j];
-        if (!(den > 0.0) || !std::isfinite(rh[j]))
-          throw std::runtime_error("BlockKrylovSchurSolver: invalid vector or non-finite residual.");
-        rr[j] = rh[j] / den;
-        converged = converged && rr[j] <= tol;
-      }
+      converged = finish(m, Csel, lam, rr);
       break;
     }
     // thick restart: Q <- [Q C_keep, Q_residual]; T <- [diag(theta) F^T; F *]
@@ -802,21 +887,11 @@ FASTEIG_INT BlockKrylovSchurSolver::solve() {
     expand(retained_blocks);
   }
 
-  eigvals_.resize(nev);
-  resnrms_.resize(nev);
-  for (int j = 0; j < nev; ++j) {
-    eigvals_(j) = lam[j];
-    resnrms_(j) = rr[j];
-  }
-  problem_.set_device_eigenvectors(std::move(X));
-  status_ = converged ? 0 : 1;
-  set_iterations(it_done);
-  return status_;
+  return publish(lam, rr, it_done, converged ? 0 : 1);
 }
 
 // ============================================================== scalar Krylov
 namespace {
-constexpr int kMaxProjected = 1024;  // largest projected matrix fe_syev takes
 
 // Scalar Krylov basis on the device with full two-pass reorthogonalization.
 // Reference: src/core/KrylovSubspace.cpp:124-260 (normalize_residual_into_basis
@@ -951,7 +1026,7 @@ struct Projection {
   void snapshot(HipExecutor& ex, Workspace& ws, int m, int count, const std::string& which, bool vectors,
                 std::vector<double>& lam, std::vector<double>& rel, MatrixXd& Csel, const char* who) {
     if (m > kMaxProjected)
-      throw std::runtime_error(std::string(who) + ": projected dimension above 1024 is not supported on the HIP backend.");
+      throw std::runtime_error(std::string(who) + ": projected dimension above 2048 is not supported on the HIP backend.");
     MatrixXd Tp(ws.cap, ws.cap);
     for (int j = 0; j < m; ++j)
       for (int i = 0; i < m; ++i) Tp(i, j) = T(i, j);
@@ -1028,7 +1103,7 @@ FASTEIG_INT LanczosSolver::solve() {
   bool converged = false;
   for (int it = 0; it < maxiter; ++it) {
     if (it >= kMaxProjected)
-      throw std::runtime_error("LanczosSolver: projected dimension above 1024 is not supported on the HIP backend.");
+      throw std::runtime_error("LanczosSolver: projected dimension above 2048 is not supported on the HIP backend.");
     const double beta = K.normalize_into(it);
     const double alpha = K.step(it);
     P.record(it, beta, alpha, K.norm());
@@ -1076,7 +1151,7 @@ FASTEIG_INT KrylovSchurSolver::solve() {
   const int64_t want = std::max<int64_t>({2LL * nev, 20LL, static_cast<int64_t>(options_.maxdim())});
   const int m = static_cast<int>(std::min<int64_t>(want, n));
   if (m > kMaxProjected)
-    throw std::invalid_argument("KrylovSchurSolver: subspace dimension above 1024 is not supported on the HIP backend.");
+    throw std::invalid_argument("KrylovSchurSolver: subspace dimension above 2048 is not supported on the HIP backend.");
   auto exp = problem_.executor(options_);
   HipExecutor& ex = *exp;
   HipCsrMatrix& A = problem_.device_matrix(options_);
@@ -1138,6 +1213,7 @@ FASTEIG_INT KrylovSchurSolver::solve() {
 std::unique_ptr<fasteig_solver> make_fasteig_solver(const std::string& solver, ::fasteig_problem& problem,
                                                     const fasteig_options& options) {
   if (has_prefix(solver, "Lanczos", 3)) return std::make_unique<LanczosSolver>(problem, options);
+  if (has_prefix(solver, "BlockLanczos", 8)) return std::make_unique<BlockLanczosSolver>(problem, options);
   if (has_prefix(solver, "BlockKrylovSchur", 11)) return std::make_unique<BlockKrylovSchurSolver>(problem, options);
   if (has_prefix(solver, "KrylovSchur", 7)) return std::make_unique<KrylovSchurSolver>(problem, options);
   if (has_prefix(solver, "Davidson", 3)) return std::make_unique<DavidsonSolver>(problem, options);
@@ -1146,7 +1222,7 @@ std::unique_ptr<fasteig_solver> make_fasteig_solver(const std::string& solver, :
   if (has_prefix(solver, "auto", 3)) return nullptr;
   throw std::invalid_argument("solver '" + solver +
                               "' is not available on the HIP backend (Chebyshev, LOBPCG, Davidson, Lanczos, "
-                              "KrylovSchur, BlockKrylovSchur)");
+                              "BlockLanczos, KrylovSchur, BlockKrylovSchur)");
 }
 
 }  // namespace fasteig

@@ -314,7 +314,7 @@ def _syev(dev, A):
     return info, Wd.download(), Vd.download()
 
 
-@pytest.mark.parametrize("m", [1, 2, 3, 5, 17, 64, 100, 128, 129, 130, 257, 512, 700])
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 17, 64, 100, 128, 129, 130, 257, 512, 700, 1500, 2048])
 def test_syev_random(dev, m):
     rng = np.random.default_rng(m)
     A = rng.standard_normal((m, m))
@@ -326,8 +326,16 @@ def test_syev_random(dev, m):
     assert np.max(np.abs(w - wr)) <= 1e-13 * scale * max(1, np.sqrt(m))
     assert np.max(np.abs(V.T @ V - np.eye(m))) <= 1e-11
     assert np.max(np.abs(A @ V - V * w)) <= 1e-12 * scale * max(1, np.sqrt(m))
-    wo, _ = orc.syev(A)
-    assert np.max(np.abs(w - wo)) <= 1e-13 * scale * max(1, np.sqrt(m))
+    if m <= 700:  # the oracle's dense QR iteration is O(m^3) on one core
+        wo, _ = orc.syev(A)
+        assert np.max(np.abs(w - wo)) <= 1e-13 * scale * max(1, np.sqrt(m))
+
+
+def test_syev_rejects_above_2048(dev):
+    A = dev.buffer((2049, 2049), np.float64, order="F")
+    W = dev.buffer((2049,), np.float64)
+    V = dev.buffer((2049, 2049), np.float64, order="F")
+    assert L().fe_syev(dev.ctx, 2049, A.ptr, 2049, W.ptr, V.ptr, 2049) == -2
 
 
 def test_syev_degenerate_and_diagonal(dev):

@@ -222,6 +222,46 @@ def test_scalar_krylov_rejects_bad_arguments():
         p.solve(solver="KrylovSchur", nev=4, which="SAI")
 
 
+def test_sio_golden_block_lanczos():
+    """reference_gpu_SiO_BlockLanczos_SR: k=5, block 10, maxiter 2000 (the
+    reference harness defaults); converges at a projected dimension above
+    1024."""
+    ref = np.array(json.load(open(os.path.join(GOLD, "sio_eigs.json")))["eigenvalues"][:5])
+    r = Problem(load_sio()).solve(solver="BlockLanczos", nev=5, tol=1e-8, maxiter=2000, seed=1, which="SR",
+                                  blksize=10)
+    assert r.status == 0
+    assert np.all(np.abs(r.eigenvalues - ref) <= 1e-10 + 1e-10 * np.abs(ref)), r.eigenvalues - ref
+    assert r.residual_norms.max() <= 1e-8
+
+
+@pytest.mark.parametrize("which", ["SR", "LR", "BE"])
+def test_block_lanczos_matches_oracle_and_closed_form(which):
+    A = G.laplacian_2d(30, 17)
+    ex = np.sort(G.laplacian_2d_eigenvalues(30, 17))
+    want = {"SR": ex[:6], "LR": ex[::-1][:6], "BE": ex[[-1, 0, -2, 1, -3, 2]]}[which]
+    p = Problem(A)
+    r = p.solve(solver="BlockLanczos", nev=6, blksize=4, maxiter=200, tol=1e-9, which=which, seed=1)
+    o = orc.block_lanczos(A, 6, block=4, maxiter=200, tol=1e-9, which=which.lower(), seed=1)
+    assert r.status == 0
+    assert np.all(np.abs(r.eigenvalues - want) <= 1e-9 * np.abs(want))
+    assert np.all(np.abs(r.eigenvalues - o["eigenvalues"]) <= 1e-10 * np.abs(want))
+    assert abs(r.iterations - o["iterations"]) <= 1
+    X = p.eigenvectors()
+    assert np.max(np.abs(X.T @ X - np.eye(6))) <= 1e-8
+
+
+def test_block_lanczos_narrow_block_and_bad_arguments():
+    A = sp.diags(np.r_[np.ones(10), 2.0 * np.ones(10), 3.0 * np.ones(10)]).tocsr()
+    r = Problem(A).solve(solver="BlockLanczos", nev=4, blksize=3, maxiter=50, tol=1e-10, which="SR", seed=2)
+    assert r.iterations == 3
+    assert np.allclose(r.eigenvalues, [1.0, 1.0, 1.0, 2.0], atol=1e-12)
+    p = Problem(G.laplacian_2d(6))
+    with pytest.raises(ValueError, match="maxiter\\*blksize"):
+        p.solve(solver="BlockLanczos", nev=8, blksize=2, maxiter=3, which="SR")
+    with pytest.raises(ValueError, match="SAI"):
+        p.solve(solver="BlockLanczos", nev=4, blksize=2, which="SAI")
+
+
 def test_generalized_diagonal_mass_matches_dense_pencil():
     K, M = G.conductivity_3d(10, seed=2, mass_seed=3)
     nev = 6
