@@ -114,8 +114,10 @@ class fasteig_params {
   FASTEIG_INT seed = -1;
   FASTEIG_INT verbosity = -1;
   FASTEIG_INT print_every = 0;
-  // MI355X backend extensions (deck keys `precision`, `gram`, `format`)
+  // MI355X backend extensions (deck keys `precision`, `gram`, `format`,
+  // `cheb_bounds`; empty cheb_bounds keeps the options' default)
   std::string precision = "fp64";
   std::string gram = "native";
   std::string format = "auto";
+  std::string cheb_bounds;
 };

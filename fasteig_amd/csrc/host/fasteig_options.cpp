@@ -173,6 +173,7 @@ void fasteig_options::configure_from(const fasteig_params& p) {
   set("precision", p.precision);
   set("gram", p.gram);
   set("format", p.format);
+  if (!p.cheb_bounds.empty()) set("cheb_bounds", p.cheb_bounds);
 }
 
 void fasteig_options::validate() const {

@@ -174,6 +174,8 @@ extern "C" int parseinput(char* filename, fasteig_params* params) {
         p.gram = value;
       } else if (starts_with_key(key, "format")) {
         p.format = value;
+      } else if (starts_with_key(key, "cheb_bounds")) {
+        p.cheb_bounds = value;
       }
       // unknown keys are ignored, as in the reference
     }

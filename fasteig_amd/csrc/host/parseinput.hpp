@@ -6,7 +6,7 @@
 // Keys: amatrix, bmatrix, logfile, nev, solver, which, maxiter, maxdim,
 // blksize, verbosity, print_every, output_prefix, output_format, verbose, tol,
 // shift, seed, and -- not parsed by the reference, which silently ignores them
-// -- degree, maxblks, precision, gram, format.
+// -- degree, maxblks, precision, gram, format, cheb_bounds.
 // Matrix Market: "coordinate real general|symmetric" files; for symmetric
 // files the dominant stored triangle is kept (as the reference does, :249-279)
 // and expanded to full storage by fasteig_problem.
