@@ -17,7 +17,8 @@ Configurations (matrices from fasteig_amd.generators, seeded):
       maxdim 192, lowest 32, tol 1e-8 (a text file of about 2 GB)
 
 Usage: python -m fasteig_amd.decks <c1|c2|c3|c4> <directory> [--small]
-(--small writes the same deck on a reduced grid, for quick checks).
+[--rsa | --fei] (--small writes the same deck on a reduced grid, for quick
+checks; --rsa / --fei write the reference's compressed-column formats).
 """
 from __future__ import annotations
 
@@ -52,6 +53,41 @@ def write_mtx(path: str, A, symmetric: bool = True) -> None:
             f.write("\n")
 
 
+def _csc_full(A):
+    C = sp.csc_matrix(A)
+    C.sort_indices()
+    return C.shape[0], C.indptr.astype(np.int64), C.indices.astype(np.int64), C.data.astype(np.float64)
+
+
+def write_rsa(path: str, A) -> None:
+    """RSA text file as the reference's readRSA reads it (src/parseinput.cpp:27-67):
+    five header lines (the third "RSA nrows ncols nnz 0"), then 1-based column
+    pointers, row indices and values. All stored entries form the matrix
+    (full storage, both triangles)."""
+    n, cp, ri, v = _csc_full(A)
+    with open(path, "w") as f:
+        f.write("fasteig_amd matrix\n")
+        f.write(f"{len(cp) + len(ri) + len(v)} {len(cp)} {len(ri)} {len(v)}\n")
+        f.write(f"RSA {n} {n} {len(v)} 0\n")
+        f.write("(10I8) (10I8) (4E26.17)\n\n")
+        for arr, fmt in ((cp + 1, "{}"), (ri + 1, "{}"), (v, "{:.17g}")):
+            vals = arr.tolist()
+            for s0 in range(0, len(vals), 10):
+                f.write(" ".join(fmt.format(x) for x in vals[s0:s0 + 10]) + "\n")
+
+
+def write_fei(path: str, A) -> None:
+    """FEI binary file (reference readFEI, src/parseinput.cpp:69-111): int32
+    nrows, ncols, nnz; int32 colptr (0-based); int32 rowind; float64 values;
+    little-endian; full storage."""
+    n, cp, ri, v = _csc_full(A)
+    with open(path, "wb") as f:
+        np.array([n, n, len(v)], dtype="<i4").tofile(f)
+        cp.astype("<i4").tofile(f)
+        ri.astype("<i4").tofile(f)
+        v.astype("<f8").tofile(f)
+
+
 def _config(name: str, small: bool):
     if name == "c1":
         g = 20 if small else 100
@@ -74,15 +110,18 @@ def _config(name: str, small: bool):
     raise ValueError(f"unknown configuration {name!r} (c1, c2, c3, c4)")
 
 
-def write_deck(name: str, directory: str, small: bool = False) -> str:
-    """Write the matrices and the deck of configuration `name`; returns the
-    deck's path."""
+def write_deck(name: str, directory: str, small: bool = False, fmt: str = "mtx") -> str:
+    """Write the matrices (fmt: mtx, rsa or fei) and the deck of configuration
+    `name`; returns the deck's path."""
     mats, opts = _config(name, small)
     os.makedirs(directory, exist_ok=True)
     lines = [f"# configuration {name}" + (" (reduced grid)" if small else "")]
+    writers = {"mtx": lambda path, A: write_mtx(path, A, symmetric=True), "rsa": write_rsa, "fei": write_fei}
+    if fmt not in writers:
+        raise ValueError(f"unknown matrix format {fmt!r} (mtx, rsa, fei)")
     for key, A in mats.items():
-        fname = f"{name}_{key}.mtx"
-        write_mtx(os.path.join(directory, fname), A, symmetric=True)
+        fname = f"{name}_{key}.{fmt}"
+        writers[fmt](os.path.join(directory, fname), A)
         lines.append(f"{key}matrix {fname}")
     lines.append(f"logfile {name}")
     lines.append("output_format text")
@@ -97,4 +136,5 @@ if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     if len(args) != 2:
         sys.exit("usage: python -m fasteig_amd.decks <c1|c2|c3|c4> <directory> [--small]")
-    print(write_deck(args[0], args[1], small="--small" in sys.argv))
+    fmt = "rsa" if "--rsa" in sys.argv else "fei" if "--fei" in sys.argv else "mtx"
+    print(write_deck(args[0], args[1], small="--small" in sys.argv, fmt=fmt))

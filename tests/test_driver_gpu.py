@@ -35,8 +35,9 @@ def raw_results(path):
     return rows
 
 
-def test_runfasteig_c1_deck(tmp_path):
-    deck = decks.write_deck("c1", str(tmp_path), small=True)
+@pytest.mark.parametrize("fmt", ["mtx", "rsa", "fei"])
+def test_runfasteig_c1_deck(tmp_path, fmt):
+    deck = decks.write_deck("c1", str(tmp_path), small=True, fmt=fmt)
     run_deck(deck, str(tmp_path))
     rows = raw_results(os.path.join(str(tmp_path), "c1_LOBPCG.log"))
     assert [r[0] for r in rows] == list(range(1, 11))
