@@ -66,6 +66,7 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
         built, reuse = C.build_blocked(8 if fmt == "blocked8" else 4)
         if not built:
             raise RuntimeError(f"row-block copy not built (reuse {reuse:.2f})")
+        C.block_schedule()  # strip order for grid stencils, as the solvers use it
         C.use(_fe.FMT_BLOCKED)
     n = A.shape[0]
     dt = np.float64 if prec == _fe.F64 else np.float32

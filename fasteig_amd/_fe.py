@@ -56,6 +56,7 @@ def lib():
         L.fe_csr_build_sell.argtypes = [vp, vp, i32, i32]
         L.fe_csr_use.argtypes = [vp, i32]
         L.fe_csr_build_blocked.argtypes = [vp, vp, i32, C.POINTER(dbl)]
+        L.fe_csr_block_schedule.argtypes = [vp, vp, i64, i64]
         L.fe_fill_random.argtypes = [vp, i32, i64, i32, vp, i64, C.c_uint64]
         L.fe_scale_rows.argtypes = [vp, i32, i64, i32, vp, i64, vp]
         L.fe_last_error.argtypes = []
@@ -167,6 +168,11 @@ class Csr:
         r = C.c_double(0.0)
         rc = check(lib().fe_csr_build_blocked(self.dev.ctx, self.h, rb, C.byref(r)))
         return rc == 0, r.value
+
+    def block_schedule(self, plane=-1, strip=0):
+        """Row-block processing order for 3D grid stencils (plane -1: detect,
+        0: remove); returns True when an order is in place."""
+        return check(lib().fe_csr_block_schedule(self.dev.ctx, self.h, plane, strip)) == 0 and plane != 0
 
     def use(self, fmt):
         check(lib().fe_csr_use(self.h, fmt))

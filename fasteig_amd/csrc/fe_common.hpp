@@ -82,6 +82,8 @@ struct fe_csr_s {
   double* bg_v64 = nullptr;
   float* bg_v32 = nullptr;
   double bg_reuse = 0.0;      // nonzeros / union entries (gathers saved)
+  int* bg_sched = nullptr;     // optional processing order of the row blocks
+  int64_t bg_plane = 0, bg_strip = 0;
   int use_bg = 0;
 };
 

@@ -450,7 +450,8 @@ int fe_csr_destroy(fe_csr A) {
   if (!A) return 0;
   if (A->ctx) hipStreamSynchronize(A->ctx->stream);
   void* ptrs[] = {A->rowptr, A->colind, A->vals64, A->vals32, A->sell_ptr, A->sell_len,
-                  A->sell_perm, A->sell_col, A->sell_v64, A->sell_v32, A->bg_bptr, A->bg_ucol, A->bg_v64, A->bg_v32};
+                  A->sell_perm, A->sell_col, A->sell_v64, A->sell_v32, A->bg_bptr, A->bg_ucol, A->bg_v64, A->bg_v32,
+                  A->bg_sched};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete A;

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, const int64_t* __restric
                                                  const int* __restrict__ ucol, const T* __restrict__ bval, int k,
                                                  const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha,
                                                  T beta, const T* Z, int64_t ldz, T gamma, int64_t per_xcd,
-                                                 int64_t ntiles, int64_t nblk) {
+                                                 int64_t ntiles, int64_t nblk, const int* __restrict__ sched) {
   constexpr int V = 16 / (int)sizeof(T);
   constexpr int GPW = 64 / LPR;  // row blocks per wavefront
   constexpr int GPB = 4 * GPW;   // row blocks per workgroup
@@ -40,11 +40,14 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, const int64_t* __restric
   // wave index made wavefront-uniform so that whole-wave groups use scalar loads
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int grp = LPR == 64 ? 0 : lane / LPR;
-  const int64_t blk = tile * GPB + wave * GPW + grp;
+  // slot in the processing order; the row block is the slot itself or the
+  // schedule's entry for it
+  const int64_t slot = tile * GPB + wave * GPW + grp;
+  const bool blkok = slot < nblk;
+  const int64_t blk = (sched && blkok) ? static_cast<int64_t>(sched[slot]) : slot;
   const int sub = LPR == 64 ? lane : lane % LPR;
   const int c = blockIdx.y * COLS + sub * V;
   const bool full = c + (RS - 1) * STRIDE + V <= k;
-  const bool blkok = blk < nblk;
   const int64_t e0 = blkok ? bptr[blk] : 0;
   const int64_t e1 = blkok ? bptr[blk + 1] : 0;
   T acc[RB][RS][V];
@@ -144,7 +147,7 @@ static int launch_rb(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y,
   const T* bv = sizeof(T) == 8 ? (const T*)A->bg_v64 : (const T*)A->bg_v32;
   hipLaunchKernelGGL((k_spmm_rb<T, RB, LPR, RS>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream, A->n,
                      A->bg_bptr, A->bg_ucol, bv, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles,
-                     A->bg_nblk);
+                     A->bg_nblk, A->bg_sched);
   FE_CHECK_LAUNCH();
   return 0;
 }
@@ -241,5 +244,73 @@ extern "C" int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse)
   A->bg_nblk = nblk;
   A->bg_nu = nu;
   A->bg_reuse = r;
+  return 0;
+}
+
+namespace {
+// Line and plane lengths of a 3D grid stencil in natural order, from the
+// distinct positive column offsets j - i of a sample of rows: three clusters
+// (within a line, to the neighbouring lines, to the neighbouring planes) whose
+// centres are 1, the line length and the plane length. Returns false for any
+// other structure.
+bool detect_grid(const std::vector<int>& rp, const std::vector<int>& ci, int n, int64_t& line, int64_t& plane) {
+  std::vector<int64_t> off;
+  const int step = std::max(1, n / 4096);
+  for (int i = 0; i < n; i += step)
+    for (int p = rp[i]; p < rp[i + 1]; ++p)
+      if (ci[p] > i) off.push_back(static_cast<int64_t>(ci[p]) - i);
+  std::sort(off.begin(), off.end());
+  off.erase(std::unique(off.begin(), off.end()), off.end());
+  if (off.empty() || off.size() > 64) return false;
+  std::vector<std::pair<int64_t, int64_t>> cl;  // [first, last] of each cluster
+  for (int64_t d : off) {
+    if (cl.empty() || d > 4 * cl.back().second) cl.push_back({d, d});
+    else cl.back().second = d;
+  }
+  if (cl.size() != 3 || cl[0].second > 2) return false;
+  line = (cl[1].first + cl[1].second) / 2;
+  plane = (cl[2].first + cl[2].second) / 2;
+  return line >= 4 && plane >= 4 * line && plane % line == 0 && n >= 3 * plane;
+}
+}  // namespace
+
+extern "C" int fe_csr_block_schedule(fe_ctx ctx, fe_csr A, int64_t plane, int64_t strip) {
+  FE_ARG(ctx && A, "fe_csr_block_schedule: null argument");
+  FE_ARG(A->bg_ucol, "fe_csr_block_schedule: build the row-block copy first");
+  if (A->bg_sched) {
+    FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    FE_HIP_TRY(hipFree(A->bg_sched));
+    A->bg_sched = nullptr;
+    A->bg_plane = A->bg_strip = 0;
+  }
+  if (plane == 0) return 0;
+  const int n = A->n, rb = A->bg_rb;
+  int64_t line = 0;
+  if (plane < 0) {
+    std::vector<int> rp(n + 1), ci(A->nnz);
+    FE_HIP_TRY(hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+    if (A->nnz)
+      FE_HIP_TRY(hipMemcpyAsync(ci.data(), A->colind, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (!detect_grid(rp, ci, n, line, plane)) return 1;
+    // one strip per XCD: each XCD sweeps its eighth of every plane
+    if (strip <= 0) strip = line * ((plane / line + 7) / 8);
+    if (plane % rb || strip % rb) return 1;
+  }
+  FE_ARG(strip > 0 && plane % rb == 0 && strip % rb == 0 && strip <= plane,
+         "fe_csr_block_schedule: plane and strip must be multiples of the row block size, strip <= plane");
+  const int64_t nblk = A->bg_nblk, planes = (n + plane - 1) / plane;
+  std::vector<int> order;
+  order.reserve(static_cast<size_t>(nblk));
+  for (int64_t s0 = 0; s0 < plane; s0 += strip)
+    for (int64_t z = 0; z < planes; ++z)
+      for (int64_t r = z * plane + s0; r < z * plane + std::min(s0 + strip, plane) && r < n; r += rb)
+        order.push_back(static_cast<int>(r / rb));
+  FE_ARG(static_cast<int64_t>(order.size()) == nblk, "fe_csr_block_schedule: schedule does not cover the row blocks");
+  FE_HIP_TRY(hipMalloc(&A->bg_sched, sizeof(int) * order.size()));
+  FE_HIP_TRY(hipMemcpyAsync(A->bg_sched, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  A->bg_plane = plane;
+  A->bg_strip = strip;
   return 0;
 }

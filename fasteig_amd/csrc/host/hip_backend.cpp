@@ -115,6 +115,8 @@ bool HipCsrMatrix::use_blocked() {
   if (blocked_state_ == 0) {
     double reuse = 0.0;
     blocked_state_ = check(fe_csr_build_blocked(ex_.ctx(), h_, 4, &reuse), "fe_csr_build_blocked") == 0 ? 1 : -1;
+    // grid stencils: strip-by-strip block order for L2 reuse across planes
+    if (blocked_state_ > 0) check(fe_csr_block_schedule(ex_.ctx(), h_, -1, 0), "fe_csr_block_schedule");
   }
   if (blocked_state_ < 0) return false;
   check(fe_csr_use(h_, FE_FMT_BLOCKED), "fe_csr_use");

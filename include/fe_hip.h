@@ -98,6 +98,17 @@ int fe_csr_use(fe_csr A, int format);
  * stencils, scattered graphs); *reuse receives nonzeros per union entry.
  * Select it with fe_csr_use(A, FE_FMT_BLOCKED). */
 int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse);
+/* Processing order of the row blocks (needs the row-block copy). For a 3D
+ * grid stencil in natural order (rows z * plane + y * line + x) the blocks are
+ * taken strip by strip -- `strip` rows of every plane, planes innermost -- so
+ * the X rows a block shares with the blocks of the neighbouring planes are
+ * still in the XCD's L2 when those run. plane < 0: detect line and plane from
+ * the column offsets (strip <= 0 then means an eighth of a plane in whole
+ * lines: one strip per XCD); returns 1 when the matrix is not such a
+ * stencil. plane == 0 removes the order. Results are
+ * bitwise the same in any order (each row's sum is unchanged). No counterpart
+ * in the reference (its CSR product is cuSPARSE's). */
+int fe_csr_block_schedule(fe_ctx ctx, fe_csr A, int64_t plane, int64_t strip);
 
 /* ---- fused block SpMM (replaces LinearOperator::apply_block,
  *      include/core/LinearOperator.hpp:32-37; CUDA: k x cusparseSpMV,

@@ -151,6 +151,50 @@ def test_spmm_blocked_ragged_tail(dev):
             assert np.array_equal(Y1.download(), Y2.download())
 
 
+@pytest.mark.parametrize("k", [4, 48, 256, 300, 512])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_spmm_block_schedule_is_bitwise(dev, k, prec):
+    """Strip-by-strip row-block order (24^3 grid: line 24, plane 576): the
+    automatic order, an explicit 2-line strip and no order give bitwise the
+    same products (each row's sum is unchanged)."""
+    A = G.hamiltonian_27pt(24, seed=4, wells=9)
+    n = A.shape[0]
+    rng = np.random.default_rng(k)
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    Xd = dev.put(rng.uniform(-1, 1, (n, k)), dt)
+    Zd = dev.put(rng.uniform(-1, 1, (n, k)), dt)
+    C = dev.csr(A)
+    assert C.build_blocked(4)[0]
+    C.use(_fe.FMT_BLOCKED)
+    outs = []
+    for plane, strip in ((0, 0), (-1, 0), (576, 48), (576, 576), (0, 0)):
+        on = C.block_schedule(plane, strip)
+        assert on == (plane != 0)
+        Y = dev.buffer((n, k), dt)
+        _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y.ptr, k, 0.7, 0.3, Zd.ptr, k, -1.1))
+        outs.append(Y.download())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_block_schedule_detection(dev, mats):
+    """Only 3D grid stencils whose line is a multiple of the row block get an
+    order; a 2D 9-point stencil and a 23-point line do not; bad explicit
+    arguments are rejected."""
+    C = dev.csr(mats["h27big"])  # 23^3: line 23
+    assert C.build_blocked(4)[0]
+    assert C.block_schedule() is False
+    T = sp.diags([1.0, 1.0, 1.0], [-1, 0, 1], shape=(64, 64))
+    C2 = dev.csr(G._finish(sp.kron(T, T).tocsr()))
+    assert C2.build_blocked(4)[0]
+    assert C2.block_schedule() is False
+    with pytest.raises(_fe.FeError):
+        C2.block_schedule(530, 8)  # plane not a multiple of the row block
+    C3 = dev.csr(mats["h27"])
+    with pytest.raises(_fe.FeError):
+        C3.block_schedule()  # no row-block copy yet
+
+
 @pytest.mark.parametrize("name", ["powlaw", "lap2d", "cond3d"])
 def test_spmm_blocked_rejects_low_reuse(dev, mats, name):
     """Scattered columns (power law) and thin stencils (5- and 7-point: under
