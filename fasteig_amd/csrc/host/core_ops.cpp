@@ -177,11 +177,13 @@ std::vector<int> select_indices(const std::vector<double>& v, const std::string&
 }
 
 RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
-                       const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R) {
+                       const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R,
+                       int res_count) {
   HipExecutor& ex = ws.ex;
   if (m <= 0 || requested <= 0 || requested > m)
     throw std::invalid_argument("extract_projected_ritz_pairs: invalid active/requested Ritz dimensions.");
-  if (Q.cols < m || AQ.cols < m || Y.cols != requested || AY.cols != requested)
+  const int nr = res_count < 0 ? requested : res_count;
+  if (nr <= 0 || nr > requested || Q.cols < m || AQ.cols < m || Y.cols != requested || AY.cols != nr)
     throw std::invalid_argument("extract_projected_ritz_pairs: multivector dimension mismatch.");
   ws.ensure(m);
   check(fe_sym_small(ex.ctx(), m, ws.G.data(), ws.cap), "fe_sym_small");
@@ -195,20 +197,20 @@ RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& 
   check(fe_gather_cols(ex.ctx(), m, requested, ws.V.data(), ws.cap, idx.data(), ws.C.data(), ws.cap),
         "fe_gather_cols");
   gemm(ex, Q.range(0, m), ws.C.data(), ws.cap, Y, 1.0, 0.0);
-  gemm(ex, AQ.range(0, m), ws.C.data(), ws.cap, AY, 1.0, 0.0);
+  gemm(ex, AQ.range(0, m), ws.C.data(), ws.cap, AY, 1.0, 0.0);  // C's first nr columns
   RitzPairs out;
   out.selected = idx;
   out.values.resize(requested);
   for (int j = 0; j < requested; ++j) out.values[j] = w[idx[j]];
-  check(fe_memcpy_h2d(ex.ctx(), ws.theta.data(), out.values.data(), sizeof(double) * requested), "h2d(theta)");
-  check(fe_ritz_residuals(ex.ctx(), Y.prec, Y.rows, requested, Y.ptr, Y.ld, AY.ptr, AY.ld, ws.theta.data(),
+  check(fe_memcpy_h2d(ex.ctx(), ws.theta.data(), out.values.data(), sizeof(double) * nr), "h2d(theta)");
+  check(fe_ritz_residuals(ex.ctx(), Y.prec, Y.rows, nr, Y.ptr, Y.ld, AY.ptr, AY.ld, ws.theta.data(),
                           R ? R->ptr : nullptr, R ? R->ld : 0, ws.rn.data(), ws.yn.data()),
         "fe_ritz_residuals");
-  std::vector<double> rn(requested), yn(requested);
-  check(fe_memcpy_d2h(ex.ctx(), rn.data(), ws.rn.data(), sizeof(double) * requested), "d2h");
-  check(fe_memcpy_d2h(ex.ctx(), yn.data(), ws.yn.data(), sizeof(double) * requested), "d2h");
-  out.relres.resize(requested);
-  for (int j = 0; j < requested; ++j) {
+  std::vector<double> rn(nr), yn(nr);
+  check(fe_memcpy_d2h(ex.ctx(), rn.data(), ws.rn.data(), sizeof(double) * nr), "d2h");
+  check(fe_memcpy_d2h(ex.ctx(), yn.data(), ws.yn.data(), sizeof(double) * nr), "d2h");
+  out.relres.resize(nr);
+  for (int j = 0; j < nr; ++j) {
     const double den = (1.0 + std::fabs(out.values[j])) * yn[j];
     out.relres[j] = den > 0.0 ? rn[j] / den : rn[j];
   }

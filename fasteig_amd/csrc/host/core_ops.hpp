@@ -56,8 +56,11 @@ struct RitzPairs {
 // and is overwritten. Y = Q C, AY = AQ C, R = AY - Y Theta (R may be null: norms
 // only). Reference: src/core/RitzExtraction.cpp:65-141, selection :162-179 and
 // include/fasteig_solver.hpp:59-82. The coefficient matrix stays in ws.C.
+// res_count >= 0: AY, R and the residual norms only for the first res_count
+// selected pairs (relres then holds res_count entries); Y for all requested.
 RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
-                       const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R);
+                       const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R,
+                       int res_count = -1);
 
 // Stable index sort by `which` (include/fasteig_solver.hpp:59-82; "be" from
 // src/core/RitzExtraction.cpp:162-179).

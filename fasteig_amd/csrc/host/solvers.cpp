@@ -217,8 +217,9 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // 3. Rayleigh-Ritz (ChebyshevSolver.cpp:175-183); Ritz vectors go straight into V
     hip::spmm(ex, A, Q.cols(0, subspace_cols), AQ.cols(0, subspace_cols), sign);
     hip::gram(ex, ws.gram_mode, Q.cols(0, subspace_cols), AQ.cols(0, subspace_cols), ws.G.data(), ws.cap);
+    // residuals of the wanted pairs only: AQ C for their nev columns
     ritz = hip::extract_ritz(ws, subspace_cols, Q.ref(), AQ.ref(), subspace_cols, "sr", V.cols(0, subspace_cols),
-                             AY.cols(0, subspace_cols), nullptr);
+                             AY.cols(0, nev), nullptr, nev);
     if (!all_finite(ritz.values) || !all_finite(ritz.relres))
       throw std::runtime_error("ChebyshevSolver: non-finite Ritz values or residuals.");
     it_done = iter + 1;
