@@ -236,14 +236,16 @@ def c4_bench(maxiter=400):
     return out
 
 
-def serving_bench(count=512, rate=200.0, baseline=48, degree=16, max_slots=128, seed=11):
+def serving_bench(count=512, rate=200.0, baseline=None, degree=16, max_slots=128, seed=11):
     """Serving workload: `count` requests (2D 5-point / 3D 7-point Laplacian +
     N(0,1) diagonal, n log-uniform in [1e3, 2e5], nev in {8,16,32}, tol in
     {1e-6,1e-8}) with Poisson arrivals at `rate`/s, served by the
-    continuous-batching server in real time; then the first `baseline`
-    requests one at a time with the plain-FP64 Chebyshev solver (same degree,
-    ritz bounds), their latencies from a FIFO queue over the measured solve
-    times at the same arrivals. Eigenvectors are returned by both."""
+    continuous-batching server in real time, and again with every request
+    waiting at time 0 (the server's capacity); then the same requests (the
+    first `baseline` when given) one at a time with the plain-FP64 Chebyshev
+    solver (same degree, ritz bounds), their latencies from a FIFO queue over
+    the measured solve times at the same arrivals. Eigenvectors are returned
+    by both."""
     from fasteig_amd import generators as G
     from fasteig_amd.api import Problem
     from fasteig_amd.serving import Server
@@ -259,39 +261,52 @@ def serving_bench(count=512, rate=200.0, baseline=48, degree=16, max_slots=128, 
         while w.pending():
             w.step()
             w.results()
-    srv = Server(max_slots=max_slots, degree=degree)
-    lat, status, cycles = {}, {}, {}
-    t0 = time.perf_counter()
-    i = 0
-    while len(lat) < count:
-        now = time.perf_counter() - t0
-        while i < count and stream[i]["arrival"] <= now:
-            srv.submit(i, mats[i], stream[i]["nev"], stream[i]["tol"])
-            i += 1
-        active, queued, _ = srv.counts()
-        if active == 0 and queued == 0 and i < count:
-            time.sleep(max(0.0, stream[i]["arrival"] - now))
-            continue
-        srv.step()
-        for r in srv.results():
-            lat[r.id] = time.perf_counter() - t0 - stream[r.id]["arrival"]
-            status[r.id] = r.status
-            cycles[r.id] = r.cycles
-    span = time.perf_counter() - t0 - stream[0]["arrival"]
-    st = srv.stats()
-    srv.close()
-    lats = np.array([lat[j] for j in range(count)])
-    out = {"requests": count, "arrival_rate_per_s": rate, "solves_per_s": count / span,
-           "p50_s": float(np.percentile(lats, 50)), "p99_s": float(np.percentile(lats, 99)),
-           "converged": int(sum(1 for v in status.values() if v == 0)), "cycles": st["cycles"],
-           "peak_active": st["peak_active"], "peak_hbm_gb": st["peak_bytes"] / 1e9,
-           "mean_problem_cycles": float(np.mean(list(cycles.values())))}
+    def run_batched(arrival):
+        """Serve every request in real time, request i submitted once
+        arrival[i] seconds have passed; latencies from arrival to result."""
+        srv = Server(max_slots=max_slots, degree=degree)
+        lat, status, cycles = {}, {}, {}
+        t0 = time.perf_counter()
+        i = 0
+        while len(lat) < count:
+            now = time.perf_counter() - t0
+            while i < count and arrival[i] <= now:
+                srv.submit(i, mats[i], stream[i]["nev"], stream[i]["tol"])
+                i += 1
+            active, queued, _ = srv.counts()
+            if active == 0 and queued == 0 and i < count:
+                time.sleep(max(0.0, arrival[i] - now))
+                continue
+            srv.step()
+            for r in srv.results():
+                lat[r.id] = time.perf_counter() - t0 - arrival[r.id]
+                status[r.id] = r.status
+                cycles[r.id] = r.cycles
+        span = time.perf_counter() - t0 - arrival[0]
+        st = srv.stats()
+        srv.close()
+        lats = np.array([lat[j] for j in range(count)])
+        return {"requests": count, "solves_per_s": count / span,
+                "p50_s": float(np.percentile(lats, 50)), "p99_s": float(np.percentile(lats, 99)),
+                "converged": int(sum(1 for v in status.values() if v == 0)), "cycles": st["cycles"],
+                "peak_active": st["peak_active"], "peak_hbm_gb": st["peak_bytes"] / 1e9,
+                "mean_problem_cycles": float(np.mean(list(cycles.values())))}
+
+    out = run_batched([r["arrival"] for r in stream])
+    out["arrival_rate_per_s"] = rate
     log(f"serving batched: {out['solves_per_s']:.1f} solves/s p50 {out['p50_s']:.3f}s p99 {out['p99_s']:.3f}s "
-        f"converged {out['converged']}/{count}, {st['cycles']} cycles, peak {st['peak_active']} active")
+        f"converged {out['converged']}/{count}, {out['cycles']} cycles, peak {out['peak_active']} active")
+    # capacity: all requests waiting at time 0
+    sat = run_batched([0.0] * count)
+    log(f"serving batched, all at t=0: {sat['solves_per_s']:.1f} solves/s p50 {sat['p50_s']:.3f}s "
+        f"p99 {sat['p99_s']:.3f}s converged {sat['converged']}/{count}")
     # one at a time, plain FP64
     durs = []
     p0 = None
-    for j in range(min(baseline, count)):
+    nb = count if baseline is None else min(baseline, count)
+    for j in range(nb):
+        if j and j % 128 == 0:
+            log(f"serving one-at-a-time: {j}/{nb} solved")
         r = stream[j]
         ts = time.perf_counter()
         p = Problem(mats[j], share_with=p0)
@@ -311,8 +326,11 @@ def serving_bench(count=512, rate=200.0, baseline=48, degree=16, max_slots=128, 
            "mean_solve_s": float(durs.mean())}
     log(f"serving one-at-a-time fp64 ({len(durs)} requests): {one['solves_per_s']:.1f} solves/s "
         f"p50 {one['p50_s']:.3f}s p99 {one['p99_s']:.3f}s")
-    return {"batched": out, "one_at_a_time_fp64": one,
-            "throughput_ratio": out["solves_per_s"] / one["solves_per_s"]}
+    # one-at-a-time with every request waiting at time 0: p99 of the FIFO queue
+    one["p99_all_at_t0_s"] = float(np.percentile(np.cumsum(durs), 99))
+    return {"batched": out, "batched_all_at_t0": sat, "one_at_a_time_fp64": one,
+            "throughput_ratio": out["solves_per_s"] / one["solves_per_s"],
+            "capacity_ratio": sat["solves_per_s"] / one["solves_per_s"]}
 
 
 def main():

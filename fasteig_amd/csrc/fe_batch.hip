@@ -137,10 +137,14 @@ __device__ void grp_gram_chunk(const fe_grp_gram& it, int64_t lc, double* xs, do
   const int64_t r1 = r0 + FE_GRP_CHUNK < it.n ? r0 + FE_GRP_CHUNK : it.n;
   const T* X = static_cast<const T*>(it.X);
   const T* Y = static_cast<const T*>(it.Y);
-  constexpr int PER = FE_GRP_MAXCOLS * FE_GRP_MAXCOLS / 256;
-  double acc[PER];
+  // thread -> 4 x 4 block of the 64 x 64 tile: rows 4 (tid % 16) .., columns
+  // 4 (tid / 16) ..; per staged row 8 LDS values feed 16 FMAs
+  const int a0 = 4 * (tid % 16), b0 = 4 * (tid / 16);
+  double acc[4][4];
 #pragma unroll
-  for (int q = 0; q < PER; ++q) acc[q] = 0.0;
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
   for (int64_t s0 = r0; s0 < r1; s0 += GRP_SUB) {
     const int rows = static_cast<int>(r1 - s0 < GRP_SUB ? r1 - s0 : GRP_SUB);
     __syncthreads();
@@ -150,27 +154,31 @@ __device__ void grp_gram_chunk(const fe_grp_gram& it, int64_t lc, double* xs, do
       ys[e] = (r < rows && j < p) ? static_cast<double>(Y[(s0 + r) * it.ldy + j]) : 0.0;
     }
     __syncthreads();
+    if (a0 < m && b0 < p) {
+      for (int r = 0; r < rows; ++r) {
+        const double2 x01 = *reinterpret_cast<const double2*>(xs + r * FE_GRP_MAXCOLS + a0);
+        const double2 x23 = *reinterpret_cast<const double2*>(xs + r * FE_GRP_MAXCOLS + a0 + 2);
+        const double2 y01 = *reinterpret_cast<const double2*>(ys + r * FE_GRP_MAXCOLS + b0);
+        const double2 y23 = *reinterpret_cast<const double2*>(ys + r * FE_GRP_MAXCOLS + b0 + 2);
+        const double xv[4] = {x01.x, x01.y, x23.x, x23.y}, yv[4] = {y01.x, y01.y, y23.x, y23.y};
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = tid + 256 * q;
-      const int a = idx % FE_GRP_MAXCOLS, b = idx / FE_GRP_MAXCOLS;
-      double s = acc[q];
-      for (int r = 0; r < rows; ++r) s += xs[r * FE_GRP_MAXCOLS + a] * ys[r * FE_GRP_MAXCOLS + b];
-      acc[q] = s;
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] += xv[i] * yv[j];
+      }
     }
   }
   double* part = it.part + lc * (int64_t)m * p;
 #pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int idx = tid + 256 * q;
-    const int a = idx % FE_GRP_MAXCOLS, b = idx / FE_GRP_MAXCOLS;
-    if (a < m && b < p) part[a + (int64_t)b * m] = acc[q];
-  }
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (a0 + i < m && b0 + j < p) part[(a0 + i) + (int64_t)(b0 + j) * m] = acc[i][j];
 }
 
 __global__ __launch_bounds__(256) void k_grp_gram_part(const void* table) {
-  __shared__ double xs[GRP_SUB * FE_GRP_MAXCOLS];
-  __shared__ double ys[GRP_SUB * FE_GRP_MAXCOLS];
+  __shared__ __align__(16) double xs[GRP_SUB * FE_GRP_MAXCOLS];
+  __shared__ __align__(16) double ys[GRP_SUB * FE_GRP_MAXCOLS];
   const fe_grp_header* h = static_cast<const fe_grp_header*>(table);
   const fe_grp_gram* items = grp_items<fe_grp_gram>(table);
   for (int64_t t = blockIdx.x; t < h->ntiles; t += gridDim.x) {
@@ -180,19 +188,20 @@ __global__ __launch_bounds__(256) void k_grp_gram_part(const void* table) {
   }
 }
 
-// one workgroup per item: chunk partials summed in chunk order
+// grid (items, 16): workgroup (i, y) sums elements 256 y .. 256 y + 255 of
+// item i's m x p partials, each in chunk order
 __global__ __launch_bounds__(256) void k_grp_gram_reduce(const void* table) {
   const fe_grp_header* h = static_cast<const fe_grp_header*>(table);
   if (static_cast<int>(blockIdx.x) >= h->nitems) return;
   const fe_grp_gram& it = grp_items<fe_grp_gram>(table)[blockIdx.x];
   const int64_t nch = (it.n + FE_GRP_CHUNK - 1) / FE_GRP_CHUNK;
   const int mp = it.m * it.p;
-  for (int e = threadIdx.x; e < mp; e += 256) {
-    double s = 0.0;
-    for (int64_t c = 0; c < nch; ++c) s += it.part[c * mp + e];
-    const int a = e % it.m, b = e / it.m;
-    it.G[a + b * it.ldg] = s;
-  }
+  const int e = static_cast<int>(blockIdx.y) * 256 + static_cast<int>(threadIdx.x);
+  if (e >= mp) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < nch; ++c) s += it.part[c * mp + e];
+  const int a = e % it.m, b = e / it.m;
+  it.G[a + b * it.ldg] = s;
 }
 
 // ------------------------------------------------------------------ residual norms
@@ -257,49 +266,84 @@ __global__ __launch_bounds__(64) void k_grp_resid_reduce(const void* table) {
 
 // 64-row tile: X rows staged in LDS, C in LDS (as T), each thread one row and
 // 16 output columns, sums over l in ascending order.
+// C (m x p, FP64 column-major) into LDS as T, [l][j]
+template <typename T>
+__device__ void grp_gemm_stage_c(const fe_grp_gemm& it, unsigned char* smem) {
+  constexpr int MC = FE_GRP_MAXCOLS;
+  T* cs = reinterpret_cast<T*>(smem) + 64 * (MC + 1);
+  for (int e = threadIdx.x; e < MC * MC; e += 256) {
+    const int l = e / MC, j = e % MC;
+    cs[e] = (l < it.m && j < it.p) ? static_cast<T>(it.C[l + j * it.ldc]) : T(0);
+  }
+}
+
+// Y[64 rows of tile lt] = X C with C already in LDS; thread -> 4 x 4 block
+// (rows 4 (tid % 16) .., columns 4 (tid / 16) ..), sums over l in order.
 template <typename T>
 __device__ void grp_gemm_tile(const fe_grp_gemm& it, int64_t lt, unsigned char* smem) {
   constexpr int MC = FE_GRP_MAXCOLS;
   T* xs = reinterpret_cast<T*>(smem);           // 64 x (MC + 1)
-  T* cs = xs + 64 * (MC + 1);                   // MC x MC, [l][j]
+  const T* cs = xs + 64 * (MC + 1);             // MC x MC, [l][j]
   const int m = it.m, p = it.p, tid = threadIdx.x;
   const int64_t r0 = lt * 64;
   const int rows = static_cast<int>(it.n - r0 < 64 ? it.n - r0 : 64);
   const T* X = static_cast<const T*>(it.X);
-  __syncthreads();
   for (int e = tid; e < 64 * MC; e += 256) {
     const int r = e / MC, l = e % MC;
     xs[r * (MC + 1) + l] = (r < rows && l < m) ? X[(r0 + r) * it.ldx + l] : T(0);
   }
-  for (int e = tid; e < MC * MC; e += 256) {
-    const int l = e / MC, j = e % MC;
-    cs[e] = (l < m && j < p) ? static_cast<T>(it.C[l + j * it.ldc]) : T(0);
+  __syncthreads();
+  const int rb = 4 * (tid % 16), c0 = 4 * (tid / 16);
+  if (rb < rows && c0 < p) {
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+    for (int l = 0; l < m; ++l) {
+      T xv[4], cv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xv[i] = xs[(rb + i) * (MC + 1) + l];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cv[j] = cs[l * MC + c0 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += xv[i] * cv[j];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (rb + i >= rows) break;
+      T* Y = static_cast<T*>(it.Y) + (r0 + rb + i) * it.ldy;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (c0 + j < p) Y[c0 + j] = acc[i][j];
+    }
   }
   __syncthreads();
-  const int r = tid / 4, j0 = (tid % 4) * 16;
-  if (r >= rows || j0 >= p) return;
-  T acc[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) acc[q] = T(0);
-  for (int l = 0; l < m; ++l) {
-    const T x = xs[r * (MC + 1) + l];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] += x * cs[l * MC + j0 + q];
-  }
-  T* Y = static_cast<T*>(it.Y) + (r0 + r) * it.ldy;
-#pragma unroll
-  for (int q = 0; q < 16; ++q)
-    if (j0 + q < p) Y[j0 + q] = acc[q];
 }
 
+// each workgroup takes 4 consecutive tiles per pass and stages C again only
+// when the item changes
 __global__ __launch_bounds__(256) void k_grp_gemm(const void* table) {
   extern __shared__ __align__(16) unsigned char smem[];
   const fe_grp_header* h = static_cast<const fe_grp_header*>(table);
   const fe_grp_gemm* items = grp_items<fe_grp_gemm>(table);
-  for (int64_t t = blockIdx.x; t < h->ntiles; t += gridDim.x) {
-    const fe_grp_gemm& it = items[grp_find(items, h->nitems, t)];
-    if (it.prec == FE_F64) grp_gemm_tile<double>(it, t - it.tile0, smem);
-    else grp_gemm_tile<float>(it, t - it.tile0, smem);
+  const int64_t groups = (h->ntiles + 3) / 4;
+  for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+    int cur = -1;
+    for (int64_t t = 4 * g; t < 4 * g + 4 && t < h->ntiles; ++t) {
+      const int i = grp_find(items, h->nitems, t);
+      const fe_grp_gemm& it = items[i];
+      if (i != cur) {
+        __syncthreads();
+        if (it.prec == FE_F64) grp_gemm_stage_c<double>(it, smem);
+        else grp_gemm_stage_c<float>(it, smem);
+        cur = i;
+      }
+      if (it.prec == FE_F64) grp_gemm_tile<double>(it, t - it.tile0, smem);
+      else grp_gemm_tile<float>(it, t - it.tile0, smem);
+    }
   }
 }
 
@@ -636,7 +680,8 @@ int fe_grp_spmm_run(fe_ctx ctx, const void* table) {
 int fe_grp_gram_run(fe_ctx ctx, const void* table, int max_items) {
   FE_ARG(ctx && table && max_items >= 1, "fe_grp_gram_run: bad argument");
   hipLaunchKernelGGL(k_grp_gram_part, dim3(GRP_GRID), dim3(256), 0, ctx->stream, table);
-  hipLaunchKernelGGL(k_grp_gram_reduce, dim3(max_items), dim3(256), 0, ctx->stream, table);
+  hipLaunchKernelGGL(k_grp_gram_reduce, dim3(max_items, FE_GRP_MAXCOLS * FE_GRP_MAXCOLS / 256), dim3(256), 0,
+                     ctx->stream, table);
   FE_CHECK_LAUNCH();
   return 0;
 }

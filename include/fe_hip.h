@@ -242,7 +242,7 @@ int fe_fill_random(fe_ctx ctx, int prec, int64_t n, int k, void* X, int64_t ldx,
  * device. Blocks are row-major with 16-byte aligned rows (ld a multiple of
  * 16 / element bytes); small matrices are FP64 column-major. */
 enum { FE_GRP_MAXCOLS = 64,   /* columns of a grouped block (m, p) */
-       FE_GRP_CHUNK = 256 };  /* rows per partial-sum chunk of gram / resid */
+       FE_GRP_CHUNK = 1024 };  /* rows per partial-sum chunk of gram / resid */
 
 typedef struct fe_grp_header {
   int32_t nitems;

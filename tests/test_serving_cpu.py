@@ -39,7 +39,7 @@ def test_footprint_counts_matrix_blocks_and_partials():
     m = 32
     csr = (n + 1) * 4 + nnz * 16
     blocks = 4 * ((n * m * 8 + 255) // 256 * 256)
-    part = ((n + 255) // 256) * m * m * 8
+    part = ((n + 1023) // 1024) * m * m * 8  # chunks of FE_GRP_CHUNK = 1024 rows
     fp = serving.footprint(n, nnz, nev)
     assert csr + blocks + part <= fp <= csr + blocks + part + 80000
     assert serving.footprint(2 * n, 2 * nnz, nev) > 1.9 * fp
@@ -82,11 +82,12 @@ def test_gram_and_gemm_plans_chunk_by_rows():
     L = _fe.lib()
     buf = (C.c_double * 64)()
     base = C.addressof(buf)
-    g = (Gram * 2)(Gram(n=1000, m=16, p=16, prec=_fe.F32, X=base, Y=base, part=base, G=base, ldg=16),
+    # chunks of FE_GRP_CHUNK = 1024 rows (include/fe_hip.h)
+    g = (Gram * 2)(Gram(n=3000, m=16, p=16, prec=_fe.F32, X=base, Y=base, part=base, G=base, ldg=16),
                    Gram(n=256, m=64, p=8, prec=_fe.F64, X=base, Y=base, part=base, G=base, ldg=64))
     h = Header()
     assert L.fe_grp_gram_plan(C.byref(h), g, 2) == 0
-    assert (g[0].tile0, g[1].tile0, h.ntiles) == (0, 4, 5)
+    assert (g[0].tile0, g[1].tile0, h.ntiles) == (0, 3, 4)
     e = (Gemm * 2)(Gemm(n=130, m=8, p=8, prec=_fe.F32, X=base, C=base, ldc=8, Y=base + 512),
                    Gemm(n=64, m=8, p=8, prec=_fe.F32, X=base, C=base, ldc=8, Y=base + 512))
     assert L.fe_grp_gemm_plan(C.byref(h), e, 2) == 0
