@@ -13,6 +13,8 @@
 
 namespace fe {
 
+typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-temporal access
+
 template <typename T, int VEC>
 struct vec_t;
 template <>
@@ -218,9 +220,9 @@ __global__ __launch_bounds__(256) void k_spmm_csr(
       if (Z && gamma != T(0))
 #pragma unroll
         for (int e = 0; e < V; ++e) out[e] += gamma * ze[r][e];
-      float4 o;
+      nt_f4 o;  // Y streams out once: non-temporal, keeps L2 for the gathered X rows
       __builtin_memcpy(&o, out, 16);
-      *reinterpret_cast<float4*>(Y + row * ldy + cr) = o;
+      __builtin_nontemporal_store(o, reinterpret_cast<nt_f4*>(Y + row * ldy + cr));
     } else if (VECTOR && cr + V <= k) {
       if (beta != T(0)) {
         const float4 f = *reinterpret_cast<const float4*>(X + row * ldx + cr);
@@ -236,9 +238,9 @@ __global__ __launch_bounds__(256) void k_spmm_csr(
 #pragma unroll
         for (int e = 0; e < V; ++e) out[e] += gamma * zv[e];
       }
-      float4 o;
+      nt_f4 o;  // Y streams out once: non-temporal, keeps L2 for the gathered X rows
       __builtin_memcpy(&o, out, 16);
-      *reinterpret_cast<float4*>(Y + row * ldy + cr) = o;
+      __builtin_nontemporal_store(o, reinterpret_cast<nt_f4*>(Y + row * ldy + cr));
     } else {
 #pragma unroll
       for (int e = 0; e < V; ++e) {

@@ -18,6 +18,8 @@
 
 namespace fe {
 
+typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-temporal access
+
 // Same lane layout as the CSR kernel: a group of LPR lanes owns one row block
 // and RS 16-byte runs per lane, spaced LPR * 16 bytes apart. The group reads
 // the union entries' columns and values directly (one address per group:
@@ -112,15 +114,17 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, const int64_t* __restric
           for (int q = 0; q < V; ++q) out[q] += beta * xv[q];
         }
         if (Z && gamma != T(0)) {
-          const float4 f = *reinterpret_cast<const float4*>(Z + row * ldz + cs);
+          // Z and Y stream through once: non-temporal, so they do not evict
+          // the gathered X rows from L2
+          const nt_f4 f = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(Z + row * ldz + cs));
           T zv[V];
           __builtin_memcpy(zv, &f, 16);
 #pragma unroll
           for (int q = 0; q < V; ++q) out[q] += gamma * zv[q];
         }
-        float4 o;
+        nt_f4 o;
         __builtin_memcpy(&o, out, 16);
-        *reinterpret_cast<float4*>(Y + row * ldy + cs) = o;
+        __builtin_nontemporal_store(o, reinterpret_cast<nt_f4*>(Y + row * ldy + cs));
       } else {
 #pragma unroll
         for (int q = 0; q < V; ++q) {
