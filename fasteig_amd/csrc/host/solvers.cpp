@@ -231,11 +231,12 @@ FASTEIG_INT ChebyshevSolver::solve() {
       // Mixed precision: FP32 cycles until the FP32 residuals stall or reach the
       // FP32 floor; then the subspace is promoted to FP64 and the remaining
       // cycles (filter, CholQR, Rayleigh-Ritz) run in FP64.
-      // A slow decay far above the FP32 floor is the filter's convergence
-      // rate, not a precision limit: the FP32 cycles continue while the
-      // residual still falls by 1% per cycle there.
+      // A slow decay far above the FP32 floor (relative residuals of 1e-6 to
+      // 1e-5) is the filter's convergence rate, not a precision limit: above
+      // 1e-4 the FP32 cycles continue while the residual still falls by 1%
+      // per cycle; below it a cycle must halve the residual.
       const bool floor_reached = mx <= std::max(tol, 2e-5);
-      const bool stalled = iter >= 2 && mx > (mx <= 1e-3 ? 0.5 : 0.99) * prev_max;
+      const bool stalled = iter >= 2 && mx > (mx <= 1e-4 ? 0.5 : 0.99) * prev_max;
       prev_max = std::min(prev_max, mx);
       if (std::isfinite(last_mx) && last_mx > mx) best_rate = std::max(best_rate, std::log10(last_mx / mx) / degree);
       if (floor_reached || stalled || iter + 1 == maxiter) {

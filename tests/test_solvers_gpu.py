@@ -296,6 +296,23 @@ def test_mixed_precision_reaches_fp64_accuracy():
         assert np.all(np.abs(rm.eigenvalues - r64.eigenvalues) <= 1e-10 * np.abs(r64.eigenvalues))
 
 
+def test_mixed_precision_slow_convergence_stays_in_fp32():
+    """Power-law graph (row-sum shifted): the residual falls slowly but steadily
+    far above the FP32 floor. The mixed controller keeps filtering in FP32
+    there, switches to FP64 near the floor, and reaches the FP64 path's
+    eigenvalues; switching on the slow decay alone (with the FP64 cycles
+    refining only the wanted columns) stalls around 4e-4."""
+    A = G.power_law(20000, seed=5)
+    p = Problem(A)
+    base = dict(solver="Chebyshev", nev=8, blksize=16, degree=20, tol=1e-8, maxiter=800, which="SR", seed=1,
+                cheb_bounds="ritz")
+    r64 = p.solve(**base)
+    rm = p.solve(precision="mixed", **base)
+    assert r64.status == 0 and rm.status == 0
+    assert r64.residual_norms.max() <= 1e-8 and rm.residual_norms.max() <= 1e-8
+    assert np.all(np.abs(rm.eigenvalues - r64.eigenvalues) <= 1e-10 * np.abs(r64.eigenvalues))
+
+
 @pytest.mark.parametrize("which", ["SR", "LR"])
 def test_chebyshev_ritz_bounds_matches_oracle(which):
     A = G.hamiltonian_27pt(16, seed=4, wells=10)
