@@ -51,10 +51,14 @@ def build_matrix(cfg):
     return G.hamiltonian_27pt(32, seed=4, wells=30), dict(nev=32, degree=20, tol=1e-8, blksize=0)
 
 
+KERNEL_OF_FORMAT = {"csr": "k_spmm_csr", "sell": "k_spmm_sell", "blocked": "k_spmm_rb", "tiled": "k_spmm_tile"}
+
+
 def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
     """Average time of the fused Chebyshev step Y = 2c A X + 2e X - Z on the
     library stream (HIP events), and its algorithmic-byte bandwidth. fmt
-    "blocked" times the row-block copy (4-row blocks sharing a column list)."""
+    "blocked" times the row-block copy (4-row blocks sharing a column list),
+    "tiled" the LDS-tiled copy (16-row tiles, pencils through the planes)."""
     from fasteig_amd import _fe
     L = _fe.lib()
     dev = _fe.Device(0)
@@ -68,6 +72,11 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
             raise RuntimeError(f"row-block copy not built (reuse {reuse:.2f})")
         C.block_schedule()  # strip order for grid stencils, as the solvers use it
         C.use(_fe.FMT_BLOCKED)
+    if fmt == "tiled":
+        built, lpr = C.build_tiled()
+        if not built:
+            raise RuntimeError(f"tiled copy not built ({lpr:.2f} X rows loaded per row)")
+        C.use(_fe.FMT_TILED)
     n = A.shape[0]
     dt = np.float64 if prec == _fe.F64 else np.float32
     s = np.dtype(dt).itemsize
@@ -340,8 +349,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c3", "c2", "small", "c4", "c5"])
     ap.add_argument("--gram", default="native", choices=["native", "bf16x3"])
-    ap.add_argument("--format", default="auto", choices=["auto", "csr", "blocked", "sell"],
-                    help="sparse format of the solve (auto: row-block gather when the matrix allows)")
+    ap.add_argument("--format", default="auto", choices=["auto", "csr", "blocked", "tiled", "sell"],
+                    help="sparse format of the solve (auto: LDS-tiled, else row-block, when the matrix allows)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--only-kernels", action="store_true",
@@ -350,8 +359,8 @@ def main():
     ap.add_argument("--kernel-precs", default="fp32,fp64", help="precisions timed by --only-kernels")
     ap.add_argument("--no-extra", action="store_true", help="skip the generalized (C2) and serving (C5) runs")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
-    ap.add_argument("--kernel-formats", default="csr,blocked,blocked8",
-                    help="sparse formats timed by --only-kernels (csr, blocked = 4-row blocks, blocked8)")
+    ap.add_argument("--kernel-formats", default="csr,blocked,tiled",
+                    help="sparse formats timed by --only-kernels (csr, blocked = 4-row blocks, blocked8, tiled)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -433,13 +442,14 @@ def main():
 
     # dominant kernel: fused Chebyshev step at the solve's shape (FP32, k = nev)
     k = cfg["nev"]
-    # the solve's own sparse format: the row-block copy when "auto" finds it pays
-    fmt = "csr" if args.format == "csr" else "blocked"
-    try:
-        per32, algo32 = spmm_roofline(A, k, _fe.F32, fmt=fmt)
-    except RuntimeError:
-        fmt = "csr"
-        per32, algo32 = spmm_roofline(A, k, _fe.F32, fmt=fmt)
+    # the solve's own sparse format: what "auto" picks (tiled, else row-block, else CSR)
+    for fmt in {"auto": ["tiled", "blocked", "csr"], "tiled": ["tiled"], "blocked": ["blocked"],
+                "csr": ["csr"], "sell": ["sell"]}[args.format]:
+        try:
+            per32, algo32 = spmm_roofline(A, k, _fe.F32, fmt=fmt)
+            break
+        except RuntimeError:
+            continue
     per64, algo64 = spmm_roofline(A, k, _fe.F64, fmt=fmt)
     achieved = algo32 / (per32 * 1e-3) / 1e9
     extra["fused_step_fp32_ms"] = per32
@@ -491,7 +501,7 @@ def main():
         "accuracy": out_acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"{'k_spmm_rb' if fmt == 'blocked' else 'k_spmm_csr'} fused Chebyshev step FP32 k={k}",
+                     "kernel": f"{KERNEL_OF_FORMAT[fmt]} fused Chebyshev step FP32 k={k}",
                      "algo_bytes": algo32},
         "cpu_baseline": cpu,
         "extra": extra,

@@ -19,7 +19,7 @@ FE_SO = os.path.join(LIBDIR, "libfe_hip.so")
 
 F64, F32 = 0, 1
 GRAM_NATIVE, GRAM_BF16X3 = 0, 1
-FMT_CSR, FMT_SELL, FMT_BLOCKED = 0, 1, 2
+FMT_CSR, FMT_SELL, FMT_BLOCKED, FMT_TILED = 0, 1, 2, 3
 
 _lib = None
 
@@ -57,6 +57,7 @@ def lib():
         L.fe_csr_use.argtypes = [vp, i32]
         L.fe_csr_build_blocked.argtypes = [vp, vp, i32, C.POINTER(dbl)]
         L.fe_csr_block_schedule.argtypes = [vp, vp, i64, i64]
+        L.fe_csr_build_tiled.argtypes = [vp, vp, C.POINTER(dbl)]
         L.fe_fill_random.argtypes = [vp, i32, i64, i32, vp, i64, C.c_uint64]
         L.fe_scale_rows.argtypes = [vp, i32, i64, i32, vp, i64, vp]
         L.fe_last_error.argtypes = []
@@ -173,6 +174,12 @@ class Csr:
         """Row-block processing order for 3D grid stencils (plane -1: detect,
         0: remove); returns True when an order is in place."""
         return check(lib().fe_csr_block_schedule(self.dev.ctx, self.h, plane, strip)) == 0 and plane != 0
+
+    def build_tiled(self):
+        """Build the LDS-tiled copy; returns (built, X rows loaded per row)."""
+        r = C.c_double(0.0)
+        rc = check(lib().fe_csr_build_tiled(self.dev.ctx, self.h, C.byref(r)))
+        return rc == 0, r.value
 
     def use(self, fmt):
         check(lib().fe_csr_use(self.h, fmt))

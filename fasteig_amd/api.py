@@ -8,7 +8,7 @@
 Options use the reference's fasteig_options keys (nev, which, solver, maxiter,
 degree, blksize, maxblks, tol, seed, verbosity, print_every) plus the HIP
 backend's precision ("fp64" | "mixed"), gram ("native" | "bf16x3"), format
-("csr" | "sell") and device. Everything runs on the GPU; a missing library is
+("auto" | "csr" | "sell" | "blocked" | "tiled") and device. Everything runs on the GPU; a missing library is
 an error, never a CPU fallback.
 """
 from __future__ import annotations

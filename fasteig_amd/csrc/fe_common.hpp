@@ -85,6 +85,22 @@ struct fe_csr_s {
   int* bg_sched = nullptr;     // optional processing order of the row blocks
   int64_t bg_plane = 0, bg_strip = 0;
   int use_bg = 0;
+  // tiled copy (optional): tiles of 16 rows taken in pencils of consecutive
+  // tiles; a workgroup keeps the X rows its tile needs in LDS slots and loads
+  // only the rows the next tile adds (fe_spmm_tile.hip)
+  int64_t tl_ntiles = 0, tl_npencils = 0, tl_loads = 0;
+  int tl_slots = 0;            // LDS row slots per workgroup; slot tl_slots is the zero row
+  int64_t* tl_pencil = nullptr;  // first tile of each pencil (npencils + 1)
+  int* tl_rows = nullptr;        // 16 rows per tile (-1: none)
+  int64_t* tl_rptr = nullptr;    // entries of each tile row (16 ntiles + 1)
+  int* tl_maxe = nullptr;        // largest row entry count of each tile
+  int* tl_eslot = nullptr;       // entry -> slot of its column
+  double* tl_ev64 = nullptr;
+  float* tl_ev32 = nullptr;
+  int64_t* tl_lptr = nullptr;    // X rows each tile loads (ntiles + 1)
+  int* tl_lslot = nullptr;
+  int* tl_lcol = nullptr;
+  int use_tl = 0;
 };
 
 namespace fe {
@@ -112,6 +128,11 @@ inline int fail(int code, const std::string& msg) {
   do {                                                 \
     if (!(cond)) return ::fe::fail(-2, msg);           \
   } while (0)
+
+// Line and plane lengths of a 3D grid stencil in natural order (rows
+// z * plane + y * line + x), from the column offsets of a sample of rows;
+// false for any other structure.
+bool detect_grid(const std::vector<int>& rp, const std::vector<int>& ci, int n, int64_t& line, int64_t& plane);
 
 // Ensure scratch slot `slot` of ctx holds at least `bytes`.
 int scratch(fe_ctx ctx, int slot, size_t bytes, void** out);

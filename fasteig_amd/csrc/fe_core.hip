@@ -451,7 +451,8 @@ int fe_csr_destroy(fe_csr A) {
   if (A->ctx) hipStreamSynchronize(A->ctx->stream);
   void* ptrs[] = {A->rowptr, A->colind, A->vals64, A->vals32, A->sell_ptr, A->sell_len,
                   A->sell_perm, A->sell_col, A->sell_v64, A->sell_v32, A->bg_bptr, A->bg_ucol, A->bg_v64, A->bg_v32,
-                  A->bg_sched};
+                  A->bg_sched, A->tl_pencil, A->tl_rows, A->tl_rptr, A->tl_maxe, A->tl_eslot, A->tl_ev64,
+                  A->tl_ev32, A->tl_lptr, A->tl_lslot, A->tl_lcol};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete A;
@@ -467,8 +468,9 @@ int fe_csr_info(fe_csr A, int* n, int64_t* nnz) {
 
 int fe_csr_scale_sym(fe_ctx ctx, fe_csr A, const double* d) {
   FE_ARG(ctx && A && d, "fe_csr_scale_sym: null argument");
-  FE_ARG(!A->sell_col, "fe_csr_scale_sym: scale before building SELL");
-  // the row-block gather copy shares the CSR values, so it stays valid
+  // the SELL, row-block and tiled copies hold their own values
+  FE_ARG(!A->sell_col && !A->bg_ucol && !A->tl_rows,
+         "fe_csr_scale_sym: scale before building the SELL, row-block or tiled copy");
   hipLaunchKernelGGL(k_scale_sym, dim3(blocks_for(A->n, 256)), dim3(256), 0, ctx->stream, A->n,
                      A->rowptr, A->colind, A->vals64, A->vals32, d);
   FE_CHECK_LAUNCH();
@@ -477,10 +479,12 @@ int fe_csr_scale_sym(fe_ctx ctx, fe_csr A, const double* d) {
 
 int fe_csr_use(fe_csr A, int format) {
   FE_ARG(A, "fe_csr_use: null matrix");
-  FE_ARG(format == FE_FMT_CSR || (format == FE_FMT_SELL && A->sell_col) || (format == FE_FMT_BLOCKED && A->bg_ucol),
+  FE_ARG(format == FE_FMT_CSR || (format == FE_FMT_SELL && A->sell_col) || (format == FE_FMT_BLOCKED && A->bg_ucol) ||
+             (format == FE_FMT_TILED && A->tl_npencils > 0),
          "fe_csr_use: requested copy not built");
   A->use_sell = format == FE_FMT_SELL;
   A->use_bg = format == FE_FMT_BLOCKED;
+  A->use_tl = format == FE_FMT_TILED;
   return 0;
 }
 

@@ -394,6 +394,8 @@ static int launch_spmv(fe_ctx ctx, fe_csr A, const T* X, int64_t ldx, T* Y, int6
 // row-block gather kernel (fe_spmm_bg.hip); needs 16-byte aligned rows
 int spmm_blocked(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
                  double alpha, double beta, const void* Z, int64_t ldz, double gamma);
+int spmm_tiled(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
+               double alpha, double beta, const void* Z, int64_t ldz, double gamma);
 
 // Choose lanes-per-row so that one row group spans the k columns with
 // 16-byte vectors when the layout allows it.
@@ -409,6 +411,7 @@ static int dispatch(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, 
   const T alpha = static_cast<T>(a), beta = static_cast<T>(b), gamma = static_cast<T>(g);
   if (k == 1) return launch_spmv<T>(ctx, A, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma);
   // the row-block kernel handles a partial last run element-wise
+  if (A->use_tl && al) return spmm_tiled(ctx, A, prec_of<T>::value, k, X, ldx, Y, ldy, a, b, Z, ldz, g);
   if (A->use_bg && al) return spmm_blocked(ctx, A, prec_of<T>::value, k, X, ldx, Y, ldy, a, b, Z, ldz, g);
   const T* v = A->use_sell ? nullptr : (sizeof(T) == 8 ? (const T*)A->vals64 : (const T*)A->vals32);
   if (A->use_sell) {

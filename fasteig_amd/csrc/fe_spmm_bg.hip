@@ -139,6 +139,155 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, const int64_t* __restric
   }
 }
 
+__device__ __forceinline__ float lane_value(float v, int u) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), u));
+}
+__device__ __forceinline__ double lane_value(double v, int u) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), u);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), u);
+  return __longlong_as_double((static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo));
+}
+
+// One wavefront per row block (the LPR = 64 layout of k_spmm_rb, same sums in
+// the same order). The gather loop does not wait on index loads: every 64
+// union entries, lane j reads entry j's column and RB values with one vector
+// load each, and the loop takes them from lane u with readlane (uniform, so
+// the gather address is scalar). The gathers of the next UNR entries are
+// issued before the FMAs of the current UNR, so a wavefront keeps two groups
+// of row loads in flight.
+template <typename T, int RB, int RS, int UNR>
+__global__ __launch_bounds__(256) void k_spmm_rbw(int n, const int64_t* __restrict__ bptr,
+                                                  const int* __restrict__ ucol, const T* __restrict__ bval, int k,
+                                                  const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha,
+                                                  T beta, const T* Z, int64_t ldz, T gamma, int64_t per_xcd,
+                                                  int64_t ntiles, int64_t nblk, const int* __restrict__ sched) {
+  constexpr int V = 16 / (int)sizeof(T);
+  constexpr int STRIDE = 64 * V;
+  constexpr int COLS = STRIDE * RS;
+  constexpr int WV = RB * (int)sizeof(T) / 16;  // 16-byte loads per entry's values
+  const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
+  if (tile >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t slot = tile * 4 + wave;
+  if (slot >= nblk) return;  // wavefront-uniform; the kernel has no barrier
+  const int64_t blk = sched ? static_cast<int64_t>(sched[slot]) : slot;
+  const int c = blockIdx.y * COLS + lane * V;
+  const bool full = c + (RS - 1) * STRIDE + V <= k;
+  const int64_t e0 = bptr[blk], e1 = bptr[blk + 1];
+  T acc[RB][RS][V];
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int s = 0; s < RS; ++s)
+#pragma unroll
+      for (int e = 0; e < V; ++e) acc[r][s][e] = T(0);
+  T xa[UNR][RS][V], xb[UNR][RS][V];
+  for (int64_t ch = e0; ch < e1; ch += 64) {
+    const int cnt = static_cast<int>(e1 - ch < 64 ? e1 - ch : 64);
+    int myc = 0;
+    T myw[RB];
+    if (lane < cnt) {
+      myc = ucol[ch + lane];
+#pragma unroll
+      for (int j = 0; j < WV; ++j) {
+        const float4 f = reinterpret_cast<const float4*>(bval + (ch + lane) * RB)[j];
+        __builtin_memcpy(&myw[j * (16 / (int)sizeof(T))], &f, 16);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RB; ++r) myw[r] = T(0);
+    }
+    auto load = [&](T (&x)[UNR][RS][V], int g) {
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        if (g + u < cnt) {
+          const T* xp = X + static_cast<int64_t>(__builtin_amdgcn_readlane(myc, g + u)) * ldx + c;
+#pragma unroll
+          for (int s = 0; s < RS; ++s) {
+            const int cs = c + s * STRIDE;
+            if (full || cs + V <= k) {
+              const float4 f = *reinterpret_cast<const float4*>(xp + s * STRIDE);
+              __builtin_memcpy(&x[u][s][0], &f, 16);
+            } else {
+#pragma unroll
+              for (int q = 0; q < V; ++q) x[u][s][q] = (cs + q < k) ? xp[s * STRIDE + q] : T(0);
+            }
+          }
+        }
+      }
+    };
+    auto fma = [&](const T (&x)[UNR][RS][V], int g) {
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        if (g + u < cnt) {
+#pragma unroll
+          for (int r = 0; r < RB; ++r) {
+            // the value is wavefront-uniform: a zero (a row of the block
+            // without this column) skips its FMAs with a scalar branch
+            const T wr = lane_value(myw[r], g + u);
+            if (wr != T(0)) {
+#pragma unroll
+              for (int s = 0; s < RS; ++s)
+#pragma unroll
+                for (int q = 0; q < V; ++q) acc[r][s][q] += wr * x[u][s][q];
+            }
+          }
+        }
+      }
+    };
+    load(xa, 0);
+    for (int g = 0; g < cnt; g += 2 * UNR) {
+      if (g + UNR < cnt) load(xb, g + UNR);
+      fma(xa, g);
+      if (g + 2 * UNR < cnt) load(xa, g + 2 * UNR);
+      if (g + UNR < cnt) fma(xb, g + UNR);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RB; ++r) {
+    const int64_t row = blk * RB + r;
+    if (row >= n) break;
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+      const int cs = c + s * STRIDE;
+      if (cs >= k) continue;
+      T out[V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) out[q] = alpha * acc[r][s][q];
+      if (cs + V <= k) {
+        if (beta != T(0)) {
+          const float4 f = *reinterpret_cast<const float4*>(X + row * ldx + cs);
+          T xv[V];
+          __builtin_memcpy(xv, &f, 16);
+#pragma unroll
+          for (int q = 0; q < V; ++q) out[q] += beta * xv[q];
+        }
+        if (Z && gamma != T(0)) {
+          const nt_f4 f = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(Z + row * ldz + cs));
+          T zv[V];
+          __builtin_memcpy(zv, &f, 16);
+#pragma unroll
+          for (int q = 0; q < V; ++q) out[q] += gamma * zv[q];
+        }
+        nt_f4 o;
+        __builtin_memcpy(&o, out, 16);
+        __builtin_nontemporal_store(o, reinterpret_cast<nt_f4*>(Y + row * ldy + cs));
+      } else {
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          if (cs + q >= k) continue;
+          T o = out[q];
+          if (beta != T(0)) o += beta * X[row * ldx + cs + q];
+          if (Z && gamma != T(0)) o += gamma * Z[row * ldz + cs + q];
+          Y[row * ldy + cs + q] = o;
+        }
+      }
+    }
+  }
+}
+
 template <typename T, int RB, int LPR, int RS>
 static int launch_rb(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
                      const T* Z, int64_t ldz, T gamma) {
@@ -149,9 +298,20 @@ static int launch_rb(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y,
   const int64_t nb8 = (ntiles + 7) / 8 * 8;
   const unsigned ycols = static_cast<unsigned>((k + COLS - 1) / COLS);
   const T* bv = sizeof(T) == 8 ? (const T*)A->bg_v64 : (const T*)A->bg_v32;
-  hipLaunchKernelGGL((k_spmm_rb<T, RB, LPR, RS>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream, A->n,
-                     A->bg_bptr, A->bg_ucol, bv, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles,
-                     A->bg_nblk, A->bg_sched);
+  if constexpr (LPR == 64) {
+#ifndef FE_RBW_UNR
+    constexpr int UNR = RS == 1 ? 8 : 4;  // two groups of UNR x RS 16-byte loads: 64 VGPRs
+#else
+    constexpr int UNR = FE_RBW_UNR;
+#endif
+    hipLaunchKernelGGL((k_spmm_rbw<T, RB, RS, UNR>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream, A->n,
+                       A->bg_bptr, A->bg_ucol, bv, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles,
+                       A->bg_nblk, A->bg_sched);
+  } else {
+    hipLaunchKernelGGL((k_spmm_rb<T, RB, LPR, RS>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream, A->n,
+                       A->bg_bptr, A->bg_ucol, bv, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles,
+                       A->bg_nblk, A->bg_sched);
+  }
   FE_CHECK_LAUNCH();
   return 0;
 }
@@ -251,12 +411,10 @@ extern "C" int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse)
   return 0;
 }
 
-namespace {
-// Line and plane lengths of a 3D grid stencil in natural order, from the
-// distinct positive column offsets j - i of a sample of rows: three clusters
-// (within a line, to the neighbouring lines, to the neighbouring planes) whose
-// centres are 1, the line length and the plane length. Returns false for any
-// other structure.
+namespace fe {
+// The distinct positive column offsets j - i of a sample of rows form three
+// clusters (within a line, to the neighbouring lines, to the neighbouring
+// planes) whose centres are 1, the line length and the plane length.
 bool detect_grid(const std::vector<int>& rp, const std::vector<int>& ci, int n, int64_t& line, int64_t& plane) {
   std::vector<int64_t> off;
   const int step = std::max(1, n / 4096);
@@ -276,7 +434,7 @@ bool detect_grid(const std::vector<int>& rp, const std::vector<int>& ci, int n, 
   plane = (cl[2].first + cl[2].second) / 2;
   return line >= 4 && plane >= 4 * line && plane % line == 0 && n >= 3 * plane;
 }
-}  // namespace
+}  // namespace fe
 
 extern "C" int fe_csr_block_schedule(fe_ctx ctx, fe_csr A, int64_t plane, int64_t strip) {
   FE_ARG(ctx && A, "fe_csr_block_schedule: null argument");

@@ -1,0 +1,508 @@
+// Tiled SpMM: Y = alpha A X + beta X + gamma Z with the X rows of a tile of
+// rows staged in LDS.
+//
+// The rows are grouped in tiles of 16 and the tiles in pencils (runs of
+// consecutive tiles). A workgroup takes one pencil and one column slice of
+// 256 bytes (64 FP32 or 32 FP64 columns) and walks the pencil's tiles in
+// order. LDS holds one 256-byte slot per X row the current tile needs (the
+// union of its rows' columns); a slot stays filled while the next tiles still
+// need its row, so a tile loads only the rows it adds. The slot plan (which
+// row goes to which slot, tile by tile) is computed on the host when the copy
+// is built, so the kernel follows lists: the X rows to load for each tile and,
+// per nonzero, the slot of its column.
+//
+// For a 3D grid stencil the tiles are 4 x 4 patches of a plane and a pencil
+// runs through consecutive planes: a 27-point tile needs 108 X rows and the
+// next plane's tile adds 36 of them, 2.25 row loads per output row against
+// 13.5 gathers per row for the 4-row block product and 27 for CSR. The X
+// traffic through the vector memory path falls by that factor; the products
+// read X from LDS instead. Other matrices take tiles of 16 consecutive rows
+// and pencils of 16 tiles, and the copy is built only when that saves loads.
+//
+// Each row's sum runs over its CSR entries in CSR order, as the CSR kernel's
+// does (bitwise the same on a matrix with sorted rows). A tile row shorter
+// than the tile's longest row pads with a reserved slot that holds zeros.
+#include "fe_common.hpp"
+
+#include <algorithm>
+#include <type_traits>
+
+namespace fe {
+
+namespace {
+
+typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-temporal access
+
+constexpr int TL_ROWS = 16;   // rows per tile: 4 wavefronts x 4 rows of 16 lanes
+constexpr int TL_PF = 4;      // 16-byte X loads per thread held in registers across a tile
+constexpr int TL_PENCIL = 16;  // tiles per pencil at most
+constexpr int TL_MAX_SLOTS = 255;  // LDS slots (+1 zero slot): at most 64 KB per workgroup
+
+// Lane j of each 16-lane row of the wavefront, broadcast to the row (DPP
+// row_newbcast).
+template <int J>
+__device__ __forceinline__ int row_bcast(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + J, 0xF, 0xF, true);
+}
+template <int J>
+__device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(row_bcast<J>(__float_as_int(v)));
+}
+template <int J>
+__device__ __forceinline__ double row_bcast(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = row_bcast<J>(static_cast<int>(b));
+  const int hi = row_bcast<J>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo));
+}
+
+// Element (row, col) of a row-major block: with O32 the byte offset fits in
+// 32 bits (the host checks the block's extent), which keeps the address
+// arithmetic 32-bit on top of the base pointer.
+template <bool O32, typename P>
+__device__ __forceinline__ P* elem(P* base, int row, int64_t ld, int col) {
+  using B = std::conditional_t<std::is_const_v<P>, const char, char>;
+  if constexpr (O32)
+    return reinterpret_cast<P*>(reinterpret_cast<B*>(base) +
+                                (static_cast<uint32_t>(row) * static_cast<uint32_t>(ld) + static_cast<uint32_t>(col)) *
+                                    static_cast<uint32_t>(sizeof(P)));
+  else
+    return base + static_cast<int64_t>(row) * ld + col;
+}
+
+template <int J, int END>
+struct Unroll {
+  template <typename F>
+  __device__ __forceinline__ static void run(F& f) {
+    f(std::integral_constant<int, J>{});
+    Unroll<J + 1, END>::run(f);
+  }
+};
+template <int END>
+struct Unroll<END, END> {
+  template <typename F>
+  __device__ __forceinline__ static void run(F&) {}
+};
+
+}  // namespace
+
+// Workgroup: 256 threads; thread tid owns tile row tid / 16 and the 16 bytes
+// at offset 16 * (tid % 16) of the slice. Work item w = pencil * nslices +
+// slice, so the slices of one pencil run side by side on one XCD and share
+// its entry lists through L2.
+//
+// Every global load a tile needs is issued one tile ahead, so its latency
+// overlaps the previous tile's products: the tile's entries (two chunks of
+// 16 per row), its rows of Z and X for the fused terms, and the new X rows
+// for LDS; the load lists of those X rows are read two tiles ahead. The
+// pencil's row, entry and load offsets are staged in LDS first.
+template <typename T, bool O32>
+__global__ __launch_bounds__(256) void k_spmm_tile(
+    const int64_t* __restrict__ pencil, const int* __restrict__ rows, const int64_t* __restrict__ rptr,
+    const int* __restrict__ maxe, const int* __restrict__ eslot, const T* __restrict__ ev,
+    const int64_t* __restrict__ lptr, const int* __restrict__ lslot, const int* __restrict__ lcol, int zslot, int k,
+    int nslices, const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta, const T* Z, int64_t ldz,
+    T gamma, int64_t per_xcd, int64_t nwork) {
+  extern __shared__ float4 xs[];  // (zslot + 1) slots of 16 x 16 bytes
+  __shared__ int s_lp[TL_PENCIL + 1];             // load offsets of the pencil's tiles
+  __shared__ int s_rp[TL_PENCIL * TL_ROWS + 1];   // entry offsets of its tile rows
+  __shared__ int s_rows[TL_PENCIL * TL_ROWS];     // its tile rows
+  __shared__ int s_maxe[TL_PENCIL];
+  constexpr int V = 16 / (int)sizeof(T);
+  constexpr int CW = 16 * V;  // columns per slice
+  const int64_t work = xcd_tile(blockIdx.x, per_xcd);
+  if (work >= nwork) return;  // the whole workgroup, before any barrier
+  const int64_t p = work / nslices;
+  const int c0 = static_cast<int>(work - p * nslices) * CW;
+  const int tid = threadIdx.x;
+  const int sub = tid & 15;
+  const int tr = tid >> 4;
+  const int c = c0 + sub * V;  // this thread's first column
+  const bool fz = Z && gamma != T(0);
+  const bool fx = beta != T(0);
+  const bool full = c0 + CW <= k;  // the whole slice lies inside the block
+  const int64_t t0 = pencil[p];
+  const int nt = static_cast<int>(pencil[p + 1] - t0);  // at most TL_PENCIL
+  const int64_t lbase = lptr[t0], ebase = rptr[t0 * TL_ROWS];
+  for (int i = tid; i <= nt; i += 256) s_lp[i] = static_cast<int>(lptr[t0 + i] - lbase);
+  for (int i = tid; i <= nt * TL_ROWS; i += 256) s_rp[i] = static_cast<int>(rptr[t0 * TL_ROWS + i] - ebase);
+  for (int i = tid; i < nt * TL_ROWS; i += 256) s_rows[i] = rows[t0 * TL_ROWS + i];
+  for (int i = tid; i < nt; i += 256) s_maxe[i] = maxe[t0 + i];
+  if (tid < 16) xs[zslot * 16 + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int* ls = lslot + lbase;
+  const int* lc = lcol + lbase;
+  const int* es = eslot + ebase;
+  const T* ew = ev + ebase;
+
+  auto load_x = [&](int col, int part) {
+    const int cc = c0 + part * V;
+    const T* src = elem<O32>(X, col, ldx, cc);
+    float4 f;
+    if (full || cc + V <= k) {
+      f = *reinterpret_cast<const float4*>(src);
+    } else {
+      T tmp[V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) tmp[q] = cc + q < k ? src[q] : T(0);
+      __builtin_memcpy(&f, tmp, 16);
+    }
+    return f;
+  };
+  // this thread's 16 bytes of row `row` of a block B (zeros outside k)
+  auto load_row = [&](const T* B, int64_t ld, int row, T (&v)[V]) {
+    if (row < 0 || c >= k) {
+#pragma unroll
+      for (int q = 0; q < V; ++q) v[q] = T(0);
+    } else if (full || c + V <= k) {
+      const nt_f4 f = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(elem<O32>(B, row, ld, c)));
+      __builtin_memcpy(v, &f, 16);
+    } else {
+#pragma unroll
+      for (int q = 0; q < V; ++q) v[q] = c + q < k ? elem<O32>(B, row, ld, c)[q] : T(0);
+    }
+  };
+  // per-tile registers, filled one tile ahead
+  struct Ahead {
+    int ms[2];
+    T mw[2];
+    T z[V], x[V];
+  };
+  auto fetch = [&](int tt, Ahead& a) {
+    const int e0 = s_rp[tt * TL_ROWS + tr];
+    const int cnt = s_rp[tt * TL_ROWS + tr + 1] - e0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      a.ms[h] = zslot;
+      a.mw[h] = T(0);
+      if (16 * h + sub < cnt) {
+        a.ms[h] = es[e0 + 16 * h + sub];
+        a.mw[h] = ew[e0 + 16 * h + sub];
+      }
+    }
+    const int row = s_rows[tt * TL_ROWS + tr];
+    if (fz) load_row(Z, ldz, row, a.z);
+    if (fx) load_row(X, ldx, row, a.x);
+  };
+  // load list entries (slot, column) of tile tt for this thread's TL_PF loads
+  auto fetch_list = [&](int tt, int (&sl)[TL_PF], int (&cl)[TL_PF]) {
+    const int l0 = tt < nt ? s_lp[tt] : 0;
+    const int nl = tt < nt ? s_lp[tt + 1] - l0 : 0;
+#pragma unroll
+    for (int j = 0; j < TL_PF; ++j) {
+      const int i = tid + 256 * j;
+      sl[j] = -1;
+      cl[j] = 0;
+      if (i < nl * 16) {
+        sl[j] = ls[l0 + (i >> 4)] * 16 + (i & 15);
+        cl[j] = lc[l0 + (i >> 4)];
+      }
+    }
+  };
+  __syncthreads();
+  // the first tile's X rows
+  for (int i = tid; i < s_lp[1] * 16; i += 256) xs[ls[i >> 4] * 16 + (i & 15)] = load_x(lc[i >> 4], i & 15);
+  Ahead cur, nxt;
+  fetch(0, cur);
+  int sl1[TL_PF], cl1[TL_PF];
+  fetch_list(1, sl1, cl1);
+  __syncthreads();
+  for (int tt = 0; tt < nt; ++tt) {
+    // new X rows of tile tt + 1 (list read one tile earlier) and the list of tile tt + 2
+    float4 pre[TL_PF];
+#pragma unroll
+    for (int j = 0; j < TL_PF; ++j)
+      if (sl1[j] >= 0) pre[j] = load_x(cl1[j], sl1[j] & 15);
+    int sl2[TL_PF], cl2[TL_PF];
+    fetch_list(tt + 2, sl2, cl2);
+    if (tt + 1 < nt) fetch(tt + 1, nxt);
+    // products of tile tt
+    const int lim = s_maxe[tt];
+    T acc[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) acc[q] = T(0);
+    for (int ch = 0; ch < lim; ch += 16) {
+      int ms;
+      T mw;
+      if (ch < 32) {
+        ms = ch == 0 ? cur.ms[0] : cur.ms[1];
+        mw = ch == 0 ? cur.mw[0] : cur.mw[1];
+      } else {  // rows longer than 32 entries: later chunks read here
+        const int e0 = s_rp[tt * TL_ROWS + tr];
+        const int cnt = s_rp[tt * TL_ROWS + tr + 1] - e0;
+        ms = zslot;
+        mw = T(0);
+        if (ch + sub < cnt) {
+          ms = es[e0 + ch + sub];
+          mw = ew[e0 + ch + sub];
+        }
+      }
+      // lim is a multiple of 4 (the host pads the count): whole chunks run
+      // without guards, the last one in guarded groups of 4 entries
+      auto step = [&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const int s = row_bcast<j>(ms);
+        const T w = row_bcast<j>(mw);
+        const float4 f = xs[s * 16 + sub];
+        T xv[V];
+        __builtin_memcpy(xv, &f, 16);
+#pragma unroll
+        for (int q = 0; q < V; ++q) acc[q] += w * xv[q];
+      };
+      if (ch + 16 <= lim) {
+        Unroll<0, 16>::run(step);
+      } else {
+        auto group = [&](auto G) {
+          constexpr int g = decltype(G)::value;
+          if (ch + 4 * g < lim) Unroll<4 * g, 4 * g + 4>::run(step);
+        };
+        Unroll<0, 4>::run(group);
+      }
+    }
+    const int row = s_rows[tt * TL_ROWS + tr];
+    if (row >= 0 && c < k) {
+      T out[V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        out[q] = alpha * acc[q];
+        if (fx) out[q] += beta * cur.x[q];
+        if (fz) out[q] += gamma * cur.z[q];
+      }
+      T* y = elem<O32>(Y, row, ldy, c);
+      if (full || c + V <= k) {
+        nt_f4 o;
+        __builtin_memcpy(&o, out, 16);
+        __builtin_nontemporal_store(o, reinterpret_cast<nt_f4*>(y));
+      } else {
+#pragma unroll
+        for (int q = 0; q < V; ++q)
+          if (c + q < k) y[q] = out[q];
+      }
+    }
+    __syncthreads();  // every thread is done with tile tt's slots
+#pragma unroll
+    for (int j = 0; j < TL_PF; ++j)
+      if (sl1[j] >= 0) xs[sl1[j]] = pre[j];
+    if (tt + 1 < nt) {  // rows past the TL_PF registers per thread
+      const int l0 = s_lp[tt + 1], nl = s_lp[tt + 2] - l0;
+      for (int i = tid + 256 * TL_PF; i < nl * 16; i += 256)
+        xs[ls[l0 + (i >> 4)] * 16 + (i & 15)] = load_x(lc[l0 + (i >> 4)], i & 15);
+    }
+    __syncthreads();
+    cur = nxt;
+#pragma unroll
+    for (int j = 0; j < TL_PF; ++j) {
+      sl1[j] = sl2[j];
+      cl1[j] = cl2[j];
+    }
+  }
+}
+
+template <typename T>
+static int launch_tile(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
+                       const T* Z, int64_t ldz, T gamma) {
+  constexpr int CW = 16 * (16 / (int)sizeof(T));
+  const int nslices = (k + CW - 1) / CW;
+  const int64_t nwork = A->tl_npencils * nslices;
+  const int64_t nb8 = (nwork + 7) / 8 * 8;
+  const size_t lds = static_cast<size_t>(A->tl_slots + 1) * 256;
+  const T* ev = sizeof(T) == 8 ? (const T*)A->tl_ev64 : (const T*)A->tl_ev32;
+  // 32-bit offsets when every block spans less than 4 GiB
+  const double lim = 4294967295.0 / sizeof(T);
+  const bool o32 = static_cast<double>(A->n) * ldx < lim && static_cast<double>(A->n) * ldy < lim &&
+                   (!Z || static_cast<double>(A->n) * ldz < lim);
+  auto kern = o32 ? k_spmm_tile<T, true> : k_spmm_tile<T, false>;
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nb8)), dim3(256), lds, ctx->stream, A->tl_pencil, A->tl_rows,
+                     A->tl_rptr, A->tl_maxe, A->tl_eslot, ev, A->tl_lptr, A->tl_lslot, A->tl_lcol, A->tl_slots, k,
+                     nslices, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, nwork);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+// Called by fe_spmm for 16-byte aligned blocks when the tiled copy is active.
+// fe_csr_use selects it only once the copy is complete (tl_npencils > 0).
+int spmm_tiled(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
+               double alpha, double beta, const void* Z, int64_t ldz, double gamma) {
+  if (prec == FE_F64)
+    return launch_tile<double>(ctx, A, k, (const double*)X, ldx, (double*)Y, ldy, alpha, beta, (const double*)Z,
+                               ldz, gamma);
+  return launch_tile<float>(ctx, A, k, (const float*)X, ldx, (float*)Y, ldy, (float)alpha, (float)beta,
+                            (const float*)Z, ldz, (float)gamma);
+}
+
+}  // namespace fe
+
+using namespace fe;
+
+namespace {
+
+// Host plan of the tiled copy.
+struct TilePlan {
+  std::vector<int64_t> pencil{0};
+  std::vector<int> rows;
+  std::vector<int64_t> rptr{0};
+  std::vector<int> maxe, eslot;
+  std::vector<double> ev;
+  std::vector<int64_t> lptr{0};
+  std::vector<int> lslot, lcol;
+  int slots = 0;
+};
+
+// Tiles of the matrix in processing order, grouped in pencils; -1 pads a tile.
+void make_tiles(int n, int64_t line, int64_t plane, std::vector<std::vector<int>>& tiles,
+                std::vector<int64_t>& pencil) {
+  if (plane > 0 && line % 4 == 0 && (plane / line) % 4 == 0 && n % plane == 0) {
+    // 4 x 4 patches; a pencil runs through up to 16 consecutive planes; the
+    // pencils of one range of planes are taken row of patches by row, so an
+    // XCD's contiguous share of pencils is a band of neighbouring patches
+    const int64_t nx = line / 4, ny = plane / line / 4, nz = n / plane, pz = TL_PENCIL;
+    for (int64_t z0 = 0; z0 < nz; z0 += pz)
+      for (int64_t py = 0; py < ny; ++py)
+        for (int64_t px = 0; px < nx; ++px) {
+          for (int64_t z = z0; z < std::min(nz, z0 + pz); ++z) {
+            std::vector<int> t;
+            for (int dy = 0; dy < 4; ++dy)
+              for (int dx = 0; dx < 4; ++dx)
+                t.push_back(static_cast<int>(z * plane + (4 * py + dy) * line + 4 * px + dx));
+            tiles.push_back(std::move(t));
+          }
+          pencil.push_back(static_cast<int64_t>(tiles.size()));
+        }
+    return;
+  }
+  const int64_t per_pencil = TL_PENCIL;
+  for (int64_t r0 = 0; r0 < n; r0 += TL_ROWS) {
+    std::vector<int> t;
+    for (int i = 0; i < TL_ROWS; ++i) t.push_back(r0 + i < n ? static_cast<int>(r0 + i) : -1);
+    tiles.push_back(std::move(t));
+    if (static_cast<int64_t>(tiles.size()) % per_pencil == 0) pencil.push_back(static_cast<int64_t>(tiles.size()));
+  }
+  if (pencil.back() != static_cast<int64_t>(tiles.size())) pencil.push_back(static_cast<int64_t>(tiles.size()));
+}
+
+// Slot plan: per pencil, a tile keeps the slots of the rows it shares with
+// the previous tile and fills freed slots with the rows it adds.
+bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, const std::vector<double>& v,
+                const std::vector<std::vector<int>>& tiles, const std::vector<int64_t>& pencil, TilePlan& P) {
+  std::vector<int> slot_of(static_cast<size_t>(n), -1), stamp(static_cast<size_t>(n), -1);
+  std::vector<int> uni, resident, freelist, kept;
+  int slots = 0;
+  for (const auto& t : tiles) {  // slots = largest tile union
+    uni.clear();
+    for (int r : t)
+      if (r >= 0) uni.insert(uni.end(), ci.begin() + rp[r], ci.begin() + rp[r + 1]);
+    std::sort(uni.begin(), uni.end());
+    slots = std::max<int>(slots, static_cast<int>(std::unique(uni.begin(), uni.end()) - uni.begin()));
+  }
+  if (slots > TL_MAX_SLOTS) return false;
+  P.slots = slots;
+  P.pencil = pencil;
+  for (size_t pi = 0; pi + 1 < pencil.size(); ++pi) {
+    for (int col : resident) slot_of[col] = -1;
+    resident.clear();
+    freelist.clear();
+    for (int s = slots - 1; s >= 0; --s) freelist.push_back(s);
+    for (int64_t ti = pencil[pi]; ti < pencil[pi + 1]; ++ti) {
+      const auto& t = tiles[static_cast<size_t>(ti)];
+      uni.clear();
+      for (int r : t)
+        if (r >= 0) uni.insert(uni.end(), ci.begin() + rp[r], ci.begin() + rp[r + 1]);
+      std::sort(uni.begin(), uni.end());
+      uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+      for (int col : uni) stamp[col] = static_cast<int>(ti);
+      kept.clear();
+      for (int col : resident) {
+        if (stamp[col] == static_cast<int>(ti)) {
+          kept.push_back(col);
+        } else {
+          freelist.push_back(slot_of[col]);
+          slot_of[col] = -1;
+        }
+      }
+      resident.swap(kept);
+      for (int col : uni) {
+        if (slot_of[col] >= 0) continue;
+        slot_of[col] = freelist.back();
+        freelist.pop_back();
+        resident.push_back(col);
+        P.lslot.push_back(slot_of[col]);
+        P.lcol.push_back(col);
+      }
+      P.lptr.push_back(static_cast<int64_t>(P.lslot.size()));
+      int me = 0;
+      for (int r : t) {
+        P.rows.push_back(r);
+        if (r >= 0) {
+          for (int q = rp[r]; q < rp[r + 1]; ++q) {
+            P.eslot.push_back(slot_of[ci[q]]);
+            P.ev.push_back(v[q]);
+          }
+          me = std::max(me, rp[r + 1] - rp[r]);
+        }
+        P.rptr.push_back(static_cast<int64_t>(P.eslot.size()));
+      }
+      P.maxe.push_back((me + 3) / 4 * 4);  // padded entries read the zero slot
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
+  FE_ARG(ctx && A, "fe_csr_build_tiled: null argument");
+  if (A->tl_npencils > 0) {
+    if (loads_per_row) *loads_per_row = A->n ? static_cast<double>(A->tl_loads) / A->n : 0.0;
+    return 0;
+  }
+  const int n = A->n;
+  if (n == 0) return 1;
+  std::vector<int> rp(n + 1), ci(A->nnz);
+  std::vector<double> v(A->nnz);
+  FE_HIP_TRY(hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+  if (A->nnz) {
+    FE_HIP_TRY(hipMemcpyAsync(ci.data(), A->colind, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    FE_HIP_TRY(hipMemcpyAsync(v.data(), A->vals64, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  int64_t line = 0, plane = 0;
+  if (!detect_grid(rp, ci, n, line, plane)) line = plane = 0;
+  std::vector<std::vector<int>> tiles;
+  std::vector<int64_t> pencil{0};
+  make_tiles(n, line, plane, tiles, pencil);
+  TilePlan P;
+  if (!plan_slots(n, rp, ci, v, tiles, pencil, P)) return 1;
+  const double lpr = static_cast<double>(P.lslot.size()) / n;
+  if (loads_per_row) *loads_per_row = lpr;
+  // worth it only when the loads per row fall below half the nonzeros per row
+  if (lpr > 0.5 * static_cast<double>(A->nnz) / n) return 1;
+  std::vector<float> ev32(P.ev.size());
+  for (size_t i = 0; i < P.ev.size(); ++i) ev32[i] = static_cast<float>(P.ev[i]);
+  auto up = [&](auto** dst, const auto& src) -> int {
+    using E = typename std::decay_t<decltype(src)>::value_type;
+    const size_t bytes = sizeof(E) * std::max<size_t>(src.size(), 1);
+    FE_HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), bytes));
+    if (!src.empty())
+      FE_HIP_TRY(hipMemcpyAsync(*dst, src.data(), sizeof(E) * src.size(), hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+  };
+  int rc = 0;
+  if ((rc = up(&A->tl_pencil, P.pencil)) || (rc = up(&A->tl_rows, P.rows)) || (rc = up(&A->tl_rptr, P.rptr)) ||
+      (rc = up(&A->tl_maxe, P.maxe)) || (rc = up(&A->tl_eslot, P.eslot)) || (rc = up(&A->tl_ev64, P.ev)) ||
+      (rc = up(&A->tl_ev32, ev32)) || (rc = up(&A->tl_lptr, P.lptr)) || (rc = up(&A->tl_lslot, P.lslot)) ||
+      (rc = up(&A->tl_lcol, P.lcol)) || (rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : -3)) {
+    hipStreamSynchronize(ctx->stream);
+    void** arrays[] = {(void**)&A->tl_pencil, (void**)&A->tl_rows,  (void**)&A->tl_rptr, (void**)&A->tl_maxe,
+                       (void**)&A->tl_eslot,  (void**)&A->tl_ev64,  (void**)&A->tl_ev32, (void**)&A->tl_lptr,
+                       (void**)&A->tl_lslot,  (void**)&A->tl_lcol};
+    for (void** a : arrays) {
+      if (*a) hipFree(*a);
+      *a = nullptr;
+    }
+    return rc < 0 ? rc : fail(-3, "fe_csr_build_tiled: upload failed");
+  }
+  A->tl_ntiles = static_cast<int64_t>(tiles.size());
+  A->tl_npencils = static_cast<int64_t>(P.pencil.size()) - 1;
+  A->tl_loads = static_cast<int64_t>(P.lslot.size());
+  A->tl_slots = P.slots;
+  return 0;
+}

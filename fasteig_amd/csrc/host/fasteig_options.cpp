@@ -89,8 +89,8 @@ bool fasteig_options::set(std::string_view key, std::string_view value) {
     gram_ = v;
   } else if (k == "format") {
     const std::string v = lower(value);
-    if (v != "csr" && v != "sell" && v != "blocked" && v != "auto")
-      throw std::invalid_argument("format must be 'csr', 'sell', 'blocked' or 'auto'");
+    if (v != "csr" && v != "sell" && v != "blocked" && v != "tiled" && v != "auto")
+      throw std::invalid_argument("format must be 'csr', 'sell', 'blocked', 'tiled' or 'auto'");
     format_ = v;
   } else if (k == "cheb_bounds") {
     const std::string v = lower(value);

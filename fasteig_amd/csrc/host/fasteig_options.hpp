@@ -6,8 +6,10 @@
 //   gram       "native" (FP64/FP32 MFMA) | "bf16x3" (split-BF16 Gram, FP32 blocks)
 //   format     "csr" | "sell" (SELL-32-sigma sparse product) | "blocked"
 //              (4-row blocks sharing one column list: one X gather per column of
-//              a block instead of one per nonzero) |
-//              "auto" (blocked when the matrix has the column locality, else csr)
+//              a block instead of one per nonzero) | "tiled" (16-row tiles whose
+//              X rows are staged in LDS, loading only the rows each tile adds) |
+//              "auto" (tiled, else blocked, when the matrix has the column
+//              locality, else csr)
 //   device     HIP device index
 //   cheb_bounds "amplification" (the reference's fixed cutoff) | "ritz" (the
 //              damped interval starts at the largest Ritz value of the current

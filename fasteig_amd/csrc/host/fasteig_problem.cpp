@@ -137,8 +137,12 @@ fasteig::hip::HipCsrMatrix& fasteig_problem::device_matrix(const fasteig_options
   const std::string& f = opts.format();
   if (f == "sell") {
     dA_->use_sell(32, 256);
+  } else if (f == "tiled") {
+    if (!dA_->use_tiled())
+      throw std::invalid_argument("format=tiled: the matrix has too little column locality for the tiled copy");
   } else if (f == "blocked" || f == "auto") {
-    if (!dA_->use_blocked()) {
+    // auto: the LDS-tiled copy, else the row-block copy, else CSR
+    if ((f == "blocked" || !dA_->use_tiled()) && !dA_->use_blocked()) {
       if (f == "blocked")
         throw std::invalid_argument("format=blocked: the matrix has too little column locality for the row-block copy");
       dA_->use_csr();

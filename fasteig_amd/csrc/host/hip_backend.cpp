@@ -109,6 +109,7 @@ void HipCsrMatrix::use_sell(int C, int sigma) {
   check(fe_csr_use(h_, FE_FMT_SELL), "fe_csr_use");
   sell_ = true;
   blocked_ = false;
+  tiled_ = false;
 }
 
 bool HipCsrMatrix::use_blocked() {
@@ -122,6 +123,20 @@ bool HipCsrMatrix::use_blocked() {
   check(fe_csr_use(h_, FE_FMT_BLOCKED), "fe_csr_use");
   blocked_ = true;
   sell_ = false;
+  tiled_ = false;
+  return true;
+}
+
+bool HipCsrMatrix::use_tiled() {
+  if (tiled_state_ == 0) {
+    double loads = 0.0;
+    tiled_state_ = check(fe_csr_build_tiled(ex_.ctx(), h_, &loads), "fe_csr_build_tiled") == 0 ? 1 : -1;
+  }
+  if (tiled_state_ < 0) return false;
+  check(fe_csr_use(h_, FE_FMT_TILED), "fe_csr_use");
+  tiled_ = true;
+  blocked_ = false;
+  sell_ = false;
   return true;
 }
 
@@ -129,6 +144,7 @@ void HipCsrMatrix::use_csr() {
   check(fe_csr_use(h_, FE_FMT_CSR), "fe_csr_use");
   sell_ = false;
   blocked_ = false;
+  tiled_ = false;
 }
 
 // ---------------------------------------------------------------- executor

@@ -113,9 +113,13 @@ class HipCsrMatrix {
   // Row-block copy (4-row blocks sharing a column list, fe_csr_build_blocked); false
   // when the matrix has too little column locality for it (CSR stays active).
   bool use_blocked();
+  // Tiled copy (16-row tiles with their X rows staged in LDS,
+  // fe_csr_build_tiled); false when it would not save loads (CSR stays active).
+  bool use_tiled();
   void use_csr();
   bool sell() const { return sell_; }
   bool blocked() const { return blocked_; }
+  bool tiled() const { return tiled_; }
 
  private:
   HipExecutor& ex_;
@@ -125,6 +129,8 @@ class HipCsrMatrix {
   bool sell_built_ = false, sell_ = false;
   int blocked_state_ = 0;  // 0 not tried, 1 built, -1 rejected
   bool blocked_ = false;
+  int tiled_state_ = 0;    // 0 not tried, 1 built, -1 rejected
+  bool tiled_ = false;
 };
 
 class HipExecutor {

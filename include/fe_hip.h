@@ -47,7 +47,7 @@ typedef struct fe_csr_s* fe_csr;
 
 enum fe_prec { FE_F64 = 0, FE_F32 = 1 };
 enum fe_gram_mode { FE_GRAM_NATIVE = 0, FE_GRAM_BF16X3 = 1 };
-enum fe_format { FE_FMT_CSR = 0, FE_FMT_SELL = 1, FE_FMT_BLOCKED = 2 };
+enum fe_format { FE_FMT_CSR = 0, FE_FMT_SELL = 1, FE_FMT_BLOCKED = 2, FE_FMT_TILED = 3 };
 
 /* ---- errors / version ------------------------------------------------- */
 const char* fe_last_error(void);
@@ -109,6 +109,18 @@ int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse);
  * bitwise the same in any order (each row's sum is unchanged). No counterpart
  * in the reference (its CSR product is cuSPARSE's). */
 int fe_csr_block_schedule(fe_ctx ctx, fe_csr A, int64_t plane, int64_t strip);
+/* Builds the tiled copy of A: tiles of 16 rows (4 x 4 patches of a plane for
+ * a 3D grid stencil in natural order, else 16 consecutive rows) taken in
+ * pencils of consecutive tiles (through up to 16 planes, or 32 tiles). The
+ * product runs one workgroup per pencil and 256-byte column slice and keeps
+ * the X rows of the current tile in LDS, loading only the rows each tile adds
+ * to the previous one (the slot plan is made here, on the host). Returns 0
+ * when built, 1 when a tile needs more than 255 X rows or the copy would not
+ * load fewer than half of the nonzeros per row; *loads_per_row receives the
+ * X rows loaded per matrix row. Sums follow the CSR entry order (bitwise the
+ * CSR kernel's). Select it with fe_csr_use(A, FE_FMT_TILED). No counterpart
+ * in the reference (its CSR product is cuSPARSE's). */
+int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row);
 
 /* ---- fused block SpMM (replaces LinearOperator::apply_block,
  *      include/core/LinearOperator.hpp:32-37; CUDA: k x cusparseSpMV,

@@ -195,6 +195,106 @@ def test_block_schedule_detection(dev, mats):
         C3.block_schedule()  # no row-block copy yet
 
 
+TILED = {
+    # 4 x 4 plane patches, pencils through the planes (line a multiple of 4)
+    "p12": lambda: G.hamiltonian_27pt(12, seed=4, wells=5),
+    "p20": lambda: G.hamiltonian_27pt(20, seed=6, wells=7),  # 20 planes: pencils of 16 and 4
+    "c12": lambda: G.conductivity_3d(12, seed=3),            # 7-point
+    # 16 consecutive rows per tile, pencils of 32 tiles (line not a multiple of 4)
+    "h27": lambda: G.hamiltonian_27pt(9, seed=4, wells=5),   # 729 rows: a padded last tile
+    "h27big": lambda: G.hamiltonian_27pt(23, seed=4, wells=9),
+    "c11": lambda: G.conductivity_3d(11, seed=2),
+}
+
+
+@pytest.fixture(scope="module")
+def tiled_mats():
+    return {name: f() for name, f in TILED.items()}
+
+
+@pytest.mark.parametrize("name", list(TILED))
+@pytest.mark.parametrize("k", [4, 16, 48, 64, 100, 256, 512])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_spmm_tiled_matches_csr(dev, tiled_mats, name, k, prec):
+    """LDS-tiled copy: each row's sum over its CSR entries in CSR order, so the
+    product is bitwise the CSR kernel's, fused epilogue included; several
+    256-byte column slices and a partial last slice (k = 100, 48 in FP64)."""
+    A = tiled_mats[name]
+    n = A.shape[0]
+    rng = np.random.default_rng(41 + k)
+    X = rng.uniform(-1, 1, (n, k))
+    Z = rng.uniform(-1, 1, (n, k))
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    C = dev.csr(A)
+    built, lpr = C.build_tiled()
+    assert built and lpr <= 0.5 * A.nnz / n
+    Xd = dev.put(X, dt)
+    Zd = dev.put(Z, dt)
+    Y1 = dev.buffer((n, k), dt)
+    Y2 = dev.buffer((n, k), dt)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y1.ptr, k, 0.9, 0.4, Zd.ptr, k, -1.2))
+    C.use(_fe.FMT_TILED)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y2.ptr, k, 0.9, 0.4, Zd.ptr, k, -1.2))
+    ref = orc.spmm(A, X, 0.9, 0.0) + 0.4 * X - 1.2 * Z
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    scale = np.max(np.abs(A).sum(axis=1)) + 2.0
+    assert np.max(np.abs(Y2.download().astype(np.float64) - ref)) <= tol * scale
+    assert np.array_equal(Y1.download(), Y2.download())
+
+
+def test_spmm_tiled_ragged_and_in_place(dev):
+    """Padded last tile, an empty row, a strided block (ld > k) with a partial
+    16-byte run, and Z aliasing Y (read before written)."""
+    A = G.hamiltonian_27pt(7, seed=1, wells=3).tolil()
+    n = A.shape[0]  # 343 = 16 * 21 + 7
+    A[n - 2, :] = 0.0
+    A[:, n - 2] = 0.0
+    A = A.tocsr()
+    A.eliminate_zeros()
+    ld = 72
+    X = np.random.default_rng(5).uniform(-1, 1, (n, ld))
+    C = dev.csr(A)
+    assert C.build_tiled()[0]
+    for k in (72, 70, 37, 3):
+        for prec, dt, tol in ((_fe.F64, np.float64, 1e-13), (_fe.F32, np.float32, 2e-6)):
+            C.use(_fe.FMT_CSR)
+            Xd = dev.put(X, dt)
+            Y1 = dev.put(np.full((n, ld), 7.0), dt)
+            Y2 = dev.put(np.full((n, ld), 7.0), dt)
+            _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, ld, Y1.ptr, ld, 1.0, 0.5, Y1.ptr, ld, 0.25))
+            C.use(_fe.FMT_TILED)
+            _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, ld, Y2.ptr, ld, 1.0, 0.5, Y2.ptr, ld, 0.25))
+            got = Y2.download().astype(np.float64)
+            Xr = X.astype(dt).astype(np.float64)
+            ref = orc.spmm(A, Xr[:, :k]) + 0.5 * Xr[:, :k] + 0.25 * 7.0
+            assert np.max(np.abs(got[:, :k] - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 2)
+            assert np.all(got[:, k:] == 7.0)
+            assert np.array_equal(Y1.download(), Y2.download())
+
+
+@pytest.mark.parametrize("name", ["powlaw", "lap2d"])
+def test_spmm_tiled_rejects_scattered(dev, mats, name):
+    """No tiled copy where a tile's rows share few columns with the next
+    tile's (power law: unions past 255 rows; 5-point: loads above half the
+    nonzeros per row)."""
+    C = dev.csr(mats[name])
+    built, _ = C.build_tiled()
+    assert not built
+    with pytest.raises(_fe.FeError):
+        C.use(_fe.FMT_TILED)
+
+
+def test_scale_after_copies_is_rejected(dev):
+    """The SELL, row-block and tiled copies hold their own values: scaling
+    the matrix after building one of them is an error, not a stale copy."""
+    A = G.hamiltonian_27pt(12, seed=4, wells=5)
+    d = dev.put(np.full(A.shape[0], 0.5), np.float64)
+    for build in (lambda C: C.build_blocked(4), lambda C: C.build_tiled()):
+        C = dev.csr(A)
+        assert build(C)[0]
+        assert L().fe_csr_scale_sym(dev.ctx, C.h, d.ptr) == -2
+
+
 @pytest.mark.parametrize("name", ["powlaw", "lap2d", "cond3d"])
 def test_spmm_blocked_rejects_low_reuse(dev, mats, name):
     """Scattered columns (power law) and thin stencils (5- and 7-point: under
