@@ -299,11 +299,7 @@ static int launch_rb(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y,
   const unsigned ycols = static_cast<unsigned>((k + COLS - 1) / COLS);
   const T* bv = sizeof(T) == 8 ? (const T*)A->bg_v64 : (const T*)A->bg_v32;
   if constexpr (LPR == 64) {
-#ifndef FE_RBW_UNR
-    constexpr int UNR = RS == 1 ? 8 : 4;  // two groups of UNR x RS 16-byte loads: 64 VGPRs
-#else
-    constexpr int UNR = FE_RBW_UNR;
-#endif
+    constexpr int UNR = 4;  // two groups of 4 x RS 16-byte loads in flight
     hipLaunchKernelGGL((k_spmm_rbw<T, RB, RS, UNR>), dim3((unsigned)nb8, ycols), dim3(256), 0, ctx->stream, A->n,
                        A->bg_bptr, A->bg_ucol, bv, k, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, ntiles,
                        A->bg_nblk, A->bg_sched);
