@@ -70,6 +70,21 @@ __device__ __forceinline__ P* elem(P* base, int row, int64_t ld, int col) {
     return base + static_cast<int64_t>(row) * ld + col;
 }
 
+// An entry's slot and value broadcast together: one 64-bit DPP move in FP32
+// (slot and value packed), two in FP64.
+template <int J>
+__device__ __forceinline__ void row_bcast2(int slot, float w, int& s, float& v) {
+  const int64_t packed = (static_cast<int64_t>(__float_as_int(w)) << 32) | static_cast<uint32_t>(slot);
+  const int64_t b = __builtin_amdgcn_mov_dpp(packed, 0x150 + J, 0xF, 0xF, true);
+  s = static_cast<int>(b);
+  v = __int_as_float(static_cast<int>(b >> 32));
+}
+template <int J>
+__device__ __forceinline__ void row_bcast2(int slot, double w, int& s, double& v) {
+  s = row_bcast<J>(slot);
+  v = __longlong_as_double(__builtin_amdgcn_mov_dpp(__double_as_longlong(w), 0x150 + J, 0xF, 0xF, true));
+}
+
 template <int J, int END>
 struct Unroll {
   template <typename F>
@@ -240,8 +255,9 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
       // without guards, the last one in guarded groups of 4 entries
       auto step = [&](auto J) {
         constexpr int j = decltype(J)::value;
-        const int s = row_bcast<j>(ms);
-        const T w = row_bcast<j>(mw);
+        int s;
+        T w;
+        row_bcast2<j>(ms, mw, s, w);
         const float4 f = xs[s * 16 + sub];
         T xv[V];
         __builtin_memcpy(xv, &f, 16);
