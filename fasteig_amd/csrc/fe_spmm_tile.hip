@@ -17,7 +17,7 @@
 // 13.5 gathers per row for the 4-row block product and 27 for CSR. The X
 // traffic through the vector memory path falls by that factor; the products
 // read X from LDS instead. Other matrices take tiles of 16 consecutive rows
-// and pencils of 16 tiles, and the copy is built only when that saves loads.
+// and pencils of 24 tiles, and the copy is built only when that saves loads.
 //
 // Each row's sum runs over its CSR entries in CSR order, as the CSR kernel's
 // does (bitwise the same on a matrix with sorted rows). A tile row shorter
@@ -35,7 +35,7 @@ typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-te
 
 constexpr int TL_ROWS = 16;   // rows per tile: 4 wavefronts x 4 rows of 16 lanes
 constexpr int TL_PF = 4;      // 16-byte X loads per thread held in registers across a tile
-constexpr int TL_PENCIL = 16;  // tiles per pencil at most
+constexpr int TL_PENCIL = 24;  // tiles per pencil at most
 constexpr int TL_MAX_SLOTS = 255;  // LDS slots (+1 zero slot): at most 64 KB per workgroup
 
 // Lane j of each 16-lane row of the wavefront, broadcast to the row (DPP
@@ -367,7 +367,7 @@ struct TilePlan {
 void make_tiles(int n, int64_t line, int64_t plane, std::vector<std::vector<int>>& tiles,
                 std::vector<int64_t>& pencil) {
   if (plane > 0 && line % 4 == 0 && (plane / line) % 4 == 0 && n % plane == 0) {
-    // 4 x 4 patches; a pencil runs through up to 16 consecutive planes; the
+    // 4 x 4 patches; a pencil runs through up to 24 consecutive planes; the
     // pencils of one range of planes are taken row of patches by row, so an
     // XCD's contiguous share of pencils is a band of neighbouring patches
     const int64_t nx = line / 4, ny = plane / line / 4, nz = n / plane, pz = TL_PENCIL;

@@ -111,7 +111,7 @@ int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse);
 int fe_csr_block_schedule(fe_ctx ctx, fe_csr A, int64_t plane, int64_t strip);
 /* Builds the tiled copy of A: tiles of 16 rows (4 x 4 patches of a plane for
  * a 3D grid stencil in natural order, else 16 consecutive rows) taken in
- * pencils of consecutive tiles (through up to 16 planes, or 32 tiles). The
+ * pencils of consecutive tiles (through up to 24 planes, or 24 tiles). The
  * product runs one workgroup per pencil and 256-byte column slice and keeps
  * the X rows of the current tile in LDS, loading only the rows each tile adds
  * to the previous one (the slot plan is made here, on the host). Returns 0
