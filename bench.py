@@ -103,6 +103,58 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
     return per, algo
 
 
+MFMA_PEAK_TFS = {"fp32": 157.3, "fp64": 78.6}
+
+
+def dense_roofline(n, m=512, reps=10):
+    """MFMA rate of the dense products of a Chebyshev cycle at the solve's
+    shape (n rows, m = 2 nev columns): the symmetric Gram Q^T Q (upper
+    triangle of tiles: n m^2 flops), the Gram Q^T (AQ) (2 n m^2) and the
+    rotation Q C (2 n m^2), each in FP32 and FP64; HIP events on the library
+    stream, average of `reps` launches. Returns {name: (ms, TF/s, fraction of
+    the MFMA peak)}."""
+    import ctypes
+    from fasteig_amd import _fe
+    L = _fe.lib()
+    dev = _fe.Device(0)
+    out = {}
+    rng = np.random.default_rng(3)
+    Cm = dev.put(np.asfortranarray(rng.uniform(-1, 1, (m, m))), np.float64, order="F")
+    G = dev.buffer((m, m), np.float64, order="F")
+    for pname, prec, dt in (("fp32", _fe.F32, np.float32), ("fp64", _fe.F64, np.float64)):
+        X = dev.buffer((n, m), dt)
+        Yb = dev.buffer((n, m), dt)
+        X.upload(rng.uniform(-1, 1, (n, m)).astype(dt))
+        Yb.upload(rng.uniform(-1, 1, (n, m)).astype(dt))
+        cases = {
+            "gram_sym": (lambda: L.fe_gram(dev.ctx, _fe.GRAM_NATIVE, prec, n, m, m, X.ptr, m, X.ptr, m, None, G.ptr, m),
+                         float(n) * m * m),
+            "gram": (lambda: L.fe_gram(dev.ctx, _fe.GRAM_NATIVE, prec, n, m, m, X.ptr, m, Yb.ptr, m, None, G.ptr, m),
+                     2.0 * n * m * m),
+            "gemm": (lambda: L.fe_gemm_nn(dev.ctx, prec, n, m, m, X.ptr, m, Cm.ptr, m, Yb.ptr, m, 1.0, 0.0),
+                     2.0 * n * m * m),
+        }
+        for name, (fn, flops) in cases.items():
+            for _ in range(2):
+                _fe.check(fn())
+            dev.sync()
+            _fe.check(L.fe_event_record(dev.ctx, 4))
+            for _ in range(reps):
+                _fe.check(fn())
+            _fe.check(L.fe_event_record(dev.ctx, 5))
+            ms = ctypes.c_float()
+            _fe.check(L.fe_event_elapsed_ms(dev.ctx, 4, 5, ctypes.byref(ms)))
+            per = ms.value / reps
+            tfs = flops / (per * 1e-3) / 1e12
+            out[f"{name}_{pname}"] = {"ms": per, "tflops": tfs, "frac": tfs / MFMA_PEAK_TFS[pname]}
+            log(f"dense {name} {pname} n={n} m={m}: {per:.3f} ms {tfs:.1f} TF/s "
+                f"({100 * tfs / MFMA_PEAK_TFS[pname]:.0f}% of {MFMA_PEAK_TFS[pname]})")
+        del X, Yb
+    del Cm, G
+    dev.close()
+    return out
+
+
 def copy_reference(nbytes=1 << 30, reps=5):
     """A plain device copy of nbytes (fe_copy2d, FP32) used to calibrate the
     FETCH_SIZE / WRITE_SIZE counters: it reads and writes exactly nbytes."""
@@ -345,7 +397,7 @@ def serving_bench(count=512, rate=200.0, baseline=None, degree=16, max_slots=128
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c3", "c2", "small", "c4", "c5"])
     ap.add_argument("--gram", default="native", choices=["native", "bf16x3"])
@@ -355,6 +407,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--only-kernels", action="store_true",
                     help="time only the fused Chebyshev step and a reference copy (for counter runs)")
+    ap.add_argument("--only-dense", action="store_true",
+                    help="time only the Gram and rotation products at the solve's shape (for counter runs)")
     ap.add_argument("--kernel-k", default="", help="block widths timed by --only-kernels (default nev, 2 nev)")
     ap.add_argument("--kernel-precs", default="fp32,fp64", help="precisions timed by --only-kernels")
     ap.add_argument("--no-extra", action="store_true", help="skip the generalized (C2) and serving (C5) runs")
@@ -376,6 +430,12 @@ def main():
         return
     if args.config == "c4":
         print(json.dumps(c4_bench()), flush=True)
+        return
+    if args.only_dense:
+        M, cfg = build_matrix(args.config)
+        A = M[0] if isinstance(M, tuple) else M
+        print(json.dumps(dense_roofline(A.shape[0], 2 * cfg["nev"] if not cfg["blksize"] else cfg["blksize"])),
+              flush=True)
         return
     if args.only_kernels:
         M, cfg = build_matrix(args.config)
@@ -457,11 +517,17 @@ def main():
     extra["fused_step_fp64_gbs"] = algo64 / (per64 * 1e-3) / 1e9
     log(f"fused step fp32 k={k}: {per32:.3f} ms, {achieved:.0f} GB/s; fp64 {per64:.3f} ms "
         f"{extra['fused_step_fp64_gbs']:.0f} GB/s")
+    # the cycle's Gram and rotation products at the solve's block width
+    extra["dense_mfma"] = dense_roofline(n, cfg["blksize"] or min(n, 2 * cfg["nev"]))
 
     cpu = None
     if not args.no_cpu:
         kc = 8
         t_step, cores = cpu_sample(A, kc)
+        from oracle import oracle as orc
+        orc.set_threads(1)
+        t_step1, _ = cpu_sample(A, kc, target_s=5.0)
+        orc.set_threads(cores)
         t_step_full = t_step * (k / kc)
         # SpMM column-applications of the FP64 GPU solve: cycles x (degree*nev + 2*nev)
         cycles = res.get("fp64", res["mixed"]).iterations
@@ -469,8 +535,9 @@ def main():
         cpu = {"value": 1.0 / t_solve, "unit": "solves/s", "cores": cores, "kind": "port",
                "sample": f"oracle FP64 fused Chebyshev step on {kc} of {k} columns at this n/nnz "
                          f"({t_step:.2f} s/step), scaled by {k}/{kc} and by the plain-FP64 GPU solve's "
-                         f"{cycles} cycles x {cfg['degree'] + 2} block products; dense work excluded"}
-        log(f"cpu sample: {t_step:.2f}s per k={kc} step on {cores} threads")
+                         f"{cycles} cycles x {cfg['degree'] + 2} block products; dense work excluded",
+               "value_1_core": 1.0 / (cycles * (cfg["degree"] + 2) * t_step1 * (k / kc))}
+        log(f"cpu sample: {t_step:.2f}s per k={kc} step on {cores} threads, {t_step1:.2f}s on 1")
 
     if not args.no_extra:
         extra["c2_generalized_fp64"] = c2_bench()

@@ -526,8 +526,10 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
     const int TT = small ? G64 : WT, RS = small ? G64R : 2 * GKS;
     const int tiles_a = (m + TT - 1) / TT, tiles_b = (p + TT - 1) / TT;
     const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
-    // about 4 work units per CU; chunks of whole stages
-    int64_t chunks = (4 * cus + ntiles - 1) / ntiles;
+    // at most 4 work units per CU (two rounds of two resident workgroups:
+    // rounding up would leave a third round of a few units); chunks of whole
+    // stages
+    int64_t chunks = std::max<int64_t>(1, 4 * cus / ntiles);
     int64_t rows_per = (n + chunks - 1) / chunks;
     rows_per = (rows_per + RS - 1) / RS * RS;
     chunks = (n + rows_per - 1) / rows_per;
@@ -566,8 +568,8 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
   }
   const int tiles_a = (m + TILE - 1) / TILE, tiles_b = (p + TILE - 1) / TILE;
   const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
-  // split rows so that about 4 workgroups per CU are live
-  int64_t chunks = (4 * cus + ntiles - 1) / ntiles;
+  // split rows so that at most 4 workgroups per CU are live
+  int64_t chunks = std::max<int64_t>(1, 4 * cus / ntiles);
   int64_t rows_per = (n + chunks - 1) / chunks;
   if (rows_per < KCG) rows_per = KCG;
   rows_per = (rows_per + KCG - 1) / KCG * KCG;
