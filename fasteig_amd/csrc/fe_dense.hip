@@ -414,9 +414,19 @@ __global__ __launch_bounds__(256) void k_gram_reduce(int m, int p, int tiles_b, 
 // registers during the MFMA work.
 constexpr int MKS = 16;
 
-template <typename T, int TR, int TC>
+// C columns (m x p, ldc, FP64) rounded to FP32 once per product (ld m), so
+// that the FP32 rotation stages 4-byte coefficients
+__global__ void k_coef_to_f32(int m, int p, const double* __restrict__ C, int64_t ldc, float* __restrict__ Cf) {
+  const int64_t total = static_cast<int64_t>(m) * p;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = i / m, r = i - j * m;
+    Cf[i] = static_cast<float>(C[j * ldc + r]);
+  }
+}
+
+template <typename T, int TR, int TC, typename CT>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
-                                                 const double* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
+                                                 const CT* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
                                                  T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles,
                                                  int vec) {
   static_assert((TR / 64) * (TC / 64) == 4, "four 64 x 64 wavefront tiles");
@@ -611,18 +621,29 @@ extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const v
   const dim3 grid((unsigned)nb8), block(256);
   if (prec == FE_F64) {
     if (thin)
-      hipLaunchKernelGGL((k_gemm_nn<double, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx, C,
-                         ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
+      hipLaunchKernelGGL((k_gemm_nn<double, 256, 64, double>), grid, block, 0, ctx->stream, n, m, p,
+                         (const double*)X, ldx, C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
     else
-      hipLaunchKernelGGL((k_gemm_nn<double, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx,
-                         C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
+      hipLaunchKernelGGL((k_gemm_nn<double, 128, 128, double>), grid, block, 0, ctx->stream, n, m, p,
+                         (const double*)X, ldx, C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
   } else {
+    // the coefficients rounded to FP32 once (bitwise what the kernel would
+    // round each time), in the Gram's scratch slot: stream order makes it free
+    void* cf = nullptr;
+    int rc = scratch(ctx, 0, sizeof(float) * static_cast<size_t>(m) * p, &cf);
+    if (rc) return rc;
+    const int64_t total = static_cast<int64_t>(m) * p;
+    hipLaunchKernelGGL(k_coef_to_f32, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 4096)), dim3(256), 0,
+                       ctx->stream, m, p, C, ldc, (float*)cf);
+    FE_CHECK_LAUNCH();
     if (thin)
-      hipLaunchKernelGGL((k_gemm_nn<float, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
-                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec);
+      hipLaunchKernelGGL((k_gemm_nn<float, 256, 64, float>), grid, block, 0, ctx->stream, n, m, p, (const float*)X,
+                         ldx, (const float*)cf, (int64_t)m, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c,
+                         nb8 / 8, ntiles, vec);
     else
-      hipLaunchKernelGGL((k_gemm_nn<float, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
-                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec);
+      hipLaunchKernelGGL((k_gemm_nn<float, 128, 128, float>), grid, block, 0, ctx->stream, n, m, p, (const float*)X,
+                         ldx, (const float*)cf, (int64_t)m, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c,
+                         nb8 / 8, ntiles, vec);
   }
   FE_CHECK_LAUNCH();
   return 0;
