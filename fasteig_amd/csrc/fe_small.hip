@@ -573,8 +573,12 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
                        nullptr, nullptr, 0);
   }
   FE_CHECK_LAUNCH();
-  int CB = 32;  // columns per workgroup, LDS slice m*CB doubles within 64 KiB
-  while (CB > 1 && (size_t)m * CB * sizeof(double) > 65536) CB >>= 1;
+  // columns per workgroup: about m / 128 (so that m = 512 spreads over 128
+  // workgroups: the reflectors are applied one after another, so more and
+  // narrower workgroups shorten each step), at least 4 (a column's 64 threads
+  // reduce within one wavefront), LDS slice m*CB doubles within 64 KiB
+  int CB = 32;
+  while (CB > 4 && (CB * 128 > m || (size_t)m * CB * sizeof(double) > 65536)) CB >>= 1;
   const size_t lds = sizeof(double) * (size_t)m * CB;
   hipLaunchKernelGGL(k_syev_back, dim3(blocks_for(m, CB)), dim3(256), lds, ctx->stream, m,
                      multi ? (const double*)Vh : (const double*)A, multi ? (int64_t)m : lda, tau, work + 5 * MM, V,
