@@ -406,10 +406,11 @@ bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, c
     uni.clear();
     for (int r : t)
       if (r >= 0) uni.insert(uni.end(), ci.begin() + rp[r], ci.begin() + rp[r + 1]);
+    if (static_cast<int>(uni.size()) > 16 * TL_MAX_SLOTS) return false;  // cannot fit: skip the sort
     std::sort(uni.begin(), uni.end());
     slots = std::max<int>(slots, static_cast<int>(std::unique(uni.begin(), uni.end()) - uni.begin()));
+    if (slots > TL_MAX_SLOTS) return false;  // first tile that does not fit ends the plan
   }
-  if (slots > TL_MAX_SLOTS) return false;
   P.slots = slots;
   P.pencil = pencil;
   for (size_t pi = 0; pi + 1 < pencil.size(); ++pi) {
