@@ -36,24 +36,13 @@ typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-te
 constexpr int TL_ROWS = 16;   // rows per tile: 4 wavefronts x 4 rows of 16 lanes
 constexpr int TL_PF = 4;      // 16-byte X loads per thread held in registers across a tile
 constexpr int TL_PENCIL = 24;  // tiles per pencil at most
-constexpr int TL_MAX_SLOTS = 255;  // LDS slots (+1 zero slot): at most 64 KB per workgroup
+constexpr int TL_MAX_SLOTS = 240;  // LDS slots: with the zero slot and the pencil offsets within 64 KB
 
 // Lane j of each 16-lane row of the wavefront, broadcast to the row (DPP
 // row_newbcast).
 template <int J>
 __device__ __forceinline__ int row_bcast(int v) {
   return __builtin_amdgcn_mov_dpp(v, 0x150 + J, 0xF, 0xF, true);
-}
-template <int J>
-__device__ __forceinline__ float row_bcast(float v) {
-  return __int_as_float(row_bcast<J>(__float_as_int(v)));
-}
-template <int J>
-__device__ __forceinline__ double row_bcast(double v) {
-  const int64_t b = __double_as_longlong(v);
-  const int lo = row_bcast<J>(static_cast<int>(b));
-  const int hi = row_bcast<J>(static_cast<int>(b >> 32));
-  return __longlong_as_double((static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo));
 }
 
 // Element (row, col) of a row-major block: with O32 the byte offset fits in

@@ -115,7 +115,7 @@ int fe_csr_block_schedule(fe_ctx ctx, fe_csr A, int64_t plane, int64_t strip);
  * product runs one workgroup per pencil and 256-byte column slice and keeps
  * the X rows of the current tile in LDS, loading only the rows each tile adds
  * to the previous one (the slot plan is made here, on the host). Returns 0
- * when built, 1 when a tile needs more than 255 X rows or the copy would not
+ * when built, 1 when a tile needs more than 240 X rows or the copy would not
  * load fewer than half of the nonzeros per row; *loads_per_row receives the
  * X rows loaded per matrix row. Sums follow the CSR entry order (bitwise the
  * CSR kernel's). Select it with fe_csr_use(A, FE_FMT_TILED). No counterpart

@@ -275,7 +275,7 @@ def test_spmm_tiled_ragged_and_in_place(dev):
 @pytest.mark.parametrize("name", ["powlaw", "lap2d"])
 def test_spmm_tiled_rejects_scattered(dev, mats, name):
     """No tiled copy where a tile's rows share few columns with the next
-    tile's (power law: unions past 255 rows; 5-point: loads above half the
+    tile's (power law: unions past 240 rows; 5-point: loads above half the
     nonzeros per row)."""
     C = dev.csr(mats[name])
     built, _ = C.build_tiled()
