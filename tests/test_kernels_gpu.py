@@ -242,6 +242,33 @@ def test_spmm_tiled_matches_csr(dev, tiled_mats, name, k, prec):
     assert np.array_equal(Y1.download(), Y2.download())
 
 
+@pytest.mark.parametrize("k,prec", [(256, _fe.F32), (128, _fe.F64)])
+def test_spmm_tiled_larger_grid(dev, k, prec):
+    """48^3 grid (110,592 rows): two pencils of 24 planes per patch column,
+    144 pencils spread over the XCDs, several column slices."""
+    A = G.hamiltonian_27pt(48, seed=7, wells=20)
+    n = A.shape[0]
+    rng = np.random.default_rng(k)
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    X = rng.uniform(-1, 1, (n, k))
+    Z = rng.uniform(-1, 1, (n, k))
+    C = dev.csr(A)
+    built, lpr = C.build_tiled()
+    assert built and lpr < 3.0
+    Xd, Zd = dev.put(X, dt), dev.put(Z, dt)
+    Y1, Y2 = dev.buffer((n, k), dt), dev.buffer((n, k), dt)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y1.ptr, k, 0.3, -0.2, Zd.ptr, k, 0.7))
+    C.use(_fe.FMT_TILED)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y2.ptr, k, 0.3, -0.2, Zd.ptr, k, 0.7))
+    got = Y2.download()
+    assert np.array_equal(Y1.download(), got)
+    rows = rng.choice(n, 2000, replace=False)  # oracle on a sample of rows
+    Xr, Zr = X.astype(dt).astype(np.float64), Z.astype(dt).astype(np.float64)
+    ref = A[rows] @ Xr * 0.3 - 0.2 * Xr[rows] + 0.7 * Zr[rows]
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    assert np.max(np.abs(got[rows].astype(np.float64) - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 2)
+
+
 def test_spmm_tiled_ragged_and_in_place(dev):
     """Padded last tile, an empty row, a strided block (ld > k) with a partial
     16-byte run, and Z aliasing Y (read before written)."""
