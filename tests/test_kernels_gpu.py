@@ -269,6 +269,42 @@ def test_spmm_tiled_larger_grid(dev, k, prec):
     assert np.max(np.abs(got[rows].astype(np.float64) - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 2)
 
 
+def _replay_tile_loads(A, line, plane, pz=24):
+    """X rows a pencil walk loads, replayed from the planning rule: tiles are
+    4 x 4 patches of a plane, pencils run through pz planes, and a tile loads
+    the columns of its rows that the previous tile of its pencil did not hold."""
+    A = A.tocsr()
+    n = A.shape[0]
+    nx, ny, nz = line // 4, plane // line // 4, n // plane
+    loads, widest = 0, 0
+    for z0 in range(0, nz, pz):
+        for py in range(ny):
+            for px in range(nx):
+                held = set()
+                for z in range(z0, min(nz, z0 + pz)):
+                    rows = [z * plane + (4 * py + dy) * line + 4 * px + dx for dy in range(4) for dx in range(4)]
+                    need = set(np.concatenate([A.indices[A.indptr[r]:A.indptr[r + 1]] for r in rows]).tolist())
+                    widest = max(widest, len(need))
+                    loads += len(need - held)
+                    held = need
+    return loads / n, widest
+
+
+@pytest.mark.parametrize("g", [24, 28])
+def test_tiled_plan_load_count(dev, g):
+    """The host slot plan loads exactly the rows the pencil walk adds: its
+    reported X rows per matrix row equal an independent replay of the rule
+    (24^3: one pencil per patch column; 28^3: a pencil of 24 planes and one
+    of 4)."""
+    A = G.hamiltonian_27pt(g, seed=2, wells=4)
+    C = dev.csr(A)
+    built, lpr = C.build_tiled()
+    assert built
+    want, widest = _replay_tile_loads(A, g, g * g)
+    assert widest <= 108
+    assert lpr == pytest.approx(want, rel=0, abs=1e-12)
+
+
 def test_spmm_tiled_ragged_and_in_place(dev):
     """Padded last tile, an empty row, a strided block (ld > k) with a partial
     16-byte run, and Z aliasing Y (read before written)."""
