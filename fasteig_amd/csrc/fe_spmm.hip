@@ -116,12 +116,8 @@ __device__ __forceinline__ void ld_runs(const T* p, int stride, bool full, int c
 }
 
 // gathers in flight per lane: rows of 16+ lanes with one run, and two runs
-#ifndef FE_CSR_UNR_WIDE
-#define FE_CSR_UNR_WIDE 16
-#endif
-#ifndef FE_CSR_UNR_RS2
-#define FE_CSR_UNR_RS2 8
-#endif
+constexpr int CSR_UNR_WIDE = 16;
+constexpr int CSR_UNR_RS2 = 8;
 
 // CSR kernel, one row per group of LPR lanes, RS 16-byte runs per lane spaced
 // LPR * 16 bytes apart (each load instruction of the wavefront reads one
@@ -142,7 +138,7 @@ __global__ __launch_bounds__(256) void k_spmm_csr(
   constexpr int RPB = 4 * RPW;
   constexpr int STRIDE = LPR * V;        // columns between a lane's runs
   constexpr int COLS = STRIDE * RS;      // columns per column tile
-  constexpr int UNR = RS == 1 ? (LPR >= 16 ? FE_CSR_UNR_WIDE : 8) : FE_CSR_UNR_RS2;
+  constexpr int UNR = RS == 1 ? (LPR >= 16 ? CSR_UNR_WIDE : 8) : CSR_UNR_RS2;
   const int64_t tile = xcd_tile(blockIdx.x, per_xcd);
   if (tile >= ntiles) return;
   const int lane = threadIdx.x & 63;
