@@ -348,12 +348,14 @@ def test_sell_format_same_result():
     assert r2.residual_norms.max() <= 1e-9
 
 
+@pytest.mark.parametrize("g", [14, 16])
 @pytest.mark.parametrize("solver", ["Chebyshev", "LOBPCG"])
-def test_blocked_format_identical(solver):
-    """The row-block gather product sums in CSR order: the whole solve is
-    bitwise the same as with format=csr; auto picks it for a stencil and
-    falls back to csr for a power-law graph."""
-    A = G.hamiltonian_27pt(14, seed=4, wells=8)
+def test_blocked_format_identical(solver, g):
+    """The row-block and LDS-tiled products sum in CSR order: the whole
+    solve is bitwise the same as with format=csr (g = 14: tiles of consecutive
+    rows; g = 16: 4 x 4 plane patches); auto picks the tiled copy for a
+    stencil and falls back to csr for a power-law graph."""
+    A = G.hamiltonian_27pt(g, seed=4, wells=8)
     p = Problem(A)
     kw = dict(solver=solver, nev=8, tol=1e-9, maxiter=400, which="SR", seed=3)
     if solver == "Chebyshev":
@@ -361,16 +363,20 @@ def test_blocked_format_identical(solver):
     r1 = p.solve(format="csr", **kw)
     r2 = p.solve(format="blocked", **kw)
     r3 = p.solve(format="auto", **kw)
-    assert r1.iterations == r2.iterations == r3.iterations
+    r4 = p.solve(format="tiled", **kw)
+    assert r1.iterations == r2.iterations == r3.iterations == r4.iterations
     assert np.array_equal(r1.eigenvalues, r2.eigenvalues)
     assert np.array_equal(r1.eigenvalues, r3.eigenvalues)
-    assert r2.residual_norms.max() <= 1e-9
+    assert np.array_equal(r1.eigenvalues, r4.eigenvalues)
+    assert r2.residual_norms.max() <= 1e-9 and r4.residual_norms.max() <= 1e-9
     q = Problem(G.power_law(3000, seed=5, dmax=300))
     ra = q.solve(solver="LOBPCG", nev=4, tol=1e-9, maxiter=2000, which="SR", format="auto")
     rc = q.solve(solver="LOBPCG", nev=4, tol=1e-9, maxiter=2000, which="SR", format="csr")
     assert np.array_equal(ra.eigenvalues, rc.eigenvalues)
     with pytest.raises(ValueError):
         q.solve(solver="LOBPCG", nev=4, which="SR", format="blocked")
+    with pytest.raises(ValueError):
+        q.solve(solver="LOBPCG", nev=4, which="SR", format="tiled")
 
 
 def test_invalid_options_raise():
