@@ -1,12 +1,16 @@
 // Small dense FP64 kernels for the Rayleigh-Ritz and Cholesky-QR steps.
 // All matrices are column-major on the device.
 //
-//   k_syev_tri    one workgroup: Householder tridiagonalization (Golub & Van
-//                 Loan, Alg. 8.3.1), Sturm-count bisection for every eigenvalue
-//                 (GvL 8.4.1), inverse iteration for the eigenvectors of the
-//                 tridiagonal (Wilkinson; shifts and clusters as in LAPACK
-//                 dstein), clusters of close eigenvalues orthogonalized by a
-//                 workgroup between the steps
+//   k_syev_tri    one workgroup (m <= 16): Householder tridiagonalization
+//                 (Golub & Van Loan, Alg. 8.3.1), Sturm-count bisection for
+//                 every eigenvalue (GvL 8.4.1), inverse iteration for the
+//                 eigenvectors of the tridiagonal (Wilkinson; shifts and
+//                 clusters as in LAPACK dstein), clusters of close eigenvalues
+//                 orthogonalized by a workgroup between the steps; for m > 16
+//                 the same inverse iteration and cluster phases run on their
+//                 own launches after
+//   k_tri_step    the multi-workgroup tridiagonalization (one launch per column)
+//   k_syev_msect  and the multisection of the tridiagonal's eigenvalues
 //   k_syev_back   multi-workgroup back-transformation V = H_0 ... H_{m-3} Z with
 //                 the eigenvector slice of each workgroup held in LDS
 #include "fe_common.hpp"
@@ -602,7 +606,9 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
   double* pb = ee + m;
   double* Vh = pb + 2 * m;
   double* work = Vh + MM;
-  const bool multi = m > 128;  // small matrices stay in one workgroup
+  // multi-workgroup steps, multisection and per-eigenvalue inverse iteration
+  // from m = 17 up (m = 96: 0.98 ms against 1.46 ms in one workgroup)
+  const bool multi = m > 16;
   if (multi) {
     rc = fe_sym_small(ctx, m, A, lda);
     if (rc) return rc;
