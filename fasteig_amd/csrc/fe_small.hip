@@ -466,6 +466,11 @@ __global__ __launch_bounds__(256) void k_syev_msect(int m, const double* __restr
   if (lane == 0) W[t] = 0.5 * (lo + hi);
 }
 
+// All of a step's global reads are issued first, so that one memory round
+// trip serves them: the reflector and p of step k-1 and column k (NQ values
+// per thread, rows k + tid + 256 q) and the first column the wavefront
+// updates (PE values per lane, rows k + 1 + lane + 64 q).
+template <int NQ, int PE>
 __global__ __launch_bounds__(256) void k_tri_step(int m, int k, double* A, int64_t lda, double* Vh, double* tau,
                                                   double* dd, double* ee, const double* __restrict__ pprev,
                                                   double* pcur) {
@@ -473,25 +478,55 @@ __global__ __launch_bounds__(256) void k_tri_step(int m, int k, double* A, int64
   __shared__ double red[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool upd = k > 0;
-  double tprev = 0.0;
+  double vq[NQ], pq[NQ], aq[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int i = k + tid + 256 * q;
+    vq[q] = pq[q] = aq[q] = 0.0;
+    if (i < m) {
+      aq[q] = A[(int64_t)k * lda + i];
+      if (upd) {
+        vq[q] = (i == k) ? 1.0 : Vh[(int64_t)(k - 1) * m + i];
+        pq[q] = pprev[i];
+      }
+    }
+  }
+  const int jfirst = k + 1 + blockIdx.x * 4 + wave;
+  double cq[PE];
+#pragma unroll
+  for (int q = 0; q < PE; ++q) {
+    const int i = k + 1 + lane + 64 * q;
+    cq[q] = (jfirst < m && i < m) ? A[(int64_t)jfirst * lda + i] : 0.0;
+  }
+  const double tprev = upd ? tau[k - 1] : 0.0;
   if (upd) {
     // v_{k-1} lives on rows k.. ; w_{k-1} = p_{k-1} - (tau/2)(p^T v) v
-    tprev = tau[k - 1];
     double part = 0.0;
-    for (int i = k + tid; i < m; i += 256) {
-      const double v = (i == k) ? 1.0 : Vh[(int64_t)(k - 1) * m + i];
-      vp[i] = v;
-      part += pprev[i] * v;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int i = k + tid + 256 * q;
+      if (i < m) {
+        vp[i] = vq[q];
+        part += pq[q] * vq[q];
+      }
     }
     const double K = 0.5 * tprev * wg_sum256(part, red);
-    for (int i = k + tid; i < m; i += 256) wp[i] = pprev[i] - K * vp[i];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int i = k + tid + 256 * q;
+      if (i < m) wp[i] = pq[q] - K * vq[q];
+    }
     __syncthreads();
   }
   // updated column k (rows k..m-1)
-  for (int i = k + tid; i < m; i += 256) {
-    double x = A[(int64_t)k * lda + i];
-    if (upd) x -= vp[i] * wp[k] + wp[i] * vp[k];
-    col[i] = x;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int i = k + tid + 256 * q;
+    if (i < m) {
+      double x = aq[q];
+      if (upd) x -= vp[i] * wp[k] + wp[i] * vp[k];
+      col[i] = x;
+    }
   }
   __syncthreads();
   double part = 0.0;
@@ -516,16 +551,31 @@ __global__ __launch_bounds__(256) void k_tri_step(int m, int k, double* A, int64
   }
   // columns j > k: apply the update of step k-1 (rows > k) and form p_k[j]
   const int nwaves = gridDim.x * 4;
-  for (int j = k + 1 + blockIdx.x * 4 + wave; j < m; j += nwaves) {
+  for (int j = jfirst; j < m; j += nwaves) {
     double* cj = A + (int64_t)j * lda;
     double s = 0.0;
-    for (int i = k + 1 + lane; i < m; i += 64) {
-      double a = cj[i];
-      if (upd) {
-        a -= vp[i] * wp[j] + wp[i] * vp[j];
-        cj[i] = a;
+    if (j == jfirst) {
+#pragma unroll
+      for (int q = 0; q < PE; ++q) {
+        const int i = k + 1 + lane + 64 * q;
+        if (i < m) {
+          double a = cq[q];
+          if (upd) {
+            a -= vp[i] * wp[j] + wp[i] * vp[j];
+            cj[i] = a;
+          }
+          s += a * vk[i];
+        }
       }
-      s += a * vk[i];
+    } else {
+      for (int i = k + 1 + lane; i < m; i += 64) {
+        double a = cj[i];
+        if (upd) {
+          a -= vp[i] * wp[j] + wp[i] * vp[j];
+          cj[i] = a;
+        }
+        s += a * vk[i];
+      }
     }
     s = wave_sum(s);
     if (lane == 0) pcur[j] = tk * s;
@@ -616,8 +666,13 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
     for (int k = 0; k + 2 < m; ++k) {
       const double* pprev = pb + ((k + 1) & 1) * m;
       double* pcur = pb + (k & 1) * m;
-      hipLaunchKernelGGL(k_tri_step, dim3(G), dim3(256), 0, ctx->stream, m, k, A, lda, Vh, tau, dd, ee, pprev,
-                         pcur);
+      // rows past k held in registers: 2 x 256 / 8 x 64 up to m = 512, else 8 x 256 / 32 x 64
+      if (m <= 512)
+        hipLaunchKernelGGL((k_tri_step<2, 8>), dim3(G), dim3(256), 0, ctx->stream, m, k, A, lda, Vh, tau, dd, ee,
+                           pprev, pcur);
+      else
+        hipLaunchKernelGGL((k_tri_step<8, 32>), dim3(G), dim3(256), 0, ctx->stream, m, k, A, lda, Vh, tau, dd, ee,
+                           pprev, pcur);
     }
     hipLaunchKernelGGL(k_tri_finish, dim3(1), dim3(64), 0, ctx->stream, m, A, lda, Vh, tau, dd, ee,
                        pb + ((m - 3) & 1) * m);
