@@ -14,6 +14,7 @@
 #include "fe_common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fe {
 
@@ -414,13 +415,15 @@ __global__ __launch_bounds__(256) void k_gram_reduce(int m, int p, int tiles_b, 
 // registers during the MFMA work.
 constexpr int MKS = 16;
 
-// C columns (m x p, ldc, FP64) rounded to FP32 once per product (ld m), so
-// that the FP32 rotation stages 4-byte coefficients
-__global__ void k_coef_to_f32(int m, int p, const double* __restrict__ C, int64_t ldc, float* __restrict__ Cf) {
+// C columns (m x p, ldc, FP64) copied once per product into columns padded
+// to 16 bytes (ldf), rounded to FP32 for the FP32 rotation: the rotation then
+// stages its coefficients with 16-byte loads, in the block's precision
+template <typename CT>
+__global__ void k_coef_stage(int m, int p, const double* __restrict__ C, int64_t ldc, CT* __restrict__ Cf, int ldf) {
   const int64_t total = static_cast<int64_t>(m) * p;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t j = i / m, r = i - j * m;
-    Cf[i] = static_cast<float>(C[j * ldc + r]);
+    Cf[j * ldf + r] = static_cast<CT>(C[j * ldc + r]);
   }
 }
 
@@ -467,6 +470,20 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
       else
 #pragma unroll
         for (int e = 0; e < 8; ++e) dst[e] = T(0);
+    }
+    // coefficient columns are 16-byte aligned (ldc a multiple of 16 bytes):
+    // whole runs read as 16-byte vectors
+    constexpr int CV = 16 / (int)sizeof(CT);
+    if (ccol < p && k0 + sck + CK <= m) {
+#pragma unroll
+      for (int e = 0; e < CK; e += CV) {
+        CT v[CV];
+        const float4 f = *reinterpret_cast<const float4*>(C + (int64_t)ccol * ldc + k0 + sck + e);
+        __builtin_memcpy(v, &f, 16);
+#pragma unroll
+        for (int q = 0; q < CV; ++q) cr[e + q] = static_cast<T>(v[q]);
+      }
+      return;
     }
 #pragma unroll
     for (int e = 0; e < CK; ++e) {
@@ -619,30 +636,36 @@ extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const v
   const int V = prec == FE_F64 ? 2 : 4;
   const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && ldx % V == 0;
   const dim3 grid((unsigned)nb8), block(256);
+  // the coefficients staged once (for FP32 rounded as the kernel would round
+  // each one), in the Gram's scratch slot: stream order makes it free
+  const int cv = prec == FE_F64 ? 2 : 4;
+  const int ldf = (m + cv - 1) / cv * cv;
+  void* cf = nullptr;
+  int rc = scratch(ctx, 0, static_cast<size_t>(prec_bytes(prec)) * ldf * p, &cf);
+  if (rc) return rc;
+  const int64_t total = static_cast<int64_t>(m) * p;
+  const dim3 sgrid((unsigned)std::min<int64_t>((total + 255) / 256, 4096));
   if (prec == FE_F64) {
+    hipLaunchKernelGGL(k_coef_stage<double>, sgrid, dim3(256), 0, ctx->stream, m, p, C, ldc, (double*)cf, ldf);
+    FE_CHECK_LAUNCH();
     if (thin)
       hipLaunchKernelGGL((k_gemm_nn<double, 256, 64, double>), grid, block, 0, ctx->stream, n, m, p,
-                         (const double*)X, ldx, C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
+                         (const double*)X, ldx, (const double*)cf, (int64_t)ldf, (double*)Y, ldy, alpha, beta, tiles_c,
+                         nb8 / 8, ntiles, vec);
     else
       hipLaunchKernelGGL((k_gemm_nn<double, 128, 128, double>), grid, block, 0, ctx->stream, n, m, p,
-                         (const double*)X, ldx, C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec);
+                         (const double*)X, ldx, (const double*)cf, (int64_t)ldf, (double*)Y, ldy, alpha, beta, tiles_c,
+                         nb8 / 8, ntiles, vec);
   } else {
-    // the coefficients rounded to FP32 once (bitwise what the kernel would
-    // round each time), in the Gram's scratch slot: stream order makes it free
-    void* cf = nullptr;
-    int rc = scratch(ctx, 0, sizeof(float) * static_cast<size_t>(m) * p, &cf);
-    if (rc) return rc;
-    const int64_t total = static_cast<int64_t>(m) * p;
-    hipLaunchKernelGGL(k_coef_to_f32, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 4096)), dim3(256), 0,
-                       ctx->stream, m, p, C, ldc, (float*)cf);
+    hipLaunchKernelGGL(k_coef_stage<float>, sgrid, dim3(256), 0, ctx->stream, m, p, C, ldc, (float*)cf, ldf);
     FE_CHECK_LAUNCH();
     if (thin)
       hipLaunchKernelGGL((k_gemm_nn<float, 256, 64, float>), grid, block, 0, ctx->stream, n, m, p, (const float*)X,
-                         ldx, (const float*)cf, (int64_t)m, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c,
+                         ldx, (const float*)cf, (int64_t)ldf, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c,
                          nb8 / 8, ntiles, vec);
     else
       hipLaunchKernelGGL((k_gemm_nn<float, 128, 128, float>), grid, block, 0, ctx->stream, n, m, p, (const float*)X,
-                         ldx, (const float*)cf, (int64_t)m, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c,
+                         ldx, (const float*)cf, (int64_t)ldf, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c,
                          nb8 / 8, ntiles, vec);
   }
   FE_CHECK_LAUNCH();
