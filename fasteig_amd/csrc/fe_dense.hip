@@ -102,8 +102,8 @@ __global__ __launch_bounds__(256, 2) void k_gram(int64_t n, int m, int p, const 
                                               const double* __restrict__ w, int tiles_b, int ntiles,
                                               int sym, int64_t rows_per, int64_t per_xcd, int64_t nunits,
                                               int vec, double* __restrict__ part) {
-  // rows per stage: 32 (two per staging thread; for FP64 too, where 16 rows
-  // per stage left the MFMA pipe a third idle: 8.6 -> 7.8 ms at n 884,736, m 512)
+  // rows per stage: 32, two per staging thread (for FP64 too: 16 rows per
+  // stage kept the FP64 MFMA pipe 68% busy at n 884,736, m 512; 32 rows, 75%)
   constexpr int GK = 2 * GKS, NR = GK / 16;
   __shared__ T Xs[GK][WT + GPAD];
   __shared__ T Ys[GK][WT + GPAD];
