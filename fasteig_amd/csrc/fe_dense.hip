@@ -416,23 +416,11 @@ __global__ __launch_bounds__(256) void k_gram_reduce(int m, int p, int tiles_b, 
 // registers during the MFMA work.
 constexpr int MKS = 16;
 
-// C columns (m x p, ldc, FP64) copied once per product into columns padded
-// to 16 bytes (ldf), rounded to FP32 for the FP32 rotation: the rotation then
-// stages its coefficients with 16-byte loads, in the block's precision
-template <typename CT>
-__global__ void k_coef_stage(int m, int p, const double* __restrict__ C, int64_t ldc, CT* __restrict__ Cf, int ldf) {
-  const int64_t total = static_cast<int64_t>(m) * p;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = i / m, r = i - j * m;
-    Cf[j * ldf + r] = static_cast<CT>(C[j * ldc + r]);
-  }
-}
-
-template <typename T, int TR, int TC, typename CT>
+template <typename T, int TR, int TC>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
-                                                 const CT* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
+                                                 const double* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
                                                  T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles,
-                                                 int vec) {
+                                                 int vec, int cvec) {
   static_assert((TR / 64) * (TC / 64) == 4, "four 64 x 64 wavefront tiles");
   constexpr int MK = MKS;  // k values per stage
   constexpr int TPR = 256 / TR, XK = MK / TPR;  // X staging: threads per row, k values per thread
@@ -472,17 +460,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
 #pragma unroll
         for (int e = 0; e < 8; ++e) dst[e] = T(0);
     }
-    // coefficient columns are 16-byte aligned (ldc a multiple of 16 bytes):
-    // whole runs read as 16-byte vectors
-    constexpr int CV = 16 / (int)sizeof(CT);
-    if (ccol < p && k0 + sck + CK <= m) {
+    // with 16-byte aligned coefficient columns (cvec), whole runs are read
+    // as 16-byte vectors: with 8-byte reads the FP32 rotation ran at 58% of
+    // the MFMA peak and the FP64 one at 75%
+    if (cvec && ccol < p && k0 + sck + CK <= m) {
 #pragma unroll
-      for (int e = 0; e < CK; e += CV) {
-        CT v[CV];
-        const float4 f = *reinterpret_cast<const float4*>(C + (int64_t)ccol * ldc + k0 + sck + e);
-        __builtin_memcpy(v, &f, 16);
-#pragma unroll
-        for (int q = 0; q < CV; ++q) cr[e + q] = static_cast<T>(v[q]);
+      for (int e = 0; e < CK; e += 2) {
+        const double2 d = *reinterpret_cast<const double2*>(C + (int64_t)ccol * ldc + k0 + sck + e);
+        cr[e] = static_cast<T>(d.x);
+        cr[e + 1] = static_cast<T>(d.y);
       }
       return;
     }
@@ -637,37 +623,22 @@ extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const v
   const int V = prec == FE_F64 ? 2 : 4;
   const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && ldx % V == 0;
   const dim3 grid((unsigned)nb8), block(256);
-  // the coefficients staged once (for FP32 rounded as the kernel would round
-  // each one), in the Gram's scratch slot: stream order makes it free
-  const int cv = prec == FE_F64 ? 2 : 4;
-  const int ldf = (m + cv - 1) / cv * cv;
-  void* cf = nullptr;
-  int rc = scratch(ctx, 0, static_cast<size_t>(prec_bytes(prec)) * ldf * p, &cf);
-  if (rc) return rc;
-  const int64_t total = static_cast<int64_t>(m) * p;
-  const dim3 sgrid((unsigned)std::min<int64_t>((total + 255) / 256, 4096));
+  // coefficient columns read as 16-byte vectors when C and ldc allow it
+  const int cvec = (reinterpret_cast<uintptr_t>(C) % 16 == 0) && ldc % 2 == 0;
   if (prec == FE_F64) {
-    hipLaunchKernelGGL(k_coef_stage<double>, sgrid, dim3(256), 0, ctx->stream, m, p, C, ldc, (double*)cf, ldf);
-    FE_CHECK_LAUNCH();
     if (thin)
-      hipLaunchKernelGGL((k_gemm_nn<double, 256, 64, double>), grid, block, 0, ctx->stream, n, m, p,
-                         (const double*)X, ldx, (const double*)cf, (int64_t)ldf, (double*)Y, ldy, alpha, beta, tiles_c,
-                         nb8 / 8, ntiles, vec);
+      hipLaunchKernelGGL((k_gemm_nn<double, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx, C,
+                         ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec);
     else
-      hipLaunchKernelGGL((k_gemm_nn<double, 128, 128, double>), grid, block, 0, ctx->stream, n, m, p,
-                         (const double*)X, ldx, (const double*)cf, (int64_t)ldf, (double*)Y, ldy, alpha, beta, tiles_c,
-                         nb8 / 8, ntiles, vec);
+      hipLaunchKernelGGL((k_gemm_nn<double, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx,
+                         C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec);
   } else {
-    hipLaunchKernelGGL(k_coef_stage<float>, sgrid, dim3(256), 0, ctx->stream, m, p, C, ldc, (float*)cf, ldf);
-    FE_CHECK_LAUNCH();
     if (thin)
-      hipLaunchKernelGGL((k_gemm_nn<float, 256, 64, float>), grid, block, 0, ctx->stream, n, m, p, (const float*)X,
-                         ldx, (const float*)cf, (int64_t)ldf, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c,
-                         nb8 / 8, ntiles, vec);
+      hipLaunchKernelGGL((k_gemm_nn<float, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
+                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec);
     else
-      hipLaunchKernelGGL((k_gemm_nn<float, 128, 128, float>), grid, block, 0, ctx->stream, n, m, p, (const float*)X,
-                         ldx, (const float*)cf, (int64_t)ldf, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c,
-                         nb8 / 8, ntiles, vec);
+      hipLaunchKernelGGL((k_gemm_nn<float, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
+                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec);
   }
   FE_CHECK_LAUNCH();
   return 0;
