@@ -231,7 +231,15 @@ int fe_memcpy_d2h_async(fe_ctx ctx, void* dst, const void* src, size_t bytes);
 int fe_host_alloc(size_t bytes, void** ptr);
 int fe_host_free(void* ptr);
 /* Capture the work issued on the context stream between begin and end into
- * a graph, and replay it (same kernels, same arguments) on that stream. */
+ * a graph, and replay it (same kernels, same arguments) on that stream.
+ * Entry points that use the context's scratch (fe_gram, fe_potrf,
+ * fe_chol_rinv, fe_syev, fe_colnorms, fe_ritz_residuals, fe_gather_cols,
+ * fe_upload_colmajor, fe_download_colmajor) may grow it with hipMalloc,
+ * which a capture cannot record, and a later growth frees the buffer a graph
+ * holds: capture them only after a call of at least the same size outside
+ * the capture, and do not let a larger call run while such a graph is in use.
+ * fe_spmm, fe_gemm_nn, the grouped fe_grp_* launches and the copies use no
+ * scratch. */
 typedef struct fe_graph_s* fe_graph;
 int fe_graph_begin(fe_ctx ctx);
 int fe_graph_end(fe_ctx ctx, fe_graph* out);
