@@ -45,6 +45,10 @@ def lib():
         L.feh_problem_results.argtypes = [vp, vp, vp, vp, C.POINTER(i64), dp]
         L.feh_problem_eigenvectors.argtypes = [vp, vp]
         L.feh_problem_results_text.argtypes = [vp, C.c_char_p, i32]
+        L.feh_core_lanczos_bounds.argtypes = [vp, C.c_double, C.c_uint, dp, dp]
+        L.feh_core_thin_qr.argtypes = [vp, i64, i32, vp, i32]
+        L.feh_core_orthonormalize_block.argtypes = [vp, i64, i32, vp, i32, vp, C.c_double, vp,
+                                                    C.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -118,6 +122,40 @@ class Problem:
         buf = C.create_string_buffer(1 << 20)
         _check(lib().feh_problem_results_text(self.h, buf, len(buf)))
         return buf.value.decode()
+
+    # ---- core numerics on this problem's device (the reference's L4 free functions)
+    def lanczos_bounds(self, sign: float = 1.0, seed: int = 1) -> tuple[float, float]:
+        """20-step Lanczos bounds of sign * A (src/ChebyshevSolver.cpp:26-49)."""
+        lo, hi = C.c_double(), C.c_double()
+        _check(lib().feh_core_lanczos_bounds(self.h, float(sign), int(seed), C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+    def thin_qr(self, X, method: str = "householder") -> tuple[np.ndarray, str]:
+        """thin_qr_in_place on the device: method "cholesky" returns (X, "failed")
+        unchanged when the Gram matrix is not positive definite; "householder"
+        (the orthonormal-basis role: CholQR2, shifted CholQR3 when needed)
+        reports "shifted" when the shifted path ran."""
+        Xf = np.array(X, dtype=np.float64, order="F", copy=True)
+        n, k = Xf.shape
+        m = {"cholesky": 0, "householder": 1}[method]
+        rc = _check(lib().feh_core_thin_qr(self.h, n, k, Xf.ctypes.data, m))
+        return Xf, {0: "ok", 1: "failed", 2: "shifted"}[rc]
+
+    def orthonormalize_block(self, basis, src, rank_tol: float = 1e-12) -> np.ndarray:
+        """orthonormalize_block (src/core/Orthogonalization.cpp:717-775) on the
+        device; basis may be None (no columns)."""
+        S = np.array(src, dtype=np.float64, order="F", copy=True)
+        n, b = S.shape
+        if basis is None:
+            B, act = None, 0
+        else:
+            B = np.array(basis, dtype=np.float64, order="F", copy=True)
+            act = B.shape[1]
+        out = np.zeros((n, b), order="F")
+        r = C.c_int()
+        _check(lib().feh_core_orthonormalize_block(self.h, n, act, None if B is None else B.ctypes.data, b,
+                                                   S.ctypes.data, float(rank_tol), out.ctypes.data, C.byref(r)))
+        return out[:, :r.value].copy(order="F")
 
     def close(self):
         if getattr(self, "h", None):

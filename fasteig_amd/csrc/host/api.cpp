@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "core_ops.hpp"
 #include "fasteig_problem.hpp"
 #include "parseinput.hpp"
 #include "serving.hpp"
@@ -166,6 +167,71 @@ int feh_problem_eigenvectors(feh_problem p, double* out) {
     if (!p || !out) throw std::invalid_argument("null argument");
     const MatrixXd& V = p->prob->eigenvectors();
     std::memcpy(out, V.data(), sizeof(double) * static_cast<size_t>(V.size()));
+    return 0;
+  });
+}
+
+int feh_core_lanczos_bounds(feh_problem p, double sign, unsigned seed, double* lower, double* upper) {
+  return guarded([&] {
+    if (!p || !lower || !upper) throw std::invalid_argument("null argument");
+    if (sign != 1.0 && sign != -1.0) throw std::invalid_argument("feh_core_lanczos_bounds: sign must be +1 or -1");
+    fasteig_options o(p->prob->n());
+    auto ex = p->prob->executor(o);
+    fasteig::hip::lanczos_bounds(*ex, p->prob->device_matrix(o), sign, seed, *lower, *upper);
+    return 0;
+  });
+}
+
+int feh_core_thin_qr(feh_problem p, int64_t n, int k, double* X, int method) {
+  return guarded([&] {
+    if (!p || !X) throw std::invalid_argument("null argument");
+    if (k <= 0 || n < k) throw std::invalid_argument("thin_qr_in_place: row count must be >= column count.");
+    if (method != 0 && method != 1) throw std::invalid_argument("feh_core_thin_qr: method must be 0 or 1");
+    fasteig_options o(p->prob->n());
+    auto ex = p->prob->executor(o);
+    namespace H = fasteig::hip;
+    MatrixXd host(static_cast<int>(n), k);
+    std::copy(X, X + n * k, host.data());
+    H::Block B(*ex, n, k, FE_F64), tmp(*ex, n, k, FE_F64);
+    H::upload(*ex, host, B.ref());
+    H::Workspace ws(*ex, k, FE_GRAM_NATIVE);
+    int rc = 0;
+    if (method == 0) {
+      if (H::cholesky_qr(ws, B, tmp) != 0) return 1;
+    } else {
+      rc = H::orthonormal_qr(ws, B, tmp) ? 2 : 0;
+    }
+    const MatrixXd out = H::download(*ex, B.ref());
+    std::copy(out.data(), out.data() + n * k, X);
+    return rc;
+  });
+}
+
+int feh_core_orthonormalize_block(feh_problem p, int64_t n, int active, const double* basis, int b,
+                                  const double* src, double rank_tol, double* out, int* rank) {
+  return guarded([&] {
+    if (!p || !src || !out || !rank || (active > 0 && !basis)) throw std::invalid_argument("null argument");
+    if (b <= 0 || n < b || active < 0 || active > n)
+      throw std::invalid_argument("orthonormalize_block: invalid basis or source dimensions.");
+    fasteig_options o(p->prob->n());
+    auto ex = p->prob->executor(o);
+    namespace H = fasteig::hip;
+    MatrixXd hs(static_cast<int>(n), b);
+    std::copy(src, src + n * b, hs.data());
+    H::Block S(*ex, n, b, FE_F64), V(*ex, n, std::max(active, 1), FE_F64);
+    H::upload(*ex, hs, S.ref());
+    if (active > 0) {
+      MatrixXd hb(static_cast<int>(n), active);
+      std::copy(basis, basis + n * active, hb.data());
+      H::upload(*ex, hb, V.ref());
+    }
+    H::Workspace ws(*ex, std::max(b, active), FE_GRAM_NATIVE);
+    H::Block Q = H::orthonormalize_block(ws, V.cols(0, active), S.ref(), rank_tol);
+    *rank = Q.empty() ? 0 : Q.ncols();
+    if (*rank > 0) {
+      const MatrixXd h = H::download(*ex, Q.ref());
+      std::copy(h.data(), h.data() + n * (*rank), out);
+    }
     return 0;
   });
 }

@@ -54,11 +54,13 @@ int cholesky_qr(Workspace& ws, Block& X, Block& tmp) {
   return 0;
 }
 
-void orthonormal_qr(Workspace& ws, Block& X, Block& tmp) {
+int orthonormal_qr(Workspace& ws, Block& X, Block& tmp) {
   HipExecutor& ex = ws.ex;
   const int k = X.ncols();
+  int shifted = 0;
   if (cholesky_qr(ws, X, tmp) != 0) {
     // shifted Cholesky QR: G + s I = R^T R with s = 11 (n k + k (k+1)) u ||X||_2^2
+    shifted = 1;
     ws.ensure(k);
     gram(ex, ws.gram_mode, X.ref(), X.ref(), ws.G.data(), ws.cap);
     MatrixXd Gh = ws.G.to_host();
@@ -79,6 +81,7 @@ void orthonormal_qr(Workspace& ws, Block& X, Block& tmp) {
     if (cholesky_qr(ws, X, tmp) != 0) throw std::runtime_error("orthonormal_qr: Cholesky QR failed after shift");
   }
   if (cholesky_qr(ws, X, tmp) != 0) throw std::runtime_error("orthonormal_qr: second Cholesky QR pass failed");
+  return shifted;
 }
 
 void block_gram_schmidt(Workspace& ws, const BlockRef& basis, const BlockRef& S, const BlockRef& out) {
@@ -93,15 +96,52 @@ void block_gram_schmidt(Workspace& ws, const BlockRef& basis, const BlockRef& S,
   gemm(ex, basis, ws.R.data(), ws.cap, out, -1.0, 1.0);
 }
 
+namespace {
+// Directions of `cur` (already projected off the basis) whose singular values
+// exceed `cut` (absolute), as an orthonormal block, at most `capacity`
+// columns, largest singular value first: Gram G = cur^T cur, G = V diag(w) V^T,
+// Q = cur V_keep diag(w_keep)^(-1/2). The Gram eigenvalues carry an absolute
+// error of a few eps * w_max, so only w > resolve * w_max are trusted here;
+// *resolved receives the singular value below which this level cannot judge.
+Block svqb_level(Workspace& ws, const BlockRef& cur, double cut, int capacity, double resolve, double* resolved) {
+  HipExecutor& ex = ws.ex;
+  const int c = cur.cols;
+  ws.ensure(c);
+  gram(ex, ws.gram_mode, cur, cur, ws.G.data(), ws.cap);
+  check(fe_syev(ex.ctx(), c, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap), "fe_syev");
+  std::vector<double> w(c);
+  check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * c), "d2h");
+  const double wmax = std::max(w.back(), 0.0);
+  *resolved = std::sqrt(resolve * wmax);
+  std::vector<int> good;
+  for (int i = c - 1; i >= 0; --i)  // descending singular values, like the SVD ordering
+    if (w[i] > resolve * wmax && w[i] > cut * cut && static_cast<int>(good.size()) < capacity) good.push_back(i);
+  if (good.empty()) return Block();
+  const int r = static_cast<int>(good.size());
+  MatrixXd Vh = ws.V.to_host();
+  MatrixXd Cm(ws.cap, r);
+  for (int j = 0; j < r; ++j) {
+    const double f = 1.0 / std::sqrt(w[good[j]]);
+    for (int i = 0; i < c; ++i) Cm(i, j) = Vh(i, good[j]) * f;
+  }
+  check(fe_memcpy_h2d(ex.ctx(), ws.C.data(), Cm.data(), sizeof(double) * Cm.size()), "h2d");
+  Block q(ex, cur.rows, r, cur.prec);
+  gemm(ex, cur, ws.C.data(), ws.cap, q.ref(), 1.0, 0.0);
+  return q;
+}
+}  // namespace
+
 Block orthonormalize_block(Workspace& ws, const BlockRef& basis, const BlockRef& src, double rank_tol) {
   HipExecutor& ex = ws.ex;
   if (src.cols <= 0 || src.rows < src.cols) throw std::invalid_argument("orthonormalize_block: invalid source block.");
   if (!std::isfinite(rank_tol) || rank_tol <= 0.0 || rank_tol >= 1.0)
     throw std::invalid_argument("orthonormalize_block: rank tolerance must be in (0,1).");
   if (basis.cols == basis.rows) return Block();
+  // Gram eigenvalues trusted per level: sqrt of it is the singular-value ratio
+  // one level resolves (1e-6 in FP64, 3e-3 in FP32)
+  const double resolve = src.prec == FE_F64 ? 1e-12 : 1e-5;
   Block cur(ex, src.rows, src.cols, src.prec);
   copy(ex, src, cur.ref());
-  const double eig_floor = src.prec == FE_F64 ? 1e-14 : 1e-7;  // resolvable Gram eigenvalue ratio
   for (int pass = 0; pass < 2; ++pass) {
     const std::vector<double> n0 = column_norms(ex, cur.ref());
     double scale = 0.0;
@@ -110,44 +150,39 @@ Block orthonormalize_block(Workspace& ws, const BlockRef& basis, const BlockRef&
       scale = std::max(scale, v);
     }
     if (scale == 0.0) return Block();
+    const double cut = rank_tol * scale;  // the reference keeps sigma(R) > rank_tol * max column norm
     block_gram_schmidt(ws, basis, cur.ref(), cur.ref());
-    const std::vector<double> n1 = column_norms(ex, cur.ref());
-    std::vector<int> keep;
-    for (int j = 0; j < cur.ncols(); ++j)
-      if (n1[j] > rank_tol * scale) keep.push_back(j);
     const int capacity = static_cast<int>(std::min<int64_t>(cur.ncols(), src.rows - basis.cols));
-    if (keep.empty() || capacity <= 0) return Block();
-    // compact the kept columns and scale them to unit norm
-    Block kept(ex, src.rows, static_cast<int>(keep.size()), src.prec);
-    gather_cols(ex, cur.ref(), keep, kept.ref());
-    const int c = kept.ncols();
-    ws.ensure(c);
-    {
-      std::vector<double> inv(c);
-      for (int j = 0; j < c; ++j) inv[j] = 1.0 / n1[keep[j]];
-      check(fe_memcpy_h2d(ex.ctx(), ws.theta.data(), inv.data(), sizeof(double) * c), "h2d");
-      scale_columns(ex, kept.ref(), ws.theta.data());
+    if (capacity <= 0) return Block();
+    // level 1: the directions the Gram of the projected block resolves
+    double resolved = 0.0;
+    Block q1 = svqb_level(ws, cur.ref(), cut, capacity, resolve, &resolved);
+    Block next;
+    const int r1 = q1.empty() ? 0 : q1.ncols();
+    if (r1 < capacity && resolved > cut) {
+      // level 2: singular values between cut and what level 1 resolved. Remove
+      // the level-1 directions (and the basis again) from the projected block;
+      // the remainder holds only the small singular values, which its own Gram
+      // resolves relative to its much smaller scale.
+      Block rest(ex, src.rows, cur.ncols(), src.prec);
+      copy(ex, cur.ref(), rest.ref());
+      if (r1 > 0) block_gram_schmidt(ws, q1.ref(), rest.ref(), rest.ref());
+      block_gram_schmidt(ws, basis, rest.ref(), rest.ref());
+      double resolved2 = 0.0;
+      Block q2 = svqb_level(ws, rest.ref(), cut, capacity - r1, resolve, &resolved2);
+      if (!q2.empty() && r1 > 0) {
+        next = Block(ex, src.rows, r1 + q2.ncols(), src.prec);
+        copy(ex, q1.ref(), next.cols(0, r1));
+        copy(ex, q2.ref(), next.cols(r1, q2.ncols()));
+      } else if (!q2.empty()) {
+        next = std::move(q2);
+      } else {
+        next = std::move(q1);
+      }
+    } else {
+      next = std::move(q1);
     }
-    // Gram eigen-decomposition of the unit-column block (SVQB, Stathopoulos & Wu 2002)
-    gram(ex, ws.gram_mode, kept.ref(), kept.ref(), ws.G.data(), ws.cap);
-    check(fe_syev(ex.ctx(), c, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap), "fe_syev");
-    std::vector<double> w(c);
-    check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * c), "d2h");
-    MatrixXd Vh = ws.V.to_host();
-    const double wmax = w.back();
-    std::vector<int> good;
-    for (int i = c - 1; i >= 0; --i)  // descending singular values, like the SVD ordering
-      if (w[i] > eig_floor * wmax && w[i] > 0.0 && static_cast<int>(good.size()) < capacity) good.push_back(i);
-    if (good.empty()) return Block();
-    const int r = static_cast<int>(good.size());
-    MatrixXd Cm(ws.cap, r);
-    for (int j = 0; j < r; ++j) {
-      const double f = 1.0 / std::sqrt(w[good[j]]);
-      for (int i = 0; i < c; ++i) Cm(i, j) = Vh(i, good[j]) * f;
-    }
-    check(fe_memcpy_h2d(ex.ctx(), ws.C.data(), Cm.data(), sizeof(double) * Cm.size()), "h2d");
-    Block next(ex, src.rows, r, src.prec);
-    gemm(ex, kept.ref(), ws.C.data(), ws.cap, next.ref(), 1.0, 0.0);
+    if (next.empty()) return Block();
     cur = std::move(next);
   }
   return cur;

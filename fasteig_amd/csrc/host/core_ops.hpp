@@ -32,8 +32,9 @@ int cholesky_qr(Workspace& ws, Block& X, Block& tmp);
 // Orthonormal basis of span(X) in place, the role Householder QR plays in the
 // reference (thin_qr_in_place(..., Householder), src/core/Orthogonalization.cpp:351-365,
 // 137-183): Cholesky QR twice; if the first factorization fails, a shifted
-// Cholesky QR (shift 11 (n k + k (k+1)) u ||X||^2, Fukaya et al. 2020) runs first.
-void orthonormal_qr(Workspace& ws, Block& X, Block& tmp);
+// Cholesky QR (shift 11 (n k + k (k+1)) u ||X||^2, Fukaya et al. 2020) runs
+// first (shifted CholQR3). Returns 1 when the shifted path ran, else 0.
+int orthonormal_qr(Workspace& ws, Block& X, Block& tmp);
 
 // Two-pass classical block Gram-Schmidt of S against basis[:, :active] into
 // out (out may be S itself). Reference: src/core/Orthogonalization.cpp:482-527.
@@ -41,9 +42,13 @@ void block_gram_schmidt(Workspace& ws, const BlockRef& basis, const BlockRef& S,
 
 // Rank-revealing orthonormalization of src against basis[:, :active]; returns
 // an empty Block when nothing survives. Two passes of [block Gram-Schmidt,
-// column scaling, Gram eigen-decomposition keeping eigenvalues above
-// tol^2 * max, Q V Lambda^{-1/2}]. Reference: src/core/Orthogonalization.cpp:717-775
-// (there: Householder QR + Jacobi SVD of R, keep sigma > 1e-12 * max column norm).
+// keep the directions with singular value above rank_tol * max column norm,
+// orthonormalized]. Reference: src/core/Orthogonalization.cpp:717-775 (there:
+// Householder QR + Jacobi SVD of R). Here the singular values come from Gram
+// eigen-decompositions (SVQB, Stathopoulos & Wu 2002) on MFMA: a first level
+// resolves those above 1e-6 of the largest (FP64); when it keeps fewer than
+// the capacity, a second level on the remainder (projected off the first
+// level's directions) resolves the rest down to the cutoff.
 Block orthonormalize_block(Workspace& ws, const BlockRef& basis, const BlockRef& src, double rank_tol = 1e-12);
 
 struct RitzPairs {

@@ -70,8 +70,31 @@ struct BatchServer::Slot {
   int stall = 0;
   // roles of this cycle
   int f = 0, g = 0, h = 0, s = 0;
-  ~Slot() = default;
+  fe_ctx ctx = nullptr;  // owner of A and mem
+  Slot() = default;
+  Slot(const Slot&) = delete;
+  Slot& operator=(const Slot&) = delete;
+  ~Slot() {
+    if (A) fe_csr_destroy(A);
+    if (mem && ctx) fe_free(ctx, mem);
+  }
 };
+
+namespace {
+// A request's CSR is usable: sizes agree, rowptr starts at 0, never
+// decreases and ends at nnz, and every column index lies in [0, n).
+bool csr_valid(const ServeRequest& r) {
+  const int64_t nnz = static_cast<int64_t>(r.values.size());
+  if (r.n <= 0 || static_cast<int64_t>(r.rowptr.size()) != static_cast<int64_t>(r.n) + 1 ||
+      static_cast<int64_t>(r.colind.size()) != nnz || r.rowptr[0] != 0 || r.rowptr[r.n] != nnz)
+    return false;
+  for (int i = 0; i < r.n; ++i)
+    if (r.rowptr[i + 1] < r.rowptr[i]) return false;
+  for (int c : r.colind)
+    if (c < 0 || c >= r.n) return false;
+  return true;
+}
+}  // namespace
 
 BatchServer::BatchServer(int device, const ServeOptions& opts) : o_(opts) {
   if (o_.max_slots < 1 || o_.max_slots > 4096) throw std::invalid_argument("serve: max_slots must be 1..4096");
@@ -106,11 +129,7 @@ BatchServer::BatchServer(int device, const ServeOptions& opts) : o_(opts) {
 BatchServer::~BatchServer() {
   if (ex_) {
     fe_ctx_sync(ex_->ctx());
-    for (auto& s : slots_)
-      if (s) {
-        if (s->A) fe_csr_destroy(s->A);
-        if (s->mem) fe_free(ex_->ctx(), s->mem);
-      }
+    slots_.clear();  // each slot frees its matrix and blocks
     if (graph_exec_) fe_graph_destroy(static_cast<fe_graph>(graph_exec_));
     if (tab_dev_) fe_free(ex_->ctx(), tab_dev_);
     if (res_dev_) fe_free(ex_->ctx(), res_dev_);
@@ -136,19 +155,20 @@ void BatchServer::admit() {
     const double queued = queue_.front().second;
     const int64_t nnz = static_cast<int64_t>(r.values.size());
     const int m = subspace_cols(r.n, r.nev);
-    const bool valid = r.n > 0 && r.nev >= 1 && 2 * r.nev <= FE_GRP_MAXCOLS && m < r.n &&
-                       static_cast<int>(r.rowptr.size()) == r.n + 1 && r.colind.size() == r.values.size() &&
-                       r.rowptr[0] == 0 && r.rowptr[r.n] == nnz && r.tol > 0.0;
+    const bool valid = r.nev >= 1 && 2 * r.nev <= FE_GRP_MAXCOLS && m < r.n && r.tol > 0.0 && csr_valid(r);
     const size_t fp = valid ? serve_footprint(r.n, nnz, r.nev) : 0;
-    if (!valid || fp > o_.hbm_budget) {  // can never run here
+    auto reject = [&](ServeRequest& q, double qs) {  // status 2: can never run here
       ServeResult res;
-      res.id = r.id;
+      res.id = q.id;
       res.status = 2;
-      res.n = r.n;
-      res.nev = r.nev;
-      res.queued_s = queued;
+      res.n = q.n;
+      res.nev = q.nev;
+      res.queued_s = qs;
       res.admitted_s = res.finished_s = now();
       done_.push_back(std::move(res));
+    };
+    if (!valid || fp > o_.hbm_budget) {
+      reject(r, queued);
       queue_.pop_front();
       continue;
     }
@@ -166,14 +186,18 @@ void BatchServer::admit() {
     s->m = m;
     s->ld = m;
     s->bytes = fp;
-    check(fe_csr_create(ex_->ctx(), s->n, nnz, s->req.rowptr.data(), s->req.colind.data(), s->req.values.data(),
-                        &s->A),
-          "fe_csr_create");
-    check(fe_csr_arrays(s->A, &s->rp, &s->ci, &s->v64, &s->v32), "fe_csr_arrays");
+    s->ctx = ex_->ctx();
     const size_t blk = align256(static_cast<size_t>(s->n) * m * 8);
     const size_t sm = align256(sizeof(double) * FE_GRP_MAXCOLS * FE_GRP_MAXCOLS);
     const size_t pb = align256(sizeof(double) * chunks(s->n) * m * m);
-    check(fe_alloc(ex_->ctx(), 4 * blk + 2 * sm + pb + 256, &s->mem), "fe_alloc(slot)");
+    if (fe_csr_create(ex_->ctx(), s->n, nnz, s->req.rowptr.data(), s->req.colind.data(), s->req.values.data(),
+                      &s->A) != 0 ||
+        fe_csr_arrays(s->A, &s->rp, &s->ci, &s->v64, &s->v32) != 0 ||
+        fe_alloc(ex_->ctx(), 4 * blk + 2 * sm + pb + 256, &s->mem) != 0) {
+      // the device could not take it (the slot's destructor frees what was made)
+      reject(s->req, s->queued_s);
+      continue;
+    }
     char* base = static_cast<char*>(s->mem);
     for (int b = 0; b < 4; ++b) s->B[b] = base + b * blk;
     s->G = reinterpret_cast<double*>(base + 4 * blk);

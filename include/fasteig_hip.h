@@ -50,6 +50,30 @@ int feh_problem_eigenvectors(feh_problem p, double* out);
 /* RAW RESULTS / DIAGNOSTICS text of the last solve into buf (truncated). */
 int feh_problem_results_text(feh_problem p, char* buf, int len);
 
+/* ---- core numerics of the block solvers on the problem's device (the
+ * reference's L4 free functions, which the solvers call between SpMMs).
+ * Host blocks are n x k column-major FP64 (copied in and out); the work runs
+ * in FP64 on the device of the problem's executor. ----------------------- */
+/* 20-step Lanczos bound estimate of sign * A (the problem's matrix), two-pass
+ * reorthogonalized, seeded start vector: lower = min(theta - |beta y_last|),
+ * upper = max(theta + |beta y_last|). Reference: estimate_bounds_with_executor,
+ * src/ChebyshevSolver.cpp:26-49; KrylovSubspace::build, src/core/KrylovSubspace.cpp:112-131. */
+int feh_core_lanczos_bounds(feh_problem p, double sign, unsigned seed, double* lower, double* upper);
+/* thin_qr_in_place (src/core/Orthogonalization.cpp:403-480) on X (n x k, in
+ * place; R is not returned). method 0: Cholesky QR (QrMethod::Cholesky) --
+ * returns 1 and leaves X unchanged when the Gram matrix is not positive
+ * definite (the reference's -1 status, :49-57,216-225). method 1: the
+ * orthonormal-basis role of QrMethod::Householder (:351-365) -- Cholesky QR
+ * twice, with a shifted Cholesky QR first when the plain one fails (shifted
+ * CholQR3); returns 2 when the shifted path ran, else 0. */
+int feh_core_thin_qr(feh_problem p, int64_t n, int k, double* X, int method);
+/* orthonormalize_block (src/core/Orthogonalization.cpp:717-775): orthonormal
+ * directions of src (n x b) against basis[:, 0:active] (n x active, orthonormal)
+ * whose singular values exceed rank_tol * max column norm (two passes).
+ * out holds n x b doubles; *rank receives the kept column count (0: none). */
+int feh_core_orthonormalize_block(feh_problem p, int64_t n, int active, const double* basis, int b,
+                                  const double* src, double rank_tol, double* out, int* rank);
+
 /* ---- continuous-batching solve server (no reference counterpart: the
  * reference runs one solve per fasteig_problem::solve call) ------------------
  * Requests (CSR matrix, nev, tol; smallest eigenpairs) queue on the host and

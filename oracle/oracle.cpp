@@ -22,6 +22,12 @@
 //   orc_spmm            dense brute force (numpy) on small matrices
 //   orc_lanczos_bounds  bounds enclose the spectrum of small matrices (eigh)
 //   orc_cheb_scalars    closed form T_d(c*lambda+e) vs numpy Chebyshev
+//   orc_cheb_filter_scaled
+//                       T_d((lambda-c)/e) / T_d((a0-c)/e) (numpy Chebyshev) on a
+//                       diagonal matrix, V p(Lambda) V^T X (eigh) on a 27-point one
+//   orc_chebyshev(ritz_bounds=1)
+//                       2D Laplacian closed form (SR, LR), eigh on a small 27-point
+//                       Hamiltonian (values and subspace)
 //   orc_chebyshev / orc_lobpcg / orc_davidson
 //                       2D Laplacian closed form, SiO spectrum from the
 //                       reference's tests/reference/SiO.out.121mpi (golden
@@ -72,33 +78,123 @@ struct Csr {
 
 // ---------------------------------------------------------------- dense kernels
 
+// The tall-skinny products below are cache-blocked (rows packed row-major in
+// small chunks, 4 x 8 register blocks): the same sums as the textbook loops,
+// in an order that keeps the operands in cache, so that the checker finishes
+// full-size problems.
+
 // C = A^T B over the first ka columns of A and kb of B (A,B n-row blocks).
+// Rows are cut into a fixed number of slabs (independent of the thread count,
+// so results do not depend on it); each slab's partial product is summed over
+// its rows in order, and the slab partials are added in slab order.
 Mat AtB(const Mat& A, int ka, const Mat& B, int kb) {
   Mat C(ka, kb);
   const long n = A.r;
-#pragma omp parallel for collapse(2) schedule(static)
+  if (n == 0 || ka == 0 || kb == 0) return C;
+  constexpr long RB = 64;     // rows packed per chunk
+  const long slabs = std::max(1L, std::min(64L, n / 1024));
+  const long per = (n + slabs - 1) / slabs;
+  std::vector<double> part(static_cast<size_t>(slabs) * ka * kb, 0.0);
+#pragma omp parallel
+  {
+    std::vector<double> ap(RB * ka), bp(RB * kb);
+#pragma omp for schedule(dynamic, 1)
+    for (long sl = 0; sl < slabs; ++sl) {
+      double* P = part.data() + static_cast<size_t>(sl) * ka * kb;  // row-major ka x kb
+      const long r0 = sl * per, r1 = std::min(n, r0 + per);
+      for (long c0 = r0; c0 < r1; c0 += RB) {
+        const long rb = std::min(RB, r1 - c0);
+        for (int i = 0; i < ka; ++i) {
+          const double* x = A.col(i) + c0;
+          for (long t = 0; t < rb; ++t) ap[t * ka + i] = x[t];
+        }
+        for (int j = 0; j < kb; ++j) {
+          const double* y = B.col(j) + c0;
+          for (long t = 0; t < rb; ++t) bp[t * kb + j] = y[t];
+        }
+        for (int i0 = 0; i0 < ka; i0 += 4) {
+          const int ib = std::min(4, ka - i0);
+          for (int j0 = 0; j0 < kb; j0 += 8) {
+            const int jb = std::min(8, kb - j0);
+            double acc[4][8] = {};
+            for (long t = 0; t < rb; ++t) {
+              const double* ar = &ap[t * ka + i0];
+              const double* br = &bp[t * kb + j0];
+              if (ib == 4 && jb == 8) {
+                for (int u = 0; u < 4; ++u)
+                  for (int v = 0; v < 8; ++v) acc[u][v] += ar[u] * br[v];
+              } else {
+                for (int u = 0; u < ib; ++u)
+                  for (int v = 0; v < jb; ++v) acc[u][v] += ar[u] * br[v];
+              }
+            }
+            for (int u = 0; u < ib; ++u)
+              for (int v = 0; v < jb; ++v) P[static_cast<size_t>(i0 + u) * kb + j0 + v] += acc[u][v];
+          }
+        }
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < ka; ++i)
     for (int j = 0; j < kb; ++j) {
-      const double* x = A.col(i);
-      const double* y = B.col(j);
       double s = 0.0;
-      for (long t = 0; t < n; ++t) s += x[t] * y[t];
+      for (long sl = 0; sl < slabs; ++sl) s += part[static_cast<size_t>(sl) * ka * kb + static_cast<size_t>(i) * kb + j];
       C(i, j) = s;
     }
   return C;
 }
 
-// Y = A[:, :k] * C   (A n x >=k, C k x p).
+// Y = A[:, :k] * C   (A n x >=k, C k x p); each entry sums over i in order.
 Mat AC(const Mat& A, int k, const Mat& C) {
   Mat Y(A.r, C.c);
   const long n = A.r;
-#pragma omp parallel for schedule(static)
-  for (long t = 0; t < n; ++t)
-    for (int j = 0; j < C.c; ++j) {
-      double s = 0.0;
-      for (int i = 0; i < k; ++i) s += A(static_cast<int>(t), i) * C(i, j);
-      Y(static_cast<int>(t), j) = s;
+  const int p = C.c;
+  if (n == 0 || p == 0) return Y;
+  std::vector<double> ct(static_cast<size_t>(k) * p);  // C row-major
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < p; ++j) ct[static_cast<size_t>(i) * p + j] = C(i, j);
+  constexpr long RB = 64;
+  const long chunks = (n + RB - 1) / RB;
+#pragma omp parallel
+  {
+    std::vector<double> ap(RB * std::max(k, 1)), yp(RB * p);
+#pragma omp for schedule(static)
+    for (long ch = 0; ch < chunks; ++ch) {
+      const long c0 = ch * RB, rb = std::min(RB, n - c0);
+      for (int i = 0; i < k; ++i) {
+        const double* x = A.col(i) + c0;
+        for (long t = 0; t < rb; ++t) ap[t * k + i] = x[t];
+      }
+      for (long t0 = 0; t0 < rb; t0 += 4) {
+        const int tb = static_cast<int>(std::min(4L, rb - t0));
+        for (int j0 = 0; j0 < p; j0 += 8) {
+          const int jb = std::min(8, p - j0);
+          double acc[4][8] = {};
+          for (int i = 0; i < k; ++i) {
+            const double* cr = &ct[static_cast<size_t>(i) * p + j0];
+            if (tb == 4 && jb == 8) {
+              for (int u = 0; u < 4; ++u) {
+                const double a = ap[(t0 + u) * k + i];
+                for (int v = 0; v < 8; ++v) acc[u][v] += a * cr[v];
+              }
+            } else {
+              for (int u = 0; u < tb; ++u) {
+                const double a = ap[(t0 + u) * k + i];
+                for (int v = 0; v < jb; ++v) acc[u][v] += a * cr[v];
+              }
+            }
+          }
+          for (int u = 0; u < tb; ++u)
+            for (int v = 0; v < jb; ++v) yp[(t0 + u) * p + j0 + v] = acc[u][v];
+        }
+      }
+      for (int j = 0; j < p; ++j) {
+        double* y = Y.col(j) + c0;
+        for (long t = 0; t < rb; ++t) y[t] = yp[t * p + j];
+      }
     }
+  }
   return Y;
 }
 
@@ -428,14 +524,27 @@ void jacobi_svd(const Mat& R, Vec& s, Mat& U) {
 // CPU product src/fasteig_problem.cpp:649-655.
 void spmm(const Csr& A, const Mat& X, int xf, int k, Mat& Y, double alpha, double beta) {
   const int n = A.n;
+  // columns in groups of JB, packed row-major so a nonzero reads JB
+  // consecutive values; each entry still sums its row's nonzeros in CSR order
+  constexpr int JB = 16;
+  std::vector<double> xr(static_cast<size_t>(X.r) * JB);
+  for (int j0 = 0; j0 < k; j0 += JB) {
+    const int jb = std::min(JB, k - j0);
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < X.r; ++t)
+      for (int j = 0; j < JB; ++j) xr[static_cast<size_t>(t) * JB + j] = j < jb ? X(t, xf + j0 + j) : 0.0;
 #pragma omp parallel for schedule(dynamic, 256)
-  for (int i = 0; i < n; ++i) {
-    for (int j = 0; j < k; ++j) {
-      const double* x = X.col(xf + j);
-      double s = 0.0;
-      for (int p = A.rowptr[i]; p < A.rowptr[i + 1]; ++p) s += A.vals[p] * x[A.colind[p]];
-      double& y = Y(i, j);
-      y = alpha * s + (beta == 0.0 ? 0.0 : beta * y);
+    for (int i = 0; i < n; ++i) {
+      double s[JB] = {};
+      for (int p = A.rowptr[i]; p < A.rowptr[i + 1]; ++p) {
+        const double v = A.vals[p];
+        const double* x = &xr[static_cast<size_t>(A.colind[p]) * JB];
+        for (int j = 0; j < JB; ++j) s[j] += v * x[j];
+      }
+      for (int j = 0; j < jb; ++j) {
+        double& y = Y(i, j0 + j);
+        y = alpha * s[j] + (beta == 0.0 ? 0.0 : beta * y);
+      }
     }
   }
 }
@@ -1637,6 +1746,16 @@ int orc_cheb_filter(int n, const int* rp, const int* ci, const double* v, int k,
   return guarded([&] {
     orc::Mat Xm = wrap(X, n, k);
     orc::cheb_filter(csr(n, rp, ci, v), 1.0, Xm, degree, lower, upper);
+    unwrap(Xm, X);
+    return 0;
+  });
+}
+
+int orc_cheb_filter_scaled(int n, const int* rp, const int* ci, const double* v, int k, double* X, int degree,
+                           double a0, double cutoff, double upper) {
+  return guarded([&] {
+    orc::Mat Xm = wrap(X, n, k);
+    orc::cheb_filter_scaled(csr(n, rp, ci, v), 1.0, Xm, degree, a0, cutoff, upper);
     unwrap(Xm, X);
     return 0;
   });

@@ -46,6 +46,8 @@ def lib():
         L.orc_cheb_scalars.argtypes = [C.c_int, C.c_double, C.c_double,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_cheb_filter.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_double]
+        L.orc_cheb_filter_scaled.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, C.c_int, C.c_double,
+                                             C.c_double, C.c_double]
         solver_tail = [C.c_double, C.c_char_p, C.c_uint, _dp, C.c_void_p, _dp, C.POINTER(C.c_long)]
         L.orc_chebyshev.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + solver_tail
         L.orc_lobpcg.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, C.c_int] + solver_tail
@@ -167,6 +169,15 @@ def cheb_filter(A, X, degree, lower, upper):
     X = _F(X).copy(order="F")
     n, k = X.shape
     _check(lib().orc_cheb_filter(n, rp, ci, v, k, X, degree, lower, upper))
+    return X
+
+
+def cheb_filter_scaled(A, X, degree, a0, cutoff, upper):
+    """Scaled filter on [cutoff, upper] normalised at a0 (the adaptive-bound mode)."""
+    rp, ci, v = _csr(A)
+    X = _F(X).copy(order="F")
+    n, k = X.shape
+    _check(lib().orc_cheb_filter_scaled(n, rp, ci, v, k, X, degree, a0, cutoff, upper))
     return X
 
 

@@ -139,6 +139,77 @@ def test_chebyshev_filter_is_chebyshev_polynomial(degree):
     assert np.isclose(abs(npcheb.chebval(c * lower + d, coef)), 1e3, rtol=1e-9)
 
 
+@pytest.mark.parametrize("degree", [1, 2, 7, 20])
+def test_scaled_chebyshev_filter_is_normalised_polynomial(degree):
+    """The adaptive-bound filter (Zhou & Saad 2007, Alg. 4.1) applied to a
+    diagonal matrix must equal T_d((lam - c)/e) / T_d((a0 - c)/e) at every
+    diagonal entry: 1 at a0, bounded by 1/|T_d((a0-c)/e)| on [cutoff, upper]."""
+    lam = np.concatenate([np.linspace(-2.0, 0.9, 9), np.linspace(1.0, 6.0, 17)])
+    A = sp.diags(lam, format="csr")
+    a0, cutoff, upper = -2.0, 1.0, 6.0
+    Y = orc.cheb_filter_scaled(A, np.eye(lam.size), degree, a0, cutoff, upper)
+    e, c = 0.5 * (upper - cutoff), 0.5 * (upper + cutoff)
+    coef = np.zeros(degree + 1)
+    coef[-1] = 1.0
+    expect = npcheb.chebval((lam - c) / e, coef) / npcheb.chebval((a0 - c) / e, coef)
+    assert np.max(np.abs(np.diag(Y) - expect)) <= 1e-12 * max(1.0, np.abs(expect).max())
+    # off-diagonal entries stay exactly zero (a diagonal operator)
+    assert np.array_equal(Y - np.diag(np.diag(Y)), np.zeros_like(Y))
+    assert np.isclose(Y[0, 0], 1.0, rtol=1e-12)
+    damped = np.abs(np.diag(Y)[lam >= cutoff])
+    assert damped.max() <= 1.0 / abs(npcheb.chebval((a0 - c) / e, coef)) * (1 + 1e-12)
+
+
+def test_scaled_chebyshev_filter_general_matrix_matches_eigh():
+    """On a non-diagonal symmetric matrix the filter is V p(Lam) V^T X."""
+    A = G.hamiltonian_27pt(6, wells=3)
+    w, V = np.linalg.eigh(A.toarray())
+    X = np.random.default_rng(11).standard_normal((A.shape[0], 5))
+    a0, cutoff, upper = w[0], w[40], w[-1] * 1.01
+    d = 9
+    e, c = 0.5 * (upper - cutoff), 0.5 * (upper + cutoff)
+    coef = np.zeros(d + 1)
+    coef[-1] = 1.0
+    p = npcheb.chebval((w - c) / e, coef) / npcheb.chebval((a0 - c) / e, coef)
+    ref = V @ (p[:, None] * (V.T @ X))
+    Y = orc.cheb_filter_scaled(A, X, d, a0, cutoff, upper)
+    assert np.max(np.abs(Y - ref)) <= 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("which", ["sr", "lr"])
+def test_chebyshev_ritz_bounds_closed_form_laplacian(which):
+    """The adaptive-bound Chebyshev mode against the 2D Laplacian closed form."""
+    A = G.laplacian_2d(24, 19)
+    exact = G.laplacian_2d_eigenvalues(24, 19)
+    nev = 6
+    r = orc.chebyshev(A, nev, degree=12, maxiter=3000, tol=1e-10, which=which, bounds="ritz")
+    ref = exact[:nev] if which == "sr" else exact[::-1][:nev]
+    assert np.max(np.abs(r["eigenvalues"] - ref) / np.abs(ref)) < 1e-10
+    assert r["residuals"].max() <= 1e-10
+    assert orc.residual_norms(A, r["eigenvalues"], r["eigenvectors"]).max() <= 1.0001e-10
+    # the adaptive mode converges in fewer cycles than the fixed amplification cutoff
+    r0 = orc.chebyshev(A, nev, degree=12, maxiter=3000, tol=1e-10, which=which)
+    assert r["iterations"] < r0["iterations"]
+
+
+@pytest.mark.parametrize("which", ["sr", "lr"])
+def test_chebyshev_ritz_bounds_hamiltonian_matches_eigh(which):
+    """A small 27-point Hamiltonian (the C3 family) in the adaptive mode against
+    LAPACK: eigenvalues to 1e-10 relative and vectors by subspace angle."""
+    A = G.hamiltonian_27pt(9, wells=5)
+    w, V = np.linalg.eigh(A.toarray())
+    nev = 16
+    r = orc.chebyshev(A, nev, degree=10, maxiter=2000, tol=1e-11, which=which, bounds="ritz")
+    ref = w[:nev] if which == "sr" else w[::-1][:nev]
+    Vr = V[:, :nev] if which == "sr" else V[:, ::-1][:, :nev]
+    scale = np.abs(w).max()
+    assert np.max(np.abs(r["eigenvalues"] - ref)) <= 1e-10 * scale
+    # sine of the largest principal angle between the computed and exact
+    # nev-subspaces: ||(I - Vr Vr^T) X||_2 for the orthonormal X
+    X = np.linalg.qr(r["eigenvectors"])[0]
+    assert np.linalg.norm(X - Vr @ (Vr.T @ X), 2) <= 1e-8
+
+
 @pytest.mark.parametrize("mat", ["lap2d", "powlaw", "h27"])
 def test_lanczos_bounds_enclose_spectrum(mat):
     A = {"lap2d": G.laplacian_2d(20), "powlaw": G.power_law(600, seed=2, dmax=60),

@@ -115,10 +115,20 @@ def test_continuous_admission(dev):
 
 
 def test_rejected_requests(dev):
+    from types import SimpleNamespace
     A = G.stencil_with_potential("2d", 900, 1)
+    bad_col = SimpleNamespace(indptr=A.indptr.copy(), indices=A.indices.copy(), data=A.data.copy())
+    bad_col.indices[17] = A.shape[0] + 5                 # column index out of range
+    bad_ptr = SimpleNamespace(indptr=A.indptr.copy(), indices=A.indices.copy(), data=A.data.copy())
+    bad_ptr.indptr[10], bad_ptr.indptr[11] = bad_ptr.indptr[11], bad_ptr.indptr[10]  # rowptr decreases
     with Server(max_slots=4) as srv:
         srv.submit(1, A, 40, 1e-8)          # nev too large for the grouped kernels
         srv.submit(2, A[:40, :40].tocsr(), 32, 1e-8)  # subspace not smaller than n
         srv.submit(3, A, 4, 1e-8)
+        srv.submit(4, bad_col, 4, 1e-8)
+        srv.submit(5, bad_ptr, 4, 1e-8)
+        srv.submit(6, A, 6, 1e-8)           # a valid request after the bad ones still runs
         res = run(srv, [])
+        assert srv.pending() == 0
     assert res[1].status == 2 and res[2].status == 2 and res[3].status == 0
+    assert res[4].status == 2 and res[5].status == 2 and res[6].status == 0
