@@ -71,9 +71,13 @@ def test_cholesky_qr_matches_oracle_and_fails_on_singular(holder):
     assert np.array_equal(Q2, Xs)
 
 
-def graded_block(n, sigmas, seed):
+def graded_block(n, sigmas, seed, avoid=None):
+    """U diag(sigmas) V^T with random orthonormal U (orthogonal to `avoid`)."""
     rng = np.random.default_rng(seed)
-    U = np.linalg.qr(rng.standard_normal((n, len(sigmas))))[0]
+    U0 = rng.standard_normal((n, len(sigmas)))
+    if avoid is not None:
+        U0 -= avoid @ (avoid.T @ U0)
+    U = np.linalg.qr(U0)[0]
     V = np.linalg.qr(rng.standard_normal((len(sigmas), len(sigmas))))[0]
     return (U * np.asarray(sigmas)) @ V.T
 
@@ -90,10 +94,12 @@ def test_orthonormal_qr_shifted_cholqr3_on_ill_conditioned_block(holder, kappa):
     assert np.max(np.abs(Q.T @ Q - np.eye(k))) <= 1e-13
     # span(Q) contains X: the projection residual is at rounding level
     assert np.linalg.norm(X - Q @ (Q.T @ X)) <= 1e-12 * np.linalg.norm(X)
-    # same subspace as the oracle's Householder QR: Q_h^T Q is orthogonal
+    # same subspace as the oracle's Householder QR, up to the perturbation bound:
+    # a rounding-level change of X turns the direction of its smallest singular
+    # value by about eps * kappa, for either method
     Qh, _ = orc.householder_qr(X)
-    s = np.linalg.svd(Qh.T @ Q, compute_uv=False)
-    assert np.max(np.abs(s - 1.0)) <= 1e-10
+    sin_max = np.linalg.norm(Q - Qh @ (Qh.T @ Q), 2)
+    assert sin_max <= max(1e-12, 10 * np.finfo(float).eps * kappa), sin_max
 
 
 def test_orthonormalize_block_rank_drop_matches_oracle(holder):
@@ -123,13 +129,14 @@ def test_orthonormalize_block_graded_singular_values_keeps_oracle_rank(holder, w
     resolve."""
     n = 1500
     sig = [1.0, 3e-2, 1e-3, 2e-5, 1e-7, 3e-9, 1e-10, 3e-14, 1e-15]
-    S = graded_block(n, sig, seed=9)
     rng = np.random.default_rng(10)
     if with_basis:
+        # the part outside span(B) has exactly these singular values
         B = np.linalg.qr(rng.standard_normal((n, 5)))[0]
-        S = S - B @ (B.T @ S) + 0.5 * B @ rng.standard_normal((5, len(sig)))
+        S = graded_block(n, sig, seed=9, avoid=B) + 0.5 * B @ rng.standard_normal((5, len(sig)))
     else:
         B = None
+        S = graded_block(n, sig, seed=9)
     Qo = orc.orthonormalize_block(B if with_basis else np.zeros((n, 1)), 5 if with_basis else 0, S)
     Q = holder.orthonormalize_block(B, S)
     assert Q.shape[1] == Qo.shape[1]
@@ -137,9 +144,9 @@ def test_orthonormalize_block_graded_singular_values_keeps_oracle_rank(holder, w
     assert np.max(np.abs(Q.T @ Q - np.eye(Q.shape[1]))) <= 1e-12
     if with_basis:
         assert np.max(np.abs(B.T @ Q)) <= 1e-12
-    # the subspaces of the well-resolved leading directions agree
-    s = np.linalg.svd(Qo[:, :4].T @ Q[:, :4], compute_uv=False)
-    assert np.max(np.abs(s - 1.0)) <= 1e-8
+    # the same 7-dimensional subspace (its weakest direction, sigma 1e-10, is
+    # fixed only to about eps / 1e-10 by either method)
+    assert np.linalg.norm(Q - Qo @ (Qo.T @ Q), 2) <= 1e-5
 
 
 def test_orthonormalize_block_nothing_survives(holder):
