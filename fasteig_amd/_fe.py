@@ -79,6 +79,7 @@ def lib():
             getattr(L, f"fe_grp_{nm}_run").argtypes = [vp, vp, i32]
         L.fe_spmm.argtypes = [vp, vp, i32, i32, vp, i64, vp, i64, dbl, dbl, vp, i64, dbl]
         L.fe_gram.argtypes = [vp, i32, i32, i64, i32, i32, vp, i64, vp, i64, vp, vp, i64]
+        L.fe_gram_static.argtypes = [vp, i32, i32, i64, i32, i32, vp, i64, vp, i64, vp, vp, i64]
         L.fe_gemm_nn.argtypes = [vp, i32, i64, i32, i32, vp, i64, vp, i64, vp, i64, dbl, dbl]
         L.fe_potrf.argtypes = [vp, i32, vp, i64]
         L.fe_chol_rinv.argtypes = [vp, i32, vp, i64, vp, i64]

@@ -49,6 +49,7 @@ struct fe_ctx_s {
   fe::Scratch ws[4];           // independent scratch slots
   fe::BlockCache cache;        // fe_alloc / fe_free reuse
   int* info_dev = nullptr;     // small device status words
+  void* gram_static = nullptr; // device address of fe_gram_static's fixed scratch on this device
   hipEvent_t events[16] = {};
 };
 
@@ -136,6 +137,11 @@ bool detect_grid(const std::vector<int>& rp, const std::vector<int>& ci, int n, 
 
 // Ensure scratch slot `slot` of ctx holds at least `bytes`.
 int scratch(fe_ctx ctx, int slot, size_t bytes, void** out);
+
+// Looks up (once per device, with ctx->device current) the device address of
+// the fixed scratch array behind fe_gram_static and stores it in
+// ctx->gram_static. Called by fe_ctx_create.
+int gram_static_init(fe_ctx ctx);
 
 // Logical tile index for an XCD-aware launch: the hardware hands workgroups
 // to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), so tile

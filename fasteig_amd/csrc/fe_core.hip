@@ -278,6 +278,13 @@ int fe_ctx_create(int device, fe_ctx* out) {
     return fail(-3, "fe_ctx_create: hipMalloc(info) failed");
   }
   for (auto& e : c->events) hipEventCreate(&e);
+  if (int rc = gram_static_init(c)) {
+    for (auto& e : c->events) hipEventDestroy(e);
+    hipFree(c->info_dev);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return 0;
 }

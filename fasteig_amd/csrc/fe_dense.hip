@@ -16,6 +16,12 @@
 #include <algorithm>
 #include <type_traits>
 
+// Fixed device scratch of fe_gram_static (split-K partial tiles): every GPU the
+// library runs on holds its own copy until the process ends. Host code never
+// uses this name as an address: the per-device address comes from
+// hipGetSymbolAddress (gram_static_init) and travels in fe_ctx_s::gram_static.
+static __device__ __align__(256) double scratch[FE_GRAM_STATIC_BYTES / sizeof(double)];
+
 namespace fe {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
@@ -522,17 +528,58 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
 
 using namespace fe;
 
-extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
-                       const void* Y, int64_t ldy, const double* w, double* G, int64_t ldg) {
-  FE_ARG(ctx && X && Y && G, "fe_gram: null argument");
-  FE_ARG(m >= 0 && p >= 0 && n >= 0 && ldx >= m && ldy >= p && ldg >= m, "fe_gram: bad dimensions");
-  FE_ARG(prec == FE_F64 || prec == FE_F32, "fe_gram: bad precision");
-  FE_ARG(mode == FE_GRAM_NATIVE || (mode == FE_GRAM_BF16X3 && prec == FE_F32), "fe_gram: bad mode");
+namespace fe {
+
+// Device addresses of the fixed scratch, one per device (filled once each).
+static std::mutex g_static_mu;
+static std::map<int, void*> g_static_ptr;
+
+int gram_static_init(fe_ctx ctx) {
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  auto it = g_static_ptr.find(ctx->device);
+  if (it == g_static_ptr.end()) {
+    void* ptr = nullptr;  // ctx->device is the current device (fe_ctx_create)
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(-3, std::string("hipGetSymbolAddress(fe_gram_static scratch): ") + hipGetErrorString(e));
+    it = g_static_ptr.emplace(ctx->device, ptr).first;
+  }
+  ctx->gram_static = it->second;
+  return 0;
+}
+
+// The Gram launch of fe_gram and fe_gram_static. fixed == nullptr: the split-K
+// partials go to the context's growable scratch. Otherwise they go to `fixed`
+// (fixed_bytes long): the row chunks are cut so that their partials fit, and a
+// product whose single chunk does not fit is rejected; nothing is allocated.
+static int gram_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                    const void* Y, int64_t ldy, const double* w, double* G, int64_t ldg, void* fixed,
+                    size_t fixed_bytes, const char* who) {
+  const std::string me(who);
+  FE_ARG(ctx && X && Y && G, me + ": null argument");
+  FE_ARG(m >= 0 && p >= 0 && n >= 0 && ldx >= m && ldy >= p && ldg >= m, me + ": bad dimensions");
+  FE_ARG(prec == FE_F64 || prec == FE_F32, me + ": bad precision");
+  FE_ARG(mode == FE_GRAM_NATIVE || (mode == FE_GRAM_BF16X3 && prec == FE_F32), me + ": bad mode");
   if (m == 0 || p == 0) return 0;
   const bool sym = (X == Y) && (m == p) && (ldx == ldy) && !w;
   const int symi = sym ? 1 : 0;
   if (n == 0) return fe_zero2d(ctx, G, p, m, ldg, 8);
   const int cus = ctx->cus > 0 ? ctx->cus : 256;
+  // partial tiles of one chunk, and the chunk count that fits the fixed buffer
+  auto cap_chunks = [&](int64_t& chunks, int64_t& rows_per, int ntiles, int tsz, int stage) -> int {
+    if (!fixed) return 0;
+    const size_t per_chunk = sizeof(double) * (size_t)ntiles * tsz * tsz;
+    const int64_t fit = static_cast<int64_t>(fixed_bytes / per_chunk);
+    if (fit < 1)
+      return fail(-2, me + ": the partial tiles of one row chunk (" + std::to_string(per_chunk) +
+                          " bytes) exceed the fixed scratch of " + std::to_string(fixed_bytes) + " bytes");
+    if (chunks > fit) {
+      rows_per = (n + fit - 1) / fit;
+      rows_per = (rows_per + stage - 1) / stage * stage;
+      chunks = (n + rows_per - 1) / rows_per;
+    }
+    return 0;
+  };
   if (mode == FE_GRAM_NATIVE) {
     // 64 x 64 tiles (rows split over the waves) for thin or small products,
     // 128 x 128 tiles otherwise
@@ -547,9 +594,10 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
     int64_t rows_per = (n + chunks - 1) / chunks;
     rows_per = (rows_per + RS - 1) / RS * RS;
     chunks = (n + rows_per - 1) / rows_per;
-    void* ws = nullptr;
-    int rc = scratch(ctx, 0, sizeof(double) * (size_t)chunks * ntiles * TT * TT, &ws);
-    if (rc) return rc;
+    if (int rc = cap_chunks(chunks, rows_per, ntiles, TT, RS)) return rc;
+    void* ws = fixed;
+    if (!ws)
+      if (int rc = fe::scratch(ctx, 0, sizeof(double) * (size_t)chunks * ntiles * TT * TT, &ws)) return rc;
     const int64_t units = chunks * ntiles, nb8 = (units + 7) / 8 * 8;
     const int V = prec == FE_F64 ? 2 : 4;
     const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && (reinterpret_cast<uintptr_t>(Y) % 16 == 0) &&
@@ -588,10 +636,10 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
   if (rows_per < KCG) rows_per = KCG;
   rows_per = (rows_per + KCG - 1) / KCG * KCG;
   chunks = (n + rows_per - 1) / rows_per;
-  void* ws = nullptr;
-  const size_t part_bytes = sizeof(double) * (size_t)chunks * ntiles * TILE * TILE;
-  int rc = scratch(ctx, 0, part_bytes, &ws);
-  if (rc) return rc;
+  if (int rc = cap_chunks(chunks, rows_per, ntiles, TILE, KCG)) return rc;
+  void* ws = fixed;
+  if (!ws)
+    if (int rc = fe::scratch(ctx, 0, sizeof(double) * (size_t)chunks * ntiles * TILE * TILE, &ws)) return rc;
   double* part = (double*)ws;
   const dim3 grid((unsigned)ntiles, (unsigned)chunks);
   hipLaunchKernelGGL(k_gram_bf16x3, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
@@ -601,6 +649,22 @@ extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, 
                      tiles_b, ntiles, (int)chunks, part, G, ldg, symi, TILE);
   FE_CHECK_LAUNCH();
   return 0;
+}
+
+}  // namespace fe
+
+extern "C" int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                       const void* Y, int64_t ldy, const double* w, double* G, int64_t ldg) {
+  return fe::gram_run(ctx, mode, prec, n, m, p, X, ldx, Y, ldy, w, G, ldg, nullptr, 0, "fe_gram");
+}
+
+extern "C" int fe_gram_static(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X,
+                              int64_t ldx, const void* Y, int64_t ldy, const double* w, double* G,
+                              int64_t ldg) {
+  FE_ARG(ctx, "fe_gram_static: null context");
+  FE_ARG(ctx->gram_static, "fe_gram_static: the context has no fixed scratch address");
+  return fe::gram_run(ctx, mode, prec, n, m, p, X, ldx, Y, ldy, w, G, ldg, ctx->gram_static,
+                      FE_GRAM_STATIC_BYTES, "fe_gram_static");
 }
 
 extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,

@@ -150,6 +150,23 @@ int fe_spmm(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, v
 int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
             const void* Y, int64_t ldy, const double* w, double* G, int64_t ldg);
 
+/* The same product without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The split-K
+ * partial tiles go to a fixed device array of FE_GRAM_STATIC_BYTES per GPU,
+ * held from the library's first context on that GPU until the process ends;
+ * fe_gram instead grows a context scratch with hipMalloc. Same kernels and
+ * arguments as fe_gram; when that product's row chunks would need more than
+ * the array, fewer (taller) chunks are used (the result then differs from
+ * fe_gram only by summation order), and a product whose single chunk needs
+ * more (over 256 tiles of 128 x 128, e.g. m = p > 2048 without symmetry) is
+ * rejected with -2. The array keeps the last call's partials: every call
+ * writes all it reads, but calls that overlap on one GPU (two contexts' streams
+ * running at once) race on it, so callers serialize them. */
+#define FE_GRAM_STATIC_BYTES ((size_t)32 << 20)
+int fe_gram_static(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                   const void* Y, int64_t ldy, const double* w, double* G, int64_t ldg);
+
 /* ---- tall-skinny times small (replaces right_multiply_active_columns,
  *      include/core/MultiVector.hpp:47-50; CUDA cublasDgemm NN,
  *      src/gpu/CudaMultiVector.cpp:310-359) ---------------------------------
