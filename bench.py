@@ -14,9 +14,12 @@ Also measured: the fused Chebyshev step (the dominant HBM-bound kernel, in
 the solve's own sparse format) with HIP events on the library's own stream,
 against algorithmic bytes nnz*(s+4) + 4(n+1) + 3 n k s; its HBM traffic from
 two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) calibrated on a 1 GiB
-copy; the CPU oracle's FP64 fused step on the host cores (a bounded sample,
-extrapolated to solves/s); the generalized 64^3 problem in plain FP64; and the
-serving stream (512 requests, continuous batching vs one at a time in FP64).
+copy; per-pair eigenvalue errors against the CPU oracle's FP64 answer
+(tests/golden/c3_eigs.json); the CPU oracle on the host cores (whole C1 and
+32^3 solves timed, and the C3 solve extrapolated from its measured parts);
+the generalized 64^3 problem in plain FP64; the serving stream (512 requests,
+continuous batching vs one at a time in FP64); and the power-law C4 matrix
+(its fused step in CSR and SELL, and its three solvers).
 
     python bench.py [--steps K] [--warmup W] [--config c3|c2|small|c5] [--gpus 1]
 """
@@ -179,21 +182,95 @@ def copy_reference(nbytes=1 << 30, reps=5):
     return ms.value / reps, rows * 256 * 4
 
 
-def cpu_sample(A, k, target_s=15.0):
-    """CPU oracle FP64 fused Chebyshev step (Y = 2c A X + 2e X - Z) on the host
-    cores, on k columns; returns seconds per step at that k."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), r
+
+
+def cpu_timed_solves(reps=3):
+    """The CPU oracle's FP64 solves timed like the reference times a solve
+    (src/fasteig_problem.cpp:389-416: the solver call only; the final residual
+    recompute is outside), median of `reps` on all the host threads OpenMP
+    gets and once on one thread, beside the HIP path's time for the same
+    solve (median of `reps` after a warm-up):
+      c1: 2D 5-point Laplacian 100 x 100, lowest 10, LOBPCG block 16, tol 1e-8
+      c3_32: the C3 Hamiltonian family on a 32^3 grid (n = 32,768, 30 wells),
+             lowest 32, Chebyshev degree 20 with adaptive bounds, tol 1e-8."""
+    from fasteig_amd import generators as G
+    from fasteig_amd.api import Problem
+    from oracle import oracle as orc
+    cores = orc.max_threads()
+    cases = {
+        "c1_lobpcg_block16": (G.laplacian_2d(100),
+                              lambda A: orc.lobpcg(A, 10, block=16, maxiter=2000, tol=1e-8, which="sr", seed=1,
+                                                   want_vectors=False),
+                              dict(solver="LOBPCG", nev=10, blksize=16, tol=1e-8, maxiter=2000, which="SR", seed=1)),
+        "c3_32cube_chebyshev": (G.hamiltonian_27pt(32, seed=4, wells=30),
+                                lambda A: orc.chebyshev(A, 32, degree=20, maxiter=300, tol=1e-8, which="sr",
+                                                        seed=1, bounds="ritz", want_vectors=False),
+                                dict(solver="Chebyshev", nev=32, degree=20, tol=1e-8, maxiter=300, which="SR",
+                                     seed=1, cheb_bounds="ritz")),
+    }
+    out = {"cpu": cpu_model(), "threads": cores}
+    for name, (A, ofn, gopts) in cases.items():
+        orc.set_threads(cores)
+        t_all, r = _median_time(lambda: ofn(A), reps)
+        orc.set_threads(1)
+        t_one, r1 = _median_time(lambda: ofn(A), 1)
+        orc.set_threads(cores)
+        p = Problem(A)
+        p.solve(**gopts)
+        t_gpu, rg = _median_time(lambda: p.solve(**gopts), reps)
+        p.close()
+        out[name] = {"n": int(A.shape[0]), "cpu_s": t_all, "cpu_1thread_s": t_one, "cpu_iterations": r["iterations"],
+                     "cpu_max_resid": float(r["residuals"].max()), "gpu_s": t_gpu, "gpu_iterations": rg.iterations,
+                     "gpu_max_resid": float(rg.residual_norms.max()),
+                     "max_rel_eig_diff": float(np.max(np.abs(rg.eigenvalues - r["eigenvalues"]) /
+                                                      np.abs(r["eigenvalues"])))}
+        log(f"cpu {name}: {t_all:.3f}s on {cores} threads, {t_one:.3f}s on 1 ({r['iterations']} iterations); "
+            f"GPU {t_gpu:.4f}s ({rg.iterations})")
+    return out
+
+
+def cpu_c3_extrapolated(A, m, degree, cycles):
+    """One FP64 oracle Chebyshev cycle at the C3 shape from measured parts,
+    times the GPU FP64 solve's cycle count: per cycle `degree` + 1 sparse
+    products on m columns (timed on 16 columns, scaled by m/16), two m x m
+    Gram products and three n x m times m x m products (timed on an eighth of
+    the rows, scaled by 8), and one m x m symmetric eigensolve (timed whole).
+    The parts are the oracle's own routines on the host threads OpenMP gets.
+    Returns (seconds per solve, detail)."""
     from oracle import oracle as orc
     n = A.shape[0]
-    X = np.asfortranarray(np.random.default_rng(1).uniform(-1, 1, (n, k)))
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        Y = orc.spmm(A, X, 0.01, -1.0, X)
-        Y += 0.02 * X
-        reps += 1
-        if time.perf_counter() - t0 > target_s or reps >= 5:
-            break
-    return (time.perf_counter() - t0) / reps, orc.max_threads()
+    rng = np.random.default_rng(2)
+    X16 = np.asfortranarray(rng.uniform(-1, 1, (n, 16)))
+    t_sp, _ = _median_time(lambda: orc.spmm(A, X16), 3)
+    nr = n // 8
+    Q = np.asfortranarray(rng.uniform(-1, 1, (nr, m)))
+    t_gram, _ = _median_time(lambda: orc.gram(Q, Q), 2)
+    Cm = np.asfortranarray(rng.uniform(-1, 1, (m, m)))
+    t_ac, _ = _median_time(lambda: orc.right_multiply(Q, Cm), 2)
+    S = rng.standard_normal((m, m))
+    t_ev, _ = _median_time(lambda: orc.syev(S + S.T), 1)
+    cycle = (degree + 1) * t_sp * (m / 16) + 2 * 8 * t_gram + 3 * 8 * t_ac + t_ev
+    detail = {"spmm_16col_s": t_sp, "gram_eighth_rows_s": t_gram, "rotation_eighth_rows_s": t_ac, "syev_s": t_ev,
+              "cycle_s": cycle, "cycles": cycles}
+    return cycles * cycle, detail
 
 
 def measure_traffic(config, k, fmt, timeout=180):
@@ -264,13 +341,17 @@ def c2_bench(steps=2):
     return out
 
 
-def c4_bench(maxiter=400):
+def c4_bench(maxiter=200):
     """Power-law SPD matrix, n = 2,000,000 (row degrees alpha 2.1 on [4, 2000],
-    symmetrized, diagonal = row abs-sum + 1): the fused step at k = 48 in CSR
-    and SELL-32-sigma (load balance), then the lowest 32 pairs by block
-    Krylov-Schur (block 48, maxdim 192) and LOBPCG block 48 (plain FP64) and
-    by Chebyshev block 48 with the mixed FP32 filter and FP64 refinement,
-    tol 1e-8."""
+    symmetrized, diagonal = row abs-sum + 1), lowest 32 pairs, tol 1e-8: the
+    fused step at k = 48 in CSR and SELL-32-sigma (load balance); Chebyshev
+    block 48 with the mixed FP32 filter and FP64 refinement (the solver that
+    converges here), and the two solvers the configuration names, LOBPCG
+    block 48 and block Krylov-Schur (block 48, maxdim 192), in plain FP64 for
+    `maxiter` iterations with their final residuals. Why those two stall is
+    read off the spectrum: the wanted 32 eigenvalues fill a sliver of the
+    interval [lambda_1, lambda_max] (relative width reported), and neither
+    method has a preconditioner in the reference."""
     from fasteig_amd import _fe
     from fasteig_amd import generators as G
     from fasteig_amd.api import Problem
@@ -285,17 +366,30 @@ def c4_bench(maxiter=400):
             out[f"step_{fmt}_{pname}_k48_gbs"] = algo / (per * 1e-3) / 1e9
             log(f"c4 fused step {fmt} {pname} k=48: {per:.3f} ms {algo / (per * 1e-3) / 1e9:.0f} GB/s")
     p = Problem(A)
-    for name, opts in (("block_krylov_schur_fp64", dict(solver="BlockKrylovSchur", blksize=48, maxdim=192)),
-                       ("lobpcg_fp64", dict(solver="LOBPCG", blksize=48, precision="fp64")),
-                       ("chebyshev_mixed", dict(solver="Chebyshev", blksize=48, degree=20, precision="mixed",
-                                                cheb_bounds="ritz"))):
+    p.solve(solver="LOBPCG", nev=32, blksize=48, tol=1e-8, which="SR", maxiter=1, seed=1)  # upload, start block
+    lo, hi = p.lanczos_bounds(1.0, 1)
+    runs = (("chebyshev_mixed", dict(solver="Chebyshev", blksize=48, degree=20, precision="mixed",
+                                     cheb_bounds="ritz", maxiter=400)),
+            ("lobpcg_fp64", dict(solver="LOBPCG", blksize=48, precision="fp64", maxiter=maxiter)),
+            ("block_krylov_schur_fp64", dict(solver="BlockKrylovSchur", blksize=48, maxdim=192, maxiter=maxiter)))
+    for name, opts in runs:
         t = time.perf_counter()
-        r = p.solve(nev=32, tol=1e-8, which="SR", maxiter=maxiter, seed=1, verbosity=2, print_every=10, **opts)
+        r = p.solve(nev=32, tol=1e-8, which="SR", seed=1, **opts)
         dt = time.perf_counter() - t
-        out[name] = {"s": dt, "iterations": r.iterations, "status": r.status,
-                     "max_resid": float(r.residual_norms.max()), "lowest": float(r.eigenvalues[0])}
+        out[name] = {"s": dt, "iterations": r.iterations, "status": r.status, "converged": bool(r.status == 0),
+                     "max_resid": float(r.residual_norms.max()), "lowest": float(r.eigenvalues[0]),
+                     "s_per_iteration": dt / max(1, r.iterations)}
+        if name == "chebyshev_mixed":
+            ev = r.eigenvalues
+            out["spectrum"] = {"lambda_1": float(ev[0]), "lambda_32": float(ev[-1]), "lanczos_upper": hi,
+                               "wanted_relative_width": float((ev[-1] - ev[0]) / (hi - ev[0]))}
         log(f"c4 {name}: {dt:.2f}s, {r.iterations} iterations, status {r.status}, "
             f"max resid {r.residual_norms.max():.2e}")
+    out["stall_reason"] = ("the 32 wanted eigenvalues fill a relative width of "
+                           f"{out['spectrum']['wanted_relative_width']:.1e} of [lambda_1, lambda_max]; unpreconditioned "
+                           "LOBPCG and block Krylov-Schur converge at a rate set by that gap ratio, the "
+                           "Chebyshev filter with adaptive bounds damps the rest of the interval directly")
+    p.close()
     return out
 
 
@@ -474,10 +568,13 @@ def main():
         modes["fp64"] = dict(base, precision="fp64")
     lat = {m: [] for m in modes}
     res = {}
+    cold_s = None
     for m, opts in modes.items():
         for w in range(args.warmup):
             t = time.perf_counter()
             r = p.solve(**opts)
+            if cold_s is None:  # matrix upload, format build, seeded start block on the host
+                cold_s = time.perf_counter() - t
             log(f"warmup {m}: {time.perf_counter() - t:.3f}s iters={r.iterations} "
                 f"maxres={r.residual_norms.max():.2e}")
         for s in range(args.steps):
@@ -500,6 +597,15 @@ def main():
         extra["mixed_over_fp64"] = extra["fp64_solves_per_s"] and value / extra["fp64_solves_per_s"]
         extra["fp64_iterations"] = res["fp64"].iterations
     extra["mixed_iterations"] = res["mixed"].iterations
+    if args.config == "c3":
+        # per-pair error against the oracle's FP64 eigenvalues (run to 1e-12)
+        ref = np.array(json.load(open(os.path.join(ROOT, "tests", "golden", "c3_eigs.json")))["eigenvalues"])
+        for m, r in res.items():
+            err = np.abs(r.eigenvalues - ref) / np.abs(ref)
+            out_acc[f"max_rel_eig_err_{m}_vs_oracle"] = float(err.max())
+            out_acc[f"median_rel_eig_err_{m}_vs_oracle"] = float(np.median(err))
+            out_acc[f"rel_eig_err_{m}_vs_oracle_per_pair"] = [float(f"{e:.2e}") for e in err]
+    extra["cold_first_solve_s"] = cold_s
     del p
 
     # dominant kernel: fused Chebyshev step at the solve's shape (FP32, k = nev)
@@ -524,26 +630,25 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        kc = 8
-        t_step, cores = cpu_sample(A, kc)
         from oracle import oracle as orc
-        orc.set_threads(1)
-        t_step1, _ = cpu_sample(A, kc, target_s=5.0)
-        orc.set_threads(cores)
-        t_step_full = t_step * (k / kc)
-        # SpMM column-applications of the FP64 GPU solve: cycles x (degree*nev + 2*nev)
+        cores = orc.max_threads()
+        m_sub = cfg["blksize"] or min(n, 2 * cfg["nev"])
         cycles = res.get("fp64", res["mixed"]).iterations
-        t_solve = cycles * (cfg["degree"] + 2) * t_step_full
+        t_solve, cdet = cpu_c3_extrapolated(A, m_sub, cfg["degree"], cycles)
+        timed = cpu_timed_solves()
         cpu = {"value": 1.0 / t_solve, "unit": "solves/s", "cores": cores, "kind": "port",
-               "sample": f"oracle FP64 fused Chebyshev step on {kc} of {k} columns at this n/nnz "
-                         f"({t_step:.2f} s/step), scaled by {k}/{kc} and by the plain-FP64 GPU solve's "
-                         f"{cycles} cycles x {cfg['degree'] + 2} block products; dense work excluded",
-               "value_1_core": 1.0 / (cycles * (cfg["degree"] + 2) * t_step1 * (k / kc))}
-        log(f"cpu sample: {t_step:.2f}s per k={kc} step on {cores} threads, {t_step1:.2f}s on 1")
+               "sample": f"EXTRAPOLATED: one FP64 oracle Chebyshev cycle at this shape from measured parts "
+                         f"({cfg['degree'] + 1} sparse products on {m_sub} columns timed on 16, two Gram and "
+                         f"three rotation products timed on an eighth of the rows, one {m_sub} x {m_sub} "
+                         f"eigensolve) x the plain-FP64 GPU solve's {cycles} cycles; timed whole solves "
+                         f"(median of 3, all threads and 1 thread) in cpu_timed",
+               "cpu": timed["cpu"], "detail": cdet, "cpu_timed": timed}
+        log(f"cpu C3 extrapolated: {t_solve:.1f}s per solve on {cores} threads ({cdet})")
 
     if not args.no_extra:
         extra["c2_generalized_fp64"] = c2_bench()
         extra["c5_serving"] = serving_bench()
+        extra["c4_power_law"] = c4_bench()
     traffic, tdetail = (None, "skipped") if args.no_traffic else measure_traffic(args.config, k, fmt)
     extra["traffic_detail"] = tdetail
     log(f"traffic: {traffic} ({tdetail})")
@@ -564,7 +669,9 @@ def main():
         "config": {"workload": {"c3": "H=-0.5*L27+V 96^3 (n=884736), nev 256, Chebyshev degree 20, tol 1e-8",
                                 "c2": "K u = l M u 7-pt 64^3, nev 64, block 96, degree 16",
                                 "small": "H=-0.5*L27+V 32^3, nev 32, degree 20"}[args.config],
-                   "n": n, "nnz": int(A.nnz), "gram": args.gram, "format": args.format},
+                   "n": n, "nnz": int(A.nnz), "gram": args.gram, "format": args.format,
+                   "timed": "solve only: matrix and the seeded start block resident on the device between "
+                            "steps (the first, cold solve is extra.cold_first_solve_s)"},
         "latency_s": {m: {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99))}
                       for m, v in lat.items()},
         "accuracy": out_acc,

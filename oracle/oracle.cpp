@@ -20,6 +20,8 @@
 //   orc_cholqr          same invariants, -1 on a singular Gram
 //   orc_jacobi_svd      numpy.linalg.svd singular values; U^T U = I
 //   orc_spmm            dense brute force (numpy) on small matrices
+//   orc_gram / orc_right_multiply
+//                       numpy X^T Y and X C
 //   orc_lanczos_bounds  bounds enclose the spectrum of small matrices (eigh)
 //   orc_cheb_scalars    closed form T_d(c*lambda+e) vs numpy Chebyshev
 //   orc_cheb_filter_scaled
@@ -78,10 +80,21 @@ struct Csr {
 
 // ---------------------------------------------------------------- dense kernels
 
-// The tall-skinny products below are cache-blocked (rows packed row-major in
-// small chunks, 4 x 8 register blocks): the same sums as the textbook loops,
-// in an order that keeps the operands in cache, so that the checker finishes
-// full-size problems.
+// The tall-skinny products below are cache-blocked: rows are taken in chunks,
+// packed into panels of 4 (left operand) and 8 (right operand) columns, and
+// multiplied in 4 x 8 register blocks (AVX2 vectors of 4 doubles). Same sums
+// as the textbook loops, in an order that keeps the operands in cache, so
+// that the checker finishes full-size problems.
+typedef double v4d __attribute__((vector_size(32)));
+
+namespace {
+inline v4d bcast(double x) { return v4d{x, x, x, x}; }
+inline v4d loadu(const double* p) {
+  v4d v;
+  std::memcpy(&v, p, sizeof(v));
+  return v;
+}
+}  // namespace
 
 // C = A^T B over the first ka columns of A and kb of B (A,B n-row blocks).
 // Rows are cut into a fixed number of slabs (independent of the thread count,
@@ -91,55 +104,70 @@ Mat AtB(const Mat& A, int ka, const Mat& B, int kb) {
   Mat C(ka, kb);
   const long n = A.r;
   if (n == 0 || ka == 0 || kb == 0) return C;
-  constexpr long RB = 64;     // rows packed per chunk
-  const long slabs = std::max(1L, std::min(64L, n / 1024));
+  constexpr long RB = 256;  // rows per packed chunk
+  const int pa = (ka + 3) / 4, pb = (kb + 7) / 8;  // panels
+  const long slabs = std::max(1L, std::min(64L, n / 2048));
   const long per = (n + slabs - 1) / slabs;
-  std::vector<double> part(static_cast<size_t>(slabs) * ka * kb, 0.0);
+  std::vector<double> part(static_cast<size_t>(slabs) * pa * 4 * pb * 8, 0.0);
 #pragma omp parallel
   {
-    std::vector<double> ap(RB * ka), bp(RB * kb);
+    std::vector<double> ap(static_cast<size_t>(pa) * RB * 4), bp(static_cast<size_t>(pb) * RB * 8);
 #pragma omp for schedule(dynamic, 1)
     for (long sl = 0; sl < slabs; ++sl) {
-      double* P = part.data() + static_cast<size_t>(sl) * ka * kb;  // row-major ka x kb
+      double* P = part.data() + static_cast<size_t>(sl) * pa * 4 * pb * 8;  // [pa*4][pb*8] row-major
       const long r0 = sl * per, r1 = std::min(n, r0 + per);
       for (long c0 = r0; c0 < r1; c0 += RB) {
         const long rb = std::min(RB, r1 - c0);
-        for (int i = 0; i < ka; ++i) {
-          const double* x = A.col(i) + c0;
-          for (long t = 0; t < rb; ++t) ap[t * ka + i] = x[t];
+        // panels: ap[(panel * RB + t) * 4 + u] = A(c0 + t, 4 panel + u), zero padded
+        for (int i = 0; i < pa * 4; ++i) {
+          double* dst = &ap[(static_cast<size_t>(i / 4) * RB) * 4 + (i % 4)];
+          if (i < ka) {
+            const double* x = A.col(i) + c0;
+            for (long t = 0; t < rb; ++t) dst[t * 4] = x[t];
+          } else {
+            for (long t = 0; t < rb; ++t) dst[t * 4] = 0.0;
+          }
         }
-        for (int j = 0; j < kb; ++j) {
-          const double* y = B.col(j) + c0;
-          for (long t = 0; t < rb; ++t) bp[t * kb + j] = y[t];
+        for (int j = 0; j < pb * 8; ++j) {
+          double* dst = &bp[(static_cast<size_t>(j / 8) * RB) * 8 + (j % 8)];
+          if (j < kb) {
+            const double* y = B.col(j) + c0;
+            for (long t = 0; t < rb; ++t) dst[t * 8] = y[t];
+          } else {
+            for (long t = 0; t < rb; ++t) dst[t * 8] = 0.0;
+          }
         }
-        for (int i0 = 0; i0 < ka; i0 += 4) {
-          const int ib = std::min(4, ka - i0);
-          for (int j0 = 0; j0 < kb; j0 += 8) {
-            const int jb = std::min(8, kb - j0);
-            double acc[4][8] = {};
+        for (int ip = 0; ip < pa; ++ip) {
+          const double* ar = &ap[static_cast<size_t>(ip) * RB * 4];
+          for (int jp = 0; jp < pb; ++jp) {
+            const double* br = &bp[static_cast<size_t>(jp) * RB * 8];
+            v4d acc[4][2] = {};
             for (long t = 0; t < rb; ++t) {
-              const double* ar = &ap[t * ka + i0];
-              const double* br = &bp[t * kb + j0];
-              if (ib == 4 && jb == 8) {
-                for (int u = 0; u < 4; ++u)
-                  for (int v = 0; v < 8; ++v) acc[u][v] += ar[u] * br[v];
-              } else {
-                for (int u = 0; u < ib; ++u)
-                  for (int v = 0; v < jb; ++v) acc[u][v] += ar[u] * br[v];
+              const v4d b0 = loadu(br + t * 8), b1 = loadu(br + t * 8 + 4);
+              for (int u = 0; u < 4; ++u) {
+                const v4d a = bcast(ar[t * 4 + u]);
+                acc[u][0] += a * b0;
+                acc[u][1] += a * b1;
               }
             }
-            for (int u = 0; u < ib; ++u)
-              for (int v = 0; v < jb; ++v) P[static_cast<size_t>(i0 + u) * kb + j0 + v] += acc[u][v];
+            for (int u = 0; u < 4; ++u) {
+              double* prow = P + static_cast<size_t>(ip * 4 + u) * pb * 8 + jp * 8;
+              for (int v = 0; v < 4; ++v) {
+                prow[v] += acc[u][0][v];
+                prow[4 + v] += acc[u][1][v];
+              }
+            }
           }
         }
       }
     }
   }
+  const size_t ld = static_cast<size_t>(pb) * 8;
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < ka; ++i)
     for (int j = 0; j < kb; ++j) {
       double s = 0.0;
-      for (long sl = 0; sl < slabs; ++sl) s += part[static_cast<size_t>(sl) * ka * kb + static_cast<size_t>(i) * kb + j];
+      for (long sl = 0; sl < slabs; ++sl) s += part[static_cast<size_t>(sl) * pa * 4 * ld + i * ld + j];
       C(i, j) = s;
     }
   return C;
@@ -151,47 +179,46 @@ Mat AC(const Mat& A, int k, const Mat& C) {
   const long n = A.r;
   const int p = C.c;
   if (n == 0 || p == 0) return Y;
-  std::vector<double> ct(static_cast<size_t>(k) * p);  // C row-major
+  const int pp = (p + 7) / 8;
+  std::vector<double> ct(static_cast<size_t>(pp) * std::max(k, 1) * 8, 0.0);  // [panel][i][8]
   for (int i = 0; i < k; ++i)
-    for (int j = 0; j < p; ++j) ct[static_cast<size_t>(i) * p + j] = C(i, j);
-  constexpr long RB = 64;
+    for (int j = 0; j < p; ++j) ct[(static_cast<size_t>(j / 8) * k + i) * 8 + (j % 8)] = C(i, j);
+  constexpr long RB = 64;  // rows per chunk (16 row quads)
   const long chunks = (n + RB - 1) / RB;
 #pragma omp parallel
   {
-    std::vector<double> ap(RB * std::max(k, 1)), yp(RB * p);
+    std::vector<double> ap(static_cast<size_t>(RB / 4) * std::max(k, 1) * 4);  // [quad][i][4]
 #pragma omp for schedule(static)
     for (long ch = 0; ch < chunks; ++ch) {
       const long c0 = ch * RB, rb = std::min(RB, n - c0);
+      const long quads = (rb + 3) / 4;
       for (int i = 0; i < k; ++i) {
         const double* x = A.col(i) + c0;
-        for (long t = 0; t < rb; ++t) ap[t * k + i] = x[t];
+        for (long t = 0; t < quads * 4; ++t)
+          ap[(static_cast<size_t>(t / 4) * k + i) * 4 + (t % 4)] = t < rb ? x[t] : 0.0;
       }
-      for (long t0 = 0; t0 < rb; t0 += 4) {
-        const int tb = static_cast<int>(std::min(4L, rb - t0));
-        for (int j0 = 0; j0 < p; j0 += 8) {
-          const int jb = std::min(8, p - j0);
-          double acc[4][8] = {};
+      for (long q = 0; q < quads; ++q) {
+        const double* ar = &ap[static_cast<size_t>(q) * k * 4];
+        for (int jp = 0; jp < pp; ++jp) {
+          const double* cr = &ct[static_cast<size_t>(jp) * k * 8];
+          v4d acc[4][2] = {};
           for (int i = 0; i < k; ++i) {
-            const double* cr = &ct[static_cast<size_t>(i) * p + j0];
-            if (tb == 4 && jb == 8) {
-              for (int u = 0; u < 4; ++u) {
-                const double a = ap[(t0 + u) * k + i];
-                for (int v = 0; v < 8; ++v) acc[u][v] += a * cr[v];
-              }
-            } else {
-              for (int u = 0; u < tb; ++u) {
-                const double a = ap[(t0 + u) * k + i];
-                for (int v = 0; v < jb; ++v) acc[u][v] += a * cr[v];
-              }
+            const v4d c0v = loadu(cr + i * 8), c1v = loadu(cr + i * 8 + 4);
+            for (int u = 0; u < 4; ++u) {
+              const v4d a = bcast(ar[i * 4 + u]);
+              acc[u][0] += a * c0v;
+              acc[u][1] += a * c1v;
             }
           }
-          for (int u = 0; u < tb; ++u)
-            for (int v = 0; v < jb; ++v) yp[(t0 + u) * p + j0 + v] = acc[u][v];
+          for (int u = 0; u < 4; ++u) {
+            const long t = q * 4 + u;
+            if (t >= rb) break;
+            for (int v = 0; v < 8; ++v) {
+              const int j = jp * 8 + v;
+              if (j < p) Y(static_cast<int>(c0 + t), j) = v < 4 ? acc[u][0][v] : acc[u][1][v - 4];
+            }
+          }
         }
-      }
-      for (int j = 0; j < p; ++j) {
-        double* y = Y.col(j) + c0;
-        for (long t = 0; t < rb; ++t) y[t] = yp[t * p + j];
       }
     }
   }
@@ -307,7 +334,7 @@ int householder_qr(Mat& X, Mat& R) {
     for (int i = j + 1; i < n; ++i) h[i] *= scale;
     h[j] = beta;
     // apply I - tau v v^T (v = [1; h[j+1:]]) to the trailing columns
-#pragma omp parallel for schedule(static) if (static_cast<long>(n) * (k - j) > 200000)
+#pragma omp parallel for schedule(static) if (static_cast<long>(n) * (k - j) > 16384)
     for (int c = j + 1; c < k; ++c) {
       double* y = H.col(c);
       double s = y[j];
@@ -326,7 +353,7 @@ int householder_qr(Mat& X, Mat& R) {
   for (int j = k - 1; j >= 0; --j) {
     if (tau[j] == 0.0) continue;
     const double* h = H.col(j);
-#pragma omp parallel for schedule(static) if (static_cast<long>(n) * (k - j) > 200000)
+#pragma omp parallel for schedule(static) if (static_cast<long>(n) * (k - j) > 16384)
     for (int c = j; c < k; ++c) {
       double* y = Q.col(c);
       double s = y[j];
@@ -1673,6 +1700,14 @@ int orc_max_threads() {
 }
 
 void orc_random_block(int n, int k, unsigned seed, double* out) { unwrap(orc::random_block(n, k, seed), out); }
+
+// G = X^T Y (ka x kb) and Y = X C (n x p): the dense products of the solvers.
+void orc_gram(int n, int ka, const double* X, int kb, const double* Y, double* G) {
+  unwrap(orc::AtB(wrap(X, n, ka), ka, wrap(Y, n, kb), kb), G);
+}
+void orc_right_multiply(int n, int k, const double* X, int p, const double* Cm, double* Y) {
+  unwrap(orc::AC(wrap(X, n, k), k, wrap(Cm, k, p)), Y);
+}
 
 void orc_spmm(int n, const int* rp, const int* ci, const double* v, int k, const double* X, double* Y,
               double alpha, double beta) {

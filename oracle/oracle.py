@@ -37,6 +37,8 @@ def lib():
         L.orc_random_block.argtypes = [C.c_int, C.c_int, C.c_uint, _dp]
         L.orc_spmm.argtypes = [C.c_int, _ip, _ip, _dp, C.c_int, _dp, _dp, C.c_double, C.c_double]
         L.orc_syev.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_gram.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.orc_right_multiply.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp]
         L.orc_cholqr.argtypes = [C.c_int, C.c_int, _dp, _dp]
         L.orc_householder_qr.argtypes = [C.c_int, C.c_int, _dp, _dp]
         L.orc_jacobi_svd.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
@@ -104,6 +106,22 @@ def spmm(A, X, alpha=1.0, beta=0.0, Y=None):
     n, k = X.shape
     Y = np.zeros((n, k), order="F") if Y is None else _F(Y).copy(order="F")
     lib().orc_spmm(n, rp, ci, v, k, X, Y, alpha, beta)
+    return Y
+
+
+def gram(X, Y):
+    """X^T Y of two n-row blocks."""
+    X, Y = _F(X), _F(Y)
+    G = np.zeros((X.shape[1], Y.shape[1]), order="F")
+    lib().orc_gram(X.shape[0], X.shape[1], X, Y.shape[1], Y, G)
+    return G
+
+
+def right_multiply(X, Cm):
+    """X C for an n x k block and a k x p matrix."""
+    X, Cm = _F(X), _F(Cm)
+    Y = np.zeros((X.shape[0], Cm.shape[1]), order="F")
+    lib().orc_right_multiply(X.shape[0], X.shape[1], X, Cm.shape[1], Cm, Y)
     return Y
 
 

@@ -111,6 +111,16 @@ def test_orthonormalize_block_drops_dependent_directions():
     assert np.max(np.abs(B.T @ Q)) < 1e-13
 
 
+@pytest.mark.parametrize("n,ka,kb", [(1, 1, 1), (70, 5, 3), (5000, 37, 20), (70001, 16, 64)])
+def test_dense_products_match_numpy(n, ka, kb):
+    rng = np.random.default_rng(n + ka)
+    X = rng.standard_normal((n, ka))
+    Y = rng.standard_normal((n, kb))
+    Cm = rng.standard_normal((ka, kb))
+    assert np.max(np.abs(orc.gram(X, Y) - X.T @ Y)) <= 1e-12 * np.sqrt(n)
+    assert np.max(np.abs(orc.right_multiply(X, Cm) - X @ Cm)) <= 1e-12 * np.sqrt(ka)
+
+
 def test_spmm_matches_dense():
     A = G.power_law(400, seed=1, dmax=50)
     X = np.random.default_rng(4).standard_normal((400, 6))
