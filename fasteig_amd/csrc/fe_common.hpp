@@ -101,6 +101,8 @@ struct fe_csr_s {
   int64_t* tl_lptr = nullptr;    // X rows each tile loads (ntiles + 1)
   int* tl_lslot = nullptr;
   int* tl_lcol = nullptr;
+  int* tl_sslot = nullptr;       // slot of each tile row's own X row (-1: not in its tile)
+  int tl_self = 0;               // every tile row's own X row is in its tile's slots
   int use_tl = 0;
 };
 

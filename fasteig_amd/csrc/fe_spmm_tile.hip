@@ -34,9 +34,10 @@ namespace {
 typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-temporal access
 
 constexpr int TL_ROWS = 16;   // rows per tile: 4 wavefronts x 4 rows of 16 lanes
-constexpr int TL_PF = 4;      // 16-byte X loads per thread held in registers across a tile
+constexpr int TL_PF = 3;      // 16-byte X loads per thread held in registers per tile in flight
 constexpr int TL_PENCIL = 24;  // tiles per pencil at most
 constexpr int TL_MAX_SLOTS = 240;  // LDS slots: with the zero slot and the pencil offsets within 64 KB
+static_assert(TL_MAX_SLOTS < 256, "own-row slots are staged as bytes");
 
 // Lane j of each 16-lane row of the wavefront, broadcast to the row (DPP
 // row_newbcast).
@@ -51,9 +52,10 @@ __device__ __forceinline__ int row_bcast(int v) {
 template <bool O32, typename P>
 __device__ __forceinline__ P* elem(P* base, int row, int64_t ld, int col) {
   using B = std::conditional_t<std::is_const_v<P>, const char, char>;
-  if constexpr (O32)
+  if constexpr (O32)  // rows and ld below 2^24: a full-rate 24-bit multiply
     return reinterpret_cast<P*>(reinterpret_cast<B*>(base) +
-                                (static_cast<uint32_t>(row) * static_cast<uint32_t>(ld) + static_cast<uint32_t>(col)) *
+                                (__umul24(static_cast<uint32_t>(row), static_cast<uint32_t>(ld)) +
+                                 static_cast<uint32_t>(col)) *
                                     static_cast<uint32_t>(sizeof(P)));
   else
     return base + static_cast<int64_t>(row) * ld + col;
@@ -88,6 +90,18 @@ struct Unroll<END, END> {
   __device__ __forceinline__ static void run(F&) {}
 };
 
+// Workgroup barrier for the LDS slots only: this wave's LDS operations have
+// completed (lgkmcnt 0) when it reaches the barrier, while its global loads
+// stay in flight. __syncthreads() also waits for every outstanding global
+// load (vmcnt 0), which made each tile of the loop below wait a whole memory
+// round trip.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // gfx9 encoding: vmcnt 63, expcnt 7, lgkmcnt 0
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 }  // namespace
 
 // Workgroup: 256 threads; thread tid owns tile row tid / 16 and the 16 bytes
@@ -95,14 +109,31 @@ struct Unroll<END, END> {
 // slice, so the slices of one pencil run side by side on one XCD and share
 // its entry lists through L2.
 //
-// Every global load a tile needs is issued one tile ahead, so its latency
-// overlaps the previous tile's products: the tile's entries (two chunks of
-// 16 per row), its rows of Z and X for the fused terms, and the new X rows
-// for LDS; the load lists of those X rows are read two tiles ahead. The
-// pencil's row, entry and load offsets are staged in LDS first.
-template <typename T, bool O32>
+// Loads are issued ahead of the tile that needs them and no wait covers a
+// load issued in the same tile: the new X rows of a tile two tiles ahead
+// (held in registers until, after the previous tile's products, the slots
+// they go to are free), the entries (two chunks of 16 per row) and the Z and
+// X rows of the fused terms one tile ahead, and the load lists three tiles
+// ahead. The loop is unrolled by two so that these registers keep fixed roles
+// (a register copy of a value still in flight would wait for it), and its
+// barriers wait for LDS operations only (with loads issued one tile ahead
+// under __syncthreads the kernel took 0.74 of its 0.91 ms with the products
+// left out: C3 shape, FP32, 256 columns). A single barrier per tile, with
+// the plan keeping a tile's new rows off the previous tile's slots (36 more
+// slots for a 27-point stencil), measured 2% slower. The pencil's row, entry
+// and load offsets are staged in LDS first.
+// FULL: every column slice lies inside the block (k a multiple of the slice
+// width), so no load needs a partial-slice path. The prefetch loads run
+// unconditionally, from a clamped address when their lane has nothing to
+// load, and validity is applied where the value is used: a load under a
+// branch, or merged with a constant, is waited for where the branches join.
+// SELF: each tile row's own X row is among its tile's slots (a stored
+// diagonal), so the beta X term reads it from LDS: no second read of X from
+// HBM and no registers holding it.
+template <typename T, bool O32, bool FULL, bool SELF>
 __global__ __launch_bounds__(256) void k_spmm_tile(
-    const int64_t* __restrict__ pencil, const int* __restrict__ rows, const int64_t* __restrict__ rptr,
+    const int64_t* __restrict__ pencil, const int* __restrict__ rows, const int* __restrict__ sslot,
+    const int64_t* __restrict__ rptr,
     const int* __restrict__ maxe, const int* __restrict__ eslot, const T* __restrict__ ev,
     const int64_t* __restrict__ lptr, const int* __restrict__ lslot, const int* __restrict__ lcol, int zslot, int k,
     int nslices, const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta, const T* Z, int64_t ldz,
@@ -112,6 +143,9 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
   __shared__ int s_rp[TL_PENCIL * TL_ROWS + 1];   // entry offsets of its tile rows
   __shared__ int s_rows[TL_PENCIL * TL_ROWS];     // its tile rows
   __shared__ int s_maxe[TL_PENCIL];
+  // own-row slots of its tile rows (slots < TL_MAX_SLOTS fit a byte: with
+  // int entries the workgroup's LDS passed the 32 KB that five per CU allow)
+  __shared__ unsigned char s_self[SELF ? TL_PENCIL * TL_ROWS : 1];
   constexpr int V = 16 / (int)sizeof(T);
   constexpr int CW = 16 * V;  // columns per slice
   const int64_t work = xcd_tile(blockIdx.x, per_xcd);
@@ -124,7 +158,7 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
   const int c = c0 + sub * V;  // this thread's first column
   const bool fz = Z && gamma != T(0);
   const bool fx = beta != T(0);
-  const bool full = c0 + CW <= k;  // the whole slice lies inside the block
+  const bool full = FULL || c0 + CW <= k;  // the whole slice lies inside the block
   const int64_t t0 = pencil[p];
   const int nt = static_cast<int>(pencil[p + 1] - t0);  // at most TL_PENCIL
   const int64_t lbase = lptr[t0], ebase = rptr[t0 * TL_ROWS];
@@ -132,12 +166,15 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
   for (int i = tid; i <= nt * TL_ROWS; i += 256) s_rp[i] = static_cast<int>(rptr[t0 * TL_ROWS + i] - ebase);
   for (int i = tid; i < nt * TL_ROWS; i += 256) s_rows[i] = rows[t0 * TL_ROWS + i];
   for (int i = tid; i < nt; i += 256) s_maxe[i] = maxe[t0 + i];
+  if (SELF)
+    for (int i = tid; i < nt * TL_ROWS; i += 256) s_self[i] = static_cast<unsigned char>(max(sslot[t0 * TL_ROWS + i], 0));
   if (tid < 16) xs[zslot * 16 + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int* ls = lslot + lbase;
   const int* lc = lcol + lbase;
   const int* es = eslot + ebase;
   const T* ew = ev + ebase;
 
+  // part `part` (16 bytes) of X row `col` in this slice (zeros past k)
   auto load_x = [&](int col, int part) {
     const int cc = c0 + part * V;
     const T* src = elem<O32>(X, col, ldx, cc);
@@ -152,12 +189,15 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
     }
     return f;
   };
-  // this thread's 16 bytes of row `row` of a block B (zeros outside k)
+  // this thread's 16 bytes of row `row` (>= 0) of a block B (zeros outside k)
   auto load_row = [&](const T* B, int64_t ld, int row, T (&v)[V]) {
-    if (row < 0 || c >= k) {
+    if (FULL) {
+      const nt_f4 f = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(elem<O32>(B, row, ld, c)));
+      __builtin_memcpy(v, &f, 16);
+    } else if (c >= k) {
 #pragma unroll
       for (int q = 0; q < V; ++q) v[q] = T(0);
-    } else if (full || c + V <= k) {
+    } else if (c + V <= k) {
       const nt_f4 f = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(elem<O32>(B, row, ld, c)));
       __builtin_memcpy(v, &f, 16);
     } else {
@@ -165,61 +205,83 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
       for (int q = 0; q < V; ++q) v[q] = c + q < k ? elem<O32>(B, row, ld, c)[q] : T(0);
     }
   };
-  // per-tile registers, filled one tile ahead
+  // per-tile registers: two chunks of 16 entries of this thread's row (as
+  // loaded: valid for lanes below cnt), and its rows of Z and X
   struct Ahead {
     int ms[2];
     T mw[2];
-    T z[V], x[V];
+    int cnt;
+    T z[V], x[SELF ? 1 : V];
   };
   auto fetch = [&](int tt, Ahead& a) {
     const int e0 = s_rp[tt * TL_ROWS + tr];
     const int cnt = s_rp[tt * TL_ROWS + tr + 1] - e0;
+    a.cnt = cnt;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      a.ms[h] = zslot;
-      a.mw[h] = T(0);
-      if (16 * h + sub < cnt) {
-        a.ms[h] = es[e0 + 16 * h + sub];
-        a.mw[h] = ew[e0 + 16 * h + sub];
-      }
+      // the pencil's first entry stands in for a lane past the row's end (the
+      // host appends one entry after the last pencil's, so it always exists)
+      const int e = 16 * h + sub < cnt ? e0 + 16 * h + sub : 0;
+      a.ms[h] = es[e];
+      a.mw[h] = ew[e];
     }
-    const int row = s_rows[tt * TL_ROWS + tr];
+    const int row = max(s_rows[tt * TL_ROWS + tr], 0);  // a padding row reads row 0, unused
     if (fz) load_row(Z, ldz, row, a.z);
-    if (fx) load_row(X, ldx, row, a.x);
+    if constexpr (!SELF)
+      if (fx) load_row(X, ldx, row, a.x);
   };
-  // load list entries (slot, column) of tile tt for this thread's TL_PF loads
-  auto fetch_list = [&](int tt, int (&sl)[TL_PF], int (&cl)[TL_PF]) {
+  // load list entries (slot, column) of tile tt for this thread's TL_PF
+  // loads, as loaded; bit j of *valid tells whether load j is one
+  auto fetch_list = [&](int tt, int (&sl)[TL_PF], int (&cl)[TL_PF], int& valid) {
     const int l0 = tt < nt ? s_lp[tt] : 0;
     const int nl = tt < nt ? s_lp[tt + 1] - l0 : 0;
+    valid = 0;
 #pragma unroll
     for (int j = 0; j < TL_PF; ++j) {
       const int i = tid + 256 * j;
-      sl[j] = -1;
-      cl[j] = 0;
-      if (i < nl * 16) {
-        sl[j] = ls[l0 + (i >> 4)] * 16 + (i & 15);
-        cl[j] = lc[l0 + (i >> 4)];
-      }
+      const bool v = i < nl * 16;
+      const int e = v ? l0 + (i >> 4) : 0;  // as for the entries: the pencil's first load stands in
+      sl[j] = ls[e];
+      cl[j] = lc[e];
+      valid |= v ? 1 << j : 0;
     }
   };
-  __syncthreads();
+  lds_barrier();
   // the first tile's X rows
   for (int i = tid; i < s_lp[1] * 16; i += 256) xs[ls[i >> 4] * 16 + (i & 15)] = load_x(lc[i >> 4], i & 15);
-  Ahead cur, nxt;
-  fetch(0, cur);
-  int sl1[TL_PF], cl1[TL_PF];
-  fetch_list(1, sl1, cl1);
-  __syncthreads();
-  for (int tt = 0; tt < nt; ++tt) {
-    // new X rows of tile tt + 1 (list read one tile earlier) and the list of tile tt + 2
-    float4 pre[TL_PF];
+  Ahead a[2];                 // entries / Z / X rows of the tiles of each parity
+  float4 pre[2][TL_PF];       // new X rows of the tiles of each parity
+  int slt[2][TL_PF];          // and their slots (slot * 16 + part, -1: none)
+  int isl[TL_PF], icl[TL_PF], ival;  // load list of the tile two ahead of the current one
+  // a list's loads: slots (-1 where none), and the X loads themselves (a lane
+  // without a load reads X row 0)
+  auto issue = [&](const int (&sl)[TL_PF], const int (&cl)[TL_PF], int valid, int (&so)[TL_PF],
+                   float4 (&po)[TL_PF]) {
 #pragma unroll
-    for (int j = 0; j < TL_PF; ++j)
-      if (sl1[j] >= 0) pre[j] = load_x(cl1[j], sl1[j] & 15);
-    int sl2[TL_PF], cl2[TL_PF];
-    fetch_list(tt + 2, sl2, cl2);
-    if (tt + 1 < nt) fetch(tt + 1, nxt);
+    for (int j = 0; j < TL_PF; ++j) {
+      const bool v = (valid >> j) & 1;
+      const int i = tid + 256 * j;
+      so[j] = v ? sl[j] * 16 + (i & 15) : -1;
+      po[j] = load_x(v ? cl[j] : 0, i & 15);
+    }
+  };
+  fetch(0, a[0]);
+  {
+    int sl[TL_PF], cl[TL_PF], vl;
+    fetch_list(1, sl, cl, vl);
+    issue(sl, cl, vl, slt[1], pre[1]);
+  }
+  fetch_list(2, isl, icl, ival);
+  lds_barrier();
+  auto tile = [&](auto Pc, int tt) {
+    constexpr int P = decltype(Pc)::value;
+    // new X rows of tile tt + 2, the load list of tile tt + 3, and the
+    // entries / Z / X rows of tile tt + 1
+    issue(isl, icl, ival, slt[P], pre[P]);
+    fetch_list(tt + 3, isl, icl, ival);
+    if (tt + 1 < nt) fetch(tt + 1, a[1 - P]);
     // products of tile tt
+    const Ahead& cur = a[P];
     const int lim = s_maxe[tt];
     T acc[V];
 #pragma unroll
@@ -228,8 +290,9 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
       int ms;
       T mw;
       if (ch < 32) {
-        ms = ch == 0 ? cur.ms[0] : cur.ms[1];
-        mw = ch == 0 ? cur.mw[0] : cur.mw[1];
+        const bool v = ch + sub < cur.cnt;
+        ms = v ? (ch == 0 ? cur.ms[0] : cur.ms[1]) : zslot;
+        mw = v ? (ch == 0 ? cur.mw[0] : cur.mw[1]) : T(0);
       } else {  // rows longer than 32 entries: later chunks read here
         const int e0 = s_rp[tt * TL_ROWS + tr];
         const int cnt = s_rp[tt * TL_ROWS + tr + 1] - e0;
@@ -265,11 +328,19 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
     }
     const int row = s_rows[tt * TL_ROWS + tr];
     if (row >= 0 && c < k) {
+      T xr[V];  // this row's part of X for the beta term
+      if constexpr (SELF) {
+        const float4 f = xs[s_self[tt * TL_ROWS + tr] * 16 + sub];
+        __builtin_memcpy(xr, &f, 16);
+      } else {
+#pragma unroll
+        for (int q = 0; q < V; ++q) xr[q] = cur.x[q];
+      }
       T out[V];
 #pragma unroll
       for (int q = 0; q < V; ++q) {
         out[q] = alpha * acc[q];
-        if (fx) out[q] += beta * cur.x[q];
+        if (fx) out[q] += beta * xr[q];
         if (fz) out[q] += gamma * cur.z[q];
       }
       T* y = elem<O32>(Y, row, ldy, c);
@@ -283,22 +354,21 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
           if (c + q < k) y[q] = out[q];
       }
     }
-    __syncthreads();  // every thread is done with tile tt's slots
+    lds_barrier();  // every thread is done with tile tt's slots
+    // tile tt + 1's new X rows (issued one tile earlier) into their slots
 #pragma unroll
     for (int j = 0; j < TL_PF; ++j)
-      if (sl1[j] >= 0) xs[sl1[j]] = pre[j];
+      if (slt[1 - P][j] >= 0) xs[slt[1 - P][j]] = pre[1 - P][j];
     if (tt + 1 < nt) {  // rows past the TL_PF registers per thread
       const int l0 = s_lp[tt + 1], nl = s_lp[tt + 2] - l0;
       for (int i = tid + 256 * TL_PF; i < nl * 16; i += 256)
         xs[ls[l0 + (i >> 4)] * 16 + (i & 15)] = load_x(lc[l0 + (i >> 4)], i & 15);
     }
-    __syncthreads();
-    cur = nxt;
-#pragma unroll
-    for (int j = 0; j < TL_PF; ++j) {
-      sl1[j] = sl2[j];
-      cl1[j] = cl2[j];
-    }
+    lds_barrier();
+  };
+  for (int tt = 0; tt < nt; tt += 2) {
+    tile(std::integral_constant<int, 0>{}, tt);
+    if (tt + 1 < nt) tile(std::integral_constant<int, 1>{}, tt + 1);
   }
 }
 
@@ -315,9 +385,15 @@ static int launch_tile(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* 
   const double lim = 4294967295.0 / sizeof(T);
   const bool o32 = static_cast<double>(A->n) * ldx < lim && static_cast<double>(A->n) * ldy < lim &&
                    (!Z || static_cast<double>(A->n) * ldz < lim);
-  auto kern = o32 ? k_spmm_tile<T, true> : k_spmm_tile<T, false>;
+  const bool full = k % CW == 0;
+  auto pick = [&](auto self) {
+    constexpr bool S = decltype(self)::value;
+    return o32 ? (full ? k_spmm_tile<T, true, true, S> : k_spmm_tile<T, true, false, S>)
+               : (full ? k_spmm_tile<T, false, true, S> : k_spmm_tile<T, false, false, S>);
+  };
+  auto kern = A->tl_self ? pick(std::true_type{}) : pick(std::false_type{});
   hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nb8)), dim3(256), lds, ctx->stream, A->tl_pencil, A->tl_rows,
-                     A->tl_rptr, A->tl_maxe, A->tl_eslot, ev, A->tl_lptr, A->tl_lslot, A->tl_lcol, A->tl_slots, k,
+                     A->tl_sslot, A->tl_rptr, A->tl_maxe, A->tl_eslot, ev, A->tl_lptr, A->tl_lslot, A->tl_lcol, A->tl_slots, k,
                      nslices, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, nwork);
   FE_CHECK_LAUNCH();
   return 0;
@@ -349,7 +425,9 @@ struct TilePlan {
   std::vector<double> ev;
   std::vector<int64_t> lptr{0};
   std::vector<int> lslot, lcol;
+  std::vector<int> sslot;  // per tile row: slot of its own X row in its tile (-1: none)
   int slots = 0;
+  bool self = true;        // every row's own X row is in its tile
 };
 
 // Tiles of the matrix in processing order, grouped in pencils; -1 pads a tile.
@@ -437,6 +515,8 @@ bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, c
       int me = 0;
       for (int r : t) {
         P.rows.push_back(r);
+        P.sslot.push_back(r >= 0 ? slot_of[r] : -1);
+        if (r >= 0 && slot_of[r] < 0) P.self = false;
         if (r >= 0) {
           for (int q = rp[r]; q < rp[r + 1]; ++q) {
             P.eslot.push_back(slot_of[ci[q]]);
@@ -481,6 +561,13 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   if (loads_per_row) *loads_per_row = lpr;
   // worth it only when the loads per row fall below half the nonzeros per row
   if (lpr > 0.5 * static_cast<double>(A->nnz) / n) return 1;
+  // one more entry and one more load after the last pencil's: a kernel lane
+  // with nothing to read reads its pencil's first one, which then exists
+  // even for a pencil without entries or loads
+  P.eslot.push_back(0);
+  P.ev.push_back(0.0);
+  P.lslot.push_back(0);
+  P.lcol.push_back(0);
   std::vector<float> ev32(P.ev.size());
   for (size_t i = 0; i < P.ev.size(); ++i) ev32[i] = static_cast<float>(P.ev[i]);
   auto up = [&](auto** dst, const auto& src) -> int {
@@ -495,11 +582,12 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   if ((rc = up(&A->tl_pencil, P.pencil)) || (rc = up(&A->tl_rows, P.rows)) || (rc = up(&A->tl_rptr, P.rptr)) ||
       (rc = up(&A->tl_maxe, P.maxe)) || (rc = up(&A->tl_eslot, P.eslot)) || (rc = up(&A->tl_ev64, P.ev)) ||
       (rc = up(&A->tl_ev32, ev32)) || (rc = up(&A->tl_lptr, P.lptr)) || (rc = up(&A->tl_lslot, P.lslot)) ||
-      (rc = up(&A->tl_lcol, P.lcol)) || (rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : -3)) {
+      (rc = up(&A->tl_lcol, P.lcol)) || (rc = up(&A->tl_sslot, P.sslot)) ||
+      (rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : -3)) {
     hipStreamSynchronize(ctx->stream);
     void** arrays[] = {(void**)&A->tl_pencil, (void**)&A->tl_rows,  (void**)&A->tl_rptr, (void**)&A->tl_maxe,
                        (void**)&A->tl_eslot,  (void**)&A->tl_ev64,  (void**)&A->tl_ev32, (void**)&A->tl_lptr,
-                       (void**)&A->tl_lslot,  (void**)&A->tl_lcol};
+                       (void**)&A->tl_lslot,  (void**)&A->tl_lcol,  (void**)&A->tl_sslot};
     for (void** a : arrays) {
       if (*a) hipFree(*a);
       *a = nullptr;
@@ -508,7 +596,8 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   }
   A->tl_ntiles = static_cast<int64_t>(tiles.size());
   A->tl_npencils = static_cast<int64_t>(P.pencil.size()) - 1;
-  A->tl_loads = static_cast<int64_t>(P.lslot.size());
+  A->tl_loads = static_cast<int64_t>(P.lslot.size()) - 1;  // without the appended stand-in
   A->tl_slots = P.slots;
+  A->tl_self = P.self ? 1 : 0;
   return 0;
 }
