@@ -81,6 +81,7 @@ def lib():
         L.fe_gram.argtypes = [vp, i32, i32, i64, i32, i32, vp, i64, vp, i64, vp, vp, i64]
         L.fe_gram_static.argtypes = [vp, i32, i32, i64, i32, i32, vp, i64, vp, i64, vp, vp, i64]
         L.fe_gemm_nn.argtypes = [vp, i32, i64, i32, i32, vp, i64, vp, i64, vp, i64, dbl, dbl]
+        L.fe_trmm_upper.argtypes = [vp, i32, i64, i32, vp, i64, vp, i64, vp, i64]
         L.fe_potrf.argtypes = [vp, i32, vp, i64]
         L.fe_chol_rinv.argtypes = [vp, i32, vp, i64, vp, i64]
         L.fe_syev.argtypes = [vp, i32, vp, i64, vp, vp, i64]

@@ -3,8 +3,9 @@
 //
 //   potrf  for each block column jb:
 //            k_chol_diag    factor the diagonal block in LDS (unblocked,
-//                           right-looking), invert it in place (the column
-//                           algorithm of LAPACK dtrti2), keep the inverse
+//                           right-looking, one wavefront, one barrier per
+//                           column), invert it (a column per thread by
+//                           forward substitution), keep the inverse
 //            k_chol_panel   L_ib = A_ib L_jj^{-T} for the blocks below
 //            k_chol_update  A_ij -= L_ib L_jb^T for the trailing lower blocks
 //   rinv   (L^T)^{-1} = (L^{-1})^T:
@@ -21,67 +22,63 @@ namespace fe {
 
 constexpr int NB = 32;
 
-// Invert the lower-triangular nb x nb block D (LDS, D[r][c]) in place.
-__device__ void invert_lower_lds(double (*D)[NB + 1], int nb, double* tmp) {
-  const int tid = threadIdx.x;
-  for (int j = nb - 1; j >= 0; --j) {
-    if (tid == 0) D[j][j] = 1.0 / D[j][j];
-    __syncthreads();
-    for (int i = j + 1 + tid; i < nb; i += blockDim.x) {
-      double s = 0.0;
-      for (int p = j + 1; p <= i; ++p) s += D[i][p] * D[p][j];
-      tmp[i] = s;
+// Inverse of the lower-triangular block D (LDS, D[r][c], unit diagonal past
+// nb) into Di: thread c < NB solves L x = e_c by forward substitution and
+// writes column c of Di; D is only read and each thread reads back only its
+// own column, so no barrier is needed inside.
+__device__ __forceinline__ void invert_lower_cols(const double (*D)[NB + 1], double (*Di)[NB + 1]) {
+  const int c = threadIdx.x;
+  if (c >= NB) return;
+  for (int i = 0; i < NB; ++i) {
+    if (i < c) {
+      Di[i][c] = 0.0;
+      continue;
     }
-    __syncthreads();
-    const double ajj = -D[j][j];
-    for (int i = j + 1 + tid; i < nb; i += blockDim.x) D[i][j] = ajj * tmp[i];
-    __syncthreads();
+    double s = i == c ? 1.0 : 0.0;
+    for (int p = c; p < i; ++p) s -= D[i][p] * Di[p][c];
+    Di[i][c] = s / D[i][i];
   }
 }
 
-__global__ __launch_bounds__(256) void k_chol_diag(int k, double* G, int64_t ldg, int j0, double* dinv, int* info) {
+// Factor the diagonal block j0 in LDS (right-looking, one barrier per column:
+// the trailing update reads the unscaled column j, whose scaled copy goes to
+// Lc), write it back, and keep its inverse in dinv. One wavefront.
+__global__ __launch_bounds__(64) void k_chol_diag(int k, double* G, int64_t ldg, int j0, double* dinv, int* info) {
   __shared__ double D[NB][NB + 1];
-  __shared__ double tmp[NB];
-  __shared__ int fail;
+  __shared__ double Lc[NB][NB + 1];
+  __shared__ double Di[NB][NB + 1];
   const int tid = threadIdx.x;
   if (info[0] != 0) return;  // an earlier block failed: leave G as is
   const int nb = min(NB, k - j0);
-  if (tid == 0) fail = 0;
   for (int t = tid; t < NB * NB; t += blockDim.x) {
     const int r = t % NB, c = t / NB;
     D[r][c] = (r < nb && c < nb) ? G[(int64_t)(j0 + c) * ldg + j0 + r] : (r == c ? 1.0 : 0.0);
+    Lc[r][c] = r == c ? 1.0 : 0.0;
   }
   __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    if (tid == 0) {
-      const double d = D[j][j];
-      if (!(d > 0.0) || !isfinite(d)) {
-        fail = 1;
-        info[0] = j0 + j + 1;
-      } else {
-        D[j][j] = sqrt(d);
-      }
+    const double d = D[j][j];
+    if (!(d > 0.0) || !isfinite(d)) {  // the same value in every thread
+      if (tid == 0) info[0] = j0 + j + 1;
+      return;
     }
-    __syncthreads();
-    if (fail) return;
-    const double inv = 1.0 / D[j][j];
-    for (int i = j + 1 + tid; i < nb; i += blockDim.x) D[i][j] *= inv;
-    __syncthreads();
+    const double sd = sqrt(d), inv = 1.0 / d;
+    for (int r = j + tid; r < NB; r += blockDim.x) Lc[r][j] = r == j ? sd : D[r][j] / sd;
     for (int t = tid; t < NB * NB; t += blockDim.x) {
       const int r = t % NB, c = t / NB;
-      if (c > j && r >= c && r < nb) D[r][c] -= D[r][j] * D[c][j];
+      if (c > j && r >= c && r < nb) D[r][c] -= D[r][j] * D[c][j] * inv;
     }
     __syncthreads();
   }
   for (int t = tid; t < NB * NB; t += blockDim.x) {
     const int r = t % NB, c = t / NB;
-    if (r < nb && c < nb && r >= c) G[(int64_t)(j0 + c) * ldg + j0 + r] = D[r][c];
+    if (r < nb && c < nb && r >= c) G[(int64_t)(j0 + c) * ldg + j0 + r] = Lc[r][c];
   }
+  invert_lower_cols(Lc, Di);
   __syncthreads();
-  invert_lower_lds(D, nb, tmp);
   for (int t = tid; t < NB * NB; t += blockDim.x) {
     const int r = t % NB, c = t / NB;
-    dinv[c * NB + r] = (r >= c) ? D[r][c] : 0.0;
+    dinv[c * NB + r] = (r >= c) ? Di[r][c] : 0.0;
   }
 }
 
@@ -141,10 +138,10 @@ __global__ __launch_bounds__(256) void k_chol_update(int k, double* G, int64_t l
 
 // Invert every diagonal block of L; write its transpose into Rinv's diagonal
 // block (Rinv = L^{-T} is upper triangular) and keep the inverse in dinv.
-__global__ __launch_bounds__(256) void k_trtri_diag(int k, const double* __restrict__ L, int64_t ldl, double* dinv,
-                                                    double* Rinv, int64_t ldr) {
+__global__ __launch_bounds__(64) void k_trtri_diag(int k, const double* __restrict__ L, int64_t ldl, double* dinv,
+                                                   double* Rinv, int64_t ldr) {
   __shared__ double D[NB][NB + 1];
-  __shared__ double tmp[NB];
+  __shared__ double Di[NB][NB + 1];
   const int tid = threadIdx.x;
   const int j0 = blockIdx.x * NB;
   const int nb = min(NB, k - j0);
@@ -153,11 +150,12 @@ __global__ __launch_bounds__(256) void k_trtri_diag(int k, const double* __restr
     D[r][c] = (r < nb && c < nb) ? (r >= c ? L[(int64_t)(j0 + c) * ldl + j0 + r] : 0.0) : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
-  invert_lower_lds(D, nb, tmp);
+  invert_lower_cols(D, Di);
+  __syncthreads();
   double* dv = dinv + (int64_t)blockIdx.x * NB * NB;
   for (int t = tid; t < NB * NB; t += blockDim.x) {
     const int r = t % NB, c = t / NB;
-    const double v = (r >= c) ? D[r][c] : 0.0;
+    const double v = (r >= c) ? Di[r][c] : 0.0;
     dv[c * NB + r] = v;
     // Rinv(j0 + c, j0 + r) = Linv(j0 + r, j0 + c)
     if (r < nb && c < nb) Rinv[(int64_t)(j0 + r) * ldr + j0 + c] = v;
@@ -220,7 +218,7 @@ extern "C" int fe_potrf(fe_ctx ctx, int k, double* G, int64_t ldg) {
   if (rc) return rc;
   for (int jb = 0; jb < nblk; ++jb) {
     const int j0 = jb * NB;
-    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, ctx->stream, k, G, ldg, j0, (double*)ws, ctx->info_dev);
+    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, ctx->stream, k, G, ldg, j0, (double*)ws, ctx->info_dev);
     const int below = nblk - jb - 1;
     if (below > 0) {
       hipLaunchKernelGGL(k_chol_panel, dim3(below), dim3(256), 0, ctx->stream, k, G, ldg, j0, (const double*)ws,
@@ -245,7 +243,7 @@ extern "C" int fe_chol_rinv(fe_ctx ctx, int k, const double* L, int64_t ldl, dou
   if (rc) return rc;
   rc = fe_zero2d(ctx, Rinv, k, k, ldr, 8);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_trtri_diag, dim3(nblk), dim3(256), 0, ctx->stream, k, L, ldl, (double*)ws, Rinv, ldr);
+  hipLaunchKernelGGL(k_trtri_diag, dim3(nblk), dim3(64), 0, ctx->stream, k, L, ldl, (double*)ws, Rinv, ldr);
   for (int d = 1; d < nblk; ++d)
     hipLaunchKernelGGL(k_trtri_off, dim3(nblk - d), dim3(256), 0, ctx->stream, k, L, ldl, (const double*)ws, Rinv,
                        ldr, d);

@@ -422,11 +422,13 @@ __global__ __launch_bounds__(256) void k_gram_reduce(int m, int p, int tiles_b, 
 // registers during the MFMA work.
 constexpr int MKS = 16;
 
+// upper != 0: C is upper triangular (m == p; nothing below its diagonal is
+// read) and a column tile's inner loop stops at its last column.
 template <typename T, int TR, int TC>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
                                                  const double* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
                                                  T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles,
-                                                 int vec, int cvec) {
+                                                 int vec, int cvec, int upper) {
   static_assert((TR / 64) * (TC / 64) == 4, "four 64 x 64 wavefront tiles");
   constexpr int MK = MKS;  // k values per stage
   constexpr int TPR = 256 / TR, XK = MK / TPR;  // X staging: threads per row, k values per thread
@@ -469,7 +471,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
     // with 16-byte aligned coefficient columns (cvec), whole runs are read
     // as 16-byte vectors: with 8-byte reads the FP32 rotation ran at 58% of
     // the MFMA peak and the FP64 one at 75%
-    if (cvec && ccol < p && k0 + sck + CK <= m) {
+    if (cvec && ccol < p && k0 + sck + CK <= m && (!upper || k0 + sck + CK - 1 <= ccol)) {
 #pragma unroll
       for (int e = 0; e < CK; e += 2) {
         const double2 d = *reinterpret_cast<const double2*>(C + (int64_t)ccol * ldc + k0 + sck + e);
@@ -481,18 +483,19 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
 #pragma unroll
     for (int e = 0; e < CK; ++e) {
       const int k = k0 + sck + e;
-      cr[e] = (ccol < p && k < m) ? static_cast<T>(C[(int64_t)ccol * ldc + k]) : T(0);
+      cr[e] = (ccol < p && k < m && (!upper || k <= ccol)) ? static_cast<T>(C[(int64_t)ccol * ldc + k]) : T(0);
     }
   };
+  const int kend = upper ? min(m, c0 + TC) : m;  // rows of C this column tile reads
   fetch(0);
-  for (int k0 = 0; k0 < m; k0 += MK) {
+  for (int k0 = 0; k0 < kend; k0 += MK) {
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < XK; ++e) Xs[sr][sk + e] = xr[e];
 #pragma unroll
     for (int e = 0; e < CK; ++e) Cs[sck + e][scc] = cr[e];
     __syncthreads();
-    if (k0 + MK < m) fetch(k0 + MK);
+    if (k0 + MK < kend) fetch(k0 + MK);
 #pragma unroll
     for (int kk = 0; kk < MK; kk += 4) {
       const int kr = kk + (lane >> 4);
@@ -667,16 +670,10 @@ extern "C" int fe_gram_static(fe_ctx ctx, int mode, int prec, int64_t n, int m, 
                       FE_GRAM_STATIC_BYTES, "fe_gram_static");
 }
 
-extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
-                          const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta) {
-  FE_ARG(ctx && X && C && Y, "fe_gemm_nn: null argument");
-  FE_ARG(X != Y, "fe_gemm_nn: Y must not alias X");
-  FE_ARG(n >= 0 && m >= 0 && p >= 0 && ldx >= m && ldy >= p && ldc >= m, "fe_gemm_nn: bad dimensions");
-  if (n == 0 || p == 0) return 0;
-  if (m == 0) {
-    if (beta == 0.0) return fe_zero2d(ctx, Y, n, p, ldy, (int)prec_bytes(prec));
-    return fe_axpby_cols(ctx, prec, n, p, Y, ldy, Y, ldy, 0.0, beta, nullptr);
-  }
+namespace fe {
+
+static int gemm_run(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx, const double* C,
+                    int64_t ldc, void* Y, int64_t ldy, double alpha, double beta, int upper) {
   // 256 x 64 tiles when the output has at most 64 columns, else 128 x 128
   const bool thin = p <= 64;
   const int TR = thin ? 256 : WT, TC = thin ? 64 : WT;
@@ -692,18 +689,42 @@ extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const v
   if (prec == FE_F64) {
     if (thin)
       hipLaunchKernelGGL((k_gemm_nn<double, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx, C,
-                         ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec);
+                         ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper);
     else
       hipLaunchKernelGGL((k_gemm_nn<double, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx,
-                         C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec);
+                         C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper);
   } else {
     if (thin)
       hipLaunchKernelGGL((k_gemm_nn<float, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
-                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec);
+                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper);
     else
       hipLaunchKernelGGL((k_gemm_nn<float, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
-                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec);
+                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper);
   }
   FE_CHECK_LAUNCH();
   return 0;
+}
+
+}  // namespace fe
+
+extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                          const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta) {
+  FE_ARG(ctx && X && C && Y, "fe_gemm_nn: null argument");
+  FE_ARG(X != Y, "fe_gemm_nn: Y must not alias X");
+  FE_ARG(n >= 0 && m >= 0 && p >= 0 && ldx >= m && ldy >= p && ldc >= m, "fe_gemm_nn: bad dimensions");
+  if (n == 0 || p == 0) return 0;
+  if (m == 0) {
+    if (beta == 0.0) return fe_zero2d(ctx, Y, n, p, ldy, (int)prec_bytes(prec));
+    return fe_axpby_cols(ctx, prec, n, p, Y, ldy, Y, ldy, 0.0, beta, nullptr);
+  }
+  return fe::gemm_run(ctx, prec, n, m, p, X, ldx, C, ldc, Y, ldy, alpha, beta, 0);
+}
+
+extern "C" int fe_trmm_upper(fe_ctx ctx, int prec, int64_t n, int m, const void* X, int64_t ldx, const double* C,
+                             int64_t ldc, void* Y, int64_t ldy) {
+  FE_ARG(ctx && X && C && Y, "fe_trmm_upper: null argument");
+  FE_ARG(X != Y, "fe_trmm_upper: Y must not alias X");
+  FE_ARG(n >= 0 && m >= 0 && ldx >= m && ldy >= m && ldc >= m, "fe_trmm_upper: bad dimensions");
+  if (n == 0 || m == 0) return 0;
+  return fe::gemm_run(ctx, prec, n, m, m, X, ldx, C, ldc, Y, ldy, 1.0, 0.0, 1);
 }

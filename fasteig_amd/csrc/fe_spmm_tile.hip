@@ -131,7 +131,7 @@ __device__ __forceinline__ void lds_barrier() {
 // diagonal), so the beta X term reads it from LDS: no second read of X from
 // HBM and no registers holding it.
 template <typename T, bool O32, bool FULL, bool SELF>
-__global__ __launch_bounds__(256) void k_spmm_tile(
+__global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void k_spmm_tile(
     const int64_t* __restrict__ pencil, const int* __restrict__ rows, const int* __restrict__ sslot,
     const int64_t* __restrict__ rptr,
     const int* __restrict__ maxe, const int* __restrict__ eslot, const T* __restrict__ ev,

@@ -42,13 +42,13 @@ int cholesky_qr(Workspace& ws, Block& X, Block& tmp) {
   if (info != 0) return -1;
   check(fe_chol_rinv(ex.ctx(), k, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
   if (tmp.ncols() == k && tmp.rows() == X.rows() && tmp.prec() == X.prec()) {
-    gemm(ex, X.ref(), ws.R.data(), ws.cap, tmp.ref(), 1.0, 0.0);
+    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, tmp.ref());
     X.swap(tmp);
   } else {
     const BlockRef t = tmp.cols(0, k);
     if (tmp.ncols() < k || tmp.rows() != X.rows() || tmp.prec() != X.prec())
       throw std::invalid_argument("cholesky_qr: scratch block too small");
-    gemm(ex, X.ref(), ws.R.data(), ws.cap, t, 1.0, 0.0);
+    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, t);
     copy(ex, t, X.ref());
   }
   return 0;
@@ -76,7 +76,7 @@ int orthonormal_qr(Workspace& ws, Block& X, Block& tmp) {
     if (info != 0) throw std::runtime_error("orthonormal_qr: shifted Cholesky QR failed (non-finite block?)");
     check(fe_chol_rinv(ex.ctx(), k, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
     const BlockRef t = tmp.cols(0, k);
-    gemm(ex, X.ref(), ws.R.data(), ws.cap, t, 1.0, 0.0);
+    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, t);
     copy(ex, t, X.ref());
     if (cholesky_qr(ws, X, tmp) != 0) throw std::runtime_error("orthonormal_qr: Cholesky QR failed after shift");
   }

@@ -198,6 +198,11 @@ void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, cons
         "fe_gemm_nn");
 }
 
+void trmm_upper(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y) {
+  if (X.prec != Y.prec || X.rows != Y.rows || X.cols != Y.cols) throw std::invalid_argument("trmm_upper: operand mismatch");
+  check(fe_trmm_upper(ex.ctx(), X.prec, X.rows, X.cols, X.ptr, X.ld, C, ldc, Y.ptr, Y.ld), "fe_trmm_upper");
+}
+
 void copy(HipExecutor& ex, const BlockRef& src, const BlockRef& dst) {
   if (src.rows != dst.rows || src.cols != dst.cols) throw std::invalid_argument("copy: shape mismatch");
   check(fe_copy2d(ex.ctx(), src.rows, src.cols, src.ptr, src.prec, src.ld, dst.ptr, dst.prec, dst.ld), "fe_copy2d");

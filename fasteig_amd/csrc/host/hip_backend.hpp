@@ -165,6 +165,9 @@ void spmm(HipExecutor& ex, const HipCsrMatrix& A, const BlockRef& X, const Block
 // G (m x p, ldg) = X^T Y
 void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, double* G, int64_t ldg,
           const double* w = nullptr);
+// Y = X C with C upper triangular (m x m, FP64 column-major; nothing below
+// its diagonal is read): the X R^{-1} of Cholesky QR
+void trmm_upper(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y);
 // Y = alpha X C + beta Y, C FP64 column-major (m x p)
 void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, double alpha,
           double beta);

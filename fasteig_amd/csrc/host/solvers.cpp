@@ -136,7 +136,13 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // 1. filter the active columns (ChebyshevSolver.cpp:151-155, 223-258)
     if (active > 0 && upper != lower) {
       BlockRef prev = F.cols(0, active), cur = P.cols(0, active);
-      hip::copy(ex, V.cols(locked, active), prev);
+      // the filter reads its start block in place when the active columns
+      // start on a 16-byte boundary (the LDS-tiled product needs aligned
+      // rows); otherwise from an aligned copy
+      const BlockRef start = V.cols(locked, active);
+      const bool in_place = (locked * static_cast<int64_t>(V.ref().esize())) % 16 == 0;
+      if (!in_place) hip::copy(ex, start, prev);
+      const BlockRef y0 = in_place ? start : prev;
       if (ritz_bounds && std::isfinite(cutoff) && cutoff > lower && cutoff < upper) {
         // scaled filter on [cutoff, upper] normalised at the lower bound
         // (Zhou & Saad 2007, Alg. 4.1): sigma_1 = e/(a0 - c),
@@ -144,10 +150,10 @@ FASTEIG_INT ChebyshevSolver::solve() {
         const double e = 0.5 * (upper - cutoff), c = 0.5 * (upper + cutoff);
         double sigma = e / (lower - c);
         const double s1 = sigma;
-        hip::spmm(ex, A, prev, cur, sign * s1 / e, -c * s1 / e);
+        hip::spmm(ex, A, y0, cur, sign * s1 / e, -c * s1 / e);
         for (int k = 2; k <= cycle_degree; ++k) {
           const double sn = 1.0 / (2.0 / s1 - sigma);
-          hip::spmm(ex, A, cur, prev, sign * 2.0 * sn / e, -c * 2.0 * sn / e, &prev, -sigma * sn);
+          hip::spmm(ex, A, cur, prev, sign * 2.0 * sn / e, -c * 2.0 * sn / e, k == 2 ? &y0 : &prev, -sigma * sn);
           std::swap(prev, cur);
           sigma = sn;
         }
@@ -157,9 +163,9 @@ FASTEIG_INT ChebyshevSolver::solve() {
         const double c = 2.0 / (upper - cut);
         const double d = -(upper + cut) / (upper - cut);
         if (!std::isfinite(c) || !std::isfinite(d)) throw std::runtime_error("ChebyshevSolver: invalid filter scaling.");
-        hip::spmm(ex, A, prev, cur, sign * c, d);                       // Y1 = (cA + dI) Y0
+        hip::spmm(ex, A, y0, cur, sign * c, d);                         // Y1 = (cA + dI) Y0
         for (int k = 2; k <= degree; ++k) {                            // Y_k = 2(cA+dI)Y_{k-1} - Y_{k-2}
-          hip::spmm(ex, A, cur, prev, 2.0 * sign * c, 2.0 * d, &prev, -1.0);
+          hip::spmm(ex, A, cur, prev, 2.0 * sign * c, 2.0 * d, k == 2 ? &y0 : &prev, -1.0);
           std::swap(prev, cur);
         }
       }
@@ -176,7 +182,11 @@ FASTEIG_INT ChebyshevSolver::solve() {
                                  inv_norms.data()),
                    "scale");
       }
-      hip::copy(ex, cur, V.cols(locked, active));
+      if (locked == 0 && active == V.ncols() && (cur.ptr == F.data() || cur.ptr == P.data())) {
+        V.swap(cur.ptr == F.data() ? F : P);  // the filtered block becomes V, the old V scratch
+      } else {
+        hip::copy(ex, cur, V.cols(locked, active));
+      }
     }
     // CholQR of V[:, :subspace_cols] into Q (ChebyshevSolver.cpp:167-171); the
     // FP32 phase runs it twice (CholQR2) for orthogonality at FP32 accuracy
@@ -186,8 +196,10 @@ FASTEIG_INT ChebyshevSolver::solve() {
     int passes = (prec == FE_F32 || cycle_degree > degree) ? 2 : 1;
     for (int pass = 0; pass < passes; ++pass) {
       ws.ensure(subspace_cols);
-      const BlockRef Vb = pass == 0 ? V.cols(0, subspace_cols) : AY.cols(0, subspace_cols);
-      if (pass > 0) hip::copy(ex, Q.cols(0, subspace_cols), Vb);
+      // pass 0 takes V into Q; a later pass takes Q into AY (scratch here)
+      // and swaps the two
+      const BlockRef Vb = pass == 0 ? V.cols(0, subspace_cols) : Q.cols(0, subspace_cols);
+      const BlockRef Qout = pass == 0 ? Q.cols(0, subspace_cols) : AY.cols(0, subspace_cols);
       hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap);
       int info = fe_potrf(ex.ctx(), subspace_cols, ws.G.data(), ws.cap);
       hip::check(info, "fe_potrf");
@@ -212,7 +224,8 @@ FASTEIG_INT ChebyshevSolver::solve() {
                                  (prec == FE_F32 ? "fp32" : "fp64") + ", degree " + std::to_string(cycle_degree) +
                                  ", column " + std::to_string(info) + ").");
       hip::check(fe_chol_rinv(ex.ctx(), subspace_cols, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
-      hip::gemm(ex, Vb, ws.R.data(), ws.cap, Q.cols(0, subspace_cols), 1.0, 0.0);
+      hip::trmm_upper(ex, Vb, ws.R.data(), ws.cap, Qout);
+      if (pass > 0) Q.swap(AY);
     }
     // 3. Rayleigh-Ritz (ChebyshevSolver.cpp:175-183); Ritz vectors go straight into V
     hip::spmm(ex, A, Q.cols(0, subspace_cols), AQ.cols(0, subspace_cols), sign);
@@ -609,12 +622,12 @@ FASTEIG_INT BlockKrylovBase::run(bool restarted) {
       ws.G.from_host(G);
       bool ok = rinv_into_r(&cn0);
       if (ok) {
-        hip::gemm(ex, src, ws.R.data(), ws.cap, Sw.ref(), 1.0, 0.0);
+        hip::trmm_upper(ex, src, ws.R.data(), ws.cap, Sw.ref());
         hip::gram(ex, FE_GRAM_NATIVE, Sw.ref(), Sw.ref(), ws.G.data(), ws.cap);
         ok = rinv_into_r(nullptr);
       }
       if (ok) {
-        hip::gemm(ex, Sw.ref(), ws.R.data(), ws.cap, Qn.ref(), 1.0, 0.0);
+        hip::trmm_upper(ex, Sw.ref(), ws.R.data(), ws.cap, Qn.ref());
         hip::gram(ex, FE_GRAM_NATIVE, Qn.ref(), Qn.ref(), ws.C.data(), ws.cap);
         MatrixXd Gq = host_small(ws.C, b, b);
         for (int i = 0; i < b; ++i) Gq(i, i) -= 1.0;

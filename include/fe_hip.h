@@ -176,6 +176,15 @@ int fe_gram_static(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, cons
 int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
                const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta);
 
+/* Y[:, 0:m] = X[:, 0:m] * C with C upper triangular m x m (FP64 column-major,
+ * ldc; nothing below its diagonal is read): the X R^{-1} of Cholesky QR, whose
+ * triangular solve (src/core/Orthogonalization.cpp:49-57,185-246) is done as a
+ * product with the inverse from fe_chol_rinv. A column tile of the output
+ * reads only the rows of C down to its last column (5/8 of the products of
+ * fe_gemm_nn at m = 512). Y must not alias X. */
+int fe_trmm_upper(fe_ctx ctx, int prec, int64_t n, int m, const void* X, int64_t ldx, const double* C,
+                  int64_t ldc, void* Y, int64_t ldy);
+
 /* ---- small dense FP64 on the device -----------------------------------------
  * Cholesky G = L L^T of the leading k x k block (lower triangle written, upper
  * left untouched); returns 0 or j+1 for the first non-positive pivot (the

@@ -456,6 +456,29 @@ def test_gram_reproducible(dev):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("n,m", [(1000, 16), (5000, 48), (3333, 200), (20000, 512), (7, 3), (200001, 64),
+                                 (999, 130), (300, 1)])
+@pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
+def test_trmm_upper(dev, n, m, prec):
+    """X C with C upper triangular: below-diagonal entries hold garbage that
+    must not be read; same result as the full product with triu(C)."""
+    rng = np.random.default_rng(n + 3 * m)
+    X = rng.uniform(-1, 1, (n, m))
+    Cm = rng.uniform(-1, 1, (m, m))
+    Cg = Cm.copy()
+    Cg[np.tril_indices(m, -1)] = np.nan  # never read
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    Xd, Yd = dev.put(X, dt), dev.buffer((n, m), dt)
+    Cd = dev.put(Cg, np.float64, order="F")
+    _fe.check(L().fe_trmm_upper(dev.ctx, prec, n, m, Xd.ptr, m, Cd.ptr, m, Yd.ptr, m))
+    Xr = X.astype(dt).astype(np.float64)
+    ref = Xr @ np.triu(Cm).astype(dt).astype(np.float64)
+    out = Yd.download().astype(np.float64)
+    assert np.all(np.isfinite(out))
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    assert rel(out, ref) <= tol * np.sqrt(m) + tol
+
+
 @pytest.mark.parametrize("n,m,p", [(1000, 16, 16), (5000, 48, 10), (20000, 96, 96), (3333, 200, 70),
                                    (100000, 256, 256), (7, 3, 2), (200001, 192, 48), (1001, 48, 64),
                                    (999, 20, 65), (300, 33, 1)])
