@@ -138,6 +138,9 @@ def dense_roofline(n, m=512, reps=10):
                      2.0 * n * m * m),
             "gemm": (lambda: L.fe_gemm_nn(dev.ctx, prec, n, m, m, X.ptr, m, Cm.ptr, m, Yb.ptr, m, 1.0, 0.0),
                      2.0 * n * m * m),
+            # the CholQR product X R^-1 (upper triangular): n m^2 useful flops
+            "trmm": (lambda: L.fe_trmm_upper(dev.ctx, prec, n, m, X.ptr, m, Cm.ptr, m, Yb.ptr, m),
+                     float(n) * m * m),
         }
         for name, (fn, flops) in cases.items():
             for _ in range(2):

@@ -2,10 +2,9 @@
 // matrix on the device (column-major), block size NB = 32:
 //
 //   potrf  for each block column jb:
-//            k_chol_diag    factor the diagonal block in LDS (unblocked,
-//                           right-looking, one wavefront, one barrier per
-//                           column), invert it (a column per thread by
-//                           forward substitution), keep the inverse
+//            k_chol_diag    factor the diagonal block and invert it in one
+//                           right-looking pass over the registers of one
+//                           wavefront, keep the inverse
 //            k_chol_panel   L_ib = A_ib L_jj^{-T} for the blocks below
 //            k_chol_update  A_ij -= L_ib L_jb^T for the trailing lower blocks
 //   rinv   (L^T)^{-1} = (L^{-1})^T:
@@ -22,64 +21,170 @@ namespace fe {
 
 constexpr int NB = 32;
 
-// Inverse of the lower-triangular block D (LDS, D[r][c], unit diagonal past
-// nb) into Di: thread c < NB solves L x = e_c by forward substitution and
-// writes column c of Di; D is only read and each thread reads back only its
-// own column, so no barrier is needed inside.
-__device__ __forceinline__ void invert_lower_cols(const double (*D)[NB + 1], double (*Di)[NB + 1]) {
-  const int c = threadIdx.x;
-  if (c >= NB) return;
-  for (int i = 0; i < NB; ++i) {
-    if (i < c) {
-      Di[i][c] = 0.0;
-      continue;
-    }
-    double s = i == c ? 1.0 : 0.0;
-    for (int p = c; p < i; ++p) s -= D[i][p] * Di[p][c];
-    Di[i][c] = s / D[i][i];
-  }
+// The 32 x 32 diagonal blocks live in registers of one wavefront: lane l
+// holds the 4 x 4 sub-block (bi, bj) = (l >> 3, l & 7), rows 4 bi .. 4 bi + 3
+// and columns 4 bj .. 4 bj + 3. Step j publishes column j of L and row j of
+// L^{-1} (32 values each) through LDS, with zeros where the update must not
+// apply, so every lane runs the same unmasked FMAs; the diagonal entry comes
+// from its owner lane by readlane. Entries above the diagonal are never read
+// back or stored, so what the unmasked updates leave there does not matter,
+// except in L^{-1}, whose upper part stays exactly zero (every row j it is
+// updated with has zeros right of column j). The LDS version of this kernel,
+// LDS round trips per entry and a correctly rounded divide per step, took
+// 60-84 us per block.
+struct Sub {
+  int bi, bj;
+  __device__ Sub() : bi(threadIdx.x >> 3), bj(threadIdx.x & 7) {}
+};
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
 
-// Factor the diagonal block j0 in LDS (right-looking, one barrier per column:
-// the trailing update reads the unscaled column j, whose scaled copy goes to
-// Lc), write it back, and keep its inverse in dinv. One wavefront.
+// 1 / sqrt(d) for finite d > 0: the hardware estimate and two Newton steps
+// y <- y (1 + (1 - d y^2) / 2) (each squares the relative error; the result
+// is within a few units in the last place).
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-d * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+// 1 / d for finite nonzero d: the hardware estimate and two Newton steps
+// y <- y + y (1 - d y).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) y = fma(y, fma(-d, y, 1.0), y);
+  return y;
+}
+
+// Lower triangle of the nb x nb block at (j0, j0) of the column-major matrix
+// M; rows and columns past nb are those of the identity. Every lane issues
+// its 16 loads (invalid ones from the block's first entry) before using any.
+__device__ __forceinline__ void load_lower(const Sub& s, const double* M, int64_t ldm, int j0, int nb,
+                                           double (&a)[4][4]) {
+  const double* base = M + (int64_t)j0 * ldm + j0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int R = 4 * s.bi + r, C = 4 * s.bj + c;
+      const bool ok = R >= C && R < nb && C < nb;
+      a[r][c] = base[ok ? (int64_t)C * ldm + R : 0];
+    }
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int R = 4 * s.bi + r, C = 4 * s.bj + c;
+      if (!(R >= C && R < nb && C < nb)) a[r][c] = R == C ? 1.0 : 0.0;
+    }
+}
+
+// FACTOR: Cholesky of the block in a (lower triangle in, L out) and x = L^{-1},
+// right-looking, in one pass. Otherwise a holds L already and only x = L^{-1}
+// is formed. Returns the first column whose pivot is not positive and finite,
+// or -1 (the same in every lane).
+template <bool FACTOR>
+__device__ int chol_inv_regs(const Sub& s, double (&a)[4][4], double (&x)[4][4], double* col, double* row) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[r][c] = (s.bi == s.bj && r == c) ? 1.0 : 0.0;
+  for (int jb = 0; jb < NB / 4; ++jb) {
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const int j = 4 * jb + jj;
+      const double d = readlane_f64(a[jj][jj], 9 * jb);  // lane 9 jb holds sub-block (jb, jb)
+      double inv;  // 1 / L[j][j]
+      if (FACTOR) {
+        if (!(d > 0.0) || !isfinite(d)) return j;
+        inv = rsqrt_nr(d);
+      } else {
+        inv = rcp_nr(d);
+      }
+      if (s.bj == jb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int R = 4 * s.bi + r;
+          if (FACTOR) {
+            if (R > j) a[r][jj] *= inv;
+            else if (R == j) a[r][jj] = d * inv;
+          }
+          col[R] = R > j ? a[r][jj] : 0.0;  // all 32 rows: no stale entries
+        }
+      }
+      if (s.bi == jb) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          x[jj][c] *= inv;  // row j of L^{-1} is final
+          row[4 * s.bj + c] = x[jj][c];
+        }
+      }
+      __syncthreads();
+      double lr[4], lc[4], xr[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        lr[e] = col[4 * s.bi + e];
+        xr[e] = row[4 * s.bj + e];
+        if (FACTOR) lc[e] = col[4 * s.bj + e];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (FACTOR) a[r][c] -= lr[r] * lc[c];
+          x[r][c] -= lr[r] * xr[c];
+        }
+      __syncthreads();
+    }
+  }
+  return -1;
+}
+
+// x as the column-major NB x NB block dv (zeros above the diagonal).
+__device__ __forceinline__ void store_block(const Sub& s, const double (&x)[4][4], double* dv) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int R = 4 * s.bi + r, C = 4 * s.bj + c;
+      dv[C * NB + R] = R >= C ? x[r][c] : 0.0;
+    }
+}
+
+// Factor the diagonal block j0, write L back into G's lower triangle, and
+// keep L^{-1} in dinv (column-major NB x NB). One wavefront.
 __global__ __launch_bounds__(64) void k_chol_diag(int k, double* G, int64_t ldg, int j0, double* dinv, int* info) {
-  __shared__ double D[NB][NB + 1];
-  __shared__ double Lc[NB][NB + 1];
-  __shared__ double Di[NB][NB + 1];
-  const int tid = threadIdx.x;
+  __shared__ double col[NB], row[NB];
   if (info[0] != 0) return;  // an earlier block failed: leave G as is
+  const Sub s;
   const int nb = min(NB, k - j0);
-  for (int t = tid; t < NB * NB; t += blockDim.x) {
-    const int r = t % NB, c = t / NB;
-    D[r][c] = (r < nb && c < nb) ? G[(int64_t)(j0 + c) * ldg + j0 + r] : (r == c ? 1.0 : 0.0);
-    Lc[r][c] = r == c ? 1.0 : 0.0;
+  double a[4][4], x[4][4];
+  load_lower(s, G, ldg, j0, nb, a);
+  const int bad = chol_inv_regs<true>(s, a, x, col, row);
+  if (bad >= 0) {
+    if (threadIdx.x == 0) info[0] = j0 + bad + 1;
+    return;
   }
-  __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    const double d = D[j][j];
-    if (!(d > 0.0) || !isfinite(d)) {  // the same value in every thread
-      if (tid == 0) info[0] = j0 + j + 1;
-      return;
-    }
-    const double sd = sqrt(d), inv = 1.0 / d;
-    for (int r = j + tid; r < NB; r += blockDim.x) Lc[r][j] = r == j ? sd : D[r][j] / sd;
-    for (int t = tid; t < NB * NB; t += blockDim.x) {
-      const int r = t % NB, c = t / NB;
-      if (c > j && r >= c && r < nb) D[r][c] -= D[r][j] * D[c][j] * inv;
-    }
-    __syncthreads();
+  if (s.bi >= s.bj) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int R = 4 * s.bi + r, C = 4 * s.bj + c;
+        if (R < nb && C < nb && R >= C) G[(int64_t)(j0 + C) * ldg + j0 + R] = a[r][c];
+      }
   }
-  for (int t = tid; t < NB * NB; t += blockDim.x) {
-    const int r = t % NB, c = t / NB;
-    if (r < nb && c < nb && r >= c) G[(int64_t)(j0 + c) * ldg + j0 + r] = Lc[r][c];
-  }
-  invert_lower_cols(Lc, Di);
-  __syncthreads();
-  for (int t = tid; t < NB * NB; t += blockDim.x) {
-    const int r = t % NB, c = t / NB;
-    dinv[c * NB + r] = (r >= c) ? Di[r][c] : 0.0;
-  }
+  store_block(s, x, dinv);
 }
 
 // L(rows of block i, cols of block jb) = A(...) * Dinv^T, for blocks below jb.
@@ -138,68 +243,94 @@ __global__ __launch_bounds__(256) void k_chol_update(int k, double* G, int64_t l
 
 // Invert every diagonal block of L; write its transpose into Rinv's diagonal
 // block (Rinv = L^{-T} is upper triangular) and keep the inverse in dinv.
+// One wavefront per block.
 __global__ __launch_bounds__(64) void k_trtri_diag(int k, const double* __restrict__ L, int64_t ldl, double* dinv,
                                                    double* Rinv, int64_t ldr) {
-  __shared__ double D[NB][NB + 1];
-  __shared__ double Di[NB][NB + 1];
-  const int tid = threadIdx.x;
+  __shared__ double col[NB], row[NB];
+  const Sub s;
   const int j0 = blockIdx.x * NB;
   const int nb = min(NB, k - j0);
-  for (int t = tid; t < NB * NB; t += blockDim.x) {
-    const int r = t % NB, c = t / NB;
-    D[r][c] = (r < nb && c < nb) ? (r >= c ? L[(int64_t)(j0 + c) * ldl + j0 + r] : 0.0) : (r == c ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  invert_lower_cols(D, Di);
-  __syncthreads();
-  double* dv = dinv + (int64_t)blockIdx.x * NB * NB;
-  for (int t = tid; t < NB * NB; t += blockDim.x) {
-    const int r = t % NB, c = t / NB;
-    const double v = (r >= c) ? Di[r][c] : 0.0;
-    dv[c * NB + r] = v;
-    // Rinv(j0 + c, j0 + r) = Linv(j0 + r, j0 + c)
-    if (r < nb && c < nb) Rinv[(int64_t)(j0 + r) * ldr + j0 + c] = v;
-  }
+  double a[4][4], x[4][4];
+  load_lower(s, L, ldl, j0, nb, a);
+  chol_inv_regs<false>(s, a, x, col, row);
+  store_block(s, x, dinv + (int64_t)blockIdx.x * NB * NB);
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int R = 4 * s.bi + r, C = 4 * s.bj + c;
+      // Rinv(j0 + C, j0 + R) = Linv(j0 + R, j0 + C)
+      if (R < nb && C < nb && R >= C) Rinv[(int64_t)(j0 + R) * ldr + j0 + C] = x[r][c];
+    }
 }
 
 // Block diagonal d of L^{-1}: X_ij = -Dinv_i * sum_{p=j}^{i-1} L_ip X_pj, i = j + d,
 // with X_pj read back from Rinv (X_pj = Rinv_jp^T). Written as Rinv_ji = X_ij^T.
+// Thread t accumulates row t & 31, columns 4 (t >> 5) .. + 3 of the sum in
+// registers; the tiles of the next p are loaded while the current ones are
+// multiplied.
 __global__ __launch_bounds__(256) void k_trtri_off(int k, const double* __restrict__ L, int64_t ldl,
                                                    const double* __restrict__ dinv, double* Rinv, int64_t ldr,
                                                    int d) {
   __shared__ double Lb[NB][NB + 1];
-  __shared__ double Xb[NB][NB + 1];
-  __shared__ double Acc[NB][NB + 1];
+  __shared__ double Xb[NB][NB + 4];
   const int tid = threadIdx.x;
   const int jb = blockIdx.x, ib = jb + d;
   const int i0 = ib * NB, j0 = jb * NB;
   const int ni = min(NB, k - i0), nj = min(NB, k - j0);
-  for (int t = tid; t < NB * NB; t += blockDim.x) Acc[t % NB][t / NB] = 0.0;
+  const int r = tid & 31, c0 = 4 * (tid >> 5);
+  double lv[4], xv[4];
+  // staging element h: L_ip[tid & 31][(tid >> 5) + 8 h] (columns of L are
+  // contiguous), X_pj[(tid >> 5) + 8 h][tid & 31] (rows of Rinv are contiguous)
+  auto fetch = [&](int pb) {
+    const int p0 = pb * NB, np = min(NB, k - p0);
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int q = (tid >> 5) + 8 * h;
+      lv[h] = (r < ni && q < np) ? L[(int64_t)(p0 + q) * ldl + i0 + r] : 0.0;
+      xv[h] = (q < np && r < nj) ? Rinv[(int64_t)(p0 + q) * ldr + j0 + r] : 0.0;
+    }
+  };
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  fetch(jb);
   for (int pb = jb; pb < ib; ++pb) {
-    const int p0 = pb * NB;
-    const int np = min(NB, k - p0);
     __syncthreads();
-    for (int t = tid; t < NB * NB; t += blockDim.x) {
-      const int r = t % NB, c = t / NB;
-      Lb[r][c] = (r < ni && c < np) ? L[(int64_t)(p0 + c) * ldl + i0 + r] : 0.0;           // L_ip[r][c]
-      Xb[r][c] = (r < np && c < nj) ? Rinv[(int64_t)(p0 + r) * ldr + j0 + c] : 0.0;        // X_pj[r][c]
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int q = (tid >> 5) + 8 * h;
+      Lb[r][q] = lv[h];
+      Xb[q][r] = xv[h];
     }
     __syncthreads();
-    for (int t = tid; t < NB * NB; t += blockDim.x) {
-      const int r = t % NB, c = t / NB;
-      double s = 0.0;
-      for (int q = 0; q < NB; ++q) s += Lb[r][q] * Xb[q][c];
-      Acc[r][c] += s;
+    if (pb + 1 < ib) fetch(pb + 1);
+#pragma unroll 8
+    for (int q = 0; q < NB; ++q) {
+      const double l = Lb[r][q];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] += l * Xb[q][c0 + c];
     }
   }
+  // Rinv_ji^T = -Dinv_i * Acc: Acc through LDS, Dinv_i (column-major) into Lb
+  const double* dv = dinv + (int64_t)ib * NB * NB;
   __syncthreads();
-  const double* dv = dinv + (int64_t)ib * NB * NB;  // Dinv_i, column-major
-  for (int t = tid; t < NB * NB; t += blockDim.x) {
-    const int r = t % NB, c = t / NB;
-    if (r >= ni || c >= nj) continue;
-    double s = 0.0;
-    for (int q = 0; q <= r; ++q) s += dv[q * NB + r] * Acc[q][c];
-    Rinv[(int64_t)(i0 + r) * ldr + j0 + c] = -s;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) Xb[r][c0 + c] = acc[c];
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int q = (tid >> 5) + 8 * h;
+    Lb[r][q] = dv[q * NB + r];
+  }
+  __syncthreads();
+  double out[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q <= r; ++q) {
+    const double l = Lb[r][q];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[c] += l * Xb[q][c0 + c];
+  }
+  if (r < ni) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c0 + c < nj) Rinv[(int64_t)(i0 + r) * ldr + j0 + c0 + c] = -out[c];
   }
 }
 

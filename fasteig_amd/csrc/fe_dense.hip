@@ -423,7 +423,11 @@ __global__ __launch_bounds__(256) void k_gram_reduce(int m, int p, int tiles_b, 
 constexpr int MKS = 16;
 
 // upper != 0: C is upper triangular (m == p; nothing below its diagonal is
-// read) and a column tile's inner loop stops at its last column.
+// read) and a column tile's inner loop stops at its last column. Column tile
+// ct then costs ct + 1 units, so one workgroup computes the pair ct = q and
+// ct = tiles_c - 1 - q of its row slab (tiles_c + 1 units for every q): with
+// one tile per workgroup the CUs that kept receiving the last column tile set
+// the finish time and the triangular product took as long as the full one.
 template <typename T, int TR, int TC>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
                                                  const double* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
@@ -437,26 +441,30 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
   __shared__ T Cs[MK][TC + GPAD];
   const int64_t t = xcd_tile(blockIdx.x, per_xcd);
   if (t >= ntiles) return;
-  const int64_t rt = t / tiles_c;
-  const int ct = static_cast<int>(t - rt * tiles_c);
+  const int per_slab = upper ? (tiles_c + 1) / 2 : tiles_c;  // workgroups per row slab
+  const int64_t rt = t / per_slab;
+  const int q = static_cast<int>(t - rt * per_slab);
+  const int npass = upper && tiles_c - 1 - q != q ? 2 : 1;
   const int64_t r0 = rt * TR;
-  const int c0 = ct * TC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / (TC / 64), wn = wave % (TC / 64);
-  // valid 16-wide blocks of the wavefront's 64 x 64 tile (wave-uniform)
+  // valid 16-wide row blocks of the wavefront's 64 x 64 tile (wave-uniform)
   const int64_t rows_left = n - r0 - wm * 64;
   const int vi = rows_left >= 64 ? 4 : static_cast<int>(max<int64_t>(0, (rows_left + 15) >> 4));
-  const int vj = min(4, max(0, (p - c0 - wn * 64 + 15) >> 4));
+  // staging: X row tid / TPR, k (tid % TPR) * XK .. + XK - 1; C column tid / TPC, k (tid % TPC) * CK ..
+  const int sr = tid / TPR, sk = (tid % TPR) * XK;
+  const int scc = tid / TPC, sck = (tid % TPC) * CK;
+  const int64_t xrow = r0 + sr;
   using acc_t = typename mfma<T>::acc;
+  for (int pass = 0; pass < npass; ++pass) {
+  const int ct = pass == 0 ? q : tiles_c - 1 - q;
+  const int c0 = ct * TC;
+  const int vj = min(4, max(0, (p - c0 - wn * 64 + 15) >> 4));
   acc_t acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{};
-  // staging: X row tid / TPR, k (tid % TPR) * XK .. + XK - 1; C column tid / TPC, k (tid % TPC) * CK ..
-  const int sr = tid / TPR, sk = (tid % TPR) * XK;
-  const int scc = tid / TPC, sck = (tid % TPC) * CK;
-  const int64_t xrow = r0 + sr;
   const int ccol = c0 + scc;
   T xr[XK], cr[CK];
   auto fetch = [&](int k0) {
@@ -525,6 +533,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
           y = beta == T(0) ? v : v + beta * y;
         }
       }
+  }  // pass (the next pass's first barrier orders its LDS writes after these reads)
 }
 
 }  // namespace fe
@@ -679,7 +688,8 @@ static int gemm_run(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X
   const int TR = thin ? 256 : WT, TC = thin ? 64 : WT;
   const int tiles_c = (p + TC - 1) / TC;
   const int64_t tiles_r = (n + TR - 1) / TR;
-  const int64_t ntiles = tiles_r * tiles_c;
+  // workgroups: one per output tile, or per pair of column tiles when C is triangular
+  const int64_t ntiles = tiles_r * (upper ? (tiles_c + 1) / 2 : tiles_c);
   const int64_t nb8 = (ntiles + 7) / 8 * 8;
   const int V = prec == FE_F64 ? 2 : 4;
   const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && ldx % V == 0;

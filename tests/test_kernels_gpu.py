@@ -498,7 +498,7 @@ def test_gemm_nn(dev, n, m, p, prec):
     assert np.max(np.abs(Yd.download() - ref)) <= tol * m
 
 
-@pytest.mark.parametrize("k", [1, 7, 31, 32, 33, 64, 130, 512])
+@pytest.mark.parametrize("k", [1, 7, 31, 32, 33, 64, 100, 130, 512])
 def test_potrf_rinv_cholqr(dev, k):
     rng = np.random.default_rng(k)
     n = max(4 * k, 50)
@@ -507,8 +507,13 @@ def test_potrf_rinv_cholqr(dev, k):
     Gd = dev.buffer((k, k), np.float64, order="F")
     _fe.check(L().fe_gram(dev.ctx, 0, _fe.F64, n, k, k, Xd.ptr, k, Xd.ptr, k, None, Gd.ptr, k))
     assert _fe.check(L().fe_potrf(dev.ctx, k, Gd.ptr, k)) == 0
+    # the factor against LAPACK's (through numpy) on the host Gram matrix
+    Lref = np.linalg.cholesky(X.T @ X)
+    assert np.max(np.abs(np.tril(Gd.download()) - Lref)) <= 1e-11 * np.max(np.abs(Lref))
     Rd = dev.buffer((k, k), np.float64, order="F")
     _fe.check(L().fe_chol_rinv(dev.ctx, k, Gd.ptr, k, Rd.ptr, k))
+    Rref = np.linalg.inv(Lref).T
+    assert np.max(np.abs(Rd.download() - Rref)) <= 1e-10 * np.max(np.abs(Rref))
     Qd = dev.buffer((n, k))
     _fe.check(L().fe_gemm_nn(dev.ctx, _fe.F64, n, k, k, Xd.ptr, k, Rd.ptr, k, Qd.ptr, k, 1.0, 0.0))
     Q = Qd.download()
