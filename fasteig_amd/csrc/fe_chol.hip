@@ -44,28 +44,6 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
 
-// 1 / sqrt(d) for finite d > 0: the hardware estimate and two Newton steps
-// y <- y (1 + (1 - d y^2) / 2) (each squares the relative error; the result
-// is within a few units in the last place).
-__device__ __forceinline__ double rsqrt_nr(double d) {
-  double y = __builtin_amdgcn_rsq(d);
-#pragma unroll
-  for (int it = 0; it < 2; ++it) {
-    const double e = fma(-d * y, y, 1.0);
-    y = fma(0.5 * y, e, y);
-  }
-  return y;
-}
-
-// 1 / d for finite nonzero d: the hardware estimate and two Newton steps
-// y <- y + y (1 - d y).
-__device__ __forceinline__ double rcp_nr(double d) {
-  double y = __builtin_amdgcn_rcp(d);
-#pragma unroll
-  for (int it = 0; it < 2; ++it) y = fma(y, fma(-d, y, 1.0), y);
-  return y;
-}
-
 // Lower triangle of the nb x nb block at (j0, j0) of the column-major matrix
 // M; rows and columns past nb are those of the identity. Every lane issues
 // its 16 loads (invalid ones from the block's first entry) before using any.

@@ -172,6 +172,28 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// 1 / sqrt(d) for finite d > 0: the hardware estimate and two Newton steps
+// y <- y (1 + (1 - d y^2) / 2) (each squares the relative error; the result
+// is within a few units in the last place).
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-d * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+// 1 / d for d whose reciprocal is a finite normal number: the hardware
+// estimate and two Newton steps y <- y + y (1 - d y).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) y = fma(y, fma(-d, y, 1.0), y);
+  return y;
+}
+
 inline unsigned blocks_for(int64_t work, int per_block) {
   return static_cast<unsigned>((work + per_block - 1) / per_block);
 }

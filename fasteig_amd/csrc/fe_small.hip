@@ -6,11 +6,12 @@
 //                 every eigenvalue (GvL 8.4.1), inverse iteration for the
 //                 eigenvectors of the tridiagonal (Wilkinson; shifts and
 //                 clusters as in LAPACK dstein), clusters of close eigenvalues
-//                 orthogonalized by a workgroup between the steps; for m > 16
-//                 the same inverse iteration and cluster phases run on their
-//                 own launches after
+//                 orthogonalized by a workgroup between the steps
+//   for m > 16:
 //   k_tri_step    the multi-workgroup tridiagonalization (one launch per column)
-//   k_syev_msect  and the multisection of the tridiagonal's eigenvalues
+//   k_syev_msect  multisection of the tridiagonal's eigenvalues
+//   k_syev_inviter  inverse iteration, one thread per eigenvalue
+//   k_syev_clusters the clusters of close eigenvalues, a workgroup each
 //   k_syev_back   multi-workgroup back-transformation V = H_0 ... H_{m-3} Z with
 //                 the eigenvector slice of each workgroup held in LDS
 #include "fe_common.hpp"
@@ -67,31 +68,106 @@ __device__ __forceinline__ double block_max(double v, double* red) {
   return s;
 }
 
-// Solve (T - x_j I) z = z in place with the LU factors of slot j (interleaved
-// [i * m + j]), then scale z to max-norm one.
+// Solve (T - x_j I) y = in_scale * z in place with the LU factors of slot j
+// (interleaved [i * m + j], u1 holding the reciprocals of the pivots, whose
+// magnitudes are at least tiny); return max |y_i| and, in *ss, sum y_i^2. The
+// recurrences carry their last values in registers, and the loads are
+// software-pipelined: the factors and z of the next TB steps are in flight
+// while the current TB steps are computed (a step waiting on its own loads
+// made each of the 2 m steps a global memory round trip).
+template <int TB>
+__device__ __forceinline__ double tri_lu_solve_raw(int m, int j, const double* __restrict__ u1,
+                                                   const double* __restrict__ u2, const double* __restrict__ u3,
+                                                   const double* __restrict__ ml, const double* __restrict__ pv,
+                                                   double tiny, double* __restrict__ Z, double in_scale,
+                                                   double* ss = nullptr) {
+  const int64_t M = m;
+  // forward: row interchanges and L; step i reads z_{i+1}, pv_i, ml_i
+  double fz[2][TB], fp[2][TB], fl[2][TB];
+  auto fload = [&](int i0, double (&zb)[TB], double (&pb)[TB], double (&lb)[TB]) {
+#pragma unroll
+    for (int u = 0; u < TB; ++u) {
+      const int i = i0 + u;
+      if (i + 1 < m) {
+        zb[u] = Z[(i + 1) * M + j];
+        pb[u] = pv[i * M + j];
+        lb[u] = ml[i * M + j];
+      }
+    }
+  };
+  double yc = Z[j] * in_scale;
+  auto fsteps = [&](int i0, const double (&zb)[TB], const double (&pb)[TB], const double (&lb)[TB]) {
+#pragma unroll
+    for (int u = 0; u < TB; ++u) {
+      const int i = i0 + u;
+      if (i + 1 < m) {
+        double yn = zb[u] * in_scale;
+        if (pb[u] != 0.0) {
+          const double tmp = yc;
+          yc = yn;
+          yn = tmp;
+        }
+        yn -= lb[u] * yc;
+        Z[i * M + j] = yc;
+        yc = yn;
+      }
+    }
+  };
+  fload(0, fz[0], fp[0], fl[0]);
+  for (int i0 = 0; i0 + 1 < m; i0 += 2 * TB) {
+    fload(i0 + TB, fz[1], fp[1], fl[1]);
+    fsteps(i0, fz[0], fp[0], fl[0]);
+    fload(i0 + 2 * TB, fz[0], fp[0], fl[0]);
+    fsteps(i0 + TB, fz[1], fp[1], fl[1]);
+  }
+  // backward: U y = z; step i reads z_i, u1_i, u2_i, u3_i
+  double bz[2][TB], b1[2][TB], b2[2][TB], b3[2][TB];
+  auto bload = [&](int i0, double (&zb)[TB], double (&p1)[TB], double (&p2)[TB], double (&p3)[TB]) {
+#pragma unroll
+    for (int u = 0; u < TB; ++u) {
+      const int i = i0 - u;
+      if (i >= 0) {
+        zb[u] = i == m - 1 ? 0.0 : Z[i * M + j];
+        p1[u] = u1[i * M + j];
+        p2[u] = i + 1 < m ? u2[i * M + j] : 0.0;
+        p3[u] = i + 2 < m ? u3[i * M + j] : 0.0;
+      }
+    }
+  };
+  double mx = 0.0, sq = 0.0, z1 = 0.0, z2 = 0.0;  // y_{i+1}, y_{i+2}
+  auto bsteps = [&](int i0, const double (&zb)[TB], const double (&p1)[TB], const double (&p2)[TB],
+                    const double (&p3)[TB]) {
+#pragma unroll
+    for (int u = 0; u < TB; ++u) {
+      const int i = i0 - u;
+      if (i >= 0) {
+        // y_{i+2} term first, off the chain through y_{i+1}; p1 holds 1 / u_ii
+        const double s = fma(-p2[u], z1, (i == m - 1 ? yc : zb[u]) - p3[u] * z2);
+        const double yi = s * p1[u];
+        Z[i * M + j] = yi;
+        mx = fmax(mx, fabs(yi));
+        sq += yi * yi;
+        z2 = z1;
+        z1 = yi;
+      }
+    }
+  };
+  bload(m - 1, bz[0], b1[0], b2[0], b3[0]);
+  for (int i0 = m - 1; i0 >= 0; i0 -= 2 * TB) {
+    bload(i0 - TB, bz[1], b1[1], b2[1], b3[1]);
+    bsteps(i0, bz[0], b1[0], b2[0], b3[0]);
+    bload(i0 - 2 * TB, bz[0], b1[0], b2[0], b3[0]);
+    bsteps(i0 - TB, bz[1], b1[1], b2[1], b3[1]);
+  }
+  if (ss) *ss = sq;
+  return mx;
+}
+
+// The same solve with z scaled to max-norm one afterwards.
+template <int TB>
 __device__ __forceinline__ void tri_lu_solve(int m, int j, const double* u1, const double* u2, const double* u3,
                                              const double* ml, const double* pv, double tiny, double* Z) {
-  for (int i = 0; i + 1 < m; ++i) {
-    double yi = Z[(int64_t)i * m + j], yi1 = Z[(int64_t)(i + 1) * m + j];
-    if (pv[(int64_t)i * m + j] != 0.0) {
-      const double tmp = yi;
-      yi = yi1;
-      yi1 = tmp;
-    }
-    yi1 -= ml[(int64_t)i * m + j] * yi;
-    Z[(int64_t)i * m + j] = yi;
-    Z[(int64_t)(i + 1) * m + j] = yi1;
-  }
-  double mx = 0.0;
-  for (int i = m - 1; i >= 0; --i) {
-    double s = Z[(int64_t)i * m + j];
-    if (i + 1 < m) s -= u2[(int64_t)i * m + j] * Z[(int64_t)(i + 1) * m + j];
-    if (i + 2 < m) s -= u3[(int64_t)i * m + j] * Z[(int64_t)(i + 2) * m + j];
-    double yi = s / u1[(int64_t)i * m + j];
-    if (!isfinite(yi)) yi = s / tiny;
-    Z[(int64_t)i * m + j] = yi;
-    mx = fmax(mx, fabs(yi));
-  }
+  const double mx = tri_lu_solve_raw<TB>(m, j, u1, u2, u3, ml, pv, tiny, Z, 1.0);
   if (mx > 0.0)
     for (int i = 0; i < m; ++i) Z[(int64_t)i * m + j] /= mx;
 }
@@ -101,7 +177,7 @@ __device__ __forceinline__ void tri_lu_solve(int m, int j, const double* u1, con
 // one column at a time (the dot products one wavefront each). The cluster is
 // copied to C with its columns contiguous (C[q * m + i]) for the sweep, so that
 // both the dot products and the updates read whole cache lines.
-__device__ void cluster_orth(int m, int t, int last, double* Z, double* C, double* zs, double* dots, double* red,
+__device__ __forceinline__ void cluster_orth(int m, int t, int last, double* Z, double* C, double* zs, double* dots, double* red,
                              int* info) {
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
   const int c = last - t + 1;
@@ -157,7 +233,8 @@ __device__ void cluster_orth(int m, int t, int last, double* Z, double* C, doubl
 // subspace, and orthogonalizes again. The members' order and shifts are those
 // of LAPACK dstein; dstein reorthogonalizes after every step, one member at a
 // time, which leaves the O(m) triangular solves on a single thread.
-__device__ void syev_clusters(int m, const double* lam, double onenrm, double pivmin, double* work, double* zs,
+template <int TB>
+__device__ __forceinline__ void syev_clusters(int m, const double* lam, double onenrm, double pivmin, double* work, double* zs,
                               double* dots, double* red, int* info) {
   const double ortol = 1.0e-5 * onenrm;
   const double tiny = fmax(DBL_EPSILON * onenrm, pivmin);
@@ -176,7 +253,7 @@ __device__ void syev_clusters(int m, const double* lam, double onenrm, double pi
     while (last + 1 < m && lam[last + 1] - lam[last] <= ortol) ++last;
     if (last > t && (ordinal++ % static_cast<int>(gridDim.x)) == static_cast<int>(blockIdx.x)) {
       cluster_orth(m, t, last, Z, C, zs, dots, red, info);
-      for (int j = t + tid; j <= last; j += nt) tri_lu_solve(m, j, u1, u2, u3, ml, pv, tiny, Z);
+      for (int j = t + tid; j <= last; j += nt) tri_lu_solve<TB>(m, j, u1, u2, u3, ml, pv, tiny, Z);
       __syncthreads();
       cluster_orth(m, t, last, Z, C, zs, dots, red, info);
     }
@@ -184,14 +261,109 @@ __device__ void syev_clusters(int m, const double* lam, double onenrm, double pi
   }
 }
 
+// Gershgorin interval [gl, gu] of the tridiagonal (d, e) widened by a few
+// units of roundoff, its 1-norm estimate and the pivot threshold of the
+// Sturm counts (LAPACK dstebz). One thread.
+__device__ void tri_bounds(int m, const double* d, const double* e, double* gl_out, double* gu_out,
+                           double* onenrm_out, double* pivmin_out) {
+  double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
+    const double er = i + 1 < m ? fabs(e[i]) : 0.0;
+    gl = fmin(gl, d[i] - el - er);
+    gu = fmax(gu, d[i] + el + er);
+    if (i + 1 < m) emax2 = fmax(emax2, e[i] * e[i]);
+  }
+  const double onenrm = fmax(fabs(gl), fabs(gu));
+  const double on = onenrm > 0.0 ? onenrm : 1.0;
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  const double pad = 2.0 * DBL_EPSILON * on * m + 2.0 * pivmin;  // widen the interval slightly
+  *onenrm_out = on;
+  *pivmin_out = pivmin;
+  *gl_out = gl - pad;
+  *gu_out = gu + pad;
+}
+
+// Eigenvector t of the tridiagonal (d, e) by inverse iteration at the
+// eigenvalue lam[t] (shifts of cluster members as LAPACK dstein), into
+// column t of Z (interleaved [i * m + t]) with its LU factors in slot t.
+// One thread; the caller's launch bounds must leave it the registers of the
+// pipelined solves (at 128 VGPRs the loop ran out of scratch memory).
+template <int TB>
+__device__ __forceinline__ void inverse_iteration(int m, int t, const double* lam, const double* d, const double* ps, double onenrm,
+                                  double pivmin, double* work, int* info) {
+  const double ortol = 1.0e-5 * onenrm;
+  const double pertol = 10.0 * DBL_EPSILON * onenrm;
+  const double tiny = fmax(DBL_EPSILON * onenrm, pivmin);
+  const int64_t MM = (int64_t)m * m;
+  double* u1 = work;
+  double* u2 = work + MM;
+  double* u3 = work + 2 * MM;
+  double* ml = work + 3 * MM;
+  double* pv_ = work + 4 * MM;
+  double* Z = work + 5 * MM;
+      // shift: inside a cluster each member sits at least pertol above the
+      // previous member's shift, so that no two factorizations coincide
+      int s0 = t;
+      while (s0 > 0 && lam[s0] - lam[s0 - 1] <= ortol) --s0;
+      double xj = lam[s0];
+      for (int q = s0 + 1; q <= t; ++q) xj = fmax(lam[q], xj + pertol);
+      // LU with partial pivoting of T - xj I into slot t (interleaved), as
+      // LAPACK dlagtf, with pivots of magnitude below tiny replaced by +-tiny as
+      // its solver dlagts (job -1) does; u1 keeps the reciprocals of the pivots.
+      // The pivot row's diagonal and superdiagonal entries are carried in
+      // registers and the step has no branches (the lanes of a wavefront take
+      // different pivots) and one reciprocal.
+      double u1c = d[0] - xj, u2c = m > 1 ? ps[0] : 0.0;
+      for (int i = 0; i + 1 < m; ++i) {
+        const double bi = ps[i];
+        const double a1 = d[i + 1] - xj;
+        const double c1 = i + 2 < m ? ps[i + 1] : 0.0;
+        const bool keep = fabs(u1c) >= fabs(bi);  // no row interchange
+        double piv = keep ? u1c : bi;
+        if (fabs(piv) < tiny) piv = copysign(tiny, piv);
+        const double rp = rcp_nr(piv);
+        const double l = (keep ? bi : u1c) * rp;
+        u1[(int64_t)i * m + t] = rp;
+        u2[(int64_t)i * m + t] = keep ? u2c : a1;
+        u3[(int64_t)i * m + t] = keep ? 0.0 : c1;
+        ml[(int64_t)i * m + t] = l;
+        pv_[(int64_t)i * m + t] = keep ? 0.0 : 1.0;
+        const double u1n = keep ? a1 - l * u2c : u2c - l * a1;
+        u2c = keep ? c1 : -l * c1;
+        u1c = u1n;
+      }
+      u1[(int64_t)(m - 1) * m + t] = rcp_nr(fabs(u1c) < tiny ? copysign(tiny, u1c) : u1c);
+      // seeded start vector, three inverse-iteration steps; each step's
+      // max-norm scaling is applied as the next one reads z (the iteration is
+      // invariant to scaling) and the 2-norm one at the end
+      for (int i = 0; i < m; ++i) Z[(int64_t)i * m + t] = hash_unit(t + 1, i + 1);
+      double scale = 1.0, ss = 0.0;
+      for (int iter = 0; iter < 3; ++iter) {
+        double mx = tri_lu_solve_raw<TB>(m, t, u1, u2, u3, ml, pv_, tiny, Z, scale, &ss);
+        if (!(mx > 0.0)) {
+          info[0] = 2;
+          mx = 1.0;
+        }
+        scale = 1.0 / mx;
+      }
+      // the last solve's sum of squares gives the 2-norm (its values are at
+      // most 1 / tiny times the max-normalized input, far from overflow)
+      double nrm = sqrt(ss);
+      if (!(nrm > 0.0) || !isfinite(nrm)) {
+        info[0] = 2;
+        nrm = 1.0;
+      }
+      const double f = 1.0 / nrm;
+  #pragma unroll 8
+      for (int i = 0; i < m; ++i) Z[(int64_t)i * m + t] *= f;
+}
+
 // work layout (doubles): u1, u2, u3, ml, piv, Z  -- each m*m, interleaved [i * m + t];
 // then m*m for the cluster copies (column q at C[q * m])
 __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda, double* W, double* tau,
-                                                   double* work, int* info, const double* __restrict__ dg,
-                                                   const double* __restrict__ eg, int phase) {
-  // phase 0: everything in one workgroup; with a precomputed tridiagonal (dg, eg)
-  // phase 1 runs the bisection and phase 2 the inverse iteration, each spread
-  // over the launch's workgroups (phase 2 reads the eigenvalues back from W).
+                                                   double* work, int* info) {
+  // m <= 16: everything in one workgroup
   __shared__ double vs[SYEV_MAX], ps[SYEV_MAX], wsv[SYEV_MAX];
   __shared__ double d[SYEV_MAX], e2[SYEV_MAX], lam[SYEV_MAX];
   __shared__ double zs[SYEV_MAX], dots[SYEV_MAX];
@@ -200,16 +372,8 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
   __shared__ int s_skip;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-  if (tid == 0 && blockIdx.x == 0 && phase < 2) info[0] = 0;
-
-  if (dg) {
-    // tridiagonal already formed by the multi-workgroup reduction
-    for (int i = tid; i < m; i += nt) {
-      d[i] = dg[i];
-      if (i + 1 < m) e2[i] = eg[i];
-    }
-    __syncthreads();
-  } else {
+  if (tid == 0) info[0] = 0;
+  {
     // symmetrize from the lower triangle
     for (int64_t t = tid; t < (int64_t)m * m; t += nt) {
       const int j = static_cast<int>(t / m), i = static_cast<int>(t % m);
@@ -275,23 +439,7 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
     __syncthreads();
   }
   // e2 currently holds e; Gershgorin bounds and squares
-  if (tid == 0) {
-    double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0;
-    for (int i = 0; i < m; ++i) {
-      const double el = i > 0 ? fabs(e2[i - 1]) : 0.0;
-      const double er = i + 1 < m ? fabs(e2[i]) : 0.0;
-      gl = fmin(gl, d[i] - el - er);
-      gu = fmax(gu, d[i] + el + er);
-      if (i + 1 < m) emax2 = fmax(emax2, e2[i] * e2[i]);
-    }
-    const double onenrm = fmax(fabs(gl), fabs(gu));
-    s_onenrm = onenrm > 0.0 ? onenrm : 1.0;
-    s_pivmin = DBL_MIN * fmax(1.0, emax2);
-    // widen the interval slightly
-    const double pad = 2.0 * DBL_EPSILON * s_onenrm * m + 2.0 * s_pivmin;
-    s_gl = gl - pad;
-    s_gu = gu + pad;
-  }
+  if (tid == 0) tri_bounds(m, d, e2, &s_gl, &s_gu, &s_onenrm, &s_pivmin);
   __syncthreads();
   // keep e (unsquared) in ps for the inverse iteration, squares in e2
   for (int i = tid; i + 1 < m; i += nt) {
@@ -302,12 +450,7 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
 
   // ---- 2. bisection: thread t finds the t-th smallest eigenvalue
   const double pivmin = s_pivmin, onenrm = s_onenrm;
-  const int t0 = blockIdx.x * nt + tid, tstride = nt * gridDim.x;
-  if (phase >= 2) {
-    for (int i = tid; i < m; i += nt) lam[i] = W[i];
-    __syncthreads();
-  }
-  for (int t = t0; phase < 2 && t < m; t += tstride) {
+  for (int t = tid; t < m; t += nt) {
     double lo = s_gl, hi = s_gu;
     for (int it = 0; it < 200; ++it) {
       const double mid = 0.5 * (lo + hi);
@@ -319,81 +462,51 @@ __global__ __launch_bounds__(1024) void k_syev_tri(int m, double* A, int64_t lda
     W[t] = lam[t];
   }
   __syncthreads();
-  if (phase == 1) return;
+  // ---- 3. inverse iteration, then the clusters of close eigenvalues
+  // (m <= 16: the solves are not pipelined, which would spill at 1024 threads)
+  for (int t = tid; t < m; t += nt) inverse_iteration<1>(m, t, lam, d, ps, onenrm, pivmin, work, info);
+  __syncthreads();
+  syev_clusters<1>(m, lam, onenrm, pivmin, work, zs, dots, red, info);
+}
 
-  // ---- 3. inverse iteration (cluster leaders process their cluster in order)
-  // Cluster threshold for reorthogonalization. LAPACK dstein uses 1e-3*||T||;
-  // 1e-5*||T|| keeps chains of merely close Ritz values short while pairs that
-  // are not reorthogonalized stay orthogonal to about eps*||T||/gap <= 1e-11.
-  const double ortol = 1.0e-5 * onenrm;
-  const double pertol = 10.0 * DBL_EPSILON * onenrm;
-  const int64_t MM = (int64_t)m * m;
-  double* u1 = work;
-  double* u2 = work + MM;
-  double* u3 = work + 2 * MM;
-  double* ml = work + 3 * MM;
-  double* pv_ = work + 4 * MM;
-  double* Z = work + 5 * MM;
-  if (phase == 3) {  // clustered eigenvalues, workgroup-cooperative
-    syev_clusters(m, lam, onenrm, pivmin, work, zs, dots, red, info);
-    return;
+// Phase 2 of the multi-workgroup path as a kernel of its own: one thread per
+// eigenvalue, 64 per workgroup, so the solves keep their values in registers.
+__global__ __launch_bounds__(64) void k_syev_inviter(int m, const double* __restrict__ dg,
+                                                     const double* __restrict__ eg, const double* __restrict__ W,
+                                                     double* work, int* info) {
+  __shared__ double d[SYEV_MAX], e[SYEV_MAX], lam[SYEV_MAX];
+  __shared__ double s_gl, s_gu, s_onenrm, s_pivmin;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < m; i += blockDim.x) {
+    d[i] = dg[i];
+    e[i] = i + 1 < m ? eg[i] : 0.0;
+    lam[i] = W[i];
   }
-  const double tiny = fmax(DBL_EPSILON * onenrm, pivmin);
-  for (int t = t0; t < m; t += tstride) {
-    // shift: inside a cluster each member sits at least pertol above the
-    // previous member's shift, so that no two factorizations coincide
-    int s0 = t;
-    while (s0 > 0 && lam[s0] - lam[s0 - 1] <= ortol) --s0;
-    double xj = lam[s0];
-    for (int q = s0 + 1; q <= t; ++q) xj = fmax(lam[q], xj + pertol);
-    // LU with partial pivoting of T - xj I into slot t (interleaved)
-    u1[0 * m + t] = d[0] - xj;
-    if (m > 1) u2[0 * m + t] = ps[0];
-    for (int i = 0; i + 1 < m; ++i) {
-      const double ui = u1[(int64_t)i * m + t];
-      const double bi = ps[i];
-      const double a1 = d[i + 1] - xj;
-      const double c1 = i + 2 < m ? ps[i + 1] : 0.0;
-      const double u2i = u2[(int64_t)i * m + t];
-      if (fabs(ui) >= fabs(bi)) {
-        const double piv = ui != 0.0 ? ui : tiny;
-        u1[(int64_t)i * m + t] = piv;
-        const double l = bi / piv;
-        ml[(int64_t)i * m + t] = l;
-        pv_[(int64_t)i * m + t] = 0.0;
-        u3[(int64_t)i * m + t] = 0.0;
-        u1[(int64_t)(i + 1) * m + t] = a1 - l * u2i;
-        if (i + 2 < m) u2[(int64_t)(i + 1) * m + t] = c1;
-      } else {
-        const double l = ui / bi;
-        ml[(int64_t)i * m + t] = l;
-        pv_[(int64_t)i * m + t] = 1.0;
-        u1[(int64_t)i * m + t] = bi;
-        u2[(int64_t)i * m + t] = a1;
-        u3[(int64_t)i * m + t] = c1;
-        u1[(int64_t)(i + 1) * m + t] = u2i - l * a1;
-        if (i + 2 < m) u2[(int64_t)(i + 1) * m + t] = -l * c1;
-      }
-    }
-    if (u1[(int64_t)(m - 1) * m + t] == 0.0) u1[(int64_t)(m - 1) * m + t] = tiny;
-    // seeded start vector, three inverse-iteration steps
-    for (int i = 0; i < m; ++i) Z[(int64_t)i * m + t] = hash_unit(t + 1, i + 1);
-    for (int iter = 0; iter < 3; ++iter) {
-      tri_lu_solve(m, t, u1, u2, u3, ml, pv_, tiny, Z);
-      double nrm = 0.0;
-      for (int i = 0; i < m; ++i) nrm += Z[(int64_t)i * m + t] * Z[(int64_t)i * m + t];
-      nrm = sqrt(nrm);
-      if (!(nrm > 0.0)) {
-        info[0] = 2;
-        nrm = 1.0;
-      }
-      for (int i = 0; i < m; ++i) Z[(int64_t)i * m + t] /= nrm;
-    }
+  __syncthreads();
+  if (tid == 0) tri_bounds(m, d, e, &s_gl, &s_gu, &s_onenrm, &s_pivmin);
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + tid;
+  if (t < m) inverse_iteration<8>(m, t, lam, d, e, s_onenrm, s_pivmin, work, info);
+}
+
+// Phase 3: the clusters of close eigenvalues, one workgroup of 256 threads
+// per cluster (dealt round-robin).
+__global__ __launch_bounds__(256) void k_syev_clusters(int m, const double* __restrict__ dg,
+                                                       const double* __restrict__ eg,
+                                                       const double* __restrict__ W, double* work, int* info) {
+  __shared__ double d[SYEV_MAX], e[SYEV_MAX], lam[SYEV_MAX], zs[SYEV_MAX], dots[SYEV_MAX];
+  __shared__ double red[16];
+  __shared__ double s_gl, s_gu, s_onenrm, s_pivmin;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < m; i += blockDim.x) {
+    d[i] = dg[i];
+    e[i] = i + 1 < m ? eg[i] : 0.0;
+    lam[i] = W[i];
   }
-  if (phase == 0) {  // one workgroup: the clusters follow on the same launch
-    __syncthreads();
-    syev_clusters(m, lam, onenrm, pivmin, work, zs, dots, red, info);
-  }
+  __syncthreads();
+  if (tid == 0) tri_bounds(m, d, e, &s_gl, &s_gu, &s_onenrm, &s_pivmin);
+  __syncthreads();
+  syev_clusters<8>(m, lam, s_onenrm, s_pivmin, work, zs, dots, red, info);
 }
 
 // ---- multi-workgroup Householder tridiagonalization (one launch per column)
@@ -491,12 +604,17 @@ __global__ __launch_bounds__(256) void k_tri_step(int m, int k, double* A, int64
       }
     }
   }
-  const int jfirst = k + 1 + blockIdx.x * 4 + wave;
-  double cq[PE];
+  // wave g updates the columns j = g (mod nwaves) right of k, the same ones
+  // in every launch; its first two are read with the step's other loads
+  const int nwaves = gridDim.x * 4, g = blockIdx.x * 4 + wave;
+  const int jfirst = g + nwaves * ((k + 1 - g + nwaves - 1) / nwaves > 0 ? (k + 1 - g + nwaves - 1) / nwaves : 0);
+  const int jsecond = jfirst + nwaves;
+  double cq[PE], cq2[PE];
 #pragma unroll
   for (int q = 0; q < PE; ++q) {
     const int i = k + 1 + lane + 64 * q;
     cq[q] = (jfirst < m && i < m) ? A[(int64_t)jfirst * lda + i] : 0.0;
+    cq2[q] = (jsecond < m && i < m) ? A[(int64_t)jsecond * lda + i] : 0.0;
   }
   const double tprev = upd ? tau[k - 1] : 0.0;
   if (upd) {
@@ -550,16 +668,15 @@ __global__ __launch_bounds__(256) void k_tri_step(int m, int k, double* A, int64
     for (int i = k + 1 + tid; i < m; i += 256) Vh[(int64_t)k * m + i] = vk[i];
   }
   // columns j > k: apply the update of step k-1 (rows > k) and form p_k[j]
-  const int nwaves = gridDim.x * 4;
   for (int j = jfirst; j < m; j += nwaves) {
     double* cj = A + (int64_t)j * lda;
     double s = 0.0;
-    if (j == jfirst) {
+    if (j == jfirst || j == jsecond) {
 #pragma unroll
       for (int q = 0; q < PE; ++q) {
         const int i = k + 1 + lane + 64 * q;
         if (i < m) {
-          double a = cq[q];
+          double a = j == jfirst ? cq[q] : cq2[q];
           if (upd) {
             a -= vp[i] * wp[j] + wp[i] * vp[j];
             cj[i] = a;
@@ -603,11 +720,16 @@ __global__ void k_tri_finish(int m, double* A, int64_t lda, const double* __rest
 }
 
 // V(:, cols) = H_0 ... H_{m-3} Z(:, cols); Householder vectors in A's lower part.
-// Each workgroup owns CB columns of Z, held in LDS as [row][CB].
+// Each workgroup owns CB columns of Z, held in LDS as [row][CB]. The
+// reflectors are staged into LDS R at a time (one memory round trip per R
+// steps instead of one per step), so a step is LDS work and one barrier.
 __global__ __launch_bounds__(256) void k_syev_back(int m, const double* __restrict__ A, int64_t lda,
                                                    const double* __restrict__ tau, const double* __restrict__ Z,
-                                                   double* V, int64_t ldv, int CB) {
-  extern __shared__ double zs[];  // m * CB
+                                                   double* V, int64_t ldv, int CB, int R) {
+  extern __shared__ double lds[];
+  double* zs = lds;          // m * CB
+  double* vs = lds + m * CB;  // R * m: reflector kh - r at vs[r * m + i], zero for rows i <= kh - r
+  __shared__ double ts[32];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CB;
   const int ncol = min(CB, m - c0);
@@ -615,22 +737,32 @@ __global__ __launch_bounds__(256) void k_syev_back(int m, const double* __restri
     const int i = t / CB, c = t % CB;
     zs[t] = c < ncol ? Z[(int64_t)i * m + c0 + c] : 0.0;
   }
-  __syncthreads();
-  const int TPC = 256 / CB;         // threads per column
+  const int TPC = 256 / CB;  // threads per column
   const int c = tid / TPC, sub = tid % TPC;
-  for (int k = m - 3; k >= 0; --k) {
-    const double tk = tau[k];
-    if (tk == 0.0) continue;
-    const int len = m - k - 1;
-    const double* v = A + (int64_t)k * lda + (k + 1);
-    double s = 0.0;
-    for (int i = sub; i < len; i += TPC) s += v[i] * zs[(k + 1 + i) * CB + c];
-    for (int o = TPC / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    s *= tk;
+  const int RC = R > 0 ? R : 1;  // R == 0: no staging (the slice fills the LDS), v read from A
+  for (int kh = m - 3; kh >= 0; kh -= RC) {
+    const int nr = min(RC, kh + 1);  // reflectors kh, kh - 1, .., kh - nr + 1
+    __syncthreads();                 // the previous chunk is no longer read
+    for (int t = tid; R > 0 && t < nr * m; t += blockDim.x) {
+      const int r = t / m, i = t % m, k = kh - r;
+      vs[t] = i > k ? A[(int64_t)k * lda + i] : 0.0;
+    }
+    if (tid < nr) ts[tid] = tau[kh - tid];
     __syncthreads();
-    for (int i = sub; i < len; i += TPC) zs[(k + 1 + i) * CB + c] -= s * v[i];
-    __syncthreads();
+    for (int r = 0; r < nr; ++r) {
+      const int k = kh - r;
+      const double tk = ts[r];
+      if (tk == 0.0) continue;  // the same in every thread
+      const double* v = R > 0 ? vs + r * m : A + (int64_t)k * lda;
+      double s = 0.0;
+      for (int i = k + 1 + sub; i < m; i += TPC) s += v[i] * zs[i * CB + c];
+      for (int o = TPC / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      s *= tk;
+      for (int i = k + 1 + sub; i < m; i += TPC) zs[i * CB + c] -= s * v[i];
+      __syncthreads();  // the next step's sums read rows other threads updated
+    }
   }
+  __syncthreads();
   for (int t = tid; t < m * CB; t += blockDim.x) {
     const int cc = t / m, i = t % m;
     if (cc < ncol) V[(int64_t)(c0 + cc) * ldv + i] = zs[i * CB + cc];
@@ -681,12 +813,10 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
   if (multi) {
     const unsigned g = blocks_for(m, 64);
     hipLaunchKernelGGL(k_syev_msect, dim3(blocks_for(m, 4)), dim3(256), 0, ctx->stream, m, dd, ee, W, ctx->info_dev);
-    hipLaunchKernelGGL(k_syev_tri, dim3(g), dim3(64), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev, dd, ee, 2);
-    hipLaunchKernelGGL(k_syev_tri, dim3(16), dim3(256), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev, dd, ee,
-                       3);
+    hipLaunchKernelGGL(k_syev_inviter, dim3(g), dim3(64), 0, ctx->stream, m, dd, ee, W, work, ctx->info_dev);
+    hipLaunchKernelGGL(k_syev_clusters, dim3(16), dim3(256), 0, ctx->stream, m, dd, ee, W, work, ctx->info_dev);
   } else {
-    hipLaunchKernelGGL(k_syev_tri, dim3(1), dim3(1024), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev,
-                       nullptr, nullptr, 0);
+    hipLaunchKernelGGL(k_syev_tri, dim3(1), dim3(1024), 0, ctx->stream, m, A, lda, W, tau, work, ctx->info_dev);
   }
   FE_CHECK_LAUNCH();
   // columns per workgroup: about m / 128 (so that m = 512 spreads over 128
@@ -695,10 +825,13 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
   // reduce within one wavefront), LDS slice m*CB doubles within 64 KiB
   int CB = 32;
   while (CB > 4 && (CB * 128 > m || (size_t)m * CB * sizeof(double) > 65536)) CB >>= 1;
-  const size_t lds = sizeof(double) * (size_t)m * CB;
+  // reflectors staged R at a time, the slice and R reflectors within 64 KiB
+  // of LDS (at most 32; none when the slice alone fills it)
+  const int R = std::max(0, std::min(32, (8192 - m * CB) / m));
+  const size_t lds = sizeof(double) * ((size_t)m * CB + (size_t)R * m);
   hipLaunchKernelGGL(k_syev_back, dim3(blocks_for(m, CB)), dim3(256), lds, ctx->stream, m,
                      multi ? (const double*)Vh : (const double*)A, multi ? (int64_t)m : lda, tau, work + 5 * MM, V,
-                     ldv, CB);
+                     ldv, CB, R);
   FE_CHECK_LAUNCH();
   int info = 0;
   FE_HIP_TRY(hipMemcpyAsync(&info, ctx->info_dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
