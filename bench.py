@@ -108,7 +108,7 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
     return per, algo
 
 
-MFMA_PEAK_TFS = {"fp32": 157.3, "fp64": 78.6}
+MFMA_PEAK_TFS = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}
 
 
 def dense_roofline(n, m=512, reps=10):
@@ -142,7 +142,27 @@ def dense_roofline(n, m=512, reps=10):
             "trmm": (lambda: L.fe_trmm_upper(dev.ctx, prec, n, m, X.ptr, m, Cm.ptr, m, Yb.ptr, m),
                      float(n) * m * m),
         }
-        for name, (fn, flops) in cases.items():
+        if prec == _fe.F32:
+            # compensated BF16 products: useful (FP32-equivalent) flops; the
+            # fraction is of the BF16 MFMA peak over the flops executed
+            for mname, mode, nprod in (("bf16x3", _fe.GRAM_BF16X3, 3), ("bf16x6", _fe.GRAM_BF16X6, 6)):
+                cases[f"gram_sym_{mname}"] = (
+                    lambda mode=mode: L.fe_gram(dev.ctx, mode, prec, n, m, m, X.ptr, m, X.ptr, m, None, G.ptr, m),
+                    float(n) * m * m, nprod)
+                cases[f"gram_{mname}"] = (
+                    lambda mode=mode: L.fe_gram(dev.ctx, mode, prec, n, m, m, X.ptr, m, Yb.ptr, m, None, G.ptr, m),
+                    2.0 * n * m * m, nprod)
+                cases[f"gemm_{mname}"] = (
+                    lambda mode=mode: L.fe_gemm_nn_mode(dev.ctx, mode, prec, n, m, m, X.ptr, m, Cm.ptr, m, Yb.ptr,
+                                                        m, 1.0, 0.0),
+                    2.0 * n * m * m, nprod)
+                cases[f"trmm_{mname}"] = (
+                    lambda mode=mode: L.fe_trmm_upper_mode(dev.ctx, mode, prec, n, m, X.ptr, m, Cm.ptr, m, Yb.ptr,
+                                                           m),
+                    float(n) * m * m, nprod)
+        for name, case in cases.items():
+            fn, flops = case[0], case[1]
+            nprod = case[2] if len(case) > 2 else 0
             for _ in range(2):
                 _fe.check(fn())
             dev.sync()
@@ -154,9 +174,11 @@ def dense_roofline(n, m=512, reps=10):
             _fe.check(L.fe_event_elapsed_ms(dev.ctx, 4, 5, ctypes.byref(ms)))
             per = ms.value / reps
             tfs = flops / (per * 1e-3) / 1e12
-            out[f"{name}_{pname}"] = {"ms": per, "tflops": tfs, "frac": tfs / MFMA_PEAK_TFS[pname]}
+            peak_name = "bf16" if nprod else pname
+            frac = tfs * max(nprod, 1) / MFMA_PEAK_TFS[peak_name]
+            out[f"{name}_{pname}"] = {"ms": per, "tflops": tfs, "frac": frac}
             log(f"dense {name} {pname} n={n} m={m}: {per:.3f} ms {tfs:.1f} TF/s "
-                f"({100 * tfs / MFMA_PEAK_TFS[pname]:.0f}% of {MFMA_PEAK_TFS[pname]})")
+                f"({100 * frac:.0f}% of the {peak_name} peak {MFMA_PEAK_TFS[peak_name]})")
         del X, Yb
     del Cm, G
     dev.close()

@@ -7,7 +7,8 @@
 
 Options use the reference's fasteig_options keys (nev, which, solver, maxiter,
 degree, blksize, maxblks, tol, seed, verbosity, print_every) plus the HIP
-backend's precision ("fp64" | "mixed"), gram ("native" | "bf16x3"), format
+backend's precision ("fp64" | "mixed"), gram ("auto" | "native" | "bf16x3" | "bf16x6": the
+product mode of FP32 blocks, "auto" measured per shape), format
 ("auto" | "csr" | "sell" | "blocked" | "tiled") and device. Everything runs on the GPU; a missing library is
 an error, never a CPU fallback.
 """
@@ -45,6 +46,7 @@ def lib():
         L.feh_problem_results.argtypes = [vp, vp, vp, vp, C.POINTER(i64), dp]
         L.feh_problem_eigenvectors.argtypes = [vp, vp]
         L.feh_problem_results_text.argtypes = [vp, C.c_char_p, i32]
+        L.feh_problem_product_tuning.argtypes = [vp, C.c_char_p, i32]
         L.feh_core_lanczos_bounds.argtypes = [vp, C.c_double, C.c_uint, dp, dp]
         L.feh_core_thin_qr.argtypes = [vp, i64, i32, vp, i32]
         L.feh_core_orthonormalize_block.argtypes = [vp, i64, i32, vp, i32, vp, C.c_double, vp,
@@ -122,6 +124,21 @@ class Problem:
         buf = C.create_string_buffer(1 << 20)
         _check(lib().feh_problem_results_text(self.h, buf, len(buf)))
         return buf.value.decode()
+
+    def product_tuning(self) -> list[dict]:
+        """Measured product modes of this problem's device context (gram="auto"):
+        per product shape the time (ms) of the native, BF16x3 and BF16x6 modes
+        and their errors relative to the FP64 product of the same FP32 data
+        (row sample)."""
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().feh_problem_product_tuning(self.h, buf, len(buf)))
+        out = []
+        for line in buf.value.decode().splitlines():
+            f = line.split()
+            out.append({"kind": f[0], "rows": int(f[1]), "m": int(f[2]), "p": int(f[3]),
+                        "ms": dict(zip(("native", "bf16x3", "bf16x6"), map(float, f[4:7]))),
+                        "err": dict(zip(("native", "bf16x3", "bf16x6"), map(float, f[7:10])))})
+        return out
 
     # ---- core numerics on this problem's device (the reference's L4 free functions)
     def lanczos_bounds(self, sign: float = 1.0, seed: int = 1) -> tuple[float, float]:

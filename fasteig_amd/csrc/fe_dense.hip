@@ -8,9 +8,9 @@
 //   B operand: lane l holds B[l / 16][l % 16]      (4 x 16)
 //   D result : lane l, item i holds D[4 (l / 16) + i][l % 16] (FP32)
 //              and D[l / 16 + 4 i][l % 16] (FP64: rows interleaved)
-// The BF16x3 Gram splits each FP32 operand into bf16 hi + lo parts and runs
-// v_mfma_f32_16x16x32_bf16 (8 bf16 per lane: lane l holds A[l % 16][8 (l/16) + 0..7]).
-// A workgroup is 4 wavefronts computing a 64 x 64 output tile (32 x 32 each).
+// The compensated BF16 products (FP32 blocks) split each operand into 2 or 3
+// bfloat16 parts and run v_mfma_f32_16x16x32_bf16 (8 bf16 per lane: lane l
+// holds A[l % 16][8 (l / 16) + 0..7] and B[8 (l / 16) + 0..7][l % 16]).
 #include "fe_common.hpp"
 
 #include <algorithm>
@@ -280,95 +280,301 @@ __global__ __launch_bounds__(256, 4) void k_gram64(int64_t n, int m, int p, cons
     }
 }
 
-// ---------------------------------------------------------------- Gram (BF16x3)
-__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
-  hi = static_cast<__bf16>(x);
-  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+// ---------------------------------------------------------------- compensated BF16 products
+// An FP32 value x is split into NS bfloat16 parts, x_0 = bf16(x),
+// x_1 = bf16(x - x_0), x_2 = bf16(x - x_0 - x_1) (each difference is exact in
+// FP32), and a product x y is summed on BF16 MFMA (v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation) over the part pairs s + t < NS:
+//   NS = 2 (BF16x3): x_1 y_0 + x_0 y_1 + x_0 y_0, about 16 significant bits;
+//   NS = 3 (BF16x6): the six pairs s + t <= 2, within a few units of the FP32
+//                    rounding of the product,
+// smallest terms first. Every element is split once, when its stage is
+// written to LDS, where the parts sit k-contiguous so that a lane reads its
+// 8 consecutive k values of a part with one 16-byte load.
+typedef float f2v_t __attribute__((ext_vector_type(2)));
+typedef __bf16 b2v_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u4v_t __attribute__((ext_vector_type(4)));  // 16 bytes of packed bf16
+
+// two FP32 values -> NS packed bfloat16 pairs (first value in the low half)
+template <int NS>
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t (&o)[NS]) {
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const uint32_t bits = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v_t{a, b}, b2v_t));
+    o[s] = bits;
+    if (s + 1 < NS) {
+      a -= __uint_as_float(bits << 16);
+      b -= __uint_as_float(bits & 0xffff0000u);
+    }
+  }
 }
 
-__global__ __launch_bounds__(256) void k_gram_bf16x3(int64_t n, int m, int p, const float* __restrict__ X,
-                                                     int64_t ldx, const float* __restrict__ Y, int64_t ldy,
-                                                     const double* __restrict__ w, int tiles_b, int ntiles,
-                                                     int sym, int64_t rows_per,
-                                                     double* __restrict__ part) {
-  // stage KCG rows; store as [col][row] so each lane's 8 consecutive k values are contiguous
-  __shared__ __bf16 Xh[TILE][KCG + 8], Xl[TILE][KCG + 8], Yh[TILE][KCG + 8], Yl[TILE][KCG + 8];
+// acc += sum over part pairs s + t < NS of a[s] b[t], smallest first
+template <int NS>
+__device__ __forceinline__ f4_t mfma_split(const bf8_t (&a)[NS], const bf8_t (&b)[NS], f4_t acc) {
+  if constexpr (NS == 3) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
+  }
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
+}
+
+constexpr int BK = 32;       // k values per stage (one 16x16x32 MFMA)
+constexpr int BKP = BK + 8;  // LDS row stride in bf16: 80 bytes, 16-byte reads of 16 lanes hit distinct banks
+
+// Gram G = X^T diag(w) Y on GT x GT = 256 x 256 tiles, split-K over row
+// chunks as k_gram, 512 threads (8 wavefronts of 128 x 64). Each staged row is
+// used for a 256-wide slice of the output: with 128 x 128 tiles the tile
+// loads alone moved 14.5 GB per 884,736 x 512 Gram and held it at the native
+// FP32 product's time. Stage: 32 rows; thread t (lane l of wavefront w) loads
+// rows 4 (l & 7) .. + 3 and columns 4 ((l >> 3) + 8 w) .. + 3 of X and Y (four
+// 16-byte loads each) and writes each column's 4 rows as 8-byte runs of the
+// parts, laid out [column][row] (eight row groups per wavefront keep the
+// transposed writes at two lanes per bank). For a diagonal tile of a
+// symmetric Gram the Y stage is the X stage.
+constexpr int GT = 256;
+template <int NS>
+__global__ __launch_bounds__(512, 1) void k_gram_bf16(int64_t n, int m, int p, const float* __restrict__ X,
+                                                   int64_t ldx, const float* __restrict__ Y, int64_t ldy,
+                                                   const double* __restrict__ w, int tiles_b, int ntiles, int sym,
+                                                   int64_t rows_per, int64_t per_xcd, int64_t nunits, int vec,
+                                                   double* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 Xl[NS][GT][BKP];
+  __shared__ __attribute__((aligned(16))) __bf16 Yl[NS][GT][BKP];
+  const int64_t u = xcd_tile(blockIdx.x, per_xcd);
+  if (u >= nunits) return;
+  const int64_t chunk = u / ntiles;
+  const int tile = static_cast<int>(u - chunk * ntiles);
   int ta, tb;
-  gram_tile(blockIdx.x, tiles_b, sym, ta, tb);
-  const int a0 = ta * TILE, b0 = tb * TILE;
-  const int64_t chunk = blockIdx.y;
+  gram_tile(tile, tiles_b, sym, ta, tb);
+  const bool same = sym && ta == tb;  // workgroup-uniform
+  const int a0 = ta * GT, b0 = tb * GT;
   const int64_t r0 = chunk * rows_per;
   const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  f4_t acc[2][2];
+  const int wm = wave >> 2, wn = wave & 3;  // rows wm * 128 .. + 127, columns wn * 64 .. + 63
+  const int vi = min(8, max(0, (m - a0 - wm * 128 + 15) >> 4)), vj = min(4, max(0, (p - b0 - wn * 64 + 15) >> 4));
+  f4_t acc[8][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4_t{};
-  const int lr = tid >> 3, lc = (tid & 7) * 8;
-  for (int64_t rb = r0; rb < r1; rb += KCG) {
-    {
-      const int64_t r = rb + lr;
-      const bool rok = r < r1;
-      const float wr = (w && rok) ? static_cast<float>(w[r]) : 1.0f;
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{};
+  const int rb = 4 * (lane & 7), cb = 4 * ((lane >> 3) + 8 * wave);
+  const int va = m - (a0 + cb), vb = p - (b0 + cb);
+  float xr[4][4], yr[4][4];
+  auto load4 = [&](const float* q, int valid, float (&v)[4]) {
+    if (vec && valid >= 4) {
+      const float4 t = *reinterpret_cast<const float4*>(q);
+      v[0] = t.x, v[1] = t.y, v[2] = t.z, v[3] = t.w;
+    } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int ca = a0 + lc + e, cb = b0 + lc + e;
-        const float xv = (rok && ca < m) ? X[r * ldx + ca] : 0.0f;
-        float yv = (rok && cb < p) ? Y[r * ldy + cb] : 0.0f;
-        if (w) yv *= wr;
-        __bf16 h, l;
-        split_bf16(xv, h, l);
-        Xh[lc + e][lr] = h;
-        Xl[lc + e][lr] = l;
-        split_bf16(yv, h, l);
-        Yh[lc + e][lr] = h;
-        Yl[lc + e][lr] = l;
+      for (int e = 0; e < 4; ++e) v[e] = e < valid ? q[e] : 0.0f;
+    }
+  };
+  auto fetch = [&](int64_t r) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t row = r + rb + i;
+      if (row < r1) {
+        load4(X + row * ldx + a0 + cb, va, xr[i]);
+        if (!same) {
+          load4(Y + row * ldy + b0 + cb, vb, yr[i]);
+          if (w) {
+            const double wr = w[row];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) yr[i][e] = static_cast<float>(static_cast<double>(yr[i][e]) * wr);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xr[i][e] = yr[i][e] = 0.0f;
       }
     }
+  };
+  typedef uint32_t u2v_t __attribute__((ext_vector_type(2)));
+  auto put = [&](const float (&v)[4][4], __bf16 (*L)[GT][BKP]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      uint32_t lo[NS], hi[NS];
+      split_pair<NS>(v[0][c], v[1][c], lo);
+      split_pair<NS>(v[2][c], v[3][c], hi);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) *reinterpret_cast<u2v_t*>(&L[s][cb + c][rb]) = u2v_t{lo[s], hi[s]};
+    }
+  };
+  fetch(r0);
+  for (int64_t r = r0; r < r1; r += BK) {
     __syncthreads();
+    put(xr, Xl);
+    if (!same) put(yr, Yl);
+    __syncthreads();
+    if (r + BK < r1) fetch(r + BK);
     const int kb = 8 * (lane >> 4);
-    bf8_t ah[2], al[2], bh[2], bl[2];
+    bf8_t bfr[4][NS];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int c = wm * 32 + i * 16 + (lane & 15);
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        ah[i][e] = Xh[c][kb + e];
-        al[i][e] = Xl[c][kb + e];
-      }
+      for (int s = 0; s < NS; ++s)
+        bfr[j][s] = *reinterpret_cast<const bf8_t*>(same ? &Xl[s][wn * 64 + j * 16 + (lane & 15)][kb]
+                                                         : &Yl[s][wn * 64 + j * 16 + (lane & 15)][kb]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      bf8_t ai[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) ai[s] = *reinterpret_cast<const bf8_t*>(&Xl[s][wm * 128 + i * 16 + (lane & 15)][kb]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (i < vi && j < vj) acc[i][j] = mfma_split<NS>(ai, bfr[j], acc[i][j]);
     }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int c = wn * 32 + j * 16 + (lane & 15);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        bh[j][e] = Yh[c][kb + e];
-        bl[j][e] = Yl[c][kb + e];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-      }
-    __syncthreads();
   }
-  double* P = part + ((int64_t)chunk * ntiles + blockIdx.x) * (TILE * TILE);
+  double* P = part + u * (GT * GT);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = wm * 32 + i * 16 + 4 * (lane >> 4) + e;
-        const int col = wn * 32 + j * 16 + (lane & 15);
-        P[col * TILE + row] = static_cast<double>(acc[i][j][e]);
+        const int row = wm * 128 + i * 16 + 4 * (lane >> 4) + e;
+        const int col = wn * 64 + j * 16 + (lane & 15);
+        P[col * GT + row] = static_cast<double>(acc[i][j][e]);
       }
+}
+
+// C (FP64 column-major m x p, ldc) -> NS bfloat16 parts of float(C), part s at
+// S + s p mpad, column j at + j mpad, k contiguous, zero for k >= m (and, with
+// upper, below the diagonal k > j).
+template <int NS>
+__global__ void k_split_coeffs(int m, int p, const double* __restrict__ C, int64_t ldc, int upper, int mpad,
+                               __bf16* __restrict__ S) {
+  const int64_t total = (int64_t)p * (mpad / 2);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = static_cast<int>(t / (mpad / 2)), k = 2 * static_cast<int>(t % (mpad / 2));
+    const bool ok0 = k < m && (!upper || k <= j), ok1 = k + 1 < m && (!upper || k + 1 <= j);
+    const float c0 = ok0 ? static_cast<float>(C[(int64_t)j * ldc + k]) : 0.0f;
+    const float c1 = ok1 ? static_cast<float>(C[(int64_t)j * ldc + k + 1]) : 0.0f;
+    uint32_t o[NS];
+    split_pair<NS>(c0, c1, o);
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      *reinterpret_cast<uint32_t*>(S + (int64_t)s * p * mpad + (int64_t)j * mpad + k) = o[s];
+  }
+}
+
+// Y = alpha X C + beta Y on 128 x 128 output tiles from the split parts of X
+// (staged: thread t loads 16 k values of row t >> 1 with four 16-byte loads)
+// and of C (pre-split by k_split_coeffs: two 16-byte loads per part). Tile
+// mapping, triangular pairing and epilogue as k_gemm_nn.
+template <int NS>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16(int64_t n, int m, int p, const float* __restrict__ X,
+                                                   int64_t ldx, const __bf16* __restrict__ S, int mpad, float* Y,
+                                                   int64_t ldy, float alpha, float beta, int tiles_c,
+                                                   int64_t per_xcd, int64_t ntiles, int vec, int upper) {
+  __shared__ __attribute__((aligned(16))) __bf16 Xl[NS][WT][BKP];
+  __shared__ __attribute__((aligned(16))) __bf16 Cl[NS][WT][BKP];
+  const int64_t t = xcd_tile(blockIdx.x, per_xcd);
+  if (t >= ntiles) return;
+  const int per_slab = upper ? (tiles_c + 1) / 2 : tiles_c;
+  const int64_t rt = t / per_slab;
+  const int q = static_cast<int>(t - rt * per_slab);
+  const int npass = upper && tiles_c - 1 - q != q ? 2 : 1;
+  const int64_t r0 = rt * WT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t rows_left = n - r0 - wm * 64;
+  const int vi = rows_left >= 64 ? 4 : static_cast<int>(max<int64_t>(0, (rows_left + 15) >> 4));
+  const int sr = tid >> 1, sk = 16 * (tid & 1);  // staging: row (X) / column (C) sr, k sk .. sk + 15
+  const int64_t xrow = r0 + sr;
+  const size_t pstride = (size_t)p * mpad;
+  for (int pass = 0; pass < npass; ++pass) {
+  const int ct = pass == 0 ? q : tiles_c - 1 - q;
+  const int c0 = ct * WT;
+  const int vj = min(4, max(0, (p - c0 - wn * 64 + 15) >> 4));
+  const int ccol = c0 + sr;
+  const int kend = upper ? min(m, c0 + WT) : m;
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{};
+  // rows past n and columns past p are read from the last row / column (their
+  // outputs are not stored); k past m reads zeros
+  const float* xp = X + min<int64_t>(xrow, n - 1) * ldx;
+  const __bf16* cp = S + (size_t)min(ccol, p - 1) * mpad;
+  float xr[16];
+  u4v_t cr[NS][2];
+  auto fetch = [&](int k0) {
+    const int kk = k0 + sk;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int kh = kk + 4 * h;
+      if (vec && kh + 4 <= m) {
+        const float4 v = *reinterpret_cast<const float4*>(xp + kh);
+        xr[4 * h] = v.x, xr[4 * h + 1] = v.y, xr[4 * h + 2] = v.z, xr[4 * h + 3] = v.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xr[4 * h + e] = kh + e < m ? xp[kh + e] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const u4v_t* src = reinterpret_cast<const u4v_t*>(cp + s * pstride + kk);
+      cr[s][0] = src[0];
+      cr[s][1] = src[1];
+    }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < kend; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint32_t o[4][NS];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) split_pair<NS>(xr[8 * h + 2 * e], xr[8 * h + 2 * e + 1], o[e]);
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        *reinterpret_cast<u4v_t*>(&Xl[s][sr][sk + 8 * h]) = u4v_t{o[0][s], o[1][s], o[2][s], o[3][s]};
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      *reinterpret_cast<u4v_t*>(&Cl[s][sr][sk]) = cr[s][0];
+      *reinterpret_cast<u4v_t*>(&Cl[s][sr][sk + 8]) = cr[s][1];
+    }
+    __syncthreads();
+    if (k0 + BK < kend) fetch(k0 + BK);
+    const int kb = 8 * (lane >> 4);
+    bf8_t af[4][NS];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) af[i][s] = *reinterpret_cast<const bf8_t*>(&Xl[s][wm * 64 + i * 16 + (lane & 15)][kb]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bf8_t bj[NS];  // one column block at a time (all four spilled at NS = 3)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) bj[s] = *reinterpret_cast<const bf8_t*>(&Cl[s][wn * 64 + j * 16 + (lane & 15)][kb]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (i < vi && j < vj) acc[i][j] = mfma_split<NS>(af[i], bj, acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = r0 + wm * 64 + i * 16 + 4 * (lane >> 4) + e;
+        const int col = c0 + wn * 64 + j * 16 + (lane & 15);
+        if (row < n && col < p) {
+          float& y = Y[row * ldy + col];
+          const float v = alpha * acc[i][j][e];
+          y = beta == 0.0f ? v : v + beta * y;
+        }
+      }
+  }  // pass (the next pass's first barrier orders its LDS writes after these reads)
 }
 
 // Fixed-order sum over chunks; mirrors the lower triangle for symmetric Grams.
@@ -571,7 +777,8 @@ static int gram_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, con
   FE_ARG(ctx && X && Y && G, me + ": null argument");
   FE_ARG(m >= 0 && p >= 0 && n >= 0 && ldx >= m && ldy >= p && ldg >= m, me + ": bad dimensions");
   FE_ARG(prec == FE_F64 || prec == FE_F32, me + ": bad precision");
-  FE_ARG(mode == FE_GRAM_NATIVE || (mode == FE_GRAM_BF16X3 && prec == FE_F32), me + ": bad mode");
+  FE_ARG(mode == FE_GRAM_NATIVE || ((mode == FE_GRAM_BF16X3 || mode == FE_GRAM_BF16X6) && prec == FE_F32),
+         me + ": bad mode");
   if (m == 0 || p == 0) return 0;
   const bool sym = (X == Y) && (m == p) && (ldx == ldy) && !w;
   const int symi = sym ? 1 : 0;
@@ -640,25 +847,30 @@ static int gram_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, con
     FE_CHECK_LAUNCH();
     return 0;
   }
-  const int tiles_a = (m + TILE - 1) / TILE, tiles_b = (p + TILE - 1) / TILE;
+  // compensated BF16 products: 256 x 256 tiles, one 512-thread workgroup per
+  // CU (two rounds), chunks of whole 32-row stages
+  const int tiles_a = (m + GT - 1) / GT, tiles_b = (p + GT - 1) / GT;
   const int ntiles = sym ? tiles_a * (tiles_a + 1) / 2 : tiles_a * tiles_b;
-  // split rows so that at most 4 workgroups per CU are live
-  int64_t chunks = std::max<int64_t>(1, 4 * cus / ntiles);
+  int64_t chunks = std::max<int64_t>(1, 2 * cus / ntiles);
   int64_t rows_per = (n + chunks - 1) / chunks;
-  if (rows_per < KCG) rows_per = KCG;
-  rows_per = (rows_per + KCG - 1) / KCG * KCG;
+  rows_per = (rows_per + BK - 1) / BK * BK;
   chunks = (n + rows_per - 1) / rows_per;
-  if (int rc = cap_chunks(chunks, rows_per, ntiles, TILE, KCG)) return rc;
+  if (int rc = cap_chunks(chunks, rows_per, ntiles, GT, BK)) return rc;
   void* ws = fixed;
   if (!ws)
-    if (int rc = fe::scratch(ctx, 0, sizeof(double) * (size_t)chunks * ntiles * TILE * TILE, &ws)) return rc;
-  double* part = (double*)ws;
-  const dim3 grid((unsigned)ntiles, (unsigned)chunks);
-  hipLaunchKernelGGL(k_gram_bf16x3, grid, dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
-                     (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, part);
+    if (int rc = fe::scratch(ctx, 0, sizeof(double) * (size_t)chunks * ntiles * GT * GT, &ws)) return rc;
+  const int64_t units = chunks * ntiles, nb8 = (units + 7) / 8 * 8;
+  const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && (reinterpret_cast<uintptr_t>(Y) % 16 == 0) &&
+                  ldx % 4 == 0 && ldy % 4 == 0;
+  if (mode == FE_GRAM_BF16X3)
+    hipLaunchKernelGGL(k_gram_bf16<2>, dim3((unsigned)nb8), dim3(512), 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                       (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec, (double*)ws);
+  else
+    hipLaunchKernelGGL(k_gram_bf16<3>, dim3((unsigned)nb8), dim3(512), 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                       (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec, (double*)ws);
   FE_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles, TILE * TILE / 64), dim3(256), 0, ctx->stream, m, p,
-                     tiles_b, ntiles, (int)chunks, part, G, ldg, symi, TILE);
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ntiles, GT * GT / 64), dim3(256), 0, ctx->stream, m, p, tiles_b,
+                     ntiles, (int)chunks, (const double*)ws, G, ldg, symi, GT);
   FE_CHECK_LAUNCH();
   return 0;
 }
@@ -681,8 +893,36 @@ extern "C" int fe_gram_static(fe_ctx ctx, int mode, int prec, int64_t n, int m, 
 
 namespace fe {
 
-static int gemm_run(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx, const double* C,
-                    int64_t ldc, void* Y, int64_t ldy, double alpha, double beta, int upper) {
+static int gemm_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                    const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta, int upper) {
+  if (mode == FE_GRAM_BF16X3 || mode == FE_GRAM_BF16X6) {
+    // C split once into its bfloat16 parts (scratch slot 4), then the product
+    const int NS = mode == FE_GRAM_BF16X3 ? 2 : 3;
+    const int mpad = (m + BK - 1) / BK * BK;
+    void* sp = nullptr;
+    if (int rc = fe::scratch(ctx, 4, sizeof(__bf16) * (size_t)NS * p * mpad, &sp)) return rc;
+    const unsigned sg = std::min<int64_t>(1024, ((int64_t)p * mpad / 2 + 255) / 256);
+    if (NS == 2)
+      hipLaunchKernelGGL(k_split_coeffs<2>, dim3(sg), dim3(256), 0, ctx->stream, m, p, C, ldc, upper, mpad, (__bf16*)sp);
+    else
+      hipLaunchKernelGGL(k_split_coeffs<3>, dim3(sg), dim3(256), 0, ctx->stream, m, p, C, ldc, upper, mpad, (__bf16*)sp);
+    FE_CHECK_LAUNCH();
+    const int tiles_c = (p + WT - 1) / WT;
+    const int64_t tiles_r = (n + WT - 1) / WT;
+    const int64_t ntiles = tiles_r * (upper ? (tiles_c + 1) / 2 : tiles_c);
+    const int64_t nb8 = (ntiles + 7) / 8 * 8;
+    const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && ldx % 4 == 0;
+    if (NS == 2)
+      hipLaunchKernelGGL(k_gemm_bf16<2>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                         (const __bf16*)sp, mpad, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles,
+                         vec, upper);
+    else
+      hipLaunchKernelGGL(k_gemm_bf16<3>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                         (const __bf16*)sp, mpad, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles,
+                         vec, upper);
+    FE_CHECK_LAUNCH();
+    return 0;
+  }
   // 256 x 64 tiles when the output has at most 64 columns, else 128 x 128
   const bool thin = p <= 64;
   const int TR = thin ? 256 : WT, TC = thin ? 64 : WT;
@@ -717,24 +957,39 @@ static int gemm_run(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X
 
 }  // namespace fe
 
-extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
-                          const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta) {
+extern "C" int fe_gemm_nn_mode(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X,
+                               int64_t ldx, const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha,
+                               double beta) {
   FE_ARG(ctx && X && C && Y, "fe_gemm_nn: null argument");
   FE_ARG(X != Y, "fe_gemm_nn: Y must not alias X");
   FE_ARG(n >= 0 && m >= 0 && p >= 0 && ldx >= m && ldy >= p && ldc >= m, "fe_gemm_nn: bad dimensions");
+  FE_ARG(mode == FE_GRAM_NATIVE || ((mode == FE_GRAM_BF16X3 || mode == FE_GRAM_BF16X6) && prec == FE_F32),
+         "fe_gemm_nn: bad mode");
   if (n == 0 || p == 0) return 0;
   if (m == 0) {
     if (beta == 0.0) return fe_zero2d(ctx, Y, n, p, ldy, (int)prec_bytes(prec));
     return fe_axpby_cols(ctx, prec, n, p, Y, ldy, Y, ldy, 0.0, beta, nullptr);
   }
-  return fe::gemm_run(ctx, prec, n, m, p, X, ldx, C, ldc, Y, ldy, alpha, beta, 0);
+  return fe::gemm_run(ctx, mode, prec, n, m, p, X, ldx, C, ldc, Y, ldy, alpha, beta, 0);
+}
+
+extern "C" int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                          const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta) {
+  return fe_gemm_nn_mode(ctx, FE_GRAM_NATIVE, prec, n, m, p, X, ldx, C, ldc, Y, ldy, alpha, beta);
+}
+
+extern "C" int fe_trmm_upper_mode(fe_ctx ctx, int mode, int prec, int64_t n, int m, const void* X, int64_t ldx,
+                                  const double* C, int64_t ldc, void* Y, int64_t ldy) {
+  FE_ARG(ctx && X && C && Y, "fe_trmm_upper: null argument");
+  FE_ARG(X != Y, "fe_trmm_upper: Y must not alias X");
+  FE_ARG(n >= 0 && m >= 0 && ldx >= m && ldy >= m && ldc >= m, "fe_trmm_upper: bad dimensions");
+  FE_ARG(mode == FE_GRAM_NATIVE || ((mode == FE_GRAM_BF16X3 || mode == FE_GRAM_BF16X6) && prec == FE_F32),
+         "fe_trmm_upper: bad mode");
+  if (n == 0 || m == 0) return 0;
+  return fe::gemm_run(ctx, mode, prec, n, m, m, X, ldx, C, ldc, Y, ldy, 1.0, 0.0, 1);
 }
 
 extern "C" int fe_trmm_upper(fe_ctx ctx, int prec, int64_t n, int m, const void* X, int64_t ldx, const double* C,
                              int64_t ldc, void* Y, int64_t ldy) {
-  FE_ARG(ctx && X && C && Y, "fe_trmm_upper: null argument");
-  FE_ARG(X != Y, "fe_trmm_upper: Y must not alias X");
-  FE_ARG(n >= 0 && m >= 0 && ldx >= m && ldy >= m && ldc >= m, "fe_trmm_upper: bad dimensions");
-  if (n == 0 || m == 0) return 0;
-  return fe::gemm_run(ctx, prec, n, m, m, X, ldx, C, ldc, Y, ldy, 1.0, 0.0, 1);
+  return fe_trmm_upper_mode(ctx, FE_GRAM_NATIVE, prec, n, m, X, ldx, C, ldc, Y, ldy);
 }

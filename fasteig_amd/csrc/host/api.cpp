@@ -354,4 +354,27 @@ int feh_problem_results_text(feh_problem p, char* buf, int len) {
   });
 }
 
+int feh_problem_product_tuning(feh_problem p, char* buf, int len) {
+  return guarded([&] {
+    if (!p || !buf || len <= 0) throw std::invalid_argument("null argument");
+    std::ostringstream os;
+    static const char* kinds[] = {"gram_sym", "gram", "gram_weighted", "gemm", "trmm"};
+    if (auto ex = p->prob->current_executor()) {
+      os.precision(6);
+      for (const auto& [key, t] : ex->tunings()) {
+        const int kind = std::get<0>(key);
+        os << (kind >= 0 && kind < 5 ? kinds[kind] : "?") << ' ' << std::get<1>(key) << ' ' << std::get<2>(key) << ' '
+           << std::get<3>(key);
+        for (int i = 0; i < 3; ++i) os << ' ' << t.ms[i];
+        for (int i = 0; i < 3; ++i) os << ' ' << t.err[i];
+        os << '\n';
+      }
+    }
+    const std::string s = os.str();
+    std::strncpy(buf, s.c_str(), static_cast<size_t>(len - 1));
+    buf[len - 1] = '\0';
+    return static_cast<int>(s.size());
+  });
+}
+
 }  // extern "C"

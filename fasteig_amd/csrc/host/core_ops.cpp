@@ -36,19 +36,19 @@ int cholesky_qr(Workspace& ws, Block& X, Block& tmp) {
   const int k = X.ncols();
   if (k <= 0 || X.rows() < k) throw std::invalid_argument("thin_qr_in_place: row count must be >= column count.");
   ws.ensure(k);
-  gram(ex, ws.gram_mode, X.ref(), X.ref(), ws.G.data(), ws.cap);
+  gram(ex, ws.gram_mode, X.ref(), X.ref(), ws.G.data(), ws.cap, nullptr, ws.need);
   const int info = fe_potrf(ex.ctx(), k, ws.G.data(), ws.cap);
   check(info, "fe_potrf");
   if (info != 0) return -1;
   check(fe_chol_rinv(ex.ctx(), k, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
   if (tmp.ncols() == k && tmp.rows() == X.rows() && tmp.prec() == X.prec()) {
-    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, tmp.ref());
+    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, tmp.ref(), ws.gram_mode, ws.need);
     X.swap(tmp);
   } else {
     const BlockRef t = tmp.cols(0, k);
     if (tmp.ncols() < k || tmp.rows() != X.rows() || tmp.prec() != X.prec())
       throw std::invalid_argument("cholesky_qr: scratch block too small");
-    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, t);
+    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, t, ws.gram_mode, ws.need);
     copy(ex, t, X.ref());
   }
   return 0;
@@ -62,7 +62,7 @@ int orthonormal_qr(Workspace& ws, Block& X, Block& tmp) {
     // shifted Cholesky QR: G + s I = R^T R with s = 11 (n k + k (k+1)) u ||X||_2^2
     shifted = 1;
     ws.ensure(k);
-    gram(ex, ws.gram_mode, X.ref(), X.ref(), ws.G.data(), ws.cap);
+    gram(ex, ws.gram_mode, X.ref(), X.ref(), ws.G.data(), ws.cap, nullptr, ws.need);
     MatrixXd Gh = ws.G.to_host();
     double tr = 0.0;
     for (int i = 0; i < k; ++i) tr += Gh(i, i);
@@ -76,7 +76,7 @@ int orthonormal_qr(Workspace& ws, Block& X, Block& tmp) {
     if (info != 0) throw std::runtime_error("orthonormal_qr: shifted Cholesky QR failed (non-finite block?)");
     check(fe_chol_rinv(ex.ctx(), k, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
     const BlockRef t = tmp.cols(0, k);
-    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, t);
+    trmm_upper(ex, X.ref(), ws.R.data(), ws.cap, t, ws.gram_mode, ws.need);
     copy(ex, t, X.ref());
     if (cholesky_qr(ws, X, tmp) != 0) throw std::runtime_error("orthonormal_qr: Cholesky QR failed after shift");
   }
@@ -90,9 +90,9 @@ void block_gram_schmidt(Workspace& ws, const BlockRef& basis, const BlockRef& S,
   const int a = basis.cols, b = S.cols;
   if (a == 0 || b == 0) return;
   ws.ensure(std::max(a, b));
-  gram(ex, ws.gram_mode, basis, out, ws.G.data(), ws.cap);
+  gram(ex, ws.gram_mode, basis, out, ws.G.data(), ws.cap, nullptr, ws.need);
   gemm(ex, basis, ws.G.data(), ws.cap, out, -1.0, 1.0);
-  gram(ex, ws.gram_mode, basis, out, ws.R.data(), ws.cap);
+  gram(ex, ws.gram_mode, basis, out, ws.R.data(), ws.cap, nullptr, ws.need);
   gemm(ex, basis, ws.R.data(), ws.cap, out, -1.0, 1.0);
 }
 
@@ -107,7 +107,7 @@ Block svqb_level(Workspace& ws, const BlockRef& cur, double cut, int capacity, d
   HipExecutor& ex = ws.ex;
   const int c = cur.cols;
   ws.ensure(c);
-  gram(ex, ws.gram_mode, cur, cur, ws.G.data(), ws.cap);
+  gram(ex, ws.gram_mode, cur, cur, ws.G.data(), ws.cap, nullptr, ws.need);
   check(fe_syev(ex.ctx(), c, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap), "fe_syev");
   std::vector<double> w(c);
   check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * c), "d2h");
@@ -126,7 +126,7 @@ Block svqb_level(Workspace& ws, const BlockRef& cur, double cut, int capacity, d
   }
   check(fe_memcpy_h2d(ex.ctx(), ws.C.data(), Cm.data(), sizeof(double) * Cm.size()), "h2d");
   Block q(ex, cur.rows, r, cur.prec);
-  gemm(ex, cur, ws.C.data(), ws.cap, q.ref(), 1.0, 0.0);
+  gemm(ex, cur, ws.C.data(), ws.cap, q.ref(), 1.0, 0.0, ws.gram_mode, ws.need);
   return q;
 }
 }  // namespace
@@ -231,8 +231,8 @@ RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& 
   idx.resize(requested);
   check(fe_gather_cols(ex.ctx(), m, requested, ws.V.data(), ws.cap, idx.data(), ws.C.data(), ws.cap),
         "fe_gather_cols");
-  gemm(ex, Q.range(0, m), ws.C.data(), ws.cap, Y, 1.0, 0.0);
-  gemm(ex, AQ.range(0, m), ws.C.data(), ws.cap, AY, 1.0, 0.0);  // C's first nr columns
+  gemm(ex, Q.range(0, m), ws.C.data(), ws.cap, Y, 1.0, 0.0, ws.gram_mode, ws.need);
+  gemm(ex, AQ.range(0, m), ws.C.data(), ws.cap, AY, 1.0, 0.0, ws.gram_mode, ws.need);  // C's first nr columns
   RitzPairs out;
   out.selected = idx;
   out.values.resize(requested);

@@ -17,7 +17,11 @@ namespace hip {
 // src/gpu/CudaMultiVector.cpp:347,389).
 struct Workspace {
   HipExecutor& ex;
+  // product mode of FP32 blocks (fe_gram_mode or kProductAuto) and, for
+  // kProductAuto, the relative product error the current step tolerates
+  // (0: native products only)
   int gram_mode = FE_GRAM_NATIVE;
+  double need = 0.0;
   Small G, R, W, V, C, theta, rn, yn;  // sized for the largest projected problem
   int cap = 0;
   Workspace(HipExecutor& e, int max_m, int gram);

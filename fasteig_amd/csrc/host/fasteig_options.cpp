@@ -85,7 +85,8 @@ bool fasteig_options::set(std::string_view key, std::string_view value) {
     precision_ = v;
   } else if (k == "gram") {
     const std::string v = lower(value);
-    if (v != "native" && v != "bf16x3") throw std::invalid_argument("gram must be 'native' or 'bf16x3'");
+    if (v != "native" && v != "bf16x3" && v != "bf16x6" && v != "auto")
+      throw std::invalid_argument("gram must be 'native', 'bf16x3', 'bf16x6' or 'auto'");
     gram_ = v;
   } else if (k == "format") {
     const std::string v = lower(value);
@@ -186,6 +187,6 @@ void fasteig_options::validate() const {
   if (blksize_ <= 0) throw std::invalid_argument("options.blksize must be > 0");
   if (verbosity_ < 1 || verbosity_ > 2) throw std::invalid_argument("options.verbosity must be 1 or 2");
   if (print_every_ < 0) throw std::invalid_argument("options.print_every must be >= 0");
-  if (gram_ == "bf16x3" && precision_ != "mixed")
-    throw std::invalid_argument("options.gram=bf16x3 requires precision=mixed");
+  if ((gram_ == "bf16x3" || gram_ == "bf16x6") && precision_ != "mixed")
+    throw std::invalid_argument("options.gram=" + gram_ + " requires precision=mixed");
 }

@@ -93,7 +93,7 @@ class fasteig_options {
   FASTEIG_INT print_every_ = 0;
   bool gpu_backend_ = true;
   std::string precision_ = "fp64";
-  std::string gram_ = "native";
+  std::string gram_ = "auto";  // product mode of FP32 blocks (mixed precision)
   std::string format_ = "auto";
   std::string cheb_bounds_ = "amplification";
   int device_ = 0;

@@ -73,6 +73,9 @@ class fasteig_problem {
   void set_log_stream(std::ostream* os) noexcept { log_ = os; }
   void write_results_text(std::ostream& os) const;
 
+  // the executor of the last solve (null before the first)
+  std::shared_ptr<fasteig::hip::HipExecutor> current_executor() const { return executor_; }
+
  private:
   void build_from_triplets(FASTEIG_INT n, const std::vector<int>& r, const std::vector<int>& c,
                            const std::vector<double>& v, bool symmetric_triangle, CsrMatrixData& out);

@@ -1,6 +1,10 @@
 // fasteig::hip host classes over the libfe_hip.so C ABI.
 #include "hip_backend.hpp"
 
+#include <cmath>
+#include <functional>
+#include <limits>
+
 #include <random>
 #include <stdexcept>
 
@@ -184,23 +188,153 @@ void spmm(HipExecutor& ex, const HipCsrMatrix& A, const BlockRef& X, const Block
         "fe_spmm");
 }
 
-void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, double* G, int64_t ldg, const double* w) {
+namespace {
+
+constexpr int kModes[3] = {FE_GRAM_NATIVE, FE_GRAM_BF16X3, FE_GRAM_BF16X6};
+enum ProductKind { kGramSym = 0, kGram = 1, kGramW = 2, kGemm = 3, kTrmm = 4 };
+constexpr int kTuneEvent0 = 14, kTuneEvent1 = 15;  // context event slots used by the tuning
+constexpr int64_t kTuneRowsGram = 65536;            // row sample of a Gram's error
+constexpr int64_t kTuneRowsTall = 8192;             // row sample of a tall product's error
+
+// Time of one call (after one untimed call that absorbs first-use costs).
+float time_call(HipExecutor& ex, const std::function<void()>& f) {
+  f();
+  check(fe_event_record(ex.ctx(), kTuneEvent0), "fe_event_record");
+  f();
+  check(fe_event_record(ex.ctx(), kTuneEvent1), "fe_event_record");
+  float ms = 0.0f;
+  check(fe_event_elapsed_ms(ex.ctx(), kTuneEvent0, kTuneEvent1, &ms), "fe_event_elapsed_ms");
+  return ms;
+}
+
+template <typename T>
+double max_rel_diff(const std::vector<T>& a, const std::vector<double>& ref) {
+  double num = 0.0, den = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    const double d = std::fabs(static_cast<double>(a[i]) - ref[i]);
+    num = std::isfinite(d) ? std::max(num, d) : std::numeric_limits<double>::infinity();
+    den = std::max(den, std::fabs(ref[i]));
+  }
+  return den > 0.0 ? num / den : num;
+}
+
+// The fastest mode whose measured error (against the FP64 product of the
+// same FP32 data) is within need, or within twice the native product's own
+// error: a mode as accurate as native FP32 qualifies for any need that
+// native FP32 products are used for.
+int choose(const HipExecutor::ProductTuning& t, double need) {
+  int best = 0;
+  for (int i = 1; i < 3; ++i)
+    if ((t.err[i] <= need || t.err[i] <= 2.0 * t.err[0]) && t.ms[i] < t.ms[best]) best = i;
+  return kModes[best];
+}
+
+// FP64 copy of the first rows of an FP32 block (the tuning reference)
+Block fp64_rows(HipExecutor& ex, const BlockRef& X, int64_t rows) {
+  Block b(ex, rows, X.cols, FE_F64);
+  BlockRef src = X;
+  src.rows = rows;
+  check(fe_copy2d(ex.ctx(), rows, X.cols, src.ptr, src.prec, src.ld, b.data(), FE_F64, b.ld()), "fe_copy2d");
+  return b;
+}
+
+}  // namespace
+
+void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, double* G, int64_t ldg, const double* w,
+          double need) {
   if (X.prec != Y.prec || X.rows != Y.rows) throw std::invalid_argument("gram: operand mismatch");
-  check(fe_gram(ex.ctx(), X.prec == FE_F64 ? FE_GRAM_NATIVE : mode, X.prec, X.rows, X.cols, Y.cols, X.ptr, X.ld,
-                Y.ptr, Y.ld, w, G, ldg),
-        "fe_gram");
+  auto run = [&](int md, const BlockRef& x, const BlockRef& y) {
+    check(fe_gram(ex.ctx(), md, x.prec, x.rows, x.cols, y.cols, x.ptr, x.ld, y.ptr, y.ld, w, G, ldg), "fe_gram");
+  };
+  if (X.prec == FE_F64 || mode == FE_GRAM_NATIVE || (mode == kProductAuto && !(need > 0.0)))
+    return run(FE_GRAM_NATIVE, X, Y);
+  if (mode != kProductAuto) return run(mode, X, Y);
+  const bool same = X.ptr == Y.ptr && X.cols == Y.cols && X.ld == Y.ld;
+  const int kind = w ? kGramW : same ? kGramSym : kGram;
+  const HipExecutor::TuningKey key{kind, X.rows, X.cols, Y.cols};
+  const HipExecutor::ProductTuning* t = ex.tuning(key);
+  if (!t) {
+    HipExecutor::ProductTuning nt;
+    const int64_t ns = std::min<int64_t>(X.rows, kTuneRowsGram);
+    BlockRef xs = X, ys = Y;
+    xs.rows = ys.rows = ns;
+    auto download = [&] {
+      std::vector<double> g(static_cast<size_t>(X.cols) * Y.cols);
+      for (int c = 0; c < Y.cols; ++c)
+        check(fe_memcpy_d2h(ex.ctx(), g.data() + static_cast<size_t>(c) * X.cols, G + static_cast<size_t>(c) * ldg,
+                            sizeof(double) * X.cols),
+              "d2h(gram)");
+      return g;
+    };
+    std::vector<double> ref;
+    {
+      Block x64 = fp64_rows(ex, X, ns);
+      Block y64 = same ? Block() : fp64_rows(ex, Y, ns);
+      run(FE_GRAM_NATIVE, x64.ref(), same ? x64.ref() : y64.ref());
+      ref = download();
+    }
+    for (int i = 0; i < 3; ++i) {
+      run(kModes[i], xs, ys);
+      nt.err[i] = max_rel_diff(download(), ref);
+      nt.ms[i] = time_call(ex, [&] { run(kModes[i], X, Y); });
+    }
+    t = &ex.set_tuning(key, nt);
+  }
+  run(choose(*t, need), X, Y);
+}
+
+// the tall products Y = X C: tuning as for the Gram, the error from a row
+// sample against the FP64 product (run(mode, x, y) with an FP64 x and y for the
+// reference)
+static void tall_product(HipExecutor& ex, int kind, int mode, double need, const BlockRef& X, const BlockRef& Y,
+                         const std::function<void(int, const BlockRef&, const BlockRef&)>& run) {
+  if (X.prec == FE_F64 || mode == FE_GRAM_NATIVE || (mode == kProductAuto && !(need > 0.0)))
+    return run(FE_GRAM_NATIVE, X, Y);
+  if (mode != kProductAuto) return run(mode, X, Y);
+  const HipExecutor::TuningKey key{kind, X.rows, X.cols, Y.cols};
+  const HipExecutor::ProductTuning* t = ex.tuning(key);
+  if (!t) {
+    HipExecutor::ProductTuning nt;
+    const int64_t ns = std::min<int64_t>(X.rows, kTuneRowsTall);
+    std::vector<double> ref(static_cast<size_t>(ns) * Y.cols);
+    {
+      Block x64 = fp64_rows(ex, X, ns), y64(ex, ns, Y.cols, FE_F64);
+      run(FE_GRAM_NATIVE, x64.ref(), y64.ref());
+      check(fe_memcpy_d2h(ex.ctx(), ref.data(), y64.data(), sizeof(double) * ref.size()), "d2h(sample)");
+    }
+    Block sample(ex, ns, Y.cols, Y.prec);
+    BlockRef xs = X;
+    xs.rows = ns;
+    std::vector<float> got(ref.size());
+    for (int i = 0; i < 3; ++i) {
+      run(kModes[i], xs, sample.ref());
+      check(fe_memcpy_d2h(ex.ctx(), got.data(), sample.data(), sizeof(float) * got.size()), "d2h(sample)");
+      nt.err[i] = max_rel_diff(got, ref);
+      nt.ms[i] = time_call(ex, [&] { run(kModes[i], X, Y); });
+    }
+    t = &ex.set_tuning(key, nt);
+  }
+  run(choose(*t, need), X, Y);
 }
 
 void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, double alpha,
-          double beta) {
+          double beta, int mode, double need) {
   if (X.prec != Y.prec || X.rows != Y.rows) throw std::invalid_argument("gemm: operand mismatch");
-  check(fe_gemm_nn(ex.ctx(), X.prec, X.rows, X.cols, Y.cols, X.ptr, X.ld, C, ldc, Y.ptr, Y.ld, alpha, beta),
-        "fe_gemm_nn");
+  if (beta != 0.0) mode = FE_GRAM_NATIVE;  // a tuning run would overwrite the Y it reads
+  tall_product(ex, kGemm, mode, need, X, Y, [&](int md, const BlockRef& x, const BlockRef& y) {
+    check(fe_gemm_nn_mode(ex.ctx(), md, x.prec, x.rows, x.cols, y.cols, x.ptr, x.ld, C, ldc, y.ptr, y.ld, alpha,
+                          beta),
+          "fe_gemm_nn");
+  });
 }
 
-void trmm_upper(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y) {
+void trmm_upper(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, int mode,
+                double need) {
   if (X.prec != Y.prec || X.rows != Y.rows || X.cols != Y.cols) throw std::invalid_argument("trmm_upper: operand mismatch");
-  check(fe_trmm_upper(ex.ctx(), X.prec, X.rows, X.cols, X.ptr, X.ld, C, ldc, Y.ptr, Y.ld), "fe_trmm_upper");
+  tall_product(ex, kTrmm, mode, need, X, Y, [&](int md, const BlockRef& x, const BlockRef& y) {
+    check(fe_trmm_upper_mode(ex.ctx(), md, x.prec, x.rows, x.cols, x.ptr, x.ld, C, ldc, y.ptr, y.ld),
+          "fe_trmm_upper");
+  });
 }
 
 void copy(HipExecutor& ex, const BlockRef& src, const BlockRef& dst) {

@@ -152,25 +152,51 @@ class HipExecutor {
   // Drop cached start blocks (frees HBM).
   void clear_cache();
 
+  // Measured time (ms) and relative error against the FP64 product of the
+  // three product modes (native, BF16x3, BF16x6) for one product shape.
+  struct ProductTuning {
+    float ms[3] = {0.0f, 0.0f, 0.0f};
+    double err[3] = {0.0, 0.0, 0.0};
+  };
+  using TuningKey = std::tuple<int, int64_t, int, int>;  // kind, rows, m, p
+  const ProductTuning* tuning(const TuningKey& key) const {
+    auto it = tuning_.find(key);
+    return it == tuning_.end() ? nullptr : &it->second;
+  }
+  const ProductTuning& set_tuning(const TuningKey& key, const ProductTuning& t) { return tuning_[key] = t; }
+  const std::map<TuningKey, ProductTuning>& tunings() const { return tuning_; }
+
  private:
   fe_ctx ctx_ = nullptr;
   int device_ = 0;
   std::map<std::tuple<int64_t, int, unsigned>, Block> random_cache_;
+  std::map<TuningKey, ProductTuning> tuning_;
 };
 
 // ---------------------------------------------------------------- primitives
 // Y = alpha A X + beta X + gamma Z (Z may alias Y)
 void spmm(HipExecutor& ex, const HipCsrMatrix& A, const BlockRef& X, const BlockRef& Y, double alpha,
           double beta = 0.0, const BlockRef* Z = nullptr, double gamma = 0.0);
+// Product modes of FP32 blocks (fe_gram_mode), plus kProductAuto: the mode
+// is chosen per product shape by measurement. On the first product of a
+// shape the three modes run on the actual operands, each timed at full size
+// (HIP events on the context stream) and its error measured on a row sample
+// (65,536 rows of a Gram, 8,192 of a tall product) against the FP64 product
+// of the same FP32 data. A call then takes the fastest mode whose error is at
+// most `need` or at most twice the native FP32 product's (need <= 0: native
+// only). FP64 blocks always use the native product.
+constexpr int kProductAuto = -1;
 // G (m x p, ldg) = X^T Y
 void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, double* G, int64_t ldg,
-          const double* w = nullptr);
+          const double* w = nullptr, double need = 0.0);
 // Y = X C with C upper triangular (m x m, FP64 column-major; nothing below
 // its diagonal is read): the X R^{-1} of Cholesky QR
-void trmm_upper(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y);
-// Y = alpha X C + beta Y, C FP64 column-major (m x p)
+void trmm_upper(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y,
+                int mode = FE_GRAM_NATIVE, double need = 0.0);
+// Y = alpha X C + beta Y, C FP64 column-major (m x p); the compensated modes
+// and their tuning apply when beta == 0 (a tuning run overwrites Y)
 void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, double alpha,
-          double beta);
+          double beta, int mode = FE_GRAM_NATIVE, double need = 0.0);
 void copy(HipExecutor& ex, const BlockRef& src, const BlockRef& dst);
 // dst[:, j] = src[:, idx[j]] in one pass (same precision)
 void gather_cols(HipExecutor& ex, const BlockRef& src, const std::vector<int>& idx, const BlockRef& dst);

@@ -102,7 +102,9 @@ FASTEIG_INT ChebyshevSolver::solve() {
                                                 : nev + static_cast<int>(std::min<int64_t>(nev, n - nev));
   const bool mixed = options_.mixed();
   int prec = mixed ? FE_F32 : FE_F64;
-  const int gram_mode = options_.gram() == "bf16x3" ? FE_GRAM_BF16X3 : FE_GRAM_NATIVE;
+  const std::string& gopt = options_.gram();
+  const int gram_mode = gopt == "bf16x3" ? FE_GRAM_BF16X3 : gopt == "bf16x6" ? FE_GRAM_BF16X6
+                        : gopt == "auto" ? hip::kProductAuto : FE_GRAM_NATIVE;
   Workspace ws(ex, capacity, mixed ? gram_mode : FE_GRAM_NATIVE);
 
   // persistent blocks: V (subspace), F/P (filter recurrence), Q/AQ (basis), AY
@@ -133,6 +135,11 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // columns then keep their FP32-phase vectors and stay in the Rayleigh-Ritz
     // subspace
     const int active = (mixed && prec == FE_F64 && refine_wanted) ? std::max(0, nev - locked) : subspace_cols - locked;
+    // accuracy the FP32 products of this cycle need (automatic product mode):
+    // while the previous cycle's residual is at least 1e-2 (or unknown) the
+    // cycle cannot resolve errors of 1e-4 and compensated BF16x3 products
+    // (about 16 bits) serve; below it FP32-class products (1e-6)
+    ws.need = prec == FE_F32 ? ((std::isfinite(last_mx) && last_mx < 1e-2) ? 1e-6 : 1e-4) : 0.0;
     // 1. filter the active columns (ChebyshevSolver.cpp:151-155, 223-258)
     if (active > 0 && upper != lower) {
       BlockRef prev = F.cols(0, active), cur = P.cols(0, active);
@@ -194,17 +201,24 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // is retried with a shift (shifted CholQR, Fukaya et al. 2020) and two
     // more passes
     int passes = (prec == FE_F32 || cycle_degree > degree) ? 2 : 1;
+    const double cycle_need = ws.need;
     for (int pass = 0; pass < passes; ++pass) {
       ws.ensure(subspace_cols);
+      // the first pass factors the filtered block, whose condition number
+      // kappa sets the accuracy it needs (a Gram error e fails the Cholesky
+      // once kappa^2 e ~ 1, and X R^{-1} moves the span by about e kappa):
+      // FP32-class products; later passes work on a nearly orthonormal block
+      // and take the cycle's accuracy
+      ws.need = pass == 0 ? std::min(cycle_need, 1e-6) : cycle_need;
       // pass 0 takes V into Q; a later pass takes Q into AY (scratch here)
       // and swaps the two
       const BlockRef Vb = pass == 0 ? V.cols(0, subspace_cols) : Q.cols(0, subspace_cols);
       const BlockRef Qout = pass == 0 ? Q.cols(0, subspace_cols) : AY.cols(0, subspace_cols);
-      hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap);
+      hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap, nullptr, ws.need);
       int info = fe_potrf(ex.ctx(), subspace_cols, ws.G.data(), ws.cap);
       hip::check(info, "fe_potrf");
       if (info != 0 && passes < 4) {
-        hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap);
+        hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap, nullptr, ws.need);
         MatrixXd Gh = ws.G.to_host();
         double tr = 0.0;
         for (int i = 0; i < subspace_cols; ++i) tr += Gh(i, i);
@@ -224,12 +238,14 @@ FASTEIG_INT ChebyshevSolver::solve() {
                                  (prec == FE_F32 ? "fp32" : "fp64") + ", degree " + std::to_string(cycle_degree) +
                                  ", column " + std::to_string(info) + ").");
       hip::check(fe_chol_rinv(ex.ctx(), subspace_cols, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
-      hip::trmm_upper(ex, Vb, ws.R.data(), ws.cap, Qout);
+      hip::trmm_upper(ex, Vb, ws.R.data(), ws.cap, Qout, ws.gram_mode, ws.need);
       if (pass > 0) Q.swap(AY);
     }
+    ws.need = cycle_need;
     // 3. Rayleigh-Ritz (ChebyshevSolver.cpp:175-183); Ritz vectors go straight into V
     hip::spmm(ex, A, Q.cols(0, subspace_cols), AQ.cols(0, subspace_cols), sign);
-    hip::gram(ex, ws.gram_mode, Q.cols(0, subspace_cols), AQ.cols(0, subspace_cols), ws.G.data(), ws.cap);
+    hip::gram(ex, ws.gram_mode, Q.cols(0, subspace_cols), AQ.cols(0, subspace_cols), ws.G.data(), ws.cap, nullptr,
+              ws.need);
     // residuals of the wanted pairs only: AQ C for their nev columns
     ritz = hip::extract_ritz(ws, subspace_cols, Q.ref(), AQ.ref(), subspace_cols, "sr", V.cols(0, subspace_cols),
                              AY.cols(0, nev), nullptr, nev);

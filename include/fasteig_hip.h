@@ -49,6 +49,13 @@ int feh_problem_results(feh_problem p, double* eigenvalues, double* residual_nor
 int feh_problem_eigenvectors(feh_problem p, double* out);
 /* RAW RESULTS / DIAGNOSTICS text of the last solve into buf (truncated). */
 int feh_problem_results_text(feh_problem p, char* buf, int len);
+/* The product-mode measurements of the problem's executor (options
+ * gram=auto; no counterpart in the reference, whose products are cuBLAS
+ * calls): one line per tuned product shape, "kind rows m p" then the time
+ * (ms) of the native, BF16x3 and BF16x6 modes and their errors relative to the
+ * FP64 product of the same FP32 data on a row sample. Returns the text
+ * length; truncated to len - 1. */
+int feh_problem_product_tuning(feh_problem p, char* buf, int len);
 
 /* ---- core numerics of the block solvers on the problem's device (the
  * reference's L4 free functions, which the solvers call between SpMMs).

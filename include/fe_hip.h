@@ -46,7 +46,7 @@ typedef struct fe_ctx_s* fe_ctx;
 typedef struct fe_csr_s* fe_csr;
 
 enum fe_prec { FE_F64 = 0, FE_F32 = 1 };
-enum fe_gram_mode { FE_GRAM_NATIVE = 0, FE_GRAM_BF16X3 = 1 };
+enum fe_gram_mode { FE_GRAM_NATIVE = 0, FE_GRAM_BF16X3 = 1, FE_GRAM_BF16X6 = 2 };
 enum fe_format { FE_FMT_CSR = 0, FE_FMT_SELL = 1, FE_FMT_BLOCKED = 2, FE_FMT_TILED = 3 };
 
 /* ---- errors / version ------------------------------------------------- */
@@ -140,11 +140,13 @@ int fe_spmm(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, v
  *   G[m x p] = X[:, 0:m]^T * diag(w) * Y[:, 0:p]     (w may be NULL)
  * G is FP64 column-major with leading dimension ldg, on the device. mode
  * FE_GRAM_NATIVE uses FP64 MFMA for FE_F64 and FP32 MFMA (FP32 accumulate per
- * row chunk, FP64 across chunks) for FE_F32; FE_GRAM_BF16X3 (FE_F32 only)
- * splits each FP32 operand into a high and a low bfloat16 part and sums the
- * three products hi*hi + hi*lo + lo*hi on BF16 MFMA with FP32 accumulation
- * (about 16 significant bits per product; a throughput option for the
- * intermediate filter cycles). If X == Y and m == p only the upper triangle of tiles is computed
+ * row chunk, FP64 across chunks) for FE_F32. The compensated modes (FE_F32
+ * only) split each FP32 operand into bfloat16 parts x = x0 + x1 (+ x2) and sum
+ * part products on BF16 MFMA with FP32 accumulation: FE_GRAM_BF16X3 the three
+ * products x0 y0 + x0 y1 + x1 y0 (about 16 significant bits per product),
+ * FE_GRAM_BF16X6 the six with part indices summing to at most 2 (within a few
+ * units of the FP32 rounding of a product). The same modes apply to
+ * fe_gemm_nn_mode and fe_trmm_upper_mode. If X == Y and m == p only the upper triangle of tiles is computed
  * and mirrored. The chunk reduction order is fixed, so results are bitwise
  * reproducible. */
 int fe_gram(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
@@ -175,6 +177,11 @@ int fe_gram_static(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, cons
  * block precision inside the kernel. Y must not alias X. */
 int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
                const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta);
+/* The same with a product mode (enum fe_gram_mode): the compensated BF16 modes
+ * (FE_F32 only) split C once per call into its bfloat16 parts in the context's
+ * scratch (so, unlike fe_gemm_nn, they may allocate). */
+int fe_gemm_nn_mode(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                    const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta);
 
 /* Y[:, 0:m] = X[:, 0:m] * C with C upper triangular m x m (FP64 column-major,
  * ldc; nothing below its diagonal is read): the X R^{-1} of Cholesky QR, whose
@@ -184,6 +191,9 @@ int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int
  * fe_gemm_nn at m = 512). Y must not alias X. */
 int fe_trmm_upper(fe_ctx ctx, int prec, int64_t n, int m, const void* X, int64_t ldx, const double* C,
                   int64_t ldc, void* Y, int64_t ldy);
+/* The same with a product mode, as fe_gemm_nn_mode. */
+int fe_trmm_upper_mode(fe_ctx ctx, int mode, int prec, int64_t n, int m, const void* X, int64_t ldx,
+                       const double* C, int64_t ldc, void* Y, int64_t ldy);
 
 /* ---- small dense FP64 on the device -----------------------------------------
  * Cholesky G = L L^T of the leading k x k block (lower triangle written, upper

@@ -3,7 +3,8 @@
 Every test calls libfe_hip.so through its C ABI (fasteig_amd._fe) on seeded
 inputs and compares with the oracle (oracle/oracle.cpp) or with numpy closed
 forms. Tolerances: FP64 kernels 1e-13 relative (different summation order
-only); FP32 kernels 1e-5 relative to the operand scale; BF16x3 Gram 2e-4.
+only); FP32 kernels 1e-5 relative to the operand scale; compensated BF16
+products 2^-14 (BF16x3) and twice the native FP32 error (BF16x6).
 """
 import ctypes
 
@@ -440,8 +441,100 @@ def test_gram_bf16x3(dev):
     Gd = dev.buffer((m, m), np.float64, order="F")
     _fe.check(L().fe_gram(dev.ctx, _fe.GRAM_BF16X3, _fe.F32, n, m, m, Xd.ptr, m, Xd.ptr, m, None, Gd.ptr, m))
     ref = X.astype(np.float64).T @ X.astype(np.float64)
-    # hi*hi + hi*lo + lo*hi keeps ~16 significant bits per product
-    assert rel(Gd.download(), ref) < 2e-4
+    # x0 y0 + x0 y1 + x1 y0 keeps about 16 significant bits per product
+    assert rel(Gd.download(), ref) < 2.0 ** -14
+
+
+@pytest.mark.parametrize("n,m,p", [(50000, 128, 128), (40000, 512, 512), (3001, 65, 64), (4000, 161, 150),
+                                   (777, 5, 3), (200003, 192, 48), (31, 300, 260)])
+def test_gram_bf16_modes(dev, n, m, p):
+    """The compensated Gram products against the FP64 product of the same FP32
+    inputs: BF16x3 to about 16 bits; BF16x6 to FP32 accuracy (within twice the
+    error of the native FP32 MFMA product). Weighted, ragged and symmetric
+    cases; the symmetric output is exactly symmetric."""
+    rng = np.random.default_rng(n + 7 * m)
+    X = rng.uniform(-1, 1, (n, m))
+    Y = X[:, :p] * 0.5 + rng.uniform(-1, 1, (n, p)) * 0.5
+    w = rng.uniform(0.5, 1.5, n)
+    Xd, Yd, wd = dev.put(X, np.float32), dev.put(Y, np.float32), dev.put(w)
+    Xr = X.astype(np.float32).astype(np.float64)
+    Yr = Y.astype(np.float32).astype(np.float64)
+    ref = Xr.T @ (w[:, None] * Yr)
+    sref = Xr.T @ Xr
+
+    def run(mode, sym):
+        Gd = dev.buffer((m, m if sym else p), np.float64, order="F")
+        if sym:
+            _fe.check(L().fe_gram(dev.ctx, mode, _fe.F32, n, m, m, Xd.ptr, m, Xd.ptr, m, None, Gd.ptr, m))
+        else:
+            _fe.check(L().fe_gram(dev.ctx, mode, _fe.F32, n, m, p, Xd.ptr, m, Yd.ptr, p, wd.ptr, Gd.ptr, m))
+        return Gd.download()
+
+    # BF16x6: FP32-class, within twice the native FP32 error or 2^-21 relative
+    # (on sums of positive terms, the diagonal of X^T X over ~600-row chunks,
+    # the BF16 MFMA's internal sums lose up to 4x more than the FP32 MFMA's;
+    # the FP32 worst-case bound there is 600 u = 3.6e-5)
+    e32, s32 = rel(run(_fe.GRAM_NATIVE, False), ref), rel(run(_fe.GRAM_NATIVE, True), sref)
+    for mode in (_fe.GRAM_BF16X3, _fe.GRAM_BF16X6):
+        G, S = run(mode, False), run(mode, True)
+        assert np.array_equal(S, S.T)
+        if mode == _fe.GRAM_BF16X3:
+            assert rel(G, ref) < 2.0 ** -14 and rel(S, sref) < 2.0 ** -14, (rel(G, ref), rel(S, sref))
+        else:
+            assert rel(G, ref) <= max(2 * e32, 2.0 ** -21), (rel(G, ref), e32)
+            assert rel(S, sref) <= max(2 * s32, 2.0 ** -21), (rel(S, sref), s32)
+
+
+@pytest.mark.parametrize("n,m,p", [(20000, 512, 512), (100000, 256, 256), (3333, 200, 70), (999, 130, 130),
+                                   (7, 3, 2), (1001, 48, 64), (300, 33, 1)])
+def test_gemm_bf16_modes(dev, n, m, p):
+    """Y = alpha X C + beta Y in the compensated modes (C rounded to FP32 and
+    split), against FP64 on the same rounded inputs; and the triangular
+    product, whose C holds NaN below the diagonal that must not be read."""
+    rng = np.random.default_rng(n + p)
+    X = rng.uniform(-1, 1, (n, m))
+    Cm = rng.uniform(-1, 1, (m, p))
+    Y0 = rng.uniform(-1, 1, (n, p))
+    Xd = dev.put(X, np.float32)
+    Cd = dev.put(Cm, np.float64, order="F")
+    Xr = X.astype(np.float32).astype(np.float64)
+    Cr = Cm.astype(np.float32).astype(np.float64)
+    ref = 0.7 * Xr @ Cr - 0.5 * Y0.astype(np.float32).astype(np.float64)
+
+    def gemm(mode):
+        Yd = dev.put(Y0, np.float32)
+        _fe.check(L().fe_gemm_nn_mode(dev.ctx, mode, _fe.F32, n, m, p, Xd.ptr, m, Cd.ptr, m, Yd.ptr, p, 0.7, -0.5))
+        return Yd.download().astype(np.float64)
+
+    e32 = np.max(np.abs(gemm(_fe.GRAM_NATIVE) - ref))
+    scale = np.max(np.abs(Xr) @ np.abs(Cr))
+    assert np.max(np.abs(gemm(_fe.GRAM_BF16X3) - ref)) <= 2.0 ** -14 * scale
+    assert np.max(np.abs(gemm(_fe.GRAM_BF16X6) - ref)) <= 2 * e32 + 1e-7 * scale
+    if m == p:
+        Cg = Cm.copy()
+        Cg[np.tril_indices(m, -1)] = np.nan
+        Ct = dev.put(Cg, np.float64, order="F")
+        tref = Xr @ np.triu(Cr)
+
+        def trmm(mode):
+            Yd = dev.buffer((n, m), np.float32)
+            _fe.check(L().fe_trmm_upper_mode(dev.ctx, mode, _fe.F32, n, m, Xd.ptr, m, Ct.ptr, m, Yd.ptr, m))
+            return Yd.download().astype(np.float64)
+
+        t32 = np.max(np.abs(trmm(_fe.GRAM_NATIVE) - tref))
+        for mode, bound in ((_fe.GRAM_BF16X3, 2.0 ** -14 * scale), (_fe.GRAM_BF16X6, 2 * t32 + 1e-7 * scale)):
+            out = trmm(mode)
+            assert np.all(np.isfinite(out)) and np.max(np.abs(out - tref)) <= bound, mode
+
+
+def test_bf16_modes_reject_fp64(dev):
+    Xd = dev.put(np.ones((10, 4)))
+    Cd = dev.put(np.eye(4), np.float64, order="F")
+    Yd = dev.buffer((10, 4), np.float64)
+    Gd = dev.buffer((4, 4), np.float64, order="F")
+    assert L().fe_gemm_nn_mode(dev.ctx, _fe.GRAM_BF16X6, _fe.F64, 10, 4, 4, Xd.ptr, 4, Cd.ptr, 4, Yd.ptr, 4,
+                               1.0, 0.0) != 0
+    assert L().fe_gram(dev.ctx, _fe.GRAM_BF16X6, _fe.F64, 10, 4, 4, Xd.ptr, 4, Xd.ptr, 4, None, Gd.ptr, 4) != 0
 
 
 def test_gram_reproducible(dev):

@@ -293,7 +293,7 @@ def test_mixed_precision_reaches_fp64_accuracy():
     r64 = p.solve(**base)
     assert r64.residual_norms.max() <= 1e-8
     assert np.all(np.abs(r64.eigenvalues - exact) <= 1e-10 * np.abs(exact))
-    for gram in ("native", "bf16x3"):
+    for gram in ("native", "bf16x3", "bf16x6", "auto"):
         rm = p.solve(precision="mixed", gram=gram, **base)
         assert rm.residual_norms.max() <= 1e-8
         assert np.all(np.abs(rm.eigenvalues - r64.eigenvalues) <= 1e-10 * np.abs(r64.eigenvalues))
