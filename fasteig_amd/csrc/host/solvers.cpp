@@ -2,6 +2,7 @@
 // solver follows the reference source line by line (cited per step); every
 // n-sized operation runs on the device through libfe_hip.so.
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <iostream>
 #include <limits>
@@ -126,6 +127,9 @@ FASTEIG_INT ChebyshevSolver::solve() {
   double best_rate = 0.0;  // fastest residual decay seen, decades per filter step
   int cycle_degree = degree;  // filter degree of the coming cycle
   bool refine_wanted = false;  // FP64 cycles filter the wanted columns only
+  bool closing = false;        // the coming cycle is the closing FP64 sweep of the mixed path
+  std::vector<double> prev_relres;  // the previous FP32 cycle's residuals of the wanted pairs
+  std::vector<int> col_deg;    // per-column degrees of the closing sweep (empty: cycle_degree for all)
   double cutoff = std::numeric_limits<double>::quiet_NaN();  // adaptive bound (ritz_bounds)
   hip::Small inv_norms(ex, capacity, 1);
 
@@ -143,12 +147,32 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // 1. filter the active columns (ChebyshevSolver.cpp:151-155, 223-258)
     if (active > 0 && upper != lower) {
       BlockRef prev = F.cols(0, active), cur = P.cols(0, active);
+      // per-column degrees (the closing sweep): the columns are gathered in
+      // order of decreasing degree, so step k filters the prefix of the
+      // columns whose degree is at least k; all degrees have the parity of the
+      // largest, so every column's last iterate ends in the same buffer
+      const bool percol = static_cast<int>(col_deg.size()) == active && locked == 0;
+      std::vector<int> perm, back;
+      if (percol) {
+        perm.resize(active);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return col_deg[a] > col_deg[b]; });
+        back.resize(active);
+        for (int i = 0; i < active; ++i) back[perm[i]] = i;
+      }
+      auto width = [&](int k) {  // columns filtered at step k
+        if (!percol) return active;
+        int w = 0;
+        while (w < active && col_deg[perm[w]] >= k) ++w;
+        return w;
+      };
       // the filter reads its start block in place when the active columns
       // start on a 16-byte boundary (the LDS-tiled product needs aligned
-      // rows); otherwise from an aligned copy
+      // rows); otherwise from an aligned copy (or the gathered one)
       const BlockRef start = V.cols(locked, active);
-      const bool in_place = (locked * static_cast<int64_t>(V.ref().esize())) % 16 == 0;
-      if (!in_place) hip::copy(ex, start, prev);
+      const bool in_place = !percol && (locked * static_cast<int64_t>(V.ref().esize())) % 16 == 0;
+      if (percol) hip::gather_cols(ex, start, perm, prev);
+      else if (!in_place) hip::copy(ex, start, prev);
       const BlockRef y0 = in_place ? start : prev;
       if (ritz_bounds && std::isfinite(cutoff) && cutoff > lower && cutoff < upper) {
         // scaled filter on [cutoff, upper] normalised at the lower bound
@@ -160,7 +184,10 @@ FASTEIG_INT ChebyshevSolver::solve() {
         hip::spmm(ex, A, y0, cur, sign * s1 / e, -c * s1 / e);
         for (int k = 2; k <= cycle_degree; ++k) {
           const double sn = 1.0 / (2.0 / s1 - sigma);
-          hip::spmm(ex, A, cur, prev, sign * 2.0 * sn / e, -c * 2.0 * sn / e, k == 2 ? &y0 : &prev, -sigma * sn);
+          const int w = width(k);
+          const BlockRef yw = y0.range(0, w), pw = prev.range(0, w);
+          hip::spmm(ex, A, cur.range(0, w), pw, sign * 2.0 * sn / e, -c * 2.0 * sn / e, k == 2 ? &yw : &pw,
+                    -sigma * sn);
           std::swap(prev, cur);
           sigma = sn;
         }
@@ -178,9 +205,9 @@ FASTEIG_INT ChebyshevSolver::solve() {
       }
       // 2. deflate against the locked Ritz vectors (ChebyshevSolver.cpp:159-166)
       if (locked > 0) hip::block_gram_schmidt(ws, V.cols(0, locked), cur, cur);
-      if (prec == FE_F32 || cycle_degree > degree) {
-        // FP32 blocks and the long closing sweep: equilibrate the columns so
-        // the Gram of the filtered block is not limited by their norm spread
+      if (prec == FE_F32 || closing || cycle_degree > degree) {
+        // FP32 blocks and the closing sweep: equilibrate the columns so the
+        // Gram of the filtered block is not limited by their norm spread
         const std::vector<double> nr = hip::column_norms(ex, cur);
         std::vector<double> inv(active);
         for (int j = 0; j < active; ++j) inv[j] = nr[j] > 0.0 ? 1.0 / nr[j] : 1.0;
@@ -189,7 +216,9 @@ FASTEIG_INT ChebyshevSolver::solve() {
                                  inv_norms.data()),
                    "scale");
       }
-      if (locked == 0 && active == V.ncols() && (cur.ptr == F.data() || cur.ptr == P.data())) {
+      if (percol) {
+        hip::gather_cols(ex, cur, back, V.cols(locked, active));  // back to the Ritz order
+      } else if (locked == 0 && active == V.ncols() && (cur.ptr == F.data() || cur.ptr == P.data())) {
         V.swap(cur.ptr == F.data() ? F : P);  // the filtered block becomes V, the old V scratch
       } else {
         hip::copy(ex, cur, V.cols(locked, active));
@@ -200,7 +229,7 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // (and CholQR2 after the long closing sweep); a failed first factorization
     // is retried with a shift (shifted CholQR, Fukaya et al. 2020) and two
     // more passes
-    int passes = (prec == FE_F32 || cycle_degree > degree) ? 2 : 1;
+    int passes = (prec == FE_F32 || closing || cycle_degree > degree) ? 2 : 1;
     const double cycle_need = ws.need;
     for (int pass = 0; pass < passes; ++pass) {
       ws.ensure(subspace_cols);
@@ -280,6 +309,33 @@ FASTEIG_INT ChebyshevSolver::solve() {
         }
         prec = FE_F64;
         refine_wanted = mx <= 1e-4;
+        closing = true;
+        // Per-column degrees: column j needs log10(r_j / (tol / 4)) / rate_j
+        // steps, rate_j its own decay over the last FP32 cycle but at least the
+        // fastest decay of the largest residual (so no column gets more steps
+        // than the uniform degree above), at least 2; the columns far below the
+        // cutoff converge many times faster than the ones next to it.
+        col_deg.clear();
+        if (ritz_bounds && best_rate > 0.0 && mx > tol && refine_wanted &&
+            static_cast<int>(prev_relres.size()) == nev) {
+          col_deg.assign(nev, 2);
+          int D = 2;
+          for (int j = 0; j < nev; ++j) {
+            const double r = ritz.relres[j], rp = prev_relres[j];
+            double rate = best_rate;
+            if (r > 0.0 && rp > r) rate = std::max(rate, std::log10(rp / r) / degree);
+            const int d = r > 0.25 * tol ? static_cast<int>(std::ceil(std::log10(r / (0.25 * tol)) / rate)) : 2;
+            col_deg[j] = std::clamp(d, 2, cycle_degree);
+            D = std::max(D, col_deg[j]);
+          }
+          for (int& d : col_deg) d += (D - d) & 1;  // the parity of the largest
+          cycle_degree = D;
+          if (options_.print_every() > 0 && options_.verbosity() >= 2) {
+            double mean = 0.0;
+            for (int d : col_deg) mean += d;
+            std::cout << "  closing sweep: column degrees mean " << mean / nev << ", max " << D << std::endl;
+          }
+        }
         Block V64(ex, n, capacity, FE_F64);
         hip::copy(ex, V.ref(), V64.ref());
         V = std::move(V64);
@@ -293,9 +349,12 @@ FASTEIG_INT ChebyshevSolver::solve() {
         if (iter + 1 == maxiter) --iter;  // always finish with an FP64 cycle
       }
       last_mx = mx;
+      prev_relres.assign(ritz.relres.begin(), ritz.relres.begin() + nev);
       continue;  // no locking on FP32 residuals
     }
     cycle_degree = degree;
+    closing = false;
+    col_deg.clear();
 
     // 4. ordered soft locking (ChebyshevSolver.cpp:188-192)
     locked = 0;
