@@ -522,7 +522,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=["c3", "c2", "small", "c4", "c5"])
-    ap.add_argument("--gram", default="native", choices=["native", "bf16x3"])
+    ap.add_argument("--gram", default="auto", choices=["auto", "native", "bf16x3", "bf16x6"])
     ap.add_argument("--format", default="auto", choices=["auto", "csr", "blocked", "tiled", "sell"],
                     help="sparse format of the solve (auto: LDS-tiled, else row-block, when the matrix allows)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -632,6 +632,8 @@ def main():
             out_acc[f"median_rel_eig_err_{m}_vs_oracle"] = float(np.median(err))
             out_acc[f"rel_eig_err_{m}_vs_oracle_per_pair"] = [float(f"{e:.2e}") for e in err]
     extra["cold_first_solve_s"] = cold_s
+    # product modes the automatic choice measured (time and error of each)
+    extra["product_tuning"] = p.product_tuning()
     del p
 
     # dominant kernel: fused Chebyshev step at the solve's shape (FP32, k = nev)

@@ -3,8 +3,8 @@ partials live in a fixed per-device array), against fe_gram through the C ABI.
 
 Bar: bitwise equal to fe_gram when fe_gram's row chunks fit the fixed array
 (same kernels, same chunks, same reduction order); within 1e-13 (FP64) /
-2e-6 (FP32) relative to the operand scale when fewer, taller chunks are used
-(summation order only). Two calls in a row, the second on a smaller input, so
+2e-6 (FP32) / 1e-5 (compensated BF16x3) relative to the operand scale when
+fewer, taller chunks are used (summation order only). Two calls in a row, the second on a smaller input, so
 the array's leftover contents from the first call must not leak into the
 second. The device's free memory does not change across a call (fe_gram's
 first call of a size, the positive control, does take memory).
@@ -82,13 +82,18 @@ def test_static_gram_two_calls_second_smaller(fresh):
 
 
 def test_static_gram_bf16x3_matches(fresh):
+    """The compensated BF16 Gram keeps 256 x 256 partial tiles: fe_gram's 512
+    row chunks need 256 MB of them at m = 128, so the static variant runs fewer,
+    taller chunks (summation order only: within 1e-5 of fe_gram, and both
+    within the mode's 2^-14 of the FP64 product); two calls in a row, the second
+    smaller."""
     rng = np.random.default_rng(22)
-    X = rng.uniform(-1, 1, (100000, 128)).astype(np.float32)
-    g, s = gram_pair(fresh, _fe.GRAM_BF16X3, _fe.F32, X, X, 128, 128)
-    assert np.array_equal(g, s)
-    X2 = rng.uniform(-1, 1, (3000, 64)).astype(np.float32)
-    g2, s2 = gram_pair(fresh, _fe.GRAM_BF16X3, _fe.F32, X2, X2, 64, 64)
-    assert np.array_equal(g2, s2)
+    for n, m in ((100000, 128), (3000, 64)):
+        X = rng.uniform(-1, 1, (n, m)).astype(np.float32)
+        g, s = gram_pair(fresh, _fe.GRAM_BF16X3, _fe.F32, X, X, m, m)
+        ref = X.astype(np.float64).T @ X.astype(np.float64)
+        assert np.array_equal(s, s.T)
+        assert rel(s, g) <= 1e-5 and rel(s, ref) <= 2.0 ** -14 and rel(g, ref) <= 2.0 ** -14
 
 
 def test_static_gram_allocates_nothing(fresh):
