@@ -516,6 +516,36 @@ def serving_bench(count=512, rate=200.0, baseline=None, degree=16, max_slots=128
             "capacity_ratio": sat["solves_per_s"] / one["solves_per_s"]}
 
 
+def trace_split(db, marks_path, top=28):
+    """Per-kernel totals of a rocprofv3 kernel trace (its SQLite results
+    database) inside each window written by --solve-trace: the time of a solve
+    by kernel, its kernel count and the idle time between its kernels."""
+    import collections
+    import sqlite3
+    marks = json.load(open(marks_path))
+    rows = sqlite3.connect(db).execute("select name, start, end from kernels order by start").fetchall()
+    out = []
+    for name, (t0, t1) in marks.items():
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        busy = gaps = 0.0
+        prev_end = None
+        for kname, s0, s1 in rows:
+            if s0 >= t0 and s1 <= t1:
+                k = kname.split("(")[0].replace("void ", "")
+                agg[k][0] += 1
+                agg[k][1] += (s1 - s0) / 1e6
+                busy += (s1 - s0) / 1e6
+                if prev_end is not None and s0 > prev_end:
+                    gaps += (s0 - prev_end) / 1e6
+                prev_end = s1 if prev_end is None else max(prev_end, s1)
+        out.append(f"== {name}: window {(t1 - t0) / 1e6:.1f} ms, kernels {busy:.1f} ms, idle between kernels "
+                   f"{gaps:.1f} ms")
+        out.append(f"{'ms':>9} {'share':>6} {'count':>6}  kernel")
+        for k, (c, ms) in sorted(agg.items(), key=lambda x: -x[1][1])[:top]:
+            out.append(f"{ms:9.2f} {100 * ms / busy:5.1f}% {c:6d}  {k[:90]}")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -537,7 +567,16 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--kernel-formats", default="csr,blocked,tiled",
                     help="sparse formats timed by --only-kernels (csr, blocked = 4-row blocks, blocked8, tiled)")
+    ap.add_argument("--solve-trace", default="", metavar="PRECISIONS",
+                    help="for a kernel trace: per precision (e.g. mixed,fp64) one warm-up and one timed solve; "
+                         "the timed solves' CLOCK_MONOTONIC windows (the trace's clock) go to --marks")
+    ap.add_argument("--marks", default="solve_marks.json", help="output of --solve-trace")
+    ap.add_argument("--trace-split", nargs=2, metavar=("DB", "MARKS"),
+                    help="per-kernel time inside each --solve-trace window of a rocprofv3 results database")
     args = ap.parse_args()
+    if args.trace_split:
+        print(trace_split(*args.trace_split))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -552,6 +591,22 @@ def main():
         return
     if args.config == "c4" and not (args.only_kernels or args.only_dense):
         print(json.dumps(c4_bench()), flush=True)
+        return
+    if args.solve_trace:
+        M, cfg = build_matrix(args.config)
+        A = M[0] if isinstance(M, tuple) else M
+        p = Problem(A)
+        base = dict(solver="Chebyshev", nev=cfg["nev"], degree=cfg["degree"], tol=cfg["tol"], which="SR",
+                    maxiter=300, seed=1, cheb_bounds="ritz", format=args.format, gram=args.gram)
+        marks = {}
+        for prec in args.solve_trace.split(","):
+            p.solve(precision=prec, **base)
+            t_0 = time.monotonic_ns()
+            r = p.solve(precision=prec, **base)
+            marks[prec] = [t_0, time.monotonic_ns()]
+            log(f"trace solve {prec}: {(marks[prec][1] - t_0) / 1e9:.3f} s, {r.iterations} cycles, "
+                f"max residual {r.residual_norms.max():.2e}")
+        json.dump(marks, open(args.marks, "w"))
         return
     if args.only_dense:
         M, cfg = build_matrix(args.config)

@@ -39,6 +39,15 @@
 //                       spectrum, dense eigh on small random problems, a
 //                       matrix with a repeated eigenvalue (invariant-subspace
 //                       restart), Lanczos at m = n against eigh exactly.
+//   orc_residual_norms  numpy on the reference's definition (unnormalized, zero
+//                       and exact-eigenpair columns; negative and zero lambda)
+//   orc_orthonormalize_block
+//                       Q^T Q = I, B^T Q = 0 and the rank count on a block with
+//                       dependent columns (invariants)
+//   orc_block_krylov_schur
+//                       2D Laplacian closed form (SR, LR), SiO golden spectrum,
+//                       full-dimension exactness against eigh, a start block in
+//                       an invariant subspace (seeded replenishment)
 //   orc_random_block    reference's seeded mt19937/uniform(-1,1) fill: pinned by
 //                       the C++ standard's required 10000th mt19937 output
 //                       (4123659995) and by a golden vector.

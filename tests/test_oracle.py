@@ -111,6 +111,34 @@ def test_orthonormalize_block_drops_dependent_directions():
     assert np.max(np.abs(B.T @ Q)) < 1e-13
 
 
+def test_residual_norms_match_numpy():
+    """orc_residual_norms against numpy on the reference's definition
+    (src/fasteig_solver.cpp:89-129): ||A x - l x|| / ((1 + |l|) ||x||), and
+    ||A x - l x|| alone when that denominator is 0; unnormalized columns,
+    negative and zero l, a zero column, and an exact eigenpair (residual at
+    the rounding level). A dropped (1 + |l|), a missing ||x|| or a sign error
+    in l x each fail one of these."""
+    rng = np.random.default_rng(5)
+    n = 60
+    A = sp.random(n, n, density=0.1, random_state=7, format="csr")
+    A = (A + A.T).tocsr()
+    X = rng.standard_normal((n, 5)) * np.array([1.0, 3.0, 0.2, 1.0, 1.0])
+    X[:, 3] = 0.0
+    lam = np.array([0.7, -2.5, 0.0, 4.0, -1.0])
+    w, V = np.linalg.eigh(A.toarray())
+    X[:, 4] = 2.0 * V[:, 0]
+    lam[4] = w[0]
+    Ad = A.toarray()
+    ref = np.empty(5)
+    for j in range(5):
+        num = np.linalg.norm(Ad @ X[:, j] - lam[j] * X[:, j])
+        den = (1.0 + abs(lam[j])) * np.linalg.norm(X[:, j])
+        ref[j] = num / den if den > 0 else num
+    got = orc.residual_norms(A, lam, X)
+    assert np.allclose(got[:4], ref[:4], rtol=1e-13, atol=0)
+    assert got[3] == 0.0 and got[4] <= 1e-14
+
+
 @pytest.mark.parametrize("n,ka,kb", [(1, 1, 1), (70, 5, 3), (5000, 37, 20), (70001, 16, 64)])
 def test_dense_products_match_numpy(n, ka, kb):
     rng = np.random.default_rng(n + ka)
