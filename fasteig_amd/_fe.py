@@ -19,6 +19,7 @@ FE_SO = os.path.join(LIBDIR, "libfe_hip.so")
 
 F64, F32 = 0, 1
 GRAM_NATIVE, GRAM_BF16X3, GRAM_BF16X6 = 0, 1, 2
+GRAM_SYMMETRIC = 16
 FMT_CSR, FMT_SELL, FMT_BLOCKED, FMT_TILED = 0, 1, 2, 3
 
 _lib = None
@@ -84,6 +85,9 @@ def lib():
         L.fe_trmm_upper.argtypes = [vp, i32, i64, i32, vp, i64, vp, i64, vp, i64]
         L.fe_gemm_nn_mode.argtypes = [vp, i32, i32, i64, i32, i32, vp, i64, vp, i64, vp, i64, dbl, dbl]
         L.fe_trmm_upper_mode.argtypes = [vp, i32, i32, i64, i32, vp, i64, vp, i64, vp, i64]
+        L.fe_gemm_nn_resid.argtypes = [vp, i32, i32, i64, i32, i32, vp, i64, vp, i64, vp, i64, vp, i64, vp, vp,
+                                       i64, vp, vp]
+        L.fe_identity_deviation.argtypes = [vp, i32, vp, i64, C.POINTER(dbl)]
         L.fe_potrf.argtypes = [vp, i32, vp, i64]
         L.fe_chol_rinv.argtypes = [vp, i32, vp, i64, vp, i64]
         L.fe_syev.argtypes = [vp, i32, vp, i64, vp, vp, i64]

@@ -46,7 +46,7 @@ struct fe_ctx_s {
   int cus = 0;
   size_t hbm = 0;
   char name[128] = {0};
-  fe::Scratch ws[5];           // independent scratch slots (4: split coefficients of the BF16 products)
+  fe::Scratch ws[6];           // independent scratch slots (4: split coefficients of the BF16 products, 5: small results)
   fe::BlockCache cache;        // fe_alloc / fe_free reuse
   int* info_dev = nullptr;     // small device status words
   void* gram_static = nullptr; // device address of fe_gram_static's fixed scratch on this device
@@ -139,6 +139,10 @@ bool detect_grid(const std::vector<int>& rp, const std::vector<int>& ci, int n, 
 
 // Ensure scratch slot `slot` of ctx holds at least `bytes`.
 int scratch(fe_ctx ctx, int slot, size_t bytes, void** out);
+
+// out[j] = sqrt(sum over c < chunks of part[c * k + j]) for j < k, the chunks
+// summed in a fixed order (launched on the context stream).
+int reduce_cols_sqrt(fe_ctx ctx, int chunks, int k, const double* part, double* out);
 
 // Looks up (once per device, with ctx->device current) the device address of
 // the fixed scratch array behind fe_gram_static and stores it in

@@ -243,6 +243,34 @@ inline unsigned grid_for(int64_t total) {
   return static_cast<unsigned>(b);
 }
 
+int reduce_cols_sqrt(fe_ctx ctx, int chunks, int k, const double* part, double* out) {
+  if (k <= 0) return 0;
+  hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 64)), dim3(256), 0, ctx->stream, chunks, k, part, out);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+// Sum of (G_ij - delta_ij)^2 over the leading m x m block, one workgroup:
+// each thread sums whole columns j = tid, tid + 256, ... in row order, then
+// the 256 thread sums are added in a fixed tree order.
+__global__ __launch_bounds__(256) void k_identity_dev(int m, const double* __restrict__ G, int64_t ldg,
+                                                      double* out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int j = threadIdx.x; j < m; j += 256)
+    for (int i = 0; i < m; ++i) {
+      const double d = G[(int64_t)j * ldg + i] - (i == j ? 1.0 : 0.0);
+      s += d * d;
+    }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (static_cast<int>(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
 }  // namespace fe
 
 using namespace fe;
@@ -549,6 +577,24 @@ static int colsq_launch(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, i
     hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 64)), dim3(256), 0, ctx->stream, (int)chunks, k,
                        part_r, rnorm);
   FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_identity_deviation(fe_ctx ctx, int m, const double* G, int64_t ldg, double* out) {
+  FE_ARG(ctx && G && out, "fe_identity_deviation: null argument");
+  FE_ARG(m >= 0 && ldg >= m, "fe_identity_deviation: bad dimensions");
+  if (m == 0) {
+    *out = 0.0;
+    return 0;
+  }
+  void* ws = nullptr;
+  if (int rc = scratch(ctx, 5, sizeof(double), &ws)) return rc;
+  hipLaunchKernelGGL(k_identity_dev, dim3(1), dim3(256), 0, ctx->stream, m, G, ldg, (double*)ws);
+  FE_CHECK_LAUNCH();
+  double s = 0.0;
+  FE_HIP_TRY(hipMemcpyAsync(&s, ws, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *out = std::sqrt(s);
   return 0;
 }
 

@@ -280,6 +280,89 @@ __global__ __launch_bounds__(256, 4) void k_gram64(int64_t n, int m, int p, cons
     }
 }
 
+// ---------------------------------------------------------------- fused Ritz residual
+// Epilogue of the rotation AY = AQ C of a Rayleigh-Ritz extraction: per
+// element v = AY[row, col] (stored if `store`), R[row, col] = v - theta[col]
+// Y[row, col] with Y = Q C read from memory (R stored if not null), and the
+// column sums of R^2 and Y^2 over the workgroup's rows into part_r / part_y
+// [row tile][p]. A lane sums its rows of each of its four columns, the four
+// lanes of a column are added by shuffles, the row wavefronts in order
+// through LDS: a fixed order, so the sums are reproducible.
+template <typename T>
+struct ResArgs {
+  const T* Y;
+  int64_t ldy;
+  const double* theta;
+  T* R;
+  int64_t ldr;
+  double* part_r;
+  double* part_y;
+  int store;
+};
+
+// acc[i][j] of the wavefront's 64 x 64 block at rows rw0 (+ 16 i + row(lane, e))
+// and columns cw0 (+ 16 j + lane % 16); red: 2 NWM TC doubles of LDS
+template <typename T, typename ROW, int NWM, int TC, typename ACC>
+__device__ __forceinline__ void resid_epilogue(const ACC (&acc)[4][4], int64_t n, int p, int64_t rw0, int cw0,
+                                               T alpha, T* AY, int64_t lday, const ResArgs<T>& res, double* red,
+                                               int wm, int wn, int64_t rt, int c0) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  // opaque copies of the tile origin: the element addresses are then formed
+  // here, after the product loop, instead of being hoisted above it (where
+  // they held registers through the loop and spilled)
+  int64_t rb = rw0;
+  int cb = cw0;
+  asm volatile("" : "+v"(rb), "+v"(cb));
+  double sr[4], sy[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    sr[j] = sy[j] = 0.0;
+    const int col = cb + j * 16 + (lane & 15);
+    if (col >= p) continue;
+    const double th = res.theta[col];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = rb + i * 16 + ROW::row(lane, e);
+        if (row < n) {
+          const T v = alpha * static_cast<T>(acc[i][j][e]);
+          if (res.store) AY[row * lday + col] = v;
+          const double yv = static_cast<double>(res.Y[row * res.ldy + col]);
+          const T rv = static_cast<T>(static_cast<double>(v) - th * yv);
+          if (res.R) res.R[row * res.ldr + col] = rv;
+          sr[j] += static_cast<double>(rv) * static_cast<double>(rv);
+          sy[j] += yv * yv;
+        }
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    sr[j] += __shfl_xor(sr[j], 16, 64);
+    sy[j] += __shfl_xor(sy[j], 16, 64);
+    sr[j] += __shfl_xor(sr[j], 32, 64);
+    sy[j] += __shfl_xor(sy[j], 32, 64);
+  }
+  __syncthreads();  // the LDS of the product stages is free
+  if (lane < 16)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      red[(2 * wm) * TC + wn * 64 + j * 16 + lane] = sr[j];
+      red[(2 * wm + 1) * TC + wn * 64 + j * 16 + lane] = sy[j];
+    }
+  __syncthreads();
+  if (tid < TC && c0 + tid < p) {
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int w = 0; w < NWM; ++w) {
+      a += red[(2 * w) * TC + tid];
+      b += red[(2 * w + 1) * TC + tid];
+    }
+    res.part_r[rt * p + c0 + tid] = a;
+    res.part_y[rt * p + c0 + tid] = b;
+  }
+}
+
 // ---------------------------------------------------------------- compensated BF16 products
 // An FP32 value x is split into NS bfloat16 parts, x_0 = bf16(x),
 // x_1 = bf16(x - x_0), x_2 = bf16(x - x_0 - x_1) (each difference is exact in
@@ -350,7 +433,7 @@ __global__ __launch_bounds__(512, 1) void k_gram_bf16(int64_t n, int m, int p, c
   const int tile = static_cast<int>(u - chunk * ntiles);
   int ta, tb;
   gram_tile(tile, tiles_b, sym, ta, tb);
-  const bool same = sym && ta == tb;  // workgroup-uniform
+  const bool same = sym == 1 && ta == tb;  // identical operands (workgroup-uniform)
   const int a0 = ta * GT, b0 = tb * GT;
   const int64_t r0 = chunk * rows_per;
   const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
@@ -467,13 +550,15 @@ __global__ void k_split_coeffs(int m, int p, const double* __restrict__ C, int64
 // (staged: thread t loads 16 k values of row t >> 1 with four 16-byte loads)
 // and of C (pre-split by k_split_coeffs: two 16-byte loads per part). Tile
 // mapping, triangular pairing and epilogue as k_gemm_nn.
-template <int NS>
+template <int NS, bool RES>
 __global__ __launch_bounds__(256, 2) void k_gemm_bf16(int64_t n, int m, int p, const float* __restrict__ X,
                                                    int64_t ldx, const __bf16* __restrict__ S, int mpad, float* Y,
                                                    int64_t ldy, float alpha, float beta, int tiles_c,
-                                                   int64_t per_xcd, int64_t ntiles, int vec, int upper) {
+                                                   int64_t per_xcd, int64_t ntiles, int vec, int upper,
+                                                   ResArgs<float> res) {
   __shared__ __attribute__((aligned(16))) __bf16 Xl[NS][WT][BKP];
   __shared__ __attribute__((aligned(16))) __bf16 Cl[NS][WT][BKP];
+  static_assert(sizeof(Xl) >= sizeof(double) * 4 * WT, "the residual sums reuse the X stage");
   const int64_t t = xcd_tile(blockIdx.x, per_xcd);
   if (t >= ntiles) return;
   const int per_slab = upper ? (tiles_c + 1) / 2 : tiles_c;
@@ -560,6 +645,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16(int64_t n, int m, int p, c
         if (i < vi && j < vj) acc[i][j] = mfma_split<NS>(af[i], bj, acc[i][j]);
     }
   }
+  if constexpr (RES) {
+    resid_epilogue<float, mfma<float>, 2, WT>(acc, n, p, r0 + wm * 64, c0 + wn * 64, alpha, Y, ldy, res,
+                                               reinterpret_cast<double*>(&Xl[0][0][0]), wm, wn, rt, c0);
+  } else {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -574,6 +663,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16(int64_t n, int m, int p, c
           y = beta == 0.0f ? v : v + beta * y;
         }
       }
+  }
   }  // pass (the next pass's first barrier orders its LDS writes after these reads)
 }
 
@@ -634,17 +724,18 @@ constexpr int MKS = 16;
 // ct = tiles_c - 1 - q of its row slab (tiles_c + 1 units for every q): with
 // one tile per workgroup the CUs that kept receiving the last column tile set
 // the finish time and the triangular product took as long as the full one.
-template <typename T, int TR, int TC>
+template <typename T, int TR, int TC, bool RES>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, const T* __restrict__ X, int64_t ldx,
                                                  const double* __restrict__ C, int64_t ldc, T* Y, int64_t ldy,
                                                  T alpha, T beta, int tiles_c, int64_t per_xcd, int64_t ntiles,
-                                                 int vec, int cvec, int upper) {
+                                                 int vec, int cvec, int upper, ResArgs<T> res) {
   static_assert((TR / 64) * (TC / 64) == 4, "four 64 x 64 wavefront tiles");
   constexpr int MK = MKS;  // k values per stage
   constexpr int TPR = 256 / TR, XK = MK / TPR;  // X staging: threads per row, k values per thread
   constexpr int TPC = 256 / TC, CK = MK / TPC;  // C staging: threads per column, k values per thread
   __shared__ T Xs[TR][MK + 1];
   __shared__ T Cs[MK][TC + GPAD];
+  __shared__ double red[RES ? 2 * (TR / 64) * TC : 1];  // residual column sums per row wavefront
   const int64_t t = xcd_tile(blockIdx.x, per_xcd);
   if (t >= ntiles) return;
   const int per_slab = upper ? (tiles_c + 1) / 2 : tiles_c;  // workgroups per row slab
@@ -725,6 +816,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
           if (i < vi && j < vj) acc[i][j] = mfma<T>::run(af[i], bf[j], acc[i][j]);
     }
   }
+  if constexpr (RES) {
+    resid_epilogue<T, mfma<T>, TR / 64, TC>(acc, n, p, r0 + wm * 64, c0 + wn * 64, alpha, Y, ldy, res, red, wm, wn,
+                                            rt, c0);
+  } else {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -739,6 +834,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
           y = beta == T(0) ? v : v + beta * y;
         }
       }
+  }
   }  // pass (the next pass's first barrier orders its LDS writes after these reads)
 }
 
@@ -777,11 +873,22 @@ static int gram_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, con
   FE_ARG(ctx && X && Y && G, me + ": null argument");
   FE_ARG(m >= 0 && p >= 0 && n >= 0 && ldx >= m && ldy >= p && ldg >= m, me + ": bad dimensions");
   FE_ARG(prec == FE_F64 || prec == FE_F32, me + ": bad precision");
-  FE_ARG(mode == FE_GRAM_NATIVE || ((mode == FE_GRAM_BF16X3 || mode == FE_GRAM_BF16X6) && prec == FE_F32),
-         me + ": bad mode");
+  {
+    const int base = mode & ~FE_GRAM_SYMMETRIC;
+    FE_ARG(base == FE_GRAM_NATIVE || ((base == FE_GRAM_BF16X3 || base == FE_GRAM_BF16X6) && prec == FE_F32),
+           me + ": bad mode");
+  }
+  // FE_GRAM_SYMMETRIC: the caller asserts a symmetric product (m == p), so
+  // only the tiles on and above the diagonal are computed and mirrored
+  const bool sym_result = (mode & FE_GRAM_SYMMETRIC) != 0;
+  mode &= ~FE_GRAM_SYMMETRIC;
+  FE_ARG(!sym_result || m == p, me + ": a symmetric product needs m == p");
   if (m == 0 || p == 0) return 0;
-  const bool sym = (X == Y) && (m == p) && (ldx == ldy) && !w;
-  const int symi = sym ? 1 : 0;
+  // symi 1: identical operands (the diagonal tiles stage one operand); 2: a
+  // symmetric product of distinct operands
+  const bool same = (X == Y) && (m == p) && (ldx == ldy) && !w;
+  const bool sym = same || sym_result;
+  const int symi = same ? 1 : sym ? 2 : 0;
   if (n == 0) return fe_zero2d(ctx, G, p, m, ldg, 8);
   const int cus = ctx->cus > 0 ? ctx->cus : 256;
   // partial tiles of one chunk, and the chunk count that fits the fixed buffer
@@ -893,8 +1000,29 @@ extern "C" int fe_gram_static(fe_ctx ctx, int mode, int prec, int64_t n, int m, 
 
 namespace fe {
 
+// Residual arguments of a fused rotation (fe_gemm_nn_resid), untyped.
+struct ResHost {
+  const void* Y;
+  int64_t ldy;
+  const double* theta;
+  void* R;
+  int64_t ldr;
+  double* part_r;
+  double* part_y;
+  int store;
+};
+
+template <typename T>
+static ResArgs<T> typed(const ResHost* r) {
+  if (!r) return ResArgs<T>{nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, 1};
+  return ResArgs<T>{static_cast<const T*>(r->Y), r->ldy, r->theta, static_cast<T*>(r->R), r->ldr, r->part_r,
+                    r->part_y, r->store};
+}
+
+// rh != nullptr: the fused residual epilogue (beta 0, not triangular)
 static int gemm_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
-                    const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta, int upper) {
+                    const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta, int upper,
+                    const ResHost* rh = nullptr) {
   if (mode == FE_GRAM_BF16X3 || mode == FE_GRAM_BF16X6) {
     // C split once into its bfloat16 parts (scratch slot 4), then the product
     const int NS = mode == FE_GRAM_BF16X3 ? 2 : 3;
@@ -912,14 +1040,12 @@ static int gemm_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, con
     const int64_t ntiles = tiles_r * (upper ? (tiles_c + 1) / 2 : tiles_c);
     const int64_t nb8 = (ntiles + 7) / 8 * 8;
     const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && ldx % 4 == 0;
-    if (NS == 2)
-      hipLaunchKernelGGL(k_gemm_bf16<2>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
-                         (const __bf16*)sp, mpad, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles,
-                         vec, upper);
-    else
-      hipLaunchKernelGGL(k_gemm_bf16<3>, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
-                         (const __bf16*)sp, mpad, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles,
-                         vec, upper);
+    const ResArgs<float> ra = typed<float>(rh);
+    auto kern = NS == 2 ? (rh ? k_gemm_bf16<2, true> : k_gemm_bf16<2, false>)
+                        : (rh ? k_gemm_bf16<3, true> : k_gemm_bf16<3, false>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
+                       (const __bf16*)sp, mpad, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles,
+                       vec, upper, ra);
     FE_CHECK_LAUNCH();
     return 0;
   }
@@ -937,22 +1063,26 @@ static int gemm_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, con
   // coefficient columns read as 16-byte vectors when C and ldc allow it
   const int cvec = (reinterpret_cast<uintptr_t>(C) % 16 == 0) && ldc % 2 == 0;
   if (prec == FE_F64) {
-    if (thin)
-      hipLaunchKernelGGL((k_gemm_nn<double, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx, C,
-                         ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper);
-    else
-      hipLaunchKernelGGL((k_gemm_nn<double, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx,
-                         C, ldc, (double*)Y, ldy, alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper);
+    const ResArgs<double> ra = typed<double>(rh);
+    auto kern = thin ? (rh ? k_gemm_nn<double, 256, 64, true> : k_gemm_nn<double, 256, 64, false>)
+                     : (rh ? k_gemm_nn<double, 128, 128, true> : k_gemm_nn<double, 128, 128, false>);
+    hipLaunchKernelGGL(kern, grid, block, 0, ctx->stream, n, m, p, (const double*)X, ldx, C, ldc, (double*)Y, ldy,
+                       alpha, beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper, ra);
   } else {
-    if (thin)
-      hipLaunchKernelGGL((k_gemm_nn<float, 256, 64>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
-                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper);
-    else
-      hipLaunchKernelGGL((k_gemm_nn<float, 128, 128>), grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C,
-                         ldc, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper);
+    const ResArgs<float> ra = typed<float>(rh);
+    auto kern = thin ? (rh ? k_gemm_nn<float, 256, 64, true> : k_gemm_nn<float, 256, 64, false>)
+                     : (rh ? k_gemm_nn<float, 128, 128, true> : k_gemm_nn<float, 128, 128, false>);
+    hipLaunchKernelGGL(kern, grid, block, 0, ctx->stream, n, m, p, (const float*)X, ldx, C, ldc, (float*)Y, ldy,
+                       (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles, vec, cvec, upper, ra);
   }
   FE_CHECK_LAUNCH();
   return 0;
+}
+
+// Row tiles of the product (the residual partials per column)
+static int64_t gemm_row_tiles(int mode, int64_t n, int p) {
+  const int TR = (mode == FE_GRAM_BF16X3 || mode == FE_GRAM_BF16X6) ? WT : (p <= 64 ? 256 : WT);
+  return (n + TR - 1) / TR;
 }
 
 }  // namespace fe
@@ -992,4 +1122,31 @@ extern "C" int fe_trmm_upper_mode(fe_ctx ctx, int mode, int prec, int64_t n, int
 extern "C" int fe_trmm_upper(fe_ctx ctx, int prec, int64_t n, int m, const void* X, int64_t ldx, const double* C,
                              int64_t ldc, void* Y, int64_t ldy) {
   return fe_trmm_upper_mode(ctx, FE_GRAM_NATIVE, prec, n, m, X, ldx, C, ldc, Y, ldy);
+}
+
+extern "C" int fe_gemm_nn_resid(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                                const double* C, int64_t ldc, void* AY, int64_t lday, const void* Y, int64_t ldy,
+                                const double* theta, void* R, int64_t ldr, double* rnorm, double* ynorm) {
+  FE_ARG(ctx && X && C && Y && theta && rnorm && ynorm, "fe_gemm_nn_resid: null argument");
+  FE_ARG(X != AY && X != Y && X != R && (!AY || AY != Y) && (!R || (R != Y && R != AY)),
+         "fe_gemm_nn_resid: outputs must not alias inputs");
+  FE_ARG(n >= 0 && m > 0 && p >= 0 && ldx >= m && ldy >= p && ldc >= m && (!AY || lday >= p) && (!R || ldr >= p),
+         "fe_gemm_nn_resid: bad dimensions");
+  FE_ARG(prec == FE_F64 || prec == FE_F32, "fe_gemm_nn_resid: bad precision");
+  FE_ARG(mode == FE_GRAM_NATIVE || ((mode == FE_GRAM_BF16X3 || mode == FE_GRAM_BF16X6) && prec == FE_F32),
+         "fe_gemm_nn_resid: bad mode");
+  if (p == 0) return 0;
+  if (n == 0) {
+    if (int rc = fe_zero2d(ctx, rnorm, 1, p, p, 8)) return rc;
+    return fe_zero2d(ctx, ynorm, 1, p, p, 8);
+  }
+  const int64_t tiles_r = fe::gemm_row_tiles(mode, n, p);
+  void* ws = nullptr;
+  if (int rc = fe::scratch(ctx, 3, sizeof(double) * 2 * (size_t)tiles_r * p, &ws)) return rc;
+  double* part_r = static_cast<double*>(ws);
+  double* part_y = part_r + tiles_r * p;
+  const fe::ResHost rh{Y, ldy, theta, R, ldr, part_r, part_y, AY ? 1 : 0};
+  if (int rc = fe::gemm_run(ctx, mode, prec, n, m, p, X, ldx, C, ldc, AY, lday, 1.0, 0.0, 0, &rh)) return rc;
+  if (int rc = fe::reduce_cols_sqrt(ctx, static_cast<int>(tiles_r), p, part_r, rnorm)) return rc;
+  return fe::reduce_cols_sqrt(ctx, static_cast<int>(tiles_r), p, part_y, ynorm);
 }

@@ -358,12 +358,12 @@ int feh_problem_product_tuning(feh_problem p, char* buf, int len) {
   return guarded([&] {
     if (!p || !buf || len <= 0) throw std::invalid_argument("null argument");
     std::ostringstream os;
-    static const char* kinds[] = {"gram_sym", "gram", "gram_weighted", "gemm", "trmm"};
+    static const char* kinds[] = {"gram_sym", "gram", "gram_weighted", "gemm", "trmm", "gram_rr"};
     if (auto ex = p->prob->current_executor()) {
       os.precision(6);
       for (const auto& [key, t] : ex->tunings()) {
         const int kind = std::get<0>(key);
-        os << (kind >= 0 && kind < 5 ? kinds[kind] : "?") << ' ' << std::get<1>(key) << ' ' << std::get<2>(key) << ' '
+        os << (kind >= 0 && kind < 6 ? kinds[kind] : "?") << ' ' << std::get<1>(key) << ' ' << std::get<2>(key) << ' '
            << std::get<3>(key);
         for (int i = 0; i < 3; ++i) os << ' ' << t.ms[i];
         for (int i = 0; i < 3; ++i) os << ' ' << t.err[i];

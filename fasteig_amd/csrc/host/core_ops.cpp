@@ -213,7 +213,7 @@ std::vector<int> select_indices(const std::vector<double>& v, const std::string&
 
 RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
                        const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R,
-                       int res_count) {
+                       int res_count, bool keep_ay) {
   HipExecutor& ex = ws.ex;
   if (m <= 0 || requested <= 0 || requested > m)
     throw std::invalid_argument("extract_projected_ritz_pairs: invalid active/requested Ritz dimensions.");
@@ -231,16 +231,17 @@ RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& 
   idx.resize(requested);
   check(fe_gather_cols(ex.ctx(), m, requested, ws.V.data(), ws.cap, idx.data(), ws.C.data(), ws.cap),
         "fe_gather_cols");
-  gemm(ex, Q.range(0, m), ws.C.data(), ws.cap, Y, 1.0, 0.0, ws.gram_mode, ws.need);
-  gemm(ex, AQ.range(0, m), ws.C.data(), ws.cap, AY, 1.0, 0.0, ws.gram_mode, ws.need);  // C's first nr columns
   RitzPairs out;
   out.selected = idx;
   out.values.resize(requested);
   for (int j = 0; j < requested; ++j) out.values[j] = w[idx[j]];
   check(fe_memcpy_h2d(ex.ctx(), ws.theta.data(), out.values.data(), sizeof(double) * nr), "h2d(theta)");
-  check(fe_ritz_residuals(ex.ctx(), Y.prec, Y.rows, nr, Y.ptr, Y.ld, AY.ptr, AY.ld, ws.theta.data(),
-                          R ? R->ptr : nullptr, R ? R->ld : 0, ws.rn.data(), ws.yn.data()),
-        "fe_ritz_residuals");
+  // Y = Q C, then AY = (AQ) C for C's first nr columns with R = AY - Y Theta
+  // and the column norms in the same product's epilogue
+  gemm(ex, Q.range(0, m), ws.C.data(), ws.cap, Y, 1.0, 0.0, ws.gram_mode, ws.need);
+  const BlockRef Yr = Y.range(0, nr);
+  gemm_resid(ex, AQ.range(0, m), ws.C.data(), ws.cap, AY, keep_ay, Yr, ws.theta.data(), R, ws.rn.data(),
+             ws.yn.data(), ws.gram_mode, ws.need);
   std::vector<double> rn(nr), yn(nr);
   check(fe_memcpy_d2h(ex.ctx(), rn.data(), ws.rn.data(), sizeof(double) * nr), "d2h");
   check(fe_memcpy_d2h(ex.ctx(), yn.data(), ws.yn.data(), sizeof(double) * nr), "d2h");

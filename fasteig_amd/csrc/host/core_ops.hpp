@@ -67,9 +67,12 @@ struct RitzPairs {
 // include/fasteig_solver.hpp:59-82. The coefficient matrix stays in ws.C.
 // res_count >= 0: AY, R and the residual norms only for the first res_count
 // selected pairs (relres then holds res_count entries); Y for all requested.
+// The residual and its norms come from the epilogue of the AY product
+// (fe_gemm_nn_resid); keep_ay false leaves AY unwritten (its block is then
+// only the tuning target of that product).
 RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
                        const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R,
-                       int res_count = -1);
+                       int res_count = -1, bool keep_ay = true);
 
 // Stable index sort by `which` (include/fasteig_solver.hpp:59-82; "be" from
 // src/core/RitzExtraction.cpp:162-179).

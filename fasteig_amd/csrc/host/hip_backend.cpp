@@ -191,7 +191,7 @@ void spmm(HipExecutor& ex, const HipCsrMatrix& A, const BlockRef& X, const Block
 namespace {
 
 constexpr int kModes[3] = {FE_GRAM_NATIVE, FE_GRAM_BF16X3, FE_GRAM_BF16X6};
-enum ProductKind { kGramSym = 0, kGram = 1, kGramW = 2, kGemm = 3, kTrmm = 4 };
+enum ProductKind { kGramSym = 0, kGram = 1, kGramW = 2, kGemm = 3, kTrmm = 4, kGramSymProd = 5 };
 constexpr int kTuneEvent0 = 14, kTuneEvent1 = 15;  // context event slots used by the tuning
 constexpr int64_t kTuneRowsGram = 65536;            // row sample of a Gram's error
 constexpr int64_t kTuneRowsTall = 8192;             // row sample of a tall product's error
@@ -241,16 +241,19 @@ Block fp64_rows(HipExecutor& ex, const BlockRef& X, int64_t rows) {
 }  // namespace
 
 void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, double* G, int64_t ldg, const double* w,
-          double need) {
+          double need, bool symmetric) {
   if (X.prec != Y.prec || X.rows != Y.rows) throw std::invalid_argument("gram: operand mismatch");
+  if (symmetric && X.cols != Y.cols) throw std::invalid_argument("gram: a symmetric product needs m == p");
+  const bool same = X.ptr == Y.ptr && X.cols == Y.cols && X.ld == Y.ld;
+  const int flag = (symmetric && !same) ? FE_GRAM_SYMMETRIC : 0;
   auto run = [&](int md, const BlockRef& x, const BlockRef& y) {
-    check(fe_gram(ex.ctx(), md, x.prec, x.rows, x.cols, y.cols, x.ptr, x.ld, y.ptr, y.ld, w, G, ldg), "fe_gram");
+    check(fe_gram(ex.ctx(), md | flag, x.prec, x.rows, x.cols, y.cols, x.ptr, x.ld, y.ptr, y.ld, w, G, ldg),
+          "fe_gram");
   };
   if (X.prec == FE_F64 || mode == FE_GRAM_NATIVE || (mode == kProductAuto && !(need > 0.0)))
     return run(FE_GRAM_NATIVE, X, Y);
   if (mode != kProductAuto) return run(mode, X, Y);
-  const bool same = X.ptr == Y.ptr && X.cols == Y.cols && X.ld == Y.ld;
-  const int kind = w ? kGramW : same ? kGramSym : kGram;
+  const int kind = w ? kGramW : same ? kGramSym : flag ? kGramSymProd : kGram;
   const HipExecutor::TuningKey key{kind, X.rows, X.cols, Y.cols};
   const HipExecutor::ProductTuning* t = ex.tuning(key);
   if (!t) {
@@ -286,11 +289,10 @@ void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, doubl
 // the tall products Y = X C: tuning as for the Gram, the error from a row
 // sample against the FP64 product (run(mode, x, y) with an FP64 x and y for the
 // reference)
-static void tall_product(HipExecutor& ex, int kind, int mode, double need, const BlockRef& X, const BlockRef& Y,
-                         const std::function<void(int, const BlockRef&, const BlockRef&)>& run) {
-  if (X.prec == FE_F64 || mode == FE_GRAM_NATIVE || (mode == kProductAuto && !(need > 0.0)))
-    return run(FE_GRAM_NATIVE, X, Y);
-  if (mode != kProductAuto) return run(mode, X, Y);
+static int tall_mode(HipExecutor& ex, int kind, int mode, double need, const BlockRef& X, const BlockRef& Y,
+                     const std::function<void(int, const BlockRef&, const BlockRef&)>& run) {
+  if (X.prec == FE_F64 || mode == FE_GRAM_NATIVE || (mode == kProductAuto && !(need > 0.0))) return FE_GRAM_NATIVE;
+  if (mode != kProductAuto) return mode;
   const HipExecutor::TuningKey key{kind, X.rows, X.cols, Y.cols};
   const HipExecutor::ProductTuning* t = ex.tuning(key);
   if (!t) {
@@ -314,7 +316,28 @@ static void tall_product(HipExecutor& ex, int kind, int mode, double need, const
     }
     t = &ex.set_tuning(key, nt);
   }
-  run(choose(*t, need), X, Y);
+  return choose(*t, need);
+}
+
+static void tall_product(HipExecutor& ex, int kind, int mode, double need, const BlockRef& X, const BlockRef& Y,
+                         const std::function<void(int, const BlockRef&, const BlockRef&)>& run) {
+  run(tall_mode(ex, kind, mode, need, X, Y, run), X, Y);
+}
+
+void gemm_resid(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& AY, bool keep_ay,
+                const BlockRef& Y, const double* theta, const BlockRef* R, double* rn, double* yn, int mode,
+                double need) {
+  if (X.prec != AY.prec || X.rows != AY.rows || Y.prec != X.prec || Y.rows != X.rows || Y.cols < AY.cols ||
+      (R && (R->prec != X.prec || R->rows != X.rows || R->cols < AY.cols)))
+    throw std::invalid_argument("gemm_resid: operand mismatch");
+  const int md = tall_mode(ex, kGemm, mode, need, X, AY, [&](int m, const BlockRef& x, const BlockRef& y) {
+    check(fe_gemm_nn_mode(ex.ctx(), m, x.prec, x.rows, x.cols, y.cols, x.ptr, x.ld, C, ldc, y.ptr, y.ld, 1.0, 0.0),
+          "fe_gemm_nn");
+  });
+  check(fe_gemm_nn_resid(ex.ctx(), md, X.prec, X.rows, X.cols, AY.cols, X.ptr, X.ld, C, ldc,
+                         keep_ay ? AY.ptr : nullptr, AY.ld, Y.ptr, Y.ld, theta, R ? R->ptr : nullptr, R ? R->ld : 0,
+                         rn, yn),
+        "fe_gemm_nn_resid");
 }
 
 void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, double alpha,

@@ -186,9 +186,10 @@ void spmm(HipExecutor& ex, const HipCsrMatrix& A, const BlockRef& X, const Block
 // most `need` or at most twice the native FP32 product's (need <= 0: native
 // only). FP64 blocks always use the native product.
 constexpr int kProductAuto = -1;
-// G (m x p, ldg) = X^T Y
+// G (m x p, ldg) = X^T Y; symmetric: the caller asserts X^T Y is symmetric
+// (the Rayleigh-Ritz matrix Q^T (A Q)), so only its upper tiles are computed
 void gram(HipExecutor& ex, int mode, const BlockRef& X, const BlockRef& Y, double* G, int64_t ldg,
-          const double* w = nullptr, double need = 0.0);
+          const double* w = nullptr, double need = 0.0, bool symmetric = false);
 // Y = X C with C upper triangular (m x m, FP64 column-major; nothing below
 // its diagonal is read): the X R^{-1} of Cholesky QR
 void trmm_upper(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y,
@@ -197,6 +198,13 @@ void trmm_upper(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc
 // and their tuning apply when beta == 0 (a tuning run overwrites Y)
 void gemm(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& Y, double alpha,
           double beta, int mode = FE_GRAM_NATIVE, double need = 0.0);
+// Rotation with the fused Ritz residual: AY = X C (stored if keep_ay),
+// R = AY - Y diag(theta) (if R), rn[j] = ||R[:, j]||, yn[j] = ||Y[:, j]||
+// (theta, rn, yn device FP64, AY.cols of them); the product mode is chosen
+// as for gemm (the tuning times the plain product into AY)
+void gemm_resid(HipExecutor& ex, const BlockRef& X, const double* C, int64_t ldc, const BlockRef& AY, bool keep_ay,
+                const BlockRef& Y, const double* theta, const BlockRef* R, double* rn, double* yn,
+                int mode = FE_GRAM_NATIVE, double need = 0.0);
 void copy(HipExecutor& ex, const BlockRef& src, const BlockRef& dst);
 // dst[:, j] = src[:, idx[j]] in one pass (same precision)
 void gather_cols(HipExecutor& ex, const BlockRef& src, const std::vector<int>& idx, const BlockRef& dst);

@@ -47,6 +47,11 @@ typedef struct fe_csr_s* fe_csr;
 
 enum fe_prec { FE_F64 = 0, FE_F32 = 1 };
 enum fe_gram_mode { FE_GRAM_NATIVE = 0, FE_GRAM_BF16X3 = 1, FE_GRAM_BF16X6 = 2 };
+/* OR'ed into the mode of fe_gram / fe_gram_static: the caller asserts that the
+ * product is symmetric (m == p, e.g. the Rayleigh-Ritz matrix Q^T (A Q) of a
+ * symmetric A), so only the tiles on and above the diagonal are computed and
+ * their upper entries mirrored (exactly symmetric G). */
+enum fe_gram_flags { FE_GRAM_SYMMETRIC = 16 };
 enum fe_format { FE_FMT_CSR = 0, FE_FMT_SELL = 1, FE_FMT_BLOCKED = 2, FE_FMT_TILED = 3 };
 
 /* ---- errors / version ------------------------------------------------- */
@@ -183,6 +188,22 @@ int fe_gemm_nn(fe_ctx ctx, int prec, int64_t n, int m, int p, const void* X, int
 int fe_gemm_nn_mode(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
                     const double* C, int64_t ldc, void* Y, int64_t ldy, double alpha, double beta);
 
+/* Rotation with the Ritz residual fused into its epilogue (replaces the
+ * AY = AQ C product of src/core/RitzExtraction.cpp:122-124 followed by
+ * R = AY - Y Theta and the two column_norms passes, :125-139):
+ *   AY[:, 0:p] = X[:, 0:m] C          (stored only when AY is not NULL)
+ *   R[:, j] = AY[:, j] - theta[j] Y[:, j]   (R may be NULL: norms only)
+ *   rnorm[j] = ||R[:, j]||, ynorm[j] = ||Y[:, j]||   (device FP64, j < p)
+ * Y (n x p, read) is the rotated basis Q C of the same extraction; theta holds
+ * p device FP64 values. The product runs in `mode` as fe_gemm_nn_mode; R
+ * rounds to the block precision before it is squared, as fe_ritz_residuals.
+ * The column sums run per row tile of the product and over the tiles in a
+ * fixed order (reproducible; the summation order differs from
+ * fe_ritz_residuals'). No output may alias X, Y or another output. */
+int fe_gemm_nn_resid(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, const void* X, int64_t ldx,
+                     const double* C, int64_t ldc, void* AY, int64_t lday, const void* Y, int64_t ldy,
+                     const double* theta, void* R, int64_t ldr, double* rnorm, double* ynorm);
+
 /* Y[:, 0:m] = X[:, 0:m] * C with C upper triangular m x m (FP64 column-major,
  * ldc; nothing below its diagonal is read): the X R^{-1} of Cholesky QR, whose
  * triangular solve (src/core/Orthogonalization.cpp:49-57,185-246) is done as a
@@ -204,6 +225,10 @@ int fe_potrf(fe_ctx ctx, int k, double* G, int64_t ldg);
 /* Rinv = (L^T)^{-1} (upper triangular k x k, column-major, ld k) from the lower
  * Cholesky factor L; the triangular solve X R^{-1} then runs as fe_gemm_nn. */
 int fe_chol_rinv(fe_ctx ctx, int k, const double* L, int64_t ldl, double* Rinv, int64_t ldr);
+/* *out = ||G - I||_F over the leading m x m block (G device FP64 column-major,
+ * out on the host; synchronizes): the distance from orthonormality of a block
+ * whose Gram is G. Decides whether one Cholesky QR pass suffices. */
+int fe_identity_deviation(fe_ctx ctx, int m, const double* G, int64_t ldg, double* out);
 /* Symmetric eigensolver for the projected Rayleigh-Ritz matrix (replaces
  * Eigen::SelfAdjointEigenSolver, src/core/RitzExtraction.cpp:103-110, and the
  * CUDA dsyevd, src/gpu/CudaLanczosWorkspace.cpp:155):

@@ -703,6 +703,99 @@ def test_colnorms_and_ritz_residuals(dev, prec):
     assert rel(cn.download(), np.linalg.norm(Yr, axis=0)) < tol
 
 
+RESID_MODES = [(_fe.F64, _fe.GRAM_NATIVE), (_fe.F32, _fe.GRAM_NATIVE), (_fe.F32, _fe.GRAM_BF16X3),
+               (_fe.F32, _fe.GRAM_BF16X6)]
+
+
+@pytest.mark.parametrize("n,m,p", [(12345, 37, 37), (20000, 512, 256), (3001, 96, 48), (777, 130, 130), (5, 8, 3),
+                                   (100003, 64, 64), (4099, 200, 70)])
+@pytest.mark.parametrize("prec,mode", RESID_MODES)
+def test_gemm_nn_resid(dev, n, m, p, prec, mode):
+    """Rotation with the fused Ritz residual (fe_gemm_nn_resid) against the
+    unfused pair it replaces (fe_gemm_nn_mode into AY, then
+    fe_ritz_residuals): AY and R bitwise (the same product kernel and element
+    formula), the norms within summation-order rounding; the variants without
+    AY or R give bitwise the same norms; and the norms against numpy on the
+    same rounded data."""
+    rng = np.random.default_rng(n + m + p)
+    X = rng.uniform(-1, 1, (n, m))
+    Cm = rng.uniform(-1, 1, (m, p)) / np.sqrt(m)
+    Y = rng.uniform(-1, 1, (n, p))
+    th = rng.uniform(-2, 2, p)
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    Xd, Yd, Cd, thd = dev.put(X, dt), dev.put(Y, dt), dev.put(Cm, np.float64, order="F"), dev.put(th)
+    AYp, Rp = dev.buffer((n, p), dt), dev.buffer((n, p), dt)
+    rn0, yn0 = dev.buffer((p,)), dev.buffer((p,))
+    _fe.check(L().fe_gemm_nn_mode(dev.ctx, mode, prec, n, m, p, Xd.ptr, m, Cd.ptr, m, AYp.ptr, p, 1.0, 0.0))
+    _fe.check(L().fe_ritz_residuals(dev.ctx, prec, n, p, Yd.ptr, p, AYp.ptr, p, thd.ptr, Rp.ptr, p, rn0.ptr, yn0.ptr))
+    AYf, Rf = dev.buffer((n, p), dt), dev.buffer((n, p), dt)
+    rn1, yn1 = dev.buffer((p,)), dev.buffer((p,))
+    _fe.check(L().fe_gemm_nn_resid(dev.ctx, mode, prec, n, m, p, Xd.ptr, m, Cd.ptr, m, AYf.ptr, p, Yd.ptr, p,
+                                   thd.ptr, Rf.ptr, p, rn1.ptr, yn1.ptr))
+    assert np.array_equal(AYf.download(), AYp.download())
+    assert np.array_equal(Rf.download(), Rp.download())
+    r0, r1 = rn0.download(), rn1.download()
+    assert rel(r1, r0) <= 1e-13 and rel(yn1.download(), yn0.download()) <= 1e-13
+    rn2, yn2 = dev.buffer((p,)), dev.buffer((p,))
+    _fe.check(L().fe_gemm_nn_resid(dev.ctx, mode, prec, n, m, p, Xd.ptr, m, Cd.ptr, m, None, p, Yd.ptr, p,
+                                   thd.ptr, None, p, rn2.ptr, yn2.ptr))
+    assert np.array_equal(rn2.download(), r1) and np.array_equal(yn2.download(), yn1.download())
+    Yr = Y.astype(dt).astype(np.float64)
+    R = AYp.download().astype(np.float64) - Yr * th
+    tol = 1e-13 if prec == _fe.F64 else 1e-6
+    assert rel(r1, np.linalg.norm(R, axis=0)) < tol
+    assert rel(yn1.download(), np.linalg.norm(Yr, axis=0)) < tol
+
+
+def test_gemm_nn_resid_rejects_aliasing(dev):
+    n, m, p = 100, 8, 8
+    Xd, Yd = dev.buffer((n, m)), dev.buffer((n, p))
+    Cd, thd, rn, yn = dev.buffer((m, p), order="F"), dev.buffer((p,)), dev.buffer((p,)), dev.buffer((p,))
+    for ay, r in ((Xd, None), (None, Yd), (Yd, None)):
+        rc = L().fe_gemm_nn_resid(dev.ctx, 0, _fe.F64, n, m, p, Xd.ptr, m, Cd.ptr, m, ay.ptr if ay else None, p,
+                                  Yd.ptr, p, thd.ptr, r.ptr if r else None, p, rn.ptr, yn.ptr)
+        assert rc == -2
+    # an empty inner dimension is rejected (there is no product to fuse the residual into)
+    assert L().fe_gemm_nn_resid(dev.ctx, 0, _fe.F64, n, 0, p, Xd.ptr, m, Cd.ptr, m, None, p, Yd.ptr, p, thd.ptr,
+                                None, p, rn.ptr, yn.ptr) == -2
+
+
+@pytest.mark.parametrize("g,m", [(20, 512), (14, 65), (20, 144), (9, 5), (30, 256), (24, 130)])
+@pytest.mark.parametrize("prec,mode", RESID_MODES)
+def test_gram_symmetric_product(dev, g, m, prec, mode):
+    """FE_GRAM_SYMMETRIC on the Rayleigh-Ritz matrix Q^T (A Q) of a symmetric
+    A: exactly symmetric output, within the mode's product accuracy of the
+    FP64 product of the same rounded data (which is symmetric up to rounding);
+    the flag with m != p is rejected."""
+    A = G.hamiltonian_27pt(g, seed=4, wells=5)
+    n = A.shape[0]
+    rng = np.random.default_rng(g + m)
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    Q = rng.uniform(-1, 1, (n, m)).astype(dt).astype(np.float64)
+    AQ = (A @ Q).astype(dt).astype(np.float64)
+    Qd, AQd = dev.put(Q, dt), dev.put(AQ, dt)
+    Gd = dev.buffer((m, m), np.float64, order="F")
+    _fe.check(L().fe_gram(dev.ctx, mode | _fe.GRAM_SYMMETRIC, prec, n, m, m, Qd.ptr, m, AQd.ptr, m, None, Gd.ptr, m))
+    S = Gd.download()
+    assert np.array_equal(S, S.T)
+    ref = Q.T @ AQ
+    scale = np.max(np.abs(Q).T @ np.abs(AQ))
+    tol = {(_fe.F64, 0): 1e-14, (_fe.F32, 0): 1e-6, (_fe.F32, 1): 2.0 ** -14, (_fe.F32, 2): 1e-6}[(prec, mode)]
+    assert np.max(np.abs(S - ref)) <= tol * scale
+    assert L().fe_gram(dev.ctx, mode | _fe.GRAM_SYMMETRIC, prec, n, m, m - 1, Qd.ptr, m, AQd.ptr, m, None,
+                       Gd.ptr, m) == -2
+
+
+@pytest.mark.parametrize("m", [1, 7, 256, 300, 512])
+def test_identity_deviation(dev, m):
+    rng = np.random.default_rng(m)
+    Gm = np.eye(m) + 1e-3 * rng.standard_normal((m, m))
+    Gd = dev.put(Gm, np.float64, order="F")
+    out = ctypes.c_double()
+    _fe.check(L().fe_identity_deviation(dev.ctx, m, Gd.ptr, m, ctypes.byref(out)))
+    assert abs(out.value - np.linalg.norm(Gm - np.eye(m))) <= 1e-14 * np.linalg.norm(Gm - np.eye(m))
+
+
 def test_axpby_copy_upload(dev):
     rng = np.random.default_rng(12)
     n, k = 1000, 9
