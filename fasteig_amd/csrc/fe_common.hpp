@@ -93,14 +93,15 @@ struct fe_csr_s {
   int tl_slots = 0;            // LDS row slots per workgroup; slot tl_slots is the zero row
   int64_t* tl_pencil = nullptr;  // first tile of each pencil (npencils + 1)
   int* tl_rows = nullptr;        // 16 rows per tile (-1: none)
-  int64_t* tl_rptr = nullptr;    // entries of each tile row (16 ntiles + 1)
-  int* tl_maxe = nullptr;        // largest row entry count of each tile
-  int* tl_eslot = nullptr;       // entry -> slot of its column
-  double* tl_ev64 = nullptr;
-  float* tl_ev32 = nullptr;
+  // entries: every row of a tile holds tl_maxe[tile] of them (its nonzeros in
+  // CSR order, then the zero slot with value 0), rows one after another
+  int64_t* tl_ebase = nullptr;   // first entry of each tile (ntiles + 1)
+  int* tl_maxe = nullptr;        // entries per row of each tile (longest row, rounded up to 4)
+  int2* tl_ent32 = nullptr;      // FP32 entries: (slot of its column, value bits)
+  int* tl_eslot = nullptr;       // FP64 entries: slot of its column
+  double* tl_ev64 = nullptr;     //               and value
   int64_t* tl_lptr = nullptr;    // X rows each tile loads (ntiles + 1)
-  int* tl_lslot = nullptr;
-  int* tl_lcol = nullptr;
+  int* tl_lpack = nullptr;       // per X row load: (LDS slot << 24) | row (rows below 2^24)
   int* tl_sslot = nullptr;       // slot of each tile row's own X row (-1: not in its tile)
   int tl_self = 0;               // every tile row's own X row is in its tile's slots
   int use_tl = 0;

@@ -486,8 +486,8 @@ int fe_csr_destroy(fe_csr A) {
   if (A->ctx) hipStreamSynchronize(A->ctx->stream);
   void* ptrs[] = {A->rowptr, A->colind, A->vals64, A->vals32, A->sell_ptr, A->sell_len,
                   A->sell_perm, A->sell_col, A->sell_v64, A->sell_v32, A->bg_bptr, A->bg_ucol, A->bg_v64, A->bg_v32,
-                  A->bg_sched, A->tl_pencil, A->tl_rows, A->tl_rptr, A->tl_maxe, A->tl_eslot, A->tl_ev64,
-                  A->tl_ev32, A->tl_lptr, A->tl_lslot, A->tl_lcol, A->tl_sslot};
+                  A->bg_sched, A->tl_pencil, A->tl_rows, A->tl_ebase, A->tl_maxe, A->tl_ent32, A->tl_eslot,
+                  A->tl_ev64, A->tl_lptr, A->tl_lpack, A->tl_sslot};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete A;

@@ -25,6 +25,7 @@
 #include "fe_common.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 namespace fe {
@@ -130,17 +131,27 @@ __device__ __forceinline__ void lds_barrier() {
 // SELF: each tile row's own X row is among its tile's slots (a stored
 // diagonal), so the beta X term reads it from LDS: no second read of X from
 // HBM and no registers holding it.
-template <typename T, bool O32, bool FULL, bool SELF>
+// FZ: the gamma Z term is present. Every iteration of the tile loop issues the
+// same loads (the next tile's entries and Z row are fetched also on the last
+// tile, from a clamped tile index): with a load under a branch the compiler's
+// wait before the first use of a load list covered every load of the previous
+// tile (vmcnt(2) instead of vmcnt(7)), so each tile waited for the entries and
+// Z rows issued one tile earlier.
+// Entries: every row of a tile holds maxe[tile] of them (padded with the zero
+// slot), so a lane reads its row's entry e0 + 16 h + sub without a bound check
+// or a select (lanes past the row read the next row's entries, which their
+// broadcasts never use; the host appends entries after the last tile).
+template <typename T, bool O32, bool FULL, bool SELF, bool FZ>
 __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void k_spmm_tile(
     const int64_t* __restrict__ pencil, const int* __restrict__ rows, const int* __restrict__ sslot,
-    const int64_t* __restrict__ rptr,
-    const int* __restrict__ maxe, const int* __restrict__ eslot, const T* __restrict__ ev,
-    const int64_t* __restrict__ lptr, const int* __restrict__ lslot, const int* __restrict__ lcol, int zslot, int k,
+    const int64_t* __restrict__ ebase, const int* __restrict__ maxe, const int2* __restrict__ ent32,
+    const int* __restrict__ eslot, const double* __restrict__ ev64,
+    const int64_t* __restrict__ lptr, const int* __restrict__ lpack, int zslot, int k,
     int nslices, const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta, const T* Z, int64_t ldz,
     T gamma, int64_t per_xcd, int64_t nwork) {
   extern __shared__ float4 xs[];  // (zslot + 1) slots of 16 x 16 bytes
   __shared__ int s_lp[TL_PENCIL + 1];             // load offsets of the pencil's tiles
-  __shared__ int s_rp[TL_PENCIL * TL_ROWS + 1];   // entry offsets of its tile rows
+  __shared__ int s_eb[TL_PENCIL + 1];             // entry offsets of its tiles
   __shared__ int s_rows[TL_PENCIL * TL_ROWS];     // its tile rows
   __shared__ int s_maxe[TL_PENCIL];
   // own-row slots of its tile rows (slots < TL_MAX_SLOTS fit a byte: with
@@ -156,23 +167,33 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
   const int sub = tid & 15;
   const int tr = tid >> 4;
   const int c = c0 + sub * V;  // this thread's first column
-  const bool fz = Z && gamma != T(0);
   const bool fx = beta != T(0);
   const bool full = FULL || c0 + CW <= k;  // the whole slice lies inside the block
   const int64_t t0 = pencil[p];
   const int nt = static_cast<int>(pencil[p + 1] - t0);  // at most TL_PENCIL
-  const int64_t lbase = lptr[t0], ebase = rptr[t0 * TL_ROWS];
+  const int64_t lbase = lptr[t0], eb0 = ebase[t0];
   for (int i = tid; i <= nt; i += 256) s_lp[i] = static_cast<int>(lptr[t0 + i] - lbase);
-  for (int i = tid; i <= nt * TL_ROWS; i += 256) s_rp[i] = static_cast<int>(rptr[t0 * TL_ROWS + i] - ebase);
+  for (int i = tid; i <= nt; i += 256) s_eb[i] = static_cast<int>(ebase[t0 + i] - eb0);
   for (int i = tid; i < nt * TL_ROWS; i += 256) s_rows[i] = rows[t0 * TL_ROWS + i];
   for (int i = tid; i < nt; i += 256) s_maxe[i] = maxe[t0 + i];
   if (SELF)
     for (int i = tid; i < nt * TL_ROWS; i += 256) s_self[i] = static_cast<unsigned char>(max(sslot[t0 * TL_ROWS + i], 0));
   if (tid < 16) xs[zslot * 16 + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int* ls = lslot + lbase;
-  const int* lc = lcol + lbase;
-  const int* es = eslot + ebase;
-  const T* ew = ev + ebase;
+  const int* lp = lpack + lbase;
+  const int2* e32 = ent32 + eb0;
+  const int* es = eslot + eb0;
+  const double* ew = ev64 + eb0;
+  // entry e of the pencil: slot and value
+  auto entry = [&](int e, int& slot, T& w) {
+    if constexpr (sizeof(T) == 4) {
+      const int2 r = e32[e];
+      slot = r.x;
+      w = __int_as_float(r.y);
+    } else {
+      slot = es[e];
+      w = ew[e];
+    }
+  };
 
   // part `part` (16 bytes) of X row `col` in this slice (zeros past k)
   auto load_x = [&](int col, int part) {
@@ -206,33 +227,23 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
     }
   };
   // per-tile registers: two chunks of 16 entries of this thread's row (as
-  // loaded: valid for lanes below cnt), and its rows of Z and X
+  // loaded), and its rows of Z and X
   struct Ahead {
     int ms[2];
     T mw[2];
-    int cnt;
     T z[V], x[SELF ? 1 : V];
   };
   auto fetch = [&](int tt, Ahead& a) {
-    const int e0 = s_rp[tt * TL_ROWS + tr];
-    const int cnt = s_rp[tt * TL_ROWS + tr + 1] - e0;
-    a.cnt = cnt;
+    const int e0 = s_eb[tt] + tr * s_maxe[tt];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      // the pencil's first entry stands in for a lane past the row's end (the
-      // host appends one entry after the last pencil's, so it always exists)
-      const int e = 16 * h + sub < cnt ? e0 + 16 * h + sub : 0;
-      a.ms[h] = es[e];
-      a.mw[h] = ew[e];
-    }
+    for (int h = 0; h < 2; ++h) entry(e0 + 16 * h + sub, a.ms[h], a.mw[h]);
     const int row = max(s_rows[tt * TL_ROWS + tr], 0);  // a padding row reads row 0, unused
-    if (fz) load_row(Z, ldz, row, a.z);
-    if constexpr (!SELF)
-      if (fx) load_row(X, ldx, row, a.x);
+    if constexpr (FZ) load_row(Z, ldz, row, a.z);
+    if constexpr (!SELF) load_row(X, ldx, row, a.x);
   };
-  // load list entries (slot, column) of tile tt for this thread's TL_PF
+  // load list entries (slot << 24 | row) of tile tt for this thread's TL_PF
   // loads, as loaded; bit j of *valid tells whether load j is one
-  auto fetch_list = [&](int tt, int (&sl)[TL_PF], int (&cl)[TL_PF], int& valid) {
+  auto fetch_list = [&](int tt, int (&pk)[TL_PF], int& valid) {
     const int l0 = tt < nt ? s_lp[tt] : 0;
     const int nl = tt < nt ? s_lp[tt + 1] - l0 : 0;
     valid = 0;
@@ -241,45 +252,52 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
       const int i = tid + 256 * j;
       const bool v = i < nl * 16;
       const int e = v ? l0 + (i >> 4) : 0;  // as for the entries: the pencil's first load stands in
-      sl[j] = ls[e];
-      cl[j] = lc[e];
+      pk[j] = lp[e];
       valid |= v ? 1 << j : 0;
     }
   };
   lds_barrier();
   // the first tile's X rows
-  for (int i = tid; i < s_lp[1] * 16; i += 256) xs[ls[i >> 4] * 16 + (i & 15)] = load_x(lc[i >> 4], i & 15);
+  for (int i = tid; i < s_lp[1] * 16; i += 256) {
+    const unsigned pk = static_cast<unsigned>(lp[i >> 4]);
+    xs[(pk >> 24) * 16 + (i & 15)] = load_x(pk & 0xffffff, i & 15);
+  }
   Ahead a[2];                 // entries / Z / X rows of the tiles of each parity
   float4 pre[2][TL_PF];       // new X rows of the tiles of each parity
   int slt[2][TL_PF];          // and their slots (slot * 16 + part, -1: none)
-  int isl[TL_PF], icl[TL_PF], ival;  // load list of the tile two ahead of the current one
+  // load lists of the tiles two ahead, by the parity of the tile that issues
+  // their loads (fixed registers per role: a copy of a register still being
+  // loaded would wait for the load)
+  int ipk[2][TL_PF], ival[2];
   // a list's loads: slots (-1 where none), and the X loads themselves (a lane
   // without a load reads X row 0)
-  auto issue = [&](const int (&sl)[TL_PF], const int (&cl)[TL_PF], int valid, int (&so)[TL_PF],
-                   float4 (&po)[TL_PF]) {
+  auto issue = [&](const int (&pk)[TL_PF], int valid, int (&so)[TL_PF], float4 (&po)[TL_PF]) {
 #pragma unroll
     for (int j = 0; j < TL_PF; ++j) {
       const bool v = (valid >> j) & 1;
       const int i = tid + 256 * j;
-      so[j] = v ? sl[j] * 16 + (i & 15) : -1;
-      po[j] = load_x(v ? cl[j] : 0, i & 15);
+      so[j] = v ? static_cast<int>(static_cast<unsigned>(pk[j]) >> 24) * 16 + (i & 15) : -1;
+      po[j] = load_x(v ? pk[j] & 0xffffff : 0, i & 15);
     }
   };
-  fetch(0, a[0]);
+  // the same load order as an iteration of the tile loop (the list of the
+  // tile two ahead, then the next tile's entries and Z row), so the waits at
+  // the top of the loop hold on the entry path as on the back edge
   {
-    int sl[TL_PF], cl[TL_PF], vl;
-    fetch_list(1, sl, cl, vl);
-    issue(sl, cl, vl, slt[1], pre[1]);
+    int pk[TL_PF], vl;
+    fetch_list(1, pk, vl);
+    issue(pk, vl, slt[1], pre[1]);
   }
-  fetch_list(2, isl, icl, ival);
+  fetch_list(2, ipk[1], ival[1]);
+  fetch(0, a[0]);
   lds_barrier();
   auto tile = [&](auto Pc, int tt) {
     constexpr int P = decltype(Pc)::value;
     // new X rows of tile tt + 2, the load list of tile tt + 3, and the
     // entries / Z / X rows of tile tt + 1
-    issue(isl, icl, ival, slt[P], pre[P]);
-    fetch_list(tt + 3, isl, icl, ival);
-    if (tt + 1 < nt) fetch(tt + 1, a[1 - P]);
+    issue(ipk[1 - P], ival[1 - P], slt[P], pre[P]);
+    fetch_list(tt + 3, ipk[P], ival[P]);
+    fetch(min(tt + 1, nt - 1), a[1 - P]);  // on the last tile: loads of its own, unused
     // products of tile tt
     const Ahead& cur = a[P];
     const int lim = s_maxe[tt];
@@ -290,18 +308,10 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
       int ms;
       T mw;
       if (ch < 32) {
-        const bool v = ch + sub < cur.cnt;
-        ms = v ? (ch == 0 ? cur.ms[0] : cur.ms[1]) : zslot;
-        mw = v ? (ch == 0 ? cur.mw[0] : cur.mw[1]) : T(0);
+        ms = ch == 0 ? cur.ms[0] : cur.ms[1];
+        mw = ch == 0 ? cur.mw[0] : cur.mw[1];
       } else {  // rows longer than 32 entries: later chunks read here
-        const int e0 = s_rp[tt * TL_ROWS + tr];
-        const int cnt = s_rp[tt * TL_ROWS + tr + 1] - e0;
-        ms = zslot;
-        mw = T(0);
-        if (ch + sub < cnt) {
-          ms = es[e0 + ch + sub];
-          mw = ew[e0 + ch + sub];
-        }
+        entry(s_eb[tt] + tr * lim + ch + sub, ms, mw);
       }
       // lim is a multiple of 4 (the host pads the count): whole chunks run
       // without guards, the last one in guarded groups of 4 entries
@@ -327,22 +337,25 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
       }
     }
     const int row = s_rows[tt * TL_ROWS + tr];
+    // the output is formed on every lane (a padding row or a lane past k
+    // discards it): the loaded Z / X values are then used on every path, and
+    // the next load into their registers needs no wait for them
+    T xr[V];  // this row's part of X for the beta term
+    if constexpr (SELF) {
+      const float4 f = xs[s_self[tt * TL_ROWS + tr] * 16 + sub];
+      __builtin_memcpy(xr, &f, 16);
+    } else {
+#pragma unroll
+      for (int q = 0; q < V; ++q) xr[q] = cur.x[q];
+    }
+    T out[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      out[q] = alpha * acc[q];
+      if (fx) out[q] += beta * xr[q];
+      if constexpr (FZ) out[q] += gamma * cur.z[q];
+    }
     if (row >= 0 && c < k) {
-      T xr[V];  // this row's part of X for the beta term
-      if constexpr (SELF) {
-        const float4 f = xs[s_self[tt * TL_ROWS + tr] * 16 + sub];
-        __builtin_memcpy(xr, &f, 16);
-      } else {
-#pragma unroll
-        for (int q = 0; q < V; ++q) xr[q] = cur.x[q];
-      }
-      T out[V];
-#pragma unroll
-      for (int q = 0; q < V; ++q) {
-        out[q] = alpha * acc[q];
-        if (fx) out[q] += beta * xr[q];
-        if (fz) out[q] += gamma * cur.z[q];
-      }
       T* y = elem<O32>(Y, row, ldy, c);
       if (full || c + V <= k) {
         nt_f4 o;
@@ -361,8 +374,10 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
       if (slt[1 - P][j] >= 0) xs[slt[1 - P][j]] = pre[1 - P][j];
     if (tt + 1 < nt) {  // rows past the TL_PF registers per thread
       const int l0 = s_lp[tt + 1], nl = s_lp[tt + 2] - l0;
-      for (int i = tid + 256 * TL_PF; i < nl * 16; i += 256)
-        xs[ls[l0 + (i >> 4)] * 16 + (i & 15)] = load_x(lc[l0 + (i >> 4)], i & 15);
+      for (int i = tid + 256 * TL_PF; i < nl * 16; i += 256) {
+        const unsigned pk = static_cast<unsigned>(lp[l0 + (i >> 4)]);
+        xs[(pk >> 24) * 16 + (i & 15)] = load_x(pk & 0xffffff, i & 15);
+      }
     }
     lds_barrier();
   };
@@ -380,20 +395,24 @@ static int launch_tile(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* 
   const int64_t nwork = A->tl_npencils * nslices;
   const int64_t nb8 = (nwork + 7) / 8 * 8;
   const size_t lds = static_cast<size_t>(A->tl_slots + 1) * 256;
-  const T* ev = sizeof(T) == 8 ? (const T*)A->tl_ev64 : (const T*)A->tl_ev32;
   // 32-bit offsets when every block spans less than 4 GiB
   const double lim = 4294967295.0 / sizeof(T);
   const bool o32 = static_cast<double>(A->n) * ldx < lim && static_cast<double>(A->n) * ldy < lim &&
                    (!Z || static_cast<double>(A->n) * ldz < lim);
   const bool full = k % CW == 0;
-  auto pick = [&](auto self) {
-    constexpr bool S = decltype(self)::value;
-    return o32 ? (full ? k_spmm_tile<T, true, true, S> : k_spmm_tile<T, true, false, S>)
-               : (full ? k_spmm_tile<T, false, true, S> : k_spmm_tile<T, false, false, S>);
+  const bool fz = Z && gamma != T(0);
+  auto pick = [&](auto self, auto zterm) {
+    constexpr bool S = decltype(self)::value, F = decltype(zterm)::value;
+    return o32 ? (full ? k_spmm_tile<T, true, true, S, F> : k_spmm_tile<T, true, false, S, F>)
+               : (full ? k_spmm_tile<T, false, true, S, F> : k_spmm_tile<T, false, false, S, F>);
   };
-  auto kern = A->tl_self ? pick(std::true_type{}) : pick(std::false_type{});
+  auto pick_z = [&](auto self) {
+    return fz ? pick(self, std::true_type{}) : pick(self, std::false_type{});
+  };
+  auto kern = A->tl_self ? pick_z(std::true_type{}) : pick_z(std::false_type{});
   hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nb8)), dim3(256), lds, ctx->stream, A->tl_pencil, A->tl_rows,
-                     A->tl_sslot, A->tl_rptr, A->tl_maxe, A->tl_eslot, ev, A->tl_lptr, A->tl_lslot, A->tl_lcol, A->tl_slots, k,
+                     A->tl_sslot, A->tl_ebase, A->tl_maxe, A->tl_ent32, A->tl_eslot, A->tl_ev64, A->tl_lptr,
+                     A->tl_lpack, A->tl_slots, k,
                      nslices, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, nwork);
   FE_CHECK_LAUNCH();
   return 0;
@@ -420,11 +439,11 @@ namespace {
 struct TilePlan {
   std::vector<int64_t> pencil{0};
   std::vector<int> rows;
-  std::vector<int64_t> rptr{0};
-  std::vector<int> maxe, eslot;
+  std::vector<int64_t> ebase{0};  // per tile: its first entry
+  std::vector<int> maxe, eslot;   // per tile: entries per row; per entry: slot
   std::vector<double> ev;
   std::vector<int64_t> lptr{0};
-  std::vector<int> lslot, lcol;
+  std::vector<int> lpack;  // per X row load: (slot << 24) | row
   std::vector<int> sslot;  // per tile row: slot of its own X row in its tile (-1: none)
   int slots = 0;
   bool self = true;        // every row's own X row is in its tile
@@ -508,25 +527,31 @@ bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, c
         slot_of[col] = freelist.back();
         freelist.pop_back();
         resident.push_back(col);
-        P.lslot.push_back(slot_of[col]);
-        P.lcol.push_back(col);
+        P.lpack.push_back(static_cast<int>((static_cast<unsigned>(slot_of[col]) << 24) | static_cast<unsigned>(col)));
       }
-      P.lptr.push_back(static_cast<int64_t>(P.lslot.size()));
+      P.lptr.push_back(static_cast<int64_t>(P.lpack.size()));
+      // every row of the tile holds the longest row's count rounded up to 4
+      // entries: its own in CSR order, then the zero slot (slot `slots`)
       int me = 0;
+      for (int r : t)
+        if (r >= 0) me = std::max(me, rp[r + 1] - rp[r]);
+      me = (me + 3) / 4 * 4;
       for (int r : t) {
         P.rows.push_back(r);
         P.sslot.push_back(r >= 0 ? slot_of[r] : -1);
         if (r >= 0 && slot_of[r] < 0) P.self = false;
-        if (r >= 0) {
-          for (int q = rp[r]; q < rp[r + 1]; ++q) {
-            P.eslot.push_back(slot_of[ci[q]]);
-            P.ev.push_back(v[q]);
-          }
-          me = std::max(me, rp[r + 1] - rp[r]);
+        const int len = r >= 0 ? rp[r + 1] - rp[r] : 0;
+        for (int q = 0; q < len; ++q) {
+          P.eslot.push_back(slot_of[ci[rp[r] + q]]);
+          P.ev.push_back(v[rp[r] + q]);
         }
-        P.rptr.push_back(static_cast<int64_t>(P.eslot.size()));
+        for (int q = len; q < me; ++q) {
+          P.eslot.push_back(slots);
+          P.ev.push_back(0.0);
+        }
       }
-      P.maxe.push_back((me + 3) / 4 * 4);  // padded entries read the zero slot
+      P.maxe.push_back(me);
+      P.ebase.push_back(static_cast<int64_t>(P.eslot.size()));
     }
   }
   return true;
@@ -542,6 +567,7 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   }
   const int n = A->n;
   if (n == 0) return 1;
+  if (n >= (1 << 24)) return 1;  // a load packs its row in 24 bits (with the slot in the top 8)
   std::vector<int> rp(n + 1), ci(A->nnz);
   std::vector<double> v(A->nnz);
   FE_HIP_TRY(hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
@@ -557,19 +583,26 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   make_tiles(n, line, plane, tiles, pencil);
   TilePlan P;
   if (!plan_slots(n, rp, ci, v, tiles, pencil, P)) return 1;
-  const double lpr = static_cast<double>(P.lslot.size()) / n;
+  const double lpr = static_cast<double>(P.lpack.size()) / n;
   if (loads_per_row) *loads_per_row = lpr;
   // worth it only when the loads per row fall below half the nonzeros per row
   if (lpr > 0.5 * static_cast<double>(A->nnz) / n) return 1;
-  // one more entry and one more load after the last pencil's: a kernel lane
-  // with nothing to read reads its pencil's first one, which then exists
-  // even for a pencil without entries or loads
-  P.eslot.push_back(0);
-  P.ev.push_back(0.0);
-  P.lslot.push_back(0);
-  P.lcol.push_back(0);
-  std::vector<float> ev32(P.ev.size());
-  for (size_t i = 0; i < P.ev.size(); ++i) ev32[i] = static_cast<float>(P.ev[i]);
+  // entries after the last tile's: a lane reads up to 31 entries past the
+  // start of its row (and 15 past its end); one load after the last pencil's:
+  // a lane with nothing to load reads its pencil's first one, which then
+  // exists even for a pencil without loads
+  for (int q = 0; q < 32; ++q) {
+    P.eslot.push_back(P.slots);
+    P.ev.push_back(0.0);
+  }
+  P.lpack.push_back(0);
+  std::vector<int2> ent32(P.ev.size());
+  for (size_t i = 0; i < P.ev.size(); ++i) {
+    const float w = static_cast<float>(P.ev[i]);
+    int bits;
+    std::memcpy(&bits, &w, 4);
+    ent32[i] = make_int2(P.eslot[i], bits);
+  }
   auto up = [&](auto** dst, const auto& src) -> int {
     using E = typename std::decay_t<decltype(src)>::value_type;
     const size_t bytes = sizeof(E) * std::max<size_t>(src.size(), 1);
@@ -579,15 +612,15 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
     return 0;
   };
   int rc = 0;
-  if ((rc = up(&A->tl_pencil, P.pencil)) || (rc = up(&A->tl_rows, P.rows)) || (rc = up(&A->tl_rptr, P.rptr)) ||
-      (rc = up(&A->tl_maxe, P.maxe)) || (rc = up(&A->tl_eslot, P.eslot)) || (rc = up(&A->tl_ev64, P.ev)) ||
-      (rc = up(&A->tl_ev32, ev32)) || (rc = up(&A->tl_lptr, P.lptr)) || (rc = up(&A->tl_lslot, P.lslot)) ||
-      (rc = up(&A->tl_lcol, P.lcol)) || (rc = up(&A->tl_sslot, P.sslot)) ||
+  if ((rc = up(&A->tl_pencil, P.pencil)) || (rc = up(&A->tl_rows, P.rows)) || (rc = up(&A->tl_ebase, P.ebase)) ||
+      (rc = up(&A->tl_maxe, P.maxe)) || (rc = up(&A->tl_ent32, ent32)) || (rc = up(&A->tl_eslot, P.eslot)) ||
+      (rc = up(&A->tl_ev64, P.ev)) || (rc = up(&A->tl_lptr, P.lptr)) || (rc = up(&A->tl_lpack, P.lpack)) ||
+      (rc = up(&A->tl_sslot, P.sslot)) ||
       (rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : -3)) {
     hipStreamSynchronize(ctx->stream);
-    void** arrays[] = {(void**)&A->tl_pencil, (void**)&A->tl_rows,  (void**)&A->tl_rptr, (void**)&A->tl_maxe,
-                       (void**)&A->tl_eslot,  (void**)&A->tl_ev64,  (void**)&A->tl_ev32, (void**)&A->tl_lptr,
-                       (void**)&A->tl_lslot,  (void**)&A->tl_lcol,  (void**)&A->tl_sslot};
+    void** arrays[] = {(void**)&A->tl_pencil, (void**)&A->tl_rows,  (void**)&A->tl_ebase, (void**)&A->tl_maxe,
+                       (void**)&A->tl_ent32,  (void**)&A->tl_eslot, (void**)&A->tl_ev64,  (void**)&A->tl_lptr,
+                       (void**)&A->tl_lpack,  (void**)&A->tl_sslot};
     for (void** a : arrays) {
       if (*a) hipFree(*a);
       *a = nullptr;
@@ -596,7 +629,7 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   }
   A->tl_ntiles = static_cast<int64_t>(tiles.size());
   A->tl_npencils = static_cast<int64_t>(P.pencil.size()) - 1;
-  A->tl_loads = static_cast<int64_t>(P.lslot.size()) - 1;  // without the appended stand-in
+  A->tl_loads = static_cast<int64_t>(P.lpack.size()) - 1;  // without the appended stand-in
   A->tl_slots = P.slots;
   A->tl_self = P.self ? 1 : 0;
   return 0;
