@@ -691,8 +691,10 @@ def main():
     extra["product_tuning"] = p.product_tuning()
     del p
 
-    # dominant kernel: fused Chebyshev step at the solve's shape (FP32, k = nev)
-    k = cfg["nev"]
+    # dominant kernel: the fused Chebyshev step at the width the solve's FP32
+    # cycles filter (the whole subspace: blksize, or 2 nev), in FP32; the
+    # step at k = nev is reported beside it
+    k = cfg["blksize"] or min(n, 2 * cfg["nev"])
     # the solve's own sparse format: what "auto" picks (tiled, else row-block, else CSR)
     for fmt in {"auto": ["tiled", "blocked", "csr"], "tiled": ["tiled"], "blocked": ["blocked"],
                 "csr": ["csr"], "sell": ["sell"]}[args.format]:
@@ -703,11 +705,21 @@ def main():
             continue
     per64, algo64 = spmm_roofline(A, k, _fe.F64, fmt=fmt)
     achieved = algo32 / (per32 * 1e-3) / 1e9
+    extra["fused_step_k"] = k
     extra["fused_step_fp32_ms"] = per32
     extra["fused_step_fp64_ms"] = per64
     extra["fused_step_fp64_gbs"] = algo64 / (per64 * 1e-3) / 1e9
     log(f"fused step fp32 k={k}: {per32:.3f} ms, {achieved:.0f} GB/s; fp64 {per64:.3f} ms "
         f"{extra['fused_step_fp64_gbs']:.0f} GB/s")
+    kn = cfg["nev"]
+    if kn != k:
+        p32n, a32n = spmm_roofline(A, kn, _fe.F32, fmt=fmt)
+        p64n, a64n = spmm_roofline(A, kn, _fe.F64, fmt=fmt)
+        extra[f"fused_step_k{kn}"] = {"fp32_ms": p32n, "fp32_gbs": a32n / (p32n * 1e-3) / 1e9,
+                                      "fp32_frac": a32n / (p32n * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                      "fp64_ms": p64n, "fp64_gbs": a64n / (p64n * 1e-3) / 1e9}
+        log(f"fused step k={kn}: fp32 {p32n:.3f} ms {a32n / (p32n * 1e-3) / 1e9:.0f} GB/s; "
+            f"fp64 {p64n:.3f} ms {a64n / (p64n * 1e-3) / 1e9:.0f} GB/s")
     # the cycle's Gram and rotation products at the solve's block width
     extra["dense_mfma"] = dense_roofline(n, cfg["blksize"] or min(n, 2 * cfg["nev"]))
 

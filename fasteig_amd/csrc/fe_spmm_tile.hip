@@ -37,6 +37,10 @@ typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-te
 constexpr int TL_ROWS = 16;   // rows per tile: 4 wavefronts x 4 rows of 16 lanes
 constexpr int TL_PF = 3;      // 16-byte X loads per thread held in registers per tile in flight
 constexpr int TL_PENCIL = 24;  // tiles per pencil at most
+#ifndef TL_BLOCK_EDGE
+#define TL_BLOCK_EDGE 4
+#endif
+constexpr int TL_BLOCK = TL_BLOCK_EDGE;  // patches per block edge in the pencil order of a grid
 constexpr int TL_MAX_SLOTS = 240;  // LDS slots: with the zero slot and the pencil offsets within 64 KB
 static_assert(TL_MAX_SLOTS < 256, "own-row slots are staged as bytes");
 
@@ -60,6 +64,14 @@ __device__ __forceinline__ P* elem(P* base, int row, int64_t ld, int col) {
                                     static_cast<uint32_t>(sizeof(P)));
   else
     return base + static_cast<int64_t>(row) * ld + col;
+}
+
+// base[e] for e >= 0 with the byte offset formed in 32 bits (the base stays
+// in scalar registers and the load takes a 32-bit vector offset)
+template <typename E, typename B>
+__device__ __forceinline__ E ld_off(const B* base, int e) {
+  return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(base) +
+                                     static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(E)));
 }
 
 // An entry's slot and value broadcast together: one 64-bit DPP move in FP32
@@ -124,10 +136,11 @@ __device__ __forceinline__ void lds_barrier() {
 // slots for a 27-point stencil), measured 2% slower. The pencil's row, entry
 // and load offsets are staged in LDS first.
 // FULL: every column slice lies inside the block (k a multiple of the slice
-// width), so no load needs a partial-slice path. The prefetch loads run
-// unconditionally, from a clamped address when their lane has nothing to
-// load, and validity is applied where the value is used: a load under a
-// branch, or merged with a constant, is waited for where the branches join.
+// width), so no load needs a partial-slice path. Every tile's load list holds
+// at least the TL_PF x 16 rows the registers take (the host pads it with loads
+// of a row into a spare slot after the zero slot), so the prefetch loads and
+// their LDS writes run without validity checks: a load under a branch, or
+// merged with a constant, is waited for where the branches join.
 // SELF: each tile row's own X row is among its tile's slots (a stored
 // diagonal), so the beta X term reads it from LDS: no second read of X from
 // HBM and no registers holding it.
@@ -149,7 +162,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
     const int64_t* __restrict__ lptr, const int* __restrict__ lpack, int zslot, int k,
     int nslices, const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta, const T* Z, int64_t ldz,
     T gamma, int64_t per_xcd, int64_t nwork) {
-  extern __shared__ float4 xs[];  // (zslot + 1) slots of 16 x 16 bytes
+  extern __shared__ float4 xs[];  // (zslot + 2) slots of 16 x 16 bytes: zslot holds zeros, zslot + 1 padding loads
   __shared__ int s_lp[TL_PENCIL + 1];             // load offsets of the pencil's tiles
   __shared__ int s_eb[TL_PENCIL + 1];             // entry offsets of its tiles
   __shared__ int s_rows[TL_PENCIL * TL_ROWS];     // its tile rows
@@ -180,18 +193,20 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
     for (int i = tid; i < nt * TL_ROWS; i += 256) s_self[i] = static_cast<unsigned char>(max(sslot[t0 * TL_ROWS + i], 0));
   if (tid < 16) xs[zslot * 16 + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int* lp = lpack + lbase;
+  auto ld32 = [&](const int* base, int e) { return ld_off<int>(base, e); };
   const int2* e32 = ent32 + eb0;
   const int* es = eslot + eb0;
   const double* ew = ev64 + eb0;
-  // entry e of the pencil: slot and value
+  // entry e of the pencil: slot and value (32-bit byte offsets from the
+  // pencil's base: the loads then take the base in SGPRs)
   auto entry = [&](int e, int& slot, T& w) {
     if constexpr (sizeof(T) == 4) {
-      const int2 r = e32[e];
+      const int2 r = ld_off<int2>(e32, e);
       slot = r.x;
       w = __int_as_float(r.y);
     } else {
-      slot = es[e];
-      w = ew[e];
+      slot = ld_off<int>(es, e);
+      w = ld_off<double>(ew, e);
     }
   };
 
@@ -234,7 +249,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
     T z[V], x[SELF ? 1 : V];
   };
   auto fetch = [&](int tt, Ahead& a) {
-    const int e0 = s_eb[tt] + tr * s_maxe[tt];
+    const int e0 = s_eb[tt] + static_cast<int>(__umul24(static_cast<uint32_t>(tr), static_cast<uint32_t>(s_maxe[tt])));
 #pragma unroll
     for (int h = 0; h < 2; ++h) entry(e0 + 16 * h + sub, a.ms[h], a.mw[h]);
     const int row = max(s_rows[tt * TL_ROWS + tr], 0);  // a padding row reads row 0, unused
@@ -242,19 +257,12 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
     if constexpr (!SELF) load_row(X, ldx, row, a.x);
   };
   // load list entries (slot << 24 | row) of tile tt for this thread's TL_PF
-  // loads, as loaded; bit j of *valid tells whether load j is one
-  auto fetch_list = [&](int tt, int (&pk)[TL_PF], int& valid) {
-    const int l0 = tt < nt ? s_lp[tt] : 0;
-    const int nl = tt < nt ? s_lp[tt + 1] - l0 : 0;
-    valid = 0;
+  // loads, as loaded (a tile past the pencil's last reads the last one's again:
+  // its rows then go after the last products, into slots nobody reads)
+  auto fetch_list = [&](int tt, int (&pk)[TL_PF]) {
+    const int l0 = s_lp[min(tt, nt - 1)] + (tid >> 4);
 #pragma unroll
-    for (int j = 0; j < TL_PF; ++j) {
-      const int i = tid + 256 * j;
-      const bool v = i < nl * 16;
-      const int e = v ? l0 + (i >> 4) : 0;  // as for the entries: the pencil's first load stands in
-      pk[j] = lp[e];
-      valid |= v ? 1 << j : 0;
-    }
+    for (int j = 0; j < TL_PF; ++j) pk[j] = ld32(lp, l0 + 16 * j);
   };
   lds_barrier();
   // the first tile's X rows
@@ -264,39 +272,36 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
   }
   Ahead a[2];                 // entries / Z / X rows of the tiles of each parity
   float4 pre[2][TL_PF];       // new X rows of the tiles of each parity
-  int slt[2][TL_PF];          // and their slots (slot * 16 + part, -1: none)
+  int slt[2][TL_PF];          // and their slots (slot * 16 + part)
   // load lists of the tiles two ahead, by the parity of the tile that issues
   // their loads (fixed registers per role: a copy of a register still being
   // loaded would wait for the load)
-  int ipk[2][TL_PF], ival[2];
-  // a list's loads: slots (-1 where none), and the X loads themselves (a lane
-  // without a load reads X row 0)
-  auto issue = [&](const int (&pk)[TL_PF], int valid, int (&so)[TL_PF], float4 (&po)[TL_PF]) {
+  int ipk[2][TL_PF];
+  // a list's loads: their LDS slots, and the X loads themselves
+  auto issue = [&](const int (&pk)[TL_PF], int (&so)[TL_PF], float4 (&po)[TL_PF]) {
 #pragma unroll
     for (int j = 0; j < TL_PF; ++j) {
-      const bool v = (valid >> j) & 1;
-      const int i = tid + 256 * j;
-      so[j] = v ? static_cast<int>(static_cast<unsigned>(pk[j]) >> 24) * 16 + (i & 15) : -1;
-      po[j] = load_x(v ? pk[j] & 0xffffff : 0, i & 15);
+      so[j] = static_cast<int>(static_cast<unsigned>(pk[j]) >> 24) * 16 + sub;
+      po[j] = load_x(pk[j] & 0xffffff, sub);
     }
   };
   // the same load order as an iteration of the tile loop (the list of the
   // tile two ahead, then the next tile's entries and Z row), so the waits at
   // the top of the loop hold on the entry path as on the back edge
   {
-    int pk[TL_PF], vl;
-    fetch_list(1, pk, vl);
-    issue(pk, vl, slt[1], pre[1]);
+    int pk[TL_PF];
+    fetch_list(1, pk);
+    issue(pk, slt[1], pre[1]);
   }
-  fetch_list(2, ipk[1], ival[1]);
+  fetch_list(2, ipk[1]);
   fetch(0, a[0]);
   lds_barrier();
   auto tile = [&](auto Pc, int tt) {
     constexpr int P = decltype(Pc)::value;
     // new X rows of tile tt + 2, the load list of tile tt + 3, and the
     // entries / Z / X rows of tile tt + 1
-    issue(ipk[1 - P], ival[1 - P], slt[P], pre[P]);
-    fetch_list(tt + 3, ipk[P], ival[P]);
+    issue(ipk[1 - P], slt[P], pre[P]);
+    fetch_list(tt + 3, ipk[P]);
     fetch(min(tt + 1, nt - 1), a[1 - P]);  // on the last tile: loads of its own, unused
     // products of tile tt
     const Ahead& cur = a[P];
@@ -370,8 +375,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
     lds_barrier();  // every thread is done with tile tt's slots
     // tile tt + 1's new X rows (issued one tile earlier) into their slots
 #pragma unroll
-    for (int j = 0; j < TL_PF; ++j)
-      if (slt[1 - P][j] >= 0) xs[slt[1 - P][j]] = pre[1 - P][j];
+    for (int j = 0; j < TL_PF; ++j) xs[slt[1 - P][j]] = pre[1 - P][j];
     if (tt + 1 < nt) {  // rows past the TL_PF registers per thread
       const int l0 = s_lp[tt + 1], nl = s_lp[tt + 2] - l0;
       for (int i = tid + 256 * TL_PF; i < nl * 16; i += 256) {
@@ -394,7 +398,7 @@ static int launch_tile(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* 
   const int nslices = (k + CW - 1) / CW;
   const int64_t nwork = A->tl_npencils * nslices;
   const int64_t nb8 = (nwork + 7) / 8 * 8;
-  const size_t lds = static_cast<size_t>(A->tl_slots + 1) * 256;
+  const size_t lds = static_cast<size_t>(A->tl_slots + 2) * 256;  // the slots, the zero slot and the spare slot
   // 32-bit offsets when every block spans less than 4 GiB
   const double lim = 4294967295.0 / sizeof(T);
   const bool o32 = static_cast<double>(A->n) * ldx < lim && static_cast<double>(A->n) * ldy < lim &&
@@ -445,6 +449,7 @@ struct TilePlan {
   std::vector<int64_t> lptr{0};
   std::vector<int> lpack;  // per X row load: (slot << 24) | row
   std::vector<int> sslot;  // per tile row: slot of its own X row in its tile (-1: none)
+  int64_t loads = 0;       // X row loads of the plan (without the padding loads)
   int slots = 0;
   bool self = true;        // every row's own X row is in its tile
 };
@@ -454,21 +459,24 @@ void make_tiles(int n, int64_t line, int64_t plane, std::vector<std::vector<int>
                 std::vector<int64_t>& pencil) {
   if (plane > 0 && line % 4 == 0 && (plane / line) % 4 == 0 && n % plane == 0) {
     // 4 x 4 patches; a pencil runs through up to 24 consecutive planes; the
-    // pencils of one range of planes are taken row of patches by row, so an
-    // XCD's contiguous share of pencils is a band of neighbouring patches
-    const int64_t nx = line / 4, ny = plane / line / 4, nz = n / plane, pz = TL_PENCIL;
+    // pencils of one range of planes are taken in blocks of TL_BLOCK x
+    // TL_BLOCK patches (row of blocks by row), so the pencils an XCD runs at
+    // one time cover a compact area whose shared halo rows its L2 serves
+    const int64_t nx = line / 4, ny = plane / line / 4, nz = n / plane, pz = TL_PENCIL, B = TL_BLOCK;
     for (int64_t z0 = 0; z0 < nz; z0 += pz)
-      for (int64_t py = 0; py < ny; ++py)
-        for (int64_t px = 0; px < nx; ++px) {
-          for (int64_t z = z0; z < std::min(nz, z0 + pz); ++z) {
-            std::vector<int> t;
-            for (int dy = 0; dy < 4; ++dy)
-              for (int dx = 0; dx < 4; ++dx)
-                t.push_back(static_cast<int>(z * plane + (4 * py + dy) * line + 4 * px + dx));
-            tiles.push_back(std::move(t));
-          }
-          pencil.push_back(static_cast<int64_t>(tiles.size()));
-        }
+      for (int64_t by = 0; by < ny; by += B)
+        for (int64_t bx = 0; bx < nx; bx += B)
+          for (int64_t py = by; py < std::min(ny, by + B); ++py)
+            for (int64_t px = bx; px < std::min(nx, bx + B); ++px) {
+              for (int64_t z = z0; z < std::min(nz, z0 + pz); ++z) {
+                std::vector<int> t;
+                for (int dy = 0; dy < 4; ++dy)
+                  for (int dx = 0; dx < 4; ++dx)
+                    t.push_back(static_cast<int>(z * plane + (4 * py + dy) * line + 4 * px + dx));
+                tiles.push_back(std::move(t));
+              }
+              pencil.push_back(static_cast<int64_t>(tiles.size()));
+            }
     return;
   }
   const int64_t per_pencil = TL_PENCIL;
@@ -522,6 +530,7 @@ bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, c
         }
       }
       resident.swap(kept);
+      const size_t first = P.lpack.size();
       for (int col : uni) {
         if (slot_of[col] >= 0) continue;
         slot_of[col] = freelist.back();
@@ -529,6 +538,12 @@ bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, c
         resident.push_back(col);
         P.lpack.push_back(static_cast<int>((static_cast<unsigned>(slot_of[col]) << 24) | static_cast<unsigned>(col)));
       }
+      P.loads += static_cast<int64_t>(P.lpack.size() - first);
+      // at least the TL_PF x 16 loads the kernel's registers take per tile:
+      // the rest load the tile's first column into the spare slot (slots + 1)
+      const int pad_row = uni.empty() ? 0 : uni.front();
+      while (P.lpack.size() - first < static_cast<size_t>(TL_PF * 16))
+        P.lpack.push_back(static_cast<int>((static_cast<unsigned>(slots + 1) << 24) | static_cast<unsigned>(pad_row)));
       P.lptr.push_back(static_cast<int64_t>(P.lpack.size()));
       // every row of the tile holds the longest row's count rounded up to 4
       // entries: its own in CSR order, then the zero slot (slot `slots`)
@@ -583,19 +598,16 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   make_tiles(n, line, plane, tiles, pencil);
   TilePlan P;
   if (!plan_slots(n, rp, ci, v, tiles, pencil, P)) return 1;
-  const double lpr = static_cast<double>(P.lpack.size()) / n;
+  const double lpr = static_cast<double>(P.loads) / n;
   if (loads_per_row) *loads_per_row = lpr;
   // worth it only when the loads per row fall below half the nonzeros per row
   if (lpr > 0.5 * static_cast<double>(A->nnz) / n) return 1;
   // entries after the last tile's: a lane reads up to 31 entries past the
-  // start of its row (and 15 past its end); one load after the last pencil's:
-  // a lane with nothing to load reads its pencil's first one, which then
-  // exists even for a pencil without loads
+  // start of its row (and 15 past its end)
   for (int q = 0; q < 32; ++q) {
     P.eslot.push_back(P.slots);
     P.ev.push_back(0.0);
   }
-  P.lpack.push_back(0);
   std::vector<int2> ent32(P.ev.size());
   for (size_t i = 0; i < P.ev.size(); ++i) {
     const float w = static_cast<float>(P.ev[i]);
@@ -629,7 +641,7 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   }
   A->tl_ntiles = static_cast<int64_t>(tiles.size());
   A->tl_npencils = static_cast<int64_t>(P.pencil.size()) - 1;
-  A->tl_loads = static_cast<int64_t>(P.lpack.size()) - 1;  // without the appended stand-in
+  A->tl_loads = P.loads;
   A->tl_slots = P.slots;
   A->tl_self = P.self ? 1 : 0;
   return 0;
