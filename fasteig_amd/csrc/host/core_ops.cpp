@@ -211,9 +211,15 @@ std::vector<int> select_indices(const std::vector<double>& v, const std::string&
   return idx;
 }
 
+void rotate_rest(Workspace& ws, int m, const BlockRef& Q, int from, const BlockRef& Y) {
+  if (from >= Y.cols) return;
+  gemm(ws.ex, Q.range(0, m), ws.C.data() + static_cast<int64_t>(from) * ws.cap, ws.cap, Y.range(from, Y.cols - from),
+       1.0, 0.0, ws.gram_mode, ws.need);
+}
+
 RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
                        const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R,
-                       int res_count, bool keep_ay) {
+                       int res_count, bool keep_ay, int rotate_count) {
   HipExecutor& ex = ws.ex;
   if (m <= 0 || requested <= 0 || requested > m)
     throw std::invalid_argument("extract_projected_ritz_pairs: invalid active/requested Ritz dimensions.");
@@ -236,9 +242,11 @@ RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& 
   out.values.resize(requested);
   for (int j = 0; j < requested; ++j) out.values[j] = w[idx[j]];
   check(fe_memcpy_h2d(ex.ctx(), ws.theta.data(), out.values.data(), sizeof(double) * nr), "h2d(theta)");
-  // Y = Q C, then AY = (AQ) C for C's first nr columns with R = AY - Y Theta
-  // and the column norms in the same product's epilogue
-  gemm(ex, Q.range(0, m), ws.C.data(), ws.cap, Y, 1.0, 0.0, ws.gram_mode, ws.need);
+  // Y = Q C (the first rotate_count columns now; rotate_rest does the others),
+  // then AY = (AQ) C for C's first nr columns with R = AY - Y Theta and the
+  // column norms in the same product's epilogue
+  const int now = rotate_count < 0 ? requested : std::max(nr, std::min(rotate_count, requested));
+  gemm(ex, Q.range(0, m), ws.C.data(), ws.cap, Y.range(0, now), 1.0, 0.0, ws.gram_mode, ws.need);
   const BlockRef Yr = Y.range(0, nr);
   gemm_resid(ex, AQ.range(0, m), ws.C.data(), ws.cap, AY, keep_ay, Yr, ws.theta.data(), R, ws.rn.data(),
              ws.yn.data(), ws.gram_mode, ws.need);

@@ -70,9 +70,15 @@ struct RitzPairs {
 // The residual and its norms come from the epilogue of the AY product
 // (fe_gemm_nn_resid); keep_ay false leaves AY unwritten (its block is then
 // only the tuning target of that product).
+// rotate_count >= 0: only the first max(res_count, rotate_count) columns of
+// Y are rotated here; rotate_rest(ws, m, Q, from, Y) rotates Y's columns from
+// `from` on later (with the coefficients kept in ws.C and Q unchanged), so a
+// solve that stops after this extraction skips the rotation of its guard
+// columns.
 RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
                        const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R,
-                       int res_count = -1, bool keep_ay = true);
+                       int res_count = -1, bool keep_ay = true, int rotate_count = -1);
+void rotate_rest(Workspace& ws, int m, const BlockRef& Q, int from, const BlockRef& Y);
 
 // Stable index sort by `which` (include/fasteig_solver.hpp:59-82; "be" from
 // src/core/RitzExtraction.cpp:162-179).
