@@ -46,7 +46,7 @@ struct fe_ctx_s {
   int cus = 0;
   size_t hbm = 0;
   char name[128] = {0};
-  fe::Scratch ws[6];           // independent scratch slots (4: split coefficients of the BF16 products, 5: small results)
+  fe::Scratch ws[7];           // independent scratch slots (4: split coefficients of the BF16 products, 5: small results, 6: column-sum segments)
   fe::BlockCache cache;        // fe_alloc / fe_free reuse
   int* info_dev = nullptr;     // small device status words
   void* gram_static = nullptr; // device address of fe_gram_static's fixed scratch on this device

@@ -243,8 +243,40 @@ inline unsigned grid_for(int64_t total) {
   return static_cast<unsigned>(b);
 }
 
+// Partial sums of a segment of the chunks: seg[s * k + j] = sum over the
+// chunks c of segment s (contiguous, in order) of part[c * k + j]; 64 columns
+// x 4 chunk residues per workgroup, the residues added in order.
+__global__ __launch_bounds__(256) void k_reduce_seg(int chunks, int k, int per_seg, const double* __restrict__ part,
+                                                    double* __restrict__ seg) {
+  __shared__ double red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + tx;
+  const int c0 = blockIdx.y * per_seg, c1 = min(chunks, c0 + per_seg);
+  double s = 0.0;
+  if (j < k)
+    for (int c = c0 + ty; c < c1; c += 4) s += part[(int64_t)c * k + j];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && j < k) seg[(int64_t)blockIdx.y * k + j] = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
+}
+
 int reduce_cols_sqrt(fe_ctx ctx, int chunks, int k, const double* part, double* out) {
   if (k <= 0) return 0;
+  // many chunks (a row tile each, from the fused residual of a tall product):
+  // first segments of 64 chunks side by side, then the segments in order
+  constexpr int PER = 64;
+  if (chunks > 2 * PER) {
+    const int nseg = (chunks + PER - 1) / PER;
+    void* ws = nullptr;
+    if (int rc = scratch(ctx, 6, sizeof(double) * (size_t)nseg * k, &ws)) return rc;
+    hipLaunchKernelGGL(k_reduce_seg, dim3(blocks_for(k, 64), nseg), dim3(256), 0, ctx->stream, chunks, k, PER, part,
+                       (double*)ws);
+    FE_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 64)), dim3(256), 0, ctx->stream, nseg, k, (const double*)ws,
+                       out);
+    FE_CHECK_LAUNCH();
+    return 0;
+  }
   hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 64)), dim3(256), 0, ctx->stream, chunks, k, part, out);
   FE_CHECK_LAUNCH();
   return 0;
@@ -571,12 +603,8 @@ static int colsq_launch(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, i
                          nullptr, 0, nullptr, nullptr, 0, nullptr, part_y, rows_per);
   }
   FE_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 64)), dim3(256), 0, ctx->stream, (int)chunks, k,
-                     part_y, ynorm);
-  if (res)
-    hipLaunchKernelGGL(k_reduce_cols, dim3(blocks_for(k, 64)), dim3(256), 0, ctx->stream, (int)chunks, k,
-                       part_r, rnorm);
-  FE_CHECK_LAUNCH();
+  if (int rc2 = reduce_cols_sqrt(ctx, (int)chunks, k, part_y, ynorm)) return rc2;
+  if (res) return reduce_cols_sqrt(ctx, (int)chunks, k, part_r, rnorm);
   return 0;
 }
 
