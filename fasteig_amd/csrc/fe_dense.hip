@@ -838,6 +838,91 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(int64_t n, int m, int p, con
   }  // pass (the next pass's first barrier orders its LDS writes after these reads)
 }
 
+// ---------------------------------------------------------------- one-column products
+// Gram against one column, g = X^T diag(w) y (X n x m, m <= GV_MAX): a
+// thread sums its rows' products for all m columns in registers, the 256
+// threads of a workgroup add theirs in a fixed tree order, one partial per
+// workgroup chunk of rows; k_gram_vec_reduce adds the chunk partials in order.
+// The MFMA Gram's 64 x 64 tiles would carry one useful column in 64.
+constexpr int GV_MAX = 32;
+
+template <typename T, int MM>
+__global__ __launch_bounds__(256) void k_gram_vec(int64_t n, int m, const T* __restrict__ X, int64_t ldx,
+                                                  const T* __restrict__ y, int64_t ldy, const double* __restrict__ w,
+                                                  int64_t rows_per, double* __restrict__ part) {
+  __shared__ double red[4][MM];
+  const int64_t r0 = blockIdx.x * rows_per, r1 = min(n, r0 + rows_per);
+  double acc[MM];
+#pragma unroll
+  for (int j = 0; j < MM; ++j) acc[j] = 0.0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    double yv = static_cast<double>(y[i * ldy]);
+    if (w) yv *= w[i];
+    const T* xr = X + i * ldx;
+#pragma unroll
+    for (int j = 0; j < MM; ++j)
+      if (j < m) acc[j] += static_cast<double>(xr[j]) * yv;
+  }
+  // each column's sum over the wavefront by shuffles, then the four
+  // wavefronts in order
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < MM; ++j) {
+    const double v = wave_sum(acc[j]);
+    if (lane == 0) red[wave][j] = v;
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < m)
+    part[(int64_t)blockIdx.x * m + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// G[j] = sum over the chunks of part[c * m + j], j < m <= GV_MAX: 8 chunk
+// residues of 32 columns, each residue in chunk order, then the residues in
+// order
+__global__ __launch_bounds__(256) void k_gram_vec_reduce(int chunks, int m, const double* __restrict__ part,
+                                                         double* G) {
+  __shared__ double red[8][GV_MAX];
+  const int j = threadIdx.x & (GV_MAX - 1), q = threadIdx.x / GV_MAX;
+  double s = 0.0;
+  if (j < m)
+    for (int c = q; c < chunks; c += 8) s += part[(int64_t)c * m + j];
+  red[q][j] = s;
+  __syncthreads();
+  if (q == 0 && j < m) {
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) t += red[r][j];
+    G[j] = t;
+  }
+}
+
+// y = alpha X c + beta y for one column c (m <= 32 FP64 values, rounded to T
+// as in k_gemm_nn): a workgroup stages 128 rows through LDS with coalesced
+// loads (consecutive threads read consecutive elements of the row-major
+// block), then a thread sums its row
+constexpr int GEMV_ROWS = 128, GEMV_MAX = 32;
+template <typename T>
+__global__ __launch_bounds__(GEMV_ROWS) void k_gemv(int64_t n, int m, const T* __restrict__ X, int64_t ldx,
+                                                    const double* __restrict__ c, T* y, int64_t ldy, T alpha,
+                                                    T beta) {
+  __shared__ T xs[GEMV_ROWS * (GEMV_MAX + 1)];
+  __shared__ T cs[GEMV_MAX];
+  const int64_t r0 = blockIdx.x * (int64_t)GEMV_ROWS;
+  const int rows = static_cast<int>(min<int64_t>(GEMV_ROWS, n - r0));
+  if (static_cast<int>(threadIdx.x) < m) cs[threadIdx.x] = static_cast<T>(c[threadIdx.x]);
+  for (int e = threadIdx.x; e < rows * m; e += GEMV_ROWS) {
+    const int r = e / m, j = e - r * m;
+    xs[r * (GEMV_MAX + 1) + j] = X[(r0 + r) * ldx + j];
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) >= rows) return;
+  T acc = T(0);
+  for (int j = 0; j < m; ++j) acc += xs[threadIdx.x * (GEMV_MAX + 1) + j] * cs[j];
+  T& out = y[(r0 + threadIdx.x) * ldy];
+  out = beta == T(0) ? alpha * acc : alpha * acc + beta * out;
+}
+
 }  // namespace fe
 
 using namespace fe;
@@ -906,6 +991,30 @@ static int gram_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, con
     }
     return 0;
   };
+  if (mode == FE_GRAM_NATIVE && p == 1 && m <= GV_MAX && !fixed) {
+    // one column (the reorthogonalization of a single vector against a
+    // narrow basis): row chunks of about 2,048 rows, at most 1,024 of them
+    const int64_t chunks = std::min<int64_t>(1024, std::max<int64_t>(1, (n + 2047) / 2048));
+    const int64_t rows_per = (n + chunks - 1) / chunks;
+    void* ws = nullptr;
+    if (int rc = fe::scratch(ctx, 0, sizeof(double) * (size_t)chunks * m, &ws)) return rc;
+    auto launch = [&](auto mm) {
+      constexpr int MM = decltype(mm)::value;
+      if (prec == FE_F64)
+        hipLaunchKernelGGL((k_gram_vec<double, MM>), dim3((unsigned)chunks), dim3(256), 0, ctx->stream, n, m,
+                           (const double*)X, ldx, (const double*)Y, ldy, w, rows_per, (double*)ws);
+      else
+        hipLaunchKernelGGL((k_gram_vec<float, MM>), dim3((unsigned)chunks), dim3(256), 0, ctx->stream, n, m,
+                           (const float*)X, ldx, (const float*)Y, ldy, w, rows_per, (double*)ws);
+    };
+    if (m <= 8) launch(std::integral_constant<int, 8>{});
+    else if (m <= 16) launch(std::integral_constant<int, 16>{});
+    else launch(std::integral_constant<int, GV_MAX>{});
+    FE_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_gram_vec_reduce, dim3(1), dim3(256), 0, ctx->stream, (int)chunks, m, (const double*)ws, G);
+    FE_CHECK_LAUNCH();
+    return 0;
+  }
   if (mode == FE_GRAM_NATIVE) {
     // 64 x 64 tiles (rows split over the waves) for thin or small products,
     // 128 x 128 tiles otherwise
@@ -1046,6 +1155,17 @@ static int gemm_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, con
     hipLaunchKernelGGL(kern, dim3((unsigned)nb8), dim3(256), 0, ctx->stream, n, m, p, (const float*)X, ldx,
                        (const __bf16*)sp, mpad, (float*)Y, ldy, (float)alpha, (float)beta, tiles_c, nb8 / 8, ntiles,
                        vec, upper, ra);
+    FE_CHECK_LAUNCH();
+    return 0;
+  }
+  if (p == 1 && m <= GEMV_MAX && !rh && !upper) {  // one column against a narrow block
+    const unsigned g = static_cast<unsigned>((n + GEMV_ROWS - 1) / GEMV_ROWS);
+    if (prec == FE_F64)
+      hipLaunchKernelGGL(k_gemv<double>, dim3(g), dim3(GEMV_ROWS), 0, ctx->stream, n, m, (const double*)X, ldx, C,
+                         (double*)Y, ldy, alpha, beta);
+    else
+      hipLaunchKernelGGL(k_gemv<float>, dim3(g), dim3(GEMV_ROWS), 0, ctx->stream, n, m, (const float*)X, ldx, C,
+                         (float*)Y, ldy, (float)alpha, (float)beta);
     FE_CHECK_LAUNCH();
     return 0;
   }

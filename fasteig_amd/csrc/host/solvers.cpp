@@ -112,7 +112,12 @@ FASTEIG_INT ChebyshevSolver::solve() {
   Block V(ex, n, capacity, prec), F(ex, n, capacity, prec), P(ex, n, capacity, prec);
   Block Q(ex, n, capacity, prec), AQ(ex, n, capacity, prec), AY(ex, n, capacity, prec);
   start_block(ex, options_, n, capacity, nev, seed, V, "ChebyshevSolver");
+  // the random start block is well conditioned (kappa near 1 for n >> m) and
+  // only spans the first filter's input: in the mixed path its Cholesky QR
+  // takes the products of a first cycle (about 16 bits)
+  ws.need = mixed ? 1e-4 : 0.0;
   if (hip::cholesky_qr(ws, V, Q) != 0) throw std::runtime_error("ChebyshevSolver: initial Cholesky QR failed.");
+  ws.need = 0.0;
 
   double lower = 0.0, upper = 0.0;
   hip::lanczos_bounds(ex, A, sign, seed, lower, upper);

@@ -409,7 +409,8 @@ def test_spmm_strided_subblock(dev, mats):
 
 @pytest.mark.parametrize("n,m,p", [(1000, 16, 16), (5000, 48, 40), (20000, 96, 96), (3000, 200, 70),
                                    (777, 5, 3), (100000, 256, 256), (40000, 512, 512), (200003, 192, 48),
-                                   (50000, 144, 144), (3001, 65, 64), (4000, 161, 150)])
+                                   (50000, 144, 144), (3001, 65, 64), (4000, 161, 150), (884736, 20, 1),
+                                   (100003, 32, 1), (5000, 1, 1), (777, 9, 1), (3000, 33, 1)])
 @pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
 def test_gram(dev, n, m, p, prec):
     rng = np.random.default_rng(n + m)
@@ -574,7 +575,7 @@ def test_trmm_upper(dev, n, m, prec):
 
 @pytest.mark.parametrize("n,m,p", [(1000, 16, 16), (5000, 48, 10), (20000, 96, 96), (3333, 200, 70),
                                    (100000, 256, 256), (7, 3, 2), (200001, 192, 48), (1001, 48, 64),
-                                   (999, 20, 65), (300, 33, 1)])
+                                   (999, 20, 65), (300, 33, 1), (100003, 20, 1), (7, 64, 1), (4097, 65, 1)])
 @pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
 def test_gemm_nn(dev, n, m, p, prec):
     rng = np.random.default_rng(n + p)
