@@ -51,6 +51,16 @@ struct fe_ctx_s {
   int* info_dev = nullptr;     // small device status words
   void* gram_static = nullptr; // device address of fe_gram_static's fixed scratch on this device
   hipEvent_t events[16] = {};
+  // captured launch sequences of the tridiagonalization (fe_syev), keyed by
+  // the size and the device addresses the launches were recorded with
+  struct TriGraph {
+    int m = 0;
+    const void* a = nullptr;
+    int64_t lda = 0;
+    const void* ws = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<TriGraph> tri_graphs;
 };
 
 struct fe_csr_s {

@@ -360,6 +360,8 @@ int fe_ctx_destroy(fe_ctx ctx) {
   if (ctx->info_dev) hipFree(ctx->info_dev);
   for (auto& e : ctx->events)
     if (e) hipEventDestroy(e);
+  for (auto& g : ctx->tri_graphs)
+    if (g.exec) hipGraphExecDestroy(g.exec);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;

@@ -795,20 +795,64 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
     rc = fe_sym_small(ctx, m, A, lda);
     if (rc) return rc;
     const int G = 64;
-    for (int k = 0; k + 2 < m; ++k) {
-      const double* pprev = pb + ((k + 1) & 1) * m;
-      double* pcur = pb + (k & 1) * m;
-      // rows past k held in registers: 2 x 256 / 8 x 64 up to m = 512, else 8 x 256 / 32 x 64
-      if (m <= 512)
-        hipLaunchKernelGGL((k_tri_step<2, 8>), dim3(G), dim3(256), 0, ctx->stream, m, k, A, lda, Vh, tau, dd, ee,
-                           pprev, pcur);
-      else
-        hipLaunchKernelGGL((k_tri_step<8, 32>), dim3(G), dim3(256), 0, ctx->stream, m, k, A, lda, Vh, tau, dd, ee,
-                           pprev, pcur);
+    auto steps = [&]() {
+      for (int k = 0; k + 2 < m; ++k) {
+        const double* pprev = pb + ((k + 1) & 1) * m;
+        double* pcur = pb + (k & 1) * m;
+        // rows past k held in registers: 2 x 256 / 8 x 64 up to m = 512, else 8 x 256 / 32 x 64
+        if (m <= 512)
+          hipLaunchKernelGGL((k_tri_step<2, 8>), dim3(G), dim3(256), 0, ctx->stream, m, k, A, lda, Vh, tau, dd, ee,
+                             pprev, pcur);
+        else
+          hipLaunchKernelGGL((k_tri_step<8, 32>), dim3(G), dim3(256), 0, ctx->stream, m, k, A, lda, Vh, tau, dd, ee,
+                             pprev, pcur);
+      }
+      hipLaunchKernelGGL(k_tri_finish, dim3(1), dim3(64), 0, ctx->stream, m, A, lda, Vh, tau, dd, ee,
+                         pb + ((m - 3) & 1) * m);
+    };
+    // The m - 2 dependent column steps as one captured graph, replayed when
+    // the size and the addresses match (the solvers call this with the same
+    // workspace every cycle): one submission instead of m - 2 launches. A
+    // caller that is itself capturing the stream gets the launches recorded
+    // into its graph instead.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    FE_HIP_TRY(hipStreamIsCapturing(ctx->stream, &cap));
+    if (cap != hipStreamCaptureStatusNone || m < 64) {
+      steps();
+      FE_CHECK_LAUNCH();
+    } else {
+      hipGraphExec_t exec = nullptr;
+      for (auto& g : ctx->tri_graphs)
+        if (g.m == m && g.a == A && g.lda == lda && g.ws == ws) exec = g.exec;
+      if (!exec) {
+        FE_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        steps();
+        hipGraph_t graph = nullptr;
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (le != hipSuccess || ce != hipSuccess || !graph) {
+          if (graph) hipGraphDestroy(graph);
+          return fail(-3, std::string("fe_syev: capturing the tridiagonalization: ") +
+                              hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail(-3, std::string("fe_syev: hipGraphInstantiate: ") + hipGetErrorString(ie));
+        if (ctx->tri_graphs.size() >= 8) {  // a few sizes at a time
+          FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+          hipGraphExecDestroy(ctx->tri_graphs.front().exec);
+          ctx->tri_graphs.erase(ctx->tri_graphs.begin());
+        }
+        fe_ctx_s::TriGraph g;
+        g.m = m;
+        g.a = A;
+        g.lda = lda;
+        g.ws = ws;
+        g.exec = exec;
+        ctx->tri_graphs.push_back(g);
+      }
+      FE_HIP_TRY(hipGraphLaunch(exec, ctx->stream));
     }
-    hipLaunchKernelGGL(k_tri_finish, dim3(1), dim3(64), 0, ctx->stream, m, A, lda, Vh, tau, dd, ee,
-                       pb + ((m - 3) & 1) * m);
-    FE_CHECK_LAUNCH();
   }
   if (multi) {
     const unsigned g = blocks_for(m, 64);

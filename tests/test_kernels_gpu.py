@@ -660,6 +660,37 @@ def test_syev_random(dev, m):
         assert np.max(np.abs(w - wo)) <= 1e-13 * scale * max(1, np.sqrt(m))
 
 
+@pytest.mark.parametrize("m", [64, 300, 512])
+def test_syev_graph_replays(dev, m):
+    """The tridiagonalization's column steps are captured once per size and
+    workspace and replayed: five different matrices through the same buffers
+    (the first call captures, the others replay), with eager work between the
+    calls (a syev of another size, a Gram, a host download), each checked
+    against LAPACK."""
+    rng = np.random.default_rng(100 + m)
+    Ad = dev.buffer((m, m), np.float64, order="F")
+    Wd = dev.buffer((m,), np.float64)
+    Vd = dev.buffer((m, m), np.float64, order="F")
+    Xd = dev.put(rng.uniform(-1, 1, (2000, 8)))
+    Gd = dev.buffer((8, 8), np.float64, order="F")
+    for rep in range(5):
+        A = rng.standard_normal((m, m))
+        A = A + A.T
+        Ad.upload(A)
+        assert _fe.check(L().fe_syev(dev.ctx, m, Ad.ptr, m, Wd.ptr, Vd.ptr, m)) == 0
+        w, V = Wd.download(), Vd.download()
+        wr = np.linalg.eigvalsh(A)
+        scale = max(np.max(np.abs(wr)), 1.0)
+        assert np.max(np.abs(w - wr)) <= 1e-13 * scale * np.sqrt(m), rep
+        assert np.max(np.abs(A @ V - V * w)) <= 1e-12 * scale * np.sqrt(m), rep
+        # eager work between the replays
+        B = rng.standard_normal((40, 40))
+        info, wb, _ = _syev(dev, B + B.T)
+        assert info == 0 and np.max(np.abs(wb - np.linalg.eigvalsh(B + B.T))) <= 1e-12 * 40
+        _fe.check(L().fe_gram(dev.ctx, 0, _fe.F64, 2000, 8, 8, Xd.ptr, 8, Xd.ptr, 8, None, Gd.ptr, 8))
+        dev.sync()
+
+
 def test_syev_rejects_above_2048(dev):
     A = dev.buffer((2049, 2049), np.float64, order="F")
     W = dev.buffer((2049,), np.float64)
