@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -51,17 +52,22 @@ struct fe_ctx_s {
   int* info_dev = nullptr;     // small device status words
   void* gram_static = nullptr; // device address of fe_gram_static's fixed scratch on this device
   hipEvent_t events[16] = {};
-  // captured launch sequences of the tridiagonalization (fe_syev), keyed by
-  // the size and the device addresses the launches were recorded with
-  struct TriGraph {
-    int m = 0;
-    const void* a = nullptr;
-    int64_t lda = 0;
-    const void* ws = nullptr;
+  // captured launch sequences (the tridiagonalization of fe_syev, the
+  // blocked Cholesky and triangular inverse), keyed by the routine, its sizes
+  // and the device addresses the launches were recorded with
+  struct SeqGraph {
+    int64_t key[8] = {};
     hipGraphExec_t exec = nullptr;
   };
-  std::vector<TriGraph> tri_graphs;
+  std::vector<SeqGraph> seq_graphs;
 };
+
+namespace fe {
+// Runs `launches` (kernel launches on ctx->stream only) as a graph captured on
+// the first call with this key and replayed afterwards; when the stream is
+// being captured by the caller, or capture is not wanted, launches directly.
+int run_captured(fe_ctx ctx, const int64_t (&key)[8], const std::function<void()>& launches);
+}  // namespace fe
 
 struct fe_csr_s {
   fe_ctx ctx = nullptr;

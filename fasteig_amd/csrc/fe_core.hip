@@ -1,7 +1,9 @@
 // Context, memory, CSR upload and the column-wise block kernels of libfe_hip.so.
 #include "fe_common.hpp"
 
+#include <algorithm>
 #include <cstring>
+#include <iterator>
 
 namespace fe {
 
@@ -260,6 +262,44 @@ __global__ __launch_bounds__(256) void k_reduce_seg(int chunks, int k, int per_s
   if (ty == 0 && j < k) seg[(int64_t)blockIdx.y * k + j] = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
 }
 
+int run_captured(fe_ctx ctx, const int64_t (&key)[8], const std::function<void()>& launches) {
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  FE_HIP_TRY(hipStreamIsCapturing(ctx->stream, &cap));
+  if (cap != hipStreamCaptureStatusNone) {
+    launches();
+    FE_CHECK_LAUNCH();
+    return 0;
+  }
+  hipGraphExec_t exec = nullptr;
+  for (auto& g : ctx->seq_graphs)
+    if (std::equal(std::begin(g.key), std::end(g.key), std::begin(key))) exec = g.exec;
+  if (!exec) {
+    FE_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    launches();
+    hipGraph_t graph = nullptr;
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    if (le != hipSuccess || ce != hipSuccess || !graph) {
+      if (graph) hipGraphDestroy(graph);
+      return fail(-3, std::string("capturing a launch sequence: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(-3, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    if (ctx->seq_graphs.size() >= 32) {  // the sequences of a few sizes and workspaces at a time
+      FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+      hipGraphExecDestroy(ctx->seq_graphs.front().exec);
+      ctx->seq_graphs.erase(ctx->seq_graphs.begin());
+    }
+    fe_ctx_s::SeqGraph g;
+    std::copy(std::begin(key), std::end(key), std::begin(g.key));
+    g.exec = exec;
+    ctx->seq_graphs.push_back(g);
+  }
+  FE_HIP_TRY(hipGraphLaunch(exec, ctx->stream));
+  return 0;
+}
+
 int reduce_cols_sqrt(fe_ctx ctx, int chunks, int k, const double* part, double* out) {
   if (k <= 0) return 0;
   // many chunks (a row tile each, from the fused residual of a tall product):
@@ -360,7 +400,7 @@ int fe_ctx_destroy(fe_ctx ctx) {
   if (ctx->info_dev) hipFree(ctx->info_dev);
   for (auto& e : ctx->events)
     if (e) hipEventDestroy(e);
-  for (auto& g : ctx->tri_graphs)
+  for (auto& g : ctx->seq_graphs)
     if (g.exec) hipGraphExecDestroy(g.exec);
   hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -814,44 +814,13 @@ extern "C" int fe_syev(fe_ctx ctx, int m, double* A, int64_t lda, double* W, dou
     // the size and the addresses match (the solvers call this with the same
     // workspace every cycle): one submission instead of m - 2 launches. A
     // caller that is itself capturing the stream gets the launches recorded
-    // into its graph instead.
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    FE_HIP_TRY(hipStreamIsCapturing(ctx->stream, &cap));
-    if (cap != hipStreamCaptureStatusNone || m < 64) {
+    // into its own graph instead.
+    if (m < 64) {
       steps();
       FE_CHECK_LAUNCH();
     } else {
-      hipGraphExec_t exec = nullptr;
-      for (auto& g : ctx->tri_graphs)
-        if (g.m == m && g.a == A && g.lda == lda && g.ws == ws) exec = g.exec;
-      if (!exec) {
-        FE_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        steps();
-        hipGraph_t graph = nullptr;
-        const hipError_t le = hipGetLastError();
-        const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-        if (le != hipSuccess || ce != hipSuccess || !graph) {
-          if (graph) hipGraphDestroy(graph);
-          return fail(-3, std::string("fe_syev: capturing the tridiagonalization: ") +
-                              hipGetErrorString(le != hipSuccess ? le : ce));
-        }
-        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
-        if (ie != hipSuccess) return fail(-3, std::string("fe_syev: hipGraphInstantiate: ") + hipGetErrorString(ie));
-        if (ctx->tri_graphs.size() >= 8) {  // a few sizes at a time
-          FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
-          hipGraphExecDestroy(ctx->tri_graphs.front().exec);
-          ctx->tri_graphs.erase(ctx->tri_graphs.begin());
-        }
-        fe_ctx_s::TriGraph g;
-        g.m = m;
-        g.a = A;
-        g.lda = lda;
-        g.ws = ws;
-        g.exec = exec;
-        ctx->tri_graphs.push_back(g);
-      }
-      FE_HIP_TRY(hipGraphLaunch(exec, ctx->stream));
+      const int64_t key[8] = {1, m, (int64_t)(uintptr_t)A, lda, (int64_t)(uintptr_t)ws, 0, 0, 0};
+      if (int rc2 = run_captured(ctx, key, steps)) return rc2;
     }
   }
   if (multi) {

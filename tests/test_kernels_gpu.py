@@ -634,6 +634,37 @@ def test_potrf_reports_failure(dev, k, bad):
     assert np.allclose(np.tril(Gd2.download()), np.eye(k) * 2.0)
 
 
+@pytest.mark.parametrize("k", [130, 512])
+def test_potrf_rinv_repeated(dev, k):
+    """Six factorizations through the same buffers (SPD matrices and, in
+    between, ones with a negative pivot whose column must be reported), with
+    eager work between them, each checked against LAPACK's factor and the
+    inverse."""
+    rng = np.random.default_rng(k)
+    Gd = dev.buffer((k, k), np.float64, order="F")
+    Rd = dev.buffer((k, k), np.float64, order="F")
+    Xd = dev.put(rng.uniform(-1, 1, (3000, 16)))
+    Sd = dev.buffer((16, 16), np.float64, order="F")
+    for rep in range(6):
+        B = rng.standard_normal((k, k))
+        Gm = B @ B.T + k * np.eye(k)
+        bad = 37 + 11 * rep if rep in (2, 4) else None
+        if bad is not None:
+            Gm[bad, bad] = -1.0
+        Gd.upload(Gm)
+        info = L().fe_potrf(dev.ctx, k, Gd.ptr, k)
+        if bad is not None:
+            assert info >= 1 and info <= bad + 1, (rep, info)
+            continue
+        assert info == 0, rep
+        Lc = np.tril(Gd.download())
+        assert np.max(np.abs(Lc - np.linalg.cholesky(Gm))) <= 1e-12 * np.sqrt(np.max(np.abs(Gm)))
+        _fe.check(L().fe_chol_rinv(dev.ctx, k, Gd.ptr, k, Rd.ptr, k))
+        assert np.max(np.abs(Lc.T @ Rd.download() - np.eye(k))) <= 1e-12
+        _fe.check(L().fe_gram(dev.ctx, 0, _fe.F64, 3000, 16, 16, Xd.ptr, 16, Xd.ptr, 16, None, Sd.ptr, 16))
+        dev.sync()
+
+
 def _syev(dev, A):
     m = A.shape[0]
     Ad = dev.put(A, np.float64, order="F")
