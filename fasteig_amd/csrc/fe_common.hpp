@@ -66,6 +66,9 @@ namespace fe {
 // Runs `launches` (kernel launches on ctx->stream only) as a graph captured on
 // the first call with this key and replayed afterwards; when the stream is
 // being captured by the caller, or capture is not wanted, launches directly.
+// FE_GRAPHS=0 in the environment (read once per process) turns the capture
+// off: the same launches then go out one by one, for tools that trace kernel
+// launches but cannot follow a captured stream.
 int run_captured(fe_ctx ctx, const int64_t (&key)[8], const std::function<void()>& launches);
 }  // namespace fe
 

@@ -265,7 +265,11 @@ __global__ __launch_bounds__(256) void k_reduce_seg(int chunks, int k, int per_s
 int run_captured(fe_ctx ctx, const int64_t (&key)[8], const std::function<void()>& launches) {
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   FE_HIP_TRY(hipStreamIsCapturing(ctx->stream, &cap));
-  if (cap != hipStreamCaptureStatusNone) {
+  static const bool graphs_off = [] {
+    const char* e = std::getenv("FE_GRAPHS");
+    return e && std::string(e) == "0";
+  }();
+  if (cap != hipStreamCaptureStatusNone || graphs_off) {
     launches();
     FE_CHECK_LAUNCH();
     return 0;
