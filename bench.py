@@ -207,6 +207,39 @@ def copy_reference(nbytes=1 << 30, reps=5):
     return ms.value / reps, rows * 256 * 4
 
 
+def stream_ceiling(nbytes=2 << 30, reps=10, grids=(16384, 65536)):
+    """HBM rates a plain streaming kernel reaches on this GPU (fe_stream_probe):
+    read only, write only, copy, and three reads per write (the fused
+    Chebyshev step's mix: X, Z and the matrix in, Y out). The best of the
+    grid sizes, each timed over `reps` launches after one untimed."""
+    import ctypes
+    from fasteig_amd import _fe
+    L = _fe.lib()
+    dev = _fe.Device(0)
+    rows = nbytes // (4 * 256)
+    bufs = [dev.buffer((rows, 256), np.float32) for _ in range(4)]
+    for b in bufs:
+        _fe.check(L.fe_zero2d(dev.ctx, b.ptr, rows, 256, 256, 4))
+    a, b, c, d = (x.ptr for x in bufs)
+    moved = {0: 1, 1: 1, 2: 2, 3: 4}
+    out = {}
+    for kind, name in ((0, "read"), (1, "write"), (2, "copy"), (3, "read3_write1")):
+        best = 0.0
+        for grid in grids:
+            _fe.check(L.fe_stream_probe(dev.ctx, kind, nbytes, a, b, c, d, grid))
+            _fe.check(L.fe_event_record(dev.ctx, 2))
+            for _ in range(reps):
+                _fe.check(L.fe_stream_probe(dev.ctx, kind, nbytes, a, b, c, d, grid))
+            _fe.check(L.fe_event_record(dev.ctx, 3))
+            ms = ctypes.c_float()
+            _fe.check(L.fe_event_elapsed_ms(dev.ctx, 2, 3, ctypes.byref(ms)))
+            best = max(best, moved[kind] * nbytes / (ms.value / reps * 1e-3) / 1e9)
+        out[name] = best
+    del bufs
+    dev.close()
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -722,6 +755,8 @@ def main():
             f"fp64 {p64n:.3f} ms {a64n / (p64n * 1e-3) / 1e9:.0f} GB/s")
     # the cycle's Gram and rotation products at the solve's block width
     extra["dense_mfma"] = dense_roofline(n, cfg["blksize"] or min(n, 2 * cfg["nev"]))
+    extra["stream_ceiling_gbs"] = stream_ceiling()
+    log("streaming ceilings (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in extra["stream_ceiling_gbs"].items()))
 
     cpu = None
     if not args.no_cpu:

@@ -103,6 +103,7 @@ def lib():
         L.fe_zero_rows_small.argtypes = [vp, i32, i32, vp, i64]
         L.fe_event_record.argtypes = [vp, i32]
         L.fe_event_elapsed_ms.argtypes = [vp, i32, i32, C.POINTER(C.c_float)]
+        L.fe_stream_probe.argtypes = [vp, i32, i64, vp, vp, vp, vp, i32]
         _lib = L
     return _lib
 

@@ -810,6 +810,61 @@ int fe_zero_rows_small(fe_ctx ctx, int r, int p, double* C, int64_t ldc) {
   return 0;
 }
 
+}  // extern "C"
+
+// Streaming probes (16 bytes per lane per access, grid-stride, nontemporal):
+// the HBM rate a kernel with a given read/write mix can reach on this GPU,
+// the ceiling the memory-bound kernels above are read against.
+typedef float fe_v4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_read(const fe_v4* __restrict__ a, int64_t n4, fe_v4* out) {
+  fe_v4 s = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n4; i += static_cast<int64_t>(gridDim.x) * 256)
+    s += __builtin_nontemporal_load(&a[i]);
+  const float t = s.x + s.y + s.z + s.w;
+  if (t == 1.25e-30f) out[0] = s;  // keeps the loads; never true for the probe's zero-filled input
+}
+__global__ __launch_bounds__(256) void k_stream_write(fe_v4* __restrict__ d, int64_t n4) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n4; i += static_cast<int64_t>(gridDim.x) * 256)
+    __builtin_nontemporal_store(fe_v4{1.f, 2.f, 3.f, 4.f}, &d[i]);
+}
+__global__ __launch_bounds__(256) void k_stream_copy(const fe_v4* __restrict__ a, fe_v4* __restrict__ d, int64_t n4) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n4; i += static_cast<int64_t>(gridDim.x) * 256)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(&a[i]), &d[i]);
+}
+__global__ __launch_bounds__(256) void k_stream_r3w1(const fe_v4* __restrict__ a, const fe_v4* __restrict__ b,
+                                                    const fe_v4* __restrict__ c, fe_v4* __restrict__ d, int64_t n4) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n4; i += static_cast<int64_t>(gridDim.x) * 256)
+    __builtin_nontemporal_store(
+        __builtin_nontemporal_load(&a[i]) + __builtin_nontemporal_load(&b[i]) + __builtin_nontemporal_load(&c[i]),
+        &d[i]);
+}
+
+extern "C" {
+
+int fe_stream_probe(fe_ctx ctx, int kind, int64_t bytes, const void* a, const void* b, const void* c, void* d,
+                    int grid) {
+  FE_ARG(ctx && kind >= 0 && kind <= 3 && bytes > 0 && bytes % 16 == 0 && grid > 0,
+         "fe_stream_probe: bad kind, size or grid");
+  FE_ARG(d && (kind == 1 || a) && (kind != 3 || (b && c)), "fe_stream_probe: null buffer");
+  const int64_t n4 = bytes / 16;
+  switch (kind) {
+    case 0:
+      hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(256), 0, ctx->stream, (const fe_v4*)a, n4, (fe_v4*)d);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_stream_write, dim3(grid), dim3(256), 0, ctx->stream, (fe_v4*)d, n4);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, (const fe_v4*)a, (fe_v4*)d, n4);
+      break;
+    default:
+      hipLaunchKernelGGL(k_stream_r3w1, dim3(grid), dim3(256), 0, ctx->stream, (const fe_v4*)a, (const fe_v4*)b,
+                         (const fe_v4*)c, (fe_v4*)d, n4);
+  }
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
 int fe_event_record(fe_ctx ctx, int slot) {
   FE_ARG(ctx && slot >= 0 && slot < 16, "fe_event_record: bad slot");
   FE_HIP_TRY(hipEventRecord(ctx->events[slot], ctx->stream));

@@ -36,11 +36,21 @@ typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-te
 
 constexpr int TL_ROWS = 16;   // rows per tile: 4 wavefronts x 4 rows of 16 lanes
 constexpr int TL_PF = 3;      // 16-byte X loads per thread held in registers per tile in flight
-constexpr int TL_PENCIL = 24;  // tiles per pencil at most
+#ifndef TL_PENCIL_TILES
+#define TL_PENCIL_TILES 24
+#endif
+constexpr int TL_PENCIL = TL_PENCIL_TILES;  // tiles per pencil at most
 #ifndef TL_BLOCK_EDGE
 #define TL_BLOCK_EDGE 4
 #endif
 constexpr int TL_BLOCK = TL_BLOCK_EDGE;  // patches per block edge in the pencil order of a grid
+// Pencils per work group: the work items of a group of consecutive pencils
+// are taken slice by slice, so the same slice of neighbouring pencils (which
+// share halo rows) starts at nearly the same time on one XCD, while the
+// group's slices still share its entry lists through L2. C3, 96^3: 4 gave
+// 1.7% fewer fetched bytes and 2-4% shorter launches than 1 (all slices of a
+// pencil first); 2 and 16 were within 1% of 4.
+constexpr int TL_GROUP = 4;
 constexpr int TL_MAX_SLOTS = 240;  // LDS slots: with the zero slot and the pencil offsets within 64 KB
 static_assert(TL_MAX_SLOTS < 256, "own-row slots are staged as bytes");
 
@@ -118,9 +128,9 @@ __device__ __forceinline__ void lds_barrier() {
 }  // namespace
 
 // Workgroup: 256 threads; thread tid owns tile row tid / 16 and the 16 bytes
-// at offset 16 * (tid % 16) of the slice. Work item w = pencil * nslices +
-// slice, so the slices of one pencil run side by side on one XCD and share
-// its entry lists through L2.
+// at offset 16 * (tid % 16) of the slice. Work items run in groups of
+// TL_GROUP pencils, slice by slice, on one XCD: neighbouring pencils share
+// halo rows and the group's slices share its entry lists through L2.
 //
 // Loads are issued ahead of the tile that needs them and no wait covers a
 // load issued in the same tile: the new X rows of a tile two tiles ahead
@@ -174,8 +184,17 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
   constexpr int CW = 16 * V;  // columns per slice
   const int64_t work = xcd_tile(blockIdx.x, per_xcd);
   if (work >= nwork) return;  // the whole workgroup, before any barrier
-  const int64_t p = work / nslices;
-  const int c0 = static_cast<int>(work - p * nslices) * CW;
+  // work item -> (pencil, slice): groups of TL_GROUP consecutive pencils,
+  // slice-major inside a group (the group's last one may be shorter)
+  const int64_t np = nwork / nslices;
+  const int64_t gsz = static_cast<int64_t>(TL_GROUP) * nslices;
+  const int64_t g = work / gsz;
+  const int64_t rem = work - g * gsz;
+  const int64_t gp0 = g * TL_GROUP;
+  const int64_t gn = min(static_cast<int64_t>(TL_GROUP), np - gp0);
+  const int64_t sl = rem / gn;
+  const int64_t p = gp0 + (rem - sl * gn);
+  const int c0 = static_cast<int>(sl) * CW;
   const int tid = threadIdx.x;
   const int sub = tid & 15;
   const int tr = tid >> 4;

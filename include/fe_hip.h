@@ -280,6 +280,14 @@ int fe_zero_rows_small(fe_ctx ctx, int r, int p, double* C, int64_t ldc);
 /* ---- timing helpers for benchmarks (HIP events on the context stream) ---- */
 int fe_event_record(fe_ctx ctx, int slot);
 int fe_event_elapsed_ms(fe_ctx ctx, int slot0, int slot1, float* ms);
+/* Streaming probe on the context stream (no counterpart in the reference: a
+ * measurement helper): kind 0 reads `bytes` of a, 1 writes `bytes` of d, 2
+ * copies a to d, 3 writes d = a + b + c (FP32, `bytes` per buffer); 16-byte
+ * nontemporal accesses over `grid` workgroups of 256. Kind 0 writes d only
+ * when the sum of a is 1.25e-30 (give it zero-filled memory and 16 bytes of
+ * d). Times the HBM rate a read/write mix reaches. */
+int fe_stream_probe(fe_ctx ctx, int kind, int64_t bytes, const void* a, const void* b, const void* c, void* d,
+                    int grid);
 
 /* ---- runtime helpers for host loops that avoid per-call synchronization --- */
 /* Device arrays of a matrix (for grouped launch tables); any may be NULL. */

@@ -859,6 +859,30 @@ def test_identity_deviation(dev, m):
     assert abs(out.value - np.linalg.norm(Gm - np.eye(m))) <= 1e-14 * np.linalg.norm(Gm - np.eye(m))
 
 
+@pytest.mark.parametrize("grid", [1, 7, 4096])
+def test_stream_probe(dev, grid):
+    """The streaming probe moves what it says: copy, the three-input sum and
+    the write pattern, over sizes that are not a multiple of the grid."""
+    rng = np.random.default_rng(grid)
+    rows = 3001  # 3001 x 4 FP32 = 3001 16-byte accesses
+    a, b, c = (rng.uniform(-1, 1, (rows, 4)).astype(np.float32) for _ in range(3))
+    Ad, Bd, Cd = dev.put(a, np.float32), dev.put(b, np.float32), dev.put(c, np.float32)
+    D = dev.buffer((rows, 4), np.float32)
+    nb = rows * 16
+    _fe.check(L().fe_stream_probe(dev.ctx, 2, nb, Ad.ptr, None, None, D.ptr, grid))
+    assert np.array_equal(D.download(), a)
+    _fe.check(L().fe_stream_probe(dev.ctx, 3, nb, Ad.ptr, Bd.ptr, Cd.ptr, D.ptr, grid))
+    assert np.array_equal(D.download(), (a + b) + c)
+    _fe.check(L().fe_stream_probe(dev.ctx, 1, nb, None, None, None, D.ptr, grid))
+    assert np.array_equal(D.download(), np.tile(np.float32([1, 2, 3, 4]), (rows, 1)))
+    Z = dev.buffer((rows, 4), np.float32)
+    _fe.check(L().fe_zero2d(dev.ctx, Z.ptr, rows, 16, 16, 1))
+    _fe.check(L().fe_stream_probe(dev.ctx, 0, nb, Z.ptr, None, None, D.ptr, grid))
+    assert np.array_equal(D.download(), np.tile(np.float32([1, 2, 3, 4]), (rows, 1)))  # untouched
+    assert L().fe_stream_probe(dev.ctx, 2, nb + 4, Ad.ptr, None, None, D.ptr, grid) == -2
+    assert L().fe_stream_probe(dev.ctx, 3, nb, Ad.ptr, None, Cd.ptr, D.ptr, grid) == -2
+
+
 def test_axpby_copy_upload(dev):
     rng = np.random.default_rng(12)
     n, k = 1000, 9
