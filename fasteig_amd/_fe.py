@@ -104,6 +104,7 @@ def lib():
         L.fe_event_record.argtypes = [vp, i32]
         L.fe_event_elapsed_ms.argtypes = [vp, i32, i32, C.POINTER(C.c_float)]
         L.fe_stream_probe.argtypes = [vp, i32, i64, vp, vp, vp, vp, i32]
+        L.fe_gemm_small.argtypes = [vp, i32, i32, i32, i32, i32, dbl, vp, i64, vp, i64, dbl, vp, i64]
         _lib = L
     return _lib
 

@@ -219,7 +219,7 @@ void rotate_rest(Workspace& ws, int m, const BlockRef& Q, int from, const BlockR
 
 RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
                        const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R,
-                       int res_count, bool keep_ay, int rotate_count) {
+                       int res_count, bool keep_ay, int rotate_count, const double* rinv) {
   HipExecutor& ex = ws.ex;
   if (m <= 0 || requested <= 0 || requested > m)
     throw std::invalid_argument("extract_projected_ritz_pairs: invalid active/requested Ritz dimensions.");
@@ -228,6 +228,13 @@ RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& 
     throw std::invalid_argument("extract_projected_ritz_pairs: multivector dimension mismatch.");
   ws.ensure(m);
   check(fe_sym_small(ex.ctx(), m, ws.G.data(), ws.cap), "fe_sym_small");
+  if (rinv) {  // rinv^T G rinv (ws.V holds G rinv in between)
+    check(fe_gemm_small(ex.ctx(), 0, 0, m, m, m, 1.0, ws.G.data(), ws.cap, rinv, ws.cap, 0.0, ws.V.data(), ws.cap),
+          "fe_gemm_small");
+    check(fe_gemm_small(ex.ctx(), 1, 0, m, m, m, 1.0, rinv, ws.cap, ws.V.data(), ws.cap, 0.0, ws.G.data(), ws.cap),
+          "fe_gemm_small");
+    check(fe_sym_small(ex.ctx(), m, ws.G.data(), ws.cap), "fe_sym_small");
+  }
   const int info = fe_syev(ex.ctx(), m, ws.G.data(), ws.cap, ws.W.data(), ws.V.data(), ws.cap);
   check(info, "fe_syev");
   if (info != 0) throw std::runtime_error("extract_projected_ritz_pairs: projected eigensolve failed.");
@@ -235,8 +242,13 @@ RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& 
   check(fe_memcpy_d2h(ex.ctx(), w.data(), ws.W.data(), sizeof(double) * m), "d2h(ritz values)");
   std::vector<int> idx = select_indices(w, which);
   idx.resize(requested);
-  check(fe_gather_cols(ex.ctx(), m, requested, ws.V.data(), ws.cap, idx.data(), ws.C.data(), ws.cap),
-        "fe_gather_cols");
+  const double* vecs = ws.V.data();
+  if (rinv) {  // rinv times every eigenvector (ws.G is free after the eigensolve), then the selection
+    check(fe_gemm_small(ex.ctx(), 0, 0, m, m, m, 1.0, rinv, ws.cap, ws.V.data(), ws.cap, 0.0, ws.G.data(), ws.cap),
+          "fe_gemm_small");
+    vecs = ws.G.data();
+  }
+  check(fe_gather_cols(ex.ctx(), m, requested, vecs, ws.cap, idx.data(), ws.C.data(), ws.cap), "fe_gather_cols");
   RitzPairs out;
   out.selected = idx;
   out.values.resize(requested);

@@ -75,9 +75,15 @@ struct RitzPairs {
 // `from` on later (with the coefficients kept in ws.C and Q unchanged), so a
 // solve that stops after this extraction skips the rotation of its guard
 // columns.
+// rinv (upper triangular m x m, ld ws.cap, zero below the diagonal): Q is not
+// orthonormal but Q rinv is (the last Cholesky QR pass left unapplied); the
+// projected matrix Q^T A Q becomes rinv^T (Q^T A Q) rinv and the coefficients
+// rinv C, so Y = Q (rinv C) and AY = AQ (rinv C) are the Ritz vectors of
+// span(Q) without the tall product Q rinv.
 RitzPairs extract_ritz(Workspace& ws, int m, const BlockRef& Q, const BlockRef& AQ, int requested,
                        const std::string& which, const BlockRef& Y, const BlockRef& AY, const BlockRef* R,
-                       int res_count = -1, bool keep_ay = true, int rotate_count = -1);
+                       int res_count = -1, bool keep_ay = true, int rotate_count = -1,
+                       const double* rinv = nullptr);
 void rotate_rest(Workspace& ws, int m, const BlockRef& Q, int from, const BlockRef& Y);
 
 // Stable index sort by `which` (include/fasteig_solver.hpp:59-82; "be" from

@@ -236,6 +236,13 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // more passes
     int passes = (prec == FE_F32 || closing || cycle_degree > degree) ? 2 : 1;
     const double cycle_need = ws.need;
+    // The last pass's R^{-1} goes into the Rayleigh-Ritz below instead of a
+    // tall product Q R^{-1}: the projected matrix becomes R^{-T} (Q^T A Q)
+    // R^{-1} and the coefficients R^{-1} C (small FP64 products), the same
+    // Ritz pairs of the same span. A first pass folds only when it covers all
+    // of V, which then trades places with Q (the basis moves to Q, V receives
+    // the Ritz vectors).
+    bool folded = false;
     for (int pass = 0; pass < passes; ++pass) {
       ws.ensure(subspace_cols);
       // the first pass factors the filtered block, whose condition number
@@ -286,6 +293,11 @@ FASTEIG_INT ChebyshevSolver::solve() {
                                  (prec == FE_F32 ? "fp32" : "fp64") + ", degree " + std::to_string(cycle_degree) +
                                  ", column " + std::to_string(info) + ").");
       hip::check(fe_chol_rinv(ex.ctx(), subspace_cols, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
+      if (pass + 1 == passes && (pass > 0 || subspace_cols == V.ncols())) {
+        if (pass == 0) V.swap(Q);
+        folded = true;
+        break;
+      }
       hip::trmm_upper(ex, Vb, ws.R.data(), ws.cap, Qout, ws.gram_mode, ws.need);
       if (pass > 0) Q.swap(AY);
     }
@@ -301,7 +313,7 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // returns only the wanted vectors)
     const int m_rr = subspace_cols;
     ritz = hip::extract_ritz(ws, m_rr, Q.ref(), AQ.ref(), m_rr, "sr", V.cols(0, m_rr), AY.cols(0, nev), nullptr, nev,
-                             false, nev);
+                             false, nev, folded ? ws.R.data() : nullptr);
     if (!all_finite(ritz.values) || !all_finite(ritz.relres))
       throw std::runtime_error("ChebyshevSolver: non-finite Ritz values or residuals.");
     it_done = iter + 1;

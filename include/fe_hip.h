@@ -276,6 +276,16 @@ int fe_sym_small(fe_ctx ctx, int m, double* A, int64_t lda);
 int fe_gather_cols(fe_ctx ctx, int m, int p, const double* V, int64_t ldv, const int* idx_host,
                    double* C, int64_t ldc);
 int fe_zero_rows_small(fe_ctx ctx, int r, int p, double* C, int64_t ldc);
+/* C[m x n] = alpha op(A) op(B) + beta C, FP64 column-major on the device, op(X)
+ * = X^T when its flag (ta, tb) is nonzero; beta = 0 does not read C; C must
+ * not alias A or B. For the small projected matrices (up to a few thousand):
+ * it folds the inverse factor of the last Cholesky QR pass into the
+ * Rayleigh-Ritz matrix, R^{-T} (X^T A X) R^{-1}, and into its coefficients,
+ * R^{-1} C, in place of the tall product X R^{-1}. No counterpart in the
+ * reference, whose Cholesky QR forms X R^{-1} (src/core/Orthogonalization.cpp:
+ * 185-246) before its host Rayleigh-Ritz (src/core/RitzExtraction.cpp:65-141). */
+int fe_gemm_small(fe_ctx ctx, int ta, int tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
+                  const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
 
 /* ---- timing helpers for benchmarks (HIP events on the context stream) ---- */
 int fe_event_record(fe_ctx ctx, int slot);

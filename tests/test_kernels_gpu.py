@@ -859,6 +859,36 @@ def test_identity_deviation(dev, m):
     assert abs(out.value - np.linalg.norm(Gm - np.eye(m))) <= 1e-14 * np.linalg.norm(Gm - np.eye(m))
 
 
+@pytest.mark.parametrize("ta", [0, 1])
+@pytest.mark.parametrize("tb", [0, 1])
+@pytest.mark.parametrize("m,n,k", [(512, 512, 512), (1, 1, 1), (65, 33, 100), (130, 7, 64), (200, 300, 17)])
+def test_gemm_small(dev, ta, tb, m, n, k):
+    """Small column-major FP64 product against numpy, with leading dimensions
+    above the sizes and a nonzero beta."""
+    rng = np.random.default_rng(m * 7 + n * 3 + k + 2 * ta + tb)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
+    C0 = rng.standard_normal((m, n))
+    pad = 3
+    def put_ld(M):
+        P = np.zeros((M.shape[0] + pad, M.shape[1]))
+        P[:M.shape[0]] = M
+        return dev.put(P, np.float64, order="F"), M.shape[0] + pad
+    Ad, lda = put_ld(A)
+    Bd, ldb = put_ld(B)
+    Cd, ldc = put_ld(C0)
+    _fe.check(L().fe_gemm_small(dev.ctx, ta, tb, m, n, k, 1.5, Ad.ptr, lda, Bd.ptr, ldb, -0.5, Cd.ptr, ldc))
+    ref = 1.5 * ((A.T if ta else A) @ (B.T if tb else B)) - 0.5 * C0
+    got = Cd.download()[:m]
+    assert rel(got, ref) < 1e-13
+    assert np.array_equal(Cd.download()[m:], np.zeros((pad, n)))  # rows past m untouched
+    # beta = 0 ignores what C holds (NaN here)
+    Cn, _ = put_ld(np.full((m, n), np.nan))
+    _fe.check(L().fe_gemm_small(dev.ctx, ta, tb, m, n, k, 1.0, Ad.ptr, lda, Bd.ptr, ldb, 0.0, Cn.ptr, ldc))
+    assert rel(Cn.download()[:m], (A.T if ta else A) @ (B.T if tb else B)) < 1e-13
+    assert L().fe_gemm_small(dev.ctx, ta, tb, m, n, k, 1.0, Ad.ptr, lda, Bd.ptr, ldb, 0.0, Ad.ptr, ldc) == -2
+
+
 @pytest.mark.parametrize("grid", [1, 7, 4096])
 def test_stream_probe(dev, grid):
     """The streaming probe moves what it says: copy, the three-input sum and
