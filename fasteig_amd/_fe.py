@@ -105,6 +105,8 @@ def lib():
         L.fe_event_elapsed_ms.argtypes = [vp, i32, i32, C.POINTER(C.c_float)]
         L.fe_stream_probe.argtypes = [vp, i32, i64, vp, vp, vp, vp, i32]
         L.fe_gemm_small.argtypes = [vp, i32, i32, i32, i32, i32, dbl, vp, i64, vp, i64, dbl, vp, i64]
+        L.fe_gram_equilibrate.argtypes = [vp, i32, vp, i64, vp]
+        L.fe_scale_rows_small.argtypes = [vp, i32, i32, vp, vp, i64]
         _lib = L
     return _lib
 

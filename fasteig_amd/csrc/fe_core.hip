@@ -294,6 +294,25 @@ __global__ void k_zero_rows_small(int r, int p, double* C, int64_t ldc) {
   }
 }
 
+// d_j = 1 / sqrt(G_jj) (1 where G_jj is not positive)
+__global__ void k_diag_rsqrt(int m, const double* __restrict__ G, int64_t ldg, double* __restrict__ d) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const double g = G[(int64_t)j * ldg + j];
+  d[j] = g > 0.0 ? 1.0 / sqrt(g) : 1.0;
+}
+
+// A[i, j] *= dl[i] * (dr ? dr[j] : 1) over m x n, column-major
+__global__ void k_scale_small(int m, int n, const double* __restrict__ dl, const double* __restrict__ dr, double* A,
+                              int64_t lda) {
+  const int64_t total = (int64_t)m * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = static_cast<int>(t / m), i = static_cast<int>(t - (int64_t)j * m);
+    A[(int64_t)j * lda + i] *= dr ? dl[i] * dr[j] : dl[i];
+  }
+}
+
 inline unsigned grid_for(int64_t total) {
   int64_t b = (total + 255) / 256;
   if (b > 65536) b = 65536;
@@ -837,6 +856,25 @@ int fe_download_colmajor(fe_ctx ctx, int64_t n, int k, const void* src, int prec
 int fe_sym_small(fe_ctx ctx, int m, double* A, int64_t lda) {
   FE_ARG(ctx && A && lda >= m, "fe_sym_small: bad argument");
   hipLaunchKernelGGL(k_sym_small, dim3(grid_for((int64_t)m * m)), dim3(256), 0, ctx->stream, m, A, lda);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_gram_equilibrate(fe_ctx ctx, int m, double* G, int64_t ldg, double* d) {
+  FE_ARG(ctx && G && d && m >= 0 && ldg >= std::max(1, m), "fe_gram_equilibrate: bad argument");
+  if (m == 0) return 0;
+  hipLaunchKernelGGL(k_diag_rsqrt, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, m, (const double*)G, ldg, d);
+  hipLaunchKernelGGL(k_scale_small, dim3(grid_for((int64_t)m * m)), dim3(256), 0, ctx->stream, m, m,
+                     (const double*)d, (const double*)d, G, ldg);
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
+int fe_scale_rows_small(fe_ctx ctx, int m, int n, const double* d, double* A, int64_t lda) {
+  FE_ARG(ctx && A && d && m >= 0 && n >= 0 && lda >= std::max(1, m), "fe_scale_rows_small: bad argument");
+  if ((int64_t)m * n == 0) return 0;
+  hipLaunchKernelGGL(k_scale_small, dim3(grid_for((int64_t)m * n)), dim3(256), 0, ctx->stream, m, n, d,
+                     (const double*)nullptr, A, lda);
   FE_CHECK_LAUNCH();
   return 0;
 }

@@ -150,6 +150,7 @@ FASTEIG_INT ChebyshevSolver::solve() {
     // (about 16 bits) serve; below it FP32-class products (1e-6)
     ws.need = prec == FE_F32 ? ((std::isfinite(last_mx) && last_mx < 1e-2) ? 1e-6 : 1e-4) : 0.0;
     // 1. filter the active columns (ChebyshevSolver.cpp:151-155, 223-258)
+    bool equil = false;
     if (active > 0 && upper != lower) {
       BlockRef prev = F.cols(0, active), cur = P.cols(0, active);
       // per-column degrees (the closing sweep): the columns are gathered in
@@ -210,9 +211,13 @@ FASTEIG_INT ChebyshevSolver::solve() {
       }
       // 2. deflate against the locked Ritz vectors (ChebyshevSolver.cpp:159-166)
       if (locked > 0) hip::block_gram_schmidt(ws, V.cols(0, locked), cur, cur);
-      if (prec == FE_F32 || closing || cycle_degree > degree) {
-        // FP32 blocks and the closing sweep: equilibrate the columns so the
-        // Gram of the filtered block is not limited by their norm spread
+      // FP32 blocks and the closing sweep: the first CholQR pass factors the
+      // block with its columns scaled to unit norm, so that its Gram is not
+      // limited by their norm spread. An FP32 block is scaled in place: the
+      // filter leaves the damped columns so small that their BF16 parts and
+      // FP32 part products leave the normal range. An FP64 block keeps its
+      // columns; the scaling then acts on the m x m Gram and factor only.
+      if (prec == FE_F32) {
         const std::vector<double> nr = hip::column_norms(ex, cur);
         std::vector<double> inv(active);
         for (int j = 0; j < active; ++j) inv[j] = nr[j] > 0.0 ? 1.0 / nr[j] : 1.0;
@@ -220,6 +225,8 @@ FASTEIG_INT ChebyshevSolver::solve() {
         hip::check(fe_axpby_cols(ex.ctx(), prec, n, active, cur.ptr, cur.ld, cur.ptr, cur.ld, 0.0, 0.0,
                                  inv_norms.data()),
                    "scale");
+      } else {
+        equil = closing || cycle_degree > degree;
       }
       if (percol) {
         hip::gather_cols(ex, cur, back, V.cols(locked, active));  // back to the Ritz order
@@ -256,6 +263,10 @@ FASTEIG_INT ChebyshevSolver::solve() {
       const BlockRef Vb = pass == 0 ? V.cols(0, subspace_cols) : Q.cols(0, subspace_cols);
       const BlockRef Qout = pass == 0 ? Q.cols(0, subspace_cols) : AY.cols(0, subspace_cols);
       hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap, nullptr, ws.need);
+      // Gram of V D (D = diag(1 / ||v_j||), kept in inv_norms)
+      const bool scaled = pass == 0 && equil;
+      if (scaled) hip::check(fe_gram_equilibrate(ex.ctx(), subspace_cols, ws.G.data(), ws.cap, inv_norms.data()),
+                             "fe_gram_equilibrate");
       if (pass == 0 && passes == 2) {
         // The equilibrated block's Gram G = I + E: kappa(V)^2 <= (1 + |E|)/(1 - |E|).
         // One pass leaves Q^T Q - I at about the Gram's error times kappa^2:
@@ -274,6 +285,8 @@ FASTEIG_INT ChebyshevSolver::solve() {
       hip::check(info, "fe_potrf");
       if (info != 0 && passes < 4) {
         hip::gram(ex, ws.gram_mode, Vb, Vb, ws.G.data(), ws.cap, nullptr, ws.need);
+        if (scaled) hip::check(fe_gram_equilibrate(ex.ctx(), subspace_cols, ws.G.data(), ws.cap, inv_norms.data()),
+                               "fe_gram_equilibrate");
         MatrixXd Gh = ws.G.to_host();
         double tr = 0.0;
         for (int i = 0; i < subspace_cols; ++i) tr += Gh(i, i);
@@ -293,6 +306,10 @@ FASTEIG_INT ChebyshevSolver::solve() {
                                  (prec == FE_F32 ? "fp32" : "fp64") + ", degree " + std::to_string(cycle_degree) +
                                  ", column " + std::to_string(info) + ").");
       hip::check(fe_chol_rinv(ex.ctx(), subspace_cols, ws.G.data(), ws.cap, ws.R.data(), ws.cap), "fe_chol_rinv");
+      // V D R^{-1} = V (D R^{-1}): the factor of the unscaled block
+      if (scaled)
+        hip::check(fe_scale_rows_small(ex.ctx(), subspace_cols, subspace_cols, inv_norms.data(), ws.R.data(), ws.cap),
+                   "fe_scale_rows_small");
       if (pass + 1 == passes && (pass > 0 || subspace_cols == V.ncols())) {
         if (pass == 0) V.swap(Q);
         folded = true;

@@ -286,6 +286,16 @@ int fe_zero_rows_small(fe_ctx ctx, int r, int p, double* C, int64_t ldc);
  * 185-246) before its host Rayleigh-Ritz (src/core/RitzExtraction.cpp:65-141). */
 int fe_gemm_small(fe_ctx ctx, int ta, int tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
                   const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
+/* Equilibration of a Gram matrix G = X^T X (m x m, FP64 column-major, device):
+ * d_j = 1 / sqrt(G_jj) (1 where G_jj <= 0) into d (device, m values) and
+ * G <- D G D, the Gram of X D (columns scaled to unit norm) without touching
+ * X; fe_scale_rows_small then turns the inverse factor R^{-1} of D G D into
+ * D R^{-1}, the factor of X itself (A <- diag(d) A over m x n). Replaces the
+ * column-norm pass and column scaling of the filtered block before its
+ * Cholesky QR (the reference does not scale: src/core/Orthogonalization.cpp:
+ * 185-246). */
+int fe_gram_equilibrate(fe_ctx ctx, int m, double* G, int64_t ldg, double* d);
+int fe_scale_rows_small(fe_ctx ctx, int m, int n, const double* d, double* A, int64_t lda);
 
 /* ---- timing helpers for benchmarks (HIP events on the context stream) ---- */
 int fe_event_record(fe_ctx ctx, int slot);

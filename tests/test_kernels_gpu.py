@@ -889,6 +889,35 @@ def test_gemm_small(dev, ta, tb, m, n, k):
     assert L().fe_gemm_small(dev.ctx, ta, tb, m, n, k, 1.0, Ad.ptr, lda, Bd.ptr, ldb, 0.0, Ad.ptr, ldc) == -2
 
 
+@pytest.mark.parametrize("m", [1, 37, 512])
+def test_gram_equilibrate_and_row_scaling(dev, m):
+    """D G D with d_j = 1/sqrt(G_jj) (1 for a zero column) and diag(d) A, on a
+    Gram whose column norms span twelve decades."""
+    rng = np.random.default_rng(m + 5)
+    X = rng.standard_normal((3 * m + 4, m)) * np.logspace(-6, 6, m)[None, :]
+    if m > 1:
+        X[:, m // 2] = 0.0
+    Gm = X.T @ X
+    ld = m + 2
+    Gp = np.zeros((ld, m))
+    Gp[:m] = Gm
+    Gd = dev.put(Gp, np.float64, order="F")
+    dd = dev.buffer((m,), np.float64)
+    _fe.check(L().fe_gram_equilibrate(dev.ctx, m, Gd.ptr, ld, dd.ptr))
+    g = np.diag(Gm)
+    d = np.where(g > 0, 1.0 / np.sqrt(np.where(g > 0, g, 1.0)), 1.0)
+    assert np.allclose(dd.download(), d, rtol=1e-15, atol=0)
+    got = Gd.download()
+    assert rel(got[:m], d[:, None] * Gm * d[None, :]) < 1e-14
+    assert np.array_equal(got[m:], np.zeros((2, m)))
+    Ap = rng.standard_normal((ld, m + 3))
+    Ad = dev.put(Ap, np.float64, order="F")
+    _fe.check(L().fe_scale_rows_small(dev.ctx, m, m + 3, dd.ptr, Ad.ptr, ld))
+    back = Ad.download()
+    assert rel(back[:m], d[:, None] * Ap[:m]) < 1e-15
+    assert np.array_equal(back[m:], Ap[m:])
+
+
 @pytest.mark.parametrize("grid", [1, 7, 4096])
 def test_stream_probe(dev, grid):
     """The streaming probe moves what it says: copy, the three-input sum and
