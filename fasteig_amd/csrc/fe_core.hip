@@ -219,54 +219,49 @@ __global__ void k_sym_small(int m, double* A, int64_t lda) {
   }
 }
 
-// C = alpha op(A) op(B) + beta C for small column-major FP64 matrices: 64 x 64
-// output tiles, 16 x 16 threads of 4 x 4 outputs, op(A) and op(B) staged in
-// LDS 16 values of the inner dimension at a time, summed in order.
+// C = alpha op(A) op(B) + beta C for small column-major FP64 matrices: 32 x 32
+// output tiles (256 workgroups at m = n = 512), 16 x 16 threads of 2 x 2
+// outputs, op(A) and op(B) staged in LDS 32 values of the inner dimension at a
+// time, summed in order.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm_small(int m, int n, int k, double alpha, const double* __restrict__ A,
                                                     int64_t lda, const double* __restrict__ B, int64_t ldb,
                                                     double beta, double* __restrict__ C, int64_t ldc) {
-  __shared__ double As[16][65];  // op(A)(i0 + i, k0 + q) at [q][i]
-  __shared__ double Bs[16][65];  // op(B)(k0 + q, j0 + j) at [q][j]
-  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  __shared__ double As[32][33];  // op(A)(i0 + i, k0 + q) at [q][i]
+  __shared__ double Bs[32][33];  // op(B)(k0 + q, j0 + j) at [q][j]
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
   const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-  double acc[4][4];
+  double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (int k0 = 0; k0 < k; k0 += 32) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
-  for (int k0 = 0; k0 < k; k0 += 16) {
     for (int e = tid; e < 1024; e += 256) {
       // consecutive threads read consecutive addresses of A and of B
-      const int ii = TA ? e >> 4 : e & 63, qa = TA ? e & 15 : e >> 6;
+      const int ii = TA ? e >> 5 : e & 31, qa = TA ? e & 31 : e >> 5;
       const int gi = i0 + ii, ga = k0 + qa;
       As[qa][ii] = gi < m && ga < k ? (TA ? A[(int64_t)gi * lda + ga] : A[(int64_t)ga * lda + gi]) : 0.0;
-      const int jj = TB ? e & 63 : e >> 4, qb = TB ? e >> 6 : e & 15;
+      const int jj = TB ? e & 31 : e >> 5, qb = TB ? e >> 5 : e & 31;
       const int gj = j0 + jj, gb = k0 + qb;
       Bs[qb][jj] = gj < n && gb < k ? (TB ? B[(int64_t)gb * ldb + gj] : B[(int64_t)gj * ldb + gb]) : 0.0;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int q = 0; q < 16; ++q) {
-      double a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[q][ti * 4 + r];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = Bs[q][tj * 4 + c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+#pragma unroll 8
+    for (int q = 0; q < 32; ++q) {
+      const double a0 = As[q][ti], a1 = As[q][ti + 16];
+      const double b0 = Bs[q][tj], b1 = Bs[q][tj + 16];
+      acc[0][0] = fma(a0, b0, acc[0][0]);
+      acc[0][1] = fma(a0, b1, acc[0][1]);
+      acc[1][0] = fma(a1, b0, acc[1][0]);
+      acc[1][1] = fma(a1, b1, acc[1][1]);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int gj = j0 + tj * 4 + c;
+  for (int c = 0; c < 2; ++c) {
+    const int gj = j0 + tj + 16 * c;
     if (gj >= n) continue;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int gi = i0 + ti * 4 + r;
+    for (int r = 0; r < 2; ++r) {
+      const int gi = i0 + ti + 16 * r;
       if (gi >= m) continue;
       double v = alpha * acc[r][c];
       if (beta != 0.0) v += beta * C[(int64_t)gj * ldc + gi];
@@ -886,7 +881,7 @@ int fe_gemm_small(fe_ctx ctx, int ta, int tb, int m, int n, int k, double alpha,
          "fe_gemm_small: leading dimension too small");
   FE_ARG(C != A && C != B, "fe_gemm_small: C aliases an operand");
   if ((int64_t)m * n == 0) return 0;
-  const dim3 grid((m + 63) / 64, (n + 63) / 64);
+  const dim3 grid((m + 31) / 32, (n + 31) / 32);
   auto kern = ta ? (tb ? k_gemm_small<true, true> : k_gemm_small<true, false>)
                  : (tb ? k_gemm_small<false, true> : k_gemm_small<false, false>);
   hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
