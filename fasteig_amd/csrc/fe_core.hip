@@ -122,6 +122,46 @@ __global__ __launch_bounds__(256) void k_colsq(int64_t n, int k, const T* __rest
   if (RES) part_r[blockIdx.x * (int64_t)k + j] = rr[0][tx] + rr[1][tx] + rr[2][tx] + rr[3][tx];
 }
 
+// The same for at most 8 columns (a vector, a few): thread t takes column
+// t % k and every (256 / k)-th row of the chunk from phase t / k; the phases
+// of a column are added in order.
+template <typename T, bool RES>
+__global__ __launch_bounds__(256) void k_colsq_narrow(int64_t n, int k, const T* __restrict__ Y, int64_t ldy,
+                                                      const T* __restrict__ AY, int64_t lday,
+                                                      const double* __restrict__ theta, T* R, int64_t ldr,
+                                                      double* part_r, double* part_y, int64_t rows_per) {
+  __shared__ double ry[256], rr[256];
+  const int t = threadIdx.x, phases = 256 / k;
+  const int j = t % k, ph = t / k;
+  const int64_t r0 = blockIdx.x * rows_per;
+  const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+  double sy = 0.0, sr = 0.0;
+  if (ph < phases) {
+    const double th = RES ? theta[j] : 0.0;
+    for (int64_t i = r0 + ph; i < r1; i += phases) {
+      const double y = static_cast<double>(Y[i * ldy + j]);
+      sy += y * y;
+      if (RES) {
+        const T rv = static_cast<T>(static_cast<double>(AY[i * lday + j]) - th * y);
+        if (R) R[i * ldr + j] = rv;
+        const double rd = static_cast<double>(rv);
+        sr += rd * rd;
+      }
+    }
+  }
+  ry[t] = sy;
+  rr[t] = sr;
+  __syncthreads();
+  if (t >= k) return;
+  double a = 0.0, b = 0.0;
+  for (int q = 0; q < phases; ++q) {
+    a += ry[q * k + t];
+    if (RES) b += rr[q * k + t];
+  }
+  part_y[blockIdx.x * (int64_t)k + t] = a;
+  if (RES) part_r[blockIdx.x * (int64_t)k + t] = b;
+}
+
 // sqrt of the chunk sums: 64 columns x 4 chunk residues per workgroup, each
 // residue summed in chunk order, then the residues in order
 __global__ __launch_bounds__(256) void k_reduce_cols(int chunks, int k, const double* __restrict__ part,
@@ -703,7 +743,23 @@ static int colsq_launch(fe_ctx ctx, int prec, int64_t n, int k, const void* Y, i
   double* part_r = part_y + chunks * k;
   dim3 grid((unsigned)chunks, (unsigned)((k + 63) / 64));
   const bool res = AY != nullptr;
-  if (prec == FE_F64) {
+  if (k <= 8) {  // a vector or a few: every thread of a workgroup on rows
+    if (prec == FE_F64) {
+      if (res)
+        hipLaunchKernelGGL((k_colsq_narrow<double, true>), grid, dim3(256), 0, ctx->stream, n, k, (const double*)Y,
+                           ldy, (const double*)AY, lday, theta, (double*)R, ldr, part_r, part_y, rows_per);
+      else
+        hipLaunchKernelGGL((k_colsq_narrow<double, false>), grid, dim3(256), 0, ctx->stream, n, k,
+                           (const double*)Y, ldy, nullptr, 0, nullptr, nullptr, 0, nullptr, part_y, rows_per);
+    } else {
+      if (res)
+        hipLaunchKernelGGL((k_colsq_narrow<float, true>), grid, dim3(256), 0, ctx->stream, n, k, (const float*)Y,
+                           ldy, (const float*)AY, lday, theta, (float*)R, ldr, part_r, part_y, rows_per);
+      else
+        hipLaunchKernelGGL((k_colsq_narrow<float, false>), grid, dim3(256), 0, ctx->stream, n, k, (const float*)Y,
+                           ldy, nullptr, 0, nullptr, nullptr, 0, nullptr, part_y, rows_per);
+    }
+  } else if (prec == FE_F64) {
     if (res)
       hipLaunchKernelGGL((k_colsq<double, true>), grid, dim3(256), 0, ctx->stream, n, k, (const double*)Y,
                          ldy, (const double*)AY, lday, theta, (double*)R, ldr, part_r, part_y, rows_per);

@@ -364,7 +364,22 @@ __global__ __launch_bounds__(256) void k_spmv_csr(int n, const int* __restrict__
   const int beg = rowok ? rowptr[row] : 0;
   const int end = rowok ? rowptr[row + 1] : 0;
   T acc = T(0);
-  for (int p = beg + sub; p < end; p += G) acc += vals[p] * x[(int64_t)colind[p] * ldx];
+  int p = beg + sub;
+  // four of the lane's entries per round: their index, value and x loads are
+  // in flight together; the sum keeps the entry order
+  for (; p + 3 * G < end; p += 4 * G) {
+    int c[4];
+    T v[4], xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = colind[p + u * G];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = vals[p + u * G];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xv[u] = x[(int64_t)c[u] * ldx];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += v[u] * xv[u];
+  }
+  for (; p < end; p += G) acc += vals[p] * x[(int64_t)colind[p] * ldx];
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if (!rowok || sub != 0) return;

@@ -744,9 +744,11 @@ def test_syev_degenerate_and_diagonal(dev):
 
 
 @pytest.mark.parametrize("prec", [_fe.F64, _fe.F32])
-def test_colnorms_and_ritz_residuals(dev, prec):
-    rng = np.random.default_rng(11)
-    n, k = 12345, 37
+@pytest.mark.parametrize("n,k", [(12345, 37), (12345, 1), (884736, 1), (1000, 3), (77, 8), (5, 9)])
+def test_colnorms_and_ritz_residuals(dev, prec, n, k):
+    """Wide blocks take 64 columns per workgroup, at most 8 columns the
+    narrow kernel (every thread on rows): both against numpy."""
+    rng = np.random.default_rng(11 + k)
     Y = rng.uniform(-1, 1, (n, k))
     AY = rng.uniform(-1, 1, (n, k))
     th = rng.uniform(-2, 2, k)
