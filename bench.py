@@ -63,7 +63,8 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
     """Average time of the fused Chebyshev step Y = 2c A X + 2e X - Z on the
     library stream (HIP events), and its algorithmic-byte bandwidth. fmt
     "blocked" times the row-block copy (4-row blocks sharing a column list),
-    "tiled" the LDS-tiled copy (16-row tiles, pencils through the planes)."""
+    "tiled" the LDS-tiled copy (on a grid 8 x 4 or 4 x 4 plane patches,
+    pencils through the planes; "tiled16" / "tiled32" force the height)."""
     from fasteig_amd import _fe
     L = _fe.lib()
     dev = _fe.Device(0)
@@ -77,8 +78,18 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
             raise RuntimeError(f"row-block copy not built (reuse {reuse:.2f})")
         C.block_schedule()  # strip order for grid stencils, as the solvers use it
         C.use(_fe.FMT_BLOCKED)
-    if fmt == "tiled":
-        built, lpr = C.build_tiled()
+    if fmt in ("tiled", "tiled16", "tiled32"):
+        # tiled16 / tiled32: the tile height forced (FE_TILE_ROWS), for comparison
+        saved = os.environ.get("FE_TILE_ROWS")
+        if fmt != "tiled":
+            os.environ["FE_TILE_ROWS"] = fmt[5:]
+        try:
+            built, lpr = C.build_tiled()
+        finally:
+            if saved is None:
+                os.environ.pop("FE_TILE_ROWS", None)
+            else:
+                os.environ["FE_TILE_ROWS"] = saved
         if not built:
             raise RuntimeError(f"tiled copy not built ({lpr:.2f} X rows loaded per row)")
         C.use(_fe.FMT_TILED)
@@ -599,7 +610,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the generalized (C2) and serving (C5) runs")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--kernel-formats", default="csr,blocked,tiled",
-                    help="sparse formats timed by --only-kernels (csr, blocked = 4-row blocks, blocked8, tiled)")
+                    help="sparse formats timed by --only-kernels (csr, blocked = 4-row blocks, blocked8, tiled, "
+                         "tiled16, tiled32)")
     ap.add_argument("--solve-trace", default="", metavar="PRECISIONS",
                     help="for a kernel trace: per precision (e.g. mixed,fp64) one warm-up and one timed solve; "
                          "the timed solves' CLOCK_MONOTONIC windows (the trace's clock) go to --marks")

@@ -111,7 +111,7 @@ struct fe_csr_s {
   int64_t tl_ntiles = 0, tl_npencils = 0, tl_loads = 0;
   int tl_slots = 0;            // LDS row slots per workgroup; slot tl_slots is the zero row
   int64_t* tl_pencil = nullptr;  // first tile of each pencil (npencils + 1)
-  int* tl_rows = nullptr;        // 16 rows per tile (-1: none)
+  int* tl_rows = nullptr;        // tl_trows rows per tile (-1: none)
   // entries: every row of a tile holds tl_maxe[tile] of them (its nonzeros in
   // CSR order, then the zero slot with value 0), rows one after another
   int64_t* tl_ebase = nullptr;   // first entry of each tile (ntiles + 1)
@@ -123,6 +123,7 @@ struct fe_csr_s {
   int* tl_lpack = nullptr;       // per X row load: (LDS slot << 24) | row (rows below 2^24)
   int* tl_sslot = nullptr;       // slot of each tile row's own X row (-1: not in its tile)
   int tl_self = 0;               // every tile row's own X row is in its tile's slots
+  int tl_trows = 16;             // rows per tile: 16 (256 threads) or 32 (512 threads)
   int use_tl = 0;
 };
 

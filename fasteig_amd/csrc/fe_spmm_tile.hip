@@ -25,6 +25,7 @@
 #include "fe_common.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -34,8 +35,22 @@ namespace {
 
 typedef float nt_f4 __attribute__((ext_vector_type(4)));  // 16 bytes for non-temporal access
 
-constexpr int TL_ROWS = 16;   // rows per tile: 4 wavefronts x 4 rows of 16 lanes
-constexpr int TL_PF = 3;      // 16-byte X loads per thread held in registers per tile in flight
+// Rows per tile: 16 (4 wavefronts x 4 rows of 16 lanes; on a grid a 4 x 4
+// patch) or 32 (8 wavefronts; on a grid an 8 x 4 patch). A wider patch loads
+// fewer halo rows per output row (27-point stencil: 60 new rows per 32 rows
+// against 36 per 16) and shares them inside one workgroup rather than
+// through L2 between workgroups that happen to run the neighbouring patch at
+// the same plane.
+constexpr int TL_ROWS_MAX = 32;
+// 16-byte X loads per thread held in registers per tile in flight: at least
+// a tile's new rows (36 of a 4 x 4 27-point patch against 3 x 16 rows, 60 of
+// an 8 x 4 patch against 2 x 32)
+template <int TR>
+constexpr int tl_pf() { return TR == 16 ? 3 : 2; }
+// minimum waves per SIMD of the FP32 kernel: the LDS of five 16-row or three
+// 32-row workgroups fits a CU
+template <int TR>
+constexpr int tl_waves() { return TR == 16 ? 5 : 6; }
 #ifndef TL_PENCIL_TILES
 #define TL_PENCIL_TILES 24
 #endif
@@ -127,8 +142,8 @@ __device__ __forceinline__ void lds_barrier() {
 
 }  // namespace
 
-// Workgroup: 256 threads; thread tid owns tile row tid / 16 and the 16 bytes
-// at offset 16 * (tid % 16) of the slice. Work items run in groups of
+// Workgroup: 16 TR threads; thread tid owns tile row tid / 16 and the 16
+// bytes at offset 16 * (tid % 16) of the slice. Work items run in groups of
 // TL_GROUP pencils, slice by slice, on one XCD: neighbouring pencils share
 // halo rows and the group's slices share its entry lists through L2.
 //
@@ -147,7 +162,7 @@ __device__ __forceinline__ void lds_barrier() {
 // and load offsets are staged in LDS first.
 // FULL: every column slice lies inside the block (k a multiple of the slice
 // width), so no load needs a partial-slice path. Every tile's load list holds
-// at least the TL_PF x 16 rows the registers take (the host pads it with loads
+// at least the PF x TR rows the registers take (the host pads it with loads
 // of a row into a spare slot after the zero slot), so the prefetch loads and
 // their LDS writes run without validity checks: a load under a branch, or
 // merged with a constant, is waited for where the branches join.
@@ -164,8 +179,8 @@ __device__ __forceinline__ void lds_barrier() {
 // slot), so a lane reads its row's entry e0 + 16 h + sub without a bound check
 // or a select (lanes past the row read the next row's entries, which their
 // broadcasts never use; the host appends entries after the last tile).
-template <typename T, bool O32, bool FULL, bool SELF, bool FZ>
-__global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void k_spmm_tile(
+template <typename T, int TR, bool O32, bool FULL, bool SELF, bool FZ>
+__global__ __launch_bounds__(16 * TR, (sizeof(T) == 4 && O32 && SELF) ? tl_waves<TR>() : 1) void k_spmm_tile(
     const int64_t* __restrict__ pencil, const int* __restrict__ rows, const int* __restrict__ sslot,
     const int64_t* __restrict__ ebase, const int* __restrict__ maxe, const int2* __restrict__ ent32,
     const int* __restrict__ eslot, const double* __restrict__ ev64,
@@ -175,11 +190,13 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
   extern __shared__ float4 xs[];  // (zslot + 2) slots of 16 x 16 bytes: zslot holds zeros, zslot + 1 padding loads
   __shared__ int s_lp[TL_PENCIL + 1];             // load offsets of the pencil's tiles
   __shared__ int s_eb[TL_PENCIL + 1];             // entry offsets of its tiles
-  __shared__ int s_rows[TL_PENCIL * TL_ROWS];     // its tile rows
+  __shared__ int s_rows[TL_PENCIL * TR];          // its tile rows
   __shared__ int s_maxe[TL_PENCIL];
   // own-row slots of its tile rows (slots < TL_MAX_SLOTS fit a byte: with
   // int entries the workgroup's LDS passed the 32 KB that five per CU allow)
-  __shared__ unsigned char s_self[SELF ? TL_PENCIL * TL_ROWS : 1];
+  __shared__ unsigned char s_self[SELF ? TL_PENCIL * TR : 1];
+  constexpr int NT = 16 * TR;       // threads
+  constexpr int PF = tl_pf<TR>();   // X row loads per thread per tile in registers
   constexpr int V = 16 / (int)sizeof(T);
   constexpr int CW = 16 * V;  // columns per slice
   const int64_t work = xcd_tile(blockIdx.x, per_xcd);
@@ -204,12 +221,12 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
   const int64_t t0 = pencil[p];
   const int nt = static_cast<int>(pencil[p + 1] - t0);  // at most TL_PENCIL
   const int64_t lbase = lptr[t0], eb0 = ebase[t0];
-  for (int i = tid; i <= nt; i += 256) s_lp[i] = static_cast<int>(lptr[t0 + i] - lbase);
-  for (int i = tid; i <= nt; i += 256) s_eb[i] = static_cast<int>(ebase[t0 + i] - eb0);
-  for (int i = tid; i < nt * TL_ROWS; i += 256) s_rows[i] = rows[t0 * TL_ROWS + i];
-  for (int i = tid; i < nt; i += 256) s_maxe[i] = maxe[t0 + i];
+  for (int i = tid; i <= nt; i += NT) s_lp[i] = static_cast<int>(lptr[t0 + i] - lbase);
+  for (int i = tid; i <= nt; i += NT) s_eb[i] = static_cast<int>(ebase[t0 + i] - eb0);
+  for (int i = tid; i < nt * TR; i += NT) s_rows[i] = rows[t0 * TR + i];
+  for (int i = tid; i < nt; i += NT) s_maxe[i] = maxe[t0 + i];
   if (SELF)
-    for (int i = tid; i < nt * TL_ROWS; i += 256) s_self[i] = static_cast<unsigned char>(max(sslot[t0 * TL_ROWS + i], 0));
+    for (int i = tid; i < nt * TR; i += NT) s_self[i] = static_cast<unsigned char>(max(sslot[t0 * TR + i], 0));
   if (tid < 16) xs[zslot * 16 + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int* lp = lpack + lbase;
   auto ld32 = [&](const int* base, int e) { return ld_off<int>(base, e); };
@@ -271,35 +288,35 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
     const int e0 = s_eb[tt] + static_cast<int>(__umul24(static_cast<uint32_t>(tr), static_cast<uint32_t>(s_maxe[tt])));
 #pragma unroll
     for (int h = 0; h < 2; ++h) entry(e0 + 16 * h + sub, a.ms[h], a.mw[h]);
-    const int row = max(s_rows[tt * TL_ROWS + tr], 0);  // a padding row reads row 0, unused
+    const int row = max(s_rows[tt * TR + tr], 0);  // a padding row reads row 0, unused
     if constexpr (FZ) load_row(Z, ldz, row, a.z);
     if constexpr (!SELF) load_row(X, ldx, row, a.x);
   };
-  // load list entries (slot << 24 | row) of tile tt for this thread's TL_PF
+  // load list entries (slot << 24 | row) of tile tt for this thread's PF
   // loads, as loaded (a tile past the pencil's last reads the last one's again:
   // its rows then go after the last products, into slots nobody reads)
-  auto fetch_list = [&](int tt, int (&pk)[TL_PF]) {
-    const int l0 = s_lp[min(tt, nt - 1)] + (tid >> 4);
+  auto fetch_list = [&](int tt, int (&pk)[PF]) {
+    const int l0 = s_lp[min(tt, nt - 1)] + tr;
 #pragma unroll
-    for (int j = 0; j < TL_PF; ++j) pk[j] = ld32(lp, l0 + 16 * j);
+    for (int j = 0; j < PF; ++j) pk[j] = ld32(lp, l0 + TR * j);
   };
   lds_barrier();
   // the first tile's X rows
-  for (int i = tid; i < s_lp[1] * 16; i += 256) {
+  for (int i = tid; i < s_lp[1] * 16; i += NT) {
     const unsigned pk = static_cast<unsigned>(lp[i >> 4]);
     xs[(pk >> 24) * 16 + (i & 15)] = load_x(pk & 0xffffff, i & 15);
   }
   Ahead a[2];                 // entries / Z / X rows of the tiles of each parity
-  float4 pre[2][TL_PF];       // new X rows of the tiles of each parity
-  int slt[2][TL_PF];          // and their slots (slot * 16 + part)
+  float4 pre[2][PF];          // new X rows of the tiles of each parity
+  int slt[2][PF];             // and their slots (slot * 16 + part)
   // load lists of the tiles two ahead, by the parity of the tile that issues
   // their loads (fixed registers per role: a copy of a register still being
   // loaded would wait for the load)
-  int ipk[2][TL_PF];
+  int ipk[2][PF];
   // a list's loads: their LDS slots, and the X loads themselves
-  auto issue = [&](const int (&pk)[TL_PF], int (&so)[TL_PF], float4 (&po)[TL_PF]) {
+  auto issue = [&](const int (&pk)[PF], int (&so)[PF], float4 (&po)[PF]) {
 #pragma unroll
-    for (int j = 0; j < TL_PF; ++j) {
+    for (int j = 0; j < PF; ++j) {
       so[j] = static_cast<int>(static_cast<unsigned>(pk[j]) >> 24) * 16 + sub;
       po[j] = load_x(pk[j] & 0xffffff, sub);
     }
@@ -308,7 +325,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
   // tile two ahead, then the next tile's entries and Z row), so the waits at
   // the top of the loop hold on the entry path as on the back edge
   {
-    int pk[TL_PF];
+    int pk[PF];
     fetch_list(1, pk);
     issue(pk, slt[1], pre[1]);
   }
@@ -360,13 +377,13 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
         Unroll<0, 4>::run(group);
       }
     }
-    const int row = s_rows[tt * TL_ROWS + tr];
+    const int row = s_rows[tt * TR + tr];
     // the output is formed on every lane (a padding row or a lane past k
     // discards it): the loaded Z / X values are then used on every path, and
     // the next load into their registers needs no wait for them
     T xr[V];  // this row's part of X for the beta term
     if constexpr (SELF) {
-      const float4 f = xs[s_self[tt * TL_ROWS + tr] * 16 + sub];
+      const float4 f = xs[s_self[tt * TR + tr] * 16 + sub];
       __builtin_memcpy(xr, &f, 16);
     } else {
 #pragma unroll
@@ -394,10 +411,10 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && O32 && SELF) ? 5 : 1) void 
     lds_barrier();  // every thread is done with tile tt's slots
     // tile tt + 1's new X rows (issued one tile earlier) into their slots
 #pragma unroll
-    for (int j = 0; j < TL_PF; ++j) xs[slt[1 - P][j]] = pre[1 - P][j];
-    if (tt + 1 < nt) {  // rows past the TL_PF registers per thread
+    for (int j = 0; j < PF; ++j) xs[slt[1 - P][j]] = pre[1 - P][j];
+    if (tt + 1 < nt) {  // rows past the PF registers per thread
       const int l0 = s_lp[tt + 1], nl = s_lp[tt + 2] - l0;
-      for (int i = tid + 256 * TL_PF; i < nl * 16; i += 256) {
+      for (int i = tid + NT * PF; i < nl * 16; i += NT) {
         const unsigned pk = static_cast<unsigned>(lp[l0 + (i >> 4)]);
         xs[(pk >> 24) * 16 + (i & 15)] = load_x(pk & 0xffffff, i & 15);
       }
@@ -424,19 +441,24 @@ static int launch_tile(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* 
                    (!Z || static_cast<double>(A->n) * ldz < lim);
   const bool full = k % CW == 0;
   const bool fz = Z && gamma != T(0);
-  auto pick = [&](auto self, auto zterm) {
+  auto pick = [&](auto rows, auto self, auto zterm) {
+    constexpr int R = decltype(rows)::value;
     constexpr bool S = decltype(self)::value, F = decltype(zterm)::value;
-    return o32 ? (full ? k_spmm_tile<T, true, true, S, F> : k_spmm_tile<T, true, false, S, F>)
-               : (full ? k_spmm_tile<T, false, true, S, F> : k_spmm_tile<T, false, false, S, F>);
+    return o32 ? (full ? k_spmm_tile<T, R, true, true, S, F> : k_spmm_tile<T, R, true, false, S, F>)
+               : (full ? k_spmm_tile<T, R, false, true, S, F> : k_spmm_tile<T, R, false, false, S, F>);
   };
-  auto pick_z = [&](auto self) {
-    return fz ? pick(self, std::true_type{}) : pick(self, std::false_type{});
+  auto pick_z = [&](auto rows, auto self) {
+    return fz ? pick(rows, self, std::true_type{}) : pick(rows, self, std::false_type{});
   };
-  auto kern = A->tl_self ? pick_z(std::true_type{}) : pick_z(std::false_type{});
-  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nb8)), dim3(256), lds, ctx->stream, A->tl_pencil, A->tl_rows,
-                     A->tl_sslot, A->tl_ebase, A->tl_maxe, A->tl_ent32, A->tl_eslot, A->tl_ev64, A->tl_lptr,
-                     A->tl_lpack, A->tl_slots, k,
-                     nslices, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, nwork);
+  auto pick_s = [&](auto rows) {
+    return A->tl_self ? pick_z(rows, std::true_type{}) : pick_z(rows, std::false_type{});
+  };
+  const bool wide = A->tl_trows == 32;
+  auto kern = wide ? pick_s(std::integral_constant<int, 32>{}) : pick_s(std::integral_constant<int, 16>{});
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nb8)), dim3(wide ? 512 : 256), lds, ctx->stream, A->tl_pencil,
+                     A->tl_rows, A->tl_sslot, A->tl_ebase, A->tl_maxe, A->tl_ent32, A->tl_eslot, A->tl_ev64,
+                     A->tl_lptr, A->tl_lpack, A->tl_slots, k, nslices, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma,
+                     nb8 / 8, nwork);
   FE_CHECK_LAUNCH();
   return 0;
 }
@@ -473,45 +495,56 @@ struct TilePlan {
   bool self = true;        // every row's own X row is in its tile
 };
 
-// Tiles of the matrix in processing order, grouped in pencils; -1 pads a tile.
-void make_tiles(int n, int64_t line, int64_t plane, std::vector<std::vector<int>>& tiles,
+// Tiles of tr rows (16 or 32) in processing order, grouped in pencils; -1
+// pads a tile. A grid whose lines and planes split into px x py patches takes
+// patches, any other matrix runs of consecutive rows; false for 32-row tiles
+// on a grid that does not split into 8 x 4 patches (its 4 x 4 patches are the
+// better plan).
+bool make_tiles(int n, int64_t line, int64_t plane, int tr, std::vector<std::vector<int>>& tiles,
                 std::vector<int64_t>& pencil) {
-  if (plane > 0 && line % 4 == 0 && (plane / line) % 4 == 0 && n % plane == 0) {
-    // 4 x 4 patches; a pencil runs through up to 24 consecutive planes; the
-    // pencils of one range of planes are taken in blocks of TL_BLOCK x
-    // TL_BLOCK patches (row of blocks by row), so the pencils an XCD runs at
-    // one time cover a compact area whose shared halo rows its L2 serves
-    const int64_t nx = line / 4, ny = plane / line / 4, nz = n / plane, pz = TL_PENCIL, B = TL_BLOCK;
+  const int px = tr == 32 ? 8 : 4, py = 4;  // patch extent along the line and across lines
+  const bool splits = plane > 0 && line % px == 0 && (plane / line) % py == 0 && n % plane == 0;
+  if (plane > 0 && !splits && tr == 32) return false;
+  if (splits) {
+    // px x py patches; a pencil runs through up to 24 consecutive planes; the
+    // pencils of one range of planes are taken in blocks of 16 x 16 points
+    // (TL_BLOCK x TL_BLOCK 4 x 4 patches; half as many 8 x 4 ones along the
+    // line), row of blocks by row, so the pencils an XCD runs at one time
+    // cover a compact area whose shared halo rows its L2 serves
+    const int64_t nx = line / px, ny = plane / line / py, nz = n / plane, pz = TL_PENCIL;
+    const int64_t bxe = std::max<int64_t>(1, TL_BLOCK * 4 / px), bye = TL_BLOCK;
     for (int64_t z0 = 0; z0 < nz; z0 += pz)
-      for (int64_t by = 0; by < ny; by += B)
-        for (int64_t bx = 0; bx < nx; bx += B)
-          for (int64_t py = by; py < std::min(ny, by + B); ++py)
-            for (int64_t px = bx; px < std::min(nx, bx + B); ++px) {
+      for (int64_t by = 0; by < ny; by += bye)
+        for (int64_t bx = 0; bx < nx; bx += bxe)
+          for (int64_t yy = by; yy < std::min(ny, by + bye); ++yy)
+            for (int64_t xx = bx; xx < std::min(nx, bx + bxe); ++xx) {
               for (int64_t z = z0; z < std::min(nz, z0 + pz); ++z) {
                 std::vector<int> t;
-                for (int dy = 0; dy < 4; ++dy)
-                  for (int dx = 0; dx < 4; ++dx)
-                    t.push_back(static_cast<int>(z * plane + (4 * py + dy) * line + 4 * px + dx));
+                for (int dy = 0; dy < py; ++dy)
+                  for (int dx = 0; dx < px; ++dx)
+                    t.push_back(static_cast<int>(z * plane + (py * yy + dy) * line + px * xx + dx));
                 tiles.push_back(std::move(t));
               }
               pencil.push_back(static_cast<int64_t>(tiles.size()));
             }
-    return;
+    return true;
   }
   const int64_t per_pencil = TL_PENCIL;
-  for (int64_t r0 = 0; r0 < n; r0 += TL_ROWS) {
+  for (int64_t r0 = 0; r0 < n; r0 += tr) {
     std::vector<int> t;
-    for (int i = 0; i < TL_ROWS; ++i) t.push_back(r0 + i < n ? static_cast<int>(r0 + i) : -1);
+    for (int i = 0; i < tr; ++i) t.push_back(r0 + i < n ? static_cast<int>(r0 + i) : -1);
     tiles.push_back(std::move(t));
     if (static_cast<int64_t>(tiles.size()) % per_pencil == 0) pencil.push_back(static_cast<int64_t>(tiles.size()));
   }
   if (pencil.back() != static_cast<int64_t>(tiles.size())) pencil.push_back(static_cast<int64_t>(tiles.size()));
+  return true;
 }
 
 // Slot plan: per pencil, a tile keeps the slots of the rows it shares with
 // the previous tile and fills freed slots with the rows it adds.
 bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, const std::vector<double>& v,
-                const std::vector<std::vector<int>>& tiles, const std::vector<int64_t>& pencil, TilePlan& P) {
+                const std::vector<std::vector<int>>& tiles, const std::vector<int64_t>& pencil, int min_loads,
+                TilePlan& P) {
   std::vector<int> slot_of(static_cast<size_t>(n), -1), stamp(static_cast<size_t>(n), -1);
   std::vector<int> uni, resident, freelist, kept;
   int slots = 0;
@@ -519,7 +552,7 @@ bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, c
     uni.clear();
     for (int r : t)
       if (r >= 0) uni.insert(uni.end(), ci.begin() + rp[r], ci.begin() + rp[r + 1]);
-    if (static_cast<int>(uni.size()) > 16 * TL_MAX_SLOTS) return false;  // cannot fit: skip the sort
+    if (uni.size() > t.size() * TL_MAX_SLOTS) return false;  // cannot fit: skip the sort
     std::sort(uni.begin(), uni.end());
     slots = std::max<int>(slots, static_cast<int>(std::unique(uni.begin(), uni.end()) - uni.begin()));
     if (slots > TL_MAX_SLOTS) return false;  // first tile that does not fit ends the plan
@@ -558,10 +591,10 @@ bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, c
         P.lpack.push_back(static_cast<int>((static_cast<unsigned>(slot_of[col]) << 24) | static_cast<unsigned>(col)));
       }
       P.loads += static_cast<int64_t>(P.lpack.size() - first);
-      // at least the TL_PF x 16 loads the kernel's registers take per tile:
+      // at least the loads the kernel's registers take per tile (min_loads):
       // the rest load the tile's first column into the spare slot (slots + 1)
       const int pad_row = uni.empty() ? 0 : uni.front();
-      while (P.lpack.size() - first < static_cast<size_t>(TL_PF * 16))
+      while (P.lpack.size() - first < static_cast<size_t>(min_loads))
         P.lpack.push_back(static_cast<int>((static_cast<unsigned>(slots + 1) << 24) | static_cast<unsigned>(pad_row)));
       P.lptr.push_back(static_cast<int64_t>(P.lpack.size()));
       // every row of the tile holds the longest row's count rounded up to 4
@@ -612,11 +645,28 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
   int64_t line = 0, plane = 0;
   if (!detect_grid(rp, ci, n, line, plane)) line = plane = 0;
+  // tile height: 32 rows (8 x 4 patches) on a grid whose lines and planes
+  // split into them, else 16; FE_TILE_ROWS=16 or 32 overrides (a 32-row plan
+  // that does not fit the LDS slots falls back to 16 rows)
+  int tr = plane > 0 ? 32 : 16;
+  if (const char* e = std::getenv("FE_TILE_ROWS")) {
+    const int want = std::atoi(e);
+    if (want == 16 || want == 32) tr = want;
+  }
   std::vector<std::vector<int>> tiles;
   std::vector<int64_t> pencil{0};
-  make_tiles(n, line, plane, tiles, pencil);
   TilePlan P;
-  if (!plan_slots(n, rp, ci, v, tiles, pencil, P)) return 1;
+  bool planned = false;
+  for (; !planned && tr >= 16; tr /= 2) {
+    tiles.clear();
+    pencil.assign(1, 0);
+    P = TilePlan{};
+    const int pf = tr == 16 ? tl_pf<16>() : tl_pf<32>();
+    planned = make_tiles(n, line, plane, tr, tiles, pencil) &&
+              plan_slots(n, rp, ci, v, tiles, pencil, pf * tr, P);
+    if (planned) break;
+  }
+  if (!planned) return 1;
   const double lpr = static_cast<double>(P.loads) / n;
   if (loads_per_row) *loads_per_row = lpr;
   // worth it only when the loads per row fall below half the nonzeros per row
@@ -663,5 +713,6 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   A->tl_loads = P.loads;
   A->tl_slots = P.slots;
   A->tl_self = P.self ? 1 : 0;
+  A->tl_trows = tr;
   return 0;
 }

@@ -201,6 +201,9 @@ TILED = {
     "p12": lambda: G.hamiltonian_27pt(12, seed=4, wells=5),
     "p20": lambda: G.hamiltonian_27pt(20, seed=6, wells=7),  # 20 planes: pencils of 16 and 4
     "c12": lambda: G.conductivity_3d(12, seed=3),            # 7-point
+    # 8 x 4 plane patches, 32-row tiles of 512 threads (line a multiple of 8)
+    "w16": lambda: G.hamiltonian_27pt(16, seed=5, wells=6),
+    "w24c": lambda: G.conductivity_3d(24, seed=4),           # 7-point, pencils of 24 planes
     # 16 consecutive rows per tile, pencils of 32 tiles (line not a multiple of 4)
     "h27": lambda: G.hamiltonian_27pt(9, seed=4, wells=5),   # 729 rows: a padded last tile
     "h27big": lambda: G.hamiltonian_27pt(23, seed=4, wells=9),
@@ -270,20 +273,20 @@ def test_spmm_tiled_larger_grid(dev, k, prec):
     assert np.max(np.abs(got[rows].astype(np.float64) - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 2)
 
 
-def _replay_tile_loads(A, line, plane, pz=24):
+def _replay_tile_loads(A, line, plane, ex=4, ey=4, pz=24):
     """X rows a pencil walk loads, replayed from the planning rule: tiles are
-    4 x 4 patches of a plane, pencils run through pz planes, and a tile loads
+    ex x ey patches of a plane, pencils run through pz planes, and a tile loads
     the columns of its rows that the previous tile of its pencil did not hold."""
     A = A.tocsr()
     n = A.shape[0]
-    nx, ny, nz = line // 4, plane // line // 4, n // plane
+    nx, ny, nz = line // ex, plane // line // ey, n // plane
     loads, widest = 0, 0
     for z0 in range(0, nz, pz):
         for py in range(ny):
             for px in range(nx):
                 held = set()
                 for z in range(z0, min(nz, z0 + pz)):
-                    rows = [z * plane + (4 * py + dy) * line + 4 * px + dx for dy in range(4) for dx in range(4)]
+                    rows = [z * plane + (ey * py + dy) * line + ex * px + dx for dy in range(ey) for dx in range(ex)]
                     need = set(np.concatenate([A.indices[A.indptr[r]:A.indptr[r + 1]] for r in rows]).tolist())
                     widest = max(widest, len(need))
                     loads += len(need - held)
@@ -291,19 +294,58 @@ def _replay_tile_loads(A, line, plane, pz=24):
     return loads / n, widest
 
 
-@pytest.mark.parametrize("g", [24, 28])
-def test_tiled_plan_load_count(dev, g):
+@pytest.mark.parametrize("g,rows", [(24, None), (24, 16), (28, None), (32, 32)])
+def test_tiled_plan_load_count(dev, monkeypatch, g, rows):
     """The host slot plan loads exactly the rows the pencil walk adds: its
     reported X rows per matrix row equal an independent replay of the rule
-    (24^3: one pencil per patch column; 28^3: a pencil of 24 planes and one
-    of 4)."""
+    (24^3: one pencil per patch column, 8 x 4 patches by default, 4 x 4 when
+    16-row tiles are asked for; 28^3: the line is no multiple of 8, so 4 x 4
+    patches, a pencil of 24 planes and one of 4; 32^3: 8 x 4 patches, pencils
+    of 24 and 8 planes)."""
+    if rows:
+        monkeypatch.setenv("FE_TILE_ROWS", str(rows))
+    else:
+        monkeypatch.delenv("FE_TILE_ROWS", raising=False)
     A = G.hamiltonian_27pt(g, seed=2, wells=4)
     C = dev.csr(A)
     built, lpr = C.build_tiled()
     assert built
-    want, widest = _replay_tile_loads(A, g, g * g)
-    assert widest <= 108
+    ex = 8 if g % 8 == 0 and rows != 16 else 4
+    want, widest = _replay_tile_loads(A, g, g * g, ex=ex)
+    assert widest <= 3 * (ex + 2) * 6
     assert lpr == pytest.approx(want, rel=0, abs=1e-12)
+    if ex == 8:  # fewer loads than the 4 x 4 plan of the same grid
+        assert lpr < _replay_tile_loads(A, g, g * g, ex=4)[0] * 0.9
+
+
+@pytest.mark.parametrize("rows", [16, 32])
+@pytest.mark.parametrize("k,prec", [(64, _fe.F32), (100, _fe.F32), (512, _fe.F32), (32, _fe.F64), (48, _fe.F64)])
+def test_spmm_tiled_both_heights(dev, monkeypatch, rows, k, prec):
+    """Both tile heights, forced, bitwise equal to the CSR kernel: a 27-point
+    grid whose line splits into 8 x 4 patches, and a banded matrix without a
+    grid (consecutive-row tiles of 16 or 32 rows, padded last tile)."""
+    monkeypatch.setenv("FE_TILE_ROWS", str(rows))
+    rng = np.random.default_rng(rows + k)
+    band = sp.diags([rng.uniform(-1, 1, 3001 - abs(o)) for o in range(-9, 10)], list(range(-9, 10)))
+    for A in (G.hamiltonian_27pt(16, seed=9, wells=4), sp.csr_matrix(band + band.T)):
+        n = A.shape[0]
+        dt = np.float64 if prec == _fe.F64 else np.float32
+        X = rng.uniform(-1, 1, (n, k))
+        Z = rng.uniform(-1, 1, (n, k))
+        C = dev.csr(A)
+        built, lpr = C.build_tiled()
+        assert built
+        Xd, Zd = dev.put(X, dt), dev.put(Z, dt)
+        Y1, Y2 = dev.buffer((n, k), dt), dev.buffer((n, k), dt)
+        _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y1.ptr, k, 0.7, 0.3, Zd.ptr, k, -0.9))
+        C.use(_fe.FMT_TILED)
+        _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y2.ptr, k, 0.7, 0.3, Zd.ptr, k, -0.9))
+        got = Y2.download()
+        assert np.array_equal(Y1.download(), got)
+        Xr, Zr = X.astype(dt).astype(np.float64), Z.astype(dt).astype(np.float64)
+        ref = orc.spmm(A, Xr, 0.7, 0.0) + 0.3 * Xr - 0.9 * Zr
+        tol = 1e-13 if prec == _fe.F64 else 2e-6
+        assert np.max(np.abs(got.astype(np.float64) - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 2)
 
 
 def test_spmm_tiled_ragged_and_in_place(dev):
