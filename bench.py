@@ -61,7 +61,9 @@ KERNEL_OF_FORMAT = {"csr": "k_spmm_csr", "sell": "k_spmm_sell", "blocked": "k_sp
 
 def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
     """Average time of the fused Chebyshev step Y = 2c A X + 2e X - Z on the
-    library stream (HIP events), and its algorithmic-byte bandwidth. fmt
+    library stream (HIP events), its algorithmic bytes and the kernel's name
+    (the stencil form of the tiled copy reads the diagonal and 27
+    coefficients instead of the CSR arrays). fmt
     "blocked" times the row-block copy (4-row blocks sharing a column list),
     "tiled" the LDS-tiled copy (on a grid 8 x 4 or 4 x 4 plane patches,
     pencils through the planes; "tiled16" / "tiled32" force the height)."""
@@ -78,21 +80,28 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
             raise RuntimeError(f"row-block copy not built (reuse {reuse:.2f})")
         C.block_schedule()  # strip order for grid stencils, as the solvers use it
         C.use(_fe.FMT_BLOCKED)
+    kernel = KERNEL_OF_FORMAT[fmt.rstrip("0123456789")]
     if fmt in ("tiled", "tiled16", "tiled32"):
-        # tiled16 / tiled32: the tile height forced (FE_TILE_ROWS), for comparison
-        saved = os.environ.get("FE_TILE_ROWS")
+        # tiled16 / tiled32: entry-list tiles of that height forced
+        # (FE_TILE_STENCIL=0, FE_TILE_ROWS), for comparison with the stencil form
+        saved = {v: os.environ.get(v) for v in ("FE_TILE_ROWS", "FE_TILE_STENCIL")}
         if fmt != "tiled":
             os.environ["FE_TILE_ROWS"] = fmt[5:]
+            os.environ["FE_TILE_STENCIL"] = "0"
         try:
             built, lpr = C.build_tiled()
         finally:
-            if saved is None:
-                os.environ.pop("FE_TILE_ROWS", None)
-            else:
-                os.environ["FE_TILE_ROWS"] = saved
+            for v, old in saved.items():
+                if old is None:
+                    os.environ.pop(v, None)
+                else:
+                    os.environ[v] = old
         if not built:
             raise RuntimeError(f"tiled copy not built ({lpr:.2f} X rows loaded per row)")
         C.use(_fe.FMT_TILED)
+        stencil = C.tiled_kind() == 64
+        if stencil:
+            kernel = "k_spmm_stencil"
     n = A.shape[0]
     dt = np.float64 if prec == _fe.F64 else np.float32
     s = np.dtype(dt).itemsize
@@ -113,10 +122,13 @@ def spmm_roofline(A, k, prec, reps=20, fmt="csr"):
     _fe.check(L.fe_event_elapsed_ms(dev.ctx, 0, 1, ctypes.byref(ms)))
     per = ms.value / reps
     nnz = A.nnz
-    algo = nnz * (s + 4) + 4 * (n + 1) + 3 * n * k * s
+    if kernel == "k_spmm_stencil":  # no entry lists: the diagonal and 27 coefficients
+        algo = (n + 27) * s + 3 * n * k * s
+    else:
+        algo = nnz * (s + 4) + 4 * (n + 1) + 3 * n * k * s
     del X, Y, C
     dev.close()
-    return per, algo
+    return per, algo, kernel
 
 
 MFMA_PEAK_TFS = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}
@@ -430,7 +442,7 @@ def c4_bench(maxiter=200):
     out = {"n": A.shape[0], "nnz": int(A.nnz)}
     for fmt in ("csr", "sell"):
         for pname, prec in (("fp32", _fe.F32), ("fp64", _fe.F64)):
-            per, algo = spmm_roofline(A, 48, prec, fmt=fmt)
+            per, algo, _ = spmm_roofline(A, 48, prec, fmt=fmt)
             out[f"step_{fmt}_{pname}_k48_ms"] = per
             out[f"step_{fmt}_{pname}_k48_gbs"] = algo / (per * 1e-3) / 1e9
             log(f"c4 fused step {fmt} {pname} k=48: {per:.3f} ms {algo / (per * 1e-3) / 1e9:.0f} GB/s")
@@ -669,7 +681,7 @@ def main():
                 for pname, prec in (("fp32", _fe.F32), ("fp64", _fe.F64)):
                     if pname not in args.kernel_precs.split(","):
                         continue
-                    per, algo = spmm_roofline(A, kk, prec, fmt=fmt)
+                    per, algo, _ = spmm_roofline(A, kk, prec, fmt=fmt)
                     out[f"{fmt}_{pname}_k{kk}_ms"] = per
                     out[f"{fmt}_{pname}_k{kk}_gbs"] = algo / (per * 1e-3) / 1e9
                     log(f"fused step {fmt} {pname} k={kk}: {per:.3f} ms {algo / (per * 1e-3) / 1e9:.0f} GB/s")
@@ -744,11 +756,11 @@ def main():
     for fmt in {"auto": ["tiled", "blocked", "csr"], "tiled": ["tiled"], "blocked": ["blocked"],
                 "csr": ["csr"], "sell": ["sell"]}[args.format]:
         try:
-            per32, algo32 = spmm_roofline(A, k, _fe.F32, fmt=fmt)
+            per32, algo32, kname = spmm_roofline(A, k, _fe.F32, fmt=fmt)
             break
         except RuntimeError:
             continue
-    per64, algo64 = spmm_roofline(A, k, _fe.F64, fmt=fmt)
+    per64, algo64, _ = spmm_roofline(A, k, _fe.F64, fmt=fmt)
     achieved = algo32 / (per32 * 1e-3) / 1e9
     extra["fused_step_k"] = k
     extra["fused_step_fp32_ms"] = per32
@@ -758,8 +770,8 @@ def main():
         f"{extra['fused_step_fp64_gbs']:.0f} GB/s")
     kn = cfg["nev"]
     if kn != k:
-        p32n, a32n = spmm_roofline(A, kn, _fe.F32, fmt=fmt)
-        p64n, a64n = spmm_roofline(A, kn, _fe.F64, fmt=fmt)
+        p32n, a32n, _ = spmm_roofline(A, kn, _fe.F32, fmt=fmt)
+        p64n, a64n, _ = spmm_roofline(A, kn, _fe.F64, fmt=fmt)
         extra[f"fused_step_k{kn}"] = {"fp32_ms": p32n, "fp32_gbs": a32n / (p32n * 1e-3) / 1e9,
                                       "fp32_frac": a32n / (p32n * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                       "fp64_ms": p64n, "fp64_gbs": a64n / (p64n * 1e-3) / 1e9}
@@ -819,7 +831,7 @@ def main():
         "accuracy": out_acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"{KERNEL_OF_FORMAT[fmt]} fused Chebyshev step FP32 k={k}",
+                     "kernel": f"{kname} fused Chebyshev step FP32 k={k}",
                      "algo_bytes": algo32},
         "cpu_baseline": cpu,
         "extra": extra,

@@ -59,6 +59,7 @@ def lib():
         L.fe_csr_build_blocked.argtypes = [vp, vp, i32, C.POINTER(dbl)]
         L.fe_csr_block_schedule.argtypes = [vp, vp, i64, i64]
         L.fe_csr_build_tiled.argtypes = [vp, vp, C.POINTER(dbl)]
+        L.fe_csr_tiled_kind.argtypes = [vp]
         L.fe_fill_random.argtypes = [vp, i32, i64, i32, vp, i64, C.c_uint64]
         L.fe_scale_rows.argtypes = [vp, i32, i64, i32, vp, i64, vp]
         L.fe_last_error.argtypes = []
@@ -192,6 +193,11 @@ class Csr:
         r = C.c_double(0.0)
         rc = check(lib().fe_csr_build_tiled(self.dev.ctx, self.h, C.byref(r)))
         return rc == 0, r.value
+
+    def tiled_kind(self):
+        """Form of the tiled copy: 0 none, 16 / 32 entry-list tiles of that
+        many rows, 64 the stencil form (8 x 8 patches)."""
+        return lib().fe_csr_tiled_kind(self.h)
 
     def use(self, fmt):
         check(lib().fe_csr_use(self.h, fmt))

@@ -124,6 +124,18 @@ struct fe_csr_s {
   int* tl_sslot = nullptr;       // slot of each tile row's own X row (-1: not in its tile)
   int tl_self = 0;               // every tile row's own X row is in its tile's slots
   int tl_trows = 16;             // rows per tile: 16 (256 threads) or 32 (512 threads)
+  // stencil form of the tiled copy (a grid operator whose off-diagonal
+  // coefficients are one constant per offset): no entry lists, only the
+  // diagonal and the offsets' coefficients; X planes of 8 x 8 patches (with
+  // their halo) staged in LDS (fe_spmm_tile.hip)
+  int st_on = 0;                 // the tiled copy is the stencil form
+  int st_p27 = 1;                // 27-point (else 7-point: faces and centre)
+  int st_group = 4;              // pencils per work group
+  int st_nx = 0, st_ny = 0, st_nz = 0;
+  double st_w[27] = {};          // coefficient of offset (dz + 1) * 9 + (dy + 1) * 3 + dx + 1 (centre: diagonal)
+  double* st_d64 = nullptr;      // diagonal (zero where a row stores none)
+  float* st_d32 = nullptr;
+  void* st_junk = nullptr;       // 2 x 512 x 16 bytes that rows outside the grid write (never read)
   int use_tl = 0;
 };
 

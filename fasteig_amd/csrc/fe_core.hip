@@ -675,7 +675,7 @@ int fe_csr_destroy(fe_csr A) {
   void* ptrs[] = {A->rowptr, A->colind, A->vals64, A->vals32, A->sell_ptr, A->sell_len,
                   A->sell_perm, A->sell_col, A->sell_v64, A->sell_v32, A->bg_bptr, A->bg_ucol, A->bg_v64, A->bg_v32,
                   A->bg_sched, A->tl_pencil, A->tl_rows, A->tl_ebase, A->tl_maxe, A->tl_ent32, A->tl_eslot,
-                  A->tl_ev64, A->tl_lptr, A->tl_lpack, A->tl_sslot};
+                  A->tl_ev64, A->tl_lptr, A->tl_lpack, A->tl_sslot, A->st_d64, A->st_d32, A->st_junk};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete A;
@@ -692,7 +692,7 @@ int fe_csr_info(fe_csr A, int* n, int64_t* nnz) {
 int fe_csr_scale_sym(fe_ctx ctx, fe_csr A, const double* d) {
   FE_ARG(ctx && A && d, "fe_csr_scale_sym: null argument");
   // the SELL, row-block and tiled copies hold their own values
-  FE_ARG(!A->sell_col && !A->bg_ucol && !A->tl_rows,
+  FE_ARG(!A->sell_col && !A->bg_ucol && !A->tl_rows && !A->st_d64,
          "fe_csr_scale_sym: scale before building the SELL, row-block or tiled copy");
   hipLaunchKernelGGL(k_scale_sym, dim3(blocks_for(A->n, 256)), dim3(256), 0, ctx->stream, A->n,
                      A->rowptr, A->colind, A->vals64, A->vals32, d);

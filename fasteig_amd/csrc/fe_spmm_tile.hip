@@ -51,14 +51,16 @@ constexpr int tl_pf() { return TR == 16 ? 3 : 2; }
 // 32-row workgroups fits a CU
 template <int TR>
 constexpr int tl_waves() { return TR == 16 ? 5 : 6; }
-#ifndef TL_PENCIL_TILES
-#define TL_PENCIL_TILES 24
-#endif
-constexpr int TL_PENCIL = TL_PENCIL_TILES;  // tiles per pencil at most
-#ifndef TL_BLOCK_EDGE
-#define TL_BLOCK_EDGE 4
-#endif
-constexpr int TL_BLOCK = TL_BLOCK_EDGE;  // patches per block edge in the pencil order of a grid
+// Tiles per pencil: 24 by default (through 24 planes of a grid; 16 planes ran
+// 1-2% slower, 8 planes 6% slower with 16-row tiles), at most tl_pencil_max
+// (the LDS tables of a pencil); FE_TILE_PENCIL overrides the default.
+constexpr int TL_PENCIL = 24;
+template <int TR>
+constexpr int tl_pencil_max() { return TR == 16 ? 24 : 32; }
+// Edge, in grid points, of the square blocks of patches whose pencils are
+// taken together (FE_TILE_BLOCK overrides): 16, i.e. 4 x 4 patches of 4 x 4
+// or 2 x 4 patches of 8 x 4.
+constexpr int TL_BLOCK_POINTS = 16;
 // Pencils per work group: the work items of a group of consecutive pencils
 // are taken slice by slice, so the same slice of neighbouring pencils (which
 // share halo rows) starts at nearly the same time on one XCD, while the
@@ -188,13 +190,14 @@ __global__ __launch_bounds__(16 * TR, (sizeof(T) == 4 && O32 && SELF) ? tl_waves
     int nslices, const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta, const T* Z, int64_t ldz,
     T gamma, int64_t per_xcd, int64_t nwork) {
   extern __shared__ float4 xs[];  // (zslot + 2) slots of 16 x 16 bytes: zslot holds zeros, zslot + 1 padding loads
-  __shared__ int s_lp[TL_PENCIL + 1];             // load offsets of the pencil's tiles
-  __shared__ int s_eb[TL_PENCIL + 1];             // entry offsets of its tiles
-  __shared__ int s_rows[TL_PENCIL * TR];          // its tile rows
-  __shared__ int s_maxe[TL_PENCIL];
+  constexpr int PM = tl_pencil_max<TR>();
+  __shared__ int s_lp[PM + 1];             // load offsets of the pencil's tiles
+  __shared__ int s_eb[PM + 1];             // entry offsets of its tiles
+  __shared__ int s_rows[PM * TR];          // its tile rows
+  __shared__ int s_maxe[PM];
   // own-row slots of its tile rows (slots < TL_MAX_SLOTS fit a byte: with
   // int entries the workgroup's LDS passed the 32 KB that five per CU allow)
-  __shared__ unsigned char s_self[SELF ? TL_PENCIL * TR : 1];
+  __shared__ unsigned char s_self[SELF ? PM * TR : 1];
   constexpr int NT = 16 * TR;       // threads
   constexpr int PF = tl_pf<TR>();   // X row loads per thread per tile in registers
   constexpr int V = 16 / (int)sizeof(T);
@@ -219,7 +222,7 @@ __global__ __launch_bounds__(16 * TR, (sizeof(T) == 4 && O32 && SELF) ? tl_waves
   const bool fx = beta != T(0);
   const bool full = FULL || c0 + CW <= k;  // the whole slice lies inside the block
   const int64_t t0 = pencil[p];
-  const int nt = static_cast<int>(pencil[p + 1] - t0);  // at most TL_PENCIL
+  const int nt = static_cast<int>(pencil[p + 1] - t0);  // at most PM
   const int64_t lbase = lptr[t0], eb0 = ebase[t0];
   for (int i = tid; i <= nt; i += NT) s_lp[i] = static_cast<int>(lptr[t0 + i] - lbase);
   for (int i = tid; i <= nt; i += NT) s_eb[i] = static_cast<int>(ebase[t0 + i] - eb0);
@@ -427,6 +430,314 @@ __global__ __launch_bounds__(16 * TR, (sizeof(T) == 4 && O32 && SELF) ? tl_waves
   }
 }
 
+// Stencil form: a grid operator (rows in natural order of an nx x ny x nz
+// grid) whose off-diagonal coefficient is one constant per offset within the
+// 3 x 3 x 3 neighbourhood, any diagonal, and no entries past the grid's faces.
+// No entry lists: the kernel reads the diagonal and takes the coefficients
+// from its arguments. A workgroup of 512 threads takes an 8 x 8 patch of a
+// plane and one 256-byte column slice (16 lanes x 16 bytes per row; a thread
+// computes two neighbouring rows of a line, whose neighbourhoods share two of
+// their three columns per line: 4 LDS reads feed both rows' 6 products), and
+// walks a pencil of planes. LDS holds three planes of the patch with its halo
+// (10 x 10 rows each) as a ring: while plane z is computed, plane z + 1 sits
+// in LDS and plane z + 3 is loaded into registers (two planes ahead), so a row
+// of X is loaded once per pencil: 100 X rows per 64 output rows reach the
+// workgroup, against 60 per 32 for the 8 x 4 entry-list tiles. Halo rows
+// outside the grid are zeros in LDS.
+//
+// Each row's sum runs over the offsets in increasing column order, the order
+// of its sorted CSR entries: the product is bitwise the CSR kernel's (a term
+// past the grid's faces, or of an offset the row does not store in 27-point
+// form, adds coefficient x 0 or 0 x X, which leaves the sum unchanged while
+// X is finite).
+// Work items: groups of ST_GROUP consecutive pencils taken slice by slice, in
+// contiguous ranges per XCD: the slices of a row are read by one XCD at
+// nearly the same time, and neighbouring pencils share halo rows through its
+// L2.
+template <typename T>
+struct StencilCoef {
+  T w[27];
+};
+
+constexpr int ST_E = 8;         // patch edge (rows per line and lines per patch)
+constexpr int ST_H = ST_E + 2;  // with the halo
+constexpr int ST_SLOTS = ST_H * ST_H;
+constexpr int ST_PENCIL = 24;   // planes per pencil
+constexpr int ST_GROUP = 4;     // pencils per work group (FE_STENCIL_GROUP overrides)
+constexpr int ST_NT = 512;      // threads
+constexpr int ST_LD = (ST_SLOTS * 16 + ST_NT - 1) / ST_NT;  // 16-byte X loads per thread per plane (4)
+
+template <typename T, bool O32, bool FULL, bool FZ, bool P27>
+__global__ __launch_bounds__(ST_NT, 4) void k_spmm_stencil(
+    const T* __restrict__ diag, StencilCoef<T> sc, int nx, int ny, int nz, int npx, int npy, int k, int nslices,
+    const T* __restrict__ X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta, const T* Z, int64_t ldz, T gamma,
+    int64_t per_xcd, int64_t nwork, int group, float4* junk) {
+  constexpr int V = 16 / (int)sizeof(T);
+  constexpr int CW = 16 * V;
+  constexpr int PB = (ST_SLOTS + 1) * 16;  // float4 per plane buffer: the slots and a spare one
+  __shared__ float4 xs[3 * PB];            // 3 plane buffers (76 KB)
+  const int64_t work = xcd_tile(blockIdx.x, per_xcd);
+  if (work >= nwork) return;  // the whole workgroup, before any barrier
+  // work item -> (pencil, slice): groups of `group` consecutive pencils,
+  // slice-major inside a group (the last group may be shorter)
+  const int64_t np = nwork / nslices;
+  const int64_t gsz = static_cast<int64_t>(group) * nslices;
+  const int64_t gi = work / gsz;
+  const int64_t grem = work - gi * gsz;
+  const int64_t gp0 = gi * group;
+  const int64_t gn = min(static_cast<int64_t>(group), np - gp0);
+  const int sl = static_cast<int>(grem / gn);
+  const int64_t pen = gp0 + (grem - sl * gn);
+  const int64_t pp = static_cast<int64_t>(npx) * npy;
+  const int zr = static_cast<int>(pen / pp);
+  const int rem = static_cast<int>(pen - zr * pp);
+  const int py = rem / npx, px = rem - py * npx;
+  const int x0 = px * ST_E, y0 = py * ST_E, z0 = zr * ST_PENCIL;
+  const int zn = min(nz - z0, ST_PENCIL);  // planes of this pencil
+  const int line = nx;
+  const int plane = nx * ny;
+  const int c0 = sl * CW;
+  const int tid = threadIdx.x;
+  const int sub = tid & 15;
+  const int g = tid >> 4;  // 0..31
+  const int c = c0 + sub * V;
+  const bool full = FULL || c0 + CW <= k;
+  // this thread's halo parts: slots g + 32 j (the last one for g < 4)
+  int hoff[ST_LD];
+  bool hok[ST_LD];
+#pragma unroll
+  for (int j = 0; j < ST_LD; ++j) {
+    const int s = g + 32 * j;
+    const int hy = s / ST_H, hx = s - hy * ST_H;
+    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+    hok[j] = s < ST_SLOTS && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    hoff[j] = hok[j] ? gy * line + gx : 0;
+  }
+  const bool last = g + 32 * (ST_LD - 1) < ST_SLOTS;  // the last part is a slot
+  // this thread's two output rows: (xa, y) and (xa + 1, y)
+  const int lx = 2 * (g & 3), ly = g >> 2;
+  const int xa = x0 + lx, yy = y0 + ly;
+  const bool oka = xa < nx && yy < ny, okb = xa + 1 < nx && yy < ny;
+  const int roffa = oka ? yy * line + xa : 0;
+  const int roffb = okb ? roffa + 1 : roffa;
+  const int ctr = (ly + 1) * ST_H + lx + 1;  // slot of row a
+
+  auto load_x = [&](int row) {  // 16 bytes of X row `row` (>= 0) in this slice, zeros past k
+    const T* src = elem<O32>(X, row, ldx, c);
+    float4 f;
+    if (full || c + V <= k) {
+      f = *reinterpret_cast<const float4*>(src);
+    } else {
+      T tmp[V];
+#pragma unroll
+      for (int q = 0; q < V; ++q) tmp[q] = c + q < k ? src[q] : T(0);
+      __builtin_memcpy(&f, tmp, 16);
+    }
+    return f;
+  };
+  // plane zz's parts (a plane outside the grid loads a plane inside it and is zeroed when stored)
+  // planes past the pencil's last neighbour plane reload that plane (an L2
+  // hit): their buffers are never read
+  const int zlo = max(z0 - 1, 0), zhi = min(z0 + zn, nz - 1);
+  auto load_plane = [&](int zz, float4 (&r)[ST_LD]) {
+    const int zc = min(max(zz, zlo), zhi);
+#pragma unroll
+    for (int j = 0; j < ST_LD; ++j) r[j] = load_x(zc * plane + hoff[j]);
+  };
+  // every memory operation of the plane loop is unconditional (a store under
+  // a branch left the compiler unable to count the loads in flight, and it
+  // waited for all of them at the top of each step): a thread without a 4th
+  // slot writes the spare slot, rows outside the grid write the junk rows
+  const int s3 = last ? g + 32 * (ST_LD - 1) : ST_SLOTS;
+  // (selects per component: a select of two float4 objects became a select
+  // of their stack addresses)
+  auto keep = [](bool ok, const float4& v) {
+    return make_float4(ok ? v.x : 0.f, ok ? v.y : 0.f, ok ? v.z : 0.f, ok ? v.w : 0.f);
+  };
+  auto store_plane = [&](int zz, int buf, const float4 (&r)[ST_LD]) {
+    const bool zin = zz >= 0 && zz < nz;
+#pragma unroll
+    for (int j = 0; j < ST_LD - 1; ++j) xs[buf * PB + (g + 32 * j) * 16 + sub] = keep(zin && hok[j], r[j]);
+    xs[buf * PB + s3 * 16 + sub] = keep(zin && hok[ST_LD - 1], r[ST_LD - 1]);
+  };
+  // Z rows and diagonals of plane zz (clamped into the pencil: past it, unused)
+  struct ZD {
+    T za[FZ ? V : 1], zb[FZ ? V : 1];
+    T da, db;
+  };
+  auto load_row = [&](int row, T (&v)[FZ ? V : 1]) {
+    if constexpr (FZ) {
+      if (full || c + V <= k) {
+        const nt_f4 f = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(elem<O32>(Z, row, ldz, c)));
+        __builtin_memcpy(v, &f, 16);
+      } else {
+#pragma unroll
+        for (int q = 0; q < V; ++q) v[q] = c + q < k ? elem<O32>(Z, row, ldz, c)[q] : T(0);
+      }
+    }
+  };
+  auto load_zd = [&](int zz, ZD& o) {
+    const int base = min(zz, z0 + zn - 1) * plane;
+    o.da = diag[base + roffa];
+    o.db = diag[base + roffb];
+    load_row(base + roffa, o.za);
+    load_row(base + roffb, o.zb);
+  };
+
+  {  // planes z0 - 1 and z0 into buffers 0 and 1
+    float4 r[ST_LD];
+    load_plane(z0 - 1, r);
+    store_plane(z0 - 1, 0, r);
+    load_plane(z0, r);
+    store_plane(z0, 1, r);
+  }
+  float4 pre[2][ST_LD];  // planes two ahead, by the parity of the step that stores them
+  ZD zd[2];
+  load_plane(z0 + 1, pre[0]);
+  load_plane(z0 + 2, pre[1]);
+  load_zd(z0, zd[0]);
+  lds_barrier();
+  int bp = 0, bc = 1, bn = 2;  // buffers of planes z - 1, z, z + 1
+
+  auto put = [&](T* y, const T (&out)[V]) {
+    if (full || c + V <= k) {
+      nt_f4 o;
+      __builtin_memcpy(&o, out, 16);
+      __builtin_nontemporal_store(o, reinterpret_cast<nt_f4*>(y));
+    } else {
+#pragma unroll
+      for (int q = 0; q < V; ++q)
+        if (c + q < k) y[q] = out[q];
+    }
+  };
+  auto step = [&](auto Pc, int p) {
+    constexpr int P = decltype(Pc)::value;
+    const int z = z0 + p;
+    store_plane(z + 1, bn, pre[P]);  // its buffer held plane z - 2, free since the last barrier
+    lds_barrier();
+    load_plane(z + 3, pre[P]);
+    load_zd(z + 1, zd[1 - P]);
+    T acca[V], accb[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) acca[q] = accb[q] = T(0);
+    // line l = (dz + 1) * 3 + dy + 1 of the neighbourhood: X at x offsets
+    // -1..2 from row a (the dx = -1 and +1 offsets only where the line has
+    // them). The reads of line l + 1 are issued before the products of line
+    // l, and a barrier that takes the sums after line l's products keeps
+    // later reads from moving up: with the products left to sink below all
+    // 36 reads the registers passed the 128 that four waves per SIMD allow.
+    T xl[2][4][V];
+    auto rd = [&](auto L) {
+      constexpr int l = decltype(L)::value;
+      constexpr int dz = l / 3 - 1, dy = l % 3 - 1;
+      constexpr bool side = P27 || (dz == 0 && dy == 0);
+      const int buf = dz < 0 ? bp : (dz == 0 ? bc : bn);
+      const float4* src = xs + buf * PB + (ctr + dy * ST_H - 1) * 16 + sub;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (side || i == 1 || i == 2) __builtin_memcpy(xl[l & 1][i], &src[16 * i], 16);
+    };
+    auto fm = [&](auto L) {
+      constexpr int l = decltype(L)::value;
+      constexpr int dz = l / 3 - 1, dy = l % 3 - 1;
+      constexpr bool mid = dz == 0 && dy == 0;
+      constexpr bool side = P27 || mid;
+      const T(&x)[4][V] = xl[l & 1];
+      const T wl = sc.w[l * 3], wc = sc.w[l * 3 + 1], wr = sc.w[l * 3 + 2];
+      const T wca = mid ? zd[P].da : wc, wcb = mid ? zd[P].db : wc;
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        if constexpr (side) acca[q] += wl * x[0][q];
+        acca[q] += wca * x[1][q];
+        if constexpr (side) acca[q] += wr * x[2][q];
+        if constexpr (side) accb[q] += wl * x[1][q];
+        accb[q] += wcb * x[2][q];
+        if constexpr (side) accb[q] += wr * x[3][q];
+      }
+#pragma unroll
+      for (int q = 0; q < V; ++q) asm volatile("" : "+v"(acca[q]), "+v"(accb[q])::"memory");
+    };
+    auto line_terms = [&](auto L) {
+      constexpr int l = decltype(L)::value;
+      if constexpr (l + 1 < 9) rd(std::integral_constant<int, l + 1>{});
+      fm(L);
+    };
+    rd(std::integral_constant<int, 0>{});
+    Unroll<0, 9>::run(line_terms);
+    T outa[V], outb[V];
+    {
+      const float4* xc = xs + bc * PB + ctr * 16 + sub;
+      T xra[V], xrb[V];
+      __builtin_memcpy(xra, &xc[0], 16);
+      __builtin_memcpy(xrb, &xc[16], 16);
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        outa[q] = alpha * acca[q];
+        outb[q] = alpha * accb[q];
+        if (beta != T(0)) {
+          outa[q] += beta * xra[q];
+          outb[q] += beta * xrb[q];
+        }
+        if constexpr (FZ) {
+          outa[q] += gamma * zd[P].za[q];
+          outb[q] += gamma * zd[P].zb[q];
+        }
+      }
+    }
+    if constexpr (FULL) {  // every lane's 16 bytes lie inside the block
+      T* ya = oka ? elem<O32>(Y, z * plane + roffa, ldy, c) : reinterpret_cast<T*>(junk + tid);
+      T* yb = okb ? elem<O32>(Y, z * plane + roffb, ldy, c) : reinterpret_cast<T*>(junk + ST_NT + tid);
+      put(ya, outa);
+      put(yb, outb);
+    } else if (c < k) {
+      if (oka) put(elem<O32>(Y, z * plane + roffa, ldy, c), outa);
+      if (okb) put(elem<O32>(Y, z * plane + roffb, ldy, c), outb);
+    }
+    lds_barrier();  // every thread is done with plane z - 1's buffer
+    const int t = bp;
+    bp = bc;
+    bc = bn;
+    bn = t;
+  };
+  for (int p = 0; p < zn; p += 2) {
+    step(std::integral_constant<int, 0>{}, p);
+    if (p + 1 < zn) step(std::integral_constant<int, 1>{}, p + 1);
+  }
+}
+
+template <typename T>
+static int launch_stencil(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
+                          const T* Z, int64_t ldz, T gamma) {
+  constexpr int CW = 16 * (16 / (int)sizeof(T));
+  const int nslices = (k + CW - 1) / CW;
+  const int npx = (A->st_nx + ST_E - 1) / ST_E, npy = (A->st_ny + ST_E - 1) / ST_E;
+  const int nzr = (A->st_nz + ST_PENCIL - 1) / ST_PENCIL;
+  const int64_t nwork = static_cast<int64_t>(npx) * npy * nzr * nslices;
+  const int64_t nb8 = (nwork + 7) / 8 * 8;
+  const double lim = 4294967295.0 / sizeof(T);
+  const bool o32 = static_cast<double>(A->n) * ldx < lim && static_cast<double>(A->n) * ldy < lim &&
+                   (!Z || static_cast<double>(A->n) * ldz < lim);
+  const bool full = k % CW == 0;
+  const bool fz = Z && gamma != T(0);
+  StencilCoef<T> sc;
+  for (int o = 0; o < 27; ++o) sc.w[o] = static_cast<T>(A->st_w[o]);
+  const T* d = sizeof(T) == 8 ? reinterpret_cast<const T*>(A->st_d64) : reinterpret_cast<const T*>(A->st_d32);
+  auto pick = [&](auto zterm, auto p27) {
+    constexpr bool F = decltype(zterm)::value, Q = decltype(p27)::value;
+    return o32 ? (full ? k_spmm_stencil<T, true, true, F, Q> : k_spmm_stencil<T, true, false, F, Q>)
+               : (full ? k_spmm_stencil<T, false, true, F, Q> : k_spmm_stencil<T, false, false, F, Q>);
+  };
+  auto pick_p = [&](auto zterm) {
+    return A->st_p27 ? pick(zterm, std::true_type{}) : pick(zterm, std::false_type{});
+  };
+  auto kern = fz ? pick_p(std::true_type{}) : pick_p(std::false_type{});
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nb8)), dim3(ST_NT), 0, ctx->stream, d, sc, A->st_nx, A->st_ny,
+                     A->st_nz, npx, npy, k, nslices, X, ldx, Y, ldy, alpha, beta, Z, ldz, gamma, nb8 / 8, nwork,
+                     A->st_group, reinterpret_cast<float4*>(A->st_junk));
+  FE_CHECK_LAUNCH();
+  return 0;
+}
+
 template <typename T>
 static int launch_tile(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* Y, int64_t ldy, T alpha, T beta,
                        const T* Z, int64_t ldz, T gamma) {
@@ -467,6 +778,13 @@ static int launch_tile(fe_ctx ctx, fe_csr A, int k, const T* X, int64_t ldx, T* 
 // fe_csr_use selects it only once the copy is complete (tl_npencils > 0).
 int spmm_tiled(fe_ctx ctx, fe_csr A, int prec, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
                double alpha, double beta, const void* Z, int64_t ldz, double gamma) {
+  if (A->st_on) {
+    if (prec == FE_F64)
+      return launch_stencil<double>(ctx, A, k, (const double*)X, ldx, (double*)Y, ldy, alpha, beta,
+                                    (const double*)Z, ldz, gamma);
+    return launch_stencil<float>(ctx, A, k, (const float*)X, ldx, (float*)Y, ldy, (float)alpha, (float)beta,
+                                 (const float*)Z, ldz, (float)gamma);
+  }
   if (prec == FE_F64)
     return launch_tile<double>(ctx, A, k, (const double*)X, ldx, (double*)Y, ldy, alpha, beta, (const double*)Z,
                                ldz, gamma);
@@ -500,19 +818,18 @@ struct TilePlan {
 // patches, any other matrix runs of consecutive rows; false for 32-row tiles
 // on a grid that does not split into 8 x 4 patches (its 4 x 4 patches are the
 // better plan).
-bool make_tiles(int n, int64_t line, int64_t plane, int tr, std::vector<std::vector<int>>& tiles,
+bool make_tiles(int n, int64_t line, int64_t plane, int tr, int pz, int block, std::vector<std::vector<int>>& tiles,
                 std::vector<int64_t>& pencil) {
   const int px = tr == 32 ? 8 : 4, py = 4;  // patch extent along the line and across lines
   const bool splits = plane > 0 && line % px == 0 && (plane / line) % py == 0 && n % plane == 0;
   if (plane > 0 && !splits && tr == 32) return false;
   if (splits) {
-    // px x py patches; a pencil runs through up to 24 consecutive planes; the
-    // pencils of one range of planes are taken in blocks of 16 x 16 points
-    // (TL_BLOCK x TL_BLOCK 4 x 4 patches; half as many 8 x 4 ones along the
-    // line), row of blocks by row, so the pencils an XCD runs at one time
+    // px x py patches; a pencil runs through up to pz consecutive planes; the
+    // pencils of one range of planes are taken in blocks of block x block
+    // points, row of blocks by row, so the pencils an XCD runs at one time
     // cover a compact area whose shared halo rows its L2 serves
-    const int64_t nx = line / px, ny = plane / line / py, nz = n / plane, pz = TL_PENCIL;
-    const int64_t bxe = std::max<int64_t>(1, TL_BLOCK * 4 / px), bye = TL_BLOCK;
+    const int64_t nx = line / px, ny = plane / line / py, nz = n / plane;
+    const int64_t bxe = std::max(1, block / px), bye = std::max(1, block / py);
     for (int64_t z0 = 0; z0 < nz; z0 += pz)
       for (int64_t by = 0; by < ny; by += bye)
         for (int64_t bx = 0; bx < nx; bx += bxe)
@@ -529,7 +846,7 @@ bool make_tiles(int n, int64_t line, int64_t plane, int tr, std::vector<std::vec
             }
     return true;
   }
-  const int64_t per_pencil = TL_PENCIL;
+  const int64_t per_pencil = pz;
   for (int64_t r0 = 0; r0 < n; r0 += tr) {
     std::vector<int> t;
     for (int i = 0; i < tr; ++i) t.push_back(r0 + i < n ? static_cast<int>(r0 + i) : -1);
@@ -626,6 +943,92 @@ bool plan_slots(int n, const std::vector<int>& rp, const std::vector<int>& ci, c
 
 }  // namespace
 
+namespace {
+
+// Stencil form: the rows are the points of an nx x ny x nz grid in natural
+// order, row (x, y, z) stores exactly its in-grid neighbours at the offsets
+// of a set S within the 3 x 3 x 3 neighbourhood (columns ascending), each
+// offset's coefficient is one constant (bitwise), and the diagonal is any
+// value or absent. S and the constants come from the point (1, 1, 1).
+struct StencilPlan {
+  int nx = 0, ny = 0, nz = 0;
+  bool p27 = false;
+  double w[27] = {};
+  std::vector<double> d;
+};
+
+bool detect_stencil(int n, int64_t line, int64_t plane, const std::vector<int>& rp, const std::vector<int>& ci,
+                    const std::vector<double>& v, StencilPlan& S) {
+  if (line <= 0 || plane <= 0 || plane % line != 0 || n % plane != 0) return false;
+  const int nx = static_cast<int>(line), ny = static_cast<int>(plane / line), nz = static_cast<int>(n / plane);
+  if (nx < 3 || ny < 3 || nz < 3) return false;
+  bool have[27] = {};
+  double w[27] = {};
+  const int64_t i0 = plane + line + 1;
+  for (int p = rp[i0]; p < rp[i0 + 1]; ++p) {
+    int64_t off = ci[p] - i0;
+    const int64_t dz = (off + plane / 2 + plane) / plane - 1;  // nearest plane
+    off -= dz * plane;
+    const int64_t dy = (off + line / 2 + line) / line - 1;
+    const int64_t dx = off - dy * line;
+    if (dz < -1 || dz > 1 || dy < -1 || dy > 1 || dx < -1 || dx > 1) return false;
+    const int o = static_cast<int>((dz + 1) * 9 + (dy + 1) * 3 + dx + 1);
+    have[o] = o != 13;
+    if (o != 13) w[o] = v[p];
+  }
+  bool p27 = false;
+  for (int o = 0; o < 27; ++o) {
+    const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+    if (have[o] && (dz != 0) + (dy != 0) + (dx != 0) > 1) p27 = true;
+  }
+  std::vector<double> d(static_cast<size_t>(n), 0.0);
+  for (int z = 0, i = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x, ++i) {
+        int p = rp[i];
+        const int end = rp[i + 1];
+        for (int o = 0; o < 27; ++o) {
+          const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+          const int64_t col = i + dz * plane + dy * line + dx;
+          if (o == 13) {  // the diagonal: optional
+            if (p < end && ci[p] == col) d[static_cast<size_t>(i)] = v[p++];
+            continue;
+          }
+          const bool in = x + dx >= 0 && x + dx < nx && y + dy >= 0 && y + dy < ny && z + dz >= 0 && z + dz < nz;
+          if (!have[o] || !in) continue;
+          if (p >= end || ci[p] != col || std::memcmp(&v[p], &w[o], sizeof(double)) != 0) return false;
+          ++p;
+        }
+        if (p != end) return false;  // entries the stencil does not hold
+      }
+  S.nx = nx;
+  S.ny = ny;
+  S.nz = nz;
+  S.p27 = p27;
+  std::memcpy(S.w, w, sizeof(w));
+  S.d.swap(d);
+  return true;
+}
+
+// X rows (inside the grid) the stencil kernel loads per pencil, summed: the
+// patch with its halo on the pencil's planes and the two around them
+int64_t stencil_loads(const StencilPlan& S) {
+  int64_t total = 0;
+  for (int z0 = 0; z0 < S.nz; z0 += ST_PENCIL) {
+    const int zn = std::min(S.nz - z0, ST_PENCIL);
+    const int planes = std::min(z0 + zn, S.nz - 1) - std::max(z0 - 1, 0) + 1;
+    for (int y0 = 0; y0 < S.ny; y0 += ST_E)
+      for (int x0 = 0; x0 < S.nx; x0 += ST_E) {
+        const int wx = std::min(x0 + ST_E, S.nx - 1) - std::max(x0 - 1, 0) + 1;
+        const int wy = std::min(y0 + ST_E, S.ny - 1) - std::max(y0 - 1, 0) + 1;
+        total += static_cast<int64_t>(planes) * wx * wy;
+      }
+  }
+  return total;
+}
+
+}  // namespace
+
 extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   FE_ARG(ctx && A, "fe_csr_build_tiled: null argument");
   if (A->tl_npencils > 0) {
@@ -645,6 +1048,45 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   FE_HIP_TRY(hipStreamSynchronize(ctx->stream));
   int64_t line = 0, plane = 0;
   if (!detect_grid(rp, ci, n, line, plane)) line = plane = 0;
+  // a constant-coefficient grid operator takes the stencil form (no entry
+  // lists); FE_TILE_STENCIL=0 keeps the entry-list tiles
+  const char* se = std::getenv("FE_TILE_STENCIL");
+  StencilPlan S;
+  if (plane > 0 && !(se && std::atoi(se) == 0) && detect_stencil(n, line, plane, rp, ci, v, S)) {
+    std::vector<float> d32(S.d.size());
+    for (size_t i = 0; i < S.d.size(); ++i) d32[i] = static_cast<float>(S.d[i]);
+    if (hipMalloc(&A->st_d64, sizeof(double) * S.d.size()) != hipSuccess ||
+        hipMalloc(&A->st_d32, sizeof(float) * S.d.size()) != hipSuccess ||
+        hipMalloc(&A->st_junk, 2 * ST_NT * 16) != hipSuccess ||
+        hipMemcpyAsync(A->st_d64, S.d.data(), sizeof(double) * S.d.size(), hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(A->st_d32, d32.data(), sizeof(float) * d32.size(), hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      hipStreamSynchronize(ctx->stream);
+      if (A->st_d64) hipFree(A->st_d64);
+      if (A->st_d32) hipFree(A->st_d32);
+      if (A->st_junk) hipFree(A->st_junk);
+      A->st_d64 = nullptr;
+      A->st_d32 = nullptr;
+      A->st_junk = nullptr;
+      return fail(-3, "fe_csr_build_tiled: upload failed");
+    }
+    A->st_on = 1;
+    A->st_group = ST_GROUP;
+    if (const char* e = std::getenv("FE_STENCIL_GROUP")) A->st_group = std::max(1, std::atoi(e));
+    A->st_p27 = S.p27 ? 1 : 0;
+    A->st_nx = S.nx;
+    A->st_ny = S.ny;
+    A->st_nz = S.nz;
+    std::memcpy(A->st_w, S.w, sizeof(S.w));
+    const int64_t npx = (S.nx + ST_E - 1) / ST_E, npy = (S.ny + ST_E - 1) / ST_E;
+    A->tl_npencils = npx * npy * ((S.nz + ST_PENCIL - 1) / ST_PENCIL);
+    A->tl_loads = stencil_loads(S);
+    A->tl_trows = ST_E * ST_E;
+    if (loads_per_row) *loads_per_row = static_cast<double>(A->tl_loads) / n;
+    return 0;
+  }
   // tile height: 32 rows (8 x 4 patches) on a grid whose lines and planes
   // split into them, else 16; FE_TILE_ROWS=16 or 32 overrides (a 32-row plan
   // that does not fit the LDS slots falls back to 16 rows)
@@ -653,6 +1095,11 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
     const int want = std::atoi(e);
     if (want == 16 || want == 32) tr = want;
   }
+  // pencil length and block edge of the grid order: FE_TILE_PENCIL (clamped to
+  // what the kernel's tables hold) and FE_TILE_BLOCK (points) override
+  int pz_want = TL_PENCIL, block = TL_BLOCK_POINTS;
+  if (const char* e = std::getenv("FE_TILE_PENCIL")) pz_want = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("FE_TILE_BLOCK")) block = std::max(1, std::atoi(e));
   std::vector<std::vector<int>> tiles;
   std::vector<int64_t> pencil{0};
   TilePlan P;
@@ -662,7 +1109,8 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
     pencil.assign(1, 0);
     P = TilePlan{};
     const int pf = tr == 16 ? tl_pf<16>() : tl_pf<32>();
-    planned = make_tiles(n, line, plane, tr, tiles, pencil) &&
+    const int pz = std::min(pz_want, tr == 16 ? tl_pencil_max<16>() : tl_pencil_max<32>());
+    planned = make_tiles(n, line, plane, tr, pz, block, tiles, pencil) &&
               plan_slots(n, rp, ci, v, tiles, pencil, pf * tr, P);
     if (planned) break;
   }
@@ -715,4 +1163,9 @@ extern "C" int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row) {
   A->tl_self = P.self ? 1 : 0;
   A->tl_trows = tr;
   return 0;
+}
+
+extern "C" int fe_csr_tiled_kind(fe_csr A) {
+  if (!A || A->tl_npencils <= 0) return 0;
+  return A->st_on ? ST_E * ST_E : A->tl_trows;
 }

@@ -114,18 +114,30 @@ int fe_csr_build_blocked(fe_ctx ctx, fe_csr A, int rb, double* reuse);
  * bitwise the same in any order (each row's sum is unchanged). No counterpart
  * in the reference (its CSR product is cuSPARSE's). */
 int fe_csr_block_schedule(fe_ctx ctx, fe_csr A, int64_t plane, int64_t strip);
-/* Builds the tiled copy of A: tiles of 16 rows (4 x 4 patches of a plane for
- * a 3D grid stencil in natural order, else 16 consecutive rows) taken in
- * pencils of consecutive tiles (through up to 24 planes, or 24 tiles). The
- * product runs one workgroup per pencil and 256-byte column slice and keeps
- * the X rows of the current tile in LDS, loading only the rows each tile adds
- * to the previous one (the slot plan is made here, on the host). Returns 0
- * when built, 1 when a tile needs more than 240 X rows or the copy would not
+/* Builds the tiled copy of A, one of two forms:
+ * - stencil form, for a grid operator whose rows are the points of a 3D grid
+ *   in natural order and store exactly their in-grid neighbours at a fixed set
+ *   of offsets within the 3 x 3 x 3 neighbourhood, each offset with one
+ *   constant coefficient (any diagonal; the Dirichlet Laplacian plus a
+ *   potential): no entry lists; a workgroup walks a pencil of 8 x 8 patches
+ *   through 24 planes with three planes of the patch and its halo in LDS;
+ * - entry-list tiles otherwise: tiles of 32 rows (8 x 4 patches of a plane for
+ *   a 3D grid stencil whose lines split into them), else 16 rows (4 x 4
+ *   patches, or consecutive rows), taken in pencils of consecutive tiles
+ *   (through up to 24 planes, or 24 tiles); a workgroup keeps the X rows of the
+ *   current tile in LDS slots planned here, on the host, and loads only the
+ *   rows each tile adds to the previous one.
+ * The product runs one workgroup per pencil and 256-byte column slice. Returns
+ * 0 when built, 1 when a tile needs more than 240 X rows or the copy would not
  * load fewer than half of the nonzeros per row; *loads_per_row receives the
  * X rows loaded per matrix row. Sums follow the CSR entry order (bitwise the
- * CSR kernel's). Select it with fe_csr_use(A, FE_FMT_TILED). No counterpart
- * in the reference (its CSR product is cuSPARSE's). */
+ * CSR kernel's while X is finite). Select it with fe_csr_use(A,
+ * FE_FMT_TILED). No counterpart in the reference (its CSR product is
+ * cuSPARSE's). */
 int fe_csr_build_tiled(fe_ctx ctx, fe_csr A, double* loads_per_row);
+/* Form of A's tiled copy: 0 none, 16 or 32 entry-list tiles of that many
+ * rows, 64 the stencil form (8 x 8 patches). No counterpart in the reference. */
+int fe_csr_tiled_kind(fe_csr A);
 
 /* ---- fused block SpMM (replaces LinearOperator::apply_block,
  *      include/core/LinearOperator.hpp:32-37; CUDA: k x cusparseSpMV,

@@ -302,6 +302,7 @@ def test_tiled_plan_load_count(dev, monkeypatch, g, rows):
     16-row tiles are asked for; 28^3: the line is no multiple of 8, so 4 x 4
     patches, a pencil of 24 planes and one of 4; 32^3: 8 x 4 patches, pencils
     of 24 and 8 planes)."""
+    monkeypatch.setenv("FE_TILE_STENCIL", "0")  # entry-list tiles, not the stencil form
     if rows:
         monkeypatch.setenv("FE_TILE_ROWS", str(rows))
     else:
@@ -325,6 +326,7 @@ def test_spmm_tiled_both_heights(dev, monkeypatch, rows, k, prec):
     grid whose line splits into 8 x 4 patches, and a banded matrix without a
     grid (consecutive-row tiles of 16 or 32 rows, padded last tile)."""
     monkeypatch.setenv("FE_TILE_ROWS", str(rows))
+    monkeypatch.setenv("FE_TILE_STENCIL", "0")
     rng = np.random.default_rng(rows + k)
     band = sp.diags([rng.uniform(-1, 1, 3001 - abs(o)) for o in range(-9, 10)], list(range(-9, 10)))
     for A in (G.hamiltonian_27pt(16, seed=9, wells=4), sp.csr_matrix(band + band.T)):
@@ -334,7 +336,7 @@ def test_spmm_tiled_both_heights(dev, monkeypatch, rows, k, prec):
         Z = rng.uniform(-1, 1, (n, k))
         C = dev.csr(A)
         built, lpr = C.build_tiled()
-        assert built
+        assert built and C.tiled_kind() == rows
         Xd, Zd = dev.put(X, dt), dev.put(Z, dt)
         Y1, Y2 = dev.buffer((n, k), dt), dev.buffer((n, k), dt)
         _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y1.ptr, k, 0.7, 0.3, Zd.ptr, k, -0.9))
@@ -346,6 +348,133 @@ def test_spmm_tiled_both_heights(dev, monkeypatch, rows, k, prec):
         ref = orc.spmm(A, Xr, 0.7, 0.0) + 0.3 * Xr - 0.9 * Zr
         tol = 1e-13 if prec == _fe.F64 else 2e-6
         assert np.max(np.abs(got.astype(np.float64) - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 2)
+
+
+def _box_stencil(nx, ny, nz, W, d):
+    """Constant-coefficient operator on an nx x ny x nz grid in natural order
+    (Dirichlet: no entries past the faces): off-diagonal W[dz+1][dy+1][dx+1]
+    (zeros not stored), diagonal d (zeros not stored)."""
+    n = nx * ny * nz
+    idx = np.arange(n).reshape(nz, ny, nx)
+    rows, cols, vals = [], [], []
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                w = W[dz + 1][dy + 1][dx + 1]
+                if (dz, dy, dx) == (0, 0, 0) or w == 0:
+                    continue
+                src = idx[max(0, -dz):nz - max(0, dz), max(0, -dy):ny - max(0, dy), max(0, -dx):nx - max(0, dx)]
+                rows.append(src.ravel())
+                cols.append((src + dz * nx * ny + dy * nx + dx).ravel())
+                vals.append(np.full(src.size, w))
+    keep = d != 0
+    rows.append(np.arange(n)[keep])
+    cols.append(np.arange(n)[keep])
+    vals.append(d[keep])
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    A.sort_indices()
+    return A
+
+
+def _w19():
+    W = -0.5 * G.isotropic_27pt_weights()
+    for dz in (0, 2):
+        for dy in (0, 2):
+            for dx in (0, 2):
+                W[dz, dy, dx] = 0.0  # no corners: 19-point
+    return W
+
+
+STENCIL = {
+    # 8 x 8 patches, one pencil of 16 planes
+    "h16": lambda: G.hamiltonian_27pt(16, seed=3, wells=5),
+    # ragged patches (30 = 3 x 8 + 6), pencils of 24 and 6 planes
+    "h30": lambda: G.hamiltonian_27pt(30, seed=8, wells=9),
+    # a box: 13 x 10 x 27 (ragged both ways, pencils of 24 and 3 planes)
+    "box": lambda: _box_stencil(13, 10, 27, -0.5 * G.isotropic_27pt_weights(),
+                                np.random.default_rng(1).normal(size=13 * 10 * 27)),
+    # 7-point constant Laplacian (faces only)
+    "lap7": lambda: G.laplacian_3d(12),
+    # 19-point (no corners: 27-point form with zero corner coefficients), some
+    # rows without a stored diagonal
+    "s19": lambda: _box_stencil(11, 9, 10, _w19(),
+                                np.where(np.arange(990) % 7 == 0, 0.0,
+                                         np.random.default_rng(2).normal(size=990))),
+}
+
+
+@pytest.fixture(scope="module")
+def stencil_mats():
+    return {name: f() for name, f in STENCIL.items()}
+
+
+@pytest.mark.parametrize("name", list(STENCIL))
+@pytest.mark.parametrize("k,prec", [(4, _fe.F32), (64, _fe.F32), (100, _fe.F32), (512, _fe.F32),
+                                    (32, _fe.F64), (48, _fe.F64), (100, _fe.F64)])
+def test_spmm_stencil_matches_csr(dev, stencil_mats, name, k, prec):
+    """Stencil form of the tiled copy (constant coefficient per offset, any
+    diagonal): bitwise the CSR kernel's product, fused epilogue included, and
+    within rounding of the oracle; ragged patches and pencils, partial column
+    slices, 7-point and 19-point neighbourhoods, rows without a diagonal."""
+    A = stencil_mats[name]
+    n = A.shape[0]
+    rng = np.random.default_rng(7 + k)
+    X = rng.uniform(-1, 1, (n, k))
+    Z = rng.uniform(-1, 1, (n, k))
+    dt = np.float64 if prec == _fe.F64 else np.float32
+    C = dev.csr(A)
+    built, lpr = C.build_tiled()
+    assert built and C.tiled_kind() == 64
+    assert lpr <= 3.0
+    Xd, Zd = dev.put(X, dt), dev.put(Z, dt)
+    Y1, Y2 = dev.buffer((n, k), dt), dev.buffer((n, k), dt)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y1.ptr, k, 0.9, 0.4, Zd.ptr, k, -1.2))
+    C.use(_fe.FMT_TILED)
+    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, k, Y2.ptr, k, 0.9, 0.4, Zd.ptr, k, -1.2))
+    got = Y2.download()
+    assert np.array_equal(Y1.download(), got)
+    Xr, Zr = X.astype(dt).astype(np.float64), Z.astype(dt).astype(np.float64)
+    ref = orc.spmm(A, Xr, 0.9, 0.0) + 0.4 * Xr - 1.2 * Zr
+    tol = 1e-13 if prec == _fe.F64 else 2e-6
+    assert np.max(np.abs(got.astype(np.float64) - ref)) <= tol * (np.max(np.abs(A).sum(axis=1)) + 2)
+
+
+def test_spmm_stencil_in_place_strided_and_without_z(dev):
+    """Stencil form with Z aliasing Y (read before written), a strided block
+    (ld > k) with a partial 16-byte run, beta = 0 and no Z term: bitwise the
+    CSR kernel's, and the columns past k untouched."""
+    A = G.hamiltonian_27pt(12, seed=5, wells=4)
+    n = A.shape[0]
+    ld = 72
+    X = np.random.default_rng(9).uniform(-1, 1, (n, ld))
+    C = dev.csr(A)
+    assert C.build_tiled()[0] and C.tiled_kind() == 64
+    for k in (72, 70, 37, 3):
+        for prec, dt in ((_fe.F64, np.float64), (_fe.F32, np.float32)):
+            for beta, zterm in ((0.5, True), (0.0, False)):
+                outs = []
+                for fmt in (_fe.FMT_CSR, _fe.FMT_TILED):
+                    C.use(fmt)
+                    Xd = dev.put(X, dt)
+                    Y = dev.put(np.full((n, ld), 7.0), dt)
+                    _fe.check(L().fe_spmm(dev.ctx, C.h, prec, k, Xd.ptr, ld, Y.ptr, ld, 1.0, beta,
+                                          Y.ptr if zterm else None, ld, 0.25 if zterm else 0.0))
+                    outs.append(Y.download())
+                assert np.array_equal(outs[0], outs[1])
+                assert np.all(outs[1][:, k:] == 7.0)
+
+
+def test_spmm_stencil_declines_varying_coefficients(dev):
+    """No stencil form where one offset's coefficient varies (per-edge
+    conductivities; one perturbed pair of a constant stencil): the entry-list
+    tiles are built instead, bitwise the CSR product as before."""
+    H = G.hamiltonian_27pt(16, seed=3, wells=5).tolil()
+    H[1000, 1001] += 1e-3
+    H[1001, 1000] += 1e-3
+    for A in (G.conductivity_3d(16, seed=1), H.tocsr()):
+        C = dev.csr(A)
+        assert C.build_tiled()[0]
+        assert C.tiled_kind() in (16, 32)
 
 
 def test_spmm_tiled_ragged_and_in_place(dev):
