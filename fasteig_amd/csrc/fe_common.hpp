@@ -130,7 +130,7 @@ struct fe_csr_s {
   // their halo) staged in LDS (fe_spmm_tile.hip)
   int st_on = 0;                 // the tiled copy is the stencil form
   int st_p27 = 1;                // 27-point (else 7-point: faces and centre)
-  int st_group = 4;              // pencils per work group
+  int st_group = 1 << 30;        // pencils per work group (more than the pencils: slice-major)
   int st_nx = 0, st_ny = 0, st_nz = 0;
   double st_w[27] = {};          // coefficient of offset (dz + 1) * 9 + (dy + 1) * 3 + dx + 1 (centre: diagonal)
   double* st_d64 = nullptr;      // diagonal (zero where a row stores none)

@@ -450,10 +450,12 @@ __global__ __launch_bounds__(16 * TR, (sizeof(T) == 4 && O32 && SELF) ? tl_waves
 // past the grid's faces, or of an offset the row does not store in 27-point
 // form, adds coefficient x 0 or 0 x X, which leaves the sum unchanged while
 // X is finite).
-// Work items: groups of ST_GROUP consecutive pencils taken slice by slice, in
-// contiguous ranges per XCD: the slices of a row are read by one XCD at
-// nearly the same time, and neighbouring pencils share halo rows through its
-// L2.
+// Work items: slice-major (every pencil of slice 0, then of slice 1, ...), in
+// contiguous ranges per XCD, so an XCD runs one slice of a compact block of
+// pencils at a time (64 on C3: five rows of 8 x 8 patches) and their shared
+// halo rows come from its L2. C3, FP32 k = 512: 10% fewer fetched bytes and a
+// 3% shorter launch than groups of four pencils slice by slice (the order of
+// the entry-list tiles, whose slices share entry lists); FP64 k = 512: 13%.
 template <typename T>
 struct StencilCoef {
   T w[27];
@@ -463,7 +465,9 @@ constexpr int ST_E = 8;         // patch edge (rows per line and lines per patch
 constexpr int ST_H = ST_E + 2;  // with the halo
 constexpr int ST_SLOTS = ST_H * ST_H;
 constexpr int ST_PENCIL = 24;   // planes per pencil
-constexpr int ST_GROUP = 4;     // pencils per work group (FE_STENCIL_GROUP overrides)
+// Pencils per work group (FE_STENCIL_GROUP overrides); the default, larger
+// than any grid's pencil count, makes the items slice-major.
+constexpr int ST_GROUP = 1 << 30;
 constexpr int ST_NT = 512;      // threads
 constexpr int ST_LD = (ST_SLOTS * 16 + ST_NT - 1) / ST_NT;  // 16-byte X loads per thread per plane (4)
 
