@@ -13,6 +13,8 @@
 // holds A[l % 16][8 (l / 16) + 0..7] and B[8 (l / 16) + 0..7][l % 16]).
 #include "fe_common.hpp"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -523,6 +525,129 @@ __global__ __launch_bounds__(512, 1) void k_gram_bf16(int64_t n, int m, int p, c
         const int row = wm * 128 + i * 16 + 4 * (lane >> 4) + e;
         const int col = wn * 64 + j * 16 + (lane & 15);
         P[col * GT + row] = static_cast<double>(acc[i][j][e]);
+      }
+}
+
+// Compensated Gram, fast path: the tiles' 256 columns all inside m and p,
+// 16-byte aligned blocks, no weights. Same tiles, stages, parts and sums as
+// k_gram_bf16 (bitwise the same partials), but two stages of loads in flight
+// instead of one, and no condition on any load of the stage loop: rows past
+// the chunk load its last row and are zeroed by a select, a diagonal tile of
+// a symmetric Gram loads its Y stage (the X rows again, cache hits) and only
+// skips writing it. With one stage in flight and a full barrier (which also
+// waits for every load) a stage took about 5.9 us at the C3 shape against
+// 0.4 us of MFMA work: one memory round trip per stage.
+__device__ __forceinline__ void lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // gfx9 encoding: vmcnt 63, expcnt 7, lgkmcnt 0
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <int NS>
+__global__ __launch_bounds__(512, 1) void k_gram_bf16f(int64_t n, const float* __restrict__ X, int64_t ldx,
+                                                    const float* __restrict__ Y, int64_t ldy, int tiles_b,
+                                                    int ntiles, int sym, int64_t rows_per, int64_t per_xcd,
+                                                    int64_t nunits, double* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 Xl[NS][GT][BKP];
+  __shared__ __attribute__((aligned(16))) __bf16 Yl[NS][GT][BKP];
+  const int64_t u = xcd_tile(blockIdx.x, per_xcd);
+  if (u >= nunits) return;
+  const int64_t chunk = u / ntiles;
+  const int tile = static_cast<int>(u - chunk * ntiles);
+  int ta, tb;
+  gram_tile(tile, tiles_b, sym, ta, tb);
+  const bool same = sym == 1 && ta == tb;
+  const int a0 = ta * GT, b0 = tb * GT;
+  const int64_t r0 = chunk * rows_per;
+  const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  f4_t acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{};
+  const int rb = 4 * (lane & 7), cb = 4 * ((lane >> 3) + 8 * wave);
+  const float* Xc = X + a0 + cb;
+  const float* Yc = Y + b0 + cb;
+  struct Stage {
+    float4 x[4], y[4];
+  };
+  auto fetch = [&](int64_t r, Stage& st) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t row = r + rb + i;
+      const int64_t rc = row < r1 ? row : r1 - 1;
+      st.x[i] = *reinterpret_cast<const float4*>(Xc + rc * ldx);
+      st.y[i] = *reinterpret_cast<const float4*>(Yc + rc * ldy);
+    }
+  };
+  typedef uint32_t u2v_t __attribute__((ext_vector_type(2)));
+  auto put = [&](int64_t r, const float4 (&v)[4], __bf16 (*L)[GT][BKP]) {
+    float e[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool ok = r + rb + i < r1;  // rows past the chunk: zeros (per component: no select of objects)
+      e[i][0] = ok ? v[i].x : 0.0f;
+      e[i][1] = ok ? v[i].y : 0.0f;
+      e[i][2] = ok ? v[i].z : 0.0f;
+      e[i][3] = ok ? v[i].w : 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      uint32_t lo[NS], hi[NS];
+      split_pair<NS>(e[0][c], e[1][c], lo);
+      split_pair<NS>(e[2][c], e[3][c], hi);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) *reinterpret_cast<u2v_t*>(&L[s][cb + c][rb]) = u2v_t{lo[s], hi[s]};
+    }
+  };
+  auto products = [&]() {
+    const int kb = 8 * (lane >> 4);
+    bf8_t bfr[4][NS];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        bfr[j][s] = *reinterpret_cast<const bf8_t*>(same ? &Xl[s][wn * 64 + j * 16 + (lane & 15)][kb]
+                                                         : &Yl[s][wn * 64 + j * 16 + (lane & 15)][kb]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      bf8_t ai[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) ai[s] = *reinterpret_cast<const bf8_t*>(&Xl[s][wm * 128 + i * 16 + (lane & 15)][kb]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma_split<NS>(ai, bfr[j], acc[i][j]);
+    }
+  };
+  Stage st[2];
+  fetch(r0, st[0]);
+  fetch(r0 + BK, st[1]);
+  auto stage = [&](auto Pc, int64_t r) {
+    constexpr int P = decltype(Pc)::value;
+    lds_sync();  // every wave is done reading the previous stage
+    put(r, st[P].x, Xl);
+    if (!same) put(r, st[P].y, Yl);
+    lds_sync();
+    fetch(r + 2 * BK, st[P]);  // two stages ahead (past the chunk: its last row again)
+    products();
+  };
+  // stages in pairs: a chunk with an odd count runs one stage of zeros
+  for (int64_t r = r0; r < r1; r += 2 * BK) {
+    stage(std::integral_constant<int, 0>{}, r);
+    stage(std::integral_constant<int, 1>{}, r + BK);
+  }
+  double* Pp = part + u * (GT * GT);
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = wm * 128 + i * 16 + 4 * (lane >> 4) + e;
+        const int col = wn * 64 + j * 16 + (lane & 15);
+        Pp[col * GT + row] = static_cast<double>(acc[i][j][e]);
       }
 }
 
@@ -1078,7 +1203,16 @@ static int gram_run(fe_ctx ctx, int mode, int prec, int64_t n, int m, int p, con
   const int64_t units = chunks * ntiles, nb8 = (units + 7) / 8 * 8;
   const int vec = (reinterpret_cast<uintptr_t>(X) % 16 == 0) && (reinterpret_cast<uintptr_t>(Y) % 16 == 0) &&
                   ldx % 4 == 0 && ldy % 4 == 0;
-  if (mode == FE_GRAM_BF16X3)
+  // fast path: whole tiles, aligned blocks, no weights (FE_GRAM_FAST=0 turns
+  // it off); BF16x3 only: the BF16x6 parts with two stages in flight need more
+  // than the 256 registers of two waves per SIMD and spill, also with the B
+  // fragments taken in halves
+  const char* fe_env = std::getenv("FE_GRAM_FAST");
+  const bool fast = !(fe_env && std::atoi(fe_env) == 0) && vec && !w && m % GT == 0 && p % GT == 0 && n > 0;
+  if (fast && mode == FE_GRAM_BF16X3)
+    hipLaunchKernelGGL(k_gram_bf16f<2>, dim3((unsigned)nb8), dim3(512), 0, ctx->stream, n, (const float*)X, ldx,
+                       (const float*)Y, ldy, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, (double*)ws);
+  else if (mode == FE_GRAM_BF16X3)
     hipLaunchKernelGGL(k_gram_bf16<2>, dim3((unsigned)nb8), dim3(512), 0, ctx->stream, n, m, p, (const float*)X, ldx,
                        (const float*)Y, ldy, w, tiles_b, ntiles, symi, rows_per, nb8 / 8, units, vec, (double*)ws);
   else

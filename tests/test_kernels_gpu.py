@@ -657,6 +657,31 @@ def test_gram_bf16_modes(dev, n, m, p):
             assert rel(S, sref) <= max(2 * s32, 2.0 ** -21), (rel(S, sref), s32)
 
 
+@pytest.mark.parametrize("n,m,p", [(40000, 512, 512), (8191, 256, 512), (100000, 256, 256), (33, 256, 256),
+                                   (65, 512, 256)])
+def test_gram_bf16x3_fast_path_bitwise(dev, monkeypatch, n, m, p):
+    """The BF16x3 Gram's fast path (whole 256-column tiles, no weights; two
+    stages of loads in flight, rows past a chunk zeroed by selects) gives the
+    general kernel's partials bitwise, symmetric and not, with a chunk whose
+    stage count is odd (a stage of zeros) and chunks shorter than a stage."""
+    rng = np.random.default_rng(n + m + p)
+    X = rng.uniform(-1, 1, (n, m)).astype(np.float32)
+    Y = rng.uniform(-1, 1, (n, p)).astype(np.float32)
+    Xd, Yd = dev.put(X, np.float32), dev.put(Y, np.float32)
+    outs = {}
+    for fast in ("1", "0"):
+        monkeypatch.setenv("FE_GRAM_FAST", fast)
+        Gd = dev.buffer((m, p), np.float64, order="F")
+        Sd = dev.buffer((m, m), np.float64, order="F")
+        _fe.check(L().fe_gram(dev.ctx, _fe.GRAM_BF16X3, _fe.F32, n, m, p, Xd.ptr, m, Yd.ptr, p, None, Gd.ptr, m))
+        _fe.check(L().fe_gram(dev.ctx, _fe.GRAM_BF16X3, _fe.F32, n, m, m, Xd.ptr, m, Xd.ptr, m, None, Sd.ptr, m))
+        outs[fast] = (Gd.download(), Sd.download())
+    assert np.array_equal(outs["1"][0], outs["0"][0])
+    assert np.array_equal(outs["1"][1], outs["0"][1])
+    ref = X.astype(np.float64).T @ Y.astype(np.float64)
+    assert rel(outs["1"][0], ref) < 2.0 ** -14
+
+
 @pytest.mark.parametrize("n,m,p", [(20000, 512, 512), (100000, 256, 256), (3333, 200, 70), (999, 130, 130),
                                    (7, 3, 2), (1001, 48, 64), (300, 33, 1)])
 def test_gemm_bf16_modes(dev, n, m, p):
